@@ -1,0 +1,667 @@
+/*
+ * orc_tsne.c -- restatement of the gradient orchestrators, the optimiser step, the FFT-path KL and
+ * the affinity code of src/tsne.cpp (FIt-SNE 1.2.1).  TEST INFRASTRUCTURE: see fitsne_oracle.h.
+ */
+#include "fitsne_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ---- grid geometry ---- */
+
+/* tsne.cpp:1042-1049: `else if` -- a value that sets a new running max is never tested against min. */
+void orc_minmax_2d_quirk(const double *Y, int N, double *min_coord, double *max_coord) {
+    double mn = INFINITY, mx = -INFINITY;
+    for (long i = 0; i < N; i++) {
+        double x = Y[i * 2 + 0], y = Y[i * 2 + 1];
+        if (x > mx) mx = x;
+        else if (x < mn) mn = x;
+        if (y > mx) mx = y;
+        else if (y < mn) mn = y;
+    }
+    *min_coord = mn;
+    *max_coord = mx;
+}
+
+/* tsne.cpp:1065-1077 */
+int orc_n_boxes_2d(double min_coord, double max_coord, double intervals_per_integer, int min_num_intervals) {
+    int nb = (int)fmax(min_num_intervals, (max_coord - min_coord) / intervals_per_integer);
+    static const int allowed[20] = {25, 36, 50, 55, 60, 65, 70, 75, 80, 85, 90, 96, 100, 110, 120, 130, 140, 150, 175, 200};
+    if (nb < allowed[19]) {
+        int c;
+        for (c = 0; allowed[c] < nb; c++) {}
+        nb = allowed[c];
+    }
+    return nb;
+}
+
+/* tsne.cpp:663 / :774 (no rounding table in 1D) */
+int orc_n_boxes_1d(double y_min, double y_max, double intervals_per_integer, int min_num_intervals) {
+    return (int)fmax(min_num_intervals, (y_max - y_min) / intervals_per_integer);
+}
+
+/* ---- attractive forces (PARALLEL_FOR over CSR rows) ---- */
+typedef struct {
+    const uint32_t *row_P, *col_P;
+    const double *val_P, *Y;
+    int D;
+    double df;
+    double *pos_f;
+} att_ctx;
+
+static void att_body(int lo, int hi, void *c_) {
+    att_ctx *c = (att_ctx *)c_;
+    const double *Y = c->Y;
+    if (c->D == 2) {
+        for (int n = lo; n < hi; n++) {
+            double dim1 = 0, dim2 = 0;
+            double xn = Y[2 * (long)n], yn = Y[2 * (long)n + 1];
+            for (uint32_t i = c->row_P[n]; i < c->row_P[n + 1]; i++) {
+                uint32_t m = c->col_P[i];
+                double dx = xn - Y[2 * (long)m], dy = yn - Y[2 * (long)m + 1];
+                double d_ij = dx * dx + dy * dy;
+                double q_ij = c->df == 1.0 ? 1 / (1 + d_ij) : 1 / (1 + d_ij / c->df);
+                dim1 += c->val_P[i] * q_ij * dx;
+                dim2 += c->val_P[i] * q_ij * dy;
+            }
+            c->pos_f[2 * (long)n] = dim1;
+            c->pos_f[2 * (long)n + 1] = dim2;
+        }
+    } else {
+        for (int n = lo; n < hi; n++) {
+            double dim1 = 0;
+            for (uint32_t i = c->row_P[n]; i < c->row_P[n + 1]; i++) {
+                double d_ij = Y[n] - Y[c->col_P[i]];
+                double q_ij = c->df == 1.0 ? 1 / (1 + d_ij * d_ij) : 1 / (1 + (d_ij * d_ij) / c->df);
+                dim1 += c->val_P[i] * q_ij * d_ij;
+            }
+            c->pos_f[n] = dim1;
+        }
+    }
+}
+
+/* tsne.cpp:1121-1137 (2D), :965-980 (2D df), :826-837 (1D), :721-732 (1D df) */
+void orc_attractive(const uint32_t *row_P, const uint32_t *col_P, const double *val_P, const double *Y,
+                    int N, int D, double df, unsigned nthreads, double *pos_f) {
+    att_ctx c = {row_P, col_P, val_P, Y, D, df, pos_f};
+    orc_parallel_for(nthreads, N, att_body, &c);
+}
+
+/* ---- tsne.cpp:1027-1169  computeFftGradient (2D, df = 1) ---- */
+void orc_grad_2d(const uint32_t *row_P, const uint32_t *col_P, const double *val_P, const double *Y, int N,
+                 double *dC, int p, double ipi, int min_num_intervals, unsigned nthreads, double *sum_Q_out) {
+    for (long i = 0; i < (long)N * 2; i++) dC[i] = 0.0;
+    double *xs = (double *)malloc(sizeof(double) * (size_t)N);
+    double *ys = (double *)malloc(sizeof(double) * (size_t)N);
+    for (long i = 0; i < N; i++) { xs[i] = Y[i * 2]; ys[i] = Y[i * 2 + 1]; }
+    double min_coord, max_coord;
+    orc_minmax_2d_quirk(Y, N, &min_coord, &max_coord);
+
+    int n_terms = 4;
+    double *charges = (double *)malloc(sizeof(double) * (size_t)N * n_terms);
+    double *pot = (double *)calloc((size_t)N * n_terms, sizeof(double));
+    for (long j = 0; j < N; j++) {
+        charges[j * n_terms + 0] = 1;
+        charges[j * n_terms + 1] = xs[j];
+        charges[j * n_terms + 2] = ys[j];
+        charges[j * n_terms + 3] = xs[j] * xs[j] + ys[j] * ys[j];
+    }
+    int nb = orc_n_boxes_2d(min_coord, max_coord, ipi, min_num_intervals);
+    int n1d = p * nb;
+    int M = 2 * n1d;
+    double *blo = (double *)malloc(sizeof(double) * 2 * (size_t)nb * nb);
+    double *bhi = (double *)malloc(sizeof(double) * 2 * (size_t)nb * nb);
+    double *sp = (double *)malloc(sizeof(double) * (size_t)p);
+    double *xt = (double *)calloc((size_t)n1d, sizeof(double));
+    double *yt = (double *)calloc((size_t)n1d, sizeof(double));
+    double *kt = (double *)malloc(sizeof(double) * 2 * (size_t)M * (M / 2 + 1));
+    /* argument order as at tsne.cpp:1092 (x_tilde and y_tilde swapped by the caller; identical grids) */
+    orc_precompute_2d(max_coord, min_coord, max_coord, min_coord, nb, p, ORC_SQUARED_CAUCHY, blo, bhi, sp, xt, yt, kt, 1.0);
+    orc_n_body_fft_2d(N, n_terms, xs, ys, charges, nb, p, blo, bhi, sp, kt, pot, nthreads);
+
+    double sum_Q = 0;
+    for (long i = 0; i < N; i++) {
+        double phi1 = pot[i * n_terms + 0], phi2 = pot[i * n_terms + 1];
+        double phi3 = pot[i * n_terms + 2], phi4 = pot[i * n_terms + 3];
+        sum_Q += (1 + xs[i] * xs[i] + ys[i] * ys[i]) * phi1 - 2 * (xs[i] * phi2 + ys[i] * phi3) + phi4;
+    }
+    sum_Q -= N;
+    *sum_Q_out = sum_Q;
+
+    double *pos_f = (double *)malloc(sizeof(double) * 2 * (size_t)N);
+    orc_attractive(row_P, col_P, val_P, Y, N, 2, 1.0, nthreads, pos_f);
+    for (long i = 0; i < N; i++) {
+        double nx = (xs[i] * pot[i * n_terms] - pot[i * n_terms + 1]) / sum_Q;
+        double ny = (ys[i] * pot[i * n_terms] - pot[i * n_terms + 2]) / sum_Q;
+        dC[i * 2 + 0] = pos_f[i * 2] - nx;
+        dC[i * 2 + 1] = pos_f[i * 2 + 1] - ny;
+    }
+    free(pos_f); free(pot); free(charges); free(xs); free(ys);
+    free(blo); free(bhi); free(sp); free(xt); free(yt); free(kt);
+}
+
+/* ---- tsne.cpp:860-1024  computeFftGradientVariableDf (2D, df != 1) ---- */
+void orc_grad_2d_df(const uint32_t *row_P, const uint32_t *col_P, const double *val_P, const double *Y, int N,
+                    double *dC, int p, double ipi, int min_num_intervals, unsigned nthreads, double df,
+                    double *sum_Q_out) {
+    for (long i = 0; i < (long)N * 2; i++) dC[i] = 0.0;
+    double *xs = (double *)malloc(sizeof(double) * (size_t)N);
+    double *ys = (double *)malloc(sizeof(double) * (size_t)N);
+    for (long i = 0; i < N; i++) { xs[i] = Y[i * 2]; ys[i] = Y[i * 2 + 1]; }
+    double min_coord, max_coord;
+    orc_minmax_2d_quirk(Y, N, &min_coord, &max_coord);
+    int nb = orc_n_boxes_2d(min_coord, max_coord, ipi, min_num_intervals);
+
+    int sq_terms = 3;
+    double *sqc = (double *)malloc(sizeof(double) * (size_t)N * sq_terms);
+    double *sqp = (double *)calloc((size_t)N * sq_terms, sizeof(double));
+    for (long j = 0; j < N; j++) {
+        sqc[j * sq_terms + 0] = xs[j];
+        sqc[j * sq_terms + 1] = ys[j];
+        sqc[j * sq_terms + 2] = 1;
+    }
+    int n1d = p * nb, M = 2 * n1d;
+    double *blo = (double *)malloc(sizeof(double) * 2 * (size_t)nb * nb);
+    double *bhi = (double *)malloc(sizeof(double) * 2 * (size_t)nb * nb);
+    double *sp = (double *)malloc(sizeof(double) * (size_t)p);
+    double *xt = (double *)calloc((size_t)n1d, sizeof(double));
+    double *yt = (double *)calloc((size_t)n1d, sizeof(double));
+    double *kt = (double *)malloc(sizeof(double) * 2 * (size_t)M * (M / 2 + 1));
+    orc_precompute_2d(max_coord, min_coord, max_coord, min_coord, nb, p, ORC_SQUARED_GENERAL, blo, bhi, sp, xt, yt, kt, df);
+    orc_n_body_fft_2d(N, sq_terms, xs, ys, sqc, nb, p, blo, bhi, sp, kt, sqp, nthreads);
+
+    double *nsc = (double *)malloc(sizeof(double) * (size_t)N);
+    double *nsp = (double *)calloc((size_t)N, sizeof(double));
+    for (long j = 0; j < N; j++) nsc[j] = 1;
+    orc_precompute_2d(max_coord, min_coord, max_coord, min_coord, nb, p, ORC_GENERAL, blo, bhi, sp, xt, yt, kt, df);
+    orc_n_body_fft_2d(N, 1, xs, ys, nsc, nb, p, blo, bhi, sp, kt, nsp, nthreads);
+
+    double sum_Q = 0;
+    for (long i = 0; i < N; i++) sum_Q += nsp[i];
+    sum_Q -= N;
+
+    double *pos_f = (double *)malloc(sizeof(double) * 2 * (size_t)N);
+    orc_attractive(row_P, col_P, val_P, Y, N, 2, df, nthreads, pos_f);
+    for (long i = 0; i < N; i++) {
+        double h2 = sqp[i * sq_terms], h3 = sqp[i * sq_terms + 1], h4 = sqp[i * sq_terms + 2];
+        double nx = (xs[i] * h4 - h2) / sum_Q;
+        double ny = (ys[i] * h4 - h3) / sum_Q;
+        dC[i * 2 + 0] = (pos_f[i * 2] - nx);
+        dC[i * 2 + 1] = (pos_f[i * 2 + 1] - ny);
+    }
+    *sum_Q_out = sum_Q;
+    free(pos_f); free(sqc); free(sqp); free(nsc); free(nsp); free(xs); free(ys);
+    free(blo); free(bhi); free(sp); free(xt); free(yt); free(kt);
+}
+
+static void minmax_1d(const double *Y, int N, double *mn, double *mx) {
+    double a = INFINITY, b = -INFINITY;
+    for (long i = 0; i < N; i++) {
+        if (Y[i] < a) a = Y[i];
+        if (Y[i] > b) b = Y[i];
+    }
+    *mn = a;
+    *mx = b;
+}
+
+/* ---- tsne.cpp:758-856  computeFftGradientOneD (1D, df = 1) ---- */
+void orc_grad_1d(const uint32_t *row_P, const uint32_t *col_P, const double *val_P, const double *Y, int N,
+                 double *dC, int p, double ipi, int min_num_intervals, unsigned nthreads, double *sum_Q_out) {
+    for (long i = 0; i < N; i++) dC[i] = 0.0;
+    double y_min, y_max;
+    minmax_1d(Y, N, &y_min, &y_max);
+    int nb = orc_n_boxes_1d(y_min, y_max, ipi, min_num_intervals);
+    int n_terms = 3;
+    double *ch = (double *)malloc(sizeof(double) * (size_t)N * n_terms);
+    double *pot = (double *)calloc((size_t)N * n_terms, sizeof(double));
+    for (long j = 0; j < N; j++) {
+        ch[j * n_terms + 0] = 1;
+        ch[j * n_terms + 1] = Y[j];
+        ch[j * n_terms + 2] = Y[j] * Y[j];
+    }
+    double *blo = (double *)malloc(sizeof(double) * (size_t)nb);
+    double *bhi = (double *)malloc(sizeof(double) * (size_t)nb);
+    double *sp = (double *)malloc(sizeof(double) * (size_t)p);
+    double *yt = (double *)calloc((size_t)p * nb, sizeof(double));
+    double *kv = (double *)malloc(sizeof(double) * 2 * 2 * (size_t)p * nb);
+    orc_precompute_1d(y_min, y_max, nb, p, ORC_SQUARED_CAUCHY, blo, bhi, sp, yt, kv, 1.0);
+    orc_n_body_fft_1d(N, n_terms, Y, ch, nb, p, blo, bhi, sp, yt, kv, pot);
+    double sum_Q = 0;
+    for (long i = 0; i < N; i++) {
+        double phi1 = pot[i * n_terms + 0], phi2 = pot[i * n_terms + 1], phi3 = pot[i * n_terms + 2];
+        sum_Q += (1 + Y[i] * Y[i]) * phi1 - 2 * (Y[i] * phi2) + phi3;
+    }
+    sum_Q -= N;
+    *sum_Q_out = sum_Q;
+    double *pos_f = (double *)malloc(sizeof(double) * (size_t)N);
+    orc_attractive(row_P, col_P, val_P, Y, N, 1, 1.0, nthreads, pos_f);
+    for (long n = 0; n < N; n++) {
+        double neg = (Y[n] * pot[n * n_terms] - pot[n * n_terms + 1]) / sum_Q;
+        dC[n] = pos_f[n] - neg;
+    }
+    free(pos_f); free(ch); free(pot); free(blo); free(bhi); free(sp); free(yt); free(kv);
+}
+
+/* ---- tsne.cpp:647-754  computeFftGradientOneDVariableDf (1D, df != 1) ---- */
+void orc_grad_1d_df(const uint32_t *row_P, const uint32_t *col_P, const double *val_P, const double *Y, int N,
+                    double *dC, int p, double ipi, int min_num_intervals, unsigned nthreads, double df,
+                    double *sum_Q_out) {
+    for (long i = 0; i < N; i++) dC[i] = 0.0;
+    double y_min, y_max;
+    minmax_1d(Y, N, &y_min, &y_max);
+    int nb = orc_n_boxes_1d(y_min, y_max, ipi, min_num_intervals);
+    int sq_terms = 2;
+    double *sqc = (double *)malloc(sizeof(double) * (size_t)N * sq_terms);
+    double *sqp = (double *)calloc((size_t)N * sq_terms, sizeof(double));
+    for (long j = 0; j < N; j++) {
+        sqc[j * sq_terms + 0] = Y[j];
+        sqc[j * sq_terms + 1] = 1;
+    }
+    double *blo = (double *)malloc(sizeof(double) * (size_t)nb);
+    double *bhi = (double *)malloc(sizeof(double) * (size_t)nb);
+    double *sp = (double *)malloc(sizeof(double) * (size_t)p);
+    double *yt = (double *)calloc((size_t)p * nb, sizeof(double));
+    double *kv = (double *)malloc(sizeof(double) * 2 * 2 * (size_t)p * nb);
+    orc_precompute_1d(y_min, y_max, nb, p, ORC_SQUARED_GENERAL, blo, bhi, sp, yt, kv, df);
+    orc_n_body_fft_1d(N, sq_terms, Y, sqc, nb, p, blo, bhi, sp, yt, kv, sqp);
+    double *nsc = (double *)malloc(sizeof(double) * (size_t)N);
+    double *nsp = (double *)calloc((size_t)N, sizeof(double));
+    for (long j = 0; j < N; j++) nsc[j] = 1;
+    orc_precompute_1d(y_min, y_max, nb, p, ORC_GENERAL, blo, bhi, sp, yt, kv, df);
+    orc_n_body_fft_1d(N, 1, Y, nsc, nb, p, blo, bhi, sp, yt, kv, nsp);
+    double sum_Q = 0;
+    for (long i = 0; i < N; i++) sum_Q += nsp[i];
+    sum_Q -= N;
+    *sum_Q_out = sum_Q;
+    double *pos_f = (double *)malloc(sizeof(double) * (size_t)N);
+    orc_attractive(row_P, col_P, val_P, Y, N, 1, df, nthreads, pos_f);
+    for (long i = 0; i < N; i++) {
+        double h2 = sqp[i * sq_terms], h4 = sqp[i * sq_terms + 1];
+        double neg = (Y[i] * h4 - h2) / sum_Q;
+        dC[i] = (pos_f[i] - neg);
+    }
+    free(pos_f); free(sqc); free(sqp); free(nsc); free(nsp); free(blo); free(bhi); free(sp); free(yt); free(kv);
+}
+
+/* tsne.cpp:444-464 */
+int orc_compute_fft_gradient(const uint32_t *row_P, const uint32_t *col_P, const double *val_P,
+                             const double *Y, int N, int D, double *dC, int p, double ipi,
+                             int min_num_intervals, unsigned nthreads, double df, double *sum_Q) {
+    if (D == 1) {
+        if (df == 1.0) orc_grad_1d(row_P, col_P, val_P, Y, N, dC, p, ipi, min_num_intervals, nthreads, sum_Q);
+        else orc_grad_1d_df(row_P, col_P, val_P, Y, N, dC, p, ipi, min_num_intervals, nthreads, df, sum_Q);
+        return 0;
+    }
+    if (D == 2) {
+        if (df == 1.0) orc_grad_2d(row_P, col_P, val_P, Y, N, dC, p, ipi, min_num_intervals, nthreads, sum_Q);
+        else orc_grad_2d_df(row_P, col_P, val_P, Y, N, dC, p, ipi, min_num_intervals, nthreads, df, sum_Q);
+        return 0;
+    }
+    return -1;
+}
+
+/* tsne.cpp:1172-1228 (repulsion and Z of computeExactGradientTest) */
+void orc_exact_repulsion(const double *Y, int N, int D, double df, double *neg_f, double *sum_Q_out) {
+    double sum_Q = 0;
+    for (long n = 0; n < N; n++)
+        for (long m = 0; m < N; m++) {
+            if (n == m) continue;
+            double dd = 0;
+            for (int d = 0; d < D; d++) dd += (Y[n * D + d] - Y[m * D + d]) * (Y[n * D + d] - Y[m * D + d]);
+            sum_Q += 1.0 / pow(1.0 + dd / df, df);
+        }
+    for (long n = 0; n < N; n++) {
+        double acc[2] = {0, 0};
+        for (long m = 0; m < N; m++) {
+            if (n == m) continue;
+            double dd = 0;
+            for (int d = 0; d < D; d++) dd += (Y[n * D + d] - Y[m * D + d]) * (Y[n * D + d] - Y[m * D + d]);
+            double q = 1.0 / pow(1.0 + dd / df, df);
+            double w = pow(q, (df + 1.0) / df);
+            for (int d = 0; d < D; d++) acc[d] += w * (Y[n * D + d] - Y[m * D + d]) / sum_Q;
+        }
+        for (int d = 0; d < D; d++) neg_f[n * D + d] = acc[d];
+    }
+    *sum_Q_out = sum_Q;
+}
+
+/* ---- optimiser ---- */
+
+/* tsne.cpp:1851-1876 */
+void orc_zero_mean(double *X, int N, int D) {
+    double *mean = (double *)calloc((size_t)D, sizeof(double));
+    long nD = 0;
+    for (int n = 0; n < N; n++) {
+        for (int d = 0; d < D; d++) mean[d] += X[nD + d];
+        nD += D;
+    }
+    for (int d = 0; d < D; d++) mean[d] /= (double)N;
+    nD = 0;
+    for (int n = 0; n < N; n++) {
+        for (int d = 0; d < D; d++) X[nD + d] -= mean[d];
+        nD += D;
+    }
+    free(mean);
+}
+
+/* tsne.h:36 */
+static inline double sgn(double x) { return (x == .0 ? .0 : (x < .0 ? -1.0 : 1.0)); }
+
+/* tsne.cpp:478-514.  plain_descent selects the no-momentum branch (tsne.cpp:487-490). */
+void orc_update(double *Y, double *uY, double *gains, const double *dY, int N, int D, double momentum,
+                double learning_rate, double max_step_norm, int plain_descent) {
+    long ND = (long)N * D;
+    if (plain_descent) {
+        for (long i = 0; i < ND; i++) Y[i] = Y[i] - dY[i];
+        return;
+    }
+    for (long i = 0; i < ND; i++) gains[i] = (sgn(dY[i]) != sgn(uY[i])) ? (gains[i] + .2) : (gains[i] * .8);
+    for (long i = 0; i < ND; i++) if (gains[i] < .01) gains[i] = .01;
+    for (long i = 0; i < ND; i++) uY[i] = momentum * uY[i] - learning_rate * gains[i] * dY[i];
+    if (max_step_norm > 0) {
+        for (long i = 0; i < N; i++) {
+            double step = 0;
+            for (int j = 0; j < D; j++) step += uY[i * D + j] * uY[i * D + j];
+            step = sqrt(step);
+            if (step > max_step_norm)
+                for (int j = 0; j < D; j++) uY[i * D + j] *= (max_step_norm / step);
+        }
+    }
+    for (long i = 0; i < ND; i++) Y[i] = Y[i] + uY[i];
+}
+
+/* tsne.cpp:1329-1355 with the `C += temp` race removed: per-row partials summed in row order. */
+double orc_evaluate_error_fft(const uint32_t *row_P, const uint32_t *col_P, const double *val_P,
+                              const double *Y, int N, int D, double sum_Q, double df) {
+    double C = .0;
+    double buff[2];
+    for (int n = 0; n < N; n++) {
+        long ind1 = (long)n * D;
+        double temp = 0;
+        for (uint32_t i = row_P[n]; i < row_P[n + 1]; i++) {
+            double Q = .0;
+            long ind2 = (long)col_P[i] * D;
+            for (int d = 0; d < D; d++) buff[d] = Y[ind1 + d];
+            for (int d = 0; d < D; d++) buff[d] -= Y[ind2 + d];
+            for (int d = 0; d < D; d++) Q += buff[d] * buff[d];
+            Q = pow(1.0 / (1.0 + Q / df), df) / sum_Q;
+            temp += val_P[i] * log((val_P[i] + FLT_MIN) / (Q + FLT_MIN));
+        }
+        C += temp;
+    }
+    return C;
+}
+
+/* tsne.cpp:389-412, :437-577 (FFT branch) */
+int orc_run_fft(const uint32_t *row_P, const uint32_t *col_P, const double *val_P_in, int N, int no_dims,
+                double *Y, const orc_run_params *prm, double *costs) {
+    long ND = (long)N * no_dims;
+    uint32_t nnz = row_P[N];
+    double *val_P = (double *)malloc(sizeof(double) * (size_t)nnz);
+    memcpy(val_P, val_P_in, sizeof(double) * (size_t)nnz);
+    double *dY = (double *)malloc(sizeof(double) * (size_t)ND);
+    double *uY = (double *)calloc((size_t)ND, sizeof(double));
+    double *gains = (double *)malloc(sizeof(double) * (size_t)ND);
+    for (long i = 0; i < ND; i++) gains[i] = 1.0;
+    double momentum = prm->momentum;
+    double early = prm->early_exag_coeff;
+    if (early == 0) {
+        double max_sum_cols = 0;
+        for (int n = 0; n < N; n++) {
+            double rs = 0;
+            for (uint32_t i = row_P[n]; i < row_P[n + 1]; i++) rs += val_P[i];
+            if (rs > max_sum_cols) max_sum_cols = rs;
+        }
+        early = (1.0 / (prm->learning_rate * max_sum_cols));
+    }
+    for (uint32_t i = 0; i < nnz; i++) val_P[i] *= early;
+    for (long i = 0; i < N; i++) costs[i < prm->max_iter ? i : 0] = costs[i < prm->max_iter ? i : 0];
+    for (int i = 0; i < prm->max_iter; i++) costs[i] = 0.0;
+
+    double sum_Q = 0;
+    for (int iter = 0; iter < prm->max_iter; iter++) {
+        int rc = orc_compute_fft_gradient(row_P, col_P, val_P, Y, N, no_dims, dY, prm->n_interpolation_points,
+                                          prm->intervals_per_integer, prm->min_num_intervals, prm->nthreads,
+                                          prm->df, &sum_Q);
+        if (rc) { free(val_P); free(dY); free(uY); free(gains); return rc; }
+        if (prm->no_momentum_during_exag) {
+            if (iter > prm->stop_lying_iter)
+                orc_update(Y, uY, gains, dY, N, no_dims, momentum, prm->learning_rate, -1.0, 0);
+            else
+                orc_update(Y, uY, gains, dY, N, no_dims, momentum, prm->learning_rate, -1.0, 1);
+        } else {
+            orc_update(Y, uY, gains, dY, N, no_dims, momentum, prm->learning_rate, prm->max_step_norm, 0);
+        }
+        orc_zero_mean(Y, N, no_dims);
+        if (iter == prm->stop_lying_iter)
+            for (uint32_t i = 0; i < nnz; i++) val_P[i] /= early;
+        if (iter == prm->start_late_exag_iter)
+            for (uint32_t i = 0; i < nnz; i++) val_P[i] *= prm->late_exag_coeff;
+        if (iter == prm->mom_switch_iter) momentum = prm->final_momentum;
+        if ((iter + 1) % 50 == 0 || iter == prm->max_iter - 1) {
+            double C = orc_evaluate_error_fft(row_P, col_P, val_P, Y, N, no_dims, sum_Q, prm->df);
+            if (iter < prm->stop_lying_iter && prm->stop_lying_iter != -1) C = C / early - log(early);
+            if (iter >= prm->start_late_exag_iter && prm->start_late_exag_iter != -1)
+                C = C / prm->late_exag_coeff - log(prm->late_exag_coeff);
+            costs[iter] = C;
+        }
+    }
+    free(val_P); free(dY); free(uY); free(gains);
+    return 0;
+}
+
+/* ---- affinities ---- */
+
+/* tsne.cpp:1394-1469 with the starting beta exposed (the reference uses 1.0 at :1415). */
+double orc_d2s_bisect(const double *D, double *P, int N, double perplexity, double beta0, int ifSquared,
+                      int *iters, double *H_out) {
+    double min_beta = -DBL_MAX, max_beta = DBL_MAX, tol = 1e-5;
+    int max_iter = 200, iter = 0, found = 0;
+    double beta = beta0, sum_P = DBL_MIN, H = 0.0;
+    while (!found && iter < max_iter) {
+        for (int m = 0; m < N; m++) P[m] = exp(-beta * (ifSquared ? D[m] : D[m] * D[m]));
+        sum_P = DBL_MIN;
+        for (int m = 0; m < N; m++) sum_P += P[m];
+        H = 0.0;
+        for (int m = 0; m < N; m++) H += beta * ((ifSquared ? D[m] : D[m] * D[m]) * P[m]);
+        H = (H / sum_P) + log(sum_P);
+        double Hdiff = H - log(perplexity);
+        if (Hdiff < tol && -Hdiff < tol) {
+            found = 1;
+        } else {
+            if (Hdiff > 0) {
+                min_beta = beta;
+                if (max_beta == DBL_MAX || max_beta == -DBL_MAX) beta *= 2.0;
+                else beta = (beta + max_beta) / 2.0;
+            } else {
+                max_beta = beta;
+                if (min_beta == -DBL_MAX || min_beta == DBL_MAX) beta /= 2.0;
+                else beta = (beta + min_beta) / 2.0;
+            }
+        }
+        iter++;
+    }
+    for (int m = 0; m < N; m++) P[m] /= sum_P;
+    if (iters) *iters = iter;
+    if (H_out) *H_out = H;
+    return beta;
+}
+
+/* tsne.cpp:1394-1469 */
+double orc_distances2similarities(const double *D, double *P, int N, int n, double perplexity,
+                                  double sigma, int ifSquared) {
+    double sum_P, beta;
+    if (perplexity > 0) {
+        double min_beta = -DBL_MAX, max_beta = DBL_MAX, tol = 1e-5;
+        int max_iter = 200, iter = 0, found = 0;
+        beta = 1.0;
+        sum_P = DBL_MIN;
+        while (!found && iter < max_iter) {
+            for (int m = 0; m < N; m++) P[m] = exp(-beta * (ifSquared ? D[m] : D[m] * D[m]));
+            if (n >= 0) P[n] = DBL_MIN;
+            sum_P = DBL_MIN;
+            for (int m = 0; m < N; m++) sum_P += P[m];
+            double H = 0.0;
+            for (int m = 0; m < N; m++) H += beta * ((ifSquared ? D[m] : D[m] * D[m]) * P[m]);
+            H = (H / sum_P) + log(sum_P);
+            double Hdiff = H - log(perplexity);
+            if (Hdiff < tol && -Hdiff < tol) {
+                found = 1;
+            } else {
+                if (Hdiff > 0) {
+                    min_beta = beta;
+                    if (max_beta == DBL_MAX || max_beta == -DBL_MAX) beta *= 2.0;
+                    else beta = (beta + max_beta) / 2.0;
+                } else {
+                    max_beta = beta;
+                    if (min_beta == -DBL_MAX || min_beta == DBL_MAX) beta /= 2.0;
+                    else beta = (beta + min_beta) / 2.0;
+                }
+            }
+            iter++;
+        }
+    } else {
+        beta = 1 / (2 * sigma * sigma);
+        for (int m = 0; m < N; m++) P[m] = exp(-beta * (ifSquared ? D[m] : D[m] * D[m]));
+        if (n >= 0) P[n] = DBL_MIN;
+        sum_P = DBL_MIN;
+        for (int m = 0; m < N; m++) sum_P += P[m];
+    }
+    for (int m = 0; m < N; m++) P[m] /= sum_P;
+    return beta;
+}
+
+/* tsne.cpp:1474-1500 */
+double orc_distances2similarities_list(const double *D, double *P, int N, int n, double perplexity,
+                                       double sigma, int ifSquared, int perplexity_list_length,
+                                       const double *perplexity_list) {
+    (void)ifSquared;
+    if (perplexity != 0) return orc_distances2similarities(D, P, N, n, perplexity, sigma, 1);
+    double *tmp = (double *)malloc(sizeof(double) * (size_t)N);
+    double beta = orc_distances2similarities(D, P, N, n, perplexity_list[0], sigma, 1);
+    for (int m = 1; m < perplexity_list_length; m++) {
+        beta = orc_distances2similarities(D, tmp, N, n, perplexity_list[m], sigma, 1);
+        for (int i = 0; i < N; i++) P[i] += tmp[i];
+    }
+    for (int i = 0; i < N; i++) P[i] /= perplexity_list_length;
+    free(tmp);
+    return beta;
+}
+
+/* Exact kNN by brute force; distances as vptree.h's euclidean_distance (sqrt of the ordered sum). */
+typedef struct {
+    const double *X;
+    int N, D, K;
+    int32_t *idx;
+    double *dist;
+} knn_ctx;
+
+static void knn_body(int lo, int hi, void *c_) {
+    knn_ctx *c = (knn_ctx *)c_;
+    int K = c->K, D = c->D, N = c->N;
+    double *bd = (double *)malloc(sizeof(double) * (size_t)(K + 1));
+    int32_t *bi = (int32_t *)malloc(sizeof(int32_t) * (size_t)(K + 1));
+    for (int n = lo; n < hi; n++) {
+        int cnt = 0;
+        const double *xn = c->X + (long)n * D;
+        for (int m = 0; m < N; m++) {
+            if (m == n) continue;
+            const double *xm = c->X + (long)m * D;
+            double dd = .0;
+            for (int d = 0; d < D; d++) { double t = xn[d] - xm[d]; dd += t * t; }
+            double dist = sqrt(dd);
+            if (cnt == K && dist >= bd[K - 1]) continue;
+            int pos = cnt < K ? cnt : K - 1;
+            while (pos > 0 && bd[pos - 1] > dist) { bd[pos] = bd[pos - 1]; bi[pos] = bi[pos - 1]; pos--; }
+            bd[pos] = dist;
+            bi[pos] = m;
+            if (cnt < K) cnt++;
+        }
+        for (int k = 0; k < K; k++) {
+            c->idx[(long)n * K + k] = bi[k];
+            c->dist[(long)n * K + k] = bd[k];
+        }
+    }
+    free(bd);
+    free(bi);
+}
+
+void orc_knn_exact(const double *X, int N, int D, int K, int32_t *idx, double *dist, unsigned nthreads) {
+    knn_ctx c = {X, N, D, K, idx, dist};
+    orc_parallel_for(nthreads, N, knn_body, &c);
+}
+
+/* tsne.cpp:1684-1706: per point, distances2similarities on the K non-squared distances
+ * (ifSquared = false, self excluded), then the row is stored in kNN order. */
+void orc_rows_from_knn(const int32_t *idx, const double *dist, int N, int K, double perplexity,
+                       double sigma, int perplexity_list_length, const double *perplexity_list,
+                       uint32_t *row_P, uint32_t *col_P, double *val_P) {
+    row_P[0] = 0;
+    for (int n = 0; n < N; n++) row_P[n + 1] = row_P[n] + (uint32_t)K;
+    double *cur = (double *)malloc(sizeof(double) * (size_t)K);
+    for (int n = 0; n < N; n++) {
+        /* The call site passes ifSquared = false, but the list overload forwards ifSquared = true
+         * (tsne.cpp:1479, :1486, :1489), so the non-squared kNN distances enter exp(-beta * d)
+         * unsquared.  Restated as the reference behaves. */
+        orc_distances2similarities_list(dist + (long)n * K, cur, K, -1, perplexity, sigma, 0,
+                                        perplexity_list_length, perplexity_list);
+        for (int m = 0; m < K; m++) {
+            col_P[row_P[n] + m] = (uint32_t)idx[(long)n * K + m];
+            val_P[row_P[n] + m] = cur[m];
+        }
+    }
+    free(cur);
+}
+
+/* tsne.cpp:1730-1828 */
+int orc_symmetrize(const uint32_t *row_P, const uint32_t *col_P, const double *val_P, int N,
+                   uint32_t *sym_row_P, uint32_t *sym_col_P, double *sym_val_P) {
+    int *row_counts = (int *)calloc((size_t)N, sizeof(int));
+    for (int n = 0; n < N; n++) {
+        for (uint32_t i = row_P[n]; i < row_P[n + 1]; i++) {
+            int present = 0;
+            for (uint32_t m = row_P[col_P[i]]; m < row_P[col_P[i] + 1]; m++)
+                if (col_P[m] == (uint32_t)n) present = 1;
+            if (present) row_counts[n]++;
+            else { row_counts[n]++; row_counts[col_P[i]]++; }
+        }
+    }
+    int no_elem = 0;
+    for (int n = 0; n < N; n++) no_elem += row_counts[n];
+    sym_row_P[0] = 0;
+    for (int n = 0; n < N; n++) sym_row_P[n + 1] = sym_row_P[n] + (uint32_t)row_counts[n];
+    int *offset = (int *)calloc((size_t)N, sizeof(int));
+    for (int n = 0; n < N; n++) {
+        for (uint32_t i = row_P[n]; i < row_P[n + 1]; i++) {
+            int present = 0;
+            uint32_t c = col_P[i];
+            for (uint32_t m = row_P[c]; m < row_P[c + 1]; m++) {
+                if (col_P[m] == (uint32_t)n) {
+                    present = 1;
+                    if ((uint32_t)n <= c) {
+                        sym_col_P[sym_row_P[n] + offset[n]] = c;
+                        sym_col_P[sym_row_P[c] + offset[c]] = (uint32_t)n;
+                        sym_val_P[sym_row_P[n] + offset[n]] = val_P[i] + val_P[m];
+                        sym_val_P[sym_row_P[c] + offset[c]] = val_P[i] + val_P[m];
+                    }
+                }
+            }
+            if (!present) {
+                sym_col_P[sym_row_P[n] + offset[n]] = c;
+                sym_col_P[sym_row_P[c] + offset[c]] = (uint32_t)n;
+                sym_val_P[sym_row_P[n] + offset[n]] = val_P[i];
+                sym_val_P[sym_row_P[c] + offset[c]] = val_P[i];
+            }
+            if (!present || ((uint32_t)n <= c)) {
+                offset[n]++;
+                if (c != (uint32_t)n) offset[c]++;
+            }
+        }
+    }
+    for (int i = 0; i < no_elem; i++) sym_val_P[i] /= 2.0;
+    free(offset);
+    free(row_counts);
+    return no_elem;
+}
