@@ -1,0 +1,197 @@
+#!/usr/bin/env python3
+"""Gradient iterations per second at N = 1,000,000 in 2D on one MI355X.
+
+A step is one iteration of the t-SNE hot loop on the device-resident workload: the FFT-interpolated
+gradient (attractive CSR term, spread, FFT convolution, interpolation) plus the gains/momentum update,
+the step-norm clip and zeroMean (TSNE::run, tsne.cpp:440-531), for the FP32 form.  The FP64 plain GPU
+path is timed the same way beside it, and the oracle (the C restatement of the reference CPU path,
+FFT by the oracle's own mixed-radix code since FFTW is absent) on a bounded sample of gradient calls.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "fit-sne_amd"))
+
+METRIC = "grad iters/sec at N=1M 2D, FP32 vs plain FP64 GPU @ rel err≤1e-4; %HBM peak"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def timed_steps(ctx, steps, warmup, prm):
+    for _ in range(warmup):
+        ctx.step(**prm)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.step(**prm)
+    ctx.sync()
+    return time.perf_counter() - t0
+
+
+def kernel_profile(ctx, steps, prm):
+    """Per-kernel HIP-event durations over `steps` further iterations on the context's stream."""
+    ctx.set_timing(True)
+    for _ in range(steps):
+        ctx.step(**prm)
+    stats = ctx.kernel_stats()
+    ctx.set_timing(False)
+    return stats
+
+
+def cpu_baseline(P, Y32, calls):
+    """The oracle's gradient (FP64 C restatement of computeFftGradient, PARALLEL_FOR threading kept)
+    on the same Y and P; returns (iters/s, threads, dC, sum_Q)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import orc
+
+    row, col, val = P
+    Pu = (row.astype(np.uint32), col.astype(np.uint32), val)
+    nthreads = os.cpu_count() or 1
+    g = zq = None
+    t0 = time.perf_counter()
+    for i in range(calls):
+        g, zq = orc.gradient(Pu, Y32, nthreads=nthreads)
+        log(f"cpu baseline gradient {i + 1}/{calls} done ({time.perf_counter() - t0:.1f}s)")
+    dt = time.perf_counter() - t0
+    return calls / dt, nthreads, g, zq
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--cpu-calls", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=5)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run one plain 64 MiB device copy (torch) as the reference kernel for "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE scaling")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one MI355X; multi-GPU runs are not part of this build (see DESIGN.md for how "
+              "the path would shard)", file=sys.stderr)
+        sys.exit(2)
+
+    import fitsne_gpu as fg
+    import synth
+
+    if args.copy_calibration:
+        import torch
+
+        a = torch.ones(64 * 2 ** 20 // 4, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log("copy calibration: 64 MiB read + 64 MiB written by one copy kernel")
+
+    N = args.n
+    log(f"generating the C2 workload N={N}")
+    P, Y, _ = synth.c2_workload(N)
+    nnz = int(P[0][-1])
+    lr = max(200.0, N / 12.0)
+    prm = dict(momentum=0.8, learning_rate=lr, max_step_norm=5.0)
+
+    log("FP32 context: upload P, Y")
+    c32 = fg.Context(no_dims=2, precision=fg.FP32)
+    c32.set_P(*P)
+    c32.set_Y(Y)
+    g32, z32 = c32.gradient()
+    geo = c32.last_geometry()
+    log(f"geometry: n_boxes={geo['n_boxes']} n1d={geo['n1d']} fft_len={geo['fft_len']} nnz={geo['nnz']}")
+
+    res64 = {}
+    g64 = z64 = None
+    if not args.no_fp64:
+        log("FP64 plain context")
+        c64 = fg.Context(no_dims=2, precision=fg.FP64_PLAIN)
+        c64.set_P(*P)
+        c64.set_Y(Y)
+        g64, z64 = c64.gradient()
+        dt64 = timed_steps(c64, args.steps, args.warmup, prm)
+        res64 = {"ms_per_step": 1e3 * dt64 / args.steps, "iters_per_s": args.steps / dt64}
+        log(f"FP64 plain: {res64['ms_per_step']:.3f} ms/step")
+        c64.close()
+
+    log("FP32 timed steps")
+    dt = timed_steps(c32, args.steps, args.warmup, prm)
+    ms = 1e3 * dt / args.steps
+    value = args.steps / dt
+    log(f"FP32: {ms:.3f} ms/step, {value:.1f} iters/s")
+    stats = kernel_profile(c32, max(args.profile_steps, 1), prm)
+    geo_end = c32.last_geometry()
+    c32.close()
+
+    cnt, tot = stats.get("attractive", (0, 0.0))
+    att_ms = tot / max(cnt, 1)
+    att_bytes = 8 * (N + 1) + nnz * (4 + 4) + 2 * N * 2 * 4
+    achieved = att_bytes / (att_ms * 1e-3) / 1e9 if att_ms > 0 else 0.0
+    kernels_ms = {k: round(v[1] / max(v[0], 1), 4) for k, v in sorted(stats.items(), key=lambda kv: -kv[1][1])}
+
+    acc = {}
+    if g64 is not None:
+        acc["fp32_vs_fp64_gpu_grad_rel"] = rel(g32, g64)
+        acc["fp32_vs_fp64_gpu_Z_rel"] = abs(z32 - z64) / abs(z64)
+    cpu = None
+    if not args.no_cpu:
+        log(f"CPU baseline: {args.cpu_calls} oracle gradient calls at N={N}")
+        Y32 = Y.astype(np.float32).astype(np.float64)
+        v, nthr, gref, zref = cpu_baseline(P, Y32, args.cpu_calls)
+        cpu = {"value": v, "unit": "grad iters/s", "cores": nthr, "kind": "port",
+               "sample": f"{args.cpu_calls} full gradient evaluations of the same N={N} workload (same Y, P) by the "
+                         f"oracle's C restatement of computeFftGradient, {nthr} threads for the reference's "
+                         f"PARALLEL_FOR stages, serial spread/FFT as in the reference, own FFT (no FFTW)"}
+        acc["fp32_vs_oracle_grad_rel"] = rel(g32, gref)
+        acc["fp32_vs_oracle_Z_rel"] = abs(z32 - zref) / abs(zref)
+        if g64 is not None:
+            acc["fp64_vs_oracle_grad_rel_at_fp32_rounded_Y"] = rel(g64, gref)
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "grad iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Gaussian mixture, generated in place)",
+        "config": {"workload": "C2: N=1,000,000 2D, df=1, symmetric P ~90 nnz/row, Y span ~[-80,80]",
+                   "N": N, "nnz": nnz, "n_boxes_per_dim": geo["n_boxes"], "n1d": geo["n1d"],
+                   "fft_len": geo["fft_len"], "n_boxes_end": geo_end["n_boxes"], "learning_rate": lr,
+                   "momentum": 0.8, "max_step_norm": 5.0, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": "attractive", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes": att_bytes, "avg_ms": att_ms},
+        "cpu_baseline": cpu,
+        "fp64_plain": res64,
+        "speedup_vs_fp64": (res64["ms_per_step"] / ms) if res64 else None,
+        "accuracy": acc,
+        "kernels_ms": kernels_ms,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,1510 @@
+// fitsne_host.hip -- host side of libfitsne.so: contexts, FFT plans, per-iteration geometry, the FP64
+// plain and FP32 gradient pipelines, the mirror tier and the C-ABI of include/fitsne_abi.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/fitsne_abi.h"
+#include "fitsne_kernels.hpp"
+
+using namespace fitsne;
+
+// ============================================================================ errors
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPTRY(expr)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail(FITSNE_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));     \
+    } while (0)
+
+#define TRY(expr)                  \
+    do {                           \
+        int r_ = (expr);           \
+        if (r_ != FITSNE_OK) return r_; \
+    } while (0)
+
+// ============================================================================ limits
+
+constexpr int kMaxTot32 = 8192;   // complex FP32 elements per FFT block (64 KB LDS)
+constexpr int kMaxTot64 = 4096;   // complex FP64 elements per FFT block (64 KB LDS)
+constexpr int kConv1dThreads = 1024;
+
+// ============================================================================ device buffers
+
+struct DBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    template <typename T>
+    T *as() const { return reinterpret_cast<T *>(p); }
+};
+
+static int ensure(DBuf &b, size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    if (bytes <= b.cap) return FITSNE_OK;
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+    size_t want = bytes + bytes / 8;  // headroom: grids change size every iteration
+    if (hipMalloc(&b.p, want) != hipSuccess) {
+        b.p = nullptr;
+        return fail(FITSNE_E_ALLOC, "hipMalloc of " + std::to_string(want) + " bytes failed");
+    }
+    b.cap = want;
+    return FITSNE_OK;
+}
+
+static void release(DBuf &b) {
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+}
+
+// ============================================================================ FFT plans
+
+static int fft_size(int n) {
+    long best = -1;
+    for (long p3 = 1; p3 < 2L * n + 2; p3 *= 3) {
+        long m = p3;
+        while (m < n) m *= 2;
+        if (best < 0 || m < best) best = m;
+    }
+    return (int)best;
+}
+
+static FftPlan make_plan(int n) {
+    FftPlan pl{};
+    pl.n = n;
+    int m = n;
+    while (m % 4 == 0) { pl.rad[pl.nst++] = 4; m /= 4; }
+    while (m % 2 == 0) { pl.rad[pl.nst++] = 2; m /= 2; }
+    while (m % 3 == 0) { pl.rad[pl.nst++] = 3; m /= 3; }
+    return pl;  // m == 1 by construction of fft_size
+}
+
+// ============================================================================ geometry (host, FP64)
+
+// tsne.cpp:1065-1077
+static int n_boxes_2d(double mn, double mx, double ipi, int mni) {
+    int nb = (int)std::fmax(mni, (mx - mn) / ipi);
+    static const int allowed[20] = {25, 36, 50, 55, 60, 65, 70, 75, 80, 85, 90, 96, 100, 110, 120, 130, 140, 150, 175, 200};
+    if (nb < allowed[19]) {
+        int c;
+        for (c = 0; allowed[c] < nb; c++) {}
+        nb = allowed[c];
+    }
+    return nb;
+}
+
+// tsne.cpp:663
+static int n_boxes_1d(double mn, double mx, double ipi, int mni) { return (int)std::fmax(mni, (mx - mn) / ipi); }
+
+// Node spacings and Lagrange denominators as nbodyfft.cpp:30-34 and :313-321.
+static void fill_spacings(Geo &g, int p) {
+    double h = 1 / (double)p;
+    g.sp[0] = h / 2;
+    for (int i = 1; i < p; i++) g.sp[i] = g.sp[i - 1] + h;
+    for (int i = 0; i < p; i++) {
+        g.den[i] = 1;
+        for (int j = 0; j < p; j++)
+            if (i != j) g.den[i] *= g.sp[i] - g.sp[j];
+    }
+}
+
+// Geometry of one gradient from the (reference-rule) min and max, as precompute_2d / precompute
+// compute it (nbodyfft.cpp:15-46, :257-280) and n_body_fft re-derives the box width (:79-80, :344-345).
+static Geo make_geo(int D, int p, double mn, double mx, int nb) {
+    Geo g{};
+    g.D = D;
+    g.p = p;
+    g.nb = nb;
+    g.mn = mn;
+    g.bw0 = (mx - mn) / (double)nb;
+    const double lower0 = 0 * g.bw0 + mn, upper0 = (0 + 1) * g.bw0 + mn;
+    g.bw = upper0 - lower0;
+    g.h = (1 / (double)p) * g.bw0;
+    fill_spacings(g, p);
+    g.n1d = p * nb;
+    g.Mp = fft_size(2 * g.n1d - 1);
+    g.H = g.Mp / 2 + 1;
+    g.nbox = D == 2 ? nb * nb : nb;
+    return g;
+}
+
+// ============================================================================ context
+
+struct KStat {
+    int count = 0;
+    double ms = 0;
+};
+
+struct fitsne_ctx {
+    int device = 0;
+    int precision = FITSNE_FP32;
+    int D = 2, p = 3, mni = 50;
+    double ipi = 1.0, df = 1.0;
+    hipStream_t stream = nullptr;
+
+    int N = 0;
+    long long nnz = 0;
+    double exag = 1.0;  // current multiplier on P
+    // P
+    DBuf rowP, colP, val;  // rowP int64[N+1], colP uint32[nnz], val T[nnz]
+    // embedding state (element type = float for FP32, double for FP64 plain)
+    DBuf Y, uY, gains, pos, dC;
+    // sort (FP32)
+    DBuf boxof, rank, count, start, sorted;
+    // grids, spectra
+    DBuf gin, gout, spec, ospec, Trow, KQ, K1, KS, KT, zpart, zbuf, part, mean, mmpart, mm;
+    // plain path extras
+    DBuf xs, ys, box, wx, wy, ch, w, V, potA, potB;
+    // twiddles per FFT length
+    std::map<int, DBuf> tw32, tw64;
+    // min/max handoff from the last step
+    double *h_mm = nullptr;  // pinned [4]
+    hipEvent_t mm_event = nullptr;
+    bool mm_pending = false, mm_valid = false;
+    double mm_min = 0, mm_max = 0;
+    // kernel-spectrum cache key
+    bool kcache = false;
+    double kc_mn = 0, kc_mx = 0;
+    int kc_nb = 0;
+    // last geometry
+    Geo last{};
+    double last_min = 0, last_max = 0;
+    double last_sumQ = 0;
+    bool have_grad = false;
+    // spatial order of the points (FP32 form): internal row r holds the caller's point perm[r]
+    bool reorder_on_set_Y = true;
+    std::vector<int> perm, inv;         // internal -> caller, caller -> internal
+    std::vector<int64_t> rowP_host;     // internal-order row offsets
+    DBuf inv_d, colP2, val2, tmp;       // device copy of inv; double buffers for permutation
+    bool permuted = false;
+    // timing
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    struct Pend {
+        const char *name;
+        hipEvent_t a, b;
+    };
+    std::vector<Pend> pend;
+    std::map<std::string, KStat> stats;
+};
+
+static hipEvent_t take_event(fitsne_ctx *c) {
+    if (!c->ev_pool.empty()) {
+        hipEvent_t e = c->ev_pool.back();
+        c->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+template <typename... KA, typename... A>
+static void launch(fitsne_ctx *c, const char *name, void (*k)(KA...), dim3 grid, dim3 block, size_t shm,
+                   A... args) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (c->timing) {
+        a = take_event(c);
+        b = take_event(c);
+        if (a) (void)hipEventRecord(a, c->stream);
+    }
+    k<<<grid, block, shm, c->stream>>>(args...);
+    if (c->timing && a && b) {
+        (void)hipEventRecord(b, c->stream);
+        c->pend.push_back({name, a, b});
+    }
+}
+
+static int flush_timing(fitsne_ctx *c) {
+    if (c->pend.empty()) return FITSNE_OK;
+    HIPTRY(hipStreamSynchronize(c->stream));
+    for (auto &p : c->pend) {
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, p.a, p.b);
+        KStat &s = c->stats[p.name];
+        s.count++;
+        s.ms += ms;
+        c->ev_pool.push_back(p.a);
+        c->ev_pool.push_back(p.b);
+    }
+    c->pend.clear();
+    return FITSNE_OK;
+}
+
+static int sync_ctx(fitsne_ctx *c) {
+    HIPTRY(hipGetLastError());
+    HIPTRY(hipStreamSynchronize(c->stream));
+    return flush_timing(c);
+}
+
+template <typename T>
+static int twiddles(fitsne_ctx *c, int n, const cx<T> **out) {
+    auto &m = std::is_same<T, float>::value ? c->tw32 : c->tw64;
+    auto it = m.find(n);
+    if (it == m.end()) {
+        DBuf b;
+        TRY(ensure(b, sizeof(cx<T>) * (size_t)n));
+        launch(c, "twiddles", k_twiddles<T>, dim3((n + 255) / 256), dim3(256), 0, b.as<cx<T>>(), n);
+        it = m.emplace(n, b).first;
+    }
+    *out = it->second.template as<cx<T>>();
+    return FITSNE_OK;
+}
+
+static int blocks_for(long n, int per = kThreads) { return (int)((n + per - 1) / per); }
+
+static int sw_for(long long nnz, int N) {
+    const double avg = N > 0 ? (double)nnz / N : 0;
+    return avg >= 48 ? 64 : (avg >= 20 ? 32 : 16);
+}
+
+// ---------------------------------------------------------------- min / max of the resident Y
+
+// Flat index (internal layout) of the caller's first coordinate, xs[0]: the value the 2D `else if`
+// rule never tests against the running min when it opens the sequence.
+static long first_coord(const fitsne_ctx *c) { return (long)(c->inv.empty() ? 0 : c->inv[0]) * c->D; }
+static long skip_coord(const fitsne_ctx *c) { return c->D == 2 ? first_coord(c) : -1; }
+
+template <typename TV>
+static int minmax_now(fitsne_ctx *c, double *mn, double *mx) {
+    const long n = (long)c->N * c->D;
+    const int nblk = std::min(blocks_for(n), 1024);
+    TRY(ensure(c->mmpart, sizeof(double) * 2 * nblk));
+    TRY(ensure(c->mm, sizeof(double) * 4));
+    launch(c, "minmax_partial", k_minmax_partial<TV>, dim3(nblk), dim3(kThreads), 0, c->Y.as<const TV>(), n,
+           skip_coord(c), c->mmpart.as<double>());
+    launch(c, "minmax_final", k_minmax_final<TV>, dim3(1), dim3(kWave), 0, c->mmpart.as<const double>(), nblk,
+           c->Y.as<const TV>(), first_coord(c), c->mm.as<double>());
+    HIPTRY(hipMemcpyAsync(c->h_mm, c->mm.p, sizeof(double) * 3, hipMemcpyDeviceToHost, c->stream));
+    TRY(sync_ctx(c));
+    *mx = c->h_mm[0];
+    *mn = c->h_mm[1];
+    return FITSNE_OK;
+}
+
+// Resolve the reference's 2D `else if` min (tsne.cpp:1045-1048): equal to min(rest) unless Y[0] is the
+// strict minimum; then replay the sequential rule on the device.
+template <typename TV>
+static int resolve_quirk(fitsne_ctx *c, double y0, double *mn, double *mx) {
+    if (c->D != 2 || !(y0 < *mn)) return FITSNE_OK;
+    launch(c, "minmax_seq", k_minmax_seq<TV>, dim3(1), dim3(1024), 0, c->Y.as<const TV>(),
+           c->permuted ? c->inv_d.as<const int>() : (const int *)nullptr, (long)c->N * 2, c->mm.as<double>());
+    HIPTRY(hipMemcpyAsync(c->h_mm, c->mm.p, sizeof(double) * 2, hipMemcpyDeviceToHost, c->stream));
+    TRY(sync_ctx(c));
+    *mx = c->h_mm[0];
+    *mn = c->h_mm[1];
+    return FITSNE_OK;
+}
+
+template <typename TV>
+static int current_minmax(fitsne_ctx *c, double *mn, double *mx) {
+    if (c->mm_pending) {
+        HIPTRY(hipEventSynchronize(c->mm_event));
+        c->mm_pending = false;
+        c->mm_valid = true;
+        c->mm_max = c->h_mm[0];
+        c->mm_min = c->h_mm[1];
+        double y0 = c->h_mm[2];
+        TRY(resolve_quirk<TV>(c, y0, &c->mm_min, &c->mm_max));
+    }
+    if (!c->mm_valid) {
+        TRY(minmax_now<TV>(c, &c->mm_min, &c->mm_max));
+        double y0 = c->h_mm[2];
+        TRY(resolve_quirk<TV>(c, y0, &c->mm_min, &c->mm_max));
+        c->mm_valid = true;
+    }
+    *mn = c->mm_min;
+    *mx = c->mm_max;
+    return FITSNE_OK;
+}
+
+static int geometry(fitsne_ctx *c, double mn, double mx, Geo *g) {
+    if (!(std::isfinite(mn) && std::isfinite(mx)))
+        return fail(FITSNE_E_ARG, "embedding contains non-finite coordinates");
+    const int nb = c->D == 2 ? n_boxes_2d(mn, mx, c->ipi, c->mni) : n_boxes_1d(mn, mx, c->ipi, c->mni);
+    if (nb < 1) return fail(FITSNE_E_ARG, "n_boxes < 1");
+    *g = make_geo(c->D, c->p, mn, mx, nb);
+    const bool f32 = c->precision == FITSNE_FP32;
+    if (c->D == 2) {
+        const int lim = f32 ? kMaxTot32 / 4 : kMaxTot64 / 2;
+        if (g->Mp > lim)
+            return fail(FITSNE_E_UNSUPPORTED, "2D grid of " + std::to_string(g->n1d) + " nodes/dim (FFT length " +
+                                                  std::to_string(g->Mp) + ") exceeds this build's limit " +
+                                                  std::to_string(lim));
+    } else {
+        const int lim = f32 ? kMaxTot32 / 2 : kMaxTot64;
+        if (g->Mp > lim)
+            return fail(FITSNE_E_UNSUPPORTED, "1D grid FFT length " + std::to_string(g->Mp) + " exceeds " +
+                                                  std::to_string(lim));
+    }
+    c->last = *g;
+    c->last_min = mn;
+    c->last_max = mx;
+    return FITSNE_OK;
+}
+
+// ============================================================================ kernel dispatch tables
+
+template <int D>
+static auto pick_spread_f32(int p) -> void (*)(const float *, const int *, const int *, Geo, float4 *) {
+    switch (p) {
+    case 2: return k_spread_f32<2, D>;
+    case 3: return k_spread_f32<3, D>;
+    case 4: return k_spread_f32<4, D>;
+    default: return k_spread_f32<5, D>;
+    }
+}
+template <int D>
+static auto pick_interp_f32(int p) -> decltype(&k_interp_f32<3, D>) {
+    switch (p) {
+    case 2: return k_interp_f32<2, D>;
+    case 3: return k_interp_f32<3, D>;
+    case 4: return k_interp_f32<4, D>;
+    default: return k_interp_f32<5, D>;
+    }
+}
+template <int D>
+static auto pick_weights(int p) -> decltype(&k_weights_f64<3, D>) {
+    switch (p) {
+    case 2: return k_weights_f64<2, D>;
+    case 3: return k_weights_f64<3, D>;
+    case 4: return k_weights_f64<4, D>;
+    default: return k_weights_f64<5, D>;
+    }
+}
+template <int D>
+static auto pick_spread_f64(int p) -> decltype(&k_spread_f64<3, D>) {
+    switch (p) {
+    case 2: return k_spread_f64<2, D>;
+    case 3: return k_spread_f64<3, D>;
+    case 4: return k_spread_f64<4, D>;
+    default: return k_spread_f64<5, D>;
+    }
+}
+template <int D>
+static auto pick_interp_f64(int p) -> decltype(&k_interp_f64<3, D>) {
+    switch (p) {
+    case 2: return k_interp_f64<2, D>;
+    case 3: return k_interp_f64<3, D>;
+    case 4: return k_interp_f64<4, D>;
+    default: return k_interp_f64<5, D>;
+    }
+}
+
+template <typename TV, int D>
+static void launch_attractive(fitsne_ctx *c, TV scale, TV *pos) {
+    const int sw = sw_for(c->nnz, c->N);
+    const TV inv_df = (TV)(1.0 / c->df);
+    const int df1 = c->df == 1.0;
+    const int rpb = kThreads / sw;
+    const int nblk = (c->N + rpb - 1) / rpb;
+    dim3 grid(8 * ((nblk + 7) / 8));
+    auto rp = c->rowP.as<const int64_t>();
+    auto cp = c->colP.as<const uint32_t>();
+    auto vp = c->val.as<const TV>();
+    auto yp = c->Y.as<const TV>();
+    if (sw == 64)
+        launch(c, "attractive", k_attractive<TV, int64_t, uint32_t, D, 64>, grid, dim3(kThreads), 0, rp, cp, vp, yp,
+               c->N, inv_df, df1, scale, pos);
+    else if (sw == 32)
+        launch(c, "attractive", k_attractive<TV, int64_t, uint32_t, D, 32>, grid, dim3(kThreads), 0, rp, cp, vp, yp,
+               c->N, inv_df, df1, scale, pos);
+    else
+        launch(c, "attractive", k_attractive<TV, int64_t, uint32_t, D, 16>, grid, dim3(kThreads), 0, rp, cp, vp, yp,
+               c->N, inv_df, df1, scale, pos);
+}
+
+// ============================================================================ FP32 pipeline
+
+// mode 0: gradient into c->dC (float); mode 1: gradient + optimiser update + zeroMean.
+template <int D>
+static int pipeline_f32(fitsne_ctx *c, int mode, const fitsne_step_params *sp) {
+    const int N = c->N;
+    const int p = c->p;
+    const bool df1 = c->df == 1.0;
+    TRY(ensure(c->pos, sizeof(float) * (size_t)N * D));
+    launch_attractive<float, D>(c, (float)c->exag, c->pos.as<float>());
+
+    double mn, mx;
+    TRY(current_minmax<float>(c, &mn, &mx));
+    Geo g;
+    TRY(geometry(c, mn, mx, &g));
+    const FftPlan pl = make_plan(g.Mp);
+    const cx<float> *tw;
+    TRY(twiddles<float>(c, g.Mp, &tw));
+
+    // counting sort of the points by box
+    TRY(ensure(c->boxof, sizeof(int) * (size_t)N));
+    TRY(ensure(c->rank, sizeof(int) * (size_t)N));
+    TRY(ensure(c->sorted, sizeof(int) * (size_t)N));
+    if (c->count.cap < sizeof(int) * (size_t)g.nbox) {
+        // fresh counters are zeroed on the context's stream (k_scan_counts re-zeroes them after use)
+        release(c->count);
+        TRY(ensure(c->count, sizeof(int) * (size_t)g.nbox));
+        const long nc = (long)(c->count.cap / sizeof(int));
+        launch(c, "fill", k_fill<int>, dim3(std::min(blocks_for(nc), 2048)), dim3(kThreads), 0, c->count.as<int>(), nc,
+               0);
+    }
+    TRY(ensure(c->start, sizeof(int) * ((size_t)g.nbox + 1)));
+    launch(c, "box_count", k_box_count<float, D>, dim3(blocks_for(N)), dim3(kThreads), 0, c->Y.as<const float>(), N, g,
+           c->boxof.as<int>(), c->rank.as<int>(), c->count.as<int>());
+    launch(c, "scan_counts", k_scan_counts, dim3(1), dim3(1024), 0, c->count.as<int>(), c->start.as<int>(), g.nbox);
+    launch(c, "scatter", k_scatter, dim3(blocks_for(N)), dim3(kThreads), 0, c->boxof.as<const int>(),
+           c->rank.as<const int>(), c->start.as<const int>(), N, c->sorted.as<int>());
+
+    const long nodes = D == 2 ? (long)g.n1d * g.n1d : g.n1d;
+    TRY(ensure(c->gin, sizeof(float4) * nodes));
+    TRY(ensure(c->gout, sizeof(float4) * nodes));
+    launch(c, "spread_f32", pick_spread_f32<D>(p), dim3(g.nbox), dim3(kWave), 0, c->Y.as<const float>(),
+           c->sorted.as<const int>(), c->start.as<const int>(), g, c->gin.as<float4>());
+
+    const int kind_q = df1 ? FITSNE_KERNEL_SQUARED_CAUCHY : FITSNE_KERNEL_SQUARED_GENERAL;
+    const int kind_1 = FITSNE_KERNEL_GENERAL;
+    TRY(ensure(c->zbuf, sizeof(double) * 4));
+    if constexpr (D == 2) {
+        const size_t HH = (size_t)g.H * g.H;
+        const size_t plane = (size_t)g.H * g.n1d;
+        TRY(ensure(c->Trow, sizeof(cx<float>) * plane));
+        TRY(ensure(c->KQ, sizeof(float) * HH));
+        TRY(ensure(c->K1, sizeof(float) * HH));
+        TRY(ensure(c->KS, sizeof(float) * HH));
+        TRY(ensure(c->KT, sizeof(float) * HH));
+        TRY(ensure(c->spec, sizeof(cx<float>) * plane * 4));
+        TRY(ensure(c->ospec, sizeof(cx<float>) * plane * 3));
+        TRY(ensure(c->zpart, sizeof(double) * 4 * g.H));
+        const int gr_rows = 8 * ((g.n1d + 7) / 8), gr_cols = 8 * ((g.H + 7) / 8);
+        if (!(c->kcache && c->kc_mn == mn && c->kc_mx == mx && c->kc_nb == g.nb)) {
+            launch(c, "kern_rows", k_kern_rows<float, kThreads, kMaxTot32>, dim3(gr_rows), dim3(kThreads),
+                   sizeof(cx<float>) * g.Mp, g, pl, tw, kind_q, kind_1, c->df, c->Trow.as<cx<float>>());
+            launch(c, "kern_cols", k_kern_cols<float, kThreads, kMaxTot32>, dim3(gr_cols), dim3(kThreads),
+                   sizeof(cx<float>) * g.Mp, g, pl, tw, c->Trow.as<const cx<float>>(), 1, c->KQ.as<float>(),
+                   c->K1.as<float>(), c->KS.as<float>(), c->KT.as<float>());
+            c->kcache = true;
+            c->kc_mn = mn;
+            c->kc_mx = mx;
+            c->kc_nb = g.nb;
+        }
+        launch(c, "rows_fwd", k_rows_fwd<float, kThreads, kMaxTot32>, dim3(gr_rows), dim3(kThreads),
+               sizeof(cx<float>) * 2 * g.Mp, g, pl, tw, c->gin.as<const float>(), 4, 4, c->spec.as<cx<float>>());
+        launch(c, "cols_f32", k_cols_f32<kThreads, kMaxTot32>, dim3(gr_cols), dim3(kThreads),
+               sizeof(cx<float>) * 4 * g.Mp, g, pl, tw, c->spec.as<const cx<float>>(), c->KQ.as<const float>(),
+               c->K1.as<const float>(), c->KS.as<const float>(), c->KT.as<const float>(), c->ospec.as<cx<float>>(),
+               c->zpart.as<double>());
+        launch(c, "zfinal", k_zfinal, dim3(1), dim3(kThreads), 0, c->zpart.as<const double>(), g.H, (int)df1,
+               1.0 / ((double)g.Mp * g.Mp), (double)N, c->zbuf.as<double>());
+        launch(c, "rows_inv", k_rows_inv<float, kThreads, kMaxTot32>, dim3(gr_rows), dim3(kThreads),
+               sizeof(cx<float>) * 2 * g.Mp, g, pl, tw, c->ospec.as<const cx<float>>(), 3,
+               (float)(1.0 / ((double)g.Mp * g.Mp)), c->gout.as<float>(), 4);
+    } else {
+        TRY(ensure(c->zpart, sizeof(double) * 4));
+        launch(c, "conv1d_f32", k_conv1d<float, kConv1dThreads, kMaxTot32>, dim3(1), dim3(kConv1dThreads),
+               sizeof(cx<float>) * 2 * g.Mp, g, pl, tw, kind_q, kind_1, c->df, 1, c->gin.as<const float>(), 4, 3, 0,
+               c->gout.as<float>(), 4, c->zpart.as<double>(), (const float *)nullptr);
+        launch(c, "zfinal", k_zfinal, dim3(1), dim3(kThreads), 0, c->zpart.as<const double>(), 1, (int)df1,
+               1.0 / (double)g.Mp, (double)N, c->zbuf.as<double>());
+    }
+
+    UpdPrm up{};
+    up.mode = mode;
+    if (sp) {
+        up.momentum = (float)sp->momentum;
+        up.lr = (float)sp->learning_rate;
+        up.max_step_norm = (float)sp->max_step_norm;
+        up.plain_descent = sp->plain_descent;
+    }
+    const int nblk = blocks_for(N);
+    TRY(ensure(c->dC, sizeof(float) * (size_t)N * D));
+    TRY(ensure(c->part, sizeof(double) * (size_t)nblk * D));
+    launch(c, "interp_f32", pick_interp_f32<D>(p), dim3(nblk), dim3(kThreads), 0, c->Y.as<float>(),
+           c->pos.as<const float>(), c->gout.as<const float4>(), g, c->zbuf.as<const double>(), up, c->dC.as<float>(),
+           c->uY.as<float>(), c->gains.as<float>(), c->part.as<double>(), N);
+    if (mode == 1) {
+        TRY(ensure(c->mean, sizeof(double) * 2));
+        const long n = (long)N * D;
+        const int cblk = std::min(blocks_for(n), 1024);
+        TRY(ensure(c->mmpart, sizeof(double) * 2 * cblk));
+        TRY(ensure(c->mm, sizeof(double) * 4));
+        launch(c, "mean_final", k_mean_final, dim3(1), dim3(kThreads), 0, c->part.as<const double>(), nblk, D,
+               (double)N, c->mean.as<double>());
+        launch(c, "center", k_center<float>, dim3(cblk), dim3(kThreads), 0, c->Y.as<float>(), n, D, skip_coord(c),
+               c->mean.as<const double>(), c->mmpart.as<double>());
+        launch(c, "minmax_final", k_minmax_final<float>, dim3(1), dim3(kWave), 0, c->mmpart.as<const double>(), cblk,
+               c->Y.as<const float>(), first_coord(c), c->mm.as<double>());
+        HIPTRY(hipMemcpyAsync(c->h_mm, c->mm.p, sizeof(double) * 3, hipMemcpyDeviceToHost, c->stream));
+        HIPTRY(hipEventRecord(c->mm_event, c->stream));
+        c->mm_pending = true;
+        c->mm_valid = false;
+    }
+    c->have_grad = true;
+    HIPTRY(hipGetLastError());
+    return FITSNE_OK;
+}
+
+// ============================================================================ FP64 plain pipeline
+
+// One convolution pass of the plain path: charges `set` (nt terms) spread, convolved with kernel
+// spectrum KC (2D) or kernel kind (1D), interpolated into pot[N][nt] (zeroed first).
+template <int D>
+static int plain_pass(fitsne_ctx *c, const Geo &g, const FftPlan &pl, const cx<double> *tw, int set, int nt,
+                      const double *KC, int kind1d, int use_k1, DBuf &pot) {
+    const int N = c->N;
+    const long nodes = D == 2 ? (long)g.n1d * g.n1d : g.n1d;
+    TRY(ensure(c->ch, sizeof(double) * (size_t)N * nt));
+    TRY(ensure(c->w, sizeof(double) * nodes * nt));
+    TRY(ensure(c->V, sizeof(double) * nodes * nt));
+    TRY(ensure(pot, sizeof(double) * (size_t)N * nt));
+    launch(c, "charges", k_charges<D>, dim3(blocks_for(N)), dim3(kThreads), 0, c->Y.as<const double>(), N, set,
+           c->ch.as<double>());
+    launch(c, "fill", k_fill<double>, dim3(std::min(blocks_for(nodes * nt), 2048)), dim3(kThreads), 0,
+           c->w.as<double>(), nodes * nt, 0.0);
+    launch(c, "spread_f64", pick_spread_f64<D>(c->p), dim3(blocks_for(N)), dim3(kThreads), 0, c->box.as<const int>(),
+           c->wx.as<const double>(), c->wy.as<const double>(), c->ch.as<const double>(), nt, N, g, c->w.as<double>());
+    if constexpr (D == 2) {
+        const size_t plane = (size_t)g.H * g.n1d;
+        TRY(ensure(c->spec, sizeof(cx<double>) * plane * nt));
+        TRY(ensure(c->ospec, sizeof(cx<double>) * plane * nt));
+        const int gr_rows = 8 * ((g.n1d + 7) / 8), gr_cols = 8 * ((g.H + 7) / 8);
+        const int lines = (nt + 1) / 2;
+        launch(c, "rows_fwd64", k_rows_fwd<double, kThreads, kMaxTot64>, dim3(gr_rows), dim3(kThreads),
+               sizeof(cx<double>) * lines * g.Mp, g, pl, tw, c->w.as<const double>(), nt, nt, c->spec.as<cx<double>>());
+        const int cl = std::min(nt, kMaxTot64 / g.Mp);
+        launch(c, "cols_plain64", k_cols_plain<double, kThreads, kMaxTot64>, dim3(gr_cols), dim3(kThreads),
+               sizeof(cx<double>) * cl * g.Mp, g, pl, tw, c->spec.as<const cx<double>>(), nt, KC,
+               c->ospec.as<cx<double>>());
+        launch(c, "rows_inv64", k_rows_inv<double, kThreads, kMaxTot64>, dim3(gr_rows), dim3(kThreads),
+               sizeof(cx<double>) * lines * g.Mp, g, pl, tw, c->ospec.as<const cx<double>>(), nt,
+               1.0 / ((double)g.Mp * g.Mp), c->V.as<double>(), nt);
+    } else {
+        launch(c, "conv1d_f64", k_conv1d<double, kConv1dThreads, kMaxTot64>, dim3(1), dim3(kConv1dThreads),
+               sizeof(cx<double>) * g.Mp, g, pl, tw, kind1d, kind1d, c->df, 0, c->w.as<const double>(), nt, nt, use_k1,
+               c->V.as<double>(), nt, (double *)nullptr, (const double *)nullptr);
+    }
+    launch(c, "fill", k_fill<double>, dim3(std::min(blocks_for((long)N * nt), 2048)), dim3(kThreads), 0,
+           pot.as<double>(), (long)N * nt, 0.0);
+    launch(c, "interp_f64", pick_interp_f64<D>(c->p), dim3(blocks_for(N)), dim3(kThreads), 0, c->box.as<const int>(),
+           c->wx.as<const double>(), c->wy.as<const double>(), c->V.as<const double>(), nt, N, g, pot.as<double>());
+    return FITSNE_OK;
+}
+
+template <int D>
+static int pipeline_f64(fitsne_ctx *c, int mode, const fitsne_step_params *sp) {
+    const int N = c->N, p = c->p;
+    const bool df1 = c->df == 1.0;
+    TRY(ensure(c->pos, sizeof(double) * (size_t)N * D));
+    launch_attractive<double, D>(c, 1.0, c->pos.as<double>());
+    double mn, mx;
+    TRY(current_minmax<double>(c, &mn, &mx));
+    Geo g;
+    TRY(geometry(c, mn, mx, &g));
+    const FftPlan pl = make_plan(g.Mp);
+    const cx<double> *tw;
+    TRY(twiddles<double>(c, g.Mp, &tw));
+
+    const double *xs = c->Y.as<const double>(), *ys = nullptr;
+    if constexpr (D == 2) {
+        TRY(ensure(c->xs, sizeof(double) * (size_t)N));
+        TRY(ensure(c->ys, sizeof(double) * (size_t)N));
+        launch(c, "split", k_split_f64, dim3(blocks_for(N)), dim3(kThreads), 0, c->Y.as<const double>(), N,
+               c->xs.as<double>(), c->ys.as<double>());
+        xs = c->xs.as<const double>();
+        ys = c->ys.as<const double>();
+    }
+    TRY(ensure(c->box, sizeof(int) * (size_t)N));
+    TRY(ensure(c->wx, sizeof(double) * (size_t)N * p));
+    TRY(ensure(c->wy, sizeof(double) * (size_t)N * p));
+    launch(c, "weights_f64", pick_weights<D>(p), dim3(blocks_for(N)), dim3(kThreads), 0, xs, ys, N, g,
+           (const double *)nullptr, (const double *)nullptr, c->box.as<int>(), c->wx.as<double>(), c->wy.as<double>());
+
+    const int kind_q = df1 ? FITSNE_KERNEL_SQUARED_CAUCHY : FITSNE_KERNEL_SQUARED_GENERAL;
+    const double *KQ = nullptr, *K1 = nullptr;
+    if constexpr (D == 2) {
+        const size_t HH = (size_t)g.H * g.H;
+        TRY(ensure(c->Trow, sizeof(cx<double>) * (size_t)g.H * g.n1d));
+        TRY(ensure(c->KQ, sizeof(double) * HH));
+        TRY(ensure(c->K1, sizeof(double) * HH));
+        const int gr_rows = 8 * ((g.n1d + 7) / 8), gr_cols = 8 * ((g.H + 7) / 8);
+        launch(c, "kern_rows64", k_kern_rows<double, kThreads, kMaxTot64>, dim3(gr_rows), dim3(kThreads),
+               sizeof(cx<double>) * g.Mp, g, pl, tw, kind_q, (int)FITSNE_KERNEL_GENERAL, c->df,
+               c->Trow.as<cx<double>>());
+        launch(c, "kern_cols64", k_kern_cols<double, kThreads, kMaxTot64>, dim3(gr_cols), dim3(kThreads),
+               sizeof(cx<double>) * g.Mp, g, pl, tw, c->Trow.as<const cx<double>>(), 0, c->KQ.as<double>(),
+               c->K1.as<double>(), (double *)nullptr, (double *)nullptr);
+        KQ = c->KQ.as<const double>();
+        K1 = c->K1.as<const double>();
+    }
+    int zform, cform;
+    DBuf *potF = &c->potA;
+    if (df1) {
+        TRY((plain_pass<D>(c, g, pl, tw, D == 2 ? 0 : 3, D == 2 ? 4 : 3, KQ, kind_q, 0, c->potA)));
+        zform = D == 2 ? 0 : 2;
+        cform = D == 2 ? 0 : 2;
+        potF = &c->potA;
+    } else {
+        TRY((plain_pass<D>(c, g, pl, tw, D == 2 ? 1 : 4, D == 2 ? 3 : 2, KQ, kind_q, 0, c->potA)));
+        TRY((plain_pass<D>(c, g, pl, tw, 2, 1, K1, FITSNE_KERNEL_GENERAL, 1, c->potB)));
+        zform = 1;
+        cform = D == 2 ? 1 : 3;
+        potF = &c->potA;
+    }
+    const int nblk = blocks_for(N);
+    TRY(ensure(c->part, sizeof(double) * (size_t)nblk * 2));
+    TRY(ensure(c->zbuf, sizeof(double) * 4));
+    launch(c, "zterms64", k_zterms_f64, dim3(nblk), dim3(kThreads), 0, c->Y.as<const double>(),
+           (df1 ? c->potA : c->potB).as<const double>(), N, D, zform, c->part.as<double>());
+    launch(c, "zsum64", k_sum_final, dim3(1), dim3(kThreads), 0, c->part.as<const double>(), nblk, (double)N,
+           c->zbuf.as<double>());
+    TRY(ensure(c->dC, sizeof(double) * (size_t)N * D));
+    launch(c, "combine64", k_combine_f64, dim3(nblk), dim3(kThreads), 0, c->Y.as<const double>(),
+           c->pos.as<const double>(), potF->as<const double>(), c->zbuf.as<const double>(), N, cform,
+           c->dC.as<double>());
+    if (mode == 1) {
+        launch(c, "update64", k_update_f64, dim3(nblk), dim3(kThreads), 0, c->Y.as<double>(), c->uY.as<double>(),
+               c->gains.as<double>(), c->dC.as<const double>(), N, D, sp->momentum, sp->learning_rate,
+               sp->max_step_norm, sp->plain_descent, c->part.as<double>());
+        TRY(ensure(c->mean, sizeof(double) * 2));
+        const long n = (long)N * D;
+        const int cblk = std::min(blocks_for(n), 1024);
+        TRY(ensure(c->mmpart, sizeof(double) * 2 * cblk));
+        TRY(ensure(c->mm, sizeof(double) * 4));
+        launch(c, "mean_final", k_mean_final, dim3(1), dim3(kThreads), 0, c->part.as<const double>(), nblk, D,
+               (double)N, c->mean.as<double>());
+        launch(c, "center64", k_center<double>, dim3(cblk), dim3(kThreads), 0, c->Y.as<double>(), n, D,
+               skip_coord(c), c->mean.as<const double>(), c->mmpart.as<double>());
+        launch(c, "minmax_final", k_minmax_final<double>, dim3(1), dim3(kWave), 0, c->mmpart.as<const double>(), cblk,
+               c->Y.as<const double>(), first_coord(c), c->mm.as<double>());
+        HIPTRY(hipMemcpyAsync(c->h_mm, c->mm.p, sizeof(double) * 3, hipMemcpyDeviceToHost, c->stream));
+        HIPTRY(hipEventRecord(c->mm_event, c->stream));
+        c->mm_pending = true;
+        c->mm_valid = false;
+    }
+    c->have_grad = true;
+    HIPTRY(hipGetLastError());
+    return FITSNE_OK;
+}
+
+static int run_pipeline(fitsne_ctx *c, int mode, const fitsne_step_params *sp) {
+    if (c->N <= 0) return fail(FITSNE_E_ARG, "P and Y must be set before computing a gradient");
+    HIPTRY(hipSetDevice(c->device));
+    if (c->precision == FITSNE_FP32) return c->D == 2 ? pipeline_f32<2>(c, mode, sp) : pipeline_f32<1>(c, mode, sp);
+    return c->D == 2 ? pipeline_f64<2>(c, mode, sp) : pipeline_f64<1>(c, mode, sp);
+}
+
+static int read_sumq(fitsne_ctx *c, double *sum_Q) {
+    double z[2];
+    HIPTRY(hipMemcpyAsync(c->h_mm + 2, c->zbuf.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    TRY(sync_ctx(c));
+    z[0] = c->h_mm[2];
+    c->last_sumQ = z[0];
+    if (sum_Q) *sum_Q = z[0];
+    return FITSNE_OK;
+}
+
+// ============================================================================ C-ABI: misc
+
+extern "C" const char *fitsne_last_error(void) { return g_err.c_str(); }
+
+extern "C" const char *fitsne_version(void) { return "fitsne-mi355x 0.1.0 (reference 1.2.1)"; }
+
+extern "C" int fitsne_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+static int check_device(int device) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(FITSNE_E_NODEVICE, "no HIP device available");
+    if (device < 0 || device >= n) return fail(FITSNE_E_ARG, "device index out of range");
+    return FITSNE_OK;
+}
+
+// ============================================================================ C-ABI: resident tier
+
+extern "C" int fitsne_ctx_create(fitsne_ctx **out, int device, int precision, int no_dims, int n_interpolation_points,
+                                 double intervals_per_integer, int min_num_intervals, double df) {
+    if (!out) return fail(FITSNE_E_ARG, "ctx out-pointer is null");
+    *out = nullptr;
+    if (no_dims != 1 && no_dims != 2) return fail(FITSNE_E_ARG, "no_dims must be 1 or 2 for the FFT gradient");
+    if (precision != FITSNE_FP32 && precision != FITSNE_FP64_PLAIN) return fail(FITSNE_E_ARG, "unknown precision");
+    if (n_interpolation_points < 2 || n_interpolation_points > kMaxP)
+        return fail(FITSNE_E_UNSUPPORTED, "n_interpolation_points must be in [2, 5] in this build");
+    if (!(intervals_per_integer > 0) || min_num_intervals < 1 || !(df > 0))
+        return fail(FITSNE_E_ARG, "intervals_per_integer, min_num_intervals and df must be positive");
+    TRY(check_device(device));
+    HIPTRY(hipSetDevice(device));
+    fitsne_ctx *c = new fitsne_ctx();
+    c->device = device;
+    c->precision = precision;
+    c->D = no_dims;
+    c->p = n_interpolation_points;
+    c->ipi = intervals_per_integer;
+    c->mni = min_num_intervals;
+    c->df = df;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipHostMalloc((void **)&c->h_mm, sizeof(double) * 4, hipHostMallocDefault) != hipSuccess ||
+        hipEventCreateWithFlags(&c->mm_event, hipEventDisableTiming) != hipSuccess) {
+        delete c;
+        return fail(FITSNE_E_HIP, "stream/event/pinned allocation failed");
+    }
+    *out = c;
+    return FITSNE_OK;
+}
+
+extern "C" int fitsne_ctx_destroy(fitsne_ctx *c) {
+    if (!c) return FITSNE_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    DBuf *all[] = {&c->rowP, &c->colP, &c->val, &c->Y, &c->uY, &c->gains, &c->pos, &c->dC, &c->boxof, &c->rank,
+                   &c->count, &c->start, &c->sorted, &c->gin, &c->gout, &c->spec, &c->ospec, &c->Trow, &c->KQ,
+                   &c->K1, &c->KS, &c->KT, &c->zpart, &c->zbuf, &c->part, &c->mean, &c->mmpart, &c->mm, &c->xs,
+                   &c->ys, &c->box, &c->wx, &c->wy, &c->ch, &c->w, &c->V, &c->potA, &c->potB,
+                   &c->inv_d, &c->colP2, &c->val2, &c->tmp};
+    for (DBuf *b : all) release(*b);
+    for (auto &kv : c->tw32) release(kv.second);
+    for (auto &kv : c->tw64) release(kv.second);
+    for (auto e : c->ev_pool) (void)hipEventDestroy(e);
+    for (auto &p : c->pend) {
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    if (c->mm_event) (void)hipEventDestroy(c->mm_event);
+    if (c->h_mm) (void)hipHostFree(c->h_mm);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return FITSNE_OK;
+}
+
+template <typename TI, typename TC>
+static int set_P_impl(fitsne_ctx *c, int N, const TI *row_P, const TC *col_P, const double *val_P) {
+    if (!c || !row_P || !col_P || !val_P || N <= 0) return fail(FITSNE_E_ARG, "invalid P arguments");
+    HIPTRY(hipSetDevice(c->device));
+    const long long nnz = (long long)row_P[N];
+    if (nnz < 0 || (long long)row_P[0] != 0) return fail(FITSNE_E_ARG, "row_P must start at 0");
+    for (int i = 0; i < N; i++)
+        if ((long long)row_P[i + 1] < (long long)row_P[i]) return fail(FITSNE_E_ARG, "row_P must be non-decreasing");
+    std::vector<int64_t> r(N + 1);
+    for (int i = 0; i <= N; i++) r[i] = (int64_t)row_P[i];
+    std::vector<uint32_t> cc((size_t)nnz);
+    for (long long k = 0; k < nnz; k++) {
+        const long long j = (long long)col_P[k];
+        if (j < 0 || j >= N) return fail(FITSNE_E_ARG, "col_P entry out of range");
+        cc[k] = (uint32_t)j;
+    }
+    TRY(sync_ctx(c));
+    TRY(ensure(c->rowP, sizeof(int64_t) * (N + 1)));
+    TRY(ensure(c->colP, sizeof(uint32_t) * (size_t)std::max<long long>(nnz, 1)));
+    HIPTRY(hipMemcpy(c->rowP.p, r.data(), sizeof(int64_t) * (N + 1), hipMemcpyHostToDevice));
+    if (nnz) HIPTRY(hipMemcpy(c->colP.p, cc.data(), sizeof(uint32_t) * nnz, hipMemcpyHostToDevice));
+    if (c->precision == FITSNE_FP32) {
+        std::vector<float> v((size_t)nnz);
+        for (long long k = 0; k < nnz; k++) v[k] = (float)val_P[k];
+        TRY(ensure(c->val, sizeof(float) * (size_t)std::max<long long>(nnz, 1)));
+        if (nnz) HIPTRY(hipMemcpy(c->val.p, v.data(), sizeof(float) * nnz, hipMemcpyHostToDevice));
+    } else {
+        TRY(ensure(c->val, sizeof(double) * (size_t)std::max<long long>(nnz, 1)));
+        if (nnz) HIPTRY(hipMemcpy(c->val.p, val_P, sizeof(double) * nnz, hipMemcpyHostToDevice));
+    }
+    if (N != c->N) c->mm_valid = c->mm_pending = false;
+    c->N = N;
+    c->nnz = nnz;
+    c->exag = 1.0;
+    c->rowP_host = std::move(r);
+    c->perm.resize(N);
+    c->inv.resize(N);
+    for (int i = 0; i < N; i++) c->perm[i] = c->inv[i] = i;
+    c->permuted = false;
+    return FITSNE_OK;
+}
+
+extern "C" int fitsne_set_P(fitsne_ctx *c, int N, const uint32_t *row_P, const uint32_t *col_P, const double *val_P) {
+    return set_P_impl(c, N, row_P, col_P, val_P);
+}
+
+extern "C" int fitsne_set_P_i64(fitsne_ctx *c, int N, const int64_t *row_P, const int32_t *col_P,
+                                const double *val_P) {
+    return set_P_impl(c, N, row_P, col_P, val_P);
+}
+
+// The FP64 plain context rescales val_P in place exactly as the reference (`*= coeff`, `/= coeff`,
+// tsne.cpp:410-411, :537, :542); the FP32 context keeps the scalar.
+__global__ void k_scale_val(double *v, long n, double mul, double div) {
+    for (long e = (long)blockIdx.x * kThreads + threadIdx.x; e < n; e += (long)gridDim.x * kThreads) {
+        double x = v[e];
+        if (div != 1.0) x /= div;
+        if (mul != 1.0) x *= mul;
+        v[e] = x;
+    }
+}
+
+extern "C" int fitsne_set_exaggeration(fitsne_ctx *c, double factor) {
+    if (!c || !(factor > 0)) return fail(FITSNE_E_ARG, "exaggeration must be positive");
+    if (factor == c->exag) return FITSNE_OK;
+    HIPTRY(hipSetDevice(c->device));
+    if (c->precision == FITSNE_FP64_PLAIN && c->nnz > 0)
+        launch(c, "scale_val", k_scale_val, dim3(std::min(blocks_for(c->nnz), 4096)), dim3(kThreads), 0,
+               c->val.as<double>(), (long)c->nnz, factor, c->exag);
+    c->exag = factor;
+    return FITSNE_OK;
+}
+
+// Spatial order of the points for the FP32 form: a stable counting sort of the internal points by the
+// Morton index of their cell on a 64 x 64 (2D) or 4096 (1D) lattice over the bounding box, so that the
+// attractive term's neighbour gathers and the spread touch nearby memory.  Deterministic (host sort).
+static inline uint32_t morton2(uint32_t x, uint32_t y) {
+    uint32_t k = 0;
+    for (int b = 0; b < 16; b++) k |= ((x >> b) & 1u) << (2 * b) | ((y >> b) & 1u) << (2 * b + 1);
+    return k;
+}
+
+static int reorder_points(fitsne_ctx *c) {
+    const int N = c->N, D = c->D;
+    if (c->precision != FITSNE_FP32 || N < 2) return FITSNE_OK;
+    TRY(sync_ctx(c));
+    std::vector<float> y((size_t)N * D);
+    HIPTRY(hipMemcpy(y.data(), c->Y.p, sizeof(float) * y.size(), hipMemcpyDeviceToHost));
+    float lo[2] = {INFINITY, INFINITY}, hi[2] = {-INFINITY, -INFINITY};
+    for (int i = 0; i < N; i++)
+        for (int d = 0; d < D; d++) {
+            lo[d] = std::min(lo[d], y[(size_t)i * D + d]);
+            hi[d] = std::max(hi[d], y[(size_t)i * D + d]);
+        }
+    const int G = D == 2 ? 64 : 4096;
+    const int nkeys = D == 2 ? G * G : G;
+    std::vector<int> key(N), cnt(nkeys + 1, 0);
+    for (int i = 0; i < N; i++) {
+        uint32_t q[2] = {0, 0};
+        for (int d = 0; d < D; d++) {
+            const float span = hi[d] - lo[d];
+            const float t = span > 0 ? (y[(size_t)i * D + d] - lo[d]) / span : 0.f;
+            int v = (int)(t * G);
+            q[d] = (uint32_t)std::min(std::max(v, 0), G - 1);
+        }
+        key[i] = D == 2 ? (int)morton2(q[0], q[1]) : (int)q[0];
+        cnt[key[i] + 1]++;
+    }
+    for (int k = 0; k < nkeys; k++) cnt[k + 1] += cnt[k];
+    std::vector<int> rel(N), irel(N);
+    for (int i = 0; i < N; i++) rel[cnt[key[i]]++] = i;  // stable
+    for (int r = 0; r < N; r++) irel[rel[r]] = r;
+    std::vector<int64_t> rown(N + 1);
+    rown[0] = 0;
+    for (int r = 0; r < N; r++) rown[r + 1] = rown[r] + (c->rowP_host[rel[r] + 1] - c->rowP_host[rel[r]]);
+    // device: permute the CSR (rows and column numbering) and the per-point state
+    TRY(ensure(c->tmp, sizeof(int) * 2 * (size_t)N));
+    TRY(ensure(c->colP2, sizeof(uint32_t) * (size_t)std::max<long long>(c->nnz, 1)));
+    TRY(ensure(c->val2, sizeof(float) * (size_t)std::max<long long>(c->nnz, 1)));
+    DBuf rowN;
+    TRY(ensure(rowN, sizeof(int64_t) * (N + 1)));
+    int *d_rel = c->tmp.as<int>(), *d_irel = c->tmp.as<int>() + N;
+    HIPTRY(hipMemcpy(d_rel, rel.data(), sizeof(int) * N, hipMemcpyHostToDevice));
+    HIPTRY(hipMemcpy(d_irel, irel.data(), sizeof(int) * N, hipMemcpyHostToDevice));
+    HIPTRY(hipMemcpy(rowN.p, rown.data(), sizeof(int64_t) * (N + 1), hipMemcpyHostToDevice));
+    launch(c, "permute_csr", k_permute_csr<float>, dim3(blocks_for((long)N * kWave)), dim3(kThreads), 0,
+           c->rowP.as<const int64_t>(), c->colP.as<const uint32_t>(), c->val.as<const float>(),
+           rowN.as<const int64_t>(), (const int *)d_rel, (const int *)d_irel, N, c->colP2.as<uint32_t>(),
+           c->val2.as<float>());
+    DBuf *state[3] = {&c->Y, &c->uY, &c->gains};
+    DBuf scratch;
+    TRY(ensure(scratch, sizeof(float) * (size_t)N * D));
+    for (DBuf *b : state) {
+        launch(c, "gather_rows", k_gather_rows<float>, dim3(blocks_for((long)N * D)), dim3(kThreads), 0,
+               b->as<const float>(), (const int *)d_rel, N, D, scratch.as<float>());
+        std::swap(*b, scratch);
+    }
+    TRY(sync_ctx(c));
+    release(scratch);
+    std::swap(c->colP, c->colP2);
+    std::swap(c->val, c->val2);
+    std::swap(c->rowP, rowN);
+    release(rowN);
+    release(c->colP2);
+    release(c->val2);
+    // compose with the previous order
+    std::vector<int> perm(N);
+    for (int r = 0; r < N; r++) perm[r] = c->perm[rel[r]];
+    c->perm.swap(perm);
+    for (int r = 0; r < N; r++) c->inv[c->perm[r]] = r;
+    c->rowP_host.swap(rown);
+    TRY(ensure(c->inv_d, sizeof(int) * (size_t)N));
+    HIPTRY(hipMemcpy(c->inv_d.p, c->inv.data(), sizeof(int) * N, hipMemcpyHostToDevice));
+    c->permuted = true;
+    c->mm_valid = c->mm_pending = false;
+    c->kcache = false;
+    return FITSNE_OK;
+}
+
+extern "C" int fitsne_reorder(fitsne_ctx *c) {
+    if (!c) return fail(FITSNE_E_ARG, "null context");
+    if (c->N <= 0 || !c->Y.p) return fail(FITSNE_E_ARG, "set P and Y before reordering");
+    HIPTRY(hipSetDevice(c->device));
+    return reorder_points(c);
+}
+
+extern "C" int fitsne_set_reorder(fitsne_ctx *c, int enable) {
+    if (!c) return fail(FITSNE_E_ARG, "null context");
+    c->reorder_on_set_Y = enable != 0;
+    return FITSNE_OK;
+}
+
+// threshold below which set_Y keeps the caller's order (everything fits in cache anyway)
+constexpr int kReorderMinN = 65536;
+
+extern "C" int fitsne_set_Y(fitsne_ctx *c, const double *Y) {
+    if (!c || !Y) return fail(FITSNE_E_ARG, "null argument");
+    if (c->N <= 0) return fail(FITSNE_E_ARG, "set P before Y");
+    HIPTRY(hipSetDevice(c->device));
+    TRY(sync_ctx(c));
+    const size_t n = (size_t)c->N * c->D;
+    const int D = c->D;
+    if (c->precision == FITSNE_FP32) {
+        std::vector<float> y(n), z(n, 0.f), one(n, 1.f);
+        for (size_t r = 0; r < (size_t)c->N; r++)
+            for (int d = 0; d < D; d++) y[r * D + d] = (float)Y[(size_t)c->perm[r] * D + d];
+        TRY(ensure(c->Y, sizeof(float) * n));
+        TRY(ensure(c->uY, sizeof(float) * n));
+        TRY(ensure(c->gains, sizeof(float) * n));
+        HIPTRY(hipMemcpy(c->Y.p, y.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+        HIPTRY(hipMemcpy(c->uY.p, z.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+        HIPTRY(hipMemcpy(c->gains.p, one.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+    } else {
+        std::vector<double> z(n, 0.0), one(n, 1.0);
+        TRY(ensure(c->Y, sizeof(double) * n));
+        TRY(ensure(c->uY, sizeof(double) * n));
+        TRY(ensure(c->gains, sizeof(double) * n));
+        HIPTRY(hipMemcpy(c->Y.p, Y, sizeof(double) * n, hipMemcpyHostToDevice));
+        HIPTRY(hipMemcpy(c->uY.p, z.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+        HIPTRY(hipMemcpy(c->gains.p, one.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    }
+    c->mm_valid = c->mm_pending = false;
+    c->have_grad = false;
+    if (c->precision == FITSNE_FP32 && c->reorder_on_set_Y && c->N >= kReorderMinN) TRY(reorder_points(c));
+    return FITSNE_OK;
+}
+
+extern "C" int fitsne_get_Y(fitsne_ctx *c, double *Y) {
+    if (!c || !Y) return fail(FITSNE_E_ARG, "null argument");
+    HIPTRY(hipSetDevice(c->device));
+    TRY(sync_ctx(c));
+    const size_t n = (size_t)c->N * c->D;
+    if (c->precision == FITSNE_FP32) {
+        std::vector<float> y(n);
+        HIPTRY(hipMemcpy(y.data(), c->Y.p, sizeof(float) * n, hipMemcpyDeviceToHost));
+        const int D = c->D;
+        for (size_t r = 0; r < (size_t)c->N; r++)
+            for (int d = 0; d < D; d++) Y[(size_t)c->perm[r] * D + d] = y[r * D + d];
+    } else {
+        HIPTRY(hipMemcpy(Y, c->Y.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    }
+    return FITSNE_OK;
+}
+
+extern "C" int fitsne_gradient(fitsne_ctx *c, double *dC, double *sum_Q) {
+    if (!c) return fail(FITSNE_E_ARG, "null context");
+    TRY(run_pipeline(c, 0, nullptr));
+    TRY(read_sumq(c, sum_Q));
+    if (dC) {
+        const size_t n = (size_t)c->N * c->D;
+        if (c->precision == FITSNE_FP32) {
+            std::vector<float> d(n);
+            HIPTRY(hipMemcpy(d.data(), c->dC.p, sizeof(float) * n, hipMemcpyDeviceToHost));
+            const int D = c->D;
+            for (size_t r = 0; r < (size_t)c->N; r++)
+                for (int e = 0; e < D; e++) dC[(size_t)c->perm[r] * D + e] = d[r * D + e];
+        } else {
+            HIPTRY(hipMemcpy(dC, c->dC.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+        }
+    }
+    return FITSNE_OK;
+}
+
+extern "C" int fitsne_step(fitsne_ctx *c, const fitsne_step_params *prm, double *sum_Q) {
+    if (!c || !prm) return fail(FITSNE_E_ARG, "null argument");
+    TRY(run_pipeline(c, 1, prm));
+    if (sum_Q) TRY(read_sumq(c, sum_Q));
+    return FITSNE_OK;
+}
+
+extern "C" int fitsne_kl(fitsne_ctx *c, double *C) {
+    if (!c || !C) return fail(FITSNE_E_ARG, "null argument");
+    if (!c->have_grad) return fail(FITSNE_E_ARG, "KL needs the Z of a previous gradient");
+    HIPTRY(hipSetDevice(c->device));
+    const int sw = sw_for(c->nnz, c->N);
+    (void)sw;
+    const long threads = (long)c->N * 32;
+    const int nblk = blocks_for(threads);
+    TRY(ensure(c->part, sizeof(double) * (size_t)std::max(nblk, 2)));
+    TRY(ensure(c->mean, sizeof(double) * 2));
+    const double pscale = c->precision == FITSNE_FP32 ? c->exag : 1.0;
+    auto rp = c->rowP.as<const int64_t>();
+    auto cp = c->colP.as<const uint32_t>();
+    if (c->precision == FITSNE_FP32) {
+        if (c->D == 2)
+            launch(c, "kl", k_kl<float, int64_t, uint32_t, 2, 32>, dim3(nblk), dim3(kThreads), 0, rp, cp,
+                   c->val.as<const float>(), pscale, c->Y.as<const float>(), c->N, c->df, c->zbuf.as<const double>(),
+                   c->part.as<double>());
+        else
+            launch(c, "kl", k_kl<float, int64_t, uint32_t, 1, 32>, dim3(nblk), dim3(kThreads), 0, rp, cp,
+                   c->val.as<const float>(), pscale, c->Y.as<const float>(), c->N, c->df, c->zbuf.as<const double>(),
+                   c->part.as<double>());
+    } else {
+        if (c->D == 2)
+            launch(c, "kl", k_kl<double, int64_t, uint32_t, 2, 32>, dim3(nblk), dim3(kThreads), 0, rp, cp,
+                   c->val.as<const double>(), pscale, c->Y.as<const double>(), c->N, c->df,
+                   c->zbuf.as<const double>(), c->part.as<double>());
+        else
+            launch(c, "kl", k_kl<double, int64_t, uint32_t, 1, 32>, dim3(nblk), dim3(kThreads), 0, rp, cp,
+                   c->val.as<const double>(), pscale, c->Y.as<const double>(), c->N, c->df,
+                   c->zbuf.as<const double>(), c->part.as<double>());
+    }
+    launch(c, "kl_sum", k_sum_final, dim3(1), dim3(kThreads), 0, c->part.as<const double>(), nblk, 0.0,
+           c->mean.as<double>());
+    HIPTRY(hipMemcpyAsync(c->h_mm + 3, c->mean.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    TRY(sync_ctx(c));
+    *C = c->h_mm[3];
+    return FITSNE_OK;
+}
+
+extern "C" int fitsne_sync(fitsne_ctx *c) {
+    if (!c) return fail(FITSNE_E_ARG, "null context");
+    HIPTRY(hipSetDevice(c->device));
+    return sync_ctx(c);
+}
+
+extern "C" int fitsne_set_timing(fitsne_ctx *c, int enable) {
+    if (!c) return fail(FITSNE_E_ARG, "null context");
+    TRY(sync_ctx(c));
+    c->timing = enable != 0;
+    c->stats.clear();
+    return FITSNE_OK;
+}
+
+extern "C" int fitsne_kernel_stats(fitsne_ctx *c, int cap, char *names, int *counts, double *total_ms) {
+    if (!c) return fail(FITSNE_E_ARG, "null context");
+    TRY(sync_ctx(c));
+    int i = 0;
+    for (auto &kv : c->stats) {
+        if (i >= cap) break;
+        if (names) {
+            std::strncpy(names + 48 * i, kv.first.c_str(), 47);
+            names[48 * i + 47] = 0;
+        }
+        if (counts) counts[i] = kv.second.count;
+        if (total_ms) total_ms[i] = kv.second.ms;
+        i++;
+    }
+    return i;
+}
+
+extern "C" int fitsne_last_geometry(fitsne_ctx *c, double *out) {
+    if (!c || !out) return fail(FITSNE_E_ARG, "null argument");
+    out[0] = c->last.nb;
+    out[1] = c->last.n1d;
+    out[2] = c->last.Mp;
+    out[3] = c->last_min;
+    out[4] = c->last_max;
+    out[5] = (double)c->nnz;
+    return FITSNE_OK;
+}
+
+extern "C" long long fitsne_debug_read(fitsne_ctx *c, const char *name, void *dst, long long bytes) {
+    if (!c || !name || !dst || bytes < 0) return fail(FITSNE_E_ARG, "null argument");
+    HIPTRY(hipSetDevice(c->device));
+    TRY(sync_ctx(c));
+    const std::string n(name);
+    DBuf *b = n == "grid_in" ? &c->gin : n == "grid_out" ? &c->gout : n == "kq" ? &c->KQ : n == "k1" ? &c->K1
+            : n == "ks" ? &c->KS : n == "kt" ? &c->KT : n == "zpart" ? &c->zpart : n == "zbuf" ? &c->zbuf
+            : n == "pos" ? &c->pos : n == "dC" ? &c->dC : nullptr;
+    if (!b) return fail(FITSNE_E_ARG, "unknown buffer name");
+    const long long nb = std::min<long long>(bytes, (long long)b->cap);
+    if (nb > 0 && b->p) HIPTRY(hipMemcpy(dst, b->p, (size_t)nb, hipMemcpyDeviceToHost));
+    return nb;
+}
+
+// ============================================================================ C-ABI: mirror tier
+
+static std::mutex g_mirror_mu;
+static fitsne_ctx *g_mirror = nullptr;
+
+static int mirror_ctx(fitsne_ctx **out, int D, int p, double ipi, int mni, double df) {
+    if (!g_mirror) TRY(fitsne_ctx_create(&g_mirror, 0, FITSNE_FP64_PLAIN, D, p, ipi, mni, df));
+    if (p < 2 || p > kMaxP) return fail(FITSNE_E_UNSUPPORTED, "n_interpolation_points must be in [2, 5]");
+    if (!(ipi > 0) || mni < 1 || !(df > 0)) return fail(FITSNE_E_ARG, "invalid FFT settings");
+    g_mirror->D = D;
+    g_mirror->p = p;
+    g_mirror->ipi = ipi;
+    g_mirror->mni = mni;
+    g_mirror->df = df;
+    g_mirror->kcache = false;
+    *out = g_mirror;
+    return FITSNE_OK;
+}
+
+template <typename TI, typename TC>
+static int mirror_gradient(const TI *row_P, const TC *col_P, const double *val_P, const double *Y, int N, int D,
+                           double *dC, int p, double ipi, int mni, double df, double *sum_Q) {
+    if (!row_P || !col_P || !val_P || !Y || !dC || N <= 0) return fail(FITSNE_E_ARG, "invalid arguments");
+    if (D != 1 && D != 2) return fail(FITSNE_E_UNSUPPORTED, "the FFT gradient supports 1 or 2 dimensions");
+    std::lock_guard<std::mutex> lk(g_mirror_mu);
+    fitsne_ctx *c;
+    TRY(mirror_ctx(&c, D, p, ipi, mni, df));
+    TRY(set_P_impl(c, N, row_P, col_P, val_P));
+    TRY(fitsne_set_Y(c, Y));
+    double z;
+    TRY(fitsne_gradient(c, dC, &z));
+    if (sum_Q) *sum_Q = z;
+    return FITSNE_OK;
+}
+
+extern "C" int fitsne_compute_fft_gradient(const uint32_t *row_P, const uint32_t *col_P, const double *val_P,
+                                           const double *Y, int N, int D, double *dC, int n_interpolation_points,
+                                           double intervals_per_integer, int min_num_intervals, unsigned nthreads,
+                                           double df, double *sum_Q) {
+    (void)nthreads;
+    return mirror_gradient(row_P, col_P, val_P, Y, N, D, dC, n_interpolation_points, intervals_per_integer,
+                           min_num_intervals, df, sum_Q);
+}
+
+extern "C" int fitsne_compute_fft_gradient_i64(const int64_t *row_P, const int32_t *col_P, const double *val_P,
+                                               const double *Y, int N, int D, double *dC, int n_interpolation_points,
+                                               double intervals_per_integer, int min_num_intervals,
+                                               unsigned nthreads, double df, double *sum_Q) {
+    (void)nthreads;
+    return mirror_gradient(row_P, col_P, val_P, Y, N, D, dC, n_interpolation_points, intervals_per_integer,
+                           min_num_intervals, df, sum_Q);
+}
+
+// Geometry of a precompute/n_body_fft pair from the caller's box layout.
+static Geo mirror_geo(int D, int p, int nb, double mn, double bw0, const double *lower, const double *upper) {
+    Geo g = make_geo(D, p, mn, mn + bw0 * nb, nb);
+    g.bw0 = bw0;
+    g.h = (1 / (double)p) * bw0;
+    if (lower && upper) {
+        g.mn = lower[0];
+        g.bw = upper[0] - lower[0];
+    }
+    return g;
+}
+
+static int kernel_kind_ok(int kernel) {
+    return kernel == FITSNE_KERNEL_SQUARED_CAUCHY || kernel == FITSNE_KERNEL_GENERAL ||
+           kernel == FITSNE_KERNEL_SQUARED_GENERAL;
+}
+
+extern "C" int fitsne_precompute_2d(double x_max, double x_min, double y_max, double y_min, int n_boxes,
+                                    int n_interpolation_points, int kernel, double *box_lower_bounds,
+                                    double *box_upper_bounds, double *y_tilde_spacings, double *y_tilde,
+                                    double *x_tilde, double *fft_kernel_tilde, double df) {
+    (void)y_max;
+    if (n_boxes < 1 || !kernel_kind_ok(kernel) || !box_lower_bounds || !box_upper_bounds || !y_tilde_spacings ||
+        !y_tilde || !x_tilde || !fft_kernel_tilde)
+        return fail(FITSNE_E_ARG, "invalid precompute_2d arguments");
+    const int p = n_interpolation_points;
+    if (p < 2 || p > kMaxP) return fail(FITSNE_E_UNSUPPORTED, "n_interpolation_points must be in [2, 5]");
+    // box bounds, spacings, nodes: the reference's loops (nbodyfft.cpp:15-46)
+    const int ntb = n_boxes * n_boxes;
+    const double box_width = (x_max - x_min) / (double)n_boxes;
+    for (int i = 0; i < n_boxes; i++)
+        for (int j = 0; j < n_boxes; j++) {
+            box_lower_bounds[i * n_boxes + j] = j * box_width + x_min;
+            box_upper_bounds[i * n_boxes + j] = (j + 1) * box_width + x_min;
+            box_lower_bounds[ntb + i * n_boxes + j] = i * box_width + y_min;
+            box_upper_bounds[ntb + i * n_boxes + j] = (i + 1) * box_width + y_min;
+        }
+    double h = 1 / (double)p;
+    y_tilde_spacings[0] = h / 2;
+    for (int i = 1; i < p; i++) y_tilde_spacings[i] = y_tilde_spacings[i - 1] + h;
+    const int n1d = p * n_boxes;
+    h = h * box_width;
+    x_tilde[0] = x_min + h / 2;
+    y_tilde[0] = y_min + h / 2;
+    for (int i = 1; i < n1d; i++) {
+        x_tilde[i] = x_tilde[i - 1] + h;
+        y_tilde[i] = y_tilde[i - 1] + h;
+    }
+    // kernel spectrum on the GPU: C(m, k) for m, k in [0, H), FP64, written into fft_kernel_tilde
+    std::lock_guard<std::mutex> lk(g_mirror_mu);
+    fitsne_ctx *c;
+    TRY(mirror_ctx(&c, 2, p, 1.0, 1, df));
+    HIPTRY(hipSetDevice(c->device));
+    Geo g = mirror_geo(2, p, n_boxes, x_min, box_width, nullptr, nullptr);
+    if (g.Mp > kMaxTot64) return fail(FITSNE_E_UNSUPPORTED, "grid too large for this build");
+    const FftPlan pl = make_plan(g.Mp);
+    const cx<double> *tw;
+    TRY(twiddles<double>(c, g.Mp, &tw));
+    const size_t HH = (size_t)g.H * g.H;
+    TRY(ensure(c->Trow, sizeof(cx<double>) * (size_t)g.H * g.n1d));
+    TRY(ensure(c->KQ, sizeof(double) * HH));
+    TRY(ensure(c->K1, sizeof(double) * HH));
+    const int gr_rows = 8 * ((g.n1d + 7) / 8), gr_cols = 8 * ((g.H + 7) / 8);
+    launch(c, "kern_rows64", k_kern_rows<double, kThreads, kMaxTot64>, dim3(gr_rows), dim3(kThreads),
+           sizeof(cx<double>) * g.Mp, g, pl, tw, kernel, kernel, df, c->Trow.as<cx<double>>());
+    launch(c, "kern_cols64", k_kern_cols<double, kThreads, kMaxTot64>, dim3(gr_cols), dim3(kThreads),
+           sizeof(cx<double>) * g.Mp, g, pl, tw, c->Trow.as<const cx<double>>(), 0, c->KQ.as<double>(),
+           c->K1.as<double>(), (double *)nullptr, (double *)nullptr);
+    HIPTRY(hipMemcpyAsync(fft_kernel_tilde, c->KQ.p, sizeof(double) * HH, hipMemcpyDeviceToHost, c->stream));
+    return sync_ctx(c);
+}
+
+extern "C" int fitsne_n_body_fft_2d(int N, int n_terms, const double *xs, const double *ys, const double *chargesQij,
+                                    int n_boxes, int n_interpolation_points, const double *box_lower_bounds,
+                                    const double *box_upper_bounds, const double *y_tilde_spacings,
+                                    const double *fft_kernel_tilde, double *potentialQij, unsigned nthreads) {
+    (void)nthreads;
+    if (N <= 0 || n_terms < 1 || n_boxes < 1 || !xs || !ys || !chargesQij || !box_lower_bounds ||
+        !box_upper_bounds || !y_tilde_spacings || !fft_kernel_tilde || !potentialQij)
+        return fail(FITSNE_E_ARG, "invalid n_body_fft_2d arguments");
+    if (n_terms > 4) return fail(FITSNE_E_UNSUPPORTED, "n_terms must be in [1, 4] in this build");
+    const int p = n_interpolation_points;
+    if (p < 2 || p > kMaxP) return fail(FITSNE_E_UNSUPPORTED, "n_interpolation_points must be in [2, 5]");
+    std::lock_guard<std::mutex> lk(g_mirror_mu);
+    fitsne_ctx *c;
+    TRY(mirror_ctx(&c, 2, p, 1.0, 1, 1.0));
+    HIPTRY(hipSetDevice(c->device));
+    const int ntb = n_boxes * n_boxes;
+    Geo g = mirror_geo(2, p, n_boxes, box_lower_bounds[0], box_upper_bounds[0] - box_lower_bounds[0],
+                       box_lower_bounds, box_upper_bounds);
+    for (int i = 0; i < p; i++) g.sp[i] = y_tilde_spacings[i];
+    for (int i = 0; i < p; i++) {
+        g.den[i] = 1;
+        for (int j = 0; j < p; j++)
+            if (i != j) g.den[i] *= g.sp[i] - g.sp[j];
+    }
+    if (g.Mp > kMaxTot64 / 2) return fail(FITSNE_E_UNSUPPORTED, "grid too large for this build");
+    const FftPlan pl = make_plan(g.Mp);
+    const cx<double> *tw;
+    TRY(twiddles<double>(c, g.Mp, &tw));
+    // caller's per-axis lower bounds: x lower of column j at [j], y lower of row i at [ntb + i*nb]
+    std::vector<double> lo(2 * (size_t)n_boxes);
+    for (int j = 0; j < n_boxes; j++) lo[j] = box_lower_bounds[j];
+    for (int i = 0; i < n_boxes; i++) lo[n_boxes + i] = box_lower_bounds[ntb + i * n_boxes];
+    c->N = N;
+    const size_t HH = (size_t)g.H * g.H;
+    TRY(ensure(c->xs, sizeof(double) * N));
+    TRY(ensure(c->ys, sizeof(double) * N));
+    TRY(ensure(c->mean, sizeof(double) * 2 * n_boxes));
+    TRY(ensure(c->ch, sizeof(double) * (size_t)N * n_terms));
+    TRY(ensure(c->potA, sizeof(double) * (size_t)N * n_terms));
+    TRY(ensure(c->KQ, sizeof(double) * HH));
+    HIPTRY(hipMemcpy(c->xs.p, xs, sizeof(double) * N, hipMemcpyHostToDevice));
+    HIPTRY(hipMemcpy(c->ys.p, ys, sizeof(double) * N, hipMemcpyHostToDevice));
+    HIPTRY(hipMemcpy(c->mean.p, lo.data(), sizeof(double) * 2 * n_boxes, hipMemcpyHostToDevice));
+    HIPTRY(hipMemcpy(c->ch.p, chargesQij, sizeof(double) * (size_t)N * n_terms, hipMemcpyHostToDevice));
+    HIPTRY(hipMemcpy(c->potA.p, potentialQij, sizeof(double) * (size_t)N * n_terms, hipMemcpyHostToDevice));
+    HIPTRY(hipMemcpy(c->KQ.p, fft_kernel_tilde, sizeof(double) * HH, hipMemcpyHostToDevice));
+    TRY(ensure(c->box, sizeof(int) * N));
+    TRY(ensure(c->wx, sizeof(double) * (size_t)N * p));
+    TRY(ensure(c->wy, sizeof(double) * (size_t)N * p));
+    launch(c, "weights_f64", pick_weights<2>(p), dim3(blocks_for(N)), dim3(kThreads), 0, c->xs.as<const double>(),
+           c->ys.as<const double>(), N, g, c->mean.as<const double>(), c->mean.as<const double>() + n_boxes,
+           c->box.as<int>(), c->wx.as<double>(), c->wy.as<double>());
+    const long nodes = (long)g.n1d * g.n1d;
+    const int nt = n_terms;
+    TRY(ensure(c->w, sizeof(double) * nodes * nt));
+    TRY(ensure(c->V, sizeof(double) * nodes * nt));
+    launch(c, "fill", k_fill<double>, dim3(std::min(blocks_for(nodes * nt), 2048)), dim3(kThreads), 0,
+           c->w.as<double>(), nodes * nt, 0.0);
+    launch(c, "spread_f64", pick_spread_f64<2>(p), dim3(blocks_for(N)), dim3(kThreads), 0, c->box.as<const int>(),
+           c->wx.as<const double>(), c->wy.as<const double>(), c->ch.as<const double>(), nt, N, g, c->w.as<double>());
+    const size_t plane = (size_t)g.H * g.n1d;
+    TRY(ensure(c->spec, sizeof(cx<double>) * plane * nt));
+    TRY(ensure(c->ospec, sizeof(cx<double>) * plane * nt));
+    const int gr_rows = 8 * ((g.n1d + 7) / 8), gr_cols = 8 * ((g.H + 7) / 8);
+    const int lines = (nt + 1) / 2;
+    launch(c, "rows_fwd64", k_rows_fwd<double, kThreads, kMaxTot64>, dim3(gr_rows), dim3(kThreads),
+           sizeof(cx<double>) * lines * g.Mp, g, pl, tw, c->w.as<const double>(), nt, nt, c->spec.as<cx<double>>());
+    const int cl = std::min(nt, kMaxTot64 / g.Mp);
+    launch(c, "cols_plain64", k_cols_plain<double, kThreads, kMaxTot64>, dim3(gr_cols), dim3(kThreads),
+           sizeof(cx<double>) * cl * g.Mp, g, pl, tw, c->spec.as<const cx<double>>(), nt, c->KQ.as<const double>(),
+           c->ospec.as<cx<double>>());
+    launch(c, "rows_inv64", k_rows_inv<double, kThreads, kMaxTot64>, dim3(gr_rows), dim3(kThreads),
+           sizeof(cx<double>) * lines * g.Mp, g, pl, tw, c->ospec.as<const cx<double>>(), nt,
+           1.0 / ((double)g.Mp * g.Mp), c->V.as<double>(), nt);
+    launch(c, "interp_f64", pick_interp_f64<2>(p), dim3(blocks_for(N)), dim3(kThreads), 0, c->box.as<const int>(),
+           c->wx.as<const double>(), c->wy.as<const double>(), c->V.as<const double>(), nt, N, g, c->potA.as<double>());
+    HIPTRY(hipMemcpyAsync(potentialQij, c->potA.p, sizeof(double) * (size_t)N * n_terms, hipMemcpyDeviceToHost,
+                          c->stream));
+    c->N = 0;
+    return sync_ctx(c);
+}
+
+extern "C" int fitsne_precompute_1d(double y_min, double y_max, int n_boxes, int n_interpolation_points, int kernel,
+                                    double *box_lower_bounds, double *box_upper_bounds, double *y_tilde_spacing,
+                                    double *y_tilde, double *fft_kernel_vector, double df) {
+    if (n_boxes < 1 || !kernel_kind_ok(kernel) || !box_lower_bounds || !box_upper_bounds || !y_tilde_spacing ||
+        !y_tilde || !fft_kernel_vector)
+        return fail(FITSNE_E_ARG, "invalid precompute arguments");
+    const int p = n_interpolation_points;
+    if (p < 2 || p > kMaxP) return fail(FITSNE_E_UNSUPPORTED, "n_interpolation_points must be in [2, 5]");
+    const double box_width = (y_max - y_min) / (double)n_boxes;
+    for (int b = 0; b < n_boxes; b++) {
+        box_lower_bounds[b] = b * box_width + y_min;
+        box_upper_bounds[b] = (b + 1) * box_width + y_min;
+    }
+    double h = 1 / (double)p;
+    y_tilde_spacing[0] = h / 2;
+    for (int i = 1; i < p; i++) y_tilde_spacing[i] = y_tilde_spacing[i - 1] + h;
+    const int total = p * n_boxes;
+    h = h * box_width;
+    y_tilde[0] = y_min + h / 2;
+    for (int i = 1; i < total; i++) y_tilde[i] = y_tilde[i - 1] + h;
+    std::lock_guard<std::mutex> lk(g_mirror_mu);
+    fitsne_ctx *c;
+    TRY(mirror_ctx(&c, 1, p, 1.0, 1, df));
+    HIPTRY(hipSetDevice(c->device));
+    Geo g = mirror_geo(1, p, n_boxes, y_min, box_width, nullptr, nullptr);
+    if (g.Mp > kMaxTot64) return fail(FITSNE_E_UNSUPPORTED, "grid too large for this build");
+    const FftPlan pl = make_plan(g.Mp);
+    const cx<double> *tw;
+    TRY(twiddles<double>(c, g.Mp, &tw));
+    TRY(ensure(c->KQ, sizeof(double) * g.Mp));
+    launch(c, "kspec1d", k_kspec1d<double, kConv1dThreads, kMaxTot64>, dim3(1), dim3(kConv1dThreads),
+           sizeof(cx<double>) * g.Mp, g, pl, tw, kernel, df, c->KQ.as<double>());
+    HIPTRY(hipMemcpyAsync(fft_kernel_vector, c->KQ.p, sizeof(double) * g.Mp, hipMemcpyDeviceToHost, c->stream));
+    return sync_ctx(c);
+}
+
+extern "C" int fitsne_n_body_fft_1d(int N, int n_terms, const double *Y, const double *chargesQij, int n_boxes,
+                                    int n_interpolation_points, const double *box_lower_bounds,
+                                    const double *box_upper_bounds, const double *y_tilde_spacings,
+                                    const double *y_tilde, const double *fft_kernel_vector, double *potentialsQij) {
+    (void)y_tilde;
+    if (N <= 0 || n_terms < 1 || n_boxes < 1 || !Y || !chargesQij || !box_lower_bounds || !box_upper_bounds ||
+        !y_tilde_spacings || !fft_kernel_vector || !potentialsQij)
+        return fail(FITSNE_E_ARG, "invalid nbodyfft arguments");
+    if (n_terms > 4) return fail(FITSNE_E_UNSUPPORTED, "n_terms must be in [1, 4] in this build");
+    const int p = n_interpolation_points;
+    if (p < 2 || p > kMaxP) return fail(FITSNE_E_UNSUPPORTED, "n_interpolation_points must be in [2, 5]");
+    std::lock_guard<std::mutex> lk(g_mirror_mu);
+    fitsne_ctx *c;
+    TRY(mirror_ctx(&c, 1, p, 1.0, 1, 1.0));
+    HIPTRY(hipSetDevice(c->device));
+    Geo g = mirror_geo(1, p, n_boxes, box_lower_bounds[0], box_upper_bounds[0] - box_lower_bounds[0],
+                       box_lower_bounds, box_upper_bounds);
+    for (int i = 0; i < p; i++) g.sp[i] = y_tilde_spacings[i];
+    for (int i = 0; i < p; i++) {
+        g.den[i] = 1;
+        for (int j = 0; j < p; j++)
+            if (i != j) g.den[i] *= g.sp[i] - g.sp[j];
+    }
+    if (g.Mp > kMaxTot64) return fail(FITSNE_E_UNSUPPORTED, "grid too large for this build");
+    const FftPlan pl = make_plan(g.Mp);
+    const cx<double> *tw;
+    TRY(twiddles<double>(c, g.Mp, &tw));
+    const int nt = n_terms;
+    TRY(ensure(c->xs, sizeof(double) * N));
+    TRY(ensure(c->mean, sizeof(double) * n_boxes));
+    TRY(ensure(c->ch, sizeof(double) * (size_t)N * nt));
+    TRY(ensure(c->potA, sizeof(double) * (size_t)N * nt));
+    TRY(ensure(c->KQ, sizeof(double) * g.Mp));
+    HIPTRY(hipMemcpy(c->xs.p, Y, sizeof(double) * N, hipMemcpyHostToDevice));
+    HIPTRY(hipMemcpy(c->mean.p, box_lower_bounds, sizeof(double) * n_boxes, hipMemcpyHostToDevice));
+    HIPTRY(hipMemcpy(c->ch.p, chargesQij, sizeof(double) * (size_t)N * nt, hipMemcpyHostToDevice));
+    HIPTRY(hipMemcpy(c->potA.p, potentialsQij, sizeof(double) * (size_t)N * nt, hipMemcpyHostToDevice));
+    HIPTRY(hipMemcpy(c->KQ.p, fft_kernel_vector, sizeof(double) * g.Mp, hipMemcpyHostToDevice));
+    TRY(ensure(c->box, sizeof(int) * N));
+    TRY(ensure(c->wx, sizeof(double) * (size_t)N * p));
+    TRY(ensure(c->wy, sizeof(double) * (size_t)N * p));
+    launch(c, "weights_f64", pick_weights<1>(p), dim3(blocks_for(N)), dim3(kThreads), 0, c->xs.as<const double>(),
+           (const double *)nullptr, N, g, c->mean.as<const double>(), (const double *)nullptr, c->box.as<int>(),
+           c->wx.as<double>(), c->wy.as<double>());
+    TRY(ensure(c->w, sizeof(double) * (size_t)g.n1d * nt));
+    TRY(ensure(c->V, sizeof(double) * (size_t)g.n1d * nt));
+    launch(c, "fill", k_fill<double>, dim3(std::min(blocks_for((long)g.n1d * nt), 2048)), dim3(kThreads), 0,
+           c->w.as<double>(), (long)g.n1d * nt, 0.0);
+    launch(c, "spread_f64", pick_spread_f64<1>(p), dim3(blocks_for(N)), dim3(kThreads), 0, c->box.as<const int>(),
+           c->wx.as<const double>(), c->wy.as<const double>(), c->ch.as<const double>(), nt, N, g, c->w.as<double>());
+    launch(c, "conv1d_f64", k_conv1d<double, kConv1dThreads, kMaxTot64>, dim3(1), dim3(kConv1dThreads),
+           sizeof(cx<double>) * g.Mp, g, pl, tw, 0, 0, 1.0, 0, c->w.as<const double>(), nt, nt, 0, c->V.as<double>(),
+           nt, (double *)nullptr, c->KQ.as<const double>());
+    launch(c, "interp_f64", pick_interp_f64<1>(p), dim3(blocks_for(N)), dim3(kThreads), 0, c->box.as<const int>(),
+           c->wx.as<const double>(), c->wy.as<const double>(), c->V.as<const double>(), nt, N, g, c->potA.as<double>());
+    HIPTRY(hipMemcpyAsync(potentialsQij, c->potA.p, sizeof(double) * (size_t)N * nt, hipMemcpyDeviceToHost,
+                          c->stream));
+    return sync_ctx(c);
+}
+
+extern "C" int fitsne_interpolate(int n_interpolation_points, int N, const double *y_in_box,
+                                  const double *y_tilde_spacings, double *interpolated_values) {
+    const int p = n_interpolation_points;
+    if (N <= 0 || !y_in_box || !y_tilde_spacings || !interpolated_values) return fail(FITSNE_E_ARG, "invalid arguments");
+    if (p < 2 || p > kMaxP) return fail(FITSNE_E_UNSUPPORTED, "n_interpolation_points must be in [2, 5]");
+    std::lock_guard<std::mutex> lk(g_mirror_mu);
+    fitsne_ctx *c;
+    TRY(mirror_ctx(&c, 1, p, 1.0, 1, 1.0));
+    HIPTRY(hipSetDevice(c->device));
+    Geo g{};
+    for (int i = 0; i < p; i++) g.sp[i] = y_tilde_spacings[i];
+    for (int i = 0; i < p; i++) {
+        g.den[i] = 1;
+        for (int j = 0; j < p; j++)
+            if (i != j) g.den[i] *= g.sp[i] - g.sp[j];
+    }
+    TRY(ensure(c->xs, sizeof(double) * N));
+    TRY(ensure(c->wx, sizeof(double) * (size_t)N * p));
+    HIPTRY(hipMemcpy(c->xs.p, y_in_box, sizeof(double) * N, hipMemcpyHostToDevice));
+    auto k = p == 2 ? k_lagrange_f64<2> : p == 3 ? k_lagrange_f64<3> : p == 4 ? k_lagrange_f64<4> : k_lagrange_f64<5>;
+    launch(c, "lagrange", k, dim3(blocks_for(N)), dim3(kThreads), 0, c->xs.as<const double>(), N, g, c->wx.as<double>());
+    HIPTRY(hipMemcpyAsync(interpolated_values, c->wx.p, sizeof(double) * (size_t)N * p, hipMemcpyDeviceToHost,
+                          c->stream));
+    return sync_ctx(c);
+}
+
+extern "C" int fitsne_evaluate_error_fft(const uint32_t *row_P, const uint32_t *col_P, const double *val_P,
+                                         const double *Y, int N, int D, double sum_Q, double df, double *C) {
+    if (!row_P || !col_P || !val_P || !Y || !C || N <= 0 || (D != 1 && D != 2))
+        return fail(FITSNE_E_ARG, "invalid evaluate_error_fft arguments");
+    std::lock_guard<std::mutex> lk(g_mirror_mu);
+    fitsne_ctx *c;
+    TRY(mirror_ctx(&c, D, 3, 1.0, 1, df));
+    TRY(set_P_impl(c, N, row_P, col_P, val_P));
+    TRY(fitsne_set_Y(c, Y));
+    TRY(ensure(c->zbuf, sizeof(double) * 4));
+    double z[2] = {sum_Q, 1.0 / sum_Q};
+    HIPTRY(hipMemcpy(c->zbuf.p, z, sizeof(z), hipMemcpyHostToDevice));
+    c->have_grad = true;
+    return fitsne_kl(c, C);
+}
+
+extern "C" int fitsne_distances2similarities_batch(const double *D, double *P, int n_points, int K,
+                                                   double perplexity, double sigma, int ifSquared,
+                                                   const double *beta0, double *beta, int *iters) {
+    if (!D || !P || !beta || n_points <= 0 || K <= 0) return fail(FITSNE_E_ARG, "invalid arguments");
+    if (!(perplexity > 0) && !(sigma > 0)) return fail(FITSNE_E_ARG, "need perplexity > 0 or sigma > 0");
+    std::lock_guard<std::mutex> lk(g_mirror_mu);
+    fitsne_ctx *c;
+    TRY(mirror_ctx(&c, 1, 3, 1.0, 1, 1.0));
+    HIPTRY(hipSetDevice(c->device));
+    const size_t nk = (size_t)n_points * K;
+    TRY(ensure(c->w, sizeof(double) * nk));
+    TRY(ensure(c->V, sizeof(double) * nk));
+    TRY(ensure(c->xs, sizeof(double) * n_points));
+    TRY(ensure(c->ys, sizeof(double) * n_points));
+    TRY(ensure(c->box, sizeof(int) * n_points));
+    HIPTRY(hipMemcpy(c->w.p, D, sizeof(double) * nk, hipMemcpyHostToDevice));
+    if (beta0) HIPTRY(hipMemcpy(c->ys.p, beta0, sizeof(double) * n_points, hipMemcpyHostToDevice));
+    const long threads = (long)n_points * kWave;
+    launch(c, "d2s", k_d2s, dim3(blocks_for(threads)), dim3(kThreads), 0, c->w.as<const double>(), c->V.as<double>(),
+           n_points, K, perplexity, sigma, ifSquared, beta0 ? c->ys.as<const double>() : (const double *)nullptr,
+           c->xs.as<double>(), c->box.as<int>());
+    HIPTRY(hipMemcpyAsync(P, c->V.p, sizeof(double) * nk, hipMemcpyDeviceToHost, c->stream));
+    HIPTRY(hipMemcpyAsync(beta, c->xs.p, sizeof(double) * n_points, hipMemcpyDeviceToHost, c->stream));
+    if (iters) HIPTRY(hipMemcpyAsync(iters, c->box.p, sizeof(int) * n_points, hipMemcpyDeviceToHost, c->stream));
+    return sync_ctx(c);
+}

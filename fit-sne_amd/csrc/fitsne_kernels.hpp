@@ -1,0 +1,1349 @@
+// fitsne_kernels.hpp -- HIP kernels of the FIt-SNE per-iteration gradient (gfx950, wave64).
+//
+// Two paths share the FFT machinery:
+//  * FP64 "plain": the reference's stages one for one -- charges [1,x,y,x^2+y^2] (or the df != 1 sets),
+//    scattered spread with FP64 atomics, per-term circulant convolution, interpolation of every
+//    potential, the Z expansion and the dC combine (tsne.cpp:1027-1169, nbodyfft.cpp:73-248).
+//  * FP32 form: counting sort by box, wave-per-box spread of node-relative charges (A, b, c), FFT
+//    convolution with the kernel and its odd moments, Z by Parseval in FP64, interpolation fused
+//    with the optimiser update.
+#pragma once
+
+#include <cfloat>
+
+#include "fitsne_common.hpp"
+#include "fitsne_fft.hpp"
+
+namespace fitsne {
+
+constexpr int kThreads = 256;
+
+// ---------------------------------------------------------------- helpers
+
+// Consecutive block indices land on different XCDs (round-robin dispatch over 8 XCDs); map them so
+// that each XCD works on a contiguous range of rows/columns and its L2 merges the partial lines.
+__device__ __forceinline__ int xcd_row(int bid, int nrows) {
+    const int per = (nrows + 7) / 8;
+    return (bid % 8) * per + bid / 8;
+}
+
+template <typename T>
+__device__ __forceinline__ T kernel_value(int kind, double r2, double df) {
+    double v;
+    if (kind == 0) {
+        double q = 1.0 / (1.0 + r2);
+        v = q * q;                            // squared_cauchy (tsne.cpp:69-71, 84-86)
+    } else if (kind == 1) {
+        v = df == 1.0 ? 1.0 / (1.0 + r2) : pow(1.0 + r2 / df, -df);          // general_kernel
+    } else {
+        v = pow(1.0 + r2 / df, -(df + 1.0));  // squared_general_kernel
+    }
+    return (T)v;
+}
+
+template <typename TV, int D>
+__device__ __forceinline__ void load_pt(const TV *__restrict__ Y, long i, TV (&y)[D]) {
+    if constexpr (D == 2) {
+        using V2 = typename std::conditional<std::is_same<TV, float>::value, float2, double2>::type;
+        V2 v = reinterpret_cast<const V2 *>(Y)[i];
+        y[0] = v.x;
+        y[1] = v.y;
+    } else {
+        y[0] = Y[i];
+    }
+}
+
+// ---------------------------------------------------------------- twiddles
+
+template <typename T>
+__global__ void k_twiddles(cx<T> *tw, int n) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) {
+        double s, c;
+        sincospi(-2.0 * (double)e / (double)n, &s, &c);
+        tw[e] = cx<T>{(T)c, (T)s};
+    }
+}
+
+// ---------------------------------------------------------------- attractive forces
+// pos_i = scale * sum_j p_ij q_ij (y_i - y_j), q_ij = 1 / (1 + |y_i - y_j|^2 / df)
+// (tsne.cpp:1121-1137, :965-980, :826-837, :721-732).  SW lanes per CSR row; column indices and
+// values are streamed once with non-temporal loads so the gathered Y stays in cache.
+template <typename TV, typename TI, typename TC, int D, int SW>
+__global__ __launch_bounds__(kThreads) void k_attractive(const TI *__restrict__ rowP, const TC *__restrict__ colP,
+                                                         const TV *__restrict__ valP, const TV *__restrict__ Y,
+                                                         int N, TV inv_df, int df_one, TV scale,
+                                                         TV *__restrict__ pos) {
+    // blocks of consecutive rows go to the same XCD, so each XCD's L2 holds the Y rows its
+    // neighbours gather (the points are kept in spatial order)
+    constexpr int RPB = kThreads / SW;
+    const long rb = (long)(blockIdx.x % 8) * (gridDim.x / 8) + blockIdx.x / 8;
+    const long row = rb * RPB + threadIdx.x / SW;
+    const int lane = threadIdx.x % SW;
+    if (row >= N) return;  // uniform across the SW lanes of a row
+    const long beg = (long)rowP[row], end = (long)rowP[row + 1];
+    TV yi[D];
+    load_pt<TV, D>(Y, row, yi);
+    TV acc[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) acc[d] = (TV)0;
+    for (long k = beg + lane; k < end; k += SW) {
+        const long j = (long)__builtin_nontemporal_load(&colP[k]);
+        const TV v = __builtin_nontemporal_load(&valP[k]);
+        TV yj[D];
+        load_pt<TV, D>(Y, j, yj);
+        TV diff[D], d2 = (TV)0;
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            diff[d] = yi[d] - yj[d];
+            d2 += diff[d] * diff[d];
+        }
+        const TV q = df_one ? (TV)1 / ((TV)1 + d2) : (TV)1 / ((TV)1 + d2 * inv_df);
+        const TV vq = v * q;
+#pragma unroll
+        for (int d = 0; d < D; d++) acc[d] += vq * diff[d];
+    }
+#pragma unroll
+    for (int d = 0; d < D; d++) acc[d] = group_sum<TV, SW>(acc[d]);
+    if (lane == 0) {
+#pragma unroll
+        for (int d = 0; d < D; d++) pos[row * D + d] = acc[d] * scale;
+    }
+}
+
+// ---------------------------------------------------------------- min / max
+// Over the flat sequence Y[0..n): max of all; min of all but index 0 (2D) or of all (1D); Y[0].
+// The 2D reference rule (`else if`, tsne.cpp:1045-1048) equals this whenever Y[0] >= min(rest);
+// otherwise the host runs k_minmax_seq.
+template <typename TV>
+__global__ __launch_bounds__(kThreads) void k_minmax_partial(const TV *__restrict__ Y, long n, long skip,
+                                                             double *__restrict__ part) {
+    double mx = -INFINITY, mn = INFINITY;
+    for (long e = (long)blockIdx.x * kThreads + threadIdx.x; e < n; e += (long)gridDim.x * kThreads) {
+        const double v = (double)Y[e];
+        mx = fmax(mx, v);
+        if (e != skip) mn = fmin(mn, v);
+    }
+    __shared__ double sh[2][kThreads / kWave];
+    mx = wave_max(mx);
+    mn = wave_min(mn);
+    const int w = threadIdx.x / kWave;
+    if (threadIdx.x % kWave == 0) {
+        sh[0][w] = mx;
+        sh[1][w] = mn;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < kThreads / kWave; i++) {
+            mx = fmax(mx, sh[0][i]);
+            mn = fmin(mn, sh[1][i]);
+        }
+        part[blockIdx.x * 2 + 0] = mx;
+        part[blockIdx.x * 2 + 1] = mn;
+    }
+}
+
+template <typename TV>
+__global__ void k_minmax_final(const double *__restrict__ part, int nblk, const TV *__restrict__ Y, long first,
+                               double *__restrict__ out) {
+    double mx = -INFINITY, mn = INFINITY;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+        mx = fmax(mx, part[2 * b]);
+        mn = fmin(mn, part[2 * b + 1]);
+    }
+    mx = wave_max(mx);
+    mn = wave_min(mn);
+    if (threadIdx.x == 0) {
+        out[0] = mx;
+        out[1] = mn;
+        out[2] = (double)Y[first];
+    }
+}
+
+// Exact sequential `if (v > max) max = v; else if (v < min) min = v;` over Y[0..n) in one block:
+// chunk maxima, their exclusive prefix maximum, then each chunk replays the rule from its prefix.
+template <typename TV>
+__device__ __forceinline__ double seq_at(const TV *__restrict__ Y, const int *__restrict__ pos, long e) {
+    return pos ? (double)Y[2L * pos[e >> 1] + (e & 1)] : (double)Y[e];
+}
+
+template <typename TV>
+__global__ __launch_bounds__(1024) void k_minmax_seq(const TV *__restrict__ Y, const int *__restrict__ pos, long n,
+                                                     double *__restrict__ out) {
+    __shared__ double cmax[1024];
+    __shared__ double cmin[1024];
+    const int t = threadIdx.x;
+    const long chunk = (n + 1023) / 1024;
+    const long b = t * chunk, e = b + chunk < n ? b + chunk : n;
+    double m = -INFINITY;
+    for (long i = b; i < e; i++) m = fmax(m, seq_at(Y, pos, i));
+    cmax[t] = m;
+    __syncthreads();
+    if (t == 0) {
+        double run = -INFINITY;
+        for (int i = 0; i < 1024; i++) {
+            double c = cmax[i];
+            cmax[i] = run;  // exclusive prefix
+            run = fmax(run, c);
+        }
+        out[0] = run;
+    }
+    __syncthreads();
+    double mx = cmax[t], mn = INFINITY;
+    for (long i = b; i < e; i++) {
+        const double v = seq_at(Y, pos, i);
+        if (v > mx) mx = v;
+        else if (v < mn) mn = v;
+    }
+    cmin[t] = mn;
+    __syncthreads();
+    if (t == 0) {
+        double r = INFINITY;
+        for (int i = 0; i < 1024; i++) r = fmin(r, cmin[i]);
+        out[1] = r;
+    }
+}
+
+// ---------------------------------------------------------------- counting sort by box (FP32 form)
+
+template <typename TV, int D>
+__global__ __launch_bounds__(kThreads) void k_box_count(const TV *__restrict__ Y, int N, Geo g,
+                                                        int *__restrict__ boxof, int *__restrict__ rank,
+                                                        int *__restrict__ count) {
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= N) return;
+    TV y[D];
+    load_pt<TV, D>(Y, i, y);
+    double t;
+    int b = box_coord<D>((double)y[0], g, t);
+    if constexpr (D == 2) b += box_coord<D>((double)y[1], g, t) * g.nb;
+    boxof[i] = b;
+    rank[i] = atomicAdd(&count[b], 1);
+}
+
+// Exclusive scan of count[0..n) into start[0..n], then count[] = 0 for the next use.  One block.
+__global__ __launch_bounds__(1024) void k_scan_counts(int *__restrict__ count, int *__restrict__ start, int n) {
+    __shared__ int sums[1024];
+    const int t = threadIdx.x;
+    const int chunk = (n + 1023) / 1024;
+    const int b = t * chunk, e = min(b + chunk, n);
+    int s = 0;
+    for (int i = b; i < e; i++) s += count[i];
+    sums[t] = s;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        int v = t >= o ? sums[t - o] : 0;
+        __syncthreads();
+        sums[t] += v;
+        __syncthreads();
+    }
+    int run = sums[t] - s;  // exclusive
+    for (int i = b; i < e; i++) {
+        const int c = count[i];
+        start[i] = run;
+        run += c;
+        count[i] = 0;
+    }
+    if (t == 1023) start[n] = sums[1023];
+}
+
+__global__ __launch_bounds__(kThreads) void k_scatter(const int *__restrict__ boxof, const int *__restrict__ rank,
+                                                      const int *__restrict__ start, int N,
+                                                      int *__restrict__ sorted) {
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i < N) sorted[start[boxof[i]] + rank[i]] = i;
+}
+
+// ---------------------------------------------------------------- spread, FP32 form
+// One wavefront per box.  Nodes are private to their box (node index = box * p + j), so a box's
+// p^D nodes are written once, with no atomics: lanes accumulate their points' node-relative charges
+// A = L, c = L |e|^2, b = L e (e = y - node) in registers, then the wave sums them through LDS.
+// Sums are exact: every FP32 contribution is scaled by 2^38 into an int64 (offsets in units of the
+// box width), so the result does not depend on the order of points within a box and the pipeline is
+// bitwise repeatable.  Output layout: float4 (A, c, b_x, b_y) per node, [u][v] with u the x node
+// (nbodyfft.cpp:137-138).
+constexpr double kFixScale = 274877906944.0;  // 2^38
+
+template <int P, int D>
+__global__ __launch_bounds__(kWave) void k_spread_f32(const float *__restrict__ Y, const int *__restrict__ sorted,
+                                                      const int *__restrict__ start, Geo g,
+                                                      float4 *__restrict__ grid) {
+    constexpr int NV = (D == 2 ? P * P : P);
+    constexpr int NVAL = NV * 4;
+    __shared__ long long sh[NV * (kWave + 1)];
+    const int b = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int bx = D == 2 ? b % g.nb : b;
+    const int by = D == 2 ? b / g.nb : 0;
+    const int s = start[b], e = start[b + 1];
+    float *gf = reinterpret_cast<float *>(grid);
+    if (s == e) {
+        for (int r = lane; r < NVAL; r += kWave) {
+            const int v = r / 4, term = r % 4;
+            long node = D == 2 ? (long)(bx * P + v / P) * g.n1d + by * P + v % P : (long)bx * P + v;
+            gf[node * 4 + term] = 0.f;
+        }
+        return;
+    }
+    long long acc[NV][4];
+#pragma unroll
+    for (int v = 0; v < NV; v++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) acc[v][q] = 0;
+    const float S = (float)kFixScale;
+    const double lox = bx * g.bw0 + g.mn, loy = by * g.bw0 + g.mn;
+    for (int k = s + lane; k < e; k += kWave) {
+        const int i = sorted[k];
+        float y[D];
+        load_pt<float, D>(Y, i, y);
+        const double tx = ((double)y[0] - lox) / g.bw;
+        float Lx[P], dx[P];
+        {
+#pragma unroll
+            for (int a = 0; a < P; a++) dx[a] = (float)(tx - g.sp[a]);
+#pragma unroll
+            for (int a = 0; a < P; a++) {
+                float v = 1.f;
+#pragma unroll
+                for (int c = 0; c < P; c++)
+                    if (c != a) v *= dx[c];
+                Lx[a] = v / (float)g.den[a];
+            }
+        }
+        if constexpr (D == 2) {
+            const double ty = ((double)y[1] - loy) / g.bw;
+            float Ly[P], dy[P];
+#pragma unroll
+            for (int a = 0; a < P; a++) dy[a] = (float)(ty - g.sp[a]);
+#pragma unroll
+            for (int a = 0; a < P; a++) {
+                float v = 1.f;
+#pragma unroll
+                for (int c = 0; c < P; c++)
+                    if (c != a) v *= dy[c];
+                Ly[a] = v / (float)g.den[a];
+            }
+#pragma unroll
+            for (int a = 0; a < P; a++)
+#pragma unroll
+                for (int c = 0; c < P; c++) {
+                    const float w = Lx[a] * Ly[c];
+                    acc[a * P + c][0] += (long long)(w * S);
+                    acc[a * P + c][1] += (long long)(w * (dx[a] * dx[a] + dy[c] * dy[c]) * S);
+                    acc[a * P + c][2] += (long long)(w * dx[a] * S);
+                    acc[a * P + c][3] += (long long)(w * dy[c] * S);
+                }
+        } else {
+#pragma unroll
+            for (int a = 0; a < P; a++) {
+                acc[a][0] += (long long)(Lx[a] * S);
+                acc[a][1] += (long long)(Lx[a] * dx[a] * dx[a] * S);
+                acc[a][2] += (long long)(Lx[a] * dx[a] * S);
+            }
+        }
+    }
+    // wave sum through LDS, one charge term at a time; scale back (offsets were in box widths)
+    const double inv = 1.0 / kFixScale;
+    const double unit[4] = {inv, inv * g.bw0 * g.bw0, inv * g.bw0, inv * g.bw0};
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+#pragma unroll
+        for (int v = 0; v < NV; v++) sh[v * (kWave + 1) + lane] = acc[v][q];
+        __syncthreads();
+        for (int v = lane; v < NV; v += kWave) {
+            long long sum = 0;
+            for (int l = 0; l < kWave; l++) sum += sh[v * (kWave + 1) + l];
+            long node = D == 2 ? (long)(bx * P + v / P) * g.n1d + by * P + v % P : (long)bx * P + v;
+            gf[node * 4 + q] = (float)((double)sum * unit[q]);
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- kernel spectra
+// Row pass: for lattice row a (x offset a h), the even real lines K_q(a, |b|) and K_1(a, |b|) over
+// b in (-n1d, n1d) wrapped into Mp, packed as one complex line; its DFT is T_q + i T_1 (both real).
+// Trow[k][a] = (T_q(a, k), T_1(a, k)), k in [0, H).
+template <typename T, int NTHR, int MAXTOT>
+__global__ __launch_bounds__(NTHR) void k_kern_rows(Geo g, FftPlan pl, const cx<T> *__restrict__ tw, int kind_q,
+                                                    int kind_1, double df, cx<T> *__restrict__ Trow) {
+    extern __shared__ unsigned char smem_raw[];
+    cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
+    const int a = xcd_row(blockIdx.x, g.n1d);
+    if (a >= g.n1d) return;
+    const int Mp = g.Mp, n1d = g.n1d;
+    const double du = a * g.h;
+    for (int b = threadIdx.x; b < Mp; b += NTHR) buf[b] = cx<T>{(T)0, (T)0};
+    __syncthreads();
+    for (int b = threadIdx.x; b < n1d; b += NTHR) {
+        const double dv = b * g.h;
+        const double r2 = du * du + dv * dv;
+        cx<T> val{kernel_value<T>(kind_q, r2, df), kernel_value<T>(kind_1, r2, df)};
+        buf[b] = val;
+        if (b > 0) buf[Mp - b] = val;
+    }
+    __syncthreads();
+    fft_lines<T, NTHR, MAXTOT>(buf, pl, 1, Mp, tw, false);
+    for (int k = threadIdx.x; k < g.H; k += NTHR) Trow[(long)k * n1d + a] = buf[k];
+}
+
+// Column pass: for column frequency k, the line z(a) = T_q(|a|,k)(1 + a h) + i T_1(|a|,k) (a signed).
+// Its DFT is C_q(m,k) + i (C_1(m,k) - S(m,k)) with C even and S odd in m, so
+//   C_q = Re, C_1 = (Im(m) + Im(-m)) / 2, S = (Im(-m) - Im(m)) / 2,
+// where S(m,k) = sum_{a,b} K_q(a,b) a h sin(2 pi m a / Mp) cos(2 pi k b / Mp): the DFT of the odd
+// kernel K_q(a,b) a h is -i S(m,k) and that of K_q(a,b) b h is -i S(k,m) (K_q symmetric in a, b).
+// Stored for m, k in [0, H):  KQ[k][m] = C_q(m,k), K1[k][m] = C_1(m,k), KS[k][m] = S(m,k),
+// KT[k][m] = S(k,m) (written by column m's block).
+template <typename T, int NTHR, int MAXTOT>
+__global__ __launch_bounds__(NTHR) void k_kern_cols(Geo g, FftPlan pl, const cx<T> *__restrict__ tw,
+                                                    const cx<T> *__restrict__ Trow, int need_s,
+                                                    T *__restrict__ KQ, T *__restrict__ K1, T *__restrict__ KS,
+                                                    T *__restrict__ KT) {
+    extern __shared__ unsigned char smem_raw[];
+    cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
+    const int k = xcd_row(blockIdx.x, g.H);
+    if (k >= g.H) return;
+    const int Mp = g.Mp, n1d = g.n1d, H = g.H;
+    for (int a = threadIdx.x; a < Mp; a += NTHR) {
+        int as;
+        if (a < n1d) as = a;
+        else if (a > Mp - n1d) as = a - Mp;
+        else {
+            buf[a] = cx<T>{(T)0, (T)0};
+            continue;
+        }
+        const cx<T> t = Trow[(long)k * n1d + (as < 0 ? -as : as)];
+        const T w = need_s ? (T)1 + (T)(as * g.h) : (T)1;
+        buf[a] = cx<T>{t.x * w, t.y};
+    }
+    __syncthreads();
+    fft_lines<T, NTHR, MAXTOT>(buf, pl, 1, Mp, tw, false);
+    for (int m = threadIdx.x; m < H; m += NTHR) {
+        const cx<T> z = buf[m];
+        const T im_neg = buf[m == 0 ? 0 : Mp - m].y;
+        KQ[(long)k * H + m] = z.x;
+        K1[(long)k * H + m] = (z.y + im_neg) * (T)0.5;
+        if (need_s) {
+            const T s = (im_neg - z.y) * (T)0.5;
+            KS[(long)k * H + m] = s;
+            KT[(long)m * H + k] = s;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- row FFT, forward
+// grid: [n1d][n1d][sin] real (u = x node, v = y node); terms 2l, 2l+1 packed as one complex line.
+// Output spec[t][k][u] = DFT over v of term t of row u, k in [0, H) (two real spectra unpacked from
+// the packed line: A = (Z(k) + conj Z(-k)) / 2, B = -i (Z(k) - conj Z(-k)) / 2).
+template <typename T, int NTHR, int MAXTOT>
+__global__ __launch_bounds__(NTHR) void k_rows_fwd(Geo g, FftPlan pl, const cx<T> *__restrict__ tw,
+                                                   const T *__restrict__ grid, int sin, int nterms,
+                                                   cx<T> *__restrict__ spec) {
+    extern __shared__ unsigned char smem_raw[];
+    cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
+    const int u = xcd_row(blockIdx.x, g.n1d);
+    if (u >= g.n1d) return;
+    const int Mp = g.Mp, n1d = g.n1d, H = g.H;
+    const int lines = (nterms + 1) / 2;
+    for (int q = threadIdx.x; q < lines * Mp; q += NTHR) {
+        const int l = q / Mp, v = q - l * Mp;
+        cx<T> z{(T)0, (T)0};
+        if (v < n1d) {
+            const T *src = grid + ((long)u * n1d + v) * sin + 2 * l;
+            z.x = src[0];
+            if (2 * l + 1 < nterms) z.y = src[1];
+        }
+        buf[q] = z;
+    }
+    __syncthreads();
+    fft_lines<T, NTHR, MAXTOT>(buf, pl, lines, Mp, tw, false);
+    const long plane = (long)H * n1d;
+    for (int q = threadIdx.x; q < lines * H; q += NTHR) {
+        const int l = q / H, k = q - l * H;
+        const cx<T> Z = buf[l * Mp + k];
+        const cx<T> Zc = conj(buf[l * Mp + (k == 0 ? 0 : Mp - k)]);
+        spec[(2 * l) * plane + (long)k * n1d + u] = (T)0.5 * (Z + Zc);
+        if (2 * l + 1 < nterms) spec[(2 * l + 1) * plane + (long)k * n1d + u] = (T)0.5 * mul_negi(Z - Zc);
+    }
+}
+
+// ---------------------------------------------------------------- column pass, FP32 form
+// For column frequency k: DFT over u of A, c, b_x, b_y; then with K_q, K_1 (even) and the odd kernels
+// K_x = -i S(m,k), K_y = -i S(k,m):
+//   V1 = K_q A,  G_x = K_x A - K_q b_x,  G_y = K_y A - K_q b_y    (inverse-transformed over u)
+// and the Parseval sums of Z (k weighted 1 at k = 0, Mp/2 and 2 elsewhere):
+//   z0 = sum K_1 |A|^2, z1 = sum Re(conj(c) K_q A), z2 = sum Re(conj(b).K_D A), z3 = sum K_q |b|^2.
+template <int NTHR, int MAXTOT>
+__global__ __launch_bounds__(NTHR) void k_cols_f32(Geo g, FftPlan pl, const cx<float> *__restrict__ tw,
+                                                   const cx<float> *__restrict__ spec, const float *__restrict__ KQ,
+                                                   const float *__restrict__ K1, const float *__restrict__ KS,
+                                                   const float *__restrict__ KT, cx<float> *__restrict__ ospec,
+                                                   double *__restrict__ zpart) {
+    extern __shared__ unsigned char smem_raw[];
+    cx<float> *buf = reinterpret_cast<cx<float> *>(smem_raw);
+    const int k = xcd_row(blockIdx.x, g.H);
+    if (k >= g.H) return;
+    const int Mp = g.Mp, n1d = g.n1d, H = g.H;
+    const long plane = (long)H * n1d;
+    for (int q = threadIdx.x; q < 4 * Mp; q += NTHR) {
+        const int t = q / Mp, u = q - t * Mp;
+        buf[q] = u < n1d ? spec[t * plane + (long)k * n1d + u] : cx<float>{0.f, 0.f};
+    }
+    __syncthreads();
+    fft_lines<float, NTHR, MAXTOT>(buf, pl, 4, Mp, tw, false);
+    const double wk = (k == 0 || 2 * k == Mp) ? 1.0 : 2.0;
+    double z0 = 0, z1 = 0, z2 = 0, z3 = 0;
+    for (int m = threadIdx.x; m < Mp; m += NTHR) {
+        const int mb = m <= Mp / 2 ? m : Mp - m;
+        const float sg = m <= Mp / 2 ? 1.f : -1.f;
+        const float cq = KQ[(long)k * H + mb], c1 = K1[(long)k * H + mb];
+        const float sx = sg * KS[(long)k * H + mb], sy = KT[(long)k * H + mb];
+        const cx<float> A = buf[m], C = buf[Mp + m], BX = buf[2 * Mp + m], BY = buf[3 * Mp + m];
+        const cx<float> KxA = mul_negi(sx * A), KyA = mul_negi(sy * A);
+        buf[m] = cq * A;
+        buf[Mp + m] = KxA - cq * BX;
+        buf[2 * Mp + m] = KyA - cq * BY;
+        const double a2 = (double)A.x * A.x + (double)A.y * A.y;
+        z0 += (double)c1 * a2;
+        z1 += (double)cq * ((double)C.x * A.x + (double)C.y * A.y);
+        // Re(conj(B) (-i s) A) = s Im(conj(B) A)
+        z2 += (double)sx * ((double)BX.x * A.y - (double)BX.y * A.x) + (double)sy * ((double)BY.x * A.y - (double)BY.y * A.x);
+        z3 += (double)cq * ((double)BX.x * BX.x + (double)BX.y * BX.y + (double)BY.x * BY.x + (double)BY.y * BY.y);
+    }
+    __syncthreads();
+    fft_lines<float, NTHR, MAXTOT>(buf, pl, 3, Mp, tw, true);
+    for (int q = threadIdx.x; q < 3 * n1d; q += NTHR) {
+        const int t = q / n1d, u = q - t * n1d;
+        ospec[t * plane + (long)k * n1d + u] = buf[t * Mp + u];
+    }
+    __shared__ double shz[kThreads / kWave];
+    z0 = block_sum(z0 * wk, shz);
+    z1 = block_sum(z1 * wk, shz);
+    z2 = block_sum(z2 * wk, shz);
+    z3 = block_sum(z3 * wk, shz);
+    if (threadIdx.x == 0) {
+        zpart[k * 4 + 0] = z0;
+        zpart[k * 4 + 1] = z1;
+        zpart[k * 4 + 2] = z2;
+        zpart[k * 4 + 3] = z3;
+    }
+}
+
+// ---------------------------------------------------------------- column pass, plain (FP64)
+// Each term: DFT over u, times the real even kernel spectrum C(m,k), inverse DFT over u.
+template <typename T, int NTHR, int MAXTOT>
+__global__ __launch_bounds__(NTHR) void k_cols_plain(Geo g, FftPlan pl, const cx<T> *__restrict__ tw,
+                                                     const cx<T> *__restrict__ spec, int nterms,
+                                                     const T *__restrict__ KC, cx<T> *__restrict__ ospec) {
+    extern __shared__ unsigned char smem_raw[];
+    cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
+    const int k = xcd_row(blockIdx.x, g.H);
+    if (k >= g.H) return;
+    const int Mp = g.Mp, n1d = g.n1d, H = g.H;
+    const long plane = (long)H * n1d;
+    const int lines_max = MAXTOT / Mp;
+    for (int t0 = 0; t0 < nterms; t0 += lines_max) {
+        const int lines = min(lines_max, nterms - t0);
+        for (int q = threadIdx.x; q < lines * Mp; q += NTHR) {
+            const int t = q / Mp, u = q - t * Mp;
+            buf[q] = u < n1d ? spec[(t0 + t) * plane + (long)k * n1d + u] : cx<T>{(T)0, (T)0};
+        }
+        __syncthreads();
+        fft_lines<T, NTHR, MAXTOT>(buf, pl, lines, Mp, tw, false);
+        for (int q = threadIdx.x; q < lines * Mp; q += NTHR) {
+            const int t = q / Mp, m = q - t * Mp;
+            const int mb = m <= Mp / 2 ? m : Mp - m;
+            buf[q] = KC[(long)k * H + mb] * buf[q];
+        }
+        __syncthreads();
+        fft_lines<T, NTHR, MAXTOT>(buf, pl, lines, Mp, tw, true);
+        for (int q = threadIdx.x; q < lines * n1d; q += NTHR) {
+            const int t = q / n1d, u = q - t * n1d;
+            ospec[(t0 + t) * plane + (long)k * n1d + u] = buf[t * Mp + u];
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- row FFT, inverse
+// ospec[t][k][u], k in [0, H): half spectra over v of real rows.  Terms 2l, 2l+1 are rebuilt into one
+// Hermitian-extended complex line (self-conjugate bins taken real), inverse-transformed, scaled, and
+// written to out[u][v][sout + ...].
+template <typename T, int NTHR, int MAXTOT>
+__global__ __launch_bounds__(NTHR) void k_rows_inv(Geo g, FftPlan pl, const cx<T> *__restrict__ tw,
+                                                   const cx<T> *__restrict__ ospec, int nterms, T scale,
+                                                   T *__restrict__ out, int sout) {
+    extern __shared__ unsigned char smem_raw[];
+    cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
+    const int u = xcd_row(blockIdx.x, g.n1d);
+    if (u >= g.n1d) return;
+    const int Mp = g.Mp, n1d = g.n1d, H = g.H;
+    const long plane = (long)H * n1d;
+    const int lines = (nterms + 1) / 2;
+    for (int q = threadIdx.x; q < lines * Mp; q += NTHR) {
+        const int l = q / Mp, k = q - l * Mp;
+        const bool lo = k < H;
+        const int kk = lo ? k : Mp - k;
+        cx<T> f1 = ospec[(2 * l) * plane + (long)kk * n1d + u];
+        cx<T> f2 = 2 * l + 1 < nterms ? ospec[(2 * l + 1) * plane + (long)kk * n1d + u] : cx<T>{(T)0, (T)0};
+        if (kk == 0 || 2 * kk == Mp) {
+            f1.y = (T)0;
+            f2.y = (T)0;
+        }
+        if (!lo) {
+            f1 = conj(f1);
+            f2 = conj(f2);
+        }
+        buf[q] = f1 + mul_i(f2);
+    }
+    __syncthreads();
+    fft_lines<T, NTHR, MAXTOT>(buf, pl, lines, Mp, tw, true);
+    for (int q = threadIdx.x; q < lines * n1d; q += NTHR) {
+        const int l = q / n1d, v = q - l * n1d;
+        const cx<T> z = buf[l * Mp + v];
+        T *dst = out + ((long)u * n1d + v) * sout + 2 * l;
+        dst[0] = z.x * scale;
+        if (2 * l + 1 < nterms) dst[1] = z.y * scale;
+    }
+}
+
+// ---------------------------------------------------------------- 1D convolution, one workgroup
+// The 1D grid is at most a few thousand nodes: one block of NTHR threads does the whole convolution.
+// The kernel line is transformed first and its spectra kept in registers (thread t owns bins
+// m = t + c NTHR); LDS is then reused for the data lines.  Requires Mp <= MAXM.
+// FP32 form (f32form = 1): kernel line K_q(|a|)(1 + a h) + i K_1(|a|) gives C_q, C_1 and S as in
+// k_kern_cols; data lines z = A + i b and c; V1 = C_q A, G = -i S A - C_q b, one inverse of V1 + i G;
+// Z partial sums z0..z3 as in k_cols_f32 (full spectrum, no half-plane weights).
+// Plain (f32form = 0): real even spectrum of K_q (or K_1 when use_k1) times each packed pair of terms.
+template <typename T, int NTHR, int MAXM>
+__global__ __launch_bounds__(NTHR) void k_conv1d(Geo g, FftPlan pl, const cx<T> *__restrict__ tw, int kind_q,
+                                                 int kind_1, double df, int f32form, const T *__restrict__ grid,
+                                                 int sin, int nterms, int use_k1, T *__restrict__ out, int sout,
+                                                 double *__restrict__ zsum, const T *__restrict__ kspec) {
+    constexpr int NC = (MAXM + NTHR - 1) / NTHR;
+    extern __shared__ unsigned char smem_raw[];
+    cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
+    const int Mp = g.Mp, n1d = g.n1d;
+    T cq[NC], c1[NC], ss[NC];
+    if (kspec) {  // plain path with a caller-given spectrum (mirror tier)
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            const int m = threadIdx.x + c * NTHR;
+            cq[c] = m < Mp ? kspec[m] : (T)0;
+            c1[c] = ss[c] = (T)0;
+        }
+    } else {
+        // kernel line
+        for (int q = threadIdx.x; q < Mp; q += NTHR) buf[q] = cx<T>{(T)0, (T)0};
+        __syncthreads();
+        for (int a = threadIdx.x; a < n1d; a += NTHR) {
+            const double d = a * g.h;
+            const T kq = kernel_value<T>(kind_q, d * d, df), k1 = kernel_value<T>(kind_1, d * d, df);
+            if (f32form) {
+                buf[a] = cx<T>{kq * ((T)1 + (T)d), k1};
+                if (a > 0) buf[Mp - a] = cx<T>{kq * ((T)1 - (T)d), k1};
+            } else {
+                buf[a] = cx<T>{use_k1 ? k1 : kq, (T)0};
+                if (a > 0) buf[Mp - a] = buf[a];
+            }
+        }
+        __syncthreads();
+        fft_lines<T, NTHR, MAXM>(buf, pl, 1, Mp, tw, false);
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            const int m = threadIdx.x + c * NTHR;
+            cq[c] = c1[c] = ss[c] = (T)0;
+            if (m < Mp) {
+                const cx<T> kz = buf[m];
+                const T imn = buf[m == 0 ? 0 : Mp - m].y;
+                cq[c] = kz.x;
+                c1[c] = (kz.y + imn) * (T)0.5;
+                ss[c] = (imn - kz.y) * (T)0.5;
+            }
+        }
+        __syncthreads();
+    }
+    const T sc = (T)1 / (T)Mp;
+    double z0 = 0, z1 = 0, z2 = 0, z3 = 0;
+    if (f32form) {
+        cx<T> *dl = buf;  // 2 lines: A + i b, c
+        for (int q = threadIdx.x; q < 2 * Mp; q += NTHR) {
+            const int l = q / Mp, a = q - l * Mp;
+            cx<T> z{(T)0, (T)0};
+            if (a < n1d) {
+                const T *s = grid + (long)a * sin;
+                z = l == 0 ? cx<T>{s[0], s[2]} : cx<T>{s[1], (T)0};
+            }
+            dl[q] = z;
+        }
+        __syncthreads();
+        fft_lines<T, NTHR, MAXM>(dl, pl, 2, Mp, tw, false);
+        cx<T> o[NC];
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            const int m = threadIdx.x + c * NTHR;
+            o[c] = cx<T>{(T)0, (T)0};
+            if (m < Mp) {
+                const int mn = m == 0 ? 0 : Mp - m;
+                const cx<T> Z = dl[m], Zc = conj(dl[mn]);
+                const cx<T> A = (T)0.5 * (Z + Zc), B = (T)0.5 * mul_negi(Z - Zc);
+                const cx<T> C = dl[Mp + m];
+                const cx<T> V1 = cq[c] * A, G = mul_negi(ss[c] * A) - cq[c] * B;
+                o[c] = V1 + mul_i(G);
+                z0 += (double)c1[c] * ((double)A.x * A.x + (double)A.y * A.y);
+                z1 += (double)cq[c] * ((double)C.x * A.x + (double)C.y * A.y);
+                z2 += (double)ss[c] * ((double)B.x * A.y - (double)B.y * A.x);
+                z3 += (double)cq[c] * ((double)B.x * B.x + (double)B.y * B.y);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            const int m = threadIdx.x + c * NTHR;
+            if (m < Mp) dl[m] = o[c];
+        }
+        __syncthreads();
+        fft_lines<T, NTHR, MAXM>(dl, pl, 1, Mp, tw, true);
+        for (int a = threadIdx.x; a < n1d; a += NTHR) {
+            out[(long)a * sout + 0] = dl[a].x * sc;
+            out[(long)a * sout + 1] = dl[a].y * sc;
+        }
+    } else {
+        const int dlines = (nterms + 1) / 2;
+        for (int l = 0; l < dlines; l++) {
+            for (int a = threadIdx.x; a < Mp; a += NTHR) {
+                cx<T> z{(T)0, (T)0};
+                if (a < n1d) {
+                    const T *s = grid + (long)a * sin + 2 * l;
+                    z = cx<T>{s[0], 2 * l + 1 < nterms ? s[1] : (T)0};
+                }
+                buf[a] = z;
+            }
+            __syncthreads();
+            fft_lines<T, NTHR, MAXM>(buf, pl, 1, Mp, tw, false);
+#pragma unroll
+            for (int c = 0; c < NC; c++) {
+                const int m = threadIdx.x + c * NTHR;
+                if (m < Mp) buf[m] = cq[c] * buf[m];
+            }
+            __syncthreads();
+            fft_lines<T, NTHR, MAXM>(buf, pl, 1, Mp, tw, true);
+            for (int a = threadIdx.x; a < n1d; a += NTHR) {
+                out[(long)a * sout + 2 * l] = buf[a].x * sc;
+                if (2 * l + 1 < nterms) out[(long)a * sout + 2 * l + 1] = buf[a].y * sc;
+            }
+            __syncthreads();
+        }
+    }
+    if (zsum) {
+        __shared__ double shz[NTHR / kWave];
+        z0 = block_sum(z0, shz);
+        z1 = block_sum(z1, shz);
+        z2 = block_sum(z2, shz);
+        z3 = block_sum(z3, shz);
+        if (threadIdx.x == 0) {
+            zsum[0] = z0;
+            zsum[1] = z1;
+            zsum[2] = z2;
+            zsum[3] = z3;
+        }
+    }
+}
+
+// Real even spectrum of one 1D kernel on the node lattice (precompute of nbodyfft.cpp:251-307,
+// in this library's layout: C(m) for m in [0, Mp)).
+template <typename T, int NTHR, int MAXM>
+__global__ __launch_bounds__(NTHR) void k_kspec1d(Geo g, FftPlan pl, const cx<T> *__restrict__ tw, int kind,
+                                                  double df, T *__restrict__ out) {
+    extern __shared__ unsigned char smem_raw[];
+    cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
+    const int Mp = g.Mp, n1d = g.n1d;
+    for (int q = threadIdx.x; q < Mp; q += NTHR) buf[q] = cx<T>{(T)0, (T)0};
+    __syncthreads();
+    for (int a = threadIdx.x; a < n1d; a += NTHR) {
+        const double d = a * g.h;
+        buf[a] = cx<T>{kernel_value<T>(kind, d * d, df), (T)0};
+        if (a > 0) buf[Mp - a] = buf[a];
+    }
+    __syncthreads();
+    fft_lines<T, NTHR, MAXM>(buf, pl, 1, Mp, tw, false);
+    for (int m = threadIdx.x; m < Mp; m += NTHR) out[m] = buf[m].x;
+}
+
+// Final Z of the FP32 form from the per-column Parseval partials, in FP64 and a fixed order:
+//   df = 1:  Z = (z0 + 2 z1 + 4 z2 - 2 z3) / Mp^D - N;   df != 1:  Z = z0 / Mp^D - N.
+__global__ void k_zfinal(const double *__restrict__ zpart, int ncols, int df_one, double norm, double N,
+                         double *__restrict__ zout) {
+    __shared__ double sh[kThreads / kWave];
+    double s = 0;
+    for (int k = threadIdx.x; k < ncols; k += blockDim.x) {
+        const double *z = zpart + 4 * k;
+        s += df_one ? (z[0] + 2.0 * z[1] + 4.0 * z[2] - 2.0 * z[3]) : z[0];
+    }
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) {
+        const double Z = s * norm - N;
+        zout[0] = Z;
+        zout[1] = 1.0 / Z;
+    }
+}
+
+// ---------------------------------------------------------------- interpolation + update, FP32 form
+struct UpdPrm {
+    float momentum, lr, max_step_norm;
+    int plain_descent;
+    int mode;  // 0: write dC only; 1: optimiser update of Y, uY, gains
+};
+
+template <int P, int D>
+__global__ __launch_bounds__(kThreads) void k_interp_f32(float *__restrict__ Y, const float *__restrict__ pos,
+                                                         const float4 *__restrict__ gout, Geo g,
+                                                         const double *__restrict__ zbuf, UpdPrm up,
+                                                         float *__restrict__ dC, float *__restrict__ uY,
+                                                         float *__restrict__ gains, double *__restrict__ part,
+                                                         int N) {
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    double ysum[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) ysum[d] = 0.0;
+    if (i < N) {
+        float y[D];
+        load_pt<float, D>(Y, i, y);
+        const float zinv = (float)zbuf[1];
+        float Lx[P], ex[P], Ly[P], ey[P];
+        int bx, by = 0;
+        {
+            double t;
+            bx = box_coord<D>((double)y[0], g, t);
+            float dt[P];
+#pragma unroll
+            for (int a = 0; a < P; a++) dt[a] = (float)(t - g.sp[a]);
+#pragma unroll
+            for (int a = 0; a < P; a++) {
+                float v = 1.f;
+#pragma unroll
+                for (int c = 0; c < P; c++)
+                    if (c != a) v *= dt[c];
+                Lx[a] = v / (float)g.den[a];
+                ex[a] = dt[a] * (float)g.bw0;
+            }
+        }
+        if constexpr (D == 2) {
+            double t;
+            by = box_coord<D>((double)y[1], g, t);
+            float dt[P];
+#pragma unroll
+            for (int a = 0; a < P; a++) dt[a] = (float)(t - g.sp[a]);
+#pragma unroll
+            for (int a = 0; a < P; a++) {
+                float v = 1.f;
+#pragma unroll
+                for (int c = 0; c < P; c++)
+                    if (c != a) v *= dt[c];
+                Ly[a] = v / (float)g.den[a];
+                ey[a] = dt[a] * (float)g.bw0;
+            }
+        }
+        float neg[D];
+#pragma unroll
+        for (int d = 0; d < D; d++) neg[d] = 0.f;
+        if constexpr (D == 2) {
+#pragma unroll
+            for (int a = 0; a < P; a++) {
+                const float4 *rowp = gout + (long)(bx * P + a) * g.n1d + by * P;
+#pragma unroll
+                for (int c = 0; c < P; c++) {
+                    const float4 G = rowp[c];
+                    const float w = Lx[a] * Ly[c];
+                    neg[0] += w * (ex[a] * G.x + G.y);
+                    neg[1] += w * (ey[c] * G.x + G.z);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int a = 0; a < P; a++) {
+                const float4 G = gout[(long)bx * P + a];
+                neg[0] += Lx[a] * (ex[a] * G.x + G.y);
+            }
+        }
+        float dy[D];
+#pragma unroll
+        for (int d = 0; d < D; d++) dy[d] = pos[(long)i * D + d] - neg[d] * zinv;
+        if (up.mode == 0) {
+#pragma unroll
+            for (int d = 0; d < D; d++) dC[(long)i * D + d] = dy[d];
+        } else {
+            float yn[D];
+            if (up.plain_descent) {
+#pragma unroll
+                for (int d = 0; d < D; d++) yn[d] = y[d] - dy[d];
+            } else {
+                float u[D], gn[D], nrm = 0.f;
+#pragma unroll
+                for (int d = 0; d < D; d++) {
+                    u[d] = uY[(long)i * D + d];
+                    gn[d] = gains[(long)i * D + d];
+                    const float sd = dy[d] == 0.f ? 0.f : (dy[d] < 0.f ? -1.f : 1.f);
+                    const float su = u[d] == 0.f ? 0.f : (u[d] < 0.f ? -1.f : 1.f);
+                    gn[d] = sd != su ? gn[d] + 0.2f : gn[d] * 0.8f;
+                    if (gn[d] < 0.01f) gn[d] = 0.01f;
+                    u[d] = up.momentum * u[d] - up.lr * gn[d] * dy[d];
+                    nrm += u[d] * u[d];
+                }
+                if (up.max_step_norm > 0.f) {
+                    nrm = sqrtf(nrm);
+                    if (nrm > up.max_step_norm) {
+#pragma unroll
+                        for (int d = 0; d < D; d++) u[d] *= up.max_step_norm / nrm;
+                    }
+                }
+#pragma unroll
+                for (int d = 0; d < D; d++) {
+                    uY[(long)i * D + d] = u[d];
+                    gains[(long)i * D + d] = gn[d];
+                    yn[d] = y[d] + u[d];
+                }
+            }
+#pragma unroll
+            for (int d = 0; d < D; d++) {
+                Y[(long)i * D + d] = yn[d];
+                ysum[d] = (double)yn[d];
+            }
+        }
+    }
+    if (up.mode == 1) {
+        __shared__ double sh[kThreads / kWave];
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            const double s = block_sum(ysum[d], sh);
+            if (threadIdx.x == 0) part[blockIdx.x * D + d] = s;
+        }
+    }
+}
+
+// Mean of the updated Y from block partials (fixed order), FP64.
+__global__ void k_mean_final(const double *__restrict__ part, int nblk, int D, double N, double *__restrict__ mean) {
+    __shared__ double sh[kThreads / kWave];
+    for (int d = 0; d < D; d++) {
+        double s = 0;
+        for (int b = threadIdx.x; b < nblk; b += blockDim.x) s += part[b * D + d];
+        s = block_sum(s, sh);
+        if (threadIdx.x == 0) mean[d] = s / N;
+        __syncthreads();
+    }
+}
+
+// zeroMean (tsne.cpp:1851-1876): Y -= mean, with the min/max partials of the result for the next
+// geometry (same layout as k_minmax_partial).
+template <typename TV>
+__global__ __launch_bounds__(kThreads) void k_center(TV *__restrict__ Y, long n, int D, long skip,
+                                                     const double *__restrict__ mean, double *__restrict__ part) {
+    double mx = -INFINITY, mn = INFINITY;
+    const double m0 = mean[0], m1 = D == 2 ? mean[1] : 0.0;
+    for (long e = (long)blockIdx.x * kThreads + threadIdx.x; e < n; e += (long)gridDim.x * kThreads) {
+        const double m = (D == 2 && (e & 1)) ? m1 : m0;
+        const TV v = (TV)((double)Y[e] - m);
+        Y[e] = v;
+        mx = fmax(mx, (double)v);
+        if (e != skip) mn = fmin(mn, (double)v);
+    }
+    __shared__ double sh[2][kThreads / kWave];
+    mx = wave_max(mx);
+    mn = wave_min(mn);
+    const int w = threadIdx.x / kWave;
+    if (threadIdx.x % kWave == 0) {
+        sh[0][w] = mx;
+        sh[1][w] = mn;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < kThreads / kWave; i++) {
+            mx = fmax(mx, sh[0][i]);
+            mn = fmin(mn, sh[1][i]);
+        }
+        part[blockIdx.x * 2 + 0] = mx;
+        part[blockIdx.x * 2 + 1] = mn;
+    }
+}
+
+// ---------------------------------------------------------------- spatial reordering (FP32 form)
+// New row r is old row rel[r]; column indices are renumbered through inv (old -> new).  One
+// wavefront per new row, entries copied in their original order (per-row sums keep their order).
+template <typename TV>
+__global__ __launch_bounds__(kThreads) void k_permute_csr(const int64_t *__restrict__ rowP_old,
+                                                          const uint32_t *__restrict__ colP_old,
+                                                          const TV *__restrict__ val_old,
+                                                          const int64_t *__restrict__ rowP_new,
+                                                          const int *__restrict__ rel, const int *__restrict__ inv,
+                                                          int N, uint32_t *__restrict__ colP_new,
+                                                          TV *__restrict__ val_new) {
+    const long r = ((long)blockIdx.x * kThreads + threadIdx.x) / kWave;
+    const int lane = threadIdx.x % kWave;
+    if (r >= N) return;
+    const long o = rel[r];
+    const long ob = rowP_old[o], ne = rowP_new[r + 1], nb = rowP_new[r];
+    for (long k = lane; k < ne - nb; k += kWave) {
+        colP_new[nb + k] = (uint32_t)inv[colP_old[ob + k]];
+        val_new[nb + k] = val_old[ob + k];
+    }
+}
+
+// out[r*D + d] = in[rel[r]*D + d]
+template <typename TV>
+__global__ __launch_bounds__(kThreads) void k_gather_rows(const TV *__restrict__ in, const int *__restrict__ rel, int N,
+                                                          int D, TV *__restrict__ out) {
+    const long e = (long)blockIdx.x * kThreads + threadIdx.x;
+    if (e < (long)N * D) out[e] = in[(long)rel[e / D] * D + e % D];
+}
+
+// ---------------------------------------------------------------- plain FP64 path stages
+
+template <typename T>
+__global__ void k_fill(T *__restrict__ p, long n, T v) {
+    for (long e = (long)blockIdx.x * kThreads + threadIdx.x; e < n; e += (long)gridDim.x * kThreads) p[e] = v;
+}
+
+// Charges of the reference (tsne.cpp:1057-1062, :904-908, :939-941, :786-790, :674-677, :689-691).
+// set: 0 = [1, x, y, x^2+y^2] (2D df=1), 1 = [x, y, 1] (2D df), 2 = [1] (general kernel),
+//      3 = [1, y, y^2] (1D df=1), 4 = [y, 1] (1D df).
+template <int D>
+__global__ __launch_bounds__(kThreads) void k_charges(const double *__restrict__ Y, int N, int set,
+                                                      double *__restrict__ ch) {
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= N) return;
+    const double x = Y[(long)i * D], y = D == 2 ? Y[(long)i * D + 1] : 0.0;
+    switch (set) {
+    case 0: ch[4L * i] = 1; ch[4L * i + 1] = x; ch[4L * i + 2] = y; ch[4L * i + 3] = x * x + y * y; break;
+    case 1: ch[3L * i] = x; ch[3L * i + 1] = y; ch[3L * i + 2] = 1; break;
+    case 2: ch[i] = 1; break;
+    case 3: ch[3L * i] = 1; ch[3L * i + 1] = x; ch[3L * i + 2] = x * x; break;
+    default: ch[2L * i] = x; ch[2L * i + 1] = 1; break;
+    }
+}
+
+// Box index, in-box coordinates and Lagrange weights per point (nbodyfft.cpp:85-127 / :349-371):
+// box[i], wx[j*N + i], wy[j*N + i] (SoA like the reference's interpolated_values).
+template <int P, int D>
+__global__ __launch_bounds__(kThreads) void k_weights_f64(const double *__restrict__ xs, const double *__restrict__ ys,
+                                                          int N, Geo g, const double *__restrict__ lox,
+                                                          const double *__restrict__ loy, int *__restrict__ box,
+                                                          double *__restrict__ wx, double *__restrict__ wy) {
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= N) return;
+    double tx, ty = 0;
+    int bx = box_coord<D>(xs[i], g, tx), by = 0;
+    if (lox) tx = (xs[i] - lox[bx]) / g.bw;  // caller-given box bounds (mirror tier)
+    if constexpr (D == 2) {
+        by = box_coord<D>(ys[i], g, ty);
+        if (loy) ty = (ys[i] - loy[by]) / g.bw;
+    }
+    box[i] = by * g.nb + bx;
+    double L[P];
+    lagrange<P, double>(tx, g, L);
+#pragma unroll
+    for (int j = 0; j < P; j++) wx[(long)j * N + i] = L[j];
+    if constexpr (D == 2) {
+        lagrange<P, double>(ty, g, L);
+#pragma unroll
+        for (int j = 0; j < P; j++) wy[(long)j * N + i] = L[j];
+    }
+}
+
+// interpolate() of nbodyfft.cpp:310-336 on the GPU: out[j*N + i] = L_j(y[i]), spacings/denominators in g.
+template <int P>
+__global__ __launch_bounds__(kThreads) void k_lagrange_f64(const double *__restrict__ y, int N, Geo g,
+                                                           double *__restrict__ out) {
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= N) return;
+    double L[P];
+    lagrange<P, double>(y[i], g, L);
+#pragma unroll
+    for (int j = 0; j < P; j++) out[(long)j * N + i] = L[j];
+}
+
+// Spread with FP64 atomics into w[node][nterms] (nbodyfft.cpp:129-147 / :373-382).
+template <int P, int D>
+__global__ __launch_bounds__(kThreads) void k_spread_f64(const int *__restrict__ box, const double *__restrict__ wx,
+                                                         const double *__restrict__ wy,
+                                                         const double *__restrict__ ch, int nterms, int N, Geo g,
+                                                         double *__restrict__ w) {
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= N) return;
+    const int b = box[i];
+    const int bx = b % g.nb, by = b / g.nb;
+    for (int a = 0; a < P; a++) {
+        const double lx = wx[(long)a * N + i];
+        if constexpr (D == 2) {
+            for (int c = 0; c < P; c++) {
+                const double ly = wy[(long)c * N + i];
+                const long idx = (long)(bx * P + a) * g.n1d + by * P + c;
+                for (int d = 0; d < nterms; d++) atomicAdd(&w[idx * nterms + d], ly * lx * ch[(long)i * nterms + d]);
+            }
+        } else {
+            const long idx = (long)bx * P + a;
+            for (int d = 0; d < nterms; d++) atomicAdd(&w[idx * nterms + d], lx * ch[(long)i * nterms + d]);
+        }
+    }
+}
+
+// Interpolate potentials back (nbodyfft.cpp:222-239 / :439-447), accumulating like the reference.
+template <int P, int D>
+__global__ __launch_bounds__(kThreads) void k_interp_f64(const int *__restrict__ box, const double *__restrict__ wx,
+                                                         const double *__restrict__ wy,
+                                                         const double *__restrict__ V, int nterms, int N, Geo g,
+                                                         double *__restrict__ pot) {
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= N) return;
+    const int b = box[i];
+    const int bx = b % g.nb, by = b / g.nb;
+    double acc[4] = {0, 0, 0, 0};
+    for (int a = 0; a < P; a++) {
+        const double lx = wx[(long)a * N + i];
+        if constexpr (D == 2) {
+            for (int c = 0; c < P; c++) {
+                const double ly = wy[(long)c * N + i];
+                const long idx = (long)(bx * P + a) * g.n1d + by * P + c;
+                for (int d = 0; d < nterms; d++) acc[d] += lx * ly * V[idx * nterms + d];
+            }
+        } else {
+            const long idx = (long)bx * P + a;
+            for (int d = 0; d < nterms; d++) acc[d] += lx * V[idx * nterms + d];
+        }
+    }
+    for (int d = 0; d < nterms; d++) pot[(long)i * nterms + d] += acc[d];
+}
+
+// Per-point Z terms and block partials (tsne.cpp:1101-1110 / :945-950 / :809-817 / :705-713).
+// zform: 0 = 2D expansion on [1,x,y,x^2+y^2]; 1 = sum of phi (general kernel); 2 = 1D expansion.
+__global__ __launch_bounds__(kThreads) void k_zterms_f64(const double *__restrict__ Y, const double *__restrict__ pot,
+                                                         int N, int D, int zform, double *__restrict__ part) {
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    double s = 0;
+    if (i < N) {
+        if (zform == 0) {
+            const double x = Y[2L * i], y = Y[2L * i + 1];
+            const double *p = pot + 4L * i;
+            s = (1 + x * x + y * y) * p[0] - 2 * (x * p[1] + y * p[2]) + p[3];
+        } else if (zform == 1) {
+            s = pot[i];
+        } else {
+            const double y = Y[i];
+            const double *p = pot + 3L * i;
+            s = (1 + y * y) * p[0] - 2 * (y * p[1]) + p[2];
+        }
+    }
+    __shared__ double sh[kThreads / kWave];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ void k_sum_final(const double *__restrict__ part, int nblk, double sub, double *__restrict__ out) {
+    __shared__ double sh[kThreads / kWave];
+    double s = 0;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) s += part[b];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) {
+        out[0] = s - sub;
+        out[1] = 1.0 / (s - sub);
+    }
+}
+
+// dC = pos_f - neg_f (tsne.cpp:1148-1155 / :990-997 / :846-850 / :735-741).
+// cform: 0 = 2D [1,x,y,..]: neg = (x phi0 - phi1, y phi0 - phi2) / Z, stride 4
+//        1 = 2D df [x,y,1]: neg = (x phi2 - phi0, y phi2 - phi1) / Z, stride 3
+//        2 = 1D [1,y,y^2]: neg = (y phi0 - phi1) / Z, stride 3;  3 = 1D df [y,1]: (y phi1 - phi0)/Z, stride 2
+__global__ __launch_bounds__(kThreads) void k_combine_f64(const double *__restrict__ Y, const double *__restrict__ pos,
+                                                          const double *__restrict__ pot, const double *__restrict__ zbuf,
+                                                          int N, int cform, double *__restrict__ dC) {
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= N) return;
+    const double Z = zbuf[0];
+    switch (cform) {
+    case 0: {
+        const double x = Y[2L * i], y = Y[2L * i + 1];
+        const double *p = pot + 4L * i;
+        dC[2L * i] = pos[2L * i] - (x * p[0] - p[1]) / Z;
+        dC[2L * i + 1] = pos[2L * i + 1] - (y * p[0] - p[2]) / Z;
+        break;
+    }
+    case 1: {
+        const double x = Y[2L * i], y = Y[2L * i + 1];
+        const double *p = pot + 3L * i;
+        dC[2L * i] = pos[2L * i] - (x * p[2] - p[0]) / Z;
+        dC[2L * i + 1] = pos[2L * i + 1] - (y * p[2] - p[1]) / Z;
+        break;
+    }
+    case 2: {
+        const double y = Y[i];
+        dC[i] = pos[i] - (y * pot[3L * i] - pot[3L * i + 1]) / Z;
+        break;
+    }
+    default: {
+        const double y = Y[i];
+        dC[i] = pos[i] - (y * pot[2L * i + 1] - pot[2L * i]) / Z;
+        break;
+    }
+    }
+}
+
+// Optimiser update, FP64, exactly the reference's elementwise loops (tsne.cpp:478-514), with the
+// block partial sums of the new Y for zeroMean.
+__global__ __launch_bounds__(kThreads) void k_update_f64(double *__restrict__ Y, double *__restrict__ uY,
+                                                         double *__restrict__ gains, const double *__restrict__ dY,
+                                                         int N, int D, double momentum, double lr, double msn,
+                                                         int plain_descent, double *__restrict__ part) {
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    double ys[2] = {0, 0};
+    if (i < N) {
+        if (plain_descent) {
+            for (int d = 0; d < D; d++) {
+                Y[(long)i * D + d] = Y[(long)i * D + d] - dY[(long)i * D + d];
+                ys[d] = Y[(long)i * D + d];
+            }
+        } else {
+            double u[2], step = 0;
+            for (int d = 0; d < D; d++) {
+                const long e = (long)i * D + d;
+                const double sd = dY[e] == .0 ? .0 : (dY[e] < .0 ? -1.0 : 1.0);
+                const double su = uY[e] == .0 ? .0 : (uY[e] < .0 ? -1.0 : 1.0);
+                double gn = sd != su ? gains[e] + .2 : gains[e] * .8;
+                if (gn < .01) gn = .01;
+                gains[e] = gn;
+                u[d] = momentum * uY[e] - lr * gn * dY[e];
+                step += u[d] * u[d];
+            }
+            if (msn > 0) {
+                step = sqrt(step);
+                if (step > msn)
+                    for (int d = 0; d < D; d++) u[d] *= (msn / step);
+            }
+            for (int d = 0; d < D; d++) {
+                const long e = (long)i * D + d;
+                uY[e] = u[d];
+                Y[e] = Y[e] + u[d];
+                ys[d] = Y[e];
+            }
+        }
+    }
+    __shared__ double sh[kThreads / kWave];
+    for (int d = 0; d < D; d++) {
+        const double s = block_sum(ys[d], sh);
+        if (threadIdx.x == 0) part[blockIdx.x * D + d] = s;
+    }
+}
+
+// Split interleaved Y (FP64) into xs, ys (tsne.cpp:1042-1044).
+__global__ void k_split_f64(const double *__restrict__ Y, int N, double *__restrict__ xs, double *__restrict__ ys) {
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i < N) {
+        xs[i] = Y[2L * i];
+        ys[i] = Y[2L * i + 1];
+    }
+}
+
+// ---------------------------------------------------------------- KL divergence over CSR edges
+// C = sum_ij p_ij log((p_ij + FLT_MIN) / (q_ij / Z + FLT_MIN)), q = (1 / (1 + d^2 / df))^df
+// (tsne.cpp:1329-1355), in FP64 with a fixed reduction order (no race).
+template <typename TV, typename TI, typename TC, int D, int SW>
+__global__ __launch_bounds__(kThreads) void k_kl(const TI *__restrict__ rowP, const TC *__restrict__ colP,
+                                                 const TV *__restrict__ valP, double pscale, const TV *__restrict__ Y,
+                                                 int N, double df, const double *__restrict__ zbuf,
+                                                 double *__restrict__ part) {
+    const long gid = (long)blockIdx.x * kThreads + threadIdx.x;
+    const long row = gid / SW;
+    const int lane = (int)(gid % SW);
+    double acc = 0;
+    if (row < N) {
+        const double Z = zbuf[0];
+        TV yi[D];
+        load_pt<TV, D>(Y, row, yi);
+        for (long k = (long)rowP[row] + lane; k < (long)rowP[row + 1]; k += SW) {
+            const long j = (long)colP[k];
+            const double p = (double)valP[k] * pscale;
+            TV yj[D];
+            load_pt<TV, D>(Y, j, yj);
+            double Q = 0;
+            for (int d = 0; d < D; d++) {
+                const double df_ = (double)yi[d] - (double)yj[d];
+                Q += df_ * df_;
+            }
+            Q = (df == 1.0 ? 1.0 / (1.0 + Q) : pow(1.0 / (1.0 + Q / df), df)) / Z;
+            acc += p * log((p + (double)FLT_MIN) / (Q + (double)FLT_MIN));
+        }
+    }
+    __shared__ double sh[kThreads / kWave];
+    acc = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+// ---------------------------------------------------------------- perplexity calibration (FP64)
+// distances2similarities (tsne.cpp:1394-1469) for many points: one wavefront per point, lanes over
+// the K distances; the bisection steps are the reference's (beta doubling/halving until bracketed,
+// tol 1e-5, at most 200 evaluations).  Sums use a fixed wave reduction order.
+__global__ __launch_bounds__(kThreads) void k_d2s(const double *__restrict__ Dist, double *__restrict__ P, int n,
+                                                  int K, double perplexity, double sigma, int ifSquared,
+                                                  const double *__restrict__ beta0, double *__restrict__ beta_out,
+                                                  int *__restrict__ iters_out) {
+    const long gid = (long)blockIdx.x * kThreads + threadIdx.x;
+    const long pt = gid / kWave;
+    const int lane = threadIdx.x % kWave;
+    if (pt >= n) return;
+    const double *Dp = Dist + pt * K;
+    double *Pp = P + pt * K;
+    double beta, beta_eval, sum_P;
+    int iter = 0;
+    if (perplexity > 0) {
+        double min_beta = -DBL_MAX, max_beta = DBL_MAX;
+        const double tol = 1e-5, logp = log(perplexity);
+        beta = beta0 ? beta0[pt] : 1.0;
+        beta_eval = beta;
+        sum_P = DBL_MIN;
+        bool found = false;
+        while (!found && iter < 200) {
+            double s = 0, hs = 0;
+            for (int m = lane; m < K; m += kWave) {
+                const double d = ifSquared ? Dp[m] : Dp[m] * Dp[m];
+                const double pv = exp(-beta * d);
+                s += pv;
+                hs += beta * (d * pv);
+            }
+            s = wave_sum(s);
+            hs = wave_sum(hs);
+            beta_eval = beta;
+            sum_P = DBL_MIN + s;
+            const double H = (hs / sum_P) + log(sum_P);
+            const double Hdiff = H - logp;
+            if (Hdiff < tol && -Hdiff < tol) {
+                found = true;
+            } else if (Hdiff > 0) {
+                min_beta = beta;
+                beta = (max_beta == DBL_MAX || max_beta == -DBL_MAX) ? beta * 2.0 : (beta + max_beta) / 2.0;
+            } else {
+                max_beta = beta;
+                beta = (min_beta == -DBL_MAX || min_beta == DBL_MAX) ? beta / 2.0 : (beta + min_beta) / 2.0;
+            }
+            iter++;
+        }
+    } else {
+        beta = 1 / (2 * sigma * sigma);
+        beta_eval = beta;
+        double s = 0;
+        for (int m = lane; m < K; m += kWave) {
+            const double d = ifSquared ? Dp[m] : Dp[m] * Dp[m];
+            s += exp(-beta * d);
+        }
+        sum_P = DBL_MIN + wave_sum(s);
+    }
+    // The reference normalises the P of the last evaluated beta (tsne.cpp:1466); when the loop ends
+    // at its iteration cap, the returned beta has already moved past it.
+    for (int m = lane; m < K; m += kWave) {
+        const double d = ifSquared ? Dp[m] : Dp[m] * Dp[m];
+        Pp[m] = exp(-beta_eval * d) / sum_P;
+    }
+    if (lane == 0) {
+        beta_out[pt] = beta;
+        if (iters_out) iters_out[pt] = iter;
+    }
+}
+
+}  // namespace fitsne

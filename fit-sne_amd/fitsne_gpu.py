@@ -1,0 +1,297 @@
+"""ctypes binding of libfitsne.so (include/fitsne_abi.h) for tests, benchmarks and the Python wrapper.
+
+The GPU library is the only implementation behind these calls: if lib/libfitsne.so is missing or no
+GPU is visible, calls raise instead of falling back to anything else.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libfitsne.so")
+
+OK, E_ARG, E_ALLOC, E_HIP, E_UNSUPPORTED, E_NODEVICE = 0, -1, -2, -3, -4, -5
+KERNEL_SQUARED_CAUCHY, KERNEL_GENERAL, KERNEL_SQUARED_GENERAL = 0, 1, 2
+FP64_PLAIN, FP32 = 0, 1
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_u32 = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_i64 = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_i32 = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_ci, _cd, _cu, _vp = C.c_int, C.c_double, C.c_uint, C.c_void_p
+
+
+class FitsneError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"fitsne error {code}: {msg}")
+        self.code = code
+
+
+class StepParams(C.Structure):
+    _fields_ = [("momentum", _cd), ("learning_rate", _cd), ("max_step_norm", _cd),
+                ("plain_descent", _ci), ("reserved", _ci)]
+
+
+# every symbol include/fitsne_abi.h declares, with its ctypes signature
+SIGNATURES = {
+    "fitsne_last_error": (C.c_char_p, []),
+    "fitsne_device_count": (_ci, []),
+    "fitsne_version": (C.c_char_p, []),
+    "fitsne_compute_fft_gradient": (_ci, [_u32, _u32, _dp, _dp, _ci, _ci, _dp, _ci, _cd, _ci, _cu, _cd, C.POINTER(_cd)]),
+    "fitsne_compute_fft_gradient_i64": (_ci, [_i64, _i32, _dp, _dp, _ci, _ci, _dp, _ci, _cd, _ci, _cu, _cd, C.POINTER(_cd)]),
+    "fitsne_precompute_2d": (_ci, [_cd, _cd, _cd, _cd, _ci, _ci, _ci, _dp, _dp, _dp, _dp, _dp, _dp, _cd]),
+    "fitsne_n_body_fft_2d": (_ci, [_ci, _ci, _dp, _dp, _dp, _ci, _ci, _dp, _dp, _dp, _dp, _dp, _cu]),
+    "fitsne_precompute_1d": (_ci, [_cd, _cd, _ci, _ci, _ci, _dp, _dp, _dp, _dp, _dp, _cd]),
+    "fitsne_n_body_fft_1d": (_ci, [_ci, _ci, _dp, _dp, _ci, _ci, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "fitsne_interpolate": (_ci, [_ci, _ci, _dp, _dp, _dp]),
+    "fitsne_evaluate_error_fft": (_ci, [_u32, _u32, _dp, _dp, _ci, _ci, _cd, _cd, C.POINTER(_cd)]),
+    "fitsne_distances2similarities_batch": (_ci, [_dp, _dp, _ci, _ci, _cd, _cd, _ci, _vp, _dp, _vp]),
+    "fitsne_ctx_create": (_ci, [C.POINTER(_vp), _ci, _ci, _ci, _ci, _cd, _ci, _cd]),
+    "fitsne_ctx_destroy": (_ci, [_vp]),
+    "fitsne_set_P": (_ci, [_vp, _ci, _u32, _u32, _dp]),
+    "fitsne_set_P_i64": (_ci, [_vp, _ci, _i64, _i32, _dp]),
+    "fitsne_set_exaggeration": (_ci, [_vp, _cd]),
+    "fitsne_set_reorder": (_ci, [_vp, _ci]),
+    "fitsne_reorder": (_ci, [_vp]),
+    "fitsne_set_Y": (_ci, [_vp, _dp]),
+    "fitsne_get_Y": (_ci, [_vp, _dp]),
+    "fitsne_gradient": (_ci, [_vp, _vp, C.POINTER(_cd)]),
+    "fitsne_step": (_ci, [_vp, C.POINTER(StepParams), _vp]),
+    "fitsne_kl": (_ci, [_vp, C.POINTER(_cd)]),
+    "fitsne_sync": (_ci, [_vp]),
+    "fitsne_set_timing": (_ci, [_vp, _ci]),
+    "fitsne_kernel_stats": (_ci, [_vp, _ci, C.c_char_p, _vp, _vp]),
+    "fitsne_last_geometry": (_ci, [_vp, _dp]),
+    "fitsne_debug_read": (C.c_longlong, [_vp, C.c_char_p, _vp, C.c_longlong]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libfitsne.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FitsneError(E_NODEVICE, f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc < 0:
+        raise FitsneError(rc, lib().fitsne_last_error().decode())
+    return rc
+
+
+def device_count():
+    return lib().fitsne_device_count()
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+# ---------------------------------------------------------------- mirror tier
+
+def compute_fft_gradient(P, Y, n_interpolation_points=3, intervals_per_integer=1.0, min_num_intervals=50,
+                         df=1.0, nthreads=0):
+    """Mirror of TSNE::computeFftGradient* (FP64 plain GPU path).  P = (row_P, col_P, val_P)."""
+    row, col, val = P
+    Y = np.ascontiguousarray(Y, dtype=np.float64)
+    N = Y.shape[0]
+    D = 1 if Y.ndim == 1 else Y.shape[1]
+    dC = np.empty(N * D)
+    sq = _cd()
+    if row.dtype == np.int64:
+        check(lib().fitsne_compute_fft_gradient_i64(row, np.ascontiguousarray(col, dtype=np.int32), val,
+                                                    Y.reshape(-1), N, D, dC, n_interpolation_points,
+                                                    intervals_per_integer, min_num_intervals, nthreads, df,
+                                                    C.byref(sq)))
+    else:
+        check(lib().fitsne_compute_fft_gradient(row, col, val, Y.reshape(-1), N, D, dC, n_interpolation_points,
+                                                intervals_per_integer, min_num_intervals, nthreads, df,
+                                                C.byref(sq)))
+    return dC.reshape(N, D), sq.value
+
+
+def precompute_2d(xmax, xmin, nb, p, kernel, df=1.0):
+    n1d = nb * p
+    M = 2 * n1d
+    lo = np.empty(2 * nb * nb)
+    hi = np.empty(2 * nb * nb)
+    sp = np.empty(p)
+    yt = np.empty(n1d)
+    xt = np.empty(n1d)
+    kt = np.empty(2 * M * (M // 2 + 1))
+    check(lib().fitsne_precompute_2d(xmax, xmin, xmax, xmin, nb, p, kernel, lo, hi, sp, yt, xt, kt, df))
+    return dict(lo=lo, hi=hi, sp=sp, yt=yt, xt=xt, kt=kt)
+
+
+def n_body_fft_2d(xs, ys, charges, nb, p, geo, potential=None):
+    xs = np.ascontiguousarray(xs, dtype=np.float64)
+    ys = np.ascontiguousarray(ys, dtype=np.float64)
+    ch = np.ascontiguousarray(charges, dtype=np.float64)
+    N, nt = ch.shape
+    pot = np.zeros((N, nt)) if potential is None else np.ascontiguousarray(potential, dtype=np.float64).copy()
+    check(lib().fitsne_n_body_fft_2d(N, nt, xs, ys, ch.reshape(-1), nb, p, geo["lo"], geo["hi"], geo["sp"],
+                                     geo["kt"], pot.reshape(-1), 0))
+    return pot
+
+
+def precompute_1d(ymin, ymax, nb, p, kernel, df=1.0):
+    lo = np.empty(nb)
+    hi = np.empty(nb)
+    sp = np.empty(p)
+    yt = np.empty(nb * p)
+    kv = np.empty(4 * nb * p)
+    check(lib().fitsne_precompute_1d(ymin, ymax, nb, p, kernel, lo, hi, sp, yt, kv, df))
+    return dict(lo=lo, hi=hi, sp=sp, yt=yt, kv=kv)
+
+
+def n_body_fft_1d(Y, charges, nb, p, geo, potential=None):
+    Y = np.ascontiguousarray(Y, dtype=np.float64)
+    ch = np.ascontiguousarray(charges, dtype=np.float64)
+    N, nt = ch.shape
+    pot = np.zeros((N, nt)) if potential is None else np.ascontiguousarray(potential, dtype=np.float64).copy()
+    check(lib().fitsne_n_body_fft_1d(N, nt, Y, ch.reshape(-1), nb, p, geo["lo"], geo["hi"], geo["sp"], geo["yt"],
+                                     geo["kv"], pot.reshape(-1)))
+    return pot
+
+
+def interpolate(p, y_in_box, spacings):
+    y = np.ascontiguousarray(y_in_box, dtype=np.float64)
+    out = np.empty(p * y.size)
+    check(lib().fitsne_interpolate(p, y.size, y, np.ascontiguousarray(spacings, dtype=np.float64), out))
+    return out.reshape(p, y.size)
+
+
+def evaluate_error_fft(P, Y, sum_Q, df=1.0):
+    row, col, val = P
+    Y = np.ascontiguousarray(Y, dtype=np.float64)
+    N = Y.shape[0]
+    D = 1 if Y.ndim == 1 else Y.shape[1]
+    out = _cd()
+    check(lib().fitsne_evaluate_error_fft(row, col, val, Y.reshape(-1), N, D, sum_Q, df, C.byref(out)))
+    return out.value
+
+
+def distances2similarities(Dist, perplexity, sigma=-1.0, squared=False, beta0=None):
+    Dist = np.ascontiguousarray(Dist, dtype=np.float64)
+    n, K = Dist.shape
+    P = np.empty((n, K))
+    beta = np.empty(n)
+    iters = np.empty(n, dtype=np.int32)
+    b0 = None if beta0 is None else np.ascontiguousarray(beta0, dtype=np.float64)
+    check(lib().fitsne_distances2similarities_batch(Dist.reshape(-1), P.reshape(-1), n, K, perplexity, sigma,
+                                                    int(squared), _ptr(b0), beta, _ptr(iters)))
+    return P, beta, iters
+
+
+# ---------------------------------------------------------------- resident tier
+
+class Context:
+    """Device-resident P, Y, momentum and gains (one optimisation iteration moves no data over PCIe)."""
+
+    def __init__(self, no_dims=2, precision=FP32, n_interpolation_points=3, intervals_per_integer=1.0,
+                 min_num_intervals=50, df=1.0, device=0):
+        h = _vp()
+        check(lib().fitsne_ctx_create(C.byref(h), device, precision, no_dims, n_interpolation_points,
+                                      intervals_per_integer, min_num_intervals, df))
+        self.h = h
+        self.D = no_dims
+        self.N = 0
+        self.precision = precision
+
+    def close(self):
+        if self.h:
+            lib().fitsne_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_P(self, row, col, val):
+        N = row.size - 1
+        val = np.ascontiguousarray(val, dtype=np.float64)
+        if row.dtype == np.int64:
+            check(lib().fitsne_set_P_i64(self.h, N, row, np.ascontiguousarray(col, dtype=np.int32), val))
+        else:
+            check(lib().fitsne_set_P(self.h, N, np.ascontiguousarray(row, dtype=np.uint32),
+                                     np.ascontiguousarray(col, dtype=np.uint32), val))
+        self.N = N
+
+    def set_exaggeration(self, f):
+        check(lib().fitsne_set_exaggeration(self.h, f))
+
+    def set_reorder(self, enable):
+        check(lib().fitsne_set_reorder(self.h, int(enable)))
+
+    def reorder(self):
+        check(lib().fitsne_reorder(self.h))
+
+    def set_Y(self, Y):
+        check(lib().fitsne_set_Y(self.h, np.ascontiguousarray(Y, dtype=np.float64).reshape(-1)))
+
+    def get_Y(self):
+        out = np.empty(self.N * self.D)
+        check(lib().fitsne_get_Y(self.h, out))
+        return out.reshape(self.N, self.D)
+
+    def gradient(self, download=True):
+        sq = _cd()
+        dC = np.empty(self.N * self.D) if download else None
+        check(lib().fitsne_gradient(self.h, _ptr(dC), C.byref(sq)))
+        return (dC.reshape(self.N, self.D) if download else None), sq.value
+
+    def step(self, momentum, learning_rate, max_step_norm=5.0, plain_descent=False, want_sumq=False):
+        prm = StepParams(momentum, learning_rate, max_step_norm, int(plain_descent), 0)
+        if want_sumq:
+            sq = _cd()
+            check(lib().fitsne_step(self.h, C.byref(prm), C.byref(sq)))
+            return sq.value
+        check(lib().fitsne_step(self.h, C.byref(prm), None))
+        return None
+
+    def kl(self):
+        out = _cd()
+        check(lib().fitsne_kl(self.h, C.byref(out)))
+        return out.value
+
+    def sync(self):
+        check(lib().fitsne_sync(self.h))
+
+    def set_timing(self, on):
+        check(lib().fitsne_set_timing(self.h, int(on)))
+
+    def kernel_stats(self, cap=64):
+        names = C.create_string_buffer(48 * cap)
+        counts = np.zeros(cap, dtype=np.int32)
+        ms = np.zeros(cap)
+        n = check(lib().fitsne_kernel_stats(self.h, cap, names, _ptr(counts), _ptr(ms)))
+        raw = names.raw
+        out = {}
+        for i in range(n):
+            nm = raw[48 * i:48 * (i + 1)].split(b"\0", 1)[0].decode()
+            out[nm] = (int(counts[i]), float(ms[i]))
+        return out
+
+    def debug_read(self, name, count, dtype):
+        out = np.zeros(count, dtype=dtype)
+        n = check(lib().fitsne_debug_read(self.h, name.encode(), _ptr(out), out.nbytes))
+        return out[: n // out.itemsize]
+
+    def last_geometry(self):
+        out = np.zeros(6)
+        check(lib().fitsne_last_geometry(self.h, out))
+        return dict(n_boxes=int(out[0]), n1d=int(out[1]), fft_len=int(out[2]), min=out[3], max=out[4],
+                    nnz=int(out[5]))

@@ -1,0 +1,90 @@
+"""Synthetic workloads of the benchmark configurations (seeded, generated in place; nothing large ships).
+
+c2_workload: N = 1,000,000 points in D = 50, 50 Gaussian clusters with Dirichlet(1) weights, centres
+N(0, 6^2 I), sd 1.  Affinities: each point draws 45 random same-cluster neighbours, weights
+exp(-d^2 / median_row(d^2)) normalised per row, then P = (P + P^T) / 2 normalised to sum 1 (about 90
+non-zeros per row, int64 row offsets, int32 columns, float64 values).  Embedding: 2D blobs, one per
+cluster, centres uniform in [-70, 70]^2, sd 3, zero mean (span about [-80, 80]).
+"""
+import sys
+import time
+
+import numpy as np
+
+
+def _log(msg):
+    print(f"[synth {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
+def symmetrize_normalize(N, rows, cols, vals):
+    """(P + P^T) / 2 as CSR (duplicates summed), then val /= sum(val)."""
+    import scipy.sparse as sp
+
+    A = sp.csr_matrix((vals, (rows, cols)), shape=(N, N))
+    A = (A + A.T).tocsr()
+    A.sum_duplicates()
+    A.sort_indices()
+    val = A.data.astype(np.float64) * 0.5
+    val /= val.sum()
+    return A.indptr.astype(np.int64), A.indices.astype(np.int32), val
+
+
+def cluster_affinities(X, labels, per_point, rng, chunk=1 << 20):
+    N = X.shape[0]
+    k = int(labels.max()) + 1
+    members = [np.flatnonzero(labels == c) for c in range(k)]
+    nbr = np.empty((N, per_point), dtype=np.int64)
+    for c in range(k):
+        idx = members[c]
+        if idx.size < 2:
+            nbr[idx] = idx[:, None]
+            continue
+        pick = rng.integers(0, idx.size - 1, size=(idx.size, per_point))
+        self_pos = np.arange(idx.size)[:, None]
+        pick = pick + (pick >= self_pos)  # skip self
+        nbr[idx] = idx[pick]
+    rows = np.repeat(np.arange(N, dtype=np.int64), per_point)
+    cols = nbr.reshape(-1)
+    d2 = np.empty(rows.size, dtype=np.float32)
+    for s in range(0, rows.size, chunk):
+        e = min(rows.size, s + chunk)
+        diff = X[rows[s:e]] - X[cols[s:e]]
+        d2[s:e] = np.einsum("ij,ij->i", diff, diff)
+    d2 = d2.reshape(N, per_point).astype(np.float64)
+    med = np.median(d2, axis=1, keepdims=True)
+    w = np.exp(-d2 / np.maximum(med, 1e-12))
+    w /= w.sum(axis=1, keepdims=True)
+    return rows, cols, w.reshape(-1)
+
+
+def c2_workload(N=1_000_000, D=50, k=50, per_point=45, seed=20261204):
+    rng = np.random.default_rng(seed)
+    t0 = time.time()
+    weights = rng.dirichlet(np.ones(k))
+    labels = rng.choice(k, size=N, p=weights)
+    centres = rng.normal(0.0, 6.0, size=(k, D)).astype(np.float32)
+    X = centres[labels] + rng.normal(0.0, 1.0, size=(N, D)).astype(np.float32)
+    _log(f"C2 data {N}x{D} in {time.time() - t0:.1f}s")
+    rows, cols, vals = cluster_affinities(X, labels, per_point, rng)
+    del X
+    _log(f"C2 kNN-style affinities ({rows.size} directed edges) in {time.time() - t0:.1f}s")
+    row_P, col_P, val_P = symmetrize_normalize(N, rows, cols, vals)
+    _log(f"C2 symmetric P nnz={val_P.size} ({val_P.size / N:.1f}/row) in {time.time() - t0:.1f}s")
+    c2 = rng.uniform(-70.0, 70.0, size=(k, 2))
+    Y = c2[labels] + rng.normal(0.0, 3.0, size=(N, 2))
+    Y -= Y.mean(axis=0)
+    return (row_P, col_P, val_P), Y, labels
+
+
+def small_mixture_workload(N=10_000, D=2, k=10, per_point=30, span=40.0, seed=7):
+    """A scaled-down C2-like workload for quick checks (same construction)."""
+    rng = np.random.default_rng(seed)
+    weights = rng.dirichlet(np.ones(k))
+    labels = rng.choice(k, size=N, p=weights)
+    X = (rng.normal(0.0, 6.0, size=(k, 50))[labels] + rng.normal(0.0, 1.0, size=(N, 50))).astype(np.float32)
+    rows, cols, vals = cluster_affinities(X, labels, per_point, rng)
+    P = symmetrize_normalize(N, rows, cols, vals)
+    cen = rng.uniform(-span / 2, span / 2, size=(k, D))
+    Y = cen[labels] + rng.normal(0.0, span / 40, size=(N, D))
+    Y -= Y.mean(axis=0)
+    return P, Y, labels

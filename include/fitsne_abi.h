@@ -1,0 +1,192 @@
+/*
+ * fitsne_abi.h -- C-ABI of the MI355X-native FIt-SNE gradient library (libfitsne.so).
+ *
+ * Plain pointers and sizes only.  Every call returns an int status (FITSNE_OK or a negative code)
+ * where the reference returns void; out-parameters carry what the reference returned or stored in
+ * TSNE members.  Two tiers:
+ *
+ *  1. Mirror tier -- host pointers, FP64 in and out, synchronous, same parameter order and meaning
+ *     as the reference functions named beside each entry point (FIt-SNE 1.2.1).  Runs the FP64
+ *     "plain" GPU path, whose stages mirror the reference's.
+ *  2. Resident tier -- a context owning device-resident P, Y, momentum and gains, so that one
+ *     optimisation iteration moves no data over PCIe.  Offers the FP64 plain path and the FP32
+ *     form (FP32 spread/FFT/interpolation, FP64 normalisation sum Z and grid geometry).
+ *
+ * The library is not re-entrant on one context; distinct contexts may be used from distinct
+ * host threads.  No host array is retained after a call returns.
+ */
+#ifndef FITSNE_ABI_H
+#define FITSNE_ABI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define FITSNE_OK 0
+#define FITSNE_E_ARG (-1)         /* invalid argument (sizes, dims, null pointers)          */
+#define FITSNE_E_ALLOC (-2)       /* device or host allocation failed                        */
+#define FITSNE_E_HIP (-3)         /* a HIP runtime call failed (message: fitsne_last_error)  */
+#define FITSNE_E_UNSUPPORTED (-4) /* valid for the reference but outside this build's limits */
+#define FITSNE_E_NODEVICE (-5)    /* no usable GPU                                            */
+
+/* kernel ids for the precompute calls: the reference passes function pointers
+ * (nbodyfft.h:13-15) to squared_cauchy[_2d], general_kernel[_2d], squared_general_kernel[_2d]
+ * (tsne.cpp:69-94). */
+#define FITSNE_KERNEL_SQUARED_CAUCHY 0
+#define FITSNE_KERNEL_GENERAL 1
+#define FITSNE_KERNEL_SQUARED_GENERAL 2
+
+/* precisions of the resident tier */
+#define FITSNE_FP64_PLAIN 0 /* FP64 throughout, stages as in the reference               */
+#define FITSNE_FP32 1       /* FP32 grid/FFT/interpolation, FP64 Z/geometry (product form) */
+
+/* Text of the last error on this thread (never NULL). */
+const char *fitsne_last_error(void);
+/* Number of visible GPUs (0 when none); no device is initialised beyond enumeration. */
+int fitsne_device_count(void);
+/* Library version string ("fitsne-mi355x <semver> (reference 1.2.1)"). */
+const char *fitsne_version(void);
+
+/* ====================================================================================== */
+/* Mirror tier                                                                             */
+/* ====================================================================================== */
+
+/* Replaces TSNE::computeFftGradient / computeFftGradientVariableDf / computeFftGradientOneD /
+ * computeFftGradientOneDVariableDf (tsne.cpp:1027, :860, :758, :647; declared tsne.h:71-94), with the
+ * dispatch of TSNE::run (tsne.cpp:444-464): D in {1,2}, df == 1 or not.  The unused dense P argument
+ * of the reference is dropped.  dC[N*D] is overwritten; *sum_Q receives what the reference stores
+ * in TSNE::current_sum_Q (tsne.h:63).  nthreads is accepted for signature parity and ignored. */
+int fitsne_compute_fft_gradient(const uint32_t *row_P, const uint32_t *col_P, const double *val_P,
+                                const double *Y, int N, int D, double *dC, int n_interpolation_points,
+                                double intervals_per_integer, int min_num_intervals, unsigned nthreads,
+                                double df, double *sum_Q);
+/* Same with 64-bit row offsets and 32-bit columns (for nnz >= 2^32 inputs). */
+int fitsne_compute_fft_gradient_i64(const int64_t *row_P, const int32_t *col_P, const double *val_P,
+                                    const double *Y, int N, int D, double *dC, int n_interpolation_points,
+                                    double intervals_per_integer, int min_num_intervals, unsigned nthreads,
+                                    double df, double *sum_Q);
+
+/* Replaces precompute_2d (nbodyfft.cpp:9-70, nbodyfft.h:17-19).  Box bounds, node spacings and node
+ * coordinates are written exactly as the reference lays them out.  fft_kernel_tilde (caller-sized
+ * for the reference: 2n x (n+1) complex doubles, n = n_boxes * n_interpolation_points) receives this
+ * library's kernel spectrum, which n_body_fft_2d consumes: like the reference, callers treat it as
+ * opaque between the two calls. */
+int fitsne_precompute_2d(double x_max, double x_min, double y_max, double y_min, int n_boxes,
+                         int n_interpolation_points, int kernel, double *box_lower_bounds,
+                         double *box_upper_bounds, double *y_tilde_spacings, double *y_tilde,
+                         double *x_tilde, double *fft_kernel_tilde, double df);
+
+/* Replaces n_body_fft_2d (nbodyfft.cpp:73-248, nbodyfft.h:21-23).  potentialQij[N*n_terms] is
+ * ACCUMULATED into (+=) as in the reference; n_terms in [1, 4]. */
+int fitsne_n_body_fft_2d(int N, int n_terms, const double *xs, const double *ys, const double *chargesQij,
+                         int n_boxes, int n_interpolation_points, const double *box_lower_bounds,
+                         const double *box_upper_bounds, const double *y_tilde_spacings,
+                         const double *fft_kernel_tilde, double *potentialQij, unsigned nthreads);
+
+/* Replaces precompute (nbodyfft.cpp:251-307, nbodyfft.h:25-27); fft_kernel_vector (2n complex
+ * doubles in the reference) receives this library's spectrum, consumed by n_body_fft_1d. */
+int fitsne_precompute_1d(double y_min, double y_max, int n_boxes, int n_interpolation_points, int kernel,
+                         double *box_lower_bounds, double *box_upper_bounds, double *y_tilde_spacing,
+                         double *y_tilde, double *fft_kernel_vector, double df);
+
+/* Replaces nbodyfft (nbodyfft.cpp:339-455, nbodyfft.h:29-31); potentialsQij accumulated (+=). */
+int fitsne_n_body_fft_1d(int N, int n_terms, const double *Y, const double *chargesQij, int n_boxes,
+                         int n_interpolation_points, const double *box_lower_bounds,
+                         const double *box_upper_bounds, const double *y_tilde_spacings,
+                         const double *y_tilde, const double *fft_kernel_vector, double *potentialsQij);
+
+/* Replaces interpolate (nbodyfft.cpp:310-336, nbodyfft.h:33-34): out[p*N], SoA (j*N + i). */
+int fitsne_interpolate(int n_interpolation_points, int N, const double *y_in_box, const double *y_tilde_spacings,
+                       double *interpolated_values);
+
+/* Replaces TSNE::evaluateErrorFft (tsne.cpp:1329-1355; tsne.h:102) given current_sum_Q, without the
+ * reference's unsynchronised `C += temp` (tsne.cpp:1349): deterministic FP64 reduction. */
+int fitsne_evaluate_error_fft(const uint32_t *row_P, const uint32_t *col_P, const double *val_P,
+                              const double *Y, int N, int D, double sum_Q, double df, double *C);
+
+/* Replaces TSNE::distances2similarities (tsne.cpp:1394-1469; tsne.h:105) for a batch of points:
+ * D[n_points*K] distances (row i = point i), P[n_points*K] row-normalised affinities, beta[n_points].
+ * beta0 (nullable) gives per-point starting betas for the bisection (the reference uses 1.0);
+ * iters (nullable) receives the reference's loop counter per point.  perplexity <= 0 selects the
+ * fixed-sigma branch.  The FP64 bisection is the reference's, step for step. */
+int fitsne_distances2similarities_batch(const double *D, double *P, int n_points, int K, double perplexity,
+                                        double sigma, int ifSquared, const double *beta0, double *beta,
+                                        int *iters);
+
+/* ====================================================================================== */
+/* Resident tier                                                                           */
+/* ====================================================================================== */
+
+typedef struct fitsne_ctx fitsne_ctx;
+
+/* Optimiser parameters of one iteration (tsne.cpp:478-514). */
+typedef struct {
+    double momentum;          /* momentum of this iteration                                      */
+    double learning_rate;     /* learning_rate                                                    */
+    double max_step_norm;     /* clip per-point step norm when > 0 (tsne.cpp:498-511)             */
+    int plain_descent;        /* 1: Y -= dY without gains/momentum (no_momentum_during_exag phase) */
+    int reserved;
+} fitsne_step_params;
+
+/* Create a context on `device` for an embedding of dimension no_dims (1 or 2) with the FFT settings
+ * of the run (n_interpolation_points = the wrapper's nterms, intervals_per_integer,
+ * min_num_intervals) and the kernel's degree of freedom df. */
+int fitsne_ctx_create(fitsne_ctx **ctx, int device, int precision, int no_dims, int n_interpolation_points,
+                      double intervals_per_integer, int min_num_intervals, double df);
+int fitsne_ctx_destroy(fitsne_ctx *ctx);
+
+/* Upload the symmetric affinities in CSR form (tsne.cpp:327-330 output).  N may differ from a
+ * previous call.  The FP32 context stages val_P in FP32 once. */
+int fitsne_set_P(fitsne_ctx *ctx, int N, const uint32_t *row_P, const uint32_t *col_P, const double *val_P);
+int fitsne_set_P_i64(fitsne_ctx *ctx, int N, const int64_t *row_P, const int32_t *col_P, const double *val_P);
+/* Scalar multiplier on P (early/late exaggeration, tsne.cpp:389-412, :534-543).  The FP64 plain
+ * context rescales its val_P copy in place like the reference; the FP32 context keeps a scalar. */
+int fitsne_set_exaggeration(fitsne_ctx *ctx, double factor);
+
+/* Spatial order (FP32 context): points are kept internally in the Morton order of their cell on a
+ * 64x64 lattice over the embedding, so that neighbour gathers and the spread stay in cache.  All host
+ * arrays remain in the caller's order.  fitsne_set_Y reorders when enabled (default) and N >= 65536;
+ * fitsne_reorder re-sorts now (e.g. once clusters have formed).  No reference counterpart. */
+int fitsne_set_reorder(fitsne_ctx *ctx, int enable);
+int fitsne_reorder(fitsne_ctx *ctx);
+
+/* Upload Y[N*no_dims] (row-major, as the reference) and reset momentum uY = 0 and gains = 1. */
+int fitsne_set_Y(fitsne_ctx *ctx, const double *Y);
+int fitsne_get_Y(fitsne_ctx *ctx, double *Y);
+
+/* Gradient at the resident Y (no update).  dC[N*no_dims] (nullable) receives it; sum_Q (nullable)
+ * receives current_sum_Q. */
+int fitsne_gradient(fitsne_ctx *ctx, double *dC, double *sum_Q);
+
+/* One iteration of TSNE::run's hot loop (tsne.cpp:440-531): gradient, gains/momentum/clip update,
+ * Y += uY and zeroMean(Y).  Asynchronous on the context's stream; sum_Q (nullable) forces a sync. */
+int fitsne_step(fitsne_ctx *ctx, const fitsne_step_params *prm, double *sum_Q);
+
+/* KL divergence at the resident Y with the Z of the last gradient (tsne.cpp:547-571 semantics:
+ * stale Z, FLT_MIN guards, exaggerated P); the exaggeration correction is the caller's. */
+int fitsne_kl(fitsne_ctx *ctx, double *C);
+
+/* Wait for the context's stream. */
+int fitsne_sync(fitsne_ctx *ctx);
+
+/* Per-kernel timing: when enabled, each kernel launch on the context's stream is bracketed by
+ * HIP events; fitsne_kernel_stats returns, for up to `cap` kernels, their names (NUL-separated in
+ * names[cap*48]), launch counts and total milliseconds since the last reset. */
+int fitsne_set_timing(fitsne_ctx *ctx, int enable);
+int fitsne_kernel_stats(fitsne_ctx *ctx, int cap, char *names, int *counts, double *total_ms);
+/* Geometry of the last gradient: out[0..5] = n_boxes_per_dim, n1d, fft_len, min_coord, max_coord,
+ * nnz of P. */
+int fitsne_last_geometry(fitsne_ctx *ctx, double *out);
+
+/* Diagnostics (no reference counterpart): copy up to `bytes` of an internal device buffer of the last
+ * gradient into dst; returns the number of bytes copied.  Names: "grid_in", "grid_out", "kq", "k1",
+ * "ks", "kt", "zpart", "zbuf", "pos", "dC".  Layouts are those documented in the kernels. */
+long long fitsne_debug_read(fitsne_ctx *ctx, const char *name, void *dst, long long bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
