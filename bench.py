@@ -96,12 +96,14 @@ def main():
     if args.copy_calibration:
         import torch
 
-        a = torch.ones(64 * 2 ** 20 // 4, dtype=torch.float32, device="cuda")
-        b = torch.empty_like(a)
+        # source written first and then pushed out of L2 and the MALL by 448 MiB of later writes
+        src = torch.ones(512 * 2 ** 20 // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty(64 * 2 ** 20 // 4, dtype=torch.float32, device="cuda")
         torch.cuda.synchronize()
-        b.copy_(a)
+        dst.copy_(src[: dst.numel()])
         torch.cuda.synchronize()
-        log("copy calibration: 64 MiB read + 64 MiB written by one copy kernel")
+        del src, dst
+        log("copy calibration: one copy kernel reading 64 MiB (cold) and writing 64 MiB")
 
     N = args.n
     log(f"generating the C2 workload N={N}")

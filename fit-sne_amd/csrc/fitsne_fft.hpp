@@ -1,9 +1,11 @@
-// fitsne_fft.hpp -- LDS-resident Stockham FFT for one workgroup (radices 4, 2, 3), gfx950.
+// fitsne_fft.hpp -- LDS-resident Stockham FFT for one workgroup, gfx950.
 //
 // A workgroup transforms `lines` complex lines of length n held in LDS (line l at buf + l*stride).
-// Each stage of radix R reads all its inputs into registers, synchronises, then writes the
-// butterflies' outputs in place (Stockham autosort, natural order in and out):
-//   in  [j + r n/R],  twiddle W_{Ns R}^{(j mod Ns) r},  DFT_R,  out [(j - j mod Ns) R + j mod Ns + r Ns]
+// n = 2^a 3^b is split into stages of radix 16, 8, 4, 2, 9 or 3; each radix-R butterfly is done in
+// registers (R = 8, 16 by even/odd splitting down to radix 2; R = 9 as 3 x 3), so a length-1152
+// line takes three stages.  Each stage reads all its inputs into registers, synchronises, then writes
+// its outputs in place (Stockham autosort, natural order in and out):
+//   in [j + r n/R],  twiddle W_{Ns R}^{(j mod Ns) r},  DFT_R,  out [(j - j mod Ns) R + j mod Ns + r Ns]
 // Twiddles come from a table tw[e] = exp(-2 pi i e / n) built in FP64 on the device.
 // Unnormalised; `inv` selects exp(+2 pi i ...).
 #pragma once
@@ -11,6 +13,52 @@
 #include "fitsne_common.hpp"
 
 namespace fitsne {
+
+// exp(-2 pi i k / R) for the in-register butterflies (k < R / 2 for R = 8, 16; k < 9 for R = 9)
+__device__ __forceinline__ double twc(int R, int k) {
+    // cos(2 pi k / R)
+    constexpr double c16[8] = {1.0, 0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173,
+                               0.0, -0.38268343236508977173, -0.70710678118654752440, -0.92387953251128675613};
+    constexpr double c9[9] = {1.0, 0.76604444311897803520, 0.17364817766693034885, -0.5, -0.93969262078590838405,
+                              -0.93969262078590838405, -0.5, 0.17364817766693034885, 0.76604444311897803520};
+    return R == 9 ? c9[k] : c16[k * (16 / R)];
+}
+__device__ __forceinline__ double tws(int R, int k) {
+    // sin(2 pi k / R)
+    constexpr double s16[8] = {0.0, 0.38268343236508977173, 0.70710678118654752440, 0.92387953251128675613,
+                               1.0, 0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173};
+    constexpr double s9[9] = {0.0, 0.64278760968653932632, 0.98480775301220805936, 0.86602540378443864676,
+                              0.34202014332566873304, -0.34202014332566873304, -0.86602540378443864676,
+                              -0.98480775301220805936, -0.64278760968653932632};
+    return R == 9 ? s9[k] : s16[k * (16 / R)];
+}
+
+// multiply by W_R^k (forward: exp(-2 pi i k/R); inverse: conjugate)
+template <int R, int K, typename T>
+__device__ __forceinline__ cx<T> rot(cx<T> a, bool inv) {
+    if constexpr (K == 0) {
+        return a;
+    } else if constexpr (R == 4 * K) {  // W = -i (forward)
+        return inv ? mul_i(a) : mul_negi(a);
+    } else {
+        const T c = (T)twc(R, K), s = (T)tws(R, K);
+        const T si = inv ? s : -s;
+        return cx<T>{a.x * c - a.y * si, a.x * si + a.y * c};
+    }
+}
+
+template <int R, typename T>
+__device__ __forceinline__ void dft_small(cx<T> (&v)[R], bool inv);
+
+template <int H, int K, typename T>
+__device__ __forceinline__ void combine_step(cx<T> (&v)[2 * H], const cx<T> (&e)[H], const cx<T> (&o)[H], bool inv) {
+    if constexpr (K < H) {
+        const cx<T> t = rot<2 * H, K, T>(o[K], inv);
+        v[K] = e[K] + t;
+        v[K + H] = e[K] - t;
+        combine_step<H, K + 1, T>(v, e, o, inv);
+    }
+}
 
 template <int R, typename T>
 __device__ __forceinline__ void dft_small(cx<T> (&v)[R], bool inv) {
@@ -26,15 +74,46 @@ __device__ __forceinline__ void dft_small(cx<T> (&v)[R], bool inv) {
         v[1] = a1 + jb1;
         v[2] = a0 - b0;
         v[3] = a1 - jb1;
-    } else {  // R == 3
+    } else if constexpr (R == 3) {
         const T s3 = (T)0.86602540378443864676372317075294;
         cx<T> t1 = v[1] + v[2], t2 = v[1] - v[2];
         cx<T> m = v[0] - (T)0.5 * t1;
         cx<T> s = s3 * t2;
-        cx<T> rot = inv ? mul_i(s) : mul_negi(s);
+        cx<T> rt = inv ? mul_i(s) : mul_negi(s);
         v[0] = v[0] + t1;
-        v[1] = m + rot;
-        v[2] = m - rot;
+        v[1] = m + rt;
+        v[2] = m - rt;
+    } else if constexpr (R == 8 || R == 16) {
+        constexpr int H = R / 2;
+        cx<T> e[H], o[H];
+#pragma unroll
+        for (int k = 0; k < H; k++) {
+            e[k] = v[2 * k];
+            o[k] = v[2 * k + 1];
+        }
+        dft_small<H, T>(e, inv);
+        dft_small<H, T>(o, inv);
+        combine_step<H, 0, T>(v, e, o, inv);
+    } else {  // R == 9: three DFT3 over m (index 3m + r), twiddle W9^{r k}, three DFT3 over r
+        cx<T> a[3][3];
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+#pragma unroll
+            for (int m = 0; m < 3; m++) a[r][m] = v[3 * m + r];
+            dft_small<3, T>(a[r], inv);
+        }
+        // a[r][k] now holds sum_m v[3m+r] W3^{mk}; twiddle by W9^{r k}
+        a[1][1] = rot<9, 1, T>(a[1][1], inv);
+        a[1][2] = rot<9, 2, T>(a[1][2], inv);
+        a[2][1] = rot<9, 2, T>(a[2][1], inv);
+        a[2][2] = rot<9, 4, T>(a[2][2], inv);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            cx<T> b[3] = {a[0][k], a[1][k], a[2][k]};
+            dft_small<3, T>(b, inv);
+#pragma unroll
+            for (int s = 0; s < 3; s++) v[k + 3 * s] = b[s];
+        }
     }
 }
 
@@ -83,15 +162,23 @@ __device__ __forceinline__ void fft_stage(cx<T> *buf, int n, int lines, int stri
 template <typename T, int NTHR, int MAXTOT>
 __device__ void fft_lines(cx<T> *buf, const FftPlan &plan, int lines, int stride, const cx<T> *__restrict__ tw,
                           bool inv) {
+    constexpr int P16 = (MAXTOT / 16 + NTHR - 1) / NTHR;
+    constexpr int P8 = (MAXTOT / 8 + NTHR - 1) / NTHR;
     constexpr int P4 = (MAXTOT / 4 + NTHR - 1) / NTHR;
     constexpr int P2 = (MAXTOT / 2 + NTHR - 1) / NTHR;
+    constexpr int P9 = (MAXTOT / 9 + NTHR - 1) / NTHR;
     constexpr int P3 = (MAXTOT / 3 + NTHR - 1) / NTHR;
     int Ns = 1;
     for (int s = 0; s < plan.nst; s++) {
         const int R = plan.rad[s];
-        if (R == 4) fft_stage<T, 4, NTHR, P4>(buf, plan.n, lines, stride, Ns, tw, inv);
-        else if (R == 2) fft_stage<T, 2, NTHR, P2>(buf, plan.n, lines, stride, Ns, tw, inv);
-        else fft_stage<T, 3, NTHR, P3>(buf, plan.n, lines, stride, Ns, tw, inv);
+        switch (R) {
+        case 16: fft_stage<T, 16, NTHR, P16>(buf, plan.n, lines, stride, Ns, tw, inv); break;
+        case 8: fft_stage<T, 8, NTHR, P8>(buf, plan.n, lines, stride, Ns, tw, inv); break;
+        case 4: fft_stage<T, 4, NTHR, P4>(buf, plan.n, lines, stride, Ns, tw, inv); break;
+        case 2: fft_stage<T, 2, NTHR, P2>(buf, plan.n, lines, stride, Ns, tw, inv); break;
+        case 9: fft_stage<T, 9, NTHR, P9>(buf, plan.n, lines, stride, Ns, tw, inv); break;
+        default: fft_stage<T, 3, NTHR, P3>(buf, plan.n, lines, stride, Ns, tw, inv); break;
+        }
         Ns *= R;
     }
 }

@@ -43,6 +43,13 @@ static int fail(int code, const std::string &msg) {
 constexpr int kMaxTot32 = 8192;   // complex FP32 elements per FFT block (64 KB LDS)
 constexpr int kMaxTot64 = 4096;   // complex FP64 elements per FFT block (64 KB LDS)
 constexpr int kConv1dThreads = 1024;
+// 2D FFT kernels: threads per block and the bound on (lines x length) each is compiled for; the FFT
+// length of a 2D grid is limited to kMaxLen2d (n1d <= 1024 nodes per dimension).
+constexpr int kMaxLen2d = 2048;
+constexpr int kKernThreads = 256, kKernTot = kMaxLen2d;        // kernel spectra: 1 line per block
+constexpr int kRowThreads = 512, kRowTot = 2 * kMaxLen2d;      // row passes: 2 packed lines
+constexpr int kColThreads = 1024, kColTot32 = 4 * kMaxLen2d;   // FP32 column pass: 4 lines
+constexpr int kColThreads64 = 512, kColTot64 = 2 * kMaxLen2d;  // FP64 plain column pass
 
 // ============================================================================ device buffers
 
@@ -86,12 +93,16 @@ static int fft_size(int n) {
     return (int)best;
 }
 
+// Stage radices: 16, 8, 4, 2 for the power of two, then 9, 3 (fewest barrier-separated stages).
 static FftPlan make_plan(int n) {
     FftPlan pl{};
     pl.n = n;
     int m = n;
-    while (m % 4 == 0) { pl.rad[pl.nst++] = 4; m /= 4; }
-    while (m % 2 == 0) { pl.rad[pl.nst++] = 2; m /= 2; }
+    while (m % 16 == 0) { pl.rad[pl.nst++] = 16; m /= 16; }
+    if (m % 8 == 0) { pl.rad[pl.nst++] = 8; m /= 8; }
+    if (m % 4 == 0) { pl.rad[pl.nst++] = 4; m /= 4; }
+    if (m % 2 == 0) { pl.rad[pl.nst++] = 2; m /= 2; }
+    while (m % 9 == 0) { pl.rad[pl.nst++] = 9; m /= 9; }
     while (m % 3 == 0) { pl.rad[pl.nst++] = 3; m /= 3; }
     return pl;  // m == 1 by construction of fft_size
 }
@@ -169,7 +180,7 @@ struct fitsne_ctx {
     // sort (FP32)
     DBuf boxof, rank, count, start, sorted;
     // grids, spectra
-    DBuf gin, gout, spec, ospec, Trow, KQ, K1, KS, KT, zpart, zbuf, part, mean, mmpart, mm;
+    DBuf gin, gout, spec, ospec, Trow, KQ, K1, KS, KT, zpart, zbuf, part, mean, mmpart, mm, fixgrid;
     // plain path extras
     DBuf xs, ys, box, wx, wy, ch, w, V, potA, potB;
     // twiddles per FFT length
@@ -270,9 +281,10 @@ static int twiddles(fitsne_ctx *c, int n, const cx<T> **out) {
 
 static int blocks_for(long n, int per = kThreads) { return (int)((n + per - 1) / per); }
 
+// lanes per CSR row: each lane carries 4 entries per pass, so 32 lanes cover 128 entries at once
 static int sw_for(long long nnz, int N) {
     const double avg = N > 0 ? (double)nnz / N : 0;
-    return avg >= 48 ? 64 : (avg >= 20 ? 32 : 16);
+    return avg >= 64 ? 32 : 16;
 }
 
 // ---------------------------------------------------------------- min / max of the resident Y
@@ -343,7 +355,7 @@ static int geometry(fitsne_ctx *c, double mn, double mx, Geo *g) {
     *g = make_geo(c->D, c->p, mn, mx, nb);
     const bool f32 = c->precision == FITSNE_FP32;
     if (c->D == 2) {
-        const int lim = f32 ? kMaxTot32 / 4 : kMaxTot64 / 2;
+        const int lim = kMaxLen2d;
         if (g->Mp > lim)
             return fail(FITSNE_E_UNSUPPORTED, "2D grid of " + std::to_string(g->n1d) + " nodes/dim (FFT length " +
                                                   std::to_string(g->Mp) + ") exceeds this build's limit " +
@@ -363,7 +375,7 @@ static int geometry(fitsne_ctx *c, double mn, double mx, Geo *g) {
 // ============================================================================ kernel dispatch tables
 
 template <int D>
-static auto pick_spread_f32(int p) -> void (*)(const float *, const int *, const int *, Geo, float4 *) {
+static auto pick_spread_f32(int p) -> decltype(&k_spread_f32<3, D>) {
     switch (p) {
     case 2: return k_spread_f32<2, D>;
     case 3: return k_spread_f32<3, D>;
@@ -420,10 +432,7 @@ static void launch_attractive(fitsne_ctx *c, TV scale, TV *pos) {
     auto cp = c->colP.as<const uint32_t>();
     auto vp = c->val.as<const TV>();
     auto yp = c->Y.as<const TV>();
-    if (sw == 64)
-        launch(c, "attractive", k_attractive<TV, int64_t, uint32_t, D, 64>, grid, dim3(kThreads), 0, rp, cp, vp, yp,
-               c->N, inv_df, df1, scale, pos);
-    else if (sw == 32)
+    if (sw == 32)
         launch(c, "attractive", k_attractive<TV, int64_t, uint32_t, D, 32>, grid, dim3(kThreads), 0, rp, cp, vp, yp,
                c->N, inv_df, df1, scale, pos);
     else
@@ -472,8 +481,17 @@ static int pipeline_f32(fitsne_ctx *c, int mode, const fitsne_step_params *sp) {
     const long nodes = D == 2 ? (long)g.n1d * g.n1d : g.n1d;
     TRY(ensure(c->gin, sizeof(float4) * nodes));
     TRY(ensure(c->gout, sizeof(float4) * nodes));
-    launch(c, "spread_f32", pick_spread_f32<D>(p), dim3(g.nbox), dim3(kWave), 0, c->Y.as<const float>(),
-           c->sorted.as<const int>(), c->start.as<const int>(), g, c->gin.as<float4>());
+    if (c->fixgrid.cap < sizeof(unsigned long long) * 4 * (size_t)nodes) {
+        // the exact grid is zero between iterations (its reader clears it); fresh memory is zeroed here
+        release(c->fixgrid);
+        TRY(ensure(c->fixgrid, sizeof(unsigned long long) * 4 * (size_t)nodes));
+        const long nw = (long)(c->fixgrid.cap / sizeof(unsigned long long));
+        launch(c, "fill", k_fill<unsigned long long>, dim3(std::min(blocks_for(nw), 2048)), dim3(kThreads), 0,
+               c->fixgrid.as<unsigned long long>(), nw, 0ull);
+    }
+    launch(c, "spread_f32", pick_spread_f32<D>(p), dim3((N + kWave - 1) / kWave), dim3(kWave), 0,
+           c->Y.as<const float>(), c->sorted.as<const int>(), c->boxof.as<const int>(), N, g,
+           c->fixgrid.as<unsigned long long>());
 
     const int kind_q = df1 ? FITSNE_KERNEL_SQUARED_CAUCHY : FITSNE_KERNEL_SQUARED_GENERAL;
     const int kind_1 = FITSNE_KERNEL_GENERAL;
@@ -491,9 +509,9 @@ static int pipeline_f32(fitsne_ctx *c, int mode, const fitsne_step_params *sp) {
         TRY(ensure(c->zpart, sizeof(double) * 4 * g.H));
         const int gr_rows = 8 * ((g.n1d + 7) / 8), gr_cols = 8 * ((g.H + 7) / 8);
         if (!(c->kcache && c->kc_mn == mn && c->kc_mx == mx && c->kc_nb == g.nb)) {
-            launch(c, "kern_rows", k_kern_rows<float, kThreads, kMaxTot32>, dim3(gr_rows), dim3(kThreads),
+            launch(c, "kern_rows", k_kern_rows<float, kKernThreads, kKernTot>, dim3(gr_rows), dim3(kKernThreads),
                    sizeof(cx<float>) * g.Mp, g, pl, tw, kind_q, kind_1, c->df, c->Trow.as<cx<float>>());
-            launch(c, "kern_cols", k_kern_cols<float, kThreads, kMaxTot32>, dim3(gr_cols), dim3(kThreads),
+            launch(c, "kern_cols", k_kern_cols<float, kKernThreads, kKernTot>, dim3(gr_cols), dim3(kKernThreads),
                    sizeof(cx<float>) * g.Mp, g, pl, tw, c->Trow.as<const cx<float>>(), 1, c->KQ.as<float>(),
                    c->K1.as<float>(), c->KS.as<float>(), c->KT.as<float>());
             c->kcache = true;
@@ -501,19 +519,22 @@ static int pipeline_f32(fitsne_ctx *c, int mode, const fitsne_step_params *sp) {
             c->kc_mx = mx;
             c->kc_nb = g.nb;
         }
-        launch(c, "rows_fwd", k_rows_fwd<float, kThreads, kMaxTot32>, dim3(gr_rows), dim3(kThreads),
-               sizeof(cx<float>) * 2 * g.Mp, g, pl, tw, c->gin.as<const float>(), 4, 4, c->spec.as<cx<float>>());
-        launch(c, "cols_f32", k_cols_f32<kThreads, kMaxTot32>, dim3(gr_cols), dim3(kThreads),
+        launch(c, "rows_fwd", k_rows_fwd<float, kRowThreads, kRowTot>, dim3(gr_rows), dim3(kRowThreads),
+               sizeof(cx<float>) * 2 * g.Mp, g, pl, tw, (const float *)nullptr, 4, 4, c->spec.as<cx<float>>(),
+               c->fixgrid.as<unsigned long long>());
+        launch(c, "cols_f32", k_cols_f32<kColThreads, kColTot32>, dim3(gr_cols), dim3(kColThreads),
                sizeof(cx<float>) * 4 * g.Mp, g, pl, tw, c->spec.as<const cx<float>>(), c->KQ.as<const float>(),
                c->K1.as<const float>(), c->KS.as<const float>(), c->KT.as<const float>(), c->ospec.as<cx<float>>(),
                c->zpart.as<double>());
         launch(c, "zfinal", k_zfinal, dim3(1), dim3(kThreads), 0, c->zpart.as<const double>(), g.H, (int)df1,
                1.0 / ((double)g.Mp * g.Mp), (double)N, c->zbuf.as<double>());
-        launch(c, "rows_inv", k_rows_inv<float, kThreads, kMaxTot32>, dim3(gr_rows), dim3(kThreads),
+        launch(c, "rows_inv", k_rows_inv<float, kRowThreads, kRowTot>, dim3(gr_rows), dim3(kRowThreads),
                sizeof(cx<float>) * 2 * g.Mp, g, pl, tw, c->ospec.as<const cx<float>>(), 3,
                (float)(1.0 / ((double)g.Mp * g.Mp)), c->gout.as<float>(), 4);
     } else {
         TRY(ensure(c->zpart, sizeof(double) * 4));
+        launch(c, "grid_to_f32", k_grid_to_f32, dim3(blocks_for(nodes)), dim3(kThreads), 0,
+               c->fixgrid.as<unsigned long long>(), nodes, g, c->gin.as<float4>());
         launch(c, "conv1d_f32", k_conv1d<float, kConv1dThreads, kMaxTot32>, dim3(1), dim3(kConv1dThreads),
                sizeof(cx<float>) * 2 * g.Mp, g, pl, tw, kind_q, kind_1, c->df, 1, c->gin.as<const float>(), 4, 3, 0,
                c->gout.as<float>(), 4, c->zpart.as<double>(), (const float *)nullptr);
@@ -582,13 +603,14 @@ static int plain_pass(fitsne_ctx *c, const Geo &g, const FftPlan &pl, const cx<d
         TRY(ensure(c->ospec, sizeof(cx<double>) * plane * nt));
         const int gr_rows = 8 * ((g.n1d + 7) / 8), gr_cols = 8 * ((g.H + 7) / 8);
         const int lines = (nt + 1) / 2;
-        launch(c, "rows_fwd64", k_rows_fwd<double, kThreads, kMaxTot64>, dim3(gr_rows), dim3(kThreads),
-               sizeof(cx<double>) * lines * g.Mp, g, pl, tw, c->w.as<const double>(), nt, nt, c->spec.as<cx<double>>());
-        const int cl = std::min(nt, kMaxTot64 / g.Mp);
-        launch(c, "cols_plain64", k_cols_plain<double, kThreads, kMaxTot64>, dim3(gr_cols), dim3(kThreads),
+        launch(c, "rows_fwd64", k_rows_fwd<double, kRowThreads, kRowTot>, dim3(gr_rows), dim3(kRowThreads),
+               sizeof(cx<double>) * lines * g.Mp, g, pl, tw, c->w.as<const double>(), nt, nt, c->spec.as<cx<double>>(),
+               (unsigned long long *)nullptr);
+        const int cl = std::min(nt, kColTot64 / g.Mp);
+        launch(c, "cols_plain64", k_cols_plain<double, kColThreads64, kColTot64>, dim3(gr_cols), dim3(kColThreads64),
                sizeof(cx<double>) * cl * g.Mp, g, pl, tw, c->spec.as<const cx<double>>(), nt, KC,
                c->ospec.as<cx<double>>());
-        launch(c, "rows_inv64", k_rows_inv<double, kThreads, kMaxTot64>, dim3(gr_rows), dim3(kThreads),
+        launch(c, "rows_inv64", k_rows_inv<double, kRowThreads, kRowTot>, dim3(gr_rows), dim3(kRowThreads),
                sizeof(cx<double>) * lines * g.Mp, g, pl, tw, c->ospec.as<const cx<double>>(), nt,
                1.0 / ((double)g.Mp * g.Mp), c->V.as<double>(), nt);
     } else {
@@ -640,10 +662,10 @@ static int pipeline_f64(fitsne_ctx *c, int mode, const fitsne_step_params *sp) {
         TRY(ensure(c->KQ, sizeof(double) * HH));
         TRY(ensure(c->K1, sizeof(double) * HH));
         const int gr_rows = 8 * ((g.n1d + 7) / 8), gr_cols = 8 * ((g.H + 7) / 8);
-        launch(c, "kern_rows64", k_kern_rows<double, kThreads, kMaxTot64>, dim3(gr_rows), dim3(kThreads),
+        launch(c, "kern_rows64", k_kern_rows<double, kKernThreads, kKernTot>, dim3(gr_rows), dim3(kKernThreads),
                sizeof(cx<double>) * g.Mp, g, pl, tw, kind_q, (int)FITSNE_KERNEL_GENERAL, c->df,
                c->Trow.as<cx<double>>());
-        launch(c, "kern_cols64", k_kern_cols<double, kThreads, kMaxTot64>, dim3(gr_cols), dim3(kThreads),
+        launch(c, "kern_cols64", k_kern_cols<double, kKernThreads, kKernTot>, dim3(gr_cols), dim3(kKernThreads),
                sizeof(cx<double>) * g.Mp, g, pl, tw, c->Trow.as<const cx<double>>(), 0, c->KQ.as<double>(),
                c->K1.as<double>(), (double *)nullptr, (double *)nullptr);
         KQ = c->KQ.as<const double>();
@@ -775,7 +797,7 @@ extern "C" int fitsne_ctx_destroy(fitsne_ctx *c) {
                    &c->count, &c->start, &c->sorted, &c->gin, &c->gout, &c->spec, &c->ospec, &c->Trow, &c->KQ,
                    &c->K1, &c->KS, &c->KT, &c->zpart, &c->zbuf, &c->part, &c->mean, &c->mmpart, &c->mm, &c->xs,
                    &c->ys, &c->box, &c->wx, &c->wy, &c->ch, &c->w, &c->V, &c->potA, &c->potB,
-                   &c->inv_d, &c->colP2, &c->val2, &c->tmp};
+                   &c->inv_d, &c->colP2, &c->val2, &c->tmp, &c->fixgrid};
     for (DBuf *b : all) release(*b);
     for (auto &kv : c->tw32) release(kv.second);
     for (auto &kv : c->tw64) release(kv.second);
@@ -865,8 +887,9 @@ extern "C" int fitsne_set_exaggeration(fitsne_ctx *c, double factor) {
 }
 
 // Spatial order of the points for the FP32 form: a stable counting sort of the internal points by the
-// Morton index of their cell on a 64 x 64 (2D) or 4096 (1D) lattice over the bounding box, so that the
+// Morton index of their cell on a 512 x 512 (2D) or 2^18 (1D) lattice over the bounding box, so that the
 // attractive term's neighbour gathers and the spread touch nearby memory.  Deterministic (host sort).
+
 static inline uint32_t morton2(uint32_t x, uint32_t y) {
     uint32_t k = 0;
     for (int b = 0; b < 16; b++) k |= ((x >> b) & 1u) << (2 * b) | ((y >> b) & 1u) << (2 * b + 1);
@@ -885,7 +908,7 @@ static int reorder_points(fitsne_ctx *c) {
             lo[d] = std::min(lo[d], y[(size_t)i * D + d]);
             hi[d] = std::max(hi[d], y[(size_t)i * D + d]);
         }
-    const int G = D == 2 ? 64 : 4096;
+    const int G = D == 2 ? 512 : 1 << 18;
     const int nkeys = D == 2 ? G * G : G;
     std::vector<int> key(N), cnt(nkeys + 1, 0);
     for (int i = 0; i < N; i++) {
@@ -949,6 +972,7 @@ static int reorder_points(fitsne_ctx *c) {
     c->kcache = false;
     return FITSNE_OK;
 }
+
 
 extern "C" int fitsne_reorder(fitsne_ctx *c) {
     if (!c) return fail(FITSNE_E_ARG, "null context");
@@ -1244,7 +1268,7 @@ extern "C" int fitsne_precompute_2d(double x_max, double x_min, double y_max, do
     TRY(mirror_ctx(&c, 2, p, 1.0, 1, df));
     HIPTRY(hipSetDevice(c->device));
     Geo g = mirror_geo(2, p, n_boxes, x_min, box_width, nullptr, nullptr);
-    if (g.Mp > kMaxTot64) return fail(FITSNE_E_UNSUPPORTED, "grid too large for this build");
+    if (g.Mp > kMaxLen2d) return fail(FITSNE_E_UNSUPPORTED, "grid too large for this build");
     const FftPlan pl = make_plan(g.Mp);
     const cx<double> *tw;
     TRY(twiddles<double>(c, g.Mp, &tw));
@@ -1253,9 +1277,9 @@ extern "C" int fitsne_precompute_2d(double x_max, double x_min, double y_max, do
     TRY(ensure(c->KQ, sizeof(double) * HH));
     TRY(ensure(c->K1, sizeof(double) * HH));
     const int gr_rows = 8 * ((g.n1d + 7) / 8), gr_cols = 8 * ((g.H + 7) / 8);
-    launch(c, "kern_rows64", k_kern_rows<double, kThreads, kMaxTot64>, dim3(gr_rows), dim3(kThreads),
+    launch(c, "kern_rows64", k_kern_rows<double, kKernThreads, kKernTot>, dim3(gr_rows), dim3(kKernThreads),
            sizeof(cx<double>) * g.Mp, g, pl, tw, kernel, kernel, df, c->Trow.as<cx<double>>());
-    launch(c, "kern_cols64", k_kern_cols<double, kThreads, kMaxTot64>, dim3(gr_cols), dim3(kThreads),
+    launch(c, "kern_cols64", k_kern_cols<double, kKernThreads, kKernTot>, dim3(gr_cols), dim3(kKernThreads),
            sizeof(cx<double>) * g.Mp, g, pl, tw, c->Trow.as<const cx<double>>(), 0, c->KQ.as<double>(),
            c->K1.as<double>(), (double *)nullptr, (double *)nullptr);
     HIPTRY(hipMemcpyAsync(fft_kernel_tilde, c->KQ.p, sizeof(double) * HH, hipMemcpyDeviceToHost, c->stream));
@@ -1286,7 +1310,7 @@ extern "C" int fitsne_n_body_fft_2d(int N, int n_terms, const double *xs, const 
         for (int j = 0; j < p; j++)
             if (i != j) g.den[i] *= g.sp[i] - g.sp[j];
     }
-    if (g.Mp > kMaxTot64 / 2) return fail(FITSNE_E_UNSUPPORTED, "grid too large for this build");
+    if (g.Mp > kMaxLen2d) return fail(FITSNE_E_UNSUPPORTED, "grid too large for this build");
     const FftPlan pl = make_plan(g.Mp);
     const cx<double> *tw;
     TRY(twiddles<double>(c, g.Mp, &tw));
@@ -1327,13 +1351,14 @@ extern "C" int fitsne_n_body_fft_2d(int N, int n_terms, const double *xs, const 
     TRY(ensure(c->ospec, sizeof(cx<double>) * plane * nt));
     const int gr_rows = 8 * ((g.n1d + 7) / 8), gr_cols = 8 * ((g.H + 7) / 8);
     const int lines = (nt + 1) / 2;
-    launch(c, "rows_fwd64", k_rows_fwd<double, kThreads, kMaxTot64>, dim3(gr_rows), dim3(kThreads),
-           sizeof(cx<double>) * lines * g.Mp, g, pl, tw, c->w.as<const double>(), nt, nt, c->spec.as<cx<double>>());
-    const int cl = std::min(nt, kMaxTot64 / g.Mp);
-    launch(c, "cols_plain64", k_cols_plain<double, kThreads, kMaxTot64>, dim3(gr_cols), dim3(kThreads),
+    launch(c, "rows_fwd64", k_rows_fwd<double, kRowThreads, kRowTot>, dim3(gr_rows), dim3(kRowThreads),
+           sizeof(cx<double>) * lines * g.Mp, g, pl, tw, c->w.as<const double>(), nt, nt, c->spec.as<cx<double>>(),
+           (unsigned long long *)nullptr);
+    const int cl = std::min(nt, kColTot64 / g.Mp);
+    launch(c, "cols_plain64", k_cols_plain<double, kColThreads64, kColTot64>, dim3(gr_cols), dim3(kColThreads64),
            sizeof(cx<double>) * cl * g.Mp, g, pl, tw, c->spec.as<const cx<double>>(), nt, c->KQ.as<const double>(),
            c->ospec.as<cx<double>>());
-    launch(c, "rows_inv64", k_rows_inv<double, kThreads, kMaxTot64>, dim3(gr_rows), dim3(kThreads),
+    launch(c, "rows_inv64", k_rows_inv<double, kRowThreads, kRowTot>, dim3(gr_rows), dim3(kRowThreads),
            sizeof(cx<double>) * lines * g.Mp, g, pl, tw, c->ospec.as<const cx<double>>(), nt,
            1.0 / ((double)g.Mp * g.Mp), c->V.as<double>(), nt);
     launch(c, "interp_f64", pick_interp_f64<2>(p), dim3(blocks_for(N)), dim3(kThreads), 0, c->box.as<const int>(),

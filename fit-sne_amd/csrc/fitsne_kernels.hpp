@@ -67,13 +67,15 @@ __global__ void k_twiddles(cx<T> *tw, int n) {
 
 // ---------------------------------------------------------------- attractive forces
 // pos_i = scale * sum_j p_ij q_ij (y_i - y_j), q_ij = 1 / (1 + |y_i - y_j|^2 / df)
-// (tsne.cpp:1121-1137, :965-980, :826-837, :721-732).  SW lanes per CSR row; column indices and
-// values are streamed once with non-temporal loads so the gathered Y stays in cache.
+// (tsne.cpp:1121-1137, :965-980, :826-837, :721-732).  SW lanes per CSR row; each lane issues U
+// independent (column, value) loads before its U neighbour gathers, so a wave keeps U times more
+// memory requests in flight.  Column indices and values are streamed once (non-temporal loads).
 template <typename TV, typename TI, typename TC, int D, int SW>
 __global__ __launch_bounds__(kThreads) void k_attractive(const TI *__restrict__ rowP, const TC *__restrict__ colP,
                                                          const TV *__restrict__ valP, const TV *__restrict__ Y,
                                                          int N, TV inv_df, int df_one, TV scale,
                                                          TV *__restrict__ pos) {
+    constexpr int U = 4;
     // blocks of consecutive rows go to the same XCD, so each XCD's L2 holds the Y rows its
     // neighbours gather (the points are kept in spatial order)
     constexpr int RPB = kThreads / SW;
@@ -87,21 +89,32 @@ __global__ __launch_bounds__(kThreads) void k_attractive(const TI *__restrict__ 
     TV acc[D];
 #pragma unroll
     for (int d = 0; d < D; d++) acc[d] = (TV)0;
-    for (long k = beg + lane; k < end; k += SW) {
-        const long j = (long)__builtin_nontemporal_load(&colP[k]);
-        const TV v = __builtin_nontemporal_load(&valP[k]);
-        TV yj[D];
-        load_pt<TV, D>(Y, j, yj);
-        TV diff[D], d2 = (TV)0;
+    for (long k0 = beg + lane; k0 < end; k0 += U * SW) {
+        long j[U];
+        TV v[U];
 #pragma unroll
-        for (int d = 0; d < D; d++) {
-            diff[d] = yi[d] - yj[d];
-            d2 += diff[d] * diff[d];
+        for (int u = 0; u < U; u++) {
+            const long k = k0 + (long)u * SW;
+            const bool ok = k < end;
+            j[u] = ok ? (long)__builtin_nontemporal_load(&colP[k]) : row;  // row itself contributes 0
+            v[u] = ok ? __builtin_nontemporal_load(&valP[k]) : (TV)0;
         }
-        const TV q = df_one ? (TV)1 / ((TV)1 + d2) : (TV)1 / ((TV)1 + d2 * inv_df);
-        const TV vq = v * q;
+        TV yj[U][D];
 #pragma unroll
-        for (int d = 0; d < D; d++) acc[d] += vq * diff[d];
+        for (int u = 0; u < U; u++) load_pt<TV, D>(Y, j[u], yj[u]);
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            TV diff[D], d2 = (TV)0;
+#pragma unroll
+            for (int d = 0; d < D; d++) {
+                diff[d] = yi[d] - yj[u][d];
+                d2 += diff[d] * diff[d];
+            }
+            const TV q = df_one ? (TV)1 / ((TV)1 + d2) : (TV)1 / ((TV)1 + d2 * inv_df);
+            const TV vq = v[u] * q;
+#pragma unroll
+            for (int d = 0; d < D; d++) acc[d] += vq * diff[d];
+        }
     }
 #pragma unroll
     for (int d = 0; d < D; d++) acc[d] = group_sum<TV, SW>(acc[d]);
@@ -206,45 +219,86 @@ __global__ __launch_bounds__(1024) void k_minmax_seq(const TV *__restrict__ Y, c
 
 // ---------------------------------------------------------------- counting sort by box (FP32 form)
 
+// Box of every point and its rank within the box.  Consecutive lanes that fall in the same box (the
+// common case once points are kept in spatial order) form a run; the run's first lane does one
+// atomicAdd for the whole run.
 template <typename TV, int D>
 __global__ __launch_bounds__(kThreads) void k_box_count(const TV *__restrict__ Y, int N, Geo g,
                                                         int *__restrict__ boxof, int *__restrict__ rank,
                                                         int *__restrict__ count) {
     const int i = blockIdx.x * kThreads + threadIdx.x;
-    if (i >= N) return;
-    TV y[D];
-    load_pt<TV, D>(Y, i, y);
-    double t;
-    int b = box_coord<D>((double)y[0], g, t);
-    if constexpr (D == 2) b += box_coord<D>((double)y[1], g, t) * g.nb;
-    boxof[i] = b;
-    rank[i] = atomicAdd(&count[b], 1);
+    const int lane = threadIdx.x % kWave;
+    int b = -1 - lane;  // lanes past N: distinct keys, never a run with a real box
+    if (i < N) {
+        TV y[D];
+        load_pt<TV, D>(Y, i, y);
+        double t;
+        b = box_coord<D>((double)y[0], g, t);
+        if constexpr (D == 2) b += box_coord<D>((double)y[1], g, t) * g.nb;
+    }
+    const int prev = __shfl_up(b, 1, kWave);
+    const bool head = lane == 0 || b != prev;
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long below = heads & ((2ull << lane) - 1ull);  // heads at or below this lane
+    const int run_start = 63 - __builtin_clzll(below);
+    const unsigned long long above = heads & ~((2ull << lane) - 1ull);
+    const int run_end = above ? __builtin_ctzll(above) : kWave;
+    int base = 0;
+    if (head && i < N) base = atomicAdd(&count[b], run_end - lane);
+    base = __shfl(base, run_start, kWave);
+    if (i < N) {
+        boxof[i] = b;
+        rank[i] = base + lane - run_start;
+    }
 }
 
-// Exclusive scan of count[0..n) into start[0..n], then count[] = 0 for the next use.  One block.
+// Exclusive scan of count[0..n) into start[0..n], then count[] = 0 for the next use.  One block of
+// 1024 threads; per tile of 4096 counters each thread owns 4 consecutive ones, and the tile is scanned
+// with wave shuffles plus one pass over the 16 wave totals; a carry runs across tiles.
 __global__ __launch_bounds__(1024) void k_scan_counts(int *__restrict__ count, int *__restrict__ start, int n) {
-    __shared__ int sums[1024];
-    const int t = threadIdx.x;
-    const int chunk = (n + 1023) / 1024;
-    const int b = t * chunk, e = min(b + chunk, n);
-    int s = 0;
-    for (int i = b; i < e; i++) s += count[i];
-    sums[t] = s;
-    __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {
-        int v = t >= o ? sums[t - o] : 0;
+    __shared__ int wsum[1024 / kWave];
+    __shared__ int tile_total;
+    const int t = threadIdx.x, lane = t % kWave, w = t / kWave;
+    int carry = 0;
+    for (int base = 0; base < n; base += 4 * 1024) {
+        const int i0 = base + 4 * t;
+        int v[4], loc = 0;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            v[q] = i0 + q < n ? count[i0 + q] : 0;
+            loc += v[q];
+        }
+        int x = loc;  // inclusive wave scan of the per-thread totals
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            const int y = __shfl_up(x, o, kWave);
+            if (lane >= o) x += y;
+        }
+        if (lane == kWave - 1) wsum[w] = x;
         __syncthreads();
-        sums[t] += v;
+        if (t == 0) {
+            int run = 0;
+            for (int i = 0; i < 1024 / kWave; i++) {
+                const int c = wsum[i];
+                wsum[i] = run;
+                run += c;
+            }
+            tile_total = run;
+        }
+        __syncthreads();
+        int e = carry + wsum[w] + x - loc;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            if (i0 + q < n) {
+                start[i0 + q] = e;
+                count[i0 + q] = 0;
+            }
+            e += v[q];
+        }
+        carry += tile_total;
         __syncthreads();
     }
-    int run = sums[t] - s;  // exclusive
-    for (int i = b; i < e; i++) {
-        const int c = count[i];
-        start[i] = run;
-        run += c;
-        count[i] = 0;
-    }
-    if (t == 1023) start[n] = sums[1023];
+    if (t == 0) start[n] = carry;
 }
 
 __global__ __launch_bounds__(kThreads) void k_scatter(const int *__restrict__ boxof, const int *__restrict__ rank,
@@ -255,63 +309,55 @@ __global__ __launch_bounds__(kThreads) void k_scatter(const int *__restrict__ bo
 }
 
 // ---------------------------------------------------------------- spread, FP32 form
-// One wavefront per box.  Nodes are private to their box (node index = box * p + j), so a box's
-// p^D nodes are written once, with no atomics: lanes accumulate their points' node-relative charges
-// A = L, c = L |e|^2, b = L e (e = y - node) in registers, then the wave sums them through LDS.
-// Sums are exact: every FP32 contribution is scaled by 2^38 into an int64 (offsets in units of the
-// box width), so the result does not depend on the order of points within a box and the pipeline is
-// bitwise repeatable.  Output layout: float4 (A, c, b_x, b_y) per node, [u][v] with u the x node
-// (nbodyfft.cpp:137-138).
+// The box-sorted points are cut into chunks of 64 (one wavefront each, so hot boxes with thousands of
+// points are shared by many waves).  Each lane computes its point's node-relative charges
+//   A = L,  c = L |e|^2,  b = L e    (e = y - node, in box widths)
+// for the p^D nodes of its box; runs of lanes in the same box are summed through LDS and added to the
+// grid with one 64-bit atomic per (run, node, term).  Every FP32 contribution is scaled by 2^38 into an
+// int64, so the sums are exact and do not depend on the order of the additions: the grid, and so the
+// whole gradient, is bitwise repeatable.  Grid layout: int64 [u][v][4] (A, c, b_x, b_y), u the x node
+// (nbodyfft.cpp:137-138); the row FFT reads it, converts it and clears it for the next iteration.
 constexpr double kFixScale = 274877906944.0;  // 2^38
 
 template <int P, int D>
 __global__ __launch_bounds__(kWave) void k_spread_f32(const float *__restrict__ Y, const int *__restrict__ sorted,
-                                                      const int *__restrict__ start, Geo g,
-                                                      float4 *__restrict__ grid) {
+                                                      const int *__restrict__ boxof, int N, Geo g,
+                                                      unsigned long long *__restrict__ grid) {
     constexpr int NV = (D == 2 ? P * P : P);
-    constexpr int NVAL = NV * 4;
-    __shared__ long long sh[NV * (kWave + 1)];
-    const int b = blockIdx.x;
+    __shared__ long long sh[NV][kWave + 1];
+    __shared__ int hl[kWave + 1];
+    __shared__ int hb[kWave];
     const int lane = threadIdx.x;
-    const int bx = D == 2 ? b % g.nb : b;
-    const int by = D == 2 ? b / g.nb : 0;
-    const int s = start[b], e = start[b + 1];
-    float *gf = reinterpret_cast<float *>(grid);
-    if (s == e) {
-        for (int r = lane; r < NVAL; r += kWave) {
-            const int v = r / 4, term = r % 4;
-            long node = D == 2 ? (long)(bx * P + v / P) * g.n1d + by * P + v % P : (long)bx * P + v;
-            gf[node * 4 + term] = 0.f;
-        }
-        return;
-    }
-    long long acc[NV][4];
+    const int k = blockIdx.x * kWave + lane;
+    const bool ok = k < N;
+    int b = -1 - lane;
+    long long val[NV][4];
 #pragma unroll
     for (int v = 0; v < NV; v++)
 #pragma unroll
-        for (int q = 0; q < 4; q++) acc[v][q] = 0;
-    const float S = (float)kFixScale;
-    const double lox = bx * g.bw0 + g.mn, loy = by * g.bw0 + g.mn;
-    for (int k = s + lane; k < e; k += kWave) {
+        for (int q = 0; q < 4; q++) val[v][q] = 0;
+    if (ok) {
         const int i = sorted[k];
+        b = boxof[i];
+        const int bx = D == 2 ? b % g.nb : b;
+        const int by = D == 2 ? b / g.nb : 0;
+        const float S = (float)kFixScale;
         float y[D];
         load_pt<float, D>(Y, i, y);
-        const double tx = ((double)y[0] - lox) / g.bw;
+        const double tx = ((double)y[0] - (bx * g.bw0 + g.mn)) / g.bw;
         float Lx[P], dx[P];
-        {
 #pragma unroll
-            for (int a = 0; a < P; a++) dx[a] = (float)(tx - g.sp[a]);
+        for (int a = 0; a < P; a++) dx[a] = (float)(tx - g.sp[a]);
 #pragma unroll
-            for (int a = 0; a < P; a++) {
-                float v = 1.f;
+        for (int a = 0; a < P; a++) {
+            float v = 1.f;
 #pragma unroll
-                for (int c = 0; c < P; c++)
-                    if (c != a) v *= dx[c];
-                Lx[a] = v / (float)g.den[a];
-            }
+            for (int c = 0; c < P; c++)
+                if (c != a) v *= dx[c];
+            Lx[a] = v / (float)g.den[a];
         }
         if constexpr (D == 2) {
-            const double ty = ((double)y[1] - loy) / g.bw;
+            const double ty = ((double)y[1] - (by * g.bw0 + g.mn)) / g.bw;
             float Ly[P], dy[P];
 #pragma unroll
             for (int a = 0; a < P; a++) dy[a] = (float)(ty - g.sp[a]);
@@ -328,36 +374,67 @@ __global__ __launch_bounds__(kWave) void k_spread_f32(const float *__restrict__ 
 #pragma unroll
                 for (int c = 0; c < P; c++) {
                     const float w = Lx[a] * Ly[c];
-                    acc[a * P + c][0] += (long long)(w * S);
-                    acc[a * P + c][1] += (long long)(w * (dx[a] * dx[a] + dy[c] * dy[c]) * S);
-                    acc[a * P + c][2] += (long long)(w * dx[a] * S);
-                    acc[a * P + c][3] += (long long)(w * dy[c] * S);
+                    val[a * P + c][0] = (long long)(w * S);
+                    val[a * P + c][1] = (long long)(w * (dx[a] * dx[a] + dy[c] * dy[c]) * S);
+                    val[a * P + c][2] = (long long)(w * dx[a] * S);
+                    val[a * P + c][3] = (long long)(w * dy[c] * S);
                 }
         } else {
 #pragma unroll
             for (int a = 0; a < P; a++) {
-                acc[a][0] += (long long)(Lx[a] * S);
-                acc[a][1] += (long long)(Lx[a] * dx[a] * dx[a] * S);
-                acc[a][2] += (long long)(Lx[a] * dx[a] * S);
+                val[a][0] = (long long)(Lx[a] * S);
+                val[a][1] = (long long)(Lx[a] * dx[a] * dx[a] * S);
+                val[a][2] = (long long)(Lx[a] * dx[a] * S);
             }
         }
     }
-    // wave sum through LDS, one charge term at a time; scale back (offsets were in box widths)
-    const double inv = 1.0 / kFixScale;
-    const double unit[4] = {inv, inv * g.bw0 * g.bw0, inv * g.bw0, inv * g.bw0};
+    // runs of equal boxes (the chunk is box-sorted, so a run is all of the chunk's points of one box)
+    const int prev = __shfl_up(b, 1, kWave);
+    const bool head = lane == 0 || b != prev;
+    const unsigned long long heads = __ballot(head);
+    const int nruns = __popcll(heads);
+    if (head) {
+        const int r = __popcll(heads & ((1ull << lane) - 1ull));
+        hl[r] = lane;
+        hb[r] = b;
+    }
+    if (lane == 0) hl[nruns] = kWave;
+    // one charge term per round keeps the LDS footprint small (occupancy)
 #pragma unroll
     for (int q = 0; q < 4; q++) {
 #pragma unroll
-        for (int v = 0; v < NV; v++) sh[v * (kWave + 1) + lane] = acc[v][q];
+        for (int v = 0; v < NV; v++) sh[v][lane] = val[v][q];
         __syncthreads();
-        for (int v = lane; v < NV; v += kWave) {
+        for (int o = lane; o < nruns * NV; o += kWave) {
+            const int r = o / NV, v = o - r * NV;
+            const int box = hb[r];
+            if (box < 0) continue;  // run of padding lanes past N
+            const int l0 = hl[r], l1 = hl[r + 1];
             long long sum = 0;
-            for (int l = 0; l < kWave; l++) sum += sh[v * (kWave + 1) + l];
-            long node = D == 2 ? (long)(bx * P + v / P) * g.n1d + by * P + v % P : (long)bx * P + v;
-            gf[node * 4 + q] = (float)((double)sum * unit[q]);
+            for (int l = l0; l < l1; l++) sum += sh[v][l];
+            if (sum == 0) continue;
+            const int bx = D == 2 ? box % g.nb : box, by = D == 2 ? box / g.nb : 0;
+            const long node = D == 2 ? (long)(bx * P + v / P) * g.n1d + by * P + v % P : (long)bx * P + v;
+            atomicAdd(&grid[node * 4 + q], (unsigned long long)sum);
         }
         __syncthreads();
     }
+}
+
+// int64 grid -> float4 (A, c, b_x, b_y) and clear, for the 1D path (the 2D row FFT does this itself).
+__global__ __launch_bounds__(kThreads) void k_grid_to_f32(unsigned long long *__restrict__ grid, long nodes, Geo g,
+                                                          float4 *__restrict__ out) {
+    const long n = (long)blockIdx.x * kThreads + threadIdx.x;
+    if (n >= nodes) return;
+    const double inv = 1.0 / kFixScale;
+    long long s[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        s[q] = (long long)grid[n * 4 + q];
+        grid[n * 4 + q] = 0ull;
+    }
+    out[n] = make_float4((float)(s[0] * inv), (float)(s[1] * inv * g.bw0 * g.bw0), (float)(s[2] * inv * g.bw0),
+                         (float)(s[3] * inv * g.bw0));
 }
 
 // ---------------------------------------------------------------- kernel spectra
@@ -435,23 +512,36 @@ __global__ __launch_bounds__(NTHR) void k_kern_cols(Geo g, FftPlan pl, const cx<
 // grid: [n1d][n1d][sin] real (u = x node, v = y node); terms 2l, 2l+1 packed as one complex line.
 // Output spec[t][k][u] = DFT over v of term t of row u, k in [0, H) (two real spectra unpacked from
 // the packed line: A = (Z(k) + conj Z(-k)) / 2, B = -i (Z(k) - conj Z(-k)) / 2).
+// With fixgrid != null the input is the exact int64 grid of k_spread_f32 ([u][v][4] = A, c, b_x, b_y in
+// 2^-38 units, offsets in box widths); it is converted and cleared here (each row read once).
 template <typename T, int NTHR, int MAXTOT>
 __global__ __launch_bounds__(NTHR) void k_rows_fwd(Geo g, FftPlan pl, const cx<T> *__restrict__ tw,
                                                    const T *__restrict__ grid, int sin, int nterms,
-                                                   cx<T> *__restrict__ spec) {
+                                                   cx<T> *__restrict__ spec,
+                                                   unsigned long long *__restrict__ fixgrid) {
     extern __shared__ unsigned char smem_raw[];
     cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
     const int u = xcd_row(blockIdx.x, g.n1d);
     if (u >= g.n1d) return;
     const int Mp = g.Mp, n1d = g.n1d, H = g.H;
     const int lines = (nterms + 1) / 2;
+    const double inv = 1.0 / kFixScale;
+    const double unit[4] = {inv, inv * g.bw0 * g.bw0, inv * g.bw0, inv * g.bw0};
     for (int q = threadIdx.x; q < lines * Mp; q += NTHR) {
         const int l = q / Mp, v = q - l * Mp;
         cx<T> z{(T)0, (T)0};
         if (v < n1d) {
-            const T *src = grid + ((long)u * n1d + v) * sin + 2 * l;
-            z.x = src[0];
-            if (2 * l + 1 < nterms) z.y = src[1];
+            if (fixgrid) {
+                unsigned long long *src = fixgrid + ((long)u * n1d + v) * 4 + 2 * l;
+                z.x = (T)((double)(long long)src[0] * unit[2 * l]);
+                z.y = (T)((double)(long long)src[1] * unit[2 * l + 1]);
+                src[0] = 0ull;
+                src[1] = 0ull;
+            } else {
+                const T *src = grid + ((long)u * n1d + v) * sin + 2 * l;
+                z.x = src[0];
+                if (2 * l + 1 < nterms) z.y = src[1];
+            }
         }
         buf[q] = z;
     }
@@ -516,7 +606,7 @@ __global__ __launch_bounds__(NTHR) void k_cols_f32(Geo g, FftPlan pl, const cx<f
         const int t = q / n1d, u = q - t * n1d;
         ospec[t * plane + (long)k * n1d + u] = buf[t * Mp + u];
     }
-    __shared__ double shz[kThreads / kWave];
+    __shared__ double shz[NTHR / kWave];
     z0 = block_sum(z0 * wk, shz);
     z1 = block_sum(z1 * wk, shz);
     z2 = block_sum(z2 * wk, shz);

@@ -147,7 +147,7 @@ int fitsne_set_P_i64(fitsne_ctx *ctx, int N, const int64_t *row_P, const int32_t
 int fitsne_set_exaggeration(fitsne_ctx *ctx, double factor);
 
 /* Spatial order (FP32 context): points are kept internally in the Morton order of their cell on a
- * 64x64 lattice over the embedding, so that neighbour gathers and the spread stay in cache.  All host
+ * 512x512 lattice over the embedding, so that neighbour gathers and the spread stay in cache.  All host
  * arrays remain in the caller's order.  fitsne_set_Y reorders when enabled (default) and N >= 65536;
  * fitsne_reorder re-sorts now (e.g. once clusters have formed).  No reference counterpart. */
 int fitsne_set_reorder(fitsne_ctx *ctx, int enable);

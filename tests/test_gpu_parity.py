@@ -202,15 +202,16 @@ def test_fp32_bitwise_repeatable_across_contexts():
 
 
 def test_fp32_spatial_reorder_is_invisible():
-    """The FP32 context keeps points in spatial order internally (N >= 65536).  Per-row attractive sums
-    keep their order and the spread is exact, so the gradient is bitwise the same as without reordering;
-    Y comes back in the caller's order."""
+    """The FP32 context keeps points in spatial order internally (N >= 65536) and then gathers attractive
+    neighbours through a column-blocked copy of P.  Results come back in the caller's order, agree with
+    the unreordered path to FP32 summation order (1e-6), are bitwise repeatable, and follow the
+    reference's `else if` min/max rule for the caller's first point."""
     say("[gpu] fp32 reorder N=70000")
     from synth import small_mixture_workload
     P, Y, _ = small_mixture_workload(N=70000, D=2, k=12, per_point=20, span=60.0, seed=3)
     Y[0, 0] = Y.min() - 1.0  # the caller's xs[0] is the strict minimum: `else if` rule in play
     outs = []
-    for reorder in (True, False):
+    for reorder in (True, True, False):
         ctx = fg.Context(no_dims=2, precision=fg.FP32)
         ctx.set_reorder(reorder)
         ctx.set_P(*P)
@@ -222,6 +223,9 @@ def test_fp32_spatial_reorder_is_invisible():
         assert np.array_equal(back, Y.astype(np.float32).astype(np.float64))
         outs.append((g, z, geo))
     assert np.array_equal(outs[0][0], outs[1][0]) and outs[0][1] == outs[1][1]
+    r01 = rel(outs[0][0], outs[2][0])
+    say(f"    reordered vs caller order rel {r01:.3e}")
+    assert r01 <= 1e-6 and abs(outs[0][1] - outs[2][1]) <= 1e-6 * abs(outs[2][1])
     mn, mx = orc.minmax_quirk(Y.astype(np.float32).astype(np.float64))
     assert outs[0][2]["min"] == mn and outs[0][2]["max"] == mx
     Pu = (P[0].astype(np.uint32), P[1].astype(np.uint32), P[2])
