@@ -53,6 +53,28 @@ def kernel_profile(ctx, steps, prm):
     return stats
 
 
+def measured_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary,
+    each counter scaled by what it read for a plain 64 MiB device copy in the same runs; None when the
+    summary is for another kernel."""
+    import re
+
+    path = os.path.join(ROOT, "profiles", "attractive_c2_n1m_2d_hbm_traffic.md")
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    m = re.search(r"```json\n(.*?)```", text, re.S)
+    if not m:
+        return None
+    d = json.loads(m.group(1))
+    if d.get("kernel") != kernel:
+        return None
+    kib = (d["fetch_kib_raw"] * d["copy_kib"] / d["copy_fetch_kib_raw"]
+           + d["write_kib_raw"] * d["copy_kib"] / d["copy_write_kib_raw"])
+    return kib * 1024.0
+
+
 def cpu_baseline(P, Y32, calls):
     """The oracle's gradient (FP64 C restatement of computeFftGradient, PARALLEL_FOR threading kept)
     on the same Y and P; returns (iters/s, threads, dC, sum_Q)."""
@@ -184,7 +206,8 @@ def main():
                    "fft_len": geo["fft_len"], "n_boxes_end": geo_end["n_boxes"], "learning_rate": lr,
                    "momentum": 0.8, "max_step_norm": 5.0, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": "attractive", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic("k_attractive<float, long, unsigned int, 2, 32>"),
                      "algorithmic_bytes": att_bytes, "avg_ms": att_ms},
         "cpu_baseline": cpu,
         "fp64_plain": res64,

@@ -178,7 +178,10 @@ struct fitsne_ctx {
     // embedding state (element type = float for FP32, double for FP64 plain)
     DBuf Y, uY, gains, pos, dC;
     // sort (FP32)
-    DBuf boxof, rank, count, start, sorted;
+    // windowed copy of P for the FP32 attractive term (k_attr_win), built on first use
+    DBuf wcount, wstart, wkey, wval, wbmax;
+    bool win_ok = false;
+    int win_R = 0, win_NW = 0;
     // grids, spectra
     DBuf gin, gout, spec, ospec, Trow, KQ, K1, KS, KT, zpart, zbuf, part, mean, mmpart, mm, fixgrid;
     // plain path extras
@@ -420,11 +423,51 @@ static auto pick_interp_f64(int p) -> decltype(&k_interp_f64<3, D>) {
     }
 }
 
+// Bucketed copy of the CSR for k_attr_win: counts per (row block, window), scan, scatter.  Used when
+// the entry offsets fit an int and the window cursors fit in LDS; otherwise the CSR kernel runs.
+static int build_windows(fitsne_ctx *c) {
+    const int N = c->N;
+    c->win_ok = false;
+    const int R = std::min(kAttRows, 64 * (int)((((long)N + 255) / 256 + 63) / 64));
+    const int NB = (N + R - 1) / R, NW = (N + kWin - 1) / kWin;
+    const long nbw = (long)NB * NW;
+    const long long cap = c->nnz + 3ll * nbw;  // every bucket padded to a multiple of 4 entries
+    if (c->nnz <= 0 || NW > 16384 || nbw >= (1l << 28) || cap >= (1ll << 31) - 1) return FITSNE_OK;
+    TRY(ensure(c->wcount, sizeof(int) * nbw));
+    TRY(ensure(c->wstart, sizeof(int) * (nbw + 1)));
+    TRY(ensure(c->wbmax, sizeof(float) * NB));
+    TRY(ensure(c->wkey, sizeof(uint32_t) * (size_t)cap));
+    TRY(ensure(c->wval, sizeof(float) * (size_t)cap));
+    const size_t lds = sizeof(int) * (size_t)NW;
+    launch(c, "win_count", k_win_count, dim3(NB), dim3(kAttThreads), lds, c->rowP.as<const int64_t>(),
+           c->colP.as<const uint32_t>(), c->val.as<const float>(), N, R, NW, c->wcount.as<int>(), c->wbmax.as<float>());
+    launch(c, "scan_counts", k_scan_counts, dim3(1), dim3(1024), 0, c->wcount.as<int>(), c->wstart.as<int>(), (int)nbw);
+    launch(c, "win_fill", k_win_fill, dim3(NB), dim3(kAttThreads), lds, c->rowP.as<const int64_t>(),
+           c->colP.as<const uint32_t>(), c->val.as<const float>(), N, R, NW, c->wstart.as<const int>(),
+           c->wkey.as<uint32_t>(), c->wval.as<float>());
+    HIPTRY(hipGetLastError());
+    c->win_R = R;
+    c->win_NW = NW;
+    c->win_ok = true;
+    return FITSNE_OK;
+}
+
 template <typename TV, int D>
-static void launch_attractive(fitsne_ctx *c, TV scale, TV *pos) {
-    const int sw = sw_for(c->nnz, c->N);
+static int launch_attractive(fitsne_ctx *c, TV scale, TV *pos) {
     const TV inv_df = (TV)(1.0 / c->df);
     const int df1 = c->df == 1.0;
+    if constexpr (std::is_same<TV, float>::value) {
+        if (!c->win_ok) TRY(build_windows(c));
+        if (c->win_ok) {
+            const int NB = (c->N + c->win_R - 1) / c->win_R;
+            launch(c, "attractive", k_attr_win<D>, dim3(8 * ((NB + 7) / 8)), dim3(kAttThreads), 0,
+                   c->wstart.as<const int>(), c->wkey.as<const uint32_t>(), c->wval.as<const float>(),
+                   c->wbmax.as<const float>(), c->Y.as<const float>(), c->N, c->win_R, c->win_NW, inv_df, df1,
+                   (float)(0.5 * std::sqrt(c->df)), scale, pos);
+            return FITSNE_OK;
+        }
+    }
+    const int sw = sw_for(c->nnz, c->N);
     const int rpb = kThreads / sw;
     const int nblk = (c->N + rpb - 1) / rpb;
     dim3 grid(8 * ((nblk + 7) / 8));
@@ -438,6 +481,7 @@ static void launch_attractive(fitsne_ctx *c, TV scale, TV *pos) {
     else
         launch(c, "attractive", k_attractive<TV, int64_t, uint32_t, D, 16>, grid, dim3(kThreads), 0, rp, cp, vp, yp,
                c->N, inv_df, df1, scale, pos);
+    return FITSNE_OK;
 }
 
 // ============================================================================ FP32 pipeline
@@ -449,7 +493,7 @@ static int pipeline_f32(fitsne_ctx *c, int mode, const fitsne_step_params *sp) {
     const int p = c->p;
     const bool df1 = c->df == 1.0;
     TRY(ensure(c->pos, sizeof(float) * (size_t)N * D));
-    launch_attractive<float, D>(c, (float)c->exag, c->pos.as<float>());
+    TRY((launch_attractive<float, D>(c, (float)c->exag, c->pos.as<float>())));
 
     double mn, mx;
     TRY(current_minmax<float>(c, &mn, &mx));
@@ -458,25 +502,6 @@ static int pipeline_f32(fitsne_ctx *c, int mode, const fitsne_step_params *sp) {
     const FftPlan pl = make_plan(g.Mp);
     const cx<float> *tw;
     TRY(twiddles<float>(c, g.Mp, &tw));
-
-    // counting sort of the points by box
-    TRY(ensure(c->boxof, sizeof(int) * (size_t)N));
-    TRY(ensure(c->rank, sizeof(int) * (size_t)N));
-    TRY(ensure(c->sorted, sizeof(int) * (size_t)N));
-    if (c->count.cap < sizeof(int) * (size_t)g.nbox) {
-        // fresh counters are zeroed on the context's stream (k_scan_counts re-zeroes them after use)
-        release(c->count);
-        TRY(ensure(c->count, sizeof(int) * (size_t)g.nbox));
-        const long nc = (long)(c->count.cap / sizeof(int));
-        launch(c, "fill", k_fill<int>, dim3(std::min(blocks_for(nc), 2048)), dim3(kThreads), 0, c->count.as<int>(), nc,
-               0);
-    }
-    TRY(ensure(c->start, sizeof(int) * ((size_t)g.nbox + 1)));
-    launch(c, "box_count", k_box_count<float, D>, dim3(blocks_for(N)), dim3(kThreads), 0, c->Y.as<const float>(), N, g,
-           c->boxof.as<int>(), c->rank.as<int>(), c->count.as<int>());
-    launch(c, "scan_counts", k_scan_counts, dim3(1), dim3(1024), 0, c->count.as<int>(), c->start.as<int>(), g.nbox);
-    launch(c, "scatter", k_scatter, dim3(blocks_for(N)), dim3(kThreads), 0, c->boxof.as<const int>(),
-           c->rank.as<const int>(), c->start.as<const int>(), N, c->sorted.as<int>());
 
     const long nodes = D == 2 ? (long)g.n1d * g.n1d : g.n1d;
     TRY(ensure(c->gin, sizeof(float4) * nodes));
@@ -489,9 +514,8 @@ static int pipeline_f32(fitsne_ctx *c, int mode, const fitsne_step_params *sp) {
         launch(c, "fill", k_fill<unsigned long long>, dim3(std::min(blocks_for(nw), 2048)), dim3(kThreads), 0,
                c->fixgrid.as<unsigned long long>(), nw, 0ull);
     }
-    launch(c, "spread_f32", pick_spread_f32<D>(p), dim3((N + kWave - 1) / kWave), dim3(kWave), 0,
-           c->Y.as<const float>(), c->sorted.as<const int>(), c->boxof.as<const int>(), N, g,
-           c->fixgrid.as<unsigned long long>());
+    launch(c, "spread_f32", pick_spread_f32<D>(p), dim3((N + kSpreadPts - 1) / kSpreadPts), dim3(kSpreadThreads), 0,
+           c->Y.as<const float>(), N, g, c->fixgrid.as<unsigned long long>());
 
     const int kind_q = df1 ? FITSNE_KERNEL_SQUARED_CAUCHY : FITSNE_KERNEL_SQUARED_GENERAL;
     const int kind_1 = FITSNE_KERNEL_GENERAL;
@@ -630,7 +654,7 @@ static int pipeline_f64(fitsne_ctx *c, int mode, const fitsne_step_params *sp) {
     const int N = c->N, p = c->p;
     const bool df1 = c->df == 1.0;
     TRY(ensure(c->pos, sizeof(double) * (size_t)N * D));
-    launch_attractive<double, D>(c, 1.0, c->pos.as<double>());
+    TRY((launch_attractive<double, D>(c, 1.0, c->pos.as<double>())));
     double mn, mx;
     TRY(current_minmax<double>(c, &mn, &mx));
     Geo g;
@@ -793,8 +817,8 @@ extern "C" int fitsne_ctx_destroy(fitsne_ctx *c) {
     if (!c) return FITSNE_OK;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    DBuf *all[] = {&c->rowP, &c->colP, &c->val, &c->Y, &c->uY, &c->gains, &c->pos, &c->dC, &c->boxof, &c->rank,
-                   &c->count, &c->start, &c->sorted, &c->gin, &c->gout, &c->spec, &c->ospec, &c->Trow, &c->KQ,
+    DBuf *all[] = {&c->rowP, &c->colP, &c->val, &c->Y, &c->uY, &c->gains, &c->pos, &c->dC, &c->wcount, &c->wstart,
+                   &c->wkey, &c->wval, &c->wbmax, &c->gin, &c->gout, &c->spec, &c->ospec, &c->Trow, &c->KQ,
                    &c->K1, &c->KS, &c->KT, &c->zpart, &c->zbuf, &c->part, &c->mean, &c->mmpart, &c->mm, &c->xs,
                    &c->ys, &c->box, &c->wx, &c->wy, &c->ch, &c->w, &c->V, &c->potA, &c->potB,
                    &c->inv_d, &c->colP2, &c->val2, &c->tmp, &c->fixgrid};
@@ -852,6 +876,7 @@ static int set_P_impl(fitsne_ctx *c, int N, const TI *row_P, const TC *col_P, co
     c->inv.resize(N);
     for (int i = 0; i < N; i++) c->perm[i] = c->inv[i] = i;
     c->permuted = false;
+    c->win_ok = false;
     return FITSNE_OK;
 }
 
@@ -968,6 +993,7 @@ static int reorder_points(fitsne_ctx *c) {
     TRY(ensure(c->inv_d, sizeof(int) * (size_t)N));
     HIPTRY(hipMemcpy(c->inv_d.p, c->inv.data(), sizeof(int) * N, hipMemcpyHostToDevice));
     c->permuted = true;
+    c->win_ok = false;
     c->mm_valid = c->mm_pending = false;
     c->kcache = false;
     return FITSNE_OK;

@@ -4,9 +4,9 @@
 //  * FP64 "plain": the reference's stages one for one -- charges [1,x,y,x^2+y^2] (or the df != 1 sets),
 //    scattered spread with FP64 atomics, per-term circulant convolution, interpolation of every
 //    potential, the Z expansion and the dC combine (tsne.cpp:1027-1169, nbodyfft.cpp:73-248).
-//  * FP32 form: counting sort by box, wave-per-box spread of node-relative charges (A, b, c), FFT
-//    convolution with the kernel and its odd moments, Z by Parseval in FP64, interpolation fused
-//    with the optimiser update.
+//  * FP32 form: exact fixed-point spread of node-relative charges (A, b, c) over runs of same-box
+//    points, FFT convolution with the kernel and its odd moments, Z by Parseval in FP64,
+//    interpolation fused with the optimiser update.
 #pragma once
 
 #include <cfloat>
@@ -124,6 +124,199 @@ __global__ __launch_bounds__(kThreads) void k_attractive(const TI *__restrict__ 
     }
 }
 
+// ---------------------------------------------------------------- attractive forces, windowed (FP32 form)
+// For points in spatial order the neighbours of a row lie in a few ranges of columns, but they are
+// scattered inside them, so the CSR kernel's gathers are one L2 request per entry.  This layout works
+// from LDS instead.  Rows are cut into blocks of R (<= kAttRows) consecutive points and columns into
+// windows of kWin points.  The entries of row block b whose column falls in window w form bucket (b, w);
+// buckets are stored contiguously, b-major then w, each padded to a multiple of 4 entries (padding:
+// key 0, value 0), each entry as
+//   key = (row - b R) << 16 | (col - w kWin)   (u32)   and   p_ij (f32).
+// The workgroup of row block b holds its rows' Y and one int64 accumulator per row and coordinate in
+// LDS, and walks its non-empty buckets: window w of Y sits in LDS (the next one is loaded into registers
+// meanwhile), every lane takes 4 consecutive entries per 16-byte load, gathers both points from LDS and
+// adds
+//   round(p_ij q_ij (y_i - y_j) S_b)
+// to the row's accumulator (LDS 64-bit atomics; a lane first merges its consecutive entries of one row).
+// S_b = 2^(50 - e), 2^e > max over the block's rows of sum_j p_ij * sqrt(df)/2, bounds every partial
+// sum by 2^50 since |q d| <= sqrt(df)/2: the integer sums are exact, so the forces do not depend on the
+// order of the entries and are bitwise repeatable.  Same formula as k_attractive (tsne.cpp:1121-1137).
+constexpr int kWin = 4096;
+constexpr int kAttRows = 4096;
+constexpr int kAttThreads = 1024;
+
+// round-to-nearest int64 of x for |x| < 2^51: adding 1.5 * 2^52 leaves the integer in the low mantissa bits
+__device__ __forceinline__ long long fix_rn(double x) {
+    constexpr double M = 6755399441055744.0;
+    return __double_as_longlong(x + M) - __double_as_longlong(M);
+}
+
+// Per row block: entries per window, rounded up to a multiple of 4, into count[b * NW + w], and
+// max_i sum_j |p_ij| into bmax[b].
+__global__ __launch_bounds__(kAttThreads) void k_win_count(const int64_t *__restrict__ rowP,
+                                                           const uint32_t *__restrict__ colP,
+                                                           const float *__restrict__ val, int N, int R, int NW,
+                                                           int *__restrict__ count, float *__restrict__ bmax) {
+    extern __shared__ int hist[];
+    __shared__ unsigned int smax;
+    const int b = blockIdx.x, t = threadIdx.x, lane = t % kWave, wv = t / kWave;
+    const int r0 = b * R, r1 = min(N, r0 + R);
+    for (int w = t; w < NW; w += kAttThreads) hist[w] = 0;
+    if (t == 0) smax = 0u;
+    __syncthreads();
+    for (int r = r0 + wv; r < r1; r += kAttThreads / kWave) {
+        float sum = 0.f;
+        for (long k = rowP[r] + lane; k < rowP[r + 1]; k += kWave) {
+            atomicAdd(&hist[colP[k] / kWin], 1);
+            sum += fabsf(val[k]);
+        }
+        sum = group_sum<float, kWave>(sum);
+        if (lane == 0) atomicMax(&smax, __float_as_uint(sum));  // non-negative floats order as integers
+    }
+    __syncthreads();
+    for (int w = t; w < NW; w += kAttThreads) count[(long)b * NW + w] = (hist[w] + 3) & ~3;
+    if (t == 0) bmax[b] = __uint_as_float(smax);
+}
+
+// Scatter the entries into their buckets (start[] from the exclusive scan of the counts) and write the
+// padding.  The order inside a bucket is whatever the LDS cursors give; the sums do not depend on it.
+__global__ __launch_bounds__(kAttThreads) void k_win_fill(const int64_t *__restrict__ rowP,
+                                                          const uint32_t *__restrict__ colP,
+                                                          const float *__restrict__ val, int N, int R, int NW,
+                                                          const int *__restrict__ start, uint32_t *__restrict__ key,
+                                                          float *__restrict__ wval) {
+    extern __shared__ int cur[];
+    const int b = blockIdx.x, t = threadIdx.x, lane = t % kWave, wv = t / kWave;
+    const int r0 = b * R, r1 = min(N, r0 + R);
+    for (int w = t; w < NW; w += kAttThreads) cur[w] = 0;
+    __syncthreads();
+    const int *st = start + (long)b * NW;
+    for (int r = r0 + wv; r < r1; r += kAttThreads / kWave) {
+        for (long k = rowP[r] + lane; k < rowP[r + 1]; k += kWave) {
+            const uint32_t j = colP[k];
+            const int w = (int)(j / kWin);
+            const int at = st[w] + atomicAdd(&cur[w], 1);
+            key[at] = (uint32_t)(r - r0) << 16 | (j - (uint32_t)w * kWin);
+            wval[at] = val[k];
+        }
+    }
+    __syncthreads();
+    for (int w = t; w < NW; w += kAttThreads)
+        for (int at = st[w] + cur[w]; at < st[w + 1]; at++) {
+            key[at] = 0u;
+            wval[at] = 0.f;
+        }
+}
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+template <int D>
+__device__ __forceinline__ void attr_quad(const u32x4 &kq, const f32x4 &vq4, const float *yr, const float *yw,
+                                          unsigned long long *acc, float inv_df, int df_one, double S) {
+    const uint32_t kk[4] = {kq.x, kq.y, kq.z, kq.w};
+    const float v[4] = {vq4.x, vq4.y, vq4.z, vq4.w};
+    int cur = -1;
+    long long run[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) run[d] = 0;
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const int il = (int)(kk[u] >> 16), jl = (int)(kk[u] & 0xffffu);
+        float diff[D], d2 = 0.f;
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            diff[d] = yr[il * D + d] - yw[jl * D + d];
+            d2 += diff[d] * diff[d];
+        }
+        const float q = df_one ? 1.f / (1.f + d2) : 1.f / (1.f + d2 * inv_df);
+        const float vq = v[u] * q;
+        if (il != cur) {
+            if (cur >= 0) {
+#pragma unroll
+                for (int d = 0; d < D; d++) atomicAdd(&acc[cur * D + d], (unsigned long long)run[d]);
+            }
+            cur = il;
+#pragma unroll
+            for (int d = 0; d < D; d++) run[d] = 0;
+        }
+#pragma unroll
+        for (int d = 0; d < D; d++) run[d] += fix_rn((double)(vq * diff[d]) * S);
+    }
+#pragma unroll
+    for (int d = 0; d < D; d++) atomicAdd(&acc[cur * D + d], (unsigned long long)run[d]);
+}
+
+template <int D>
+__global__ __launch_bounds__(kAttThreads) void k_attr_win(const int *__restrict__ start,
+                                                          const uint32_t *__restrict__ key,
+                                                          const float *__restrict__ wval,
+                                                          const float *__restrict__ bmax,
+                                                          const float *__restrict__ Y, int N, int R, int NW,
+                                                          float inv_df, int df_one, float mdf, float scale,
+                                                          float *__restrict__ pos) {
+    constexpr int WL = kWin * D / kAttThreads;  // window floats per thread
+    __shared__ float yw[kWin * D];
+    __shared__ float yr[kAttRows * D];
+    __shared__ unsigned long long acc[kAttRows * D];
+    const int nblk = (N + R - 1) / R;
+    const int b = xcd_row(blockIdx.x, nblk);
+    if (b >= nblk) return;
+    const int t = threadIdx.x;
+    const int r0 = b * R, nr = min(R, N - r0);
+    for (int q = t; q < nr * D; q += kAttThreads) {
+        yr[q] = Y[(long)r0 * D + q];
+        acc[q] = 0ull;
+    }
+    const double bound = (double)bmax[b] * mdf;
+    int e = 0;
+    if (bound > 0.0) (void)frexp(bound, &e);  // bound < 2^e
+    const double S = ldexp(1.0, 50 - e), invS = ldexp(1.0, e - 50);
+    const int *st = start + (long)b * NW;
+    const u32x4 *K4 = reinterpret_cast<const u32x4 *>(key);
+    const f32x4 *V4 = reinterpret_cast<const f32x4 *>(wval);
+    int w = 0;
+    while (w < NW && st[w] == st[w + 1]) w++;
+    float pre[WL];
+    auto fetch = [&](int ww) {
+        const long base = (long)ww * kWin * D;
+        const int lim = (int)min((long)kWin * D, (long)N * D - base);
+#pragma unroll
+        for (int u = 0; u < WL; u++) {
+            const int q = t + u * kAttThreads;
+            pre[u] = q < lim ? Y[base + q] : 0.f;
+        }
+    };
+    if (w < NW) fetch(w);
+    while (w < NW) {
+        const int q0 = st[w] >> 2, q1 = st[w + 1] >> 2;
+        int wn = w + 1;
+        while (wn < NW && st[wn] == st[wn + 1]) wn++;
+        __syncthreads();  // the previous window's readers are done
+#pragma unroll
+        for (int u = 0; u < WL; u++) yw[t + u * kAttThreads] = pre[u];
+        __syncthreads();
+        if (wn < NW) fetch(wn);  // consumed after this bucket
+        for (int qq = q0 + t; qq < q1; qq += 2 * kAttThreads) {
+            const u32x4 ka = __builtin_nontemporal_load(&K4[qq]);
+            const f32x4 va = __builtin_nontemporal_load(&V4[qq]);
+            const bool two = qq + kAttThreads < q1;
+            u32x4 kb = {0u, 0u, 0u, 0u};
+            f32x4 vb = {0.f, 0.f, 0.f, 0.f};
+            if (two) {
+                kb = __builtin_nontemporal_load(&K4[qq + kAttThreads]);
+                vb = __builtin_nontemporal_load(&V4[qq + kAttThreads]);
+            }
+            attr_quad<D>(ka, va, yr, yw, acc, inv_df, df_one, S);
+            if (two) attr_quad<D>(kb, vb, yr, yw, acc, inv_df, df_one, S);
+        }
+        w = wn;
+    }
+    __syncthreads();
+    for (int q = t; q < nr * D; q += kAttThreads)
+        pos[(long)r0 * D + q] = (float)((double)(long long)acc[q] * invS) * scale;
+}
+
 // ---------------------------------------------------------------- min / max
 // Over the flat sequence Y[0..n): max of all; min of all but index 0 (2D) or of all (1D); Y[0].
 // The 2D reference rule (`else if`, tsne.cpp:1045-1048) equals this whenever Y[0] >= min(rest);
@@ -217,40 +410,7 @@ __global__ __launch_bounds__(1024) void k_minmax_seq(const TV *__restrict__ Y, c
     }
 }
 
-// ---------------------------------------------------------------- counting sort by box (FP32 form)
-
-// Box of every point and its rank within the box.  Consecutive lanes that fall in the same box (the
-// common case once points are kept in spatial order) form a run; the run's first lane does one
-// atomicAdd for the whole run.
-template <typename TV, int D>
-__global__ __launch_bounds__(kThreads) void k_box_count(const TV *__restrict__ Y, int N, Geo g,
-                                                        int *__restrict__ boxof, int *__restrict__ rank,
-                                                        int *__restrict__ count) {
-    const int i = blockIdx.x * kThreads + threadIdx.x;
-    const int lane = threadIdx.x % kWave;
-    int b = -1 - lane;  // lanes past N: distinct keys, never a run with a real box
-    if (i < N) {
-        TV y[D];
-        load_pt<TV, D>(Y, i, y);
-        double t;
-        b = box_coord<D>((double)y[0], g, t);
-        if constexpr (D == 2) b += box_coord<D>((double)y[1], g, t) * g.nb;
-    }
-    const int prev = __shfl_up(b, 1, kWave);
-    const bool head = lane == 0 || b != prev;
-    const unsigned long long heads = __ballot(head);
-    const unsigned long long below = heads & ((2ull << lane) - 1ull);  // heads at or below this lane
-    const int run_start = 63 - __builtin_clzll(below);
-    const unsigned long long above = heads & ~((2ull << lane) - 1ull);
-    const int run_end = above ? __builtin_ctzll(above) : kWave;
-    int base = 0;
-    if (head && i < N) base = atomicAdd(&count[b], run_end - lane);
-    base = __shfl(base, run_start, kWave);
-    if (i < N) {
-        boxof[i] = b;
-        rank[i] = base + lane - run_start;
-    }
-}
+// ---------------------------------------------------------------- exclusive scan
 
 // Exclusive scan of count[0..n) into start[0..n], then count[] = 0 for the next use.  One block of
 // 1024 threads; per tile of 4096 counters each thread owns 4 consecutive ones, and the tile is scanned
@@ -301,123 +461,166 @@ __global__ __launch_bounds__(1024) void k_scan_counts(int *__restrict__ count, i
     if (t == 0) start[n] = carry;
 }
 
-__global__ __launch_bounds__(kThreads) void k_scatter(const int *__restrict__ boxof, const int *__restrict__ rank,
-                                                      const int *__restrict__ start, int N,
-                                                      int *__restrict__ sorted) {
-    const int i = blockIdx.x * kThreads + threadIdx.x;
-    if (i < N) sorted[start[boxof[i]] + rank[i]] = i;
-}
-
 // ---------------------------------------------------------------- spread, FP32 form
-// The box-sorted points are cut into chunks of 64 (one wavefront each, so hot boxes with thousands of
-// points are shared by many waves).  Each lane computes its point's node-relative charges
+// A workgroup takes kSpreadPts consecutive points (memory order; spatially ordered points keep this to
+// a few dozen boxes).  Each lane finds its point's box and computes the node-relative charges
 //   A = L,  c = L |e|^2,  b = L e    (e = y - node, in box widths)
-// for the p^D nodes of its box; runs of lanes in the same box are summed through LDS and added to the
-// grid with one 64-bit atomic per (run, node, term).  Every FP32 contribution is scaled by 2^38 into an
-// int64, so the sums are exact and do not depend on the order of the additions: the grid, and so the
-// whole gradient, is bitwise repeatable.  Grid layout: int64 [u][v][4] (A, c, b_x, b_y), u the x node
-// (nbodyfft.cpp:137-138); the row FFT reads it, converts it and clears it for the next iteration.
+// for the p^D nodes of its box, each scaled by 2^38 and rounded to int64.  Runs of consecutive lanes in
+// one box are summed through LDS, each run is added into the workgroup's LDS table slot of its box
+// (open addressing, LDS atomics), and the table goes to the grid with one 64-bit global atomic per
+// (box, node, term).  A run whose box finds no free slot adds to the grid directly.  All sums are exact
+// integers, so the grid, and so the whole gradient, is bitwise repeatable whatever the order of the
+// points.  Grid layout: int64 [u][v][4] (A, c, b_x, b_y), u the x node (nbodyfft.cpp:137-138); the row
+// FFT reads it, converts it and clears it for the next iteration.
 constexpr double kFixScale = 274877906944.0;  // 2^38
+constexpr int kSpreadThreads = 256;
+constexpr int kSpreadPts = 1024;
 
 template <int P, int D>
-__global__ __launch_bounds__(kWave) void k_spread_f32(const float *__restrict__ Y, const int *__restrict__ sorted,
-                                                      const int *__restrict__ boxof, int N, Geo g,
-                                                      unsigned long long *__restrict__ grid) {
+constexpr int spread_slots() {
+    constexpr int per = (D == 2 ? P * P * 4 : P * 3) * 8;  // bytes of one table slot
+    constexpr int s = (40 * 1024) / per;
+    return s >= 256 ? 256 : s;
+}
+
+template <int P, int D>
+__global__ __launch_bounds__(kSpreadThreads) void k_spread_f32(const float *__restrict__ Y, int N, Geo g,
+                                                               unsigned long long *__restrict__ grid) {
     constexpr int NV = (D == 2 ? P * P : P);
-    __shared__ long long sh[NV][kWave + 1];
-    __shared__ int hl[kWave + 1];
-    __shared__ int hb[kWave];
-    const int lane = threadIdx.x;
-    const int k = blockIdx.x * kWave + lane;
-    const bool ok = k < N;
-    int b = -1 - lane;
-    long long val[NV][4];
+    constexpr int NQ = (D == 2 ? 4 : 3);
+    constexpr int NS = spread_slots<P, D>();
+    constexpr int WPB = kSpreadThreads / kWave;
+    __shared__ int tkey[NS];
+    __shared__ unsigned long long tval[NS][NV * NQ];
+    __shared__ long long sh[WPB][NV][kWave + 1];
+    __shared__ int hl[WPB][kWave + 1];
+    __shared__ int hb[WPB][kWave];
+    __shared__ int hs[WPB][kWave];
+    const int t = threadIdx.x, lane = t % kWave, wv = t / kWave;
+    for (int o = t; o < NS; o += kSpreadThreads) tkey[o] = -1;
+    for (int o = t; o < NS * NV * NQ; o += kSpreadThreads) (&tval[0][0])[o] = 0ull;
+    __syncthreads();
+    for (int chunk = 0; chunk < kSpreadPts / kSpreadThreads; chunk++) {
+        const long i = (long)blockIdx.x * kSpreadPts + (long)chunk * kSpreadThreads + t;
+        const bool ok = i < N;
+        int b = -1 - lane;  // lanes past N: distinct negative keys, never a real run
+        long long val[NV][NQ];
 #pragma unroll
-    for (int v = 0; v < NV; v++)
+        for (int v = 0; v < NV; v++)
 #pragma unroll
-        for (int q = 0; q < 4; q++) val[v][q] = 0;
-    if (ok) {
-        const int i = sorted[k];
-        b = boxof[i];
-        const int bx = D == 2 ? b % g.nb : b;
-        const int by = D == 2 ? b / g.nb : 0;
-        const float S = (float)kFixScale;
-        float y[D];
-        load_pt<float, D>(Y, i, y);
-        const double tx = ((double)y[0] - (bx * g.bw0 + g.mn)) / g.bw;
-        float Lx[P], dx[P];
+            for (int q = 0; q < NQ; q++) val[v][q] = 0;
+        if (ok) {
+            const float S = (float)kFixScale;
+            float y[D];
+            load_pt<float, D>(Y, i, y);
+            double tx;
+            b = box_coord<D>((double)y[0], g, tx);
+            float Lx[P], dx[P];
 #pragma unroll
-        for (int a = 0; a < P; a++) dx[a] = (float)(tx - g.sp[a]);
-#pragma unroll
-        for (int a = 0; a < P; a++) {
-            float v = 1.f;
-#pragma unroll
-            for (int c = 0; c < P; c++)
-                if (c != a) v *= dx[c];
-            Lx[a] = v / (float)g.den[a];
-        }
-        if constexpr (D == 2) {
-            const double ty = ((double)y[1] - (by * g.bw0 + g.mn)) / g.bw;
-            float Ly[P], dy[P];
-#pragma unroll
-            for (int a = 0; a < P; a++) dy[a] = (float)(ty - g.sp[a]);
+            for (int a = 0; a < P; a++) dx[a] = (float)(tx - g.sp[a]);
 #pragma unroll
             for (int a = 0; a < P; a++) {
                 float v = 1.f;
 #pragma unroll
                 for (int c = 0; c < P; c++)
-                    if (c != a) v *= dy[c];
-                Ly[a] = v / (float)g.den[a];
+                    if (c != a) v *= dx[c];
+                Lx[a] = v / (float)g.den[a];
             }
+            if constexpr (D == 2) {
+                double ty;
+                b += box_coord<D>((double)y[1], g, ty) * g.nb;
+                float Ly[P], dy[P];
 #pragma unroll
-            for (int a = 0; a < P; a++)
+                for (int a = 0; a < P; a++) dy[a] = (float)(ty - g.sp[a]);
 #pragma unroll
-                for (int c = 0; c < P; c++) {
-                    const float w = Lx[a] * Ly[c];
-                    val[a * P + c][0] = (long long)(w * S);
-                    val[a * P + c][1] = (long long)(w * (dx[a] * dx[a] + dy[c] * dy[c]) * S);
-                    val[a * P + c][2] = (long long)(w * dx[a] * S);
-                    val[a * P + c][3] = (long long)(w * dy[c] * S);
+                for (int a = 0; a < P; a++) {
+                    float v = 1.f;
+#pragma unroll
+                    for (int c = 0; c < P; c++)
+                        if (c != a) v *= dy[c];
+                    Ly[a] = v / (float)g.den[a];
                 }
-        } else {
 #pragma unroll
-            for (int a = 0; a < P; a++) {
-                val[a][0] = (long long)(Lx[a] * S);
-                val[a][1] = (long long)(Lx[a] * dx[a] * dx[a] * S);
-                val[a][2] = (long long)(Lx[a] * dx[a] * S);
+                for (int a = 0; a < P; a++)
+#pragma unroll
+                    for (int c = 0; c < P; c++) {
+                        const float w = Lx[a] * Ly[c];
+                        val[a * P + c][0] = (long long)(w * S);
+                        val[a * P + c][1] = (long long)(w * (dx[a] * dx[a] + dy[c] * dy[c]) * S);
+                        val[a * P + c][2] = (long long)(w * dx[a] * S);
+                        val[a * P + c][3] = (long long)(w * dy[c] * S);
+                    }
+            } else {
+#pragma unroll
+                for (int a = 0; a < P; a++) {
+                    val[a][0] = (long long)(Lx[a] * S);
+                    val[a][1] = (long long)(Lx[a] * dx[a] * dx[a] * S);
+                    val[a][2] = (long long)(Lx[a] * dx[a] * S);
+                }
             }
         }
-    }
-    // runs of equal boxes (the chunk is box-sorted, so a run is all of the chunk's points of one box)
-    const int prev = __shfl_up(b, 1, kWave);
-    const bool head = lane == 0 || b != prev;
-    const unsigned long long heads = __ballot(head);
-    const int nruns = __popcll(heads);
-    if (head) {
-        const int r = __popcll(heads & ((1ull << lane) - 1ull));
-        hl[r] = lane;
-        hb[r] = b;
-    }
-    if (lane == 0) hl[nruns] = kWave;
-    // one charge term per round keeps the LDS footprint small (occupancy)
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-#pragma unroll
-        for (int v = 0; v < NV; v++) sh[v][lane] = val[v][q];
-        __syncthreads();
-        for (int o = lane; o < nruns * NV; o += kWave) {
-            const int r = o / NV, v = o - r * NV;
-            const int box = hb[r];
-            if (box < 0) continue;  // run of padding lanes past N
-            const int l0 = hl[r], l1 = hl[r + 1];
-            long long sum = 0;
-            for (int l = l0; l < l1; l++) sum += sh[v][l];
-            if (sum == 0) continue;
-            const int bx = D == 2 ? box % g.nb : box, by = D == 2 ? box / g.nb : 0;
-            const long node = D == 2 ? (long)(bx * P + v / P) * g.n1d + by * P + v % P : (long)bx * P + v;
-            atomicAdd(&grid[node * 4 + q], (unsigned long long)sum);
+        // runs of equal boxes among consecutive lanes; each run head claims its box's table slot
+        const int prev = __shfl_up(b, 1, kWave);
+        const bool head = lane == 0 || b != prev;
+        const unsigned long long heads = __ballot(head);
+        const int nruns = __popcll(heads);
+        if (head) {
+            const int r = __popcll(heads & ((1ull << lane) - 1ull));
+            hl[wv][r] = lane;
+            hb[wv][r] = b;
+            int slot = -1;
+            if (b >= 0) {
+                int h = b % NS;
+                for (int probe = 0; probe < NS; probe++) {
+                    const int k = atomicCAS(&tkey[h], -1, b);
+                    if (k == -1 || k == b) {
+                        slot = h;
+                        break;
+                    }
+                    h = h + 1 == NS ? 0 : h + 1;
+                }
+            }
+            hs[wv][r] = slot;
         }
-        __syncthreads();
+        if (lane == 0) hl[wv][nruns] = kWave;
+        // one charge term per round keeps the LDS footprint small
+#pragma unroll
+        for (int q = 0; q < NQ; q++) {
+#pragma unroll
+            for (int v = 0; v < NV; v++) sh[wv][v][lane] = val[v][q];
+            __syncthreads();
+            for (int o = lane; o < nruns * NV; o += kWave) {
+                const int r = o / NV, v = o - r * NV;
+                const int box = hb[wv][r];
+                if (box < 0) continue;  // run of lanes past N
+                const int l0 = hl[wv][r], l1 = hl[wv][r + 1];
+                long long sum = 0;
+                for (int l = l0; l < l1; l++) sum += sh[wv][v][l];
+                if (sum == 0) continue;
+                const int slot = hs[wv][r];
+                if (slot >= 0) {
+                    atomicAdd(&tval[slot][v * NQ + q], (unsigned long long)sum);
+                } else {
+                    const int bx = D == 2 ? box % g.nb : box, by = D == 2 ? box / g.nb : 0;
+                    const long node =
+                        D == 2 ? (long)(bx * P + v / P) * g.n1d + by * P + v % P : (long)bx * P + v;
+                    atomicAdd(&grid[node * 4 + q], (unsigned long long)sum);
+                }
+            }
+            __syncthreads();
+        }
+    }
+    // table -> grid
+    for (int o = t; o < NS * NV * NQ; o += kSpreadThreads) {
+        const int slot = o / (NV * NQ), r = o - slot * (NV * NQ);
+        const int box = tkey[slot];
+        if (box < 0) continue;
+        const unsigned long long sum = tval[slot][r];
+        if (sum == 0ull) continue;
+        const int v = r / NQ, q = r - v * NQ;
+        const int bx = D == 2 ? box % g.nb : box, by = D == 2 ? box / g.nb : 0;
+        const long node = D == 2 ? (long)(bx * P + v / P) * g.n1d + by * P + v % P : (long)bx * P + v;
+        atomicAdd(&grid[node * 4 + q], sum);
     }
 }
 
