@@ -13,6 +13,7 @@
 
 #include "../../include/fitsne_abi.h"
 #include "fitsne_kernels.hpp"
+#include "fitsne_knn.hpp"
 
 using namespace fitsne;
 
@@ -178,6 +179,8 @@ struct fitsne_ctx {
     // embedding state (element type = float for FP32, double for FP64 plain)
     DBuf Y, uY, gains, pos, dC;
     // sort (FP32)
+    // exact kNN (affinity build)
+    DBuf knnX, knnXT, knnI, knnD;
     // windowed copy of P for the FP32 attractive term (k_attr_win), built on first use
     DBuf wcount, wstart, wkey, wval, wbmax;
     bool win_ok = false;
@@ -818,7 +821,7 @@ extern "C" int fitsne_ctx_destroy(fitsne_ctx *c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     DBuf *all[] = {&c->rowP, &c->colP, &c->val, &c->Y, &c->uY, &c->gains, &c->pos, &c->dC, &c->wcount, &c->wstart,
-                   &c->wkey, &c->wval, &c->wbmax, &c->gin, &c->gout, &c->spec, &c->ospec, &c->Trow, &c->KQ,
+                   &c->wkey, &c->wval, &c->wbmax, &c->knnX, &c->knnXT, &c->knnI, &c->knnD, &c->gin, &c->gout, &c->spec, &c->ospec, &c->Trow, &c->KQ,
                    &c->K1, &c->KS, &c->KT, &c->zpart, &c->zbuf, &c->part, &c->mean, &c->mmpart, &c->mm, &c->xs,
                    &c->ys, &c->box, &c->wx, &c->wy, &c->ch, &c->w, &c->V, &c->potA, &c->potB,
                    &c->inv_d, &c->colP2, &c->val2, &c->tmp, &c->fixgrid};
@@ -1557,5 +1560,34 @@ extern "C" int fitsne_distances2similarities_batch(const double *D, double *P, i
     HIPTRY(hipMemcpyAsync(P, c->V.p, sizeof(double) * nk, hipMemcpyDeviceToHost, c->stream));
     HIPTRY(hipMemcpyAsync(beta, c->xs.p, sizeof(double) * n_points, hipMemcpyDeviceToHost, c->stream));
     if (iters) HIPTRY(hipMemcpyAsync(iters, c->box.p, sizeof(int) * n_points, hipMemcpyDeviceToHost, c->stream));
+    return sync_ctx(c);
+}
+
+extern "C" int fitsne_knn_exact(const double *X, int N, int D, int K, int32_t *idx, double *dist) {
+    if (!X || !idx || !dist || N < 2 || D < 1 || K < 1) return fail(FITSNE_E_ARG, "invalid kNN arguments");
+    if (K > N - 1) return fail(FITSNE_E_ARG, "K must be at most N - 1");
+    if (K > 1000) return fail(FITSNE_E_UNSUPPORTED, "K above 1000 is not supported");
+    const size_t nd = (size_t)N * D;
+    for (size_t i = 0; i < nd; i++)
+        if (!std::isfinite(X[i])) return fail(FITSNE_E_ARG, "X contains non-finite values");
+    std::lock_guard<std::mutex> lk(g_mirror_mu);
+    fitsne_ctx *c;
+    TRY(mirror_ctx(&c, 1, 3, 1.0, 1, 1.0));
+    HIPTRY(hipSetDevice(c->device));
+    TRY(ensure(c->knnX, sizeof(double) * nd));
+    TRY(ensure(c->knnXT, sizeof(double) * nd));
+    TRY(ensure(c->knnI, sizeof(int) * (size_t)N * K));
+    TRY(ensure(c->knnD, sizeof(double) * (size_t)N * K));
+    HIPTRY(hipMemcpy(c->knnX.p, X, sizeof(double) * nd, hipMemcpyHostToDevice));
+    launch(c, "transpose64", k_transpose_f64, dim3((N + 31) / 32, (D + 31) / 32), dim3(256), 0,
+           c->knnX.as<const double>(), (long)N, D, c->knnXT.as<double>());
+    int B = 128;
+    while (B < 2 * K || B < K + kWave) B *= 2;
+    const size_t lds = (size_t)kKnnQ * B * (sizeof(double) + sizeof(int));
+    launch(c, "knn64", k_knn_f64, dim3((N + kKnnQ - 1) / kKnnQ), dim3(kWave), lds, c->knnX.as<const double>(),
+           c->knnXT.as<const double>(), N, D, K, B, c->knnI.as<int>(), c->knnD.as<double>());
+    HIPTRY(hipGetLastError());
+    HIPTRY(hipMemcpyAsync(idx, c->knnI.p, sizeof(int) * (size_t)N * K, hipMemcpyDeviceToHost, c->stream));
+    HIPTRY(hipMemcpyAsync(dist, c->knnD.p, sizeof(double) * (size_t)N * K, hipMemcpyDeviceToHost, c->stream));
     return sync_ctx(c);
 }

@@ -47,6 +47,7 @@ SIGNATURES = {
     "fitsne_interpolate": (_ci, [_ci, _ci, _dp, _dp, _dp]),
     "fitsne_evaluate_error_fft": (_ci, [_u32, _u32, _dp, _dp, _ci, _ci, _cd, _cd, C.POINTER(_cd)]),
     "fitsne_distances2similarities_batch": (_ci, [_dp, _dp, _ci, _ci, _cd, _cd, _ci, _vp, _dp, _vp]),
+    "fitsne_knn_exact": (_ci, [_dp, _ci, _ci, _ci, _i32, _dp]),
     "fitsne_ctx_create": (_ci, [C.POINTER(_vp), _ci, _ci, _ci, _ci, _cd, _ci, _cd]),
     "fitsne_ctx_destroy": (_ci, [_vp]),
     "fitsne_set_P": (_ci, [_vp, _ci, _u32, _u32, _dp]),
@@ -192,6 +193,16 @@ def distances2similarities(Dist, perplexity, sigma=-1.0, squared=False, beta0=No
     check(lib().fitsne_distances2similarities_batch(Dist.reshape(-1), P.reshape(-1), n, K, perplexity, sigma,
                                                     int(squared), _ptr(b0), beta, _ptr(iters)))
     return P, beta, iters
+
+
+def knn_exact(X, K):
+    """Exact K nearest other rows of X (FP64), ascending by (distance, index): (idx[N,K], dist[N,K])."""
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    N, D = X.shape
+    idx = np.empty((N, K), dtype=np.int32)
+    dist = np.empty((N, K))
+    check(lib().fitsne_knn_exact(X.reshape(-1), N, D, K, idx.reshape(-1), dist.reshape(-1)))
+    return idx, dist
 
 
 # ---------------------------------------------------------------- resident tier

@@ -76,6 +76,16 @@ def c2_workload(N=1_000_000, D=50, k=50, per_point=45, seed=20261204):
     return (row_P, col_P, val_P), Y, labels
 
 
+def c1_data(N=10_000, D=50, k=10, seed=20261202):
+    """C1: N points in D dimensions, k equal-weight isotropic Gaussians, centres N(0, 4^2 I), sd 1."""
+    rng = np.random.default_rng(seed)
+    labels = np.arange(N) % k
+    rng.shuffle(labels)
+    centres = rng.normal(0.0, 4.0, size=(k, D))
+    X = centres[labels] + rng.normal(0.0, 1.0, size=(N, D))
+    return X, labels
+
+
 def small_mixture_workload(N=10_000, D=2, k=10, per_point=30, span=40.0, seed=7):
     """A scaled-down C2-like workload for quick checks (same construction)."""
     rng = np.random.default_rng(seed)

@@ -116,6 +116,14 @@ int fitsne_distances2similarities_batch(const double *D, double *P, int n_points
                                         double sigma, int ifSquared, const double *beta0, double *beta,
                                         int *iters);
 
+/* Exact K nearest neighbours of every row of X[N*D] (row-major) among the other rows, for the
+ * affinity build of TSNE::run: replaces the VP-tree search of computeGaussianPerplexity
+ * (tsne.cpp:1643-1726; Annoy's approximate search, tsne.cpp:1531-1640, is served by the same exact
+ * search).  idx[N*K] and dist[N*K] per row ascending by (distance, index); the distance is
+ * sqrt(sum_d (x_d - y_d)^2) in order d = 0..D-1 without fused multiply-adds (vptree.h:97-107).
+ * K <= min(N - 1, 1000); X must be finite. */
+int fitsne_knn_exact(const double *X, int N, int D, int K, int32_t *idx, double *dist);
+
 /* ====================================================================================== */
 /* Resident tier                                                                           */
 /* ====================================================================================== */

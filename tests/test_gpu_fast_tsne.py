@@ -1,0 +1,88 @@
+"""The drop-in program and its affinity build on the GPU, against the oracle.
+
+* fitsne_knn_exact vs the oracle's exact kNN (the reference's VP-tree search returns the exact
+  neighbours; its distance is vptree.h:97-107): same indices, same distances.
+* bin/fast_tsne with load_affinities='save' writes P_row/P_col/P_val.dat; they must equal the oracle's
+  computeGaussianPerplexity + symmetrizeMatrix + normalisation (tsne.cpp:151-330) on the same data.
+* C1 end to end (N = 10,000, 1000 iterations): the program's final KL within 0.5% of the oracle's run
+  of TSNE::run on the same data and initialisation (tests/golden/c1_oracle_kl.json, made by
+  tests/golden/make_c1_golden.py), in the FP32 form and the FP64 plain path.
+"""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import fast_tsne as ft
+import fitsne_gpu as fg
+import orc
+import synth
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "golden"))
+import make_c1_golden as c1  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def _note(msg):
+    print(f"[fast_tsne test] {msg}", flush=True)
+
+
+def test_knn_exact_matches_the_oracle():
+    rng = np.random.default_rng(11)
+    X = np.concatenate([rng.normal(c, 1.0, size=(700, 20)) for c in rng.normal(0, 3, size=(4, 20))])
+    X[5] = X[9]  # one exact duplicate: a zero distance, ties broken by the smaller index
+    K = 45
+    _note(f"knn N={X.shape[0]} D=20 K={K}")
+    idx, dist = fg.knn_exact(X, K)
+    ridx, rdist = orc.knn_exact(X, K)
+    np.testing.assert_array_equal(idx, ridx)
+    np.testing.assert_array_equal(dist, rdist)
+    assert idx[5, 0] == 9 and dist[5, 0] == 0.0 and idx[9, 0] == 5
+
+
+def _run_program(tmp_path, X, Y0, max_iter, precision=None, **kw):
+    env_before = os.environ.get("FITSNE_PRECISION")
+    cwd = os.getcwd()
+    try:
+        if precision:
+            os.environ["FITSNE_PRECISION"] = precision
+        os.chdir(tmp_path)
+        return ft.fast_tsne(X, perplexity=30, max_iter=max_iter, knn_algo="vp-tree", initialization=Y0,
+                            return_loss=True, **kw)
+    finally:
+        os.chdir(cwd)
+        if env_before is None:
+            os.environ.pop("FITSNE_PRECISION", None)
+        else:
+            os.environ["FITSNE_PRECISION"] = env_before
+
+
+def test_program_affinities_match_the_oracle(tmp_path):
+    X, _ = synth.c1_data(N=2000, D=30, k=6, seed=5)
+    Y0 = c1.c1_init(X)
+    _note("program N=2000 load_affinities=save max_iter=2")
+    _run_program(tmp_path, X, Y0, max_iter=2, load_affinities="save")
+    row = np.fromfile(tmp_path / "P_row.dat", dtype=np.uint32)
+    col = np.fromfile(tmp_path / "P_col.dat", dtype=np.uint32)
+    val = np.fromfile(tmp_path / "P_val.dat", dtype=np.float64)
+    rrow, rcol, rval = c1.c1_affinities(X)
+    np.testing.assert_array_equal(row, rrow)
+    np.testing.assert_array_equal(col, rcol)
+    np.testing.assert_allclose(val, rval, rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_c1_final_kl_within_half_percent_of_the_oracle(tmp_path, precision):
+    gold = json.load(open(os.path.join(HERE, "golden", "c1_oracle_kl.json")))
+    X, _ = synth.c1_data()
+    Y0 = c1.c1_init(X)
+    _note(f"program C1 N=10000 1000 iterations {precision}")
+    Y, loss = _run_program(tmp_path, X, Y0, max_iter=1000, precision=precision)
+    kl = loss[np.arange(49, 1000, 50)]
+    _note(f"{precision}: KL {kl[[0, 4, 9, -1]]} vs oracle {np.array(gold['kl'])[[0, 4, 9, -1]]}")
+    assert np.isfinite(Y).all() and Y.shape == (10000, 2)
+    assert abs(kl[-1] - gold["kl"][-1]) / gold["kl"][-1] < 5e-3
