@@ -14,6 +14,7 @@
 #include "../../include/fitsne_abi.h"
 #include "fitsne_kernels.hpp"
 #include "fitsne_knn.hpp"
+#include "fitsne_mlp.hpp"
 
 using namespace fitsne;
 
@@ -537,9 +538,9 @@ static int pipeline_f32(fitsne_ctx *c, int mode, const fitsne_step_params *sp) {
         const int gr_rows = 8 * ((g.n1d + 7) / 8), gr_cols = 8 * ((g.H + 7) / 8);
         if (!(c->kcache && c->kc_mn == mn && c->kc_mx == mx && c->kc_nb == g.nb)) {
             launch(c, "kern_rows", k_kern_rows<float, kKernThreads, kKernTot>, dim3(gr_rows), dim3(kKernThreads),
-                   sizeof(cx<float>) * g.Mp, g, pl, tw, kind_q, kind_1, c->df, c->Trow.as<cx<float>>());
+                   sizeof(cx<float>) * (1 + 1) * g.Mp, g, pl, tw, kind_q, kind_1, c->df, c->Trow.as<cx<float>>());
             launch(c, "kern_cols", k_kern_cols<float, kKernThreads, kKernTot>, dim3(gr_cols), dim3(kKernThreads),
-                   sizeof(cx<float>) * g.Mp, g, pl, tw, c->Trow.as<const cx<float>>(), 1, c->KQ.as<float>(),
+                   sizeof(cx<float>) * (1 + 1) * g.Mp, g, pl, tw, c->Trow.as<const cx<float>>(), 1, c->KQ.as<float>(),
                    c->K1.as<float>(), c->KS.as<float>(), c->KT.as<float>());
             c->kcache = true;
             c->kc_mn = mn;
@@ -547,16 +548,16 @@ static int pipeline_f32(fitsne_ctx *c, int mode, const fitsne_step_params *sp) {
             c->kc_nb = g.nb;
         }
         launch(c, "rows_fwd", k_rows_fwd<float, kRowThreads, kRowTot>, dim3(gr_rows), dim3(kRowThreads),
-               sizeof(cx<float>) * 2 * g.Mp, g, pl, tw, (const float *)nullptr, 4, 4, c->spec.as<cx<float>>(),
+               sizeof(cx<float>) * (2 + 1) * g.Mp, g, pl, tw, (const float *)nullptr, 4, 4, c->spec.as<cx<float>>(),
                c->fixgrid.as<unsigned long long>());
         launch(c, "cols_f32", k_cols_f32<kColThreads, kColTot32>, dim3(gr_cols), dim3(kColThreads),
-               sizeof(cx<float>) * 4 * g.Mp, g, pl, tw, c->spec.as<const cx<float>>(), c->KQ.as<const float>(),
+               sizeof(cx<float>) * (4 + 1) * g.Mp, g, pl, tw, c->spec.as<const cx<float>>(), c->KQ.as<const float>(),
                c->K1.as<const float>(), c->KS.as<const float>(), c->KT.as<const float>(), c->ospec.as<cx<float>>(),
                c->zpart.as<double>());
         launch(c, "zfinal", k_zfinal, dim3(1), dim3(kThreads), 0, c->zpart.as<const double>(), g.H, (int)df1,
                1.0 / ((double)g.Mp * g.Mp), (double)N, c->zbuf.as<double>());
         launch(c, "rows_inv", k_rows_inv<float, kRowThreads, kRowTot>, dim3(gr_rows), dim3(kRowThreads),
-               sizeof(cx<float>) * 2 * g.Mp, g, pl, tw, c->ospec.as<const cx<float>>(), 3,
+               sizeof(cx<float>) * (2 + 1) * g.Mp, g, pl, tw, c->ospec.as<const cx<float>>(), 3,
                (float)(1.0 / ((double)g.Mp * g.Mp)), c->gout.as<float>(), 4);
     } else {
         TRY(ensure(c->zpart, sizeof(double) * 4));
@@ -631,14 +632,14 @@ static int plain_pass(fitsne_ctx *c, const Geo &g, const FftPlan &pl, const cx<d
         const int gr_rows = 8 * ((g.n1d + 7) / 8), gr_cols = 8 * ((g.H + 7) / 8);
         const int lines = (nt + 1) / 2;
         launch(c, "rows_fwd64", k_rows_fwd<double, kRowThreads, kRowTot>, dim3(gr_rows), dim3(kRowThreads),
-               sizeof(cx<double>) * lines * g.Mp, g, pl, tw, c->w.as<const double>(), nt, nt, c->spec.as<cx<double>>(),
+               sizeof(cx<double>) * (lines + 1) * g.Mp, g, pl, tw, c->w.as<const double>(), nt, nt, c->spec.as<cx<double>>(),
                (unsigned long long *)nullptr);
         const int cl = std::min(nt, kColTot64 / g.Mp);
         launch(c, "cols_plain64", k_cols_plain<double, kColThreads64, kColTot64>, dim3(gr_cols), dim3(kColThreads64),
-               sizeof(cx<double>) * cl * g.Mp, g, pl, tw, c->spec.as<const cx<double>>(), nt, KC,
+               sizeof(cx<double>) * (cl + 1) * g.Mp, g, pl, tw, c->spec.as<const cx<double>>(), nt, KC,
                c->ospec.as<cx<double>>());
         launch(c, "rows_inv64", k_rows_inv<double, kRowThreads, kRowTot>, dim3(gr_rows), dim3(kRowThreads),
-               sizeof(cx<double>) * lines * g.Mp, g, pl, tw, c->ospec.as<const cx<double>>(), nt,
+               sizeof(cx<double>) * (lines + 1) * g.Mp, g, pl, tw, c->ospec.as<const cx<double>>(), nt,
                1.0 / ((double)g.Mp * g.Mp), c->V.as<double>(), nt);
     } else {
         launch(c, "conv1d_f64", k_conv1d<double, kConv1dThreads, kMaxTot64>, dim3(1), dim3(kConv1dThreads),
@@ -690,10 +691,10 @@ static int pipeline_f64(fitsne_ctx *c, int mode, const fitsne_step_params *sp) {
         TRY(ensure(c->K1, sizeof(double) * HH));
         const int gr_rows = 8 * ((g.n1d + 7) / 8), gr_cols = 8 * ((g.H + 7) / 8);
         launch(c, "kern_rows64", k_kern_rows<double, kKernThreads, kKernTot>, dim3(gr_rows), dim3(kKernThreads),
-               sizeof(cx<double>) * g.Mp, g, pl, tw, kind_q, (int)FITSNE_KERNEL_GENERAL, c->df,
+               sizeof(cx<double>) * (1 + 1) * g.Mp, g, pl, tw, kind_q, (int)FITSNE_KERNEL_GENERAL, c->df,
                c->Trow.as<cx<double>>());
         launch(c, "kern_cols64", k_kern_cols<double, kKernThreads, kKernTot>, dim3(gr_cols), dim3(kKernThreads),
-               sizeof(cx<double>) * g.Mp, g, pl, tw, c->Trow.as<const cx<double>>(), 0, c->KQ.as<double>(),
+               sizeof(cx<double>) * (1 + 1) * g.Mp, g, pl, tw, c->Trow.as<const cx<double>>(), 0, c->KQ.as<double>(),
                c->K1.as<double>(), (double *)nullptr, (double *)nullptr);
         KQ = c->KQ.as<const double>();
         K1 = c->K1.as<const double>();
@@ -1307,9 +1308,9 @@ extern "C" int fitsne_precompute_2d(double x_max, double x_min, double y_max, do
     TRY(ensure(c->K1, sizeof(double) * HH));
     const int gr_rows = 8 * ((g.n1d + 7) / 8), gr_cols = 8 * ((g.H + 7) / 8);
     launch(c, "kern_rows64", k_kern_rows<double, kKernThreads, kKernTot>, dim3(gr_rows), dim3(kKernThreads),
-           sizeof(cx<double>) * g.Mp, g, pl, tw, kernel, kernel, df, c->Trow.as<cx<double>>());
+           sizeof(cx<double>) * (1 + 1) * g.Mp, g, pl, tw, kernel, kernel, df, c->Trow.as<cx<double>>());
     launch(c, "kern_cols64", k_kern_cols<double, kKernThreads, kKernTot>, dim3(gr_cols), dim3(kKernThreads),
-           sizeof(cx<double>) * g.Mp, g, pl, tw, c->Trow.as<const cx<double>>(), 0, c->KQ.as<double>(),
+           sizeof(cx<double>) * (1 + 1) * g.Mp, g, pl, tw, c->Trow.as<const cx<double>>(), 0, c->KQ.as<double>(),
            c->K1.as<double>(), (double *)nullptr, (double *)nullptr);
     HIPTRY(hipMemcpyAsync(fft_kernel_tilde, c->KQ.p, sizeof(double) * HH, hipMemcpyDeviceToHost, c->stream));
     return sync_ctx(c);
@@ -1381,14 +1382,14 @@ extern "C" int fitsne_n_body_fft_2d(int N, int n_terms, const double *xs, const 
     const int gr_rows = 8 * ((g.n1d + 7) / 8), gr_cols = 8 * ((g.H + 7) / 8);
     const int lines = (nt + 1) / 2;
     launch(c, "rows_fwd64", k_rows_fwd<double, kRowThreads, kRowTot>, dim3(gr_rows), dim3(kRowThreads),
-           sizeof(cx<double>) * lines * g.Mp, g, pl, tw, c->w.as<const double>(), nt, nt, c->spec.as<cx<double>>(),
+           sizeof(cx<double>) * (lines + 1) * g.Mp, g, pl, tw, c->w.as<const double>(), nt, nt, c->spec.as<cx<double>>(),
            (unsigned long long *)nullptr);
     const int cl = std::min(nt, kColTot64 / g.Mp);
     launch(c, "cols_plain64", k_cols_plain<double, kColThreads64, kColTot64>, dim3(gr_cols), dim3(kColThreads64),
-           sizeof(cx<double>) * cl * g.Mp, g, pl, tw, c->spec.as<const cx<double>>(), nt, c->KQ.as<const double>(),
+           sizeof(cx<double>) * (cl + 1) * g.Mp, g, pl, tw, c->spec.as<const cx<double>>(), nt, c->KQ.as<const double>(),
            c->ospec.as<cx<double>>());
     launch(c, "rows_inv64", k_rows_inv<double, kRowThreads, kRowTot>, dim3(gr_rows), dim3(kRowThreads),
-           sizeof(cx<double>) * lines * g.Mp, g, pl, tw, c->ospec.as<const cx<double>>(), nt,
+           sizeof(cx<double>) * (lines + 1) * g.Mp, g, pl, tw, c->ospec.as<const cx<double>>(), nt,
            1.0 / ((double)g.Mp * g.Mp), c->V.as<double>(), nt);
     launch(c, "interp_f64", pick_interp_f64<2>(p), dim3(blocks_for(N)), dim3(kThreads), 0, c->box.as<const int>(),
            c->wx.as<const double>(), c->wy.as<const double>(), c->V.as<const double>(), nt, N, g, c->potA.as<double>());
@@ -1590,4 +1591,249 @@ extern "C" int fitsne_knn_exact(const double *X, int N, int D, int K, int32_t *i
     HIPTRY(hipMemcpyAsync(idx, c->knnI.p, sizeof(int) * (size_t)N * K, hipMemcpyDeviceToHost, c->stream));
     HIPTRY(hipMemcpyAsync(dist, c->knnD.p, sizeof(double) * (size_t)N * K, hipMemcpyDeviceToHost, c->stream));
     return sync_ctx(c);
+}
+
+// ============================================================================ learned β warm start
+
+struct fitsne_beta_model {
+    int device = 0, K = 0;
+    hipStream_t stream = nullptr;
+    DBuf prm, m1, m2, grad, gpart, lpart, w1t, w2t, w2, X, tgt, dist, beta0;
+    long steps = 0;
+};
+
+namespace {
+
+uint64_t splitmix64(uint64_t &x) {
+    uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+double gauss(uint64_t &st) {
+    const double u1 = ((splitmix64(st) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+    const double u2 = ((splitmix64(st) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+    return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+}
+
+int mlp_sync(fitsne_beta_model *m) {
+    HIPTRY(hipStreamSynchronize(m->stream));
+    return FITSNE_OK;
+}
+
+// features of dist[n][K] into m->X (bf16 [n][96]) with the model's mu / sd
+int mlp_features(fitsne_beta_model *m, const double *dist, long n) {
+    TRY(ensure(m->dist, sizeof(double) * (size_t)n * m->K));
+    TRY(ensure(m->X, sizeof(unsigned short) * (size_t)n * kMlpIn));
+    HIPTRY(hipMemcpyAsync(m->dist.p, dist, sizeof(double) * (size_t)n * m->K, hipMemcpyHostToDevice, m->stream));
+    const long tot = n * kMlpIn;
+    k_mlp_features<<<dim3((unsigned)((tot + kThreads - 1) / kThreads)), dim3(kThreads), 0, m->stream>>>(
+        m->dist.as<const double>(), (int)n, m->K, m->prm.as<const float>(), m->X.as<unsigned short>());
+    HIPTRY(hipGetLastError());
+    return FITSNE_OK;
+}
+
+int mlp_pack(fitsne_beta_model *m) {
+    k_mlp_pack<<<dim3((kMlpHid * kMlpIn + kThreads - 1) / kThreads), dim3(kThreads), 0, m->stream>>>(
+        m->prm.as<const float>(), m->w1t.as<unsigned short>(), m->w2t.as<unsigned short>(), m->w2.as<unsigned short>());
+    HIPTRY(hipGetLastError());
+    return FITSNE_OK;
+}
+
+// gradient over X rows [off, off + n) with targets tgt[off ..]; optional Adam step
+int mlp_grad_step(fitsne_beta_model *m, long off, long n, bool step, float lr, int *nparts_out) {
+    const long ntiles = (n + 63) / 64;
+    const int G = (int)std::min<long>(ntiles, 256);
+    TRY(ensure(m->gpart, sizeof(float) * (size_t)G * kMlpTrainN));
+    TRY(ensure(m->lpart, sizeof(double) * (size_t)G));
+    TRY(mlp_pack(m));
+    k_mlp_grad<<<dim3(G), dim3(kMlpThreads), 0, m->stream>>>(
+        m->X.as<const unsigned short>() + off * kMlpIn, m->tgt.as<const float>() + off, n, 1.0f / (float)n,
+        m->w1t.as<const unsigned short>(), m->w2t.as<const unsigned short>(), m->w2.as<const unsigned short>(),
+        m->prm.as<const float>(), m->gpart.as<float>(), m->lpart.as<double>());
+    HIPTRY(hipGetLastError());
+    float c1 = 1.f, c2 = 1.f;
+    if (step) {
+        m->steps++;
+        c1 = (float)(1.0 - std::pow(0.9, (double)m->steps));
+        c2 = (float)(1.0 - std::pow(0.999, (double)m->steps));
+    }
+    k_mlp_reduce_adam<<<dim3((kMlpTrainN + kThreads - 1) / kThreads), dim3(kThreads), 0, m->stream>>>(
+        m->gpart.as<const float>(), G, m->grad.as<float>(), m->prm.as<float>(), m->m1.as<float>(), m->m2.as<float>(),
+        step ? 1 : 0, lr, c1, c2);
+    HIPTRY(hipGetLastError());
+    if (nparts_out) *nparts_out = G;
+    return FITSNE_OK;
+}
+
+int mlp_loss(fitsne_beta_model *m, int G, long n, double *loss) {
+    std::vector<double> lp(G);
+    HIPTRY(hipMemcpyAsync(lp.data(), m->lpart.p, sizeof(double) * G, hipMemcpyDeviceToHost, m->stream));
+    TRY(mlp_sync(m));
+    double s = 0.0;
+    for (double v : lp) s += v;
+    *loss = s / (double)n;
+    return FITSNE_OK;
+}
+
+}  // namespace
+
+extern "C" int fitsne_beta_model_create(fitsne_beta_model **out, int K, unsigned long long seed) {
+    if (!out) return fail(FITSNE_E_ARG, "null model pointer");
+    *out = nullptr;
+    if (K < 1 || K > kMlpIn) return fail(FITSNE_E_UNSUPPORTED, "the β model takes 1..96 neighbour distances");
+    int n = 0;
+    HIPTRY(hipGetDeviceCount(&n));
+    if (n < 1) return fail(FITSNE_E_NODEVICE, "no GPU device");
+    auto *m = new fitsne_beta_model();
+    m->K = K;
+    HIPTRY(hipSetDevice(m->device));
+    HIPTRY(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+    std::vector<float> p(kMlpParams, 0.f);
+    for (int k = 0; k < kMlpIn; k++) p[kMlpOffSd + k] = 1.f;
+    uint64_t st = seed;
+    const double s1 = std::sqrt(2.0 / K), s2 = std::sqrt(2.0 / kMlpHid), s3 = std::sqrt(1.0 / kMlpHid);
+    for (int k = 0; k < kMlpIn; k++)
+        for (int j = 0; j < kMlpHid; j++) p[kMlpOffW1 + k * kMlpHid + j] = k < K ? (float)(s1 * gauss(st)) : 0.f;
+    for (int e = 0; e < kMlpHid * kMlpHid; e++) p[kMlpOffW2 + e] = (float)(s2 * gauss(st));
+    for (int j = 0; j < kMlpHid; j++) p[kMlpOffW3 + j] = (float)(s3 * gauss(st));
+    int rc = ensure(m->prm, sizeof(float) * kMlpParams);
+    if (rc == FITSNE_OK) rc = ensure(m->m1, sizeof(float) * kMlpTrainN);
+    if (rc == FITSNE_OK) rc = ensure(m->m2, sizeof(float) * kMlpTrainN);
+    if (rc == FITSNE_OK) rc = ensure(m->grad, sizeof(float) * kMlpTrainN);
+    if (rc == FITSNE_OK) rc = ensure(m->w1t, sizeof(unsigned short) * kMlpHid * kMlpIn);
+    if (rc == FITSNE_OK) rc = ensure(m->w2t, sizeof(unsigned short) * kMlpHid * kMlpHid);
+    if (rc == FITSNE_OK) rc = ensure(m->w2, sizeof(unsigned short) * kMlpHid * kMlpHid);
+    if (rc != FITSNE_OK) {
+        fitsne_beta_model_destroy(m);
+        return rc;
+    }
+    std::vector<float> zeros(kMlpTrainN, 0.f);
+    if (hipMemcpy(m->prm.p, p.data(), sizeof(float) * kMlpParams, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(m->m1.p, zeros.data(), sizeof(float) * kMlpTrainN, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(m->m2.p, zeros.data(), sizeof(float) * kMlpTrainN, hipMemcpyHostToDevice) != hipSuccess) {
+        fitsne_beta_model_destroy(m);
+        return fail(FITSNE_E_HIP, "upload of the model parameters failed");
+    }
+    *out = m;
+    return FITSNE_OK;
+}
+
+extern "C" int fitsne_beta_model_destroy(fitsne_beta_model *m) {
+    if (!m) return FITSNE_OK;
+    (void)hipSetDevice(m->device);
+    if (m->stream) (void)hipStreamSynchronize(m->stream);
+    DBuf *all[] = {&m->prm, &m->m1, &m->m2, &m->grad, &m->gpart, &m->lpart, &m->w1t, &m->w2t, &m->w2,
+                   &m->X, &m->tgt, &m->dist, &m->beta0};
+    for (DBuf *b : all) release(*b);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
+    delete m;
+    return FITSNE_OK;
+}
+
+extern "C" int fitsne_beta_model_params(fitsne_beta_model *m, float *out) {
+    if (!m || !out) return fail(FITSNE_E_ARG, "null argument");
+    HIPTRY(hipSetDevice(m->device));
+    HIPTRY(hipMemcpyAsync(out, m->prm.p, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost, m->stream));
+    return mlp_sync(m);
+}
+
+extern "C" int fitsne_beta_model_set_params(fitsne_beta_model *m, const float *in) {
+    if (!m || !in) return fail(FITSNE_E_ARG, "null argument");
+    for (int k = 0; k < kMlpIn; k++)
+        if (!(in[kMlpOffSd + k] > 0.f)) return fail(FITSNE_E_ARG, "feature scales must be positive");
+    HIPTRY(hipSetDevice(m->device));
+    HIPTRY(hipMemcpyAsync(m->prm.p, in, sizeof(float) * kMlpParams, hipMemcpyHostToDevice, m->stream));
+    return mlp_sync(m);
+}
+
+extern "C" int fitsne_beta_model_gradient(fitsne_beta_model *m, const double *dist, const double *beta, int n,
+                                          float *grad, double *loss) {
+    if (!m || !dist || !beta || n <= 0) return fail(FITSNE_E_ARG, "invalid arguments");
+    HIPTRY(hipSetDevice(m->device));
+    std::vector<float> t(n);
+    for (int s = 0; s < n; s++) {
+        if (!(beta[s] > 0.0)) return fail(FITSNE_E_ARG, "targets must be positive");
+        t[s] = (float)std::log(beta[s]);
+    }
+    TRY(mlp_features(m, dist, n));
+    TRY(ensure(m->tgt, sizeof(float) * (size_t)n));
+    HIPTRY(hipMemcpyAsync(m->tgt.p, t.data(), sizeof(float) * n, hipMemcpyHostToDevice, m->stream));
+    int G = 0;
+    TRY(mlp_grad_step(m, 0, n, false, 0.f, &G));
+    if (grad) HIPTRY(hipMemcpyAsync(grad, m->grad.p, sizeof(float) * kMlpTrainN, hipMemcpyDeviceToHost, m->stream));
+    double l = 0.0;
+    TRY(mlp_loss(m, G, n, &l));
+    if (loss) *loss = l;
+    return FITSNE_OK;
+}
+
+extern "C" int fitsne_beta_model_fit(fitsne_beta_model *m, const double *dist, const double *beta, int n, int epochs,
+                                     int batch, double lr, unsigned long long seed, double *final_loss) {
+    if (!m || !dist || !beta || n < 64 || epochs < 1 || batch < 64) return fail(FITSNE_E_ARG, "invalid arguments");
+    HIPTRY(hipSetDevice(m->device));
+    const int K = m->K;
+    // feature standardisation from the training set
+    std::vector<double> mu(K, 0.0), var(K, 0.0);
+    for (int s = 0; s < n; s++)
+        for (int k = 0; k < K; k++) mu[k] += std::log(std::fmax(dist[(size_t)s * K + k] * dist[(size_t)s * K + k], 1e-30));
+    for (int k = 0; k < K; k++) mu[k] /= n;
+    for (int s = 0; s < n; s++)
+        for (int k = 0; k < K; k++) {
+            const double f = std::log(std::fmax(dist[(size_t)s * K + k] * dist[(size_t)s * K + k], 1e-30)) - mu[k];
+            var[k] += f * f;
+        }
+    std::vector<float> p(kMlpParams);
+    HIPTRY(hipMemcpy(p.data(), m->prm.p, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost));
+    double tmean = 0.0;
+    for (int s = 0; s < n; s++) {
+        if (!(beta[s] > 0.0)) return fail(FITSNE_E_ARG, "targets must be positive");
+        tmean += std::log(beta[s]);
+    }
+    for (int k = 0; k < kMlpIn; k++) {
+        p[kMlpOffMu + k] = k < K ? (float)mu[k] : 0.f;
+        p[kMlpOffSd + k] = k < K ? (float)std::fmax(std::sqrt(var[k] / n), 1e-6) : 1.f;
+    }
+    p[kMlpOffB3] = (float)(tmean / n);
+    HIPTRY(hipMemcpy(m->prm.p, p.data(), sizeof(float) * kMlpParams, hipMemcpyHostToDevice));
+    // one shuffled copy of the samples, then contiguous mini-batches
+    std::vector<int> order(n);
+    for (int s = 0; s < n; s++) order[s] = s;
+    uint64_t st = seed;
+    for (int s = n - 1; s > 0; s--) std::swap(order[s], order[(int)(splitmix64(st) % (uint64_t)(s + 1))]);
+    std::vector<double> dsh((size_t)n * K);
+    std::vector<float> t(n);
+    for (int s = 0; s < n; s++) {
+        std::memcpy(&dsh[(size_t)s * K], &dist[(size_t)order[s] * K], sizeof(double) * K);
+        t[s] = (float)std::log(beta[order[s]]);
+    }
+    TRY(mlp_features(m, dsh.data(), n));
+    TRY(ensure(m->tgt, sizeof(float) * (size_t)n));
+    HIPTRY(hipMemcpyAsync(m->tgt.p, t.data(), sizeof(float) * n, hipMemcpyHostToDevice, m->stream));
+    for (int e = 0; e < epochs; e++)
+        for (long off = 0; off + 64 <= n; off += batch) {
+            const long nb = std::min<long>(batch, n - off);
+            TRY(mlp_grad_step(m, off, nb, true, (float)lr, nullptr));
+        }
+    int G = 0;
+    TRY(mlp_grad_step(m, 0, n, false, 0.f, &G));
+    double l = 0.0;
+    TRY(mlp_loss(m, G, n, &l));
+    if (final_loss) *final_loss = l;
+    return FITSNE_OK;
+}
+
+extern "C" int fitsne_beta_model_predict(fitsne_beta_model *m, const double *dist, int n, double *beta0) {
+    if (!m || !dist || !beta0 || n <= 0) return fail(FITSNE_E_ARG, "invalid arguments");
+    HIPTRY(hipSetDevice(m->device));
+    TRY(mlp_features(m, dist, n));
+    TRY(ensure(m->beta0, sizeof(double) * (size_t)n));
+    TRY(mlp_pack(m));
+    const long blocks = ((long)n + 16 * kMlpWaves - 1) / (16 * kMlpWaves);
+    k_mlp_predict<<<dim3((unsigned)blocks), dim3(kMlpThreads), 0, m->stream>>>(
+        m->X.as<const unsigned short>(), n, m->w1t.as<const unsigned short>(), m->w2t.as<const unsigned short>(),
+        m->prm.as<const float>(), m->beta0.as<double>());
+    HIPTRY(hipGetLastError());
+    HIPTRY(hipMemcpyAsync(beta0, m->beta0.p, sizeof(double) * n, hipMemcpyDeviceToHost, m->stream));
+    return mlp_sync(m);
 }

@@ -53,6 +53,14 @@ __device__ __forceinline__ void load_pt(const TV *__restrict__ Y, long i, TV (&y
     }
 }
 
+// Copy the twiddle table into LDS behind the workgroup's lines (the FFT stages read it once per
+// butterfly input); visible after the caller's next __syncthreads.
+template <typename T>
+__device__ __forceinline__ const cx<T> *stage_twiddles(cx<T> *lds, const cx<T> *__restrict__ tw, int n) {
+    for (int e = threadIdx.x; e < n; e += blockDim.x) lds[e] = tw[e];
+    return lds;
+}
+
 // ---------------------------------------------------------------- twiddles
 
 template <typename T>
@@ -653,6 +661,7 @@ __global__ __launch_bounds__(NTHR) void k_kern_rows(Geo g, FftPlan pl, const cx<
     if (a >= g.n1d) return;
     const int Mp = g.Mp, n1d = g.n1d;
     const double du = a * g.h;
+    const cx<T> *twl = stage_twiddles(buf + Mp, tw, Mp);
     for (int b = threadIdx.x; b < Mp; b += NTHR) buf[b] = cx<T>{(T)0, (T)0};
     __syncthreads();
     for (int b = threadIdx.x; b < n1d; b += NTHR) {
@@ -663,7 +672,7 @@ __global__ __launch_bounds__(NTHR) void k_kern_rows(Geo g, FftPlan pl, const cx<
         if (b > 0) buf[Mp - b] = val;
     }
     __syncthreads();
-    fft_lines<T, NTHR, MAXTOT>(buf, pl, 1, Mp, tw, false);
+    fft_lines<T, NTHR, MAXTOT>(buf, pl, 1, Mp, twl, false);
     for (int k = threadIdx.x; k < g.H; k += NTHR) Trow[(long)k * n1d + a] = buf[k];
 }
 
@@ -684,6 +693,7 @@ __global__ __launch_bounds__(NTHR) void k_kern_cols(Geo g, FftPlan pl, const cx<
     const int k = xcd_row(blockIdx.x, g.H);
     if (k >= g.H) return;
     const int Mp = g.Mp, n1d = g.n1d, H = g.H;
+    const cx<T> *twl = stage_twiddles(buf + Mp, tw, Mp);
     for (int a = threadIdx.x; a < Mp; a += NTHR) {
         int as;
         if (a < n1d) as = a;
@@ -697,7 +707,7 @@ __global__ __launch_bounds__(NTHR) void k_kern_cols(Geo g, FftPlan pl, const cx<
         buf[a] = cx<T>{t.x * w, t.y};
     }
     __syncthreads();
-    fft_lines<T, NTHR, MAXTOT>(buf, pl, 1, Mp, tw, false);
+    fft_lines<T, NTHR, MAXTOT>(buf, pl, 1, Mp, twl, false);
     for (int m = threadIdx.x; m < H; m += NTHR) {
         const cx<T> z = buf[m];
         const T im_neg = buf[m == 0 ? 0 : Mp - m].y;
@@ -728,6 +738,7 @@ __global__ __launch_bounds__(NTHR) void k_rows_fwd(Geo g, FftPlan pl, const cx<T
     if (u >= g.n1d) return;
     const int Mp = g.Mp, n1d = g.n1d, H = g.H;
     const int lines = (nterms + 1) / 2;
+    const cx<T> *twl = stage_twiddles(buf + lines * Mp, tw, Mp);
     const double inv = 1.0 / kFixScale;
     const double unit[4] = {inv, inv * g.bw0 * g.bw0, inv * g.bw0, inv * g.bw0};
     for (int q = threadIdx.x; q < lines * Mp; q += NTHR) {
@@ -749,7 +760,7 @@ __global__ __launch_bounds__(NTHR) void k_rows_fwd(Geo g, FftPlan pl, const cx<T
         buf[q] = z;
     }
     __syncthreads();
-    fft_lines<T, NTHR, MAXTOT>(buf, pl, lines, Mp, tw, false);
+    fft_lines<T, NTHR, MAXTOT>(buf, pl, lines, Mp, twl, false);
     const long plane = (long)H * n1d;
     for (int q = threadIdx.x; q < lines * H; q += NTHR) {
         const int l = q / H, k = q - l * H;
@@ -778,12 +789,13 @@ __global__ __launch_bounds__(NTHR) void k_cols_f32(Geo g, FftPlan pl, const cx<f
     if (k >= g.H) return;
     const int Mp = g.Mp, n1d = g.n1d, H = g.H;
     const long plane = (long)H * n1d;
+    const cx<float> *twl = stage_twiddles(buf + 4 * Mp, tw, Mp);
     for (int q = threadIdx.x; q < 4 * Mp; q += NTHR) {
         const int t = q / Mp, u = q - t * Mp;
         buf[q] = u < n1d ? spec[t * plane + (long)k * n1d + u] : cx<float>{0.f, 0.f};
     }
     __syncthreads();
-    fft_lines<float, NTHR, MAXTOT>(buf, pl, 4, Mp, tw, false);
+    fft_lines<float, NTHR, MAXTOT>(buf, pl, 4, Mp, twl, false);
     const double wk = (k == 0 || 2 * k == Mp) ? 1.0 : 2.0;
     double z0 = 0, z1 = 0, z2 = 0, z3 = 0;
     for (int m = threadIdx.x; m < Mp; m += NTHR) {
@@ -804,7 +816,7 @@ __global__ __launch_bounds__(NTHR) void k_cols_f32(Geo g, FftPlan pl, const cx<f
         z3 += (double)cq * ((double)BX.x * BX.x + (double)BX.y * BX.y + (double)BY.x * BY.x + (double)BY.y * BY.y);
     }
     __syncthreads();
-    fft_lines<float, NTHR, MAXTOT>(buf, pl, 3, Mp, tw, true);
+    fft_lines<float, NTHR, MAXTOT>(buf, pl, 3, Mp, twl, true);
     for (int q = threadIdx.x; q < 3 * n1d; q += NTHR) {
         const int t = q / n1d, u = q - t * n1d;
         ospec[t * plane + (long)k * n1d + u] = buf[t * Mp + u];
@@ -835,6 +847,7 @@ __global__ __launch_bounds__(NTHR) void k_cols_plain(Geo g, FftPlan pl, const cx
     const int Mp = g.Mp, n1d = g.n1d, H = g.H;
     const long plane = (long)H * n1d;
     const int lines_max = MAXTOT / Mp;
+    const cx<T> *twl = stage_twiddles(buf + min(nterms, lines_max) * Mp, tw, Mp);
     for (int t0 = 0; t0 < nterms; t0 += lines_max) {
         const int lines = min(lines_max, nterms - t0);
         for (int q = threadIdx.x; q < lines * Mp; q += NTHR) {
@@ -842,14 +855,14 @@ __global__ __launch_bounds__(NTHR) void k_cols_plain(Geo g, FftPlan pl, const cx
             buf[q] = u < n1d ? spec[(t0 + t) * plane + (long)k * n1d + u] : cx<T>{(T)0, (T)0};
         }
         __syncthreads();
-        fft_lines<T, NTHR, MAXTOT>(buf, pl, lines, Mp, tw, false);
+        fft_lines<T, NTHR, MAXTOT>(buf, pl, lines, Mp, twl, false);
         for (int q = threadIdx.x; q < lines * Mp; q += NTHR) {
             const int t = q / Mp, m = q - t * Mp;
             const int mb = m <= Mp / 2 ? m : Mp - m;
             buf[q] = KC[(long)k * H + mb] * buf[q];
         }
         __syncthreads();
-        fft_lines<T, NTHR, MAXTOT>(buf, pl, lines, Mp, tw, true);
+        fft_lines<T, NTHR, MAXTOT>(buf, pl, lines, Mp, twl, true);
         for (int q = threadIdx.x; q < lines * n1d; q += NTHR) {
             const int t = q / n1d, u = q - t * n1d;
             ospec[(t0 + t) * plane + (long)k * n1d + u] = buf[t * Mp + u];
@@ -873,6 +886,7 @@ __global__ __launch_bounds__(NTHR) void k_rows_inv(Geo g, FftPlan pl, const cx<T
     const int Mp = g.Mp, n1d = g.n1d, H = g.H;
     const long plane = (long)H * n1d;
     const int lines = (nterms + 1) / 2;
+    const cx<T> *twl = stage_twiddles(buf + lines * Mp, tw, Mp);
     for (int q = threadIdx.x; q < lines * Mp; q += NTHR) {
         const int l = q / Mp, k = q - l * Mp;
         const bool lo = k < H;
@@ -890,7 +904,7 @@ __global__ __launch_bounds__(NTHR) void k_rows_inv(Geo g, FftPlan pl, const cx<T
         buf[q] = f1 + mul_i(f2);
     }
     __syncthreads();
-    fft_lines<T, NTHR, MAXTOT>(buf, pl, lines, Mp, tw, true);
+    fft_lines<T, NTHR, MAXTOT>(buf, pl, lines, Mp, twl, true);
     for (int q = threadIdx.x; q < lines * n1d; q += NTHR) {
         const int l = q / n1d, v = q - l * n1d;
         const cx<T> z = buf[l * Mp + v];

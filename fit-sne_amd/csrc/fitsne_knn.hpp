@@ -4,8 +4,8 @@
 // distance is sqrt(sum_d (x_d - y_d)^2) accumulated in order d = 0..D-1 without fused multiply-adds
 // (vptree.h:97-107).  Here one wavefront serves kKnnQ query points: every lane takes one candidate per
 // round, accumulates its squared distances to the kKnnQ queries in the same order and rounding
-// (__dmul_rn / __dadd_rn; the root by the correctly rounded __dsqrt_rn), and candidates below the query's current K-th distance are appended to an
-// LDS buffer of B slots per query.  When a buffer is nearly full it is sorted by (distance, index)
+// (__dmul_rn / __dadd_rn; the root by the correctly rounded __dsqrt_rn), and candidates below the
+// query's current K-th distance are appended to an LDS buffer of B slots per query.  When a buffer is nearly full it is sorted by (distance, index)
 // with a bitonic network and cut back to K, which tightens the threshold.  The output is sorted by
 // (distance, index), so equal distances are broken by the smaller index and the result is
 // deterministic.  X is read transposed (XT[d][n]) so that the lanes' candidate loads coalesce.

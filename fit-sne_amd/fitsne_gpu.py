@@ -19,6 +19,7 @@ _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _u32 = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
 _i64 = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
 _i32 = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_f32 = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 _ci, _cd, _cu, _vp = C.c_int, C.c_double, C.c_uint, C.c_void_p
 
 
@@ -48,6 +49,13 @@ SIGNATURES = {
     "fitsne_evaluate_error_fft": (_ci, [_u32, _u32, _dp, _dp, _ci, _ci, _cd, _cd, C.POINTER(_cd)]),
     "fitsne_distances2similarities_batch": (_ci, [_dp, _dp, _ci, _ci, _cd, _cd, _ci, _vp, _dp, _vp]),
     "fitsne_knn_exact": (_ci, [_dp, _ci, _ci, _ci, _i32, _dp]),
+    "fitsne_beta_model_create": (_ci, [C.POINTER(_vp), _ci, C.c_ulonglong]),
+    "fitsne_beta_model_destroy": (_ci, [_vp]),
+    "fitsne_beta_model_fit": (_ci, [_vp, _dp, _dp, _ci, _ci, _ci, _cd, C.c_ulonglong, C.POINTER(_cd)]),
+    "fitsne_beta_model_predict": (_ci, [_vp, _dp, _ci, _dp]),
+    "fitsne_beta_model_gradient": (_ci, [_vp, _dp, _dp, _ci, _f32, C.POINTER(_cd)]),
+    "fitsne_beta_model_params": (_ci, [_vp, _f32]),
+    "fitsne_beta_model_set_params": (_ci, [_vp, _f32]),
     "fitsne_ctx_create": (_ci, [C.POINTER(_vp), _ci, _ci, _ci, _ci, _cd, _ci, _cd]),
     "fitsne_ctx_destroy": (_ci, [_vp]),
     "fitsne_set_P": (_ci, [_vp, _ci, _u32, _u32, _dp]),
@@ -203,6 +211,76 @@ def knn_exact(X, K):
     dist = np.empty((N, K))
     check(lib().fitsne_knn_exact(X.reshape(-1), N, D, K, idx.reshape(-1), dist.reshape(-1)))
     return idx, dist
+
+
+# ---------------------------------------------------------------- learned β warm start
+
+MLP_IN, MLP_HID = 96, 64
+MLP_PARAMS = 2 * MLP_IN + MLP_IN * MLP_HID + MLP_HID + MLP_HID * MLP_HID + MLP_HID + MLP_HID + 1
+MLP_TRAIN0 = 2 * MLP_IN
+
+
+def mlp_unpack(p):
+    """Split the FP32 parameter vector into named arrays (views)."""
+    o = 0
+    out = {}
+    for name, shape in [("mu", (MLP_IN,)), ("sd", (MLP_IN,)), ("W1", (MLP_IN, MLP_HID)), ("b1", (MLP_HID,)),
+                        ("W2", (MLP_HID, MLP_HID)), ("b2", (MLP_HID,)), ("w3", (MLP_HID,)), ("b3", (1,))]:
+        n = int(np.prod(shape))
+        out[name] = p[o:o + n].reshape(shape)
+        o += n
+    return out
+
+
+class BetaModel:
+    """MLP warm start of the β bisection (BF16 MFMA on the GPU)."""
+
+    def __init__(self, K=90, seed=1):
+        h = _vp()
+        check(lib().fitsne_beta_model_create(C.byref(h), K, seed))
+        self.h = h
+        self.K = K
+
+    def close(self):
+        if self.h:
+            lib().fitsne_beta_model_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def fit(self, dist, beta, epochs=30, batch=1024, lr=1e-3, seed=7):
+        dist = np.ascontiguousarray(dist, dtype=np.float64)
+        beta = np.ascontiguousarray(beta, dtype=np.float64)
+        loss = _cd()
+        check(lib().fitsne_beta_model_fit(self.h, dist.reshape(-1), beta, dist.shape[0], epochs, batch, lr, seed,
+                                          C.byref(loss)))
+        return loss.value
+
+    def predict(self, dist):
+        dist = np.ascontiguousarray(dist, dtype=np.float64)
+        out = np.empty(dist.shape[0])
+        check(lib().fitsne_beta_model_predict(self.h, dist.reshape(-1), dist.shape[0], out))
+        return out
+
+    def gradient(self, dist, beta):
+        dist = np.ascontiguousarray(dist, dtype=np.float64)
+        beta = np.ascontiguousarray(beta, dtype=np.float64)
+        g = np.empty(MLP_PARAMS - MLP_TRAIN0, dtype=np.float32)
+        loss = _cd()
+        check(lib().fitsne_beta_model_gradient(self.h, dist.reshape(-1), beta, dist.shape[0], g, C.byref(loss)))
+        return g, loss.value
+
+    def params(self):
+        p = np.empty(MLP_PARAMS, dtype=np.float32)
+        check(lib().fitsne_beta_model_params(self.h, p))
+        return p
+
+    def set_params(self, p):
+        check(lib().fitsne_beta_model_set_params(self.h, np.ascontiguousarray(p, dtype=np.float32)))
 
 
 # ---------------------------------------------------------------- resident tier

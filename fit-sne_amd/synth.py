@@ -76,6 +76,37 @@ def c2_workload(N=1_000_000, D=50, k=50, per_point=45, seed=20261204):
     return (row_P, col_P, val_P), Y, labels
 
 
+def c3_workload(N=2_000_000, k=100, per_point=30, seed=20261205):
+    """C3's gradient inputs: N points in 1D from k clusters with Dirichlet(0.5) weights, centres uniform
+    in [-280, 280], sd 5 (span about [-300, 300]); P built like c2_workload's from per_point random
+    same-cluster neighbours with weights on the cluster-local 1D gaps (int64 row offsets, int32
+    columns), about 2 * per_point non-zeros per row.  The kNN step of the full configuration (D = 100,
+    K = 150) is not part of the gradient path."""
+    rng = np.random.default_rng(seed)
+    t0 = time.time()
+    weights = rng.dirichlet(np.full(k, 0.5))
+    labels = rng.choice(k, size=N, p=weights)
+    centres = rng.uniform(-280.0, 280.0, size=k)
+    Y = centres[labels] + rng.normal(0.0, 5.0, size=N)
+    X = (Y[:, None] + rng.normal(0.0, 1.0, size=(N, 1))).astype(np.float32)
+    rows, cols, vals = cluster_affinities(X, labels, per_point, rng)
+    _log(f"C3 affinities ({rows.size} directed edges) in {time.time() - t0:.1f}s")
+    P = symmetrize_normalize(N, rows, cols, vals)
+    _log(f"C3 symmetric P nnz={P[2].size} in {time.time() - t0:.1f}s")
+    Y -= Y.mean()
+    return P, Y.reshape(N, 1), labels
+
+
+def c5_mixture(N=16_000, D=100, k=40, seed=1):
+    """C5-style data: k equal-weight Gaussian clusters in D dimensions, centres N(0, 6^2 I), per-cluster
+    sd log-uniform in [0.3, 3]."""
+    rng = np.random.default_rng(seed)
+    labels = rng.integers(0, k, size=N)
+    centres = rng.normal(0.0, 6.0, size=(k, D))
+    sds = np.exp(rng.uniform(np.log(0.3), np.log(3.0), size=k))
+    return centres[labels] + rng.normal(size=(N, D)) * sds[labels][:, None], labels
+
+
 def c1_data(N=10_000, D=50, k=10, seed=20261202):
     """C1: N points in D dimensions, k equal-weight isotropic Gaussians, centres N(0, 4^2 I), sd 1."""
     rng = np.random.default_rng(seed)

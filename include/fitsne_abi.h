@@ -125,6 +125,33 @@ int fitsne_distances2similarities_batch(const double *D, double *P, int n_points
 int fitsne_knn_exact(const double *X, int N, int D, int K, int32_t *idx, double *dist);
 
 /* ====================================================================================== */
+/* Learned warm start of the perplexity calibration (no reference counterpart: the reference */
+/* starts every bisection of distances2similarities, tsne.cpp:1394-1469, at beta = 1).      */
+/* ====================================================================================== */
+
+/* A small MLP (BF16 MFMA, FP32 accumulation) mapping a point's K <= 96 ascending kNN distances to a
+ * starting beta for fitsne_distances2similarities_batch (beta0), whose unchanged bisection remains the
+ * judge of the entropy error.  Features: standardised log(d^2); layers 96 -> 64 -> 64 -> 1 (ReLU);
+ * output log(beta).  Parameter vector (FP32, 10,625 values): mu[96] sd[96] W1[96][64] b1[64] W2[64][64]
+ * b2[64] w3[64] b3. */
+typedef struct fitsne_beta_model fitsne_beta_model;
+int fitsne_beta_model_create(fitsne_beta_model **model, int K, unsigned long long seed);
+int fitsne_beta_model_destroy(fitsne_beta_model *model);
+/* Train on n points (dist[n*K] as fitsne_knn_exact returns them, beta[n] from the bisection): feature
+ * scales from these data, then `epochs` passes of Adam (learning rate lr) over shuffled mini-batches of
+ * `batch` points, mean squared error on log(beta).  *final_loss (nullable): loss over all n after. */
+int fitsne_beta_model_fit(fitsne_beta_model *model, const double *dist, const double *beta, int n, int epochs,
+                          int batch, double lr, unsigned long long seed, double *final_loss);
+/* beta0[n] = exp(model output) for dist[n*K]. */
+int fitsne_beta_model_predict(fitsne_beta_model *model, const double *dist, int n, double *beta0);
+/* Mean squared error over the n points and its gradient with respect to W1 .. b3 (grad[10,433], the
+ * parameter layout above without mu and sd), no update. */
+int fitsne_beta_model_gradient(fitsne_beta_model *model, const double *dist, const double *beta, int n,
+                               float *grad, double *loss);
+int fitsne_beta_model_params(fitsne_beta_model *model, float *params);
+int fitsne_beta_model_set_params(fitsne_beta_model *model, const float *params);
+
+/* ====================================================================================== */
 /* Resident tier                                                                           */
 /* ====================================================================================== */
 
