@@ -36,8 +36,9 @@ __device__ __forceinline__ cx<T> mul_i(cx<T> a) { return {-a.y, a.x}; }
 template <typename T>
 __device__ __forceinline__ cx<T> mul_negi(cx<T> a) { return {a.y, -a.x}; }
 
-// Grid geometry of one gradient evaluation, computed on the host from (min, max) exactly as
-// tsne.cpp:1042-1092 / nbodyfft.cpp:15-46 (2D) and tsne.cpp:656-663 / nbodyfft.cpp:257-280 (1D).
+// Grid geometry of one gradient evaluation, from (min, max) exactly as tsne.cpp:1042-1092 /
+// nbodyfft.cpp:15-46 (2D) and tsne.cpp:656-663 / nbodyfft.cpp:257-280 (1D).  Computed by the same
+// functions on the host (one-shot calls) and on the device (captured steps).
 struct Geo {
     double mn;       // coord_min = box_lower_bounds[0]
     double bw0;      // box width used for box bounds and nodes: (max - min) / n_boxes
@@ -54,12 +55,118 @@ struct Geo {
     int p;
 };
 
-// Stage radices of one FFT length (Stockham, radices 4/2/3).
+// Stage radices of one FFT length (Stockham).
 struct FftPlan {
     int n;
     int nst;
     int rad[24];
 };
+
+// Geometry and plan of the current step as the kernels read them (device memory).  `live` is 0 when
+// the step's grid exceeds the build's limits: every kernel of the step then returns at once and the
+// state is left as it was (the host reports the error).
+struct DevGeo {
+    Geo g;
+    FftPlan pl;
+    int live;
+    int kspec_stale;  // kernel spectra must be recomputed this step
+    int tw_stale;     // twiddles must be recomputed this step
+    int err;          // set when a step was refused for its grid size
+    double kc_mn, kc_mx;  // (min, max, nb) the kernel spectra were computed for
+    int kc_nb;
+    int tw_n;         // FFT length of the twiddle table
+    double mn, mx;    // the step's (reference-rule) min and max
+};
+
+// Optimiser parameters of one captured step (tsne.cpp:478-514) and the P multiplier.
+struct StepDev {
+    float momentum, lr, max_step_norm;
+    int plain_descent;
+    int mode;       // 0: gradient only; 1: update
+    float exag;
+};
+
+// tsne.cpp:1065-1077
+__host__ __device__ inline int n_boxes_2d(double mn, double mx, double ipi, int mni) {
+    int nb = (int)fmax((double)mni, (mx - mn) / ipi);
+    const int allowed[20] = {25, 36, 50, 55, 60, 65, 70, 75, 80, 85, 90, 96, 100, 110, 120, 130, 140, 150, 175, 200};
+    if (nb < allowed[19]) {
+        int c = 0;
+        while (allowed[c] < nb) c++;
+        nb = allowed[c];
+    }
+    return nb;
+}
+
+// tsne.cpp:663
+__host__ __device__ inline int n_boxes_1d(double mn, double mx, double ipi, int mni) {
+    return (int)fmax((double)mni, (mx - mn) / ipi);
+}
+
+// smallest 2^a 3^b >= n
+__host__ __device__ inline int fft_size(int n) {
+    long best = -1;
+    for (long p3 = 1; p3 < 2L * n + 2; p3 *= 3) {
+        long m = p3;
+        while (m < n) m *= 2;
+        if (best < 0 || m < best) best = m;
+    }
+    return (int)best;
+}
+
+// Stage radices: 16, 8, 4, 2 for the power of two, then 9, 3 (fewest barrier-separated stages).
+__host__ __device__ inline FftPlan make_plan(int n) {
+    FftPlan pl{};
+    pl.n = n;
+    int m = n;
+    while (m % 16 == 0) { pl.rad[pl.nst++] = 16; m /= 16; }
+    if (m % 8 == 0) { pl.rad[pl.nst++] = 8; m /= 8; }
+    if (m % 4 == 0) { pl.rad[pl.nst++] = 4; m /= 4; }
+    if (m % 2 == 0) { pl.rad[pl.nst++] = 2; m /= 2; }
+    while (m % 9 == 0) { pl.rad[pl.nst++] = 9; m /= 9; }
+    while (m % 3 == 0) { pl.rad[pl.nst++] = 3; m /= 3; }
+    return pl;  // m == 1 by construction of fft_size
+}
+
+// Node spacings and Lagrange denominators as nbodyfft.cpp:30-34 and :313-321.
+__host__ __device__ inline void fill_spacings(Geo &g, int p) {
+    double h = 1 / (double)p;
+    g.sp[0] = h / 2;
+    for (int i = 1; i < p; i++) g.sp[i] = g.sp[i - 1] + h;
+    for (int i = 0; i < p; i++) {
+        g.den[i] = 1;
+        for (int j = 0; j < p; j++)
+            if (i != j) g.den[i] *= g.sp[i] - g.sp[j];
+    }
+}
+
+// Geometry of one gradient from the (reference-rule) min and max, as precompute_2d / precompute
+// compute it (nbodyfft.cpp:15-46, :257-280) and n_body_fft re-derives the box width (:79-80, :344-345).
+__host__ __device__ inline Geo make_geo(int D, int p, double mn, double mx, int nb) {
+    Geo g{};
+    g.D = D;
+    g.p = p;
+    g.nb = nb;
+    g.mn = mn;
+    g.bw0 = (mx - mn) / (double)nb;
+    const double lower0 = 0 * g.bw0 + mn, upper0 = (0 + 1) * g.bw0 + mn;
+    g.bw = upper0 - lower0;
+    g.h = (1 / (double)p) * g.bw0;
+    fill_spacings(g, p);
+    g.n1d = p * nb;
+    g.Mp = fft_size(2 * g.n1d - 1);
+    g.H = g.Mp / 2 + 1;
+    g.nbox = D == 2 ? nb * nb : nb;
+    return g;
+}
+
+// Workgroup-to-row map over npad = 8 ceil(n / 8) slots: consecutive workgroups go to different XCDs
+// (round-robin dispatch), so each XCD takes a contiguous range of rows and its L2 merges their
+// lines.  A bijection of [0, npad); slots mapping to rows >= n are skipped by the caller.
+__device__ __forceinline__ int xcd_slot_row(int slot, int n) {
+    const int per = (n + 7) / 8;
+    return (slot % 8) * per + slot / 8;
+}
 
 // ---- reductions ----
 template <typename T>

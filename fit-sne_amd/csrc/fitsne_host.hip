@@ -83,80 +83,8 @@ static void release(DBuf &b) {
     b.cap = 0;
 }
 
-// ============================================================================ FFT plans
-
-static int fft_size(int n) {
-    long best = -1;
-    for (long p3 = 1; p3 < 2L * n + 2; p3 *= 3) {
-        long m = p3;
-        while (m < n) m *= 2;
-        if (best < 0 || m < best) best = m;
-    }
-    return (int)best;
-}
-
-// Stage radices: 16, 8, 4, 2 for the power of two, then 9, 3 (fewest barrier-separated stages).
-static FftPlan make_plan(int n) {
-    FftPlan pl{};
-    pl.n = n;
-    int m = n;
-    while (m % 16 == 0) { pl.rad[pl.nst++] = 16; m /= 16; }
-    if (m % 8 == 0) { pl.rad[pl.nst++] = 8; m /= 8; }
-    if (m % 4 == 0) { pl.rad[pl.nst++] = 4; m /= 4; }
-    if (m % 2 == 0) { pl.rad[pl.nst++] = 2; m /= 2; }
-    while (m % 9 == 0) { pl.rad[pl.nst++] = 9; m /= 9; }
-    while (m % 3 == 0) { pl.rad[pl.nst++] = 3; m /= 3; }
-    return pl;  // m == 1 by construction of fft_size
-}
-
-// ============================================================================ geometry (host, FP64)
-
-// tsne.cpp:1065-1077
-static int n_boxes_2d(double mn, double mx, double ipi, int mni) {
-    int nb = (int)std::fmax(mni, (mx - mn) / ipi);
-    static const int allowed[20] = {25, 36, 50, 55, 60, 65, 70, 75, 80, 85, 90, 96, 100, 110, 120, 130, 140, 150, 175, 200};
-    if (nb < allowed[19]) {
-        int c;
-        for (c = 0; allowed[c] < nb; c++) {}
-        nb = allowed[c];
-    }
-    return nb;
-}
-
-// tsne.cpp:663
-static int n_boxes_1d(double mn, double mx, double ipi, int mni) { return (int)std::fmax(mni, (mx - mn) / ipi); }
-
-// Node spacings and Lagrange denominators as nbodyfft.cpp:30-34 and :313-321.
-static void fill_spacings(Geo &g, int p) {
-    double h = 1 / (double)p;
-    g.sp[0] = h / 2;
-    for (int i = 1; i < p; i++) g.sp[i] = g.sp[i - 1] + h;
-    for (int i = 0; i < p; i++) {
-        g.den[i] = 1;
-        for (int j = 0; j < p; j++)
-            if (i != j) g.den[i] *= g.sp[i] - g.sp[j];
-    }
-}
-
-// Geometry of one gradient from the (reference-rule) min and max, as precompute_2d / precompute
-// compute it (nbodyfft.cpp:15-46, :257-280) and n_body_fft re-derives the box width (:79-80, :344-345).
-static Geo make_geo(int D, int p, double mn, double mx, int nb) {
-    Geo g{};
-    g.D = D;
-    g.p = p;
-    g.nb = nb;
-    g.mn = mn;
-    g.bw0 = (mx - mn) / (double)nb;
-    const double lower0 = 0 * g.bw0 + mn, upper0 = (0 + 1) * g.bw0 + mn;
-    g.bw = upper0 - lower0;
-    g.h = (1 / (double)p) * g.bw0;
-    fill_spacings(g, p);
-    g.n1d = p * nb;
-    g.Mp = fft_size(2 * g.n1d - 1);
-    g.H = g.Mp / 2 + 1;
-    g.nbox = D == 2 ? nb * nb : nb;
-    return g;
-}
+// FFT plans and the geometry rules (fft_size, make_plan, n_boxes_2d / _1d, make_geo) live in
+// fitsne_common.hpp, shared with the device.
 
 // ============================================================================ context
 
@@ -188,6 +116,17 @@ struct fitsne_ctx {
     int win_R = 0, win_NW = 0;
     // grids, spectra
     DBuf gin, gout, spec, ospec, Trow, KQ, K1, KS, KT, zpart, zbuf, part, mean, mmpart, mm, fixgrid;
+    // device records of the step (geometry and plan; optimiser parameters) and the captured step
+    DBuf dgeo, dstep, dtw;
+    DBuf dlive;                  // a geometry record with live = 1, for launches before the step's geometry
+    DBuf klpart;                 // KL block partials (never a buffer a captured step uses)
+    bool dlive_set = false;
+    long graph_gen = 0;          // incremented at every capture
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    bool use_graph = false;      // fitsne_step replays a captured step (FP32 form)
+    bool mm_parts_valid = false; // mmpart holds the min / max partials of the resident Y
+    bool graph_ran = false;      // a captured step ran since the last error check
     // plain path extras
     DBuf xs, ys, box, wx, wy, ch, w, V, potA, potB;
     // twiddles per FFT length
@@ -271,6 +210,9 @@ static int sync_ctx(fitsne_ctx *c) {
     HIPTRY(hipStreamSynchronize(c->stream));
     return flush_timing(c);
 }
+
+static void drop_graph(fitsne_ctx *c);
+static int check_graph(fitsne_ctx *c);
 
 template <typename T>
 static int twiddles(fitsne_ctx *c, int n, const cx<T> **out) {
@@ -379,6 +321,37 @@ static int geometry(fitsne_ctx *c, double mn, double mx, Geo *g) {
     return FITSNE_OK;
 }
 
+// Host-planned geometry of this step into the device record the kernels read (live; kernel spectra
+// stale when they must be recomputed).  The cache fields of the captured path are reset, so a later
+// captured step recomputes its spectra and twiddles.
+static int put_geo(fitsne_ctx *c, const Geo &g, bool kspec_stale) {
+    TRY(ensure(c->dgeo, sizeof(DevGeo)));
+    DevGeo v{};
+    v.g = g;
+    v.pl = make_plan(g.Mp);
+    v.live = 1;
+    v.kspec_stale = kspec_stale ? 1 : 0;
+    v.mn = c->last_min;
+    v.mx = c->last_max;
+    launch(c, "set_geo", k_set_geo, dim3(1), dim3(kWave), 0, v, c->dgeo.as<DevGeo>());
+    return FITSNE_OK;
+}
+
+static int put_step(fitsne_ctx *c, int mode, const fitsne_step_params *sp) {
+    TRY(ensure(c->dstep, sizeof(StepDev)));
+    StepDev v{};
+    v.mode = mode;
+    v.exag = (float)c->exag;
+    if (sp) {
+        v.momentum = (float)sp->momentum;
+        v.lr = (float)sp->learning_rate;
+        v.max_step_norm = (float)sp->max_step_norm;
+        v.plain_descent = sp->plain_descent;
+    }
+    launch(c, "set_step", k_set_step, dim3(1), dim3(kWave), 0, v, c->dstep.as<StepDev>());
+    return FITSNE_OK;
+}
+
 // ============================================================================ kernel dispatch tables
 
 template <int D>
@@ -456,8 +429,22 @@ static int build_windows(fitsne_ctx *c) {
     return FITSNE_OK;
 }
 
+static int live_record(fitsne_ctx *c, const DevGeo **out) {
+    if (!c->dlive_set) {
+        TRY(ensure(c->dlive, sizeof(DevGeo)));
+        DevGeo v{};
+        v.live = 1;
+        launch(c, "set_geo", k_set_geo, dim3(1), dim3(kWave), 0, v, c->dlive.as<DevGeo>());
+        c->dlive_set = true;
+    }
+    *out = c->dlive.as<const DevGeo>();
+    return FITSNE_OK;
+}
+
+// FP32: the windowed kernel reads the P multiplier from the step record (scale is for the CSR kernel)
+// and skips the step when the geometry record dg says so.
 template <typename TV, int D>
-static int launch_attractive(fitsne_ctx *c, TV scale, TV *pos) {
+static int launch_attractive(fitsne_ctx *c, TV scale, TV *pos, const DevGeo *dg = nullptr) {
     const TV inv_df = (TV)(1.0 / c->df);
     const int df1 = c->df == 1.0;
     if constexpr (std::is_same<TV, float>::value) {
@@ -467,7 +454,7 @@ static int launch_attractive(fitsne_ctx *c, TV scale, TV *pos) {
             launch(c, "attractive", k_attr_win<D>, dim3(8 * ((NB + 7) / 8)), dim3(kAttThreads), 0,
                    c->wstart.as<const int>(), c->wkey.as<const uint32_t>(), c->wval.as<const float>(),
                    c->wbmax.as<const float>(), c->Y.as<const float>(), c->N, c->win_R, c->win_NW, inv_df, df1,
-                   (float)(0.5 * std::sqrt(c->df)), scale, pos);
+                   (float)(0.5 * std::sqrt(c->df)), c->dstep.as<const StepDev>(), dg, pos);
             return FITSNE_OK;
         }
     }
@@ -490,24 +477,87 @@ static int launch_attractive(fitsne_ctx *c, TV scale, TV *pos) {
 
 // ============================================================================ FP32 pipeline
 
-// mode 0: gradient into c->dC (float); mode 1: gradient + optimiser update + zeroMean.
+// Largest grids of the captured path (the build's limits): 2D FFT length 2048 (n1d <= 1024), 1D 4096.
+constexpr int kCapN1d2 = kMaxLen2d / 2, kCapMp2 = kMaxLen2d;
+constexpr int kCapMp1 = kMaxTot32 / 2, kCapN1d1 = kCapMp1 / 2;
+
+// Launch sizes of the grid stages: exact for a host-planned step, the build's limits for a captured one.
+struct StageDims {
+    int Mp;          // FFT length the LDS is sized for
+    int gr_rows;     // workgroups of the row passes
+    int gr_cols;     // workgroups of the column passes
+    int gr_nodes;    // workgroups of the 1D grid conversion
+};
+
+// Spread, FFT convolution, Z, interpolation (+ update, zeroMean) of the FP32 form, reading geometry,
+// plan and step parameters from the context's device records.
 template <int D>
-static int pipeline_f32(fitsne_ctx *c, int mode, const fitsne_step_params *sp) {
-    const int N = c->N;
-    const int p = c->p;
+static int launch_f32_stages(fitsne_ctx *c, const cx<float> *tw, const StageDims &sd, bool kern, int mode) {
+    const int N = c->N, p = c->p;
     const bool df1 = c->df == 1.0;
+    const DevGeo *dg = c->dgeo.as<const DevGeo>();
+    launch(c, "spread_f32", pick_spread_f32<D>(p), dim3((N + kSpreadPts - 1) / kSpreadPts), dim3(kSpreadThreads), 0,
+           c->Y.as<const float>(), N, dg, c->fixgrid.as<unsigned long long>());
+    const int kind_q = df1 ? FITSNE_KERNEL_SQUARED_CAUCHY : FITSNE_KERNEL_SQUARED_GENERAL;
+    const int kind_1 = FITSNE_KERNEL_GENERAL;
+    if constexpr (D == 2) {
+        const size_t Mp = (size_t)sd.Mp;
+        if (kern) {
+            launch(c, "kern_rows", k_kern_rows<float, kKernThreads, kKernTot>, dim3(sd.gr_rows), dim3(kKernThreads),
+                   sizeof(cx<float>) * 2 * Mp, dg, tw, kind_q, kind_1, c->df, c->Trow.as<cx<float>>());
+            launch(c, "kern_cols", k_kern_cols<float, kKernThreads, kKernTot>, dim3(sd.gr_cols), dim3(kKernThreads),
+                   sizeof(cx<float>) * 2 * Mp, dg, tw, c->Trow.as<const cx<float>>(), 1, c->KQ.as<float>(),
+                   c->K1.as<float>(), c->KS.as<float>(), c->KT.as<float>());
+        }
+        launch(c, "rows_fwd", k_rows_fwd<float, kRowThreads, kRowTot>, dim3(sd.gr_rows), dim3(kRowThreads),
+               sizeof(cx<float>) * 3 * Mp, dg, tw, (const float *)nullptr, 4, 4, c->spec.as<cx<float>>(),
+               c->fixgrid.as<unsigned long long>());
+        launch(c, "cols_f32", k_cols_f32<kColThreads, kColTot32>, dim3(sd.gr_cols), dim3(kColThreads),
+               sizeof(cx<float>) * 5 * Mp, dg, tw, c->spec.as<const cx<float>>(), c->KQ.as<const float>(),
+               c->K1.as<const float>(), c->KS.as<const float>(), c->KT.as<const float>(), c->ospec.as<cx<float>>(),
+               c->zpart.as<double>());
+        launch(c, "zfinal", k_zfinal, dim3(1), dim3(kThreads), 0, dg, c->zpart.as<const double>(), (int)df1, (double)N,
+               c->zbuf.as<double>());
+        launch(c, "rows_inv", k_rows_inv<float, kRowThreads, kRowTot>, dim3(sd.gr_rows), dim3(kRowThreads),
+               sizeof(cx<float>) * 3 * Mp, dg, tw, c->ospec.as<const cx<float>>(), 3, c->gout.as<float>(), 4);
+    } else {
+        launch(c, "grid_to_f32", k_grid_to_f32, dim3(sd.gr_nodes), dim3(kThreads), 0, dg,
+               c->fixgrid.as<unsigned long long>(), c->gin.as<float4>());
+        launch(c, "conv1d_f32", k_conv1d<float, kConv1dThreads, kMaxTot32>, dim3(1), dim3(kConv1dThreads),
+               sizeof(cx<float>) * 2 * (size_t)sd.Mp, dg, tw, kind_q, kind_1, c->df, 1, c->gin.as<const float>(), 4, 3,
+               0, c->gout.as<float>(), 4, c->zpart.as<double>(), (const float *)nullptr);
+        launch(c, "zfinal", k_zfinal, dim3(1), dim3(kThreads), 0, dg, c->zpart.as<const double>(), (int)df1, (double)N,
+               c->zbuf.as<double>());
+    }
+    const int nblk = blocks_for(N);
+    launch(c, "interp_f32", pick_interp_f32<D>(p), dim3(nblk), dim3(kThreads), 0, c->Y.as<float>(),
+           c->pos.as<const float>(), c->gout.as<const float4>(), dg, c->zbuf.as<const double>(),
+           c->dstep.as<const StepDev>(), c->dC.as<float>(), c->uY.as<float>(), c->gains.as<float>(),
+           c->part.as<double>(), N);
+    if (mode == 1) {
+        const long n = (long)N * D;
+        const int cblk = std::min(blocks_for(n), 1024);
+        launch(c, "mean_final", k_mean_final, dim3(1), dim3(kThreads), 0, dg, c->part.as<const double>(), nblk, D,
+               (double)N, c->mean.as<double>());
+        launch(c, "center", k_center<float>, dim3(cblk), dim3(kThreads), 0, dg, c->Y.as<float>(), n, D, skip_coord(c),
+               c->mean.as<const double>(), c->mmpart.as<double>());
+    }
+    return FITSNE_OK;
+}
+
+// Buffers of the grid stages for n1d nodes per dimension (FFT length Mp).
+template <int D>
+static int ensure_f32_buffers(fitsne_ctx *c, int n1d, int Mp) {
+    const int N = c->N;
+    const int H = Mp / 2 + 1;
+    const long nodes = D == 2 ? (long)n1d * n1d : n1d;
     TRY(ensure(c->pos, sizeof(float) * (size_t)N * D));
-    TRY((launch_attractive<float, D>(c, (float)c->exag, c->pos.as<float>())));
-
-    double mn, mx;
-    TRY(current_minmax<float>(c, &mn, &mx));
-    Geo g;
-    TRY(geometry(c, mn, mx, &g));
-    const FftPlan pl = make_plan(g.Mp);
-    const cx<float> *tw;
-    TRY(twiddles<float>(c, g.Mp, &tw));
-
-    const long nodes = D == 2 ? (long)g.n1d * g.n1d : g.n1d;
+    TRY(ensure(c->dC, sizeof(float) * (size_t)N * D));
+    TRY(ensure(c->part, sizeof(double) * (size_t)blocks_for(N) * D));
+    TRY(ensure(c->mean, sizeof(double) * 2));
+    TRY(ensure(c->mmpart, sizeof(double) * 2 * std::min(blocks_for((long)N * D), 1024)));
+    TRY(ensure(c->mm, sizeof(double) * 4));
+    TRY(ensure(c->zbuf, sizeof(double) * 4));
     TRY(ensure(c->gin, sizeof(float4) * nodes));
     TRY(ensure(c->gout, sizeof(float4) * nodes));
     if (c->fixgrid.cap < sizeof(unsigned long long) * 4 * (size_t)nodes) {
@@ -518,15 +568,9 @@ static int pipeline_f32(fitsne_ctx *c, int mode, const fitsne_step_params *sp) {
         launch(c, "fill", k_fill<unsigned long long>, dim3(std::min(blocks_for(nw), 2048)), dim3(kThreads), 0,
                c->fixgrid.as<unsigned long long>(), nw, 0ull);
     }
-    launch(c, "spread_f32", pick_spread_f32<D>(p), dim3((N + kSpreadPts - 1) / kSpreadPts), dim3(kSpreadThreads), 0,
-           c->Y.as<const float>(), N, g, c->fixgrid.as<unsigned long long>());
-
-    const int kind_q = df1 ? FITSNE_KERNEL_SQUARED_CAUCHY : FITSNE_KERNEL_SQUARED_GENERAL;
-    const int kind_1 = FITSNE_KERNEL_GENERAL;
-    TRY(ensure(c->zbuf, sizeof(double) * 4));
     if constexpr (D == 2) {
-        const size_t HH = (size_t)g.H * g.H;
-        const size_t plane = (size_t)g.H * g.n1d;
+        const size_t HH = (size_t)H * H;
+        const size_t plane = (size_t)H * n1d;
         TRY(ensure(c->Trow, sizeof(cx<float>) * plane));
         TRY(ensure(c->KQ, sizeof(float) * HH));
         TRY(ensure(c->K1, sizeof(float) * HH));
@@ -534,75 +578,72 @@ static int pipeline_f32(fitsne_ctx *c, int mode, const fitsne_step_params *sp) {
         TRY(ensure(c->KT, sizeof(float) * HH));
         TRY(ensure(c->spec, sizeof(cx<float>) * plane * 4));
         TRY(ensure(c->ospec, sizeof(cx<float>) * plane * 3));
-        TRY(ensure(c->zpart, sizeof(double) * 4 * g.H));
-        const int gr_rows = 8 * ((g.n1d + 7) / 8), gr_cols = 8 * ((g.H + 7) / 8);
-        if (!(c->kcache && c->kc_mn == mn && c->kc_mx == mx && c->kc_nb == g.nb)) {
-            launch(c, "kern_rows", k_kern_rows<float, kKernThreads, kKernTot>, dim3(gr_rows), dim3(kKernThreads),
-                   sizeof(cx<float>) * (1 + 1) * g.Mp, g, pl, tw, kind_q, kind_1, c->df, c->Trow.as<cx<float>>());
-            launch(c, "kern_cols", k_kern_cols<float, kKernThreads, kKernTot>, dim3(gr_cols), dim3(kKernThreads),
-                   sizeof(cx<float>) * (1 + 1) * g.Mp, g, pl, tw, c->Trow.as<const cx<float>>(), 1, c->KQ.as<float>(),
-                   c->K1.as<float>(), c->KS.as<float>(), c->KT.as<float>());
-            c->kcache = true;
-            c->kc_mn = mn;
-            c->kc_mx = mx;
-            c->kc_nb = g.nb;
-        }
-        launch(c, "rows_fwd", k_rows_fwd<float, kRowThreads, kRowTot>, dim3(gr_rows), dim3(kRowThreads),
-               sizeof(cx<float>) * (2 + 1) * g.Mp, g, pl, tw, (const float *)nullptr, 4, 4, c->spec.as<cx<float>>(),
-               c->fixgrid.as<unsigned long long>());
-        launch(c, "cols_f32", k_cols_f32<kColThreads, kColTot32>, dim3(gr_cols), dim3(kColThreads),
-               sizeof(cx<float>) * (4 + 1) * g.Mp, g, pl, tw, c->spec.as<const cx<float>>(), c->KQ.as<const float>(),
-               c->K1.as<const float>(), c->KS.as<const float>(), c->KT.as<const float>(), c->ospec.as<cx<float>>(),
-               c->zpart.as<double>());
-        launch(c, "zfinal", k_zfinal, dim3(1), dim3(kThreads), 0, c->zpart.as<const double>(), g.H, (int)df1,
-               1.0 / ((double)g.Mp * g.Mp), (double)N, c->zbuf.as<double>());
-        launch(c, "rows_inv", k_rows_inv<float, kRowThreads, kRowTot>, dim3(gr_rows), dim3(kRowThreads),
-               sizeof(cx<float>) * (2 + 1) * g.Mp, g, pl, tw, c->ospec.as<const cx<float>>(), 3,
-               (float)(1.0 / ((double)g.Mp * g.Mp)), c->gout.as<float>(), 4);
+        TRY(ensure(c->zpart, sizeof(double) * 4 * H));
     } else {
         TRY(ensure(c->zpart, sizeof(double) * 4));
-        launch(c, "grid_to_f32", k_grid_to_f32, dim3(blocks_for(nodes)), dim3(kThreads), 0,
-               c->fixgrid.as<unsigned long long>(), nodes, g, c->gin.as<float4>());
-        launch(c, "conv1d_f32", k_conv1d<float, kConv1dThreads, kMaxTot32>, dim3(1), dim3(kConv1dThreads),
-               sizeof(cx<float>) * 2 * g.Mp, g, pl, tw, kind_q, kind_1, c->df, 1, c->gin.as<const float>(), 4, 3, 0,
-               c->gout.as<float>(), 4, c->zpart.as<double>(), (const float *)nullptr);
-        launch(c, "zfinal", k_zfinal, dim3(1), dim3(kThreads), 0, c->zpart.as<const double>(), 1, (int)df1,
-               1.0 / (double)g.Mp, (double)N, c->zbuf.as<double>());
     }
+    return FITSNE_OK;
+}
 
-    UpdPrm up{};
-    up.mode = mode;
-    if (sp) {
-        up.momentum = (float)sp->momentum;
-        up.lr = (float)sp->learning_rate;
-        up.max_step_norm = (float)sp->max_step_norm;
-        up.plain_descent = sp->plain_descent;
+// mode 0: gradient into c->dC (float); mode 1: gradient + optimiser update + zeroMean.  The host plans
+// the geometry from the previous step's min / max (fetched while the attractive kernel runs).
+template <int D>
+static int pipeline_f32(fitsne_ctx *c, int mode, const fitsne_step_params *sp) {
+    TRY(ensure(c->pos, sizeof(float) * (size_t)c->N * D));
+    TRY(put_step(c, mode, sp));
+    const DevGeo *live;
+    TRY(live_record(c, &live));
+    TRY((launch_attractive<float, D>(c, (float)c->exag, c->pos.as<float>(), live)));
+    double mn, mx;
+    TRY(current_minmax<float>(c, &mn, &mx));
+    Geo g;
+    TRY(geometry(c, mn, mx, &g));
+    const cx<float> *tw;
+    TRY(twiddles<float>(c, g.Mp, &tw));
+    TRY(ensure_f32_buffers<D>(c, g.n1d, g.Mp));
+    const bool kern = D == 2 && !(c->kcache && c->kc_mn == mn && c->kc_mx == mx && c->kc_nb == g.nb);
+    TRY(put_geo(c, g, kern));
+    if (kern) {
+        c->kcache = true;
+        c->kc_mn = mn;
+        c->kc_mx = mx;
+        c->kc_nb = g.nb;
     }
-    const int nblk = blocks_for(N);
-    TRY(ensure(c->dC, sizeof(float) * (size_t)N * D));
-    TRY(ensure(c->part, sizeof(double) * (size_t)nblk * D));
-    launch(c, "interp_f32", pick_interp_f32<D>(p), dim3(nblk), dim3(kThreads), 0, c->Y.as<float>(),
-           c->pos.as<const float>(), c->gout.as<const float4>(), g, c->zbuf.as<const double>(), up, c->dC.as<float>(),
-           c->uY.as<float>(), c->gains.as<float>(), c->part.as<double>(), N);
+    const StageDims sd{g.Mp, 8 * ((g.n1d + 7) / 8), 8 * ((g.H + 7) / 8), blocks_for(g.n1d)};
+    TRY(launch_f32_stages<D>(c, tw, sd, kern, mode));
     if (mode == 1) {
-        TRY(ensure(c->mean, sizeof(double) * 2));
-        const long n = (long)N * D;
-        const int cblk = std::min(blocks_for(n), 1024);
-        TRY(ensure(c->mmpart, sizeof(double) * 2 * cblk));
-        TRY(ensure(c->mm, sizeof(double) * 4));
-        launch(c, "mean_final", k_mean_final, dim3(1), dim3(kThreads), 0, c->part.as<const double>(), nblk, D,
-               (double)N, c->mean.as<double>());
-        launch(c, "center", k_center<float>, dim3(cblk), dim3(kThreads), 0, c->Y.as<float>(), n, D, skip_coord(c),
-               c->mean.as<const double>(), c->mmpart.as<double>());
+        const int cblk = std::min(blocks_for((long)c->N * D), 1024);
         launch(c, "minmax_final", k_minmax_final<float>, dim3(1), dim3(kWave), 0, c->mmpart.as<const double>(), cblk,
                c->Y.as<const float>(), first_coord(c), c->mm.as<double>());
         HIPTRY(hipMemcpyAsync(c->h_mm, c->mm.p, sizeof(double) * 3, hipMemcpyDeviceToHost, c->stream));
         HIPTRY(hipEventRecord(c->mm_event, c->stream));
         c->mm_pending = true;
         c->mm_valid = false;
+        c->mm_parts_valid = true;
     }
     c->have_grad = true;
     HIPTRY(hipGetLastError());
+    return FITSNE_OK;
+}
+
+// One step of the FP32 form with its geometry planned on the device: no host round trip, so the
+// sequence can be captured once and replayed (hipGraph).  Grids and buffers are sized for the
+// build's limits; a step whose grid exceeds them is refused on the device (DevGeo.live = 0,
+// DevGeo.err = 1) and leaves the state unchanged.
+template <int D>
+static int launch_step_dev(fitsne_ctx *c) {
+    const int n1d_max = D == 2 ? kCapN1d2 : kCapN1d1, mp_max = D == 2 ? kCapMp2 : kCapMp1;
+    const int cblk = std::min(blocks_for((long)c->N * D), 1024);
+    launch(c, "geometry", k_geometry<float>, dim3(1), dim3(1024), 0, c->mmpart.as<const double>(), cblk,
+           c->Y.as<const float>(), first_coord(c), c->permuted ? c->inv_d.as<const int>() : (const int *)nullptr,
+           (long)c->N * D, D, c->p, c->ipi, c->mni, n1d_max, mp_max, c->dgeo.as<DevGeo>());
+    launch(c, "twiddles", k_twiddles_dev<float>, dim3(8), dim3(256), 0, c->dgeo.as<const DevGeo>(),
+           c->dtw.as<cx<float>>());
+    TRY((launch_attractive<float, D>(c, 1.f, c->pos.as<float>(), c->dgeo.as<const DevGeo>())));
+    const int H_max = mp_max / 2 + 1;
+    const StageDims sd{mp_max, std::min(8 * ((n1d_max + 7) / 8), 512), std::min(8 * ((H_max + 7) / 8), 512),
+                       std::min(blocks_for(n1d_max), 64)};
+    TRY(launch_f32_stages<D>(c, c->dtw.as<const cx<float>>(), sd, D == 2, 1));
     return FITSNE_OK;
 }
 
@@ -611,7 +652,7 @@ static int pipeline_f32(fitsne_ctx *c, int mode, const fitsne_step_params *sp) {
 // One convolution pass of the plain path: charges `set` (nt terms) spread, convolved with kernel
 // spectrum KC (2D) or kernel kind (1D), interpolated into pot[N][nt] (zeroed first).
 template <int D>
-static int plain_pass(fitsne_ctx *c, const Geo &g, const FftPlan &pl, const cx<double> *tw, int set, int nt,
+static int plain_pass(fitsne_ctx *c, const Geo &g, const cx<double> *tw, int set, int nt,
                       const double *KC, int kind1d, int use_k1, DBuf &pot) {
     const int N = c->N;
     const long nodes = D == 2 ? (long)g.n1d * g.n1d : g.n1d;
@@ -632,18 +673,18 @@ static int plain_pass(fitsne_ctx *c, const Geo &g, const FftPlan &pl, const cx<d
         const int gr_rows = 8 * ((g.n1d + 7) / 8), gr_cols = 8 * ((g.H + 7) / 8);
         const int lines = (nt + 1) / 2;
         launch(c, "rows_fwd64", k_rows_fwd<double, kRowThreads, kRowTot>, dim3(gr_rows), dim3(kRowThreads),
-               sizeof(cx<double>) * (lines + 1) * g.Mp, g, pl, tw, c->w.as<const double>(), nt, nt, c->spec.as<cx<double>>(),
-               (unsigned long long *)nullptr);
+               sizeof(cx<double>) * (lines + 1) * g.Mp, c->dgeo.as<const DevGeo>(), tw, c->w.as<const double>(), nt, nt,
+               c->spec.as<cx<double>>(), (unsigned long long *)nullptr);
         const int cl = std::min(nt, kColTot64 / g.Mp);
         launch(c, "cols_plain64", k_cols_plain<double, kColThreads64, kColTot64>, dim3(gr_cols), dim3(kColThreads64),
-               sizeof(cx<double>) * (cl + 1) * g.Mp, g, pl, tw, c->spec.as<const cx<double>>(), nt, KC,
+               sizeof(cx<double>) * (cl + 1) * g.Mp, c->dgeo.as<const DevGeo>(), tw, c->spec.as<const cx<double>>(), nt, KC,
                c->ospec.as<cx<double>>());
         launch(c, "rows_inv64", k_rows_inv<double, kRowThreads, kRowTot>, dim3(gr_rows), dim3(kRowThreads),
-               sizeof(cx<double>) * (lines + 1) * g.Mp, g, pl, tw, c->ospec.as<const cx<double>>(), nt,
-               1.0 / ((double)g.Mp * g.Mp), c->V.as<double>(), nt);
+               sizeof(cx<double>) * (lines + 1) * g.Mp, c->dgeo.as<const DevGeo>(), tw, c->ospec.as<const cx<double>>(), nt,
+               c->V.as<double>(), nt);
     } else {
         launch(c, "conv1d_f64", k_conv1d<double, kConv1dThreads, kMaxTot64>, dim3(1), dim3(kConv1dThreads),
-               sizeof(cx<double>) * g.Mp, g, pl, tw, kind1d, kind1d, c->df, 0, c->w.as<const double>(), nt, nt, use_k1,
+               sizeof(cx<double>) * g.Mp, c->dgeo.as<const DevGeo>(), tw, kind1d, kind1d, c->df, 0, c->w.as<const double>(), nt, nt, use_k1,
                c->V.as<double>(), nt, (double *)nullptr, (const double *)nullptr);
     }
     launch(c, "fill", k_fill<double>, dim3(std::min(blocks_for((long)N * nt), 2048)), dim3(kThreads), 0,
@@ -663,9 +704,9 @@ static int pipeline_f64(fitsne_ctx *c, int mode, const fitsne_step_params *sp) {
     TRY(current_minmax<double>(c, &mn, &mx));
     Geo g;
     TRY(geometry(c, mn, mx, &g));
-    const FftPlan pl = make_plan(g.Mp);
     const cx<double> *tw;
     TRY(twiddles<double>(c, g.Mp, &tw));
+    TRY(put_geo(c, g, true));
 
     const double *xs = c->Y.as<const double>(), *ys = nullptr;
     if constexpr (D == 2) {
@@ -691,10 +732,10 @@ static int pipeline_f64(fitsne_ctx *c, int mode, const fitsne_step_params *sp) {
         TRY(ensure(c->K1, sizeof(double) * HH));
         const int gr_rows = 8 * ((g.n1d + 7) / 8), gr_cols = 8 * ((g.H + 7) / 8);
         launch(c, "kern_rows64", k_kern_rows<double, kKernThreads, kKernTot>, dim3(gr_rows), dim3(kKernThreads),
-               sizeof(cx<double>) * (1 + 1) * g.Mp, g, pl, tw, kind_q, (int)FITSNE_KERNEL_GENERAL, c->df,
+               sizeof(cx<double>) * (1 + 1) * g.Mp, c->dgeo.as<const DevGeo>(), tw, kind_q, (int)FITSNE_KERNEL_GENERAL, c->df,
                c->Trow.as<cx<double>>());
         launch(c, "kern_cols64", k_kern_cols<double, kKernThreads, kKernTot>, dim3(gr_cols), dim3(kKernThreads),
-               sizeof(cx<double>) * (1 + 1) * g.Mp, g, pl, tw, c->Trow.as<const cx<double>>(), 0, c->KQ.as<double>(),
+               sizeof(cx<double>) * (1 + 1) * g.Mp, c->dgeo.as<const DevGeo>(), tw, c->Trow.as<const cx<double>>(), 0, c->KQ.as<double>(),
                c->K1.as<double>(), (double *)nullptr, (double *)nullptr);
         KQ = c->KQ.as<const double>();
         K1 = c->K1.as<const double>();
@@ -702,13 +743,13 @@ static int pipeline_f64(fitsne_ctx *c, int mode, const fitsne_step_params *sp) {
     int zform, cform;
     DBuf *potF = &c->potA;
     if (df1) {
-        TRY((plain_pass<D>(c, g, pl, tw, D == 2 ? 0 : 3, D == 2 ? 4 : 3, KQ, kind_q, 0, c->potA)));
+        TRY((plain_pass<D>(c, g, tw, D == 2 ? 0 : 3, D == 2 ? 4 : 3, KQ, kind_q, 0, c->potA)));
         zform = D == 2 ? 0 : 2;
         cform = D == 2 ? 0 : 2;
         potF = &c->potA;
     } else {
-        TRY((plain_pass<D>(c, g, pl, tw, D == 2 ? 1 : 4, D == 2 ? 3 : 2, KQ, kind_q, 0, c->potA)));
-        TRY((plain_pass<D>(c, g, pl, tw, 2, 1, K1, FITSNE_KERNEL_GENERAL, 1, c->potB)));
+        TRY((plain_pass<D>(c, g, tw, D == 2 ? 1 : 4, D == 2 ? 3 : 2, KQ, kind_q, 0, c->potA)));
+        TRY((plain_pass<D>(c, g, tw, 2, 1, K1, FITSNE_KERNEL_GENERAL, 1, c->potB)));
         zform = 1;
         cform = D == 2 ? 1 : 3;
         potF = &c->potA;
@@ -733,9 +774,9 @@ static int pipeline_f64(fitsne_ctx *c, int mode, const fitsne_step_params *sp) {
         const int cblk = std::min(blocks_for(n), 1024);
         TRY(ensure(c->mmpart, sizeof(double) * 2 * cblk));
         TRY(ensure(c->mm, sizeof(double) * 4));
-        launch(c, "mean_final", k_mean_final, dim3(1), dim3(kThreads), 0, c->part.as<const double>(), nblk, D,
+        launch(c, "mean_final", k_mean_final, dim3(1), dim3(kThreads), 0, c->dgeo.as<const DevGeo>(), c->part.as<const double>(), nblk, D,
                (double)N, c->mean.as<double>());
-        launch(c, "center64", k_center<double>, dim3(cblk), dim3(kThreads), 0, c->Y.as<double>(), n, D,
+        launch(c, "center64", k_center<double>, dim3(cblk), dim3(kThreads), 0, c->dgeo.as<const DevGeo>(), c->Y.as<double>(), n, D,
                skip_coord(c), c->mean.as<const double>(), c->mmpart.as<double>());
         launch(c, "minmax_final", k_minmax_final<double>, dim3(1), dim3(kWave), 0, c->mmpart.as<const double>(), cblk,
                c->Y.as<const double>(), first_coord(c), c->mm.as<double>());
@@ -821,8 +862,10 @@ extern "C" int fitsne_ctx_destroy(fitsne_ctx *c) {
     if (!c) return FITSNE_OK;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    drop_graph(c);
     DBuf *all[] = {&c->rowP, &c->colP, &c->val, &c->Y, &c->uY, &c->gains, &c->pos, &c->dC, &c->wcount, &c->wstart,
-                   &c->wkey, &c->wval, &c->wbmax, &c->knnX, &c->knnXT, &c->knnI, &c->knnD, &c->gin, &c->gout, &c->spec, &c->ospec, &c->Trow, &c->KQ,
+                   &c->wkey, &c->wval, &c->wbmax, &c->knnX, &c->knnXT, &c->knnI, &c->knnD, &c->dgeo,
+                   &c->dstep, &c->dtw, &c->dlive, &c->klpart, &c->gin, &c->gout, &c->spec, &c->ospec, &c->Trow, &c->KQ,
                    &c->K1, &c->KS, &c->KT, &c->zpart, &c->zbuf, &c->part, &c->mean, &c->mmpart, &c->mm, &c->xs,
                    &c->ys, &c->box, &c->wx, &c->wy, &c->ch, &c->w, &c->V, &c->potA, &c->potB,
                    &c->inv_d, &c->colP2, &c->val2, &c->tmp, &c->fixgrid};
@@ -881,6 +924,8 @@ static int set_P_impl(fitsne_ctx *c, int N, const TI *row_P, const TC *col_P, co
     for (int i = 0; i < N; i++) c->perm[i] = c->inv[i] = i;
     c->permuted = false;
     c->win_ok = false;
+    drop_graph(c);
+    c->mm_parts_valid = false;
     return FITSNE_OK;
 }
 
@@ -929,6 +974,8 @@ static int reorder_points(fitsne_ctx *c) {
     const int N = c->N, D = c->D;
     if (c->precision != FITSNE_FP32 || N < 2) return FITSNE_OK;
     TRY(sync_ctx(c));
+    drop_graph(c);
+    c->mm_parts_valid = false;
     std::vector<float> y((size_t)N * D);
     HIPTRY(hipMemcpy(y.data(), c->Y.p, sizeof(float) * y.size(), hipMemcpyDeviceToHost));
     float lo[2] = {INFINITY, INFINITY}, hi[2] = {-INFINITY, -INFINITY};
@@ -1048,6 +1095,8 @@ extern "C" int fitsne_set_Y(fitsne_ctx *c, const double *Y) {
     }
     c->mm_valid = c->mm_pending = false;
     c->have_grad = false;
+    drop_graph(c);
+    c->mm_parts_valid = false;
     if (c->precision == FITSNE_FP32 && c->reorder_on_set_Y && c->N >= kReorderMinN) TRY(reorder_points(c));
     return FITSNE_OK;
 }
@@ -1055,6 +1104,7 @@ extern "C" int fitsne_set_Y(fitsne_ctx *c, const double *Y) {
 extern "C" int fitsne_get_Y(fitsne_ctx *c, double *Y) {
     if (!c || !Y) return fail(FITSNE_E_ARG, "null argument");
     HIPTRY(hipSetDevice(c->device));
+    TRY(check_graph(c));
     TRY(sync_ctx(c));
     const size_t n = (size_t)c->N * c->D;
     if (c->precision == FITSNE_FP32) {
@@ -1071,6 +1121,7 @@ extern "C" int fitsne_get_Y(fitsne_ctx *c, double *Y) {
 
 extern "C" int fitsne_gradient(fitsne_ctx *c, double *dC, double *sum_Q) {
     if (!c) return fail(FITSNE_E_ARG, "null context");
+    TRY(check_graph(c));
     TRY(run_pipeline(c, 0, nullptr));
     TRY(read_sumq(c, sum_Q));
     if (dC) {
@@ -1088,22 +1139,255 @@ extern "C" int fitsne_gradient(fitsne_ctx *c, double *dC, double *sum_Q) {
     return FITSNE_OK;
 }
 
+// ============================================================================ captured steps
+
+static void drop_graph(fitsne_ctx *c) {
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->gexec = nullptr;
+    c->graph = nullptr;
+}
+
+// Capture one device-planned step of the FP32 form (launch_step_dev) on the context's stream.  All
+// buffers are allocated first at the build's largest grid; nothing is allocated during capture.
+static int capture_step(fitsne_ctx *c) {
+    drop_graph(c);
+    if (c->precision != FITSNE_FP32) return fail(FITSNE_E_UNSUPPORTED, "captured steps are for the FP32 form");
+    if (c->N <= 0) return fail(FITSNE_E_ARG, "P and Y must be set before a step");
+    HIPTRY(hipSetDevice(c->device));
+    if (!c->win_ok) TRY(build_windows(c));
+    if (!c->win_ok) return fail(FITSNE_E_UNSUPPORTED, "captured steps need the windowed attractive layout");
+    const int D = c->D;
+    TRY(ensure(c->dgeo, sizeof(DevGeo)));
+    TRY(ensure(c->dstep, sizeof(StepDev)));
+    TRY(ensure(c->dtw, sizeof(cx<float>) * (size_t)(D == 2 ? kCapMp2 : kCapMp1)));
+    TRY(D == 2 ? ensure_f32_buffers<2>(c, kCapN1d2, kCapMp2) : ensure_f32_buffers<1>(c, kCapN1d1, kCapMp1));
+    DevGeo v{};  // no spectra or twiddles cached yet
+    v.live = 1;
+    launch(c, "set_geo", k_set_geo, dim3(1), dim3(kWave), 0, v, c->dgeo.as<DevGeo>());
+    TRY(sync_ctx(c));
+    const bool timing = c->timing;
+    c->timing = false;
+    HIPTRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = D == 2 ? launch_step_dev<2>(c) : launch_step_dev<1>(c);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    c->timing = timing;
+    if (rc != FITSNE_OK || e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        if (rc != FITSNE_OK) return rc;
+        HIPTRY(e);
+    }
+    hipGraphExec_t x = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        HIPTRY(ie);
+    }
+    c->graph = g;
+    c->gexec = x;
+    c->graph_gen++;
+    return FITSNE_OK;
+}
+
+// Min / max partials of the resident Y (what a captured step's geometry starts from).
+static int ensure_partials(fitsne_ctx *c) {
+    if (c->mm_parts_valid) return FITSNE_OK;
+    const long n = (long)c->N * c->D;
+    const int nblk = std::min(blocks_for(n), 1024);
+    TRY(ensure(c->mmpart, sizeof(double) * 2 * nblk));
+    launch(c, "minmax_partial", k_minmax_partial<float>, dim3(nblk), dim3(kThreads), 0, c->Y.as<const float>(), n,
+           skip_coord(c), c->mmpart.as<double>());
+    c->mm_parts_valid = true;
+    return FITSNE_OK;
+}
+
+// After captured steps: the host's min / max must be recomputed, and a refused step is an error.
+static void after_graph_steps(fitsne_ctx *c) {
+    c->graph_ran = true;
+    c->have_grad = true;
+    c->mm_valid = false;
+    c->mm_pending = false;
+}
+
+static int check_graph(fitsne_ctx *c) {
+    if (!c->graph_ran) return FITSNE_OK;
+    DevGeo v;
+    HIPTRY(hipMemcpyAsync(&v, c->dgeo.p, sizeof(DevGeo), hipMemcpyDeviceToHost, c->stream));
+    TRY(sync_ctx(c));
+    c->graph_ran = false;
+    c->last = v.g;
+    c->last_min = v.mn;
+    c->last_max = v.mx;
+    if (v.err) {
+        launch(c, "clear_err", k_clear_err, dim3(1), dim3(kWave), 0, c->dgeo.as<DevGeo>());
+        return fail(FITSNE_E_UNSUPPORTED, "a captured step's grid exceeded this build's limits (FFT length " +
+                                              std::to_string(c->D == 2 ? kCapMp2 : kCapMp1) +
+                                              "); the embedding was left at the previous step");
+    }
+    return FITSNE_OK;
+}
+
+extern "C" int fitsne_set_graph(fitsne_ctx *c, int enable) {
+    if (!c) return fail(FITSNE_E_ARG, "null context");
+    if (enable && c->precision != FITSNE_FP32) return fail(FITSNE_E_UNSUPPORTED, "captured steps are for the FP32 form");
+    drop_graph(c);
+    c->use_graph = enable != 0;
+    return FITSNE_OK;
+}
+
 extern "C" int fitsne_step(fitsne_ctx *c, const fitsne_step_params *prm, double *sum_Q) {
     if (!c || !prm) return fail(FITSNE_E_ARG, "null argument");
+    if (c->use_graph) {
+        if (c->N <= 0) return fail(FITSNE_E_ARG, "P and Y must be set before a step");
+        HIPTRY(hipSetDevice(c->device));
+        if (!c->gexec) TRY(capture_step(c));
+        TRY(ensure_partials(c));
+        TRY(put_step(c, 1, prm));
+        HIPTRY(hipGraphLaunch(c->gexec, c->stream));
+        after_graph_steps(c);
+        if (sum_Q) {
+            TRY(check_graph(c));
+            TRY(read_sumq(c, sum_Q));
+        }
+        return FITSNE_OK;
+    }
     TRY(run_pipeline(c, 1, prm));
     if (sum_Q) TRY(read_sumq(c, sum_Q));
     return FITSNE_OK;
 }
 
+// ============================================================================ batched embeddings
+
+struct fitsne_batch {
+    std::vector<fitsne_ctx *> ctx;
+    std::vector<long> gen;  // capture generation of each context's step in the parent graph
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t done = nullptr;
+    std::vector<hipEvent_t> ready;
+    DBuf dst;  // StepDev* per context
+};
+
+constexpr int kBatchChunk = 64;
+struct BatchSteps {
+    StepDev v[kBatchChunk];
+};
+__global__ void k_set_steps(BatchSteps v, StepDev *const *__restrict__ dst, int n) {
+    const int i = threadIdx.x;
+    if (i < n) *dst[i] = v.v[i];
+}
+
+extern "C" int fitsne_batch_create(fitsne_batch **out, fitsne_ctx **ctxs, int n) {
+    if (!out || !ctxs || n <= 0) return fail(FITSNE_E_ARG, "invalid batch");
+    *out = nullptr;
+    for (int i = 0; i < n; i++) {
+        if (!ctxs[i] || ctxs[i]->precision != FITSNE_FP32 || ctxs[i]->device != ctxs[0]->device)
+            return fail(FITSNE_E_ARG, "a batch holds FP32 contexts on one device");
+        for (int j = 0; j < i; j++)
+            if (ctxs[j] == ctxs[i]) return fail(FITSNE_E_ARG, "a context appears twice in the batch");
+    }
+    auto *b = new fitsne_batch();
+    b->ctx.assign(ctxs, ctxs + n);
+    b->gen.assign(n, -1);
+    b->ready.assign(n, nullptr);
+    const hipError_t e1 = hipSetDevice(ctxs[0]->device);
+    const hipError_t e2 = e1 == hipSuccess ? hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) : e1;
+    const hipError_t e3 = e2 == hipSuccess ? hipEventCreateWithFlags(&b->done, hipEventDisableTiming) : e2;
+    hipError_t e4 = e3;
+    for (int i = 0; i < n && e4 == hipSuccess; i++) e4 = hipEventCreateWithFlags(&b->ready[i], hipEventDisableTiming);
+    if (e4 != hipSuccess) {
+        fitsne_batch_destroy(b);
+        return fail(FITSNE_E_HIP, std::string("batch setup: ") + hipGetErrorString(e4));
+    }
+    *out = b;
+    return FITSNE_OK;
+}
+
+extern "C" int fitsne_batch_destroy(fitsne_batch *b) {
+    if (!b) return FITSNE_OK;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (auto e : b->ready)
+        if (e) (void)hipEventDestroy(e);
+    if (b->done) (void)hipEventDestroy(b->done);
+    release(b->dst);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return FITSNE_OK;
+}
+
+// One step of every embedding of the batch: their captured steps as parallel branches of one graph,
+// launched once on the batch's stream.
+extern "C" int fitsne_batch_step(fitsne_batch *b, const fitsne_step_params *prm) {
+    if (!b || !prm) return fail(FITSNE_E_ARG, "null argument");
+    const int n = (int)b->ctx.size();
+    HIPTRY(hipSetDevice(b->ctx[0]->device));
+    bool rebuild = b->exec == nullptr;
+    for (int i = 0; i < n; i++) {
+        fitsne_ctx *c = b->ctx[i];
+        c->use_graph = true;
+        if (!c->gexec) TRY(capture_step(c));
+        if (c->graph_gen != b->gen[i]) rebuild = true;
+        TRY(ensure_partials(c));
+        HIPTRY(hipEventRecord(b->ready[i], c->stream));
+        HIPTRY(hipStreamWaitEvent(b->stream, b->ready[i], 0));
+    }
+    if (rebuild) {
+        if (b->exec) (void)hipGraphExecDestroy(b->exec);
+        if (b->graph) (void)hipGraphDestroy(b->graph);
+        b->exec = nullptr;
+        b->graph = nullptr;
+        HIPTRY(hipGraphCreate(&b->graph, 0));
+        for (int i = 0; i < n; i++) {
+            hipGraphNode_t node;
+            HIPTRY(hipGraphAddChildGraphNode(&node, b->graph, nullptr, 0, b->ctx[i]->graph));
+            b->gen[i] = b->ctx[i]->graph_gen;
+        }
+        HIPTRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+        std::vector<StepDev *> dst(n);
+        for (int i = 0; i < n; i++) dst[i] = b->ctx[i]->dstep.as<StepDev>();
+        TRY(ensure(b->dst, sizeof(StepDev *) * n));
+        HIPTRY(hipMemcpyAsync(b->dst.p, dst.data(), sizeof(StepDev *) * n, hipMemcpyHostToDevice, b->stream));
+        HIPTRY(hipStreamSynchronize(b->stream));
+    }
+    for (int i0 = 0; i0 < n; i0 += kBatchChunk) {
+        BatchSteps v{};
+        const int m = std::min(kBatchChunk, n - i0);
+        for (int i = 0; i < m; i++) {
+            const fitsne_step_params &q = prm[i0 + i];
+            StepDev &d = v.v[i];
+            d.mode = 1;
+            d.exag = (float)b->ctx[i0 + i]->exag;
+            d.momentum = (float)q.momentum;
+            d.lr = (float)q.learning_rate;
+            d.max_step_norm = (float)q.max_step_norm;
+            d.plain_descent = q.plain_descent;
+        }
+        k_set_steps<<<1, kBatchChunk, 0, b->stream>>>(v, b->dst.as<StepDev *const>() + i0, m);
+        HIPTRY(hipGetLastError());
+    }
+    HIPTRY(hipGraphLaunch(b->exec, b->stream));
+    HIPTRY(hipEventRecord(b->done, b->stream));
+    for (int i = 0; i < n; i++) {
+        HIPTRY(hipStreamWaitEvent(b->ctx[i]->stream, b->done, 0));
+        after_graph_steps(b->ctx[i]);
+    }
+    return FITSNE_OK;
+}
+
 extern "C" int fitsne_kl(fitsne_ctx *c, double *C) {
     if (!c || !C) return fail(FITSNE_E_ARG, "null argument");
+    TRY(check_graph(c));
     if (!c->have_grad) return fail(FITSNE_E_ARG, "KL needs the Z of a previous gradient");
     HIPTRY(hipSetDevice(c->device));
     const int sw = sw_for(c->nnz, c->N);
     (void)sw;
     const long threads = (long)c->N * 32;
     const int nblk = blocks_for(threads);
-    TRY(ensure(c->part, sizeof(double) * (size_t)std::max(nblk, 2)));
+    TRY(ensure(c->klpart, sizeof(double) * (size_t)std::max(nblk, 2)));
     TRY(ensure(c->mean, sizeof(double) * 2));
     const double pscale = c->precision == FITSNE_FP32 ? c->exag : 1.0;
     auto rp = c->rowP.as<const int64_t>();
@@ -1112,22 +1396,22 @@ extern "C" int fitsne_kl(fitsne_ctx *c, double *C) {
         if (c->D == 2)
             launch(c, "kl", k_kl<float, int64_t, uint32_t, 2, 32>, dim3(nblk), dim3(kThreads), 0, rp, cp,
                    c->val.as<const float>(), pscale, c->Y.as<const float>(), c->N, c->df, c->zbuf.as<const double>(),
-                   c->part.as<double>());
+                   c->klpart.as<double>());
         else
             launch(c, "kl", k_kl<float, int64_t, uint32_t, 1, 32>, dim3(nblk), dim3(kThreads), 0, rp, cp,
                    c->val.as<const float>(), pscale, c->Y.as<const float>(), c->N, c->df, c->zbuf.as<const double>(),
-                   c->part.as<double>());
+                   c->klpart.as<double>());
     } else {
         if (c->D == 2)
             launch(c, "kl", k_kl<double, int64_t, uint32_t, 2, 32>, dim3(nblk), dim3(kThreads), 0, rp, cp,
                    c->val.as<const double>(), pscale, c->Y.as<const double>(), c->N, c->df,
-                   c->zbuf.as<const double>(), c->part.as<double>());
+                   c->zbuf.as<const double>(), c->klpart.as<double>());
         else
             launch(c, "kl", k_kl<double, int64_t, uint32_t, 1, 32>, dim3(nblk), dim3(kThreads), 0, rp, cp,
                    c->val.as<const double>(), pscale, c->Y.as<const double>(), c->N, c->df,
-                   c->zbuf.as<const double>(), c->part.as<double>());
+                   c->zbuf.as<const double>(), c->klpart.as<double>());
     }
-    launch(c, "kl_sum", k_sum_final, dim3(1), dim3(kThreads), 0, c->part.as<const double>(), nblk, 0.0,
+    launch(c, "kl_sum", k_sum_final, dim3(1), dim3(kThreads), 0, c->klpart.as<const double>(), nblk, 0.0,
            c->mean.as<double>());
     HIPTRY(hipMemcpyAsync(c->h_mm + 3, c->mean.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     TRY(sync_ctx(c));
@@ -1138,7 +1422,8 @@ extern "C" int fitsne_kl(fitsne_ctx *c, double *C) {
 extern "C" int fitsne_sync(fitsne_ctx *c) {
     if (!c) return fail(FITSNE_E_ARG, "null context");
     HIPTRY(hipSetDevice(c->device));
-    return sync_ctx(c);
+    TRY(sync_ctx(c));
+    return check_graph(c);
 }
 
 extern "C" int fitsne_set_timing(fitsne_ctx *c, int enable) {
@@ -1168,6 +1453,8 @@ extern "C" int fitsne_kernel_stats(fitsne_ctx *c, int cap, char *names, int *cou
 
 extern "C" int fitsne_last_geometry(fitsne_ctx *c, double *out) {
     if (!c || !out) return fail(FITSNE_E_ARG, "null argument");
+    HIPTRY(hipSetDevice(c->device));
+    TRY(check_graph(c));
     out[0] = c->last.nb;
     out[1] = c->last.n1d;
     out[2] = c->last.Mp;
@@ -1299,18 +1586,18 @@ extern "C" int fitsne_precompute_2d(double x_max, double x_min, double y_max, do
     HIPTRY(hipSetDevice(c->device));
     Geo g = mirror_geo(2, p, n_boxes, x_min, box_width, nullptr, nullptr);
     if (g.Mp > kMaxLen2d) return fail(FITSNE_E_UNSUPPORTED, "grid too large for this build");
-    const FftPlan pl = make_plan(g.Mp);
     const cx<double> *tw;
     TRY(twiddles<double>(c, g.Mp, &tw));
+    TRY(put_geo(c, g, true));
     const size_t HH = (size_t)g.H * g.H;
     TRY(ensure(c->Trow, sizeof(cx<double>) * (size_t)g.H * g.n1d));
     TRY(ensure(c->KQ, sizeof(double) * HH));
     TRY(ensure(c->K1, sizeof(double) * HH));
     const int gr_rows = 8 * ((g.n1d + 7) / 8), gr_cols = 8 * ((g.H + 7) / 8);
     launch(c, "kern_rows64", k_kern_rows<double, kKernThreads, kKernTot>, dim3(gr_rows), dim3(kKernThreads),
-           sizeof(cx<double>) * (1 + 1) * g.Mp, g, pl, tw, kernel, kernel, df, c->Trow.as<cx<double>>());
+           sizeof(cx<double>) * (1 + 1) * g.Mp, c->dgeo.as<const DevGeo>(), tw, kernel, kernel, df, c->Trow.as<cx<double>>());
     launch(c, "kern_cols64", k_kern_cols<double, kKernThreads, kKernTot>, dim3(gr_cols), dim3(kKernThreads),
-           sizeof(cx<double>) * (1 + 1) * g.Mp, g, pl, tw, c->Trow.as<const cx<double>>(), 0, c->KQ.as<double>(),
+           sizeof(cx<double>) * (1 + 1) * g.Mp, c->dgeo.as<const DevGeo>(), tw, c->Trow.as<const cx<double>>(), 0, c->KQ.as<double>(),
            c->K1.as<double>(), (double *)nullptr, (double *)nullptr);
     HIPTRY(hipMemcpyAsync(fft_kernel_tilde, c->KQ.p, sizeof(double) * HH, hipMemcpyDeviceToHost, c->stream));
     return sync_ctx(c);
@@ -1341,9 +1628,9 @@ extern "C" int fitsne_n_body_fft_2d(int N, int n_terms, const double *xs, const 
             if (i != j) g.den[i] *= g.sp[i] - g.sp[j];
     }
     if (g.Mp > kMaxLen2d) return fail(FITSNE_E_UNSUPPORTED, "grid too large for this build");
-    const FftPlan pl = make_plan(g.Mp);
     const cx<double> *tw;
     TRY(twiddles<double>(c, g.Mp, &tw));
+    TRY(put_geo(c, g, true));
     // caller's per-axis lower bounds: x lower of column j at [j], y lower of row i at [ntb + i*nb]
     std::vector<double> lo(2 * (size_t)n_boxes);
     for (int j = 0; j < n_boxes; j++) lo[j] = box_lower_bounds[j];
@@ -1382,15 +1669,15 @@ extern "C" int fitsne_n_body_fft_2d(int N, int n_terms, const double *xs, const 
     const int gr_rows = 8 * ((g.n1d + 7) / 8), gr_cols = 8 * ((g.H + 7) / 8);
     const int lines = (nt + 1) / 2;
     launch(c, "rows_fwd64", k_rows_fwd<double, kRowThreads, kRowTot>, dim3(gr_rows), dim3(kRowThreads),
-           sizeof(cx<double>) * (lines + 1) * g.Mp, g, pl, tw, c->w.as<const double>(), nt, nt, c->spec.as<cx<double>>(),
+           sizeof(cx<double>) * (lines + 1) * g.Mp, c->dgeo.as<const DevGeo>(), tw, c->w.as<const double>(), nt, nt, c->spec.as<cx<double>>(),
            (unsigned long long *)nullptr);
     const int cl = std::min(nt, kColTot64 / g.Mp);
     launch(c, "cols_plain64", k_cols_plain<double, kColThreads64, kColTot64>, dim3(gr_cols), dim3(kColThreads64),
-           sizeof(cx<double>) * (cl + 1) * g.Mp, g, pl, tw, c->spec.as<const cx<double>>(), nt, c->KQ.as<const double>(),
+           sizeof(cx<double>) * (cl + 1) * g.Mp, c->dgeo.as<const DevGeo>(), tw, c->spec.as<const cx<double>>(), nt, c->KQ.as<const double>(),
            c->ospec.as<cx<double>>());
     launch(c, "rows_inv64", k_rows_inv<double, kRowThreads, kRowTot>, dim3(gr_rows), dim3(kRowThreads),
-           sizeof(cx<double>) * (lines + 1) * g.Mp, g, pl, tw, c->ospec.as<const cx<double>>(), nt,
-           1.0 / ((double)g.Mp * g.Mp), c->V.as<double>(), nt);
+           sizeof(cx<double>) * (lines + 1) * g.Mp, c->dgeo.as<const DevGeo>(), tw, c->ospec.as<const cx<double>>(), nt,
+           c->V.as<double>(), nt);
     launch(c, "interp_f64", pick_interp_f64<2>(p), dim3(blocks_for(N)), dim3(kThreads), 0, c->box.as<const int>(),
            c->wx.as<const double>(), c->wy.as<const double>(), c->V.as<const double>(), nt, N, g, c->potA.as<double>());
     HIPTRY(hipMemcpyAsync(potentialQij, c->potA.p, sizeof(double) * (size_t)N * n_terms, hipMemcpyDeviceToHost,
@@ -1425,12 +1712,12 @@ extern "C" int fitsne_precompute_1d(double y_min, double y_max, int n_boxes, int
     HIPTRY(hipSetDevice(c->device));
     Geo g = mirror_geo(1, p, n_boxes, y_min, box_width, nullptr, nullptr);
     if (g.Mp > kMaxTot64) return fail(FITSNE_E_UNSUPPORTED, "grid too large for this build");
-    const FftPlan pl = make_plan(g.Mp);
     const cx<double> *tw;
     TRY(twiddles<double>(c, g.Mp, &tw));
+    TRY(put_geo(c, g, true));
     TRY(ensure(c->KQ, sizeof(double) * g.Mp));
     launch(c, "kspec1d", k_kspec1d<double, kConv1dThreads, kMaxTot64>, dim3(1), dim3(kConv1dThreads),
-           sizeof(cx<double>) * g.Mp, g, pl, tw, kernel, df, c->KQ.as<double>());
+           sizeof(cx<double>) * g.Mp, c->dgeo.as<const DevGeo>(), tw, kernel, df, c->KQ.as<double>());
     HIPTRY(hipMemcpyAsync(fft_kernel_vector, c->KQ.p, sizeof(double) * g.Mp, hipMemcpyDeviceToHost, c->stream));
     return sync_ctx(c);
 }
@@ -1459,9 +1746,9 @@ extern "C" int fitsne_n_body_fft_1d(int N, int n_terms, const double *Y, const d
             if (i != j) g.den[i] *= g.sp[i] - g.sp[j];
     }
     if (g.Mp > kMaxTot64) return fail(FITSNE_E_UNSUPPORTED, "grid too large for this build");
-    const FftPlan pl = make_plan(g.Mp);
     const cx<double> *tw;
     TRY(twiddles<double>(c, g.Mp, &tw));
+    TRY(put_geo(c, g, true));
     const int nt = n_terms;
     TRY(ensure(c->xs, sizeof(double) * N));
     TRY(ensure(c->mean, sizeof(double) * n_boxes));
@@ -1486,7 +1773,7 @@ extern "C" int fitsne_n_body_fft_1d(int N, int n_terms, const double *Y, const d
     launch(c, "spread_f64", pick_spread_f64<1>(p), dim3(blocks_for(N)), dim3(kThreads), 0, c->box.as<const int>(),
            c->wx.as<const double>(), c->wy.as<const double>(), c->ch.as<const double>(), nt, N, g, c->w.as<double>());
     launch(c, "conv1d_f64", k_conv1d<double, kConv1dThreads, kMaxTot64>, dim3(1), dim3(kConv1dThreads),
-           sizeof(cx<double>) * g.Mp, g, pl, tw, 0, 0, 1.0, 0, c->w.as<const double>(), nt, nt, 0, c->V.as<double>(),
+           sizeof(cx<double>) * g.Mp, c->dgeo.as<const DevGeo>(), tw, 0, 0, 1.0, 0, c->w.as<const double>(), nt, nt, 0, c->V.as<double>(),
            nt, (double *)nullptr, c->KQ.as<const double>());
     launch(c, "interp_f64", pick_interp_f64<1>(p), dim3(blocks_for(N)), dim3(kThreads), 0, c->box.as<const int>(),
            c->wx.as<const double>(), c->wy.as<const double>(), c->V.as<const double>(), nt, N, g, c->potA.as<double>());

@@ -261,9 +261,12 @@ __global__ __launch_bounds__(kAttThreads) void k_attr_win(const int *__restrict_
                                                           const float *__restrict__ wval,
                                                           const float *__restrict__ bmax,
                                                           const float *__restrict__ Y, int N, int R, int NW,
-                                                          float inv_df, int df_one, float mdf, float scale,
-                                                          float *__restrict__ pos) {
+                                                          float inv_df, int df_one, float mdf,
+                                                          const StepDev *__restrict__ sd,
+                                                          const DevGeo *__restrict__ dg, float *__restrict__ pos) {
     constexpr int WL = kWin * D / kAttThreads;  // window floats per thread
+    if (!dg->live) return;
+    const float scale = sd->exag;
     __shared__ float yw[kWin * D];
     __shared__ float yr[kAttRows * D];
     __shared__ unsigned long long acc[kAttRows * D];
@@ -381,9 +384,9 @@ __device__ __forceinline__ double seq_at(const TV *__restrict__ Y, const int *__
     return pos ? (double)Y[2L * pos[e >> 1] + (e & 1)] : (double)Y[e];
 }
 
+// One block of 1024 threads; out[0] = max, out[1] = min.
 template <typename TV>
-__global__ __launch_bounds__(1024) void k_minmax_seq(const TV *__restrict__ Y, const int *__restrict__ pos, long n,
-                                                     double *__restrict__ out) {
+__device__ void minmax_seq_block(const TV *__restrict__ Y, const int *__restrict__ pos, long n, double *out) {
     __shared__ double cmax[1024];
     __shared__ double cmin[1024];
     const int t = threadIdx.x;
@@ -416,6 +419,104 @@ __global__ __launch_bounds__(1024) void k_minmax_seq(const TV *__restrict__ Y, c
         for (int i = 0; i < 1024; i++) r = fmin(r, cmin[i]);
         out[1] = r;
     }
+    __syncthreads();
+}
+
+template <typename TV>
+__global__ __launch_bounds__(1024) void k_minmax_seq(const TV *__restrict__ Y, const int *__restrict__ pos, long n,
+                                                     double *__restrict__ out) {
+    minmax_seq_block(Y, pos, n, out);
+}
+
+// ---------------------------------------------------------------- step setup on the device
+// The geometry of a captured step, from the min / max partials the previous step (or the upload of Y)
+// left: max and min(rest) from the partials; when Y[first] is the strict minimum the sequential
+// 2D rule is replayed (tsne.cpp:1045-1048); then the box count, geometry and plan of make_geo /
+// make_plan, the refusal of grids beyond n1d_max / mp_max (live = 0, err = 1), and whether the
+// kernel spectra and twiddles of the previous step still apply.  One block of 1024 threads.
+template <typename TV>
+__global__ __launch_bounds__(1024) void k_geometry(const double *__restrict__ part, int nblk, const TV *__restrict__ Y,
+                                                   long first, const int *__restrict__ pos, long n, int D, int p,
+                                                   double ipi, int mni, int n1d_max, int mp_max,
+                                                   DevGeo *__restrict__ dg) {
+    __shared__ double shx[1024 / kWave], shn[1024 / kWave];
+    __shared__ double res[2];
+    double mx = -INFINITY, mn = INFINITY;
+    for (int b = threadIdx.x; b < nblk; b += 1024) {
+        mx = fmax(mx, part[2 * b]);
+        mn = fmin(mn, part[2 * b + 1]);
+    }
+    mx = wave_max(mx);
+    mn = wave_min(mn);
+    const int w = threadIdx.x / kWave;
+    if (threadIdx.x % kWave == 0) {
+        shx[w] = mx;
+        shn[w] = mn;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < 1024 / kWave; i++) {
+            mx = fmax(mx, shx[i]);
+            mn = fmin(mn, shn[i]);
+        }
+        res[0] = mx;
+        res[1] = mn;
+    }
+    __syncthreads();
+    mx = res[0];
+    mn = res[1];
+    if (D == 2 && (double)Y[first] < mn) {  // uniform over the block
+        __syncthreads();
+        minmax_seq_block(Y, pos, n, res);
+        mx = res[0];
+        mn = res[1];
+    }
+    if (threadIdx.x == 0) {
+        DevGeo o = *dg;
+        o.mn = mn;
+        o.mx = mx;
+        const bool fin = isfinite(mn) && isfinite(mx);
+        const int nb = fin ? (D == 2 ? n_boxes_2d(mn, mx, ipi, mni) : n_boxes_1d(mn, mx, ipi, mni)) : 0;
+        Geo g{};
+        if (nb >= 1) g = make_geo(D, p, mn, mx, nb);
+        o.live = fin && nb >= 1 && g.n1d <= n1d_max && g.Mp <= mp_max;
+        if (o.live) {
+            o.g = g;
+            o.pl = make_plan(g.Mp);
+            o.kspec_stale = !(o.kc_nb == nb && o.kc_mn == mn && o.kc_mx == mx);
+            o.kc_nb = nb;
+            o.kc_mn = mn;
+            o.kc_mx = mx;
+            o.tw_stale = o.tw_n != g.Mp;
+            o.tw_n = g.Mp;
+        } else {
+            o.err = 1;
+        }
+        *dg = o;
+    }
+}
+
+// Twiddle table of the step's FFT length, when it changed.
+template <typename T>
+__global__ void k_twiddles_dev(const DevGeo *__restrict__ dg, cx<T> *__restrict__ tw) {
+    if (!dg->live || !dg->tw_stale) return;
+    const int n = dg->g.Mp;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+        double sn, cs;
+        sincospi(-2.0 * (double)e / (double)n, &sn, &cs);
+        tw[e] = cx<T>{(T)cs, (T)sn};
+    }
+}
+
+// Host-computed geometry / step parameters into device memory (stream-ordered, no host buffer kept).
+__global__ void k_set_geo(DevGeo v, DevGeo *__restrict__ out) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *out = v;
+}
+__global__ void k_set_step(StepDev v, StepDev *__restrict__ out) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *out = v;
+}
+__global__ void k_clear_err(DevGeo *__restrict__ dg) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) dg->err = 0;
 }
 
 // ---------------------------------------------------------------- exclusive scan
@@ -492,7 +593,8 @@ constexpr int spread_slots() {
 }
 
 template <int P, int D>
-__global__ __launch_bounds__(kSpreadThreads) void k_spread_f32(const float *__restrict__ Y, int N, Geo g,
+__global__ __launch_bounds__(kSpreadThreads) void k_spread_f32(const float *__restrict__ Y, int N,
+                                                               const DevGeo *__restrict__ dg,
                                                                unsigned long long *__restrict__ grid) {
     constexpr int NV = (D == 2 ? P * P : P);
     constexpr int NQ = (D == 2 ? 4 : 3);
@@ -504,6 +606,8 @@ __global__ __launch_bounds__(kSpreadThreads) void k_spread_f32(const float *__re
     __shared__ int hl[WPB][kWave + 1];
     __shared__ int hb[WPB][kWave];
     __shared__ int hs[WPB][kWave];
+    if (!dg->live) return;
+    const Geo &g = dg->g;
     const int t = threadIdx.x, lane = t % kWave, wv = t / kWave;
     for (int o = t; o < NS; o += kSpreadThreads) tkey[o] = -1;
     for (int o = t; o < NS * NV * NQ; o += kSpreadThreads) (&tval[0][0])[o] = 0ull;
@@ -633,284 +737,327 @@ __global__ __launch_bounds__(kSpreadThreads) void k_spread_f32(const float *__re
 }
 
 // int64 grid -> float4 (A, c, b_x, b_y) and clear, for the 1D path (the 2D row FFT does this itself).
-__global__ __launch_bounds__(kThreads) void k_grid_to_f32(unsigned long long *__restrict__ grid, long nodes, Geo g,
+__global__ __launch_bounds__(kThreads) void k_grid_to_f32(const DevGeo *__restrict__ dg,
+                                                          unsigned long long *__restrict__ grid,
                                                           float4 *__restrict__ out) {
-    const long n = (long)blockIdx.x * kThreads + threadIdx.x;
-    if (n >= nodes) return;
+    if (!dg->live) return;
+    const Geo &g = dg->g;
     const double inv = 1.0 / kFixScale;
-    long long s[4];
+    for (long n = (long)blockIdx.x * kThreads + threadIdx.x; n < g.n1d; n += (long)gridDim.x * kThreads) {
+        long long s[4];
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-        s[q] = (long long)grid[n * 4 + q];
-        grid[n * 4 + q] = 0ull;
+        for (int q = 0; q < 4; q++) {
+            s[q] = (long long)grid[n * 4 + q];
+            grid[n * 4 + q] = 0ull;
+        }
+        out[n] = make_float4((float)(s[0] * inv), (float)(s[1] * inv * g.bw0 * g.bw0), (float)(s[2] * inv * g.bw0),
+                             (float)(s[3] * inv * g.bw0));
     }
-    out[n] = make_float4((float)(s[0] * inv), (float)(s[1] * inv * g.bw0 * g.bw0), (float)(s[2] * inv * g.bw0),
-                         (float)(s[3] * inv * g.bw0));
 }
 
-// ---------------------------------------------------------------- kernel spectra
-// Row pass: for lattice row a (x offset a h), the even real lines K_q(a, |b|) and K_1(a, |b|) over
-// b in (-n1d, n1d) wrapped into Mp, packed as one complex line; its DFT is T_q + i T_1 (both real).
-// Trow[k][a] = (T_q(a, k), T_1(a, k)), k in [0, H).
+// ---------------------------------------------------------------- 2D FFT passes
+// Every pass reads the step's geometry and plan from DevGeo and walks its lines (rows u in [0, n1d) or
+// column frequencies k in [0, H)) over XCD slots with a workgroup stride, so one launch sized for the
+// largest grid serves any grid of the step; each line ends with a barrier because the next one reuses
+// the LDS.  Twiddles are staged in LDS once per workgroup, behind the lines.
+
+// Kernel spectra, row pass: for lattice row a (x offset a h), the even real lines K_q(a, |b|) and
+// K_1(a, |b|) over b in (-n1d, n1d) wrapped into Mp, packed as one complex line; its DFT is
+// T_q + i T_1 (both real).  Trow[k][a] = (T_q(a, k), T_1(a, k)), k in [0, H).  Runs only when the
+// step marks the spectra stale.
 template <typename T, int NTHR, int MAXTOT>
-__global__ __launch_bounds__(NTHR) void k_kern_rows(Geo g, FftPlan pl, const cx<T> *__restrict__ tw, int kind_q,
-                                                    int kind_1, double df, cx<T> *__restrict__ Trow) {
+__global__ __launch_bounds__(NTHR) void k_kern_rows(const DevGeo *__restrict__ dg, const cx<T> *__restrict__ tw,
+                                                    int kind_q, int kind_1, double df, cx<T> *__restrict__ Trow) {
+    if (!dg->live || !dg->kspec_stale) return;
+    const Geo &g = dg->g;
     extern __shared__ unsigned char smem_raw[];
     cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
-    const int a = xcd_row(blockIdx.x, g.n1d);
-    if (a >= g.n1d) return;
     const int Mp = g.Mp, n1d = g.n1d;
-    const double du = a * g.h;
     const cx<T> *twl = stage_twiddles(buf + Mp, tw, Mp);
-    for (int b = threadIdx.x; b < Mp; b += NTHR) buf[b] = cx<T>{(T)0, (T)0};
-    __syncthreads();
-    for (int b = threadIdx.x; b < n1d; b += NTHR) {
-        const double dv = b * g.h;
-        const double r2 = du * du + dv * dv;
-        cx<T> val{kernel_value<T>(kind_q, r2, df), kernel_value<T>(kind_1, r2, df)};
-        buf[b] = val;
-        if (b > 0) buf[Mp - b] = val;
+    const int npad = 8 * ((n1d + 7) / 8);
+    for (int slot = blockIdx.x; slot < npad; slot += gridDim.x) {
+        const int a = xcd_slot_row(slot, n1d);
+        if (a >= n1d) continue;
+        const double du = a * g.h;
+        for (int b = threadIdx.x; b < Mp; b += NTHR) buf[b] = cx<T>{(T)0, (T)0};
+        __syncthreads();
+        for (int b = threadIdx.x; b < n1d; b += NTHR) {
+            const double dv = b * g.h;
+            const double r2 = du * du + dv * dv;
+            cx<T> val{kernel_value<T>(kind_q, r2, df), kernel_value<T>(kind_1, r2, df)};
+            buf[b] = val;
+            if (b > 0) buf[Mp - b] = val;
+        }
+        __syncthreads();
+        fft_lines<T, NTHR, MAXTOT>(buf, dg->pl, 1, Mp, twl, false);
+        for (int k = threadIdx.x; k < g.H; k += NTHR) Trow[(long)k * n1d + a] = buf[k];
+        __syncthreads();
     }
-    __syncthreads();
-    fft_lines<T, NTHR, MAXTOT>(buf, pl, 1, Mp, twl, false);
-    for (int k = threadIdx.x; k < g.H; k += NTHR) Trow[(long)k * n1d + a] = buf[k];
 }
 
-// Column pass: for column frequency k, the line z(a) = T_q(|a|,k)(1 + a h) + i T_1(|a|,k) (a signed).
-// Its DFT is C_q(m,k) + i (C_1(m,k) - S(m,k)) with C even and S odd in m, so
+// Kernel spectra, column pass: for column frequency k, the line z(a) = T_q(|a|,k)(1 + a h) + i T_1(|a|,k)
+// (a signed).  Its DFT is C_q(m,k) + i (C_1(m,k) - S(m,k)) with C even and S odd in m, so
 //   C_q = Re, C_1 = (Im(m) + Im(-m)) / 2, S = (Im(-m) - Im(m)) / 2,
 // where S(m,k) = sum_{a,b} K_q(a,b) a h sin(2 pi m a / Mp) cos(2 pi k b / Mp): the DFT of the odd
 // kernel K_q(a,b) a h is -i S(m,k) and that of K_q(a,b) b h is -i S(k,m) (K_q symmetric in a, b).
 // Stored for m, k in [0, H):  KQ[k][m] = C_q(m,k), K1[k][m] = C_1(m,k), KS[k][m] = S(m,k),
-// KT[k][m] = S(k,m) (written by column m's block).
+// KT[k][m] = S(k,m) (written by column m's workgroup).
 template <typename T, int NTHR, int MAXTOT>
-__global__ __launch_bounds__(NTHR) void k_kern_cols(Geo g, FftPlan pl, const cx<T> *__restrict__ tw,
+__global__ __launch_bounds__(NTHR) void k_kern_cols(const DevGeo *__restrict__ dg, const cx<T> *__restrict__ tw,
                                                     const cx<T> *__restrict__ Trow, int need_s,
                                                     T *__restrict__ KQ, T *__restrict__ K1, T *__restrict__ KS,
                                                     T *__restrict__ KT) {
+    if (!dg->live || !dg->kspec_stale) return;
+    const Geo &g = dg->g;
     extern __shared__ unsigned char smem_raw[];
     cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
-    const int k = xcd_row(blockIdx.x, g.H);
-    if (k >= g.H) return;
     const int Mp = g.Mp, n1d = g.n1d, H = g.H;
     const cx<T> *twl = stage_twiddles(buf + Mp, tw, Mp);
-    for (int a = threadIdx.x; a < Mp; a += NTHR) {
-        int as;
-        if (a < n1d) as = a;
-        else if (a > Mp - n1d) as = a - Mp;
-        else {
-            buf[a] = cx<T>{(T)0, (T)0};
-            continue;
+    const int npad = 8 * ((H + 7) / 8);
+    for (int slot = blockIdx.x; slot < npad; slot += gridDim.x) {
+        const int k = xcd_slot_row(slot, H);
+        if (k >= H) continue;
+        for (int a = threadIdx.x; a < Mp; a += NTHR) {
+            int as;
+            if (a < n1d) as = a;
+            else if (a > Mp - n1d) as = a - Mp;
+            else {
+                buf[a] = cx<T>{(T)0, (T)0};
+                continue;
+            }
+            const cx<T> t = Trow[(long)k * n1d + (as < 0 ? -as : as)];
+            const T w = need_s ? (T)1 + (T)(as * g.h) : (T)1;
+            buf[a] = cx<T>{t.x * w, t.y};
         }
-        const cx<T> t = Trow[(long)k * n1d + (as < 0 ? -as : as)];
-        const T w = need_s ? (T)1 + (T)(as * g.h) : (T)1;
-        buf[a] = cx<T>{t.x * w, t.y};
-    }
-    __syncthreads();
-    fft_lines<T, NTHR, MAXTOT>(buf, pl, 1, Mp, twl, false);
-    for (int m = threadIdx.x; m < H; m += NTHR) {
-        const cx<T> z = buf[m];
-        const T im_neg = buf[m == 0 ? 0 : Mp - m].y;
-        KQ[(long)k * H + m] = z.x;
-        K1[(long)k * H + m] = (z.y + im_neg) * (T)0.5;
-        if (need_s) {
-            const T s = (im_neg - z.y) * (T)0.5;
-            KS[(long)k * H + m] = s;
-            KT[(long)m * H + k] = s;
+        __syncthreads();
+        fft_lines<T, NTHR, MAXTOT>(buf, dg->pl, 1, Mp, twl, false);
+        for (int m = threadIdx.x; m < H; m += NTHR) {
+            const cx<T> z = buf[m];
+            const T im_neg = buf[m == 0 ? 0 : Mp - m].y;
+            KQ[(long)k * H + m] = z.x;
+            K1[(long)k * H + m] = (z.y + im_neg) * (T)0.5;
+            if (need_s) {
+                const T sv = (im_neg - z.y) * (T)0.5;
+                KS[(long)k * H + m] = sv;
+                KT[(long)m * H + k] = sv;
+            }
         }
+        __syncthreads();
     }
 }
 
-// ---------------------------------------------------------------- row FFT, forward
-// grid: [n1d][n1d][sin] real (u = x node, v = y node); terms 2l, 2l+1 packed as one complex line.
-// Output spec[t][k][u] = DFT over v of term t of row u, k in [0, H) (two real spectra unpacked from
-// the packed line: A = (Z(k) + conj Z(-k)) / 2, B = -i (Z(k) - conj Z(-k)) / 2).
+// Row FFT, forward.  grid: [n1d][n1d][sin] real (u = x node, v = y node); terms 2l, 2l+1 packed as
+// one complex line.  Output spec[t][k][u] = DFT over v of term t of row u, k in [0, H) (two real
+// spectra unpacked from the packed line: A = (Z(k) + conj Z(-k)) / 2, B = -i (Z(k) - conj Z(-k)) / 2).
 // With fixgrid != null the input is the exact int64 grid of k_spread_f32 ([u][v][4] = A, c, b_x, b_y in
 // 2^-38 units, offsets in box widths); it is converted and cleared here (each row read once).
 template <typename T, int NTHR, int MAXTOT>
-__global__ __launch_bounds__(NTHR) void k_rows_fwd(Geo g, FftPlan pl, const cx<T> *__restrict__ tw,
+__global__ __launch_bounds__(NTHR) void k_rows_fwd(const DevGeo *__restrict__ dg, const cx<T> *__restrict__ tw,
                                                    const T *__restrict__ grid, int sin, int nterms,
                                                    cx<T> *__restrict__ spec,
                                                    unsigned long long *__restrict__ fixgrid) {
+    if (!dg->live) return;
+    const Geo &g = dg->g;
     extern __shared__ unsigned char smem_raw[];
     cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
-    const int u = xcd_row(blockIdx.x, g.n1d);
-    if (u >= g.n1d) return;
     const int Mp = g.Mp, n1d = g.n1d, H = g.H;
     const int lines = (nterms + 1) / 2;
     const cx<T> *twl = stage_twiddles(buf + lines * Mp, tw, Mp);
     const double inv = 1.0 / kFixScale;
     const double unit[4] = {inv, inv * g.bw0 * g.bw0, inv * g.bw0, inv * g.bw0};
-    for (int q = threadIdx.x; q < lines * Mp; q += NTHR) {
-        const int l = q / Mp, v = q - l * Mp;
-        cx<T> z{(T)0, (T)0};
-        if (v < n1d) {
-            if (fixgrid) {
-                unsigned long long *src = fixgrid + ((long)u * n1d + v) * 4 + 2 * l;
-                z.x = (T)((double)(long long)src[0] * unit[2 * l]);
-                z.y = (T)((double)(long long)src[1] * unit[2 * l + 1]);
-                src[0] = 0ull;
-                src[1] = 0ull;
-            } else {
-                const T *src = grid + ((long)u * n1d + v) * sin + 2 * l;
-                z.x = src[0];
-                if (2 * l + 1 < nterms) z.y = src[1];
-            }
-        }
-        buf[q] = z;
-    }
-    __syncthreads();
-    fft_lines<T, NTHR, MAXTOT>(buf, pl, lines, Mp, twl, false);
     const long plane = (long)H * n1d;
-    for (int q = threadIdx.x; q < lines * H; q += NTHR) {
-        const int l = q / H, k = q - l * H;
-        const cx<T> Z = buf[l * Mp + k];
-        const cx<T> Zc = conj(buf[l * Mp + (k == 0 ? 0 : Mp - k)]);
-        spec[(2 * l) * plane + (long)k * n1d + u] = (T)0.5 * (Z + Zc);
-        if (2 * l + 1 < nterms) spec[(2 * l + 1) * plane + (long)k * n1d + u] = (T)0.5 * mul_negi(Z - Zc);
+    const int npad = 8 * ((n1d + 7) / 8);
+    for (int slot = blockIdx.x; slot < npad; slot += gridDim.x) {
+        const int u = xcd_slot_row(slot, n1d);
+        if (u >= n1d) continue;
+        for (int q = threadIdx.x; q < lines * Mp; q += NTHR) {
+            const int l = q / Mp, v = q - l * Mp;
+            cx<T> z{(T)0, (T)0};
+            if (v < n1d) {
+                if (fixgrid) {
+                    unsigned long long *src = fixgrid + ((long)u * n1d + v) * 4 + 2 * l;
+                    z.x = (T)((double)(long long)src[0] * unit[2 * l]);
+                    z.y = (T)((double)(long long)src[1] * unit[2 * l + 1]);
+                    src[0] = 0ull;
+                    src[1] = 0ull;
+                } else {
+                    const T *src = grid + ((long)u * n1d + v) * sin + 2 * l;
+                    z.x = src[0];
+                    if (2 * l + 1 < nterms) z.y = src[1];
+                }
+            }
+            buf[q] = z;
+        }
+        __syncthreads();
+        fft_lines<T, NTHR, MAXTOT>(buf, dg->pl, lines, Mp, twl, false);
+        for (int q = threadIdx.x; q < lines * H; q += NTHR) {
+            const int l = q / H, k = q - l * H;
+            const cx<T> Z = buf[l * Mp + k];
+            const cx<T> Zc = conj(buf[l * Mp + (k == 0 ? 0 : Mp - k)]);
+            spec[(2 * l) * plane + (long)k * n1d + u] = (T)0.5 * (Z + Zc);
+            if (2 * l + 1 < nterms) spec[(2 * l + 1) * plane + (long)k * n1d + u] = (T)0.5 * mul_negi(Z - Zc);
+        }
+        __syncthreads();
     }
 }
 
-// ---------------------------------------------------------------- column pass, FP32 form
-// For column frequency k: DFT over u of A, c, b_x, b_y; then with K_q, K_1 (even) and the odd kernels
-// K_x = -i S(m,k), K_y = -i S(k,m):
+// Column pass, FP32 form.  For column frequency k: DFT over u of A, c, b_x, b_y; then with K_q, K_1
+// (even) and the odd kernels K_x = -i S(m,k), K_y = -i S(k,m):
 //   V1 = K_q A,  G_x = K_x A - K_q b_x,  G_y = K_y A - K_q b_y    (inverse-transformed over u)
 // and the Parseval sums of Z (k weighted 1 at k = 0, Mp/2 and 2 elsewhere):
 //   z0 = sum K_1 |A|^2, z1 = sum Re(conj(c) K_q A), z2 = sum Re(conj(b).K_D A), z3 = sum K_q |b|^2.
 template <int NTHR, int MAXTOT>
-__global__ __launch_bounds__(NTHR) void k_cols_f32(Geo g, FftPlan pl, const cx<float> *__restrict__ tw,
+__global__ __launch_bounds__(NTHR) void k_cols_f32(const DevGeo *__restrict__ dg, const cx<float> *__restrict__ tw,
                                                    const cx<float> *__restrict__ spec, const float *__restrict__ KQ,
                                                    const float *__restrict__ K1, const float *__restrict__ KS,
                                                    const float *__restrict__ KT, cx<float> *__restrict__ ospec,
                                                    double *__restrict__ zpart) {
+    if (!dg->live) return;
+    const Geo &g = dg->g;
     extern __shared__ unsigned char smem_raw[];
     cx<float> *buf = reinterpret_cast<cx<float> *>(smem_raw);
-    const int k = xcd_row(blockIdx.x, g.H);
-    if (k >= g.H) return;
+    __shared__ double shz[NTHR / kWave];
     const int Mp = g.Mp, n1d = g.n1d, H = g.H;
     const long plane = (long)H * n1d;
     const cx<float> *twl = stage_twiddles(buf + 4 * Mp, tw, Mp);
-    for (int q = threadIdx.x; q < 4 * Mp; q += NTHR) {
-        const int t = q / Mp, u = q - t * Mp;
-        buf[q] = u < n1d ? spec[t * plane + (long)k * n1d + u] : cx<float>{0.f, 0.f};
-    }
-    __syncthreads();
-    fft_lines<float, NTHR, MAXTOT>(buf, pl, 4, Mp, twl, false);
-    const double wk = (k == 0 || 2 * k == Mp) ? 1.0 : 2.0;
-    double z0 = 0, z1 = 0, z2 = 0, z3 = 0;
-    for (int m = threadIdx.x; m < Mp; m += NTHR) {
-        const int mb = m <= Mp / 2 ? m : Mp - m;
-        const float sg = m <= Mp / 2 ? 1.f : -1.f;
-        const float cq = KQ[(long)k * H + mb], c1 = K1[(long)k * H + mb];
-        const float sx = sg * KS[(long)k * H + mb], sy = KT[(long)k * H + mb];
-        const cx<float> A = buf[m], C = buf[Mp + m], BX = buf[2 * Mp + m], BY = buf[3 * Mp + m];
-        const cx<float> KxA = mul_negi(sx * A), KyA = mul_negi(sy * A);
-        buf[m] = cq * A;
-        buf[Mp + m] = KxA - cq * BX;
-        buf[2 * Mp + m] = KyA - cq * BY;
-        const double a2 = (double)A.x * A.x + (double)A.y * A.y;
-        z0 += (double)c1 * a2;
-        z1 += (double)cq * ((double)C.x * A.x + (double)C.y * A.y);
-        // Re(conj(B) (-i s) A) = s Im(conj(B) A)
-        z2 += (double)sx * ((double)BX.x * A.y - (double)BX.y * A.x) + (double)sy * ((double)BY.x * A.y - (double)BY.y * A.x);
-        z3 += (double)cq * ((double)BX.x * BX.x + (double)BX.y * BX.y + (double)BY.x * BY.x + (double)BY.y * BY.y);
-    }
-    __syncthreads();
-    fft_lines<float, NTHR, MAXTOT>(buf, pl, 3, Mp, twl, true);
-    for (int q = threadIdx.x; q < 3 * n1d; q += NTHR) {
-        const int t = q / n1d, u = q - t * n1d;
-        ospec[t * plane + (long)k * n1d + u] = buf[t * Mp + u];
-    }
-    __shared__ double shz[NTHR / kWave];
-    z0 = block_sum(z0 * wk, shz);
-    z1 = block_sum(z1 * wk, shz);
-    z2 = block_sum(z2 * wk, shz);
-    z3 = block_sum(z3 * wk, shz);
-    if (threadIdx.x == 0) {
-        zpart[k * 4 + 0] = z0;
-        zpart[k * 4 + 1] = z1;
-        zpart[k * 4 + 2] = z2;
-        zpart[k * 4 + 3] = z3;
+    const int npad = 8 * ((H + 7) / 8);
+    for (int slot = blockIdx.x; slot < npad; slot += gridDim.x) {
+        const int k = xcd_slot_row(slot, H);
+        if (k >= H) continue;
+        for (int q = threadIdx.x; q < 4 * Mp; q += NTHR) {
+            const int t = q / Mp, u = q - t * Mp;
+            buf[q] = u < n1d ? spec[t * plane + (long)k * n1d + u] : cx<float>{0.f, 0.f};
+        }
+        __syncthreads();
+        fft_lines<float, NTHR, MAXTOT>(buf, dg->pl, 4, Mp, twl, false);
+        const double wk = (k == 0 || 2 * k == Mp) ? 1.0 : 2.0;
+        double z0 = 0, z1 = 0, z2 = 0, z3 = 0;
+        for (int m = threadIdx.x; m < Mp; m += NTHR) {
+            const int mb = m <= Mp / 2 ? m : Mp - m;
+            const float sg = m <= Mp / 2 ? 1.f : -1.f;
+            const float cq = KQ[(long)k * H + mb], c1 = K1[(long)k * H + mb];
+            const float sx = sg * KS[(long)k * H + mb], sy = KT[(long)k * H + mb];
+            const cx<float> A = buf[m], C = buf[Mp + m], BX = buf[2 * Mp + m], BY = buf[3 * Mp + m];
+            const cx<float> KxA = mul_negi(sx * A), KyA = mul_negi(sy * A);
+            buf[m] = cq * A;
+            buf[Mp + m] = KxA - cq * BX;
+            buf[2 * Mp + m] = KyA - cq * BY;
+            const double a2 = (double)A.x * A.x + (double)A.y * A.y;
+            z0 += (double)c1 * a2;
+            z1 += (double)cq * ((double)C.x * A.x + (double)C.y * A.y);
+            // Re(conj(B) (-i s) A) = s Im(conj(B) A)
+            z2 += (double)sx * ((double)BX.x * A.y - (double)BX.y * A.x) +
+                  (double)sy * ((double)BY.x * A.y - (double)BY.y * A.x);
+            z3 += (double)cq * ((double)BX.x * BX.x + (double)BX.y * BX.y + (double)BY.x * BY.x + (double)BY.y * BY.y);
+        }
+        __syncthreads();
+        fft_lines<float, NTHR, MAXTOT>(buf, dg->pl, 3, Mp, twl, true);
+        for (int q = threadIdx.x; q < 3 * n1d; q += NTHR) {
+            const int t = q / n1d, u = q - t * n1d;
+            ospec[t * plane + (long)k * n1d + u] = buf[t * Mp + u];
+        }
+        z0 = block_sum(z0 * wk, shz);
+        z1 = block_sum(z1 * wk, shz);
+        z2 = block_sum(z2 * wk, shz);
+        z3 = block_sum(z3 * wk, shz);
+        if (threadIdx.x == 0) {
+            zpart[k * 4 + 0] = z0;
+            zpart[k * 4 + 1] = z1;
+            zpart[k * 4 + 2] = z2;
+            zpart[k * 4 + 3] = z3;
+        }
+        __syncthreads();
     }
 }
 
-// ---------------------------------------------------------------- column pass, plain (FP64)
-// Each term: DFT over u, times the real even kernel spectrum C(m,k), inverse DFT over u.
+// Column pass, plain (FP64).  Each term: DFT over u, times the real even kernel spectrum C(m,k),
+// inverse DFT over u.
 template <typename T, int NTHR, int MAXTOT>
-__global__ __launch_bounds__(NTHR) void k_cols_plain(Geo g, FftPlan pl, const cx<T> *__restrict__ tw,
+__global__ __launch_bounds__(NTHR) void k_cols_plain(const DevGeo *__restrict__ dg, const cx<T> *__restrict__ tw,
                                                      const cx<T> *__restrict__ spec, int nterms,
                                                      const T *__restrict__ KC, cx<T> *__restrict__ ospec) {
+    if (!dg->live) return;
+    const Geo &g = dg->g;
     extern __shared__ unsigned char smem_raw[];
     cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
-    const int k = xcd_row(blockIdx.x, g.H);
-    if (k >= g.H) return;
     const int Mp = g.Mp, n1d = g.n1d, H = g.H;
     const long plane = (long)H * n1d;
     const int lines_max = MAXTOT / Mp;
     const cx<T> *twl = stage_twiddles(buf + min(nterms, lines_max) * Mp, tw, Mp);
-    for (int t0 = 0; t0 < nterms; t0 += lines_max) {
-        const int lines = min(lines_max, nterms - t0);
-        for (int q = threadIdx.x; q < lines * Mp; q += NTHR) {
-            const int t = q / Mp, u = q - t * Mp;
-            buf[q] = u < n1d ? spec[(t0 + t) * plane + (long)k * n1d + u] : cx<T>{(T)0, (T)0};
+    const int npad = 8 * ((H + 7) / 8);
+    for (int slot = blockIdx.x; slot < npad; slot += gridDim.x) {
+        const int k = xcd_slot_row(slot, H);
+        if (k >= H) continue;
+        for (int t0 = 0; t0 < nterms; t0 += lines_max) {
+            const int lines = min(lines_max, nterms - t0);
+            for (int q = threadIdx.x; q < lines * Mp; q += NTHR) {
+                const int t = q / Mp, u = q - t * Mp;
+                buf[q] = u < n1d ? spec[(t0 + t) * plane + (long)k * n1d + u] : cx<T>{(T)0, (T)0};
+            }
+            __syncthreads();
+            fft_lines<T, NTHR, MAXTOT>(buf, dg->pl, lines, Mp, twl, false);
+            for (int q = threadIdx.x; q < lines * Mp; q += NTHR) {
+                const int t = q / Mp, m = q - t * Mp;
+                const int mb = m <= Mp / 2 ? m : Mp - m;
+                buf[q] = KC[(long)k * H + mb] * buf[q];
+            }
+            __syncthreads();
+            fft_lines<T, NTHR, MAXTOT>(buf, dg->pl, lines, Mp, twl, true);
+            for (int q = threadIdx.x; q < lines * n1d; q += NTHR) {
+                const int t = q / n1d, u = q - t * n1d;
+                ospec[(t0 + t) * plane + (long)k * n1d + u] = buf[t * Mp + u];
+            }
+            __syncthreads();
         }
-        __syncthreads();
-        fft_lines<T, NTHR, MAXTOT>(buf, pl, lines, Mp, twl, false);
-        for (int q = threadIdx.x; q < lines * Mp; q += NTHR) {
-            const int t = q / Mp, m = q - t * Mp;
-            const int mb = m <= Mp / 2 ? m : Mp - m;
-            buf[q] = KC[(long)k * H + mb] * buf[q];
-        }
-        __syncthreads();
-        fft_lines<T, NTHR, MAXTOT>(buf, pl, lines, Mp, twl, true);
-        for (int q = threadIdx.x; q < lines * n1d; q += NTHR) {
-            const int t = q / n1d, u = q - t * n1d;
-            ospec[(t0 + t) * plane + (long)k * n1d + u] = buf[t * Mp + u];
-        }
-        __syncthreads();
     }
 }
 
-// ---------------------------------------------------------------- row FFT, inverse
-// ospec[t][k][u], k in [0, H): half spectra over v of real rows.  Terms 2l, 2l+1 are rebuilt into one
-// Hermitian-extended complex line (self-conjugate bins taken real), inverse-transformed, scaled, and
-// written to out[u][v][sout + ...].
+// Row FFT, inverse.  ospec[t][k][u], k in [0, H): half spectra over v of real rows.  Terms 2l, 2l+1 are
+// rebuilt into one Hermitian-extended complex line (self-conjugate bins taken real), inverse-
+// transformed, scaled by 1 / Mp^2, and written to out[u][v][sout + ...].
 template <typename T, int NTHR, int MAXTOT>
-__global__ __launch_bounds__(NTHR) void k_rows_inv(Geo g, FftPlan pl, const cx<T> *__restrict__ tw,
-                                                   const cx<T> *__restrict__ ospec, int nterms, T scale,
+__global__ __launch_bounds__(NTHR) void k_rows_inv(const DevGeo *__restrict__ dg, const cx<T> *__restrict__ tw,
+                                                   const cx<T> *__restrict__ ospec, int nterms,
                                                    T *__restrict__ out, int sout) {
+    if (!dg->live) return;
+    const Geo &g = dg->g;
     extern __shared__ unsigned char smem_raw[];
     cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
-    const int u = xcd_row(blockIdx.x, g.n1d);
-    if (u >= g.n1d) return;
     const int Mp = g.Mp, n1d = g.n1d, H = g.H;
+    const T scale = (T)(1.0 / ((double)Mp * Mp));
     const long plane = (long)H * n1d;
     const int lines = (nterms + 1) / 2;
     const cx<T> *twl = stage_twiddles(buf + lines * Mp, tw, Mp);
-    for (int q = threadIdx.x; q < lines * Mp; q += NTHR) {
-        const int l = q / Mp, k = q - l * Mp;
-        const bool lo = k < H;
-        const int kk = lo ? k : Mp - k;
-        cx<T> f1 = ospec[(2 * l) * plane + (long)kk * n1d + u];
-        cx<T> f2 = 2 * l + 1 < nterms ? ospec[(2 * l + 1) * plane + (long)kk * n1d + u] : cx<T>{(T)0, (T)0};
-        if (kk == 0 || 2 * kk == Mp) {
-            f1.y = (T)0;
-            f2.y = (T)0;
+    const int npad = 8 * ((n1d + 7) / 8);
+    for (int slot = blockIdx.x; slot < npad; slot += gridDim.x) {
+        const int u = xcd_slot_row(slot, n1d);
+        if (u >= n1d) continue;
+        for (int q = threadIdx.x; q < lines * Mp; q += NTHR) {
+            const int l = q / Mp, k = q - l * Mp;
+            const bool lo = k < H;
+            const int kk = lo ? k : Mp - k;
+            cx<T> f1 = ospec[(2 * l) * plane + (long)kk * n1d + u];
+            cx<T> f2 = 2 * l + 1 < nterms ? ospec[(2 * l + 1) * plane + (long)kk * n1d + u] : cx<T>{(T)0, (T)0};
+            if (kk == 0 || 2 * kk == Mp) {
+                f1.y = (T)0;
+                f2.y = (T)0;
+            }
+            if (!lo) {
+                f1 = conj(f1);
+                f2 = conj(f2);
+            }
+            buf[q] = f1 + mul_i(f2);
         }
-        if (!lo) {
-            f1 = conj(f1);
-            f2 = conj(f2);
+        __syncthreads();
+        fft_lines<T, NTHR, MAXTOT>(buf, dg->pl, lines, Mp, twl, true);
+        for (int q = threadIdx.x; q < lines * n1d; q += NTHR) {
+            const int l = q / n1d, v = q - l * n1d;
+            const cx<T> z = buf[l * Mp + v];
+            T *dst = out + ((long)u * n1d + v) * sout + 2 * l;
+            dst[0] = z.x * scale;
+            if (2 * l + 1 < nterms) dst[1] = z.y * scale;
         }
-        buf[q] = f1 + mul_i(f2);
-    }
-    __syncthreads();
-    fft_lines<T, NTHR, MAXTOT>(buf, pl, lines, Mp, twl, true);
-    for (int q = threadIdx.x; q < lines * n1d; q += NTHR) {
-        const int l = q / n1d, v = q - l * n1d;
-        const cx<T> z = buf[l * Mp + v];
-        T *dst = out + ((long)u * n1d + v) * sout + 2 * l;
-        dst[0] = z.x * scale;
-        if (2 * l + 1 < nterms) dst[1] = z.y * scale;
+        __syncthreads();
     }
 }
 
@@ -923,11 +1070,14 @@ __global__ __launch_bounds__(NTHR) void k_rows_inv(Geo g, FftPlan pl, const cx<T
 // Z partial sums z0..z3 as in k_cols_f32 (full spectrum, no half-plane weights).
 // Plain (f32form = 0): real even spectrum of K_q (or K_1 when use_k1) times each packed pair of terms.
 template <typename T, int NTHR, int MAXM>
-__global__ __launch_bounds__(NTHR) void k_conv1d(Geo g, FftPlan pl, const cx<T> *__restrict__ tw, int kind_q,
+__global__ __launch_bounds__(NTHR) void k_conv1d(const DevGeo *__restrict__ dg, const cx<T> *__restrict__ tw, int kind_q,
                                                  int kind_1, double df, int f32form, const T *__restrict__ grid,
                                                  int sin, int nterms, int use_k1, T *__restrict__ out, int sout,
                                                  double *__restrict__ zsum, const T *__restrict__ kspec) {
     constexpr int NC = (MAXM + NTHR - 1) / NTHR;
+    if (!dg->live) return;
+    const Geo &g = dg->g;
+    const FftPlan &pl = dg->pl;
     extern __shared__ unsigned char smem_raw[];
     cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
     const int Mp = g.Mp, n1d = g.n1d;
@@ -1060,8 +1210,11 @@ __global__ __launch_bounds__(NTHR) void k_conv1d(Geo g, FftPlan pl, const cx<T> 
 // Real even spectrum of one 1D kernel on the node lattice (precompute of nbodyfft.cpp:251-307,
 // in this library's layout: C(m) for m in [0, Mp)).
 template <typename T, int NTHR, int MAXM>
-__global__ __launch_bounds__(NTHR) void k_kspec1d(Geo g, FftPlan pl, const cx<T> *__restrict__ tw, int kind,
-                                                  double df, T *__restrict__ out) {
+__global__ __launch_bounds__(NTHR) void k_kspec1d(const DevGeo *__restrict__ dg, const cx<T> *__restrict__ tw,
+                                                  int kind, double df, T *__restrict__ out) {
+    if (!dg->live) return;
+    const Geo &g = dg->g;
+    const FftPlan &pl = dg->pl;
     extern __shared__ unsigned char smem_raw[];
     cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
     const int Mp = g.Mp, n1d = g.n1d;
@@ -1079,8 +1232,12 @@ __global__ __launch_bounds__(NTHR) void k_kspec1d(Geo g, FftPlan pl, const cx<T>
 
 // Final Z of the FP32 form from the per-column Parseval partials, in FP64 and a fixed order:
 //   df = 1:  Z = (z0 + 2 z1 + 4 z2 - 2 z3) / Mp^D - N;   df != 1:  Z = z0 / Mp^D - N.
-__global__ void k_zfinal(const double *__restrict__ zpart, int ncols, int df_one, double norm, double N,
+__global__ void k_zfinal(const DevGeo *__restrict__ dg, const double *__restrict__ zpart, int df_one, double N,
                          double *__restrict__ zout) {
+    if (!dg->live) return;
+    const Geo &g = dg->g;
+    const int ncols = g.D == 2 ? g.H : 1;
+    const double norm = g.D == 2 ? 1.0 / ((double)g.Mp * g.Mp) : 1.0 / (double)g.Mp;
     __shared__ double sh[kThreads / kWave];
     double s = 0;
     for (int k = threadIdx.x; k < ncols; k += blockDim.x) {
@@ -1096,19 +1253,18 @@ __global__ void k_zfinal(const double *__restrict__ zpart, int ncols, int df_one
 }
 
 // ---------------------------------------------------------------- interpolation + update, FP32 form
-struct UpdPrm {
-    float momentum, lr, max_step_norm;
-    int plain_descent;
-    int mode;  // 0: write dC only; 1: optimiser update of Y, uY, gains
-};
-
 template <int P, int D>
 __global__ __launch_bounds__(kThreads) void k_interp_f32(float *__restrict__ Y, const float *__restrict__ pos,
-                                                         const float4 *__restrict__ gout, Geo g,
-                                                         const double *__restrict__ zbuf, UpdPrm up,
+                                                         const float4 *__restrict__ gout,
+                                                         const DevGeo *__restrict__ dg,
+                                                         const double *__restrict__ zbuf,
+                                                         const StepDev *__restrict__ sd,
                                                          float *__restrict__ dC, float *__restrict__ uY,
                                                          float *__restrict__ gains, double *__restrict__ part,
                                                          int N) {
+    if (!dg->live) return;
+    const Geo &g = dg->g;
+    const StepDev up = *sd;  // mode 0: write dC only; 1: optimiser update of Y, uY, gains
     const int i = blockIdx.x * kThreads + threadIdx.x;
     double ysum[D];
 #pragma unroll
@@ -1229,7 +1385,9 @@ __global__ __launch_bounds__(kThreads) void k_interp_f32(float *__restrict__ Y, 
 }
 
 // Mean of the updated Y from block partials (fixed order), FP64.
-__global__ void k_mean_final(const double *__restrict__ part, int nblk, int D, double N, double *__restrict__ mean) {
+__global__ void k_mean_final(const DevGeo *__restrict__ dg, const double *__restrict__ part, int nblk, int D,
+                             double N, double *__restrict__ mean) {
+    if (!dg->live) return;
     __shared__ double sh[kThreads / kWave];
     for (int d = 0; d < D; d++) {
         double s = 0;
@@ -1243,8 +1401,10 @@ __global__ void k_mean_final(const double *__restrict__ part, int nblk, int D, d
 // zeroMean (tsne.cpp:1851-1876): Y -= mean, with the min/max partials of the result for the next
 // geometry (same layout as k_minmax_partial).
 template <typename TV>
-__global__ __launch_bounds__(kThreads) void k_center(TV *__restrict__ Y, long n, int D, long skip,
-                                                     const double *__restrict__ mean, double *__restrict__ part) {
+__global__ __launch_bounds__(kThreads) void k_center(const DevGeo *__restrict__ dg, TV *__restrict__ Y, long n,
+                                                     int D, long skip, const double *__restrict__ mean,
+                                                     double *__restrict__ part) {
+    if (!dg->live) return;
     double mx = -INFINITY, mn = INFINITY;
     const double m0 = mean[0], m1 = D == 2 ? mean[1] : 0.0;
     for (long e = (long)blockIdx.x * kThreads + threadIdx.x; e < n; e += (long)gridDim.x * kThreads) {

@@ -4,8 +4,9 @@
 // distance is sqrt(sum_d (x_d - y_d)^2) accumulated in order d = 0..D-1 without fused multiply-adds
 // (vptree.h:97-107).  Here one wavefront serves kKnnQ query points: every lane takes one candidate per
 // round, accumulates its squared distances to the kKnnQ queries in the same order and rounding
-// (__dmul_rn / __dadd_rn; the root by the correctly rounded __dsqrt_rn), and candidates below the
-// query's current K-th distance are appended to an LDS buffer of B slots per query.  When a buffer is nearly full it is sorted by (distance, index)
+// (contraction to fused multiply-adds switched off; the root by the correctly rounded __dsqrt_rn),
+// and candidates below the query's current K-th distance are appended to an LDS buffer of B slots
+// per query.  When a buffer is nearly full it is sorted by (distance, index)
 // with a bitonic network and cut back to K, which tightens the threshold.  The output is sorted by
 // (distance, index), so equal distances are broken by the smaller index and the result is
 // deterministic.  X is read transposed (XT[d][n]) so that the lanes' candidate loads coalesce.
@@ -97,13 +98,15 @@ __global__ __launch_bounds__(kWave) void k_knn_f64(const double *__restrict__ X,
 #pragma unroll
         for (int j = 0; j < kKnnQ; j++) acc[j] = 0.0;
         if (c < N) {
+            // separate multiply and add, as the host's loop (no fused multiply-add)
+#pragma clang fp contract(off)
             for (int d = 0; d < D; d++) {
                 const double xc = XT[(long)d * N + c];
 #pragma unroll
                 for (int j = 0; j < kKnnQ; j++) {
                     const int qi = min(q0 + j, N - 1);
                     const double diff = X[(long)qi * D + d] - xc;
-                    acc[j] = __dadd_rn(acc[j], __dmul_rn(diff, diff));
+                    acc[j] += diff * diff;
                 }
             }
         }

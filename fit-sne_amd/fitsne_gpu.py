@@ -49,6 +49,10 @@ SIGNATURES = {
     "fitsne_evaluate_error_fft": (_ci, [_u32, _u32, _dp, _dp, _ci, _ci, _cd, _cd, C.POINTER(_cd)]),
     "fitsne_distances2similarities_batch": (_ci, [_dp, _dp, _ci, _ci, _cd, _cd, _ci, _vp, _dp, _vp]),
     "fitsne_knn_exact": (_ci, [_dp, _ci, _ci, _ci, _i32, _dp]),
+    "fitsne_set_graph": (_ci, [_vp, _ci]),
+    "fitsne_batch_create": (_ci, [C.POINTER(_vp), C.POINTER(_vp), _ci]),
+    "fitsne_batch_step": (_ci, [_vp, C.POINTER(StepParams)]),
+    "fitsne_batch_destroy": (_ci, [_vp]),
     "fitsne_beta_model_create": (_ci, [C.POINTER(_vp), _ci, C.c_ulonglong]),
     "fitsne_beta_model_destroy": (_ci, [_vp]),
     "fitsne_beta_model_fit": (_ci, [_vp, _dp, _dp, _ci, _ci, _ci, _cd, C.c_ulonglong, C.POINTER(_cd)]),
@@ -213,6 +217,35 @@ def knn_exact(X, K):
     return idx, dist
 
 
+class Batch:
+    """One step of several FP32 contexts (independent embeddings) as one graph launch."""
+
+    def __init__(self, contexts):
+        self.contexts = list(contexts)
+        arr = (_vp * len(self.contexts))(*[c.h for c in self.contexts])
+        h = _vp()
+        check(lib().fitsne_batch_create(C.byref(h), arr, len(self.contexts)))
+        self.h = h
+
+    def step(self, params):
+        """params: one (momentum, learning_rate, max_step_norm) tuple per context, or one for all."""
+        if isinstance(params, tuple):
+            params = [params] * len(self.contexts)
+        arr = (StepParams * len(self.contexts))(*[StepParams(m, lr, msn, 0, 0) for (m, lr, msn) in params])
+        check(lib().fitsne_batch_step(self.h, arr))
+
+    def close(self):
+        if self.h:
+            lib().fitsne_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 # ---------------------------------------------------------------- learned β warm start
 
 MLP_IN, MLP_HID = 96, 64
@@ -324,6 +357,10 @@ class Context:
 
     def set_reorder(self, enable):
         check(lib().fitsne_set_reorder(self.h, int(enable)))
+
+    def set_graph(self, enable):
+        """Replay steps as one captured graph with device-planned geometry (FP32 form)."""
+        check(lib().fitsne_set_graph(self.h, int(enable)))
 
     def reorder(self):
         check(lib().fitsne_reorder(self.h))

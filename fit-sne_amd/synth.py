@@ -107,6 +107,28 @@ def c5_mixture(N=16_000, D=100, k=40, seed=1):
     return centres[labels] + rng.normal(size=(N, D)) * sds[labels][:, None], labels
 
 
+def c4_problems(n=64, nmin=2_000, nmax=200_000, per_point=30, seed=20261206):
+    """C4: n independent embedding problems with N log-uniform in [nmin, nmax]; each a D = 50 mixture of
+    5-30 equal-weight Gaussian clusters (centres N(0, 6^2 I), sd 1), P from per_point random
+    same-cluster neighbours (as c2_workload), Y 2D blobs spanning about [-40, 40]."""
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(nmin), np.log(nmax), size=n)).astype(int)
+    out = []
+    t0 = time.time()
+    for i, N in enumerate(sizes):
+        k = int(rng.integers(5, 31))
+        labels = rng.integers(0, k, size=N)
+        X = (rng.normal(0.0, 6.0, size=(k, 50))[labels] + rng.normal(0.0, 1.0, size=(N, 50))).astype(np.float32)
+        rows, cols, vals = cluster_affinities(X, labels, per_point, rng)
+        P = symmetrize_normalize(int(N), rows, cols, vals)
+        cen = rng.uniform(-35.0, 35.0, size=(k, 2))
+        Y = cen[labels] + rng.normal(0.0, 2.0, size=(N, 2))
+        Y -= Y.mean(axis=0)
+        out.append((P, Y))
+    _log(f"C4: {n} problems, N in [{sizes.min()}, {sizes.max()}], total {sizes.sum()} points in {time.time() - t0:.1f}s")
+    return out
+
+
 def c1_data(N=10_000, D=50, k=10, seed=20261202):
     """C1: N points in D dimensions, k equal-weight isotropic Gaussians, centres N(0, 4^2 I), sd 1."""
     rng = np.random.default_rng(seed)

@@ -200,6 +200,23 @@ int fitsne_gradient(fitsne_ctx *ctx, double *dC, double *sum_Q);
  * Y += uY and zeroMean(Y).  Asynchronous on the context's stream; sum_Q (nullable) forces a sync. */
 int fitsne_step(fitsne_ctx *ctx, const fitsne_step_params *prm, double *sum_Q);
 
+/* Captured steps (FP32 form; no reference counterpart).  When enabled, fitsne_step replays one hipGraph
+ * holding the whole iteration with its grid geometry planned on the device (no host round trip per
+ * step): results are bitwise those of the host-planned step.  Grids and buffers are sized for this
+ * build's largest grid (2D FFT length 2048, 1D 4096); a step whose grid would exceed it is refused
+ * on the device, leaves the embedding unchanged and makes the next synchronising call (fitsne_sync,
+ * fitsne_get_Y, fitsne_kl, fitsne_gradient, fitsne_last_geometry, or fitsne_step with sum_Q) return
+ * FITSNE_E_UNSUPPORTED.  Re-captured after set_P / set_Y / reorder. */
+int fitsne_set_graph(fitsne_ctx *ctx, int enable);
+
+/* Batched embeddings (no reference counterpart): one optimisation step of every context of the batch
+ * (FP32 contexts on one device, each with its own P and Y) as one launch of a graph whose parallel
+ * branches are the contexts' captured steps.  prm[n] holds each context's parameters. */
+typedef struct fitsne_batch fitsne_batch;
+int fitsne_batch_create(fitsne_batch **batch, fitsne_ctx **ctxs, int n);
+int fitsne_batch_step(fitsne_batch *batch, const fitsne_step_params *prm);
+int fitsne_batch_destroy(fitsne_batch *batch);
+
 /* KL divergence at the resident Y with the Z of the last gradient (tsne.cpp:547-571 semantics:
  * stale Z, FLT_MIN guards, exaggerated P); the exaggeration correction is the caller's. */
 int fitsne_kl(fitsne_ctx *ctx, double *C);
