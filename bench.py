@@ -124,7 +124,22 @@ def bench_c4(args, fg, synth):
             c.step(m, lr, msn)
         c.sync()
     t_seq = time.perf_counter() - t0
-    log("C4 batched: one graph launch per iteration for all problems")
+    log("C4 captured, one problem after another (device-planned steps, no concurrency)")
+    cap = make()
+    for c, (m, lr, msn) in zip(cap, prm):
+        c.set_graph(True)
+        for _ in range(args.warmup):
+            c.step(m, lr, msn)
+        c.sync()
+    t0 = time.perf_counter()
+    for c, (m, lr, msn) in zip(cap, prm):
+        for _ in range(args.steps):
+            c.step(m, lr, msn)
+        c.sync()
+    t_cap = time.perf_counter() - t0
+    for c in cap:
+        c.close()
+    log("C4 batched: every problem's captured step on its own stream, each iteration")
     bat = make()
     batch = fg.Batch(bat)
     for _ in range(args.warmup):
@@ -149,6 +164,8 @@ def bench_c4(args, fg, synth):
                    "N_total": int(sum(sizes)), "N_min": int(min(sizes)), "N_max": int(max(sizes))},
         "sequential": {"value": n * args.steps / t_seq, "unit": "problem-iters/s",
                        "ms_per_round": 1e3 * t_seq / args.steps},
+        "captured_one_by_one": {"value": n * args.steps / t_cap, "unit": "problem-iters/s",
+                                "ms_per_round": 1e3 * t_cap / args.steps},
         "speedup_vs_sequential": t_seq / t_bat,
         "bitwise_equal_to_sequential": bool(same),
     }

@@ -49,8 +49,8 @@ constexpr int kConv1dThreads = 1024;
 // length of a 2D grid is limited to kMaxLen2d (n1d <= 1024 nodes per dimension).
 constexpr int kMaxLen2d = 2048;
 constexpr int kKernThreads = 256, kKernTot = kMaxLen2d;        // kernel spectra: 1 line per block
-constexpr int kRowThreads = 512, kRowTot = 2 * kMaxLen2d;      // row passes: 2 packed lines
-constexpr int kColThreads = 1024, kColTot32 = 4 * kMaxLen2d;   // FP32 column pass: 4 lines
+constexpr int kRowThreads = 256, kRowTot = 2 * kMaxLen2d;      // row passes: 2 packed lines
+constexpr int kColThreads = 512, kColTot32 = 4 * kMaxLen2d;   // FP32 column pass: 4 lines
 constexpr int kColThreads64 = 512, kColTot64 = 2 * kMaxLen2d;  // FP64 plain column pass
 
 // ============================================================================ device buffers
@@ -1259,25 +1259,12 @@ extern "C" int fitsne_step(fitsne_ctx *c, const fitsne_step_params *prm, double 
 
 // ============================================================================ batched embeddings
 
+// One step of every embedding of the batch: each context's captured step is launched on that
+// context's own stream, so the steps of different embeddings overlap on the device (one graph launch
+// per embedding and iteration, no host round trip).
 struct fitsne_batch {
     std::vector<fitsne_ctx *> ctx;
-    std::vector<long> gen;  // capture generation of each context's step in the parent graph
-    hipStream_t stream = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    hipEvent_t done = nullptr;
-    std::vector<hipEvent_t> ready;
-    DBuf dst;  // StepDev* per context
 };
-
-constexpr int kBatchChunk = 64;
-struct BatchSteps {
-    StepDev v[kBatchChunk];
-};
-__global__ void k_set_steps(BatchSteps v, StepDev *const *__restrict__ dst, int n) {
-    const int i = threadIdx.x;
-    if (i < n) *dst[i] = v.v[i];
-}
 
 extern "C" int fitsne_batch_create(fitsne_batch **out, fitsne_ctx **ctxs, int n) {
     if (!out || !ctxs || n <= 0) return fail(FITSNE_E_ARG, "invalid batch");
@@ -1290,90 +1277,27 @@ extern "C" int fitsne_batch_create(fitsne_batch **out, fitsne_ctx **ctxs, int n)
     }
     auto *b = new fitsne_batch();
     b->ctx.assign(ctxs, ctxs + n);
-    b->gen.assign(n, -1);
-    b->ready.assign(n, nullptr);
-    const hipError_t e1 = hipSetDevice(ctxs[0]->device);
-    const hipError_t e2 = e1 == hipSuccess ? hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) : e1;
-    const hipError_t e3 = e2 == hipSuccess ? hipEventCreateWithFlags(&b->done, hipEventDisableTiming) : e2;
-    hipError_t e4 = e3;
-    for (int i = 0; i < n && e4 == hipSuccess; i++) e4 = hipEventCreateWithFlags(&b->ready[i], hipEventDisableTiming);
-    if (e4 != hipSuccess) {
-        fitsne_batch_destroy(b);
-        return fail(FITSNE_E_HIP, std::string("batch setup: ") + hipGetErrorString(e4));
-    }
     *out = b;
     return FITSNE_OK;
 }
 
 extern "C" int fitsne_batch_destroy(fitsne_batch *b) {
-    if (!b) return FITSNE_OK;
-    if (b->stream) (void)hipStreamSynchronize(b->stream);
-    if (b->exec) (void)hipGraphExecDestroy(b->exec);
-    if (b->graph) (void)hipGraphDestroy(b->graph);
-    for (auto e : b->ready)
-        if (e) (void)hipEventDestroy(e);
-    if (b->done) (void)hipEventDestroy(b->done);
-    release(b->dst);
-    if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return FITSNE_OK;
 }
 
-// One step of every embedding of the batch: their captured steps as parallel branches of one graph,
-// launched once on the batch's stream.
 extern "C" int fitsne_batch_step(fitsne_batch *b, const fitsne_step_params *prm) {
     if (!b || !prm) return fail(FITSNE_E_ARG, "null argument");
     const int n = (int)b->ctx.size();
-    HIPTRY(hipSetDevice(b->ctx[0]->device));
-    bool rebuild = b->exec == nullptr;
     for (int i = 0; i < n; i++) {
         fitsne_ctx *c = b->ctx[i];
+        HIPTRY(hipSetDevice(c->device));
         c->use_graph = true;
         if (!c->gexec) TRY(capture_step(c));
-        if (c->graph_gen != b->gen[i]) rebuild = true;
         TRY(ensure_partials(c));
-        HIPTRY(hipEventRecord(b->ready[i], c->stream));
-        HIPTRY(hipStreamWaitEvent(b->stream, b->ready[i], 0));
-    }
-    if (rebuild) {
-        if (b->exec) (void)hipGraphExecDestroy(b->exec);
-        if (b->graph) (void)hipGraphDestroy(b->graph);
-        b->exec = nullptr;
-        b->graph = nullptr;
-        HIPTRY(hipGraphCreate(&b->graph, 0));
-        for (int i = 0; i < n; i++) {
-            hipGraphNode_t node;
-            HIPTRY(hipGraphAddChildGraphNode(&node, b->graph, nullptr, 0, b->ctx[i]->graph));
-            b->gen[i] = b->ctx[i]->graph_gen;
-        }
-        HIPTRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
-        std::vector<StepDev *> dst(n);
-        for (int i = 0; i < n; i++) dst[i] = b->ctx[i]->dstep.as<StepDev>();
-        TRY(ensure(b->dst, sizeof(StepDev *) * n));
-        HIPTRY(hipMemcpyAsync(b->dst.p, dst.data(), sizeof(StepDev *) * n, hipMemcpyHostToDevice, b->stream));
-        HIPTRY(hipStreamSynchronize(b->stream));
-    }
-    for (int i0 = 0; i0 < n; i0 += kBatchChunk) {
-        BatchSteps v{};
-        const int m = std::min(kBatchChunk, n - i0);
-        for (int i = 0; i < m; i++) {
-            const fitsne_step_params &q = prm[i0 + i];
-            StepDev &d = v.v[i];
-            d.mode = 1;
-            d.exag = (float)b->ctx[i0 + i]->exag;
-            d.momentum = (float)q.momentum;
-            d.lr = (float)q.learning_rate;
-            d.max_step_norm = (float)q.max_step_norm;
-            d.plain_descent = q.plain_descent;
-        }
-        k_set_steps<<<1, kBatchChunk, 0, b->stream>>>(v, b->dst.as<StepDev *const>() + i0, m);
-        HIPTRY(hipGetLastError());
-    }
-    HIPTRY(hipGraphLaunch(b->exec, b->stream));
-    HIPTRY(hipEventRecord(b->done, b->stream));
-    for (int i = 0; i < n; i++) {
-        HIPTRY(hipStreamWaitEvent(b->ctx[i]->stream, b->done, 0));
-        after_graph_steps(b->ctx[i]);
+        TRY(put_step(c, 1, &prm[i]));
+        HIPTRY(hipGraphLaunch(c->gexec, c->stream));
+        after_graph_steps(c);
     }
     return FITSNE_OK;
 }

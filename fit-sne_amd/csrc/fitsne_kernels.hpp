@@ -149,7 +149,7 @@ __global__ __launch_bounds__(kThreads) void k_attractive(const TI *__restrict__ 
 // S_b = 2^(50 - e), 2^e > max over the block's rows of sum_j p_ij * sqrt(df)/2, bounds every partial
 // sum by 2^50 since |q d| <= sqrt(df)/2: the integer sums are exact, so the forces do not depend on the
 // order of the entries and are bitwise repeatable.  Same formula as k_attractive (tsne.cpp:1121-1137).
-constexpr int kWin = 4096;
+constexpr int kWin = 8192;
 constexpr int kAttRows = 4096;
 constexpr int kAttThreads = 1024;
 

@@ -210,8 +210,9 @@ int fitsne_step(fitsne_ctx *ctx, const fitsne_step_params *prm, double *sum_Q);
 int fitsne_set_graph(fitsne_ctx *ctx, int enable);
 
 /* Batched embeddings (no reference counterpart): one optimisation step of every context of the batch
- * (FP32 contexts on one device, each with its own P and Y) as one launch of a graph whose parallel
- * branches are the contexts' captured steps.  prm[n] holds each context's parameters. */
+ * (FP32 contexts on one device, each with its own P and Y): each context's captured step is launched
+ * on its own stream, so the embeddings' steps overlap on the device.  prm[n] holds each context's
+ * parameters.  Results are bitwise those of stepping each context alone. */
 typedef struct fitsne_batch fitsne_batch;
 int fitsne_batch_create(fitsne_batch **batch, fitsne_ctx **ctxs, int n);
 int fitsne_batch_step(fitsne_batch *batch, const fitsne_step_params *prm);
