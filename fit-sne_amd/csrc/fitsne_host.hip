@@ -13,6 +13,7 @@
 
 #include "../../include/fitsne_abi.h"
 #include "fitsne_kernels.hpp"
+#include "fitsne_exact.hpp"
 #include "fitsne_knn.hpp"
 #include "fitsne_mlp.hpp"
 
@@ -129,6 +130,8 @@ struct fitsne_ctx {
     bool graph_ran = false;      // a captured step ran since the last error check
     // plain path extras
     DBuf xs, ys, box, wx, wy, ch, w, V, potA, potB;
+    // exact path (dense P in val): per-row attractive / repulsive sums and Z shares
+    DBuf att, rep, zrow;
     // twiddles per FFT length
     std::map<int, DBuf> tw32, tw64;
     // min/max handoff from the last step
@@ -790,10 +793,93 @@ static int pipeline_f64(fitsne_ctx *c, int mode, const fitsne_step_params *sp) {
     return FITSNE_OK;
 }
 
+// ============================================================================ exact path (theta = 0)
+
+// Fixed grid of the partial sums over the N^2 entries of a dense P.
+constexpr int kDenseSumBlocks = 1024;
+
+// Gradient of computeExactGradient (tsne.cpp:1232-1282) at the resident Y over the dense P; mode 1 adds
+// the update and zeroMean of the step (tsne.cpp:478-531).  zbuf[0] = Z (the reference's sum_Q).
+template <int D>
+static int pipeline_exact(fitsne_ctx *c, int mode, const fitsne_step_params *sp) {
+    const int N = c->N;
+    const long nd = (long)N * D;
+    TRY(ensure(c->att, sizeof(double) * (size_t)nd));
+    TRY(ensure(c->rep, sizeof(double) * (size_t)nd));
+    TRY(ensure(c->zrow, sizeof(double) * (size_t)N));
+    TRY(ensure(c->zbuf, sizeof(double) * 4));
+    TRY(ensure(c->dC, sizeof(double) * (size_t)nd));
+    launch(c, "exact_rows", c->df == 1.0 ? k_exact_rows<D, true> : k_exact_rows<D, false>,
+           dim3((N + kExRows - 1) / kExRows), dim3(kThreads), 0,
+           c->val.as<const double>(), c->Y.as<const double>(), N, c->df, c->att.as<double>(), c->rep.as<double>(),
+           c->zrow.as<double>());
+    launch(c, "zsum64", k_sum_final, dim3(1), dim3(kThreads), 0, c->zrow.as<const double>(), N, 0.0,
+           c->zbuf.as<double>());
+    const int eblk = std::min(blocks_for(nd), 4096);
+    launch(c, "exact_combine", k_exact_combine, dim3(eblk), dim3(kThreads), 0, c->att.as<const double>(),
+           c->rep.as<const double>(), c->zbuf.as<const double>(), nd, c->dC.as<double>());
+    if (mode == 1) {
+        const DevGeo *live;
+        TRY(live_record(c, &live));
+        const int nblk = blocks_for(N);
+        TRY(ensure(c->part, sizeof(double) * (size_t)nblk * D));
+        TRY(ensure(c->mean, sizeof(double) * kExMaxDims));
+        const int cblk = std::min(blocks_for(nd), 1024);
+        TRY(ensure(c->mmpart, sizeof(double) * 2 * cblk));
+        launch(c, "update64", k_update_f64, dim3(nblk), dim3(kThreads), 0, c->Y.as<double>(), c->uY.as<double>(),
+               c->gains.as<double>(), c->dC.as<const double>(), N, D, sp->momentum, sp->learning_rate,
+               sp->max_step_norm, sp->plain_descent, c->part.as<double>());
+        launch(c, "mean_final", k_mean_final, dim3(1), dim3(kThreads), 0, live, c->part.as<const double>(), nblk, D,
+               (double)N, c->mean.as<double>());
+        launch(c, "center64", k_center<double>, dim3(cblk), dim3(kThreads), 0, live, c->Y.as<double>(), nd, D, -1L,
+               c->mean.as<const double>(), c->mmpart.as<double>());
+    }
+    c->have_grad = true;
+    HIPTRY(hipGetLastError());
+    return FITSNE_OK;
+}
+
+template <int D>
+static int kl_exact_d(fitsne_ctx *c) {
+    const int N = c->N;
+    const int nblk = (N + kExRows - 1) / kExRows;
+    TRY(ensure(c->zrow, sizeof(double) * (size_t)N));
+    TRY(ensure(c->klpart, sizeof(double) * (size_t)std::max(nblk, 2)));
+    TRY(ensure(c->mean, sizeof(double) * kExMaxDims));
+    // sum_Q of the current Y (evaluateError computes its own, tsne.cpp:1298-1308)
+    launch(c, "exact_z", c->df == 1.0 ? k_exact_rows<D, true> : k_exact_rows<D, false>, dim3(nblk), dim3(kThreads), 0, (const double *)nullptr,
+           c->Y.as<const double>(), N, c->df, (double *)nullptr, (double *)nullptr, c->zrow.as<double>());
+    launch(c, "kl_zsum", k_sum_final, dim3(1), dim3(kThreads), 0, c->zrow.as<const double>(), N, 0.0,
+           c->mean.as<double>() + 2);
+    launch(c, "exact_kl", c->df == 1.0 ? k_exact_kl<D, true> : k_exact_kl<D, false>, dim3(nblk), dim3(kThreads), 0, c->val.as<const double>(),
+           c->Y.as<const double>(), N, c->df, c->mean.as<const double>() + 2, c->klpart.as<double>());
+    launch(c, "kl_sum", k_sum_final, dim3(1), dim3(kThreads), 0, c->klpart.as<const double>(), nblk, 0.0,
+           c->mean.as<double>());
+    HIPTRY(hipGetLastError());
+    return FITSNE_OK;
+}
+
+static int kl_exact(fitsne_ctx *c) {
+    switch (c->D) {
+    case 1: return kl_exact_d<1>(c);
+    case 2: return kl_exact_d<2>(c);
+    case 3: return kl_exact_d<3>(c);
+    default: return kl_exact_d<4>(c);
+    }
+}
+
 static int run_pipeline(fitsne_ctx *c, int mode, const fitsne_step_params *sp) {
     if (c->N <= 0) return fail(FITSNE_E_ARG, "P and Y must be set before computing a gradient");
     HIPTRY(hipSetDevice(c->device));
     if (c->precision == FITSNE_FP32) return c->D == 2 ? pipeline_f32<2>(c, mode, sp) : pipeline_f32<1>(c, mode, sp);
+    if (c->precision == FITSNE_FP64_EXACT) {
+        switch (c->D) {
+        case 1: return pipeline_exact<1>(c, mode, sp);
+        case 2: return pipeline_exact<2>(c, mode, sp);
+        case 3: return pipeline_exact<3>(c, mode, sp);
+        default: return pipeline_exact<4>(c, mode, sp);
+        }
+    }
     return c->D == 2 ? pipeline_f64<2>(c, mode, sp) : pipeline_f64<1>(c, mode, sp);
 }
 
@@ -832,8 +918,14 @@ extern "C" int fitsne_ctx_create(fitsne_ctx **out, int device, int precision, in
                                  double intervals_per_integer, int min_num_intervals, double df) {
     if (!out) return fail(FITSNE_E_ARG, "ctx out-pointer is null");
     *out = nullptr;
-    if (no_dims != 1 && no_dims != 2) return fail(FITSNE_E_ARG, "no_dims must be 1 or 2 for the FFT gradient");
-    if (precision != FITSNE_FP32 && precision != FITSNE_FP64_PLAIN) return fail(FITSNE_E_ARG, "unknown precision");
+    if (precision != FITSNE_FP32 && precision != FITSNE_FP64_PLAIN && precision != FITSNE_FP64_EXACT)
+        return fail(FITSNE_E_ARG, "unknown precision");
+    if (precision == FITSNE_FP64_EXACT) {
+        if (no_dims < 1) return fail(FITSNE_E_ARG, "no_dims must be positive");
+        if (no_dims > kExMaxDims) return fail(FITSNE_E_UNSUPPORTED, "the exact path supports no_dims <= 4 in this build");
+    } else if (no_dims != 1 && no_dims != 2) {
+        return fail(FITSNE_E_ARG, "no_dims must be 1 or 2 for the FFT gradient");
+    }
     if (n_interpolation_points < 2 || n_interpolation_points > kMaxP)
         return fail(FITSNE_E_UNSUPPORTED, "n_interpolation_points must be in [2, 5] in this build");
     if (!(intervals_per_integer > 0) || min_num_intervals < 1 || !(df > 0))
@@ -868,7 +960,7 @@ extern "C" int fitsne_ctx_destroy(fitsne_ctx *c) {
                    &c->dstep, &c->dtw, &c->dlive, &c->klpart, &c->gin, &c->gout, &c->spec, &c->ospec, &c->Trow, &c->KQ,
                    &c->K1, &c->KS, &c->KT, &c->zpart, &c->zbuf, &c->part, &c->mean, &c->mmpart, &c->mm, &c->xs,
                    &c->ys, &c->box, &c->wx, &c->wy, &c->ch, &c->w, &c->V, &c->potA, &c->potB,
-                   &c->inv_d, &c->colP2, &c->val2, &c->tmp, &c->fixgrid};
+                   &c->inv_d, &c->colP2, &c->val2, &c->tmp, &c->fixgrid, &c->att, &c->rep, &c->zrow};
     for (DBuf *b : all) release(*b);
     for (auto &kv : c->tw32) release(kv.second);
     for (auto &kv : c->tw64) release(kv.second);
@@ -887,6 +979,8 @@ extern "C" int fitsne_ctx_destroy(fitsne_ctx *c) {
 template <typename TI, typename TC>
 static int set_P_impl(fitsne_ctx *c, int N, const TI *row_P, const TC *col_P, const double *val_P) {
     if (!c || !row_P || !col_P || !val_P || N <= 0) return fail(FITSNE_E_ARG, "invalid P arguments");
+    if (c->precision == FITSNE_FP64_EXACT)
+        return fail(FITSNE_E_ARG, "an exact context takes a dense P (fitsne_set_P_dense / fitsne_exact_affinities)");
     HIPTRY(hipSetDevice(c->device));
     const long long nnz = (long long)row_P[N];
     if (nnz < 0 || (long long)row_P[0] != 0) return fail(FITSNE_E_ARG, "row_P must start at 0");
@@ -953,10 +1047,111 @@ extern "C" int fitsne_set_exaggeration(fitsne_ctx *c, double factor) {
     if (!c || !(factor > 0)) return fail(FITSNE_E_ARG, "exaggeration must be positive");
     if (factor == c->exag) return FITSNE_OK;
     HIPTRY(hipSetDevice(c->device));
-    if (c->precision == FITSNE_FP64_PLAIN && c->nnz > 0)
+    if ((c->precision == FITSNE_FP64_PLAIN || c->precision == FITSNE_FP64_EXACT) && c->nnz > 0)
         launch(c, "scale_val", k_scale_val, dim3(std::min(blocks_for(c->nnz), 4096)), dim3(kThreads), 0,
                c->val.as<double>(), (long)c->nnz, factor, c->exag);
     c->exag = factor;
+    return FITSNE_OK;
+}
+
+// ---------------------------------------------------------------- dense P of the exact path
+
+static int exact_ctx_check(fitsne_ctx *c, int N) {
+    if (!c) return fail(FITSNE_E_ARG, "null context");
+    if (c->precision != FITSNE_FP64_EXACT) return fail(FITSNE_E_ARG, "a dense P needs an FITSNE_FP64_EXACT context");
+    if (N < 2) return fail(FITSNE_E_ARG, "N must be at least 2");
+    return FITSNE_OK;
+}
+
+// State of a context after its P changed (N may differ).
+static void dense_P_set(fitsne_ctx *c, int N) {
+    if (N != c->N) c->mm_valid = c->mm_pending = false;
+    c->N = N;
+    c->nnz = (long long)N * N;
+    c->exag = 1.0;
+    c->have_grad = false;
+    c->perm.resize(N);
+    c->inv.resize(N);
+    for (int i = 0; i < N; i++) c->perm[i] = c->inv[i] = i;
+    c->permuted = false;
+}
+
+extern "C" int fitsne_set_P_dense(fitsne_ctx *c, int N, const double *P) {
+    TRY(exact_ctx_check(c, N));
+    if (!P) return fail(FITSNE_E_ARG, "null P");
+    HIPTRY(hipSetDevice(c->device));
+    TRY(sync_ctx(c));
+    const size_t nn = (size_t)N * N;
+    TRY(ensure(c->val, sizeof(double) * nn));
+    HIPTRY(hipMemcpy(c->val.p, P, sizeof(double) * nn, hipMemcpyHostToDevice));
+    dense_P_set(c, N);
+    return FITSNE_OK;
+}
+
+extern "C" int fitsne_exact_affinities(fitsne_ctx *c, const double *X, int N, int D, double perplexity, double sigma,
+                                       int perplexity_list_length, const double *perplexity_list, double *P_out,
+                                       double *sum_P) {
+    TRY(exact_ctx_check(c, N));
+    if (!X || D < 1) return fail(FITSNE_E_ARG, "invalid X");
+    std::vector<double> plist;
+    if (perplexity > 0) {
+        plist.push_back(perplexity);
+    } else if (perplexity == 0) {
+        if (perplexity_list_length < 1 || !perplexity_list) return fail(FITSNE_E_ARG, "empty perplexity list");
+        for (int i = 0; i < perplexity_list_length; i++) {
+            if (!(perplexity_list[i] > 0)) return fail(FITSNE_E_ARG, "perplexities must be positive");
+            plist.push_back(perplexity_list[i]);
+        }
+    } else {
+        if (!(sigma > 0)) return fail(FITSNE_E_ARG, "a fixed kernel width needs sigma > 0");
+        plist.push_back(perplexity);  // the kernel takes the fixed width when perplexity <= 0
+    }
+    const size_t nd = (size_t)N * D, nn = (size_t)N * N;
+    for (size_t i = 0; i < nd; i++)
+        if (!std::isfinite(X[i])) return fail(FITSNE_E_ARG, "X contains non-finite values");
+    HIPTRY(hipSetDevice(c->device));
+    TRY(sync_ctx(c));
+    TRY(ensure(c->knnX, sizeof(double) * nd));
+    TRY(ensure(c->knnXT, sizeof(double) * nd));
+    TRY(ensure(c->w, sizeof(double) * nn));
+    TRY(ensure(c->val, sizeof(double) * nn));
+    TRY(ensure(c->xs, sizeof(double) * (size_t)N));
+    TRY(ensure(c->klpart, sizeof(double) * kDenseSumBlocks));
+    TRY(ensure(c->zbuf, sizeof(double) * 4));
+    HIPTRY(hipMemcpy(c->knnX.p, X, sizeof(double) * nd, hipMemcpyHostToDevice));
+    launch(c, "transpose64", k_transpose_f64, dim3((N + 31) / 32, (D + 31) / 32), dim3(256), 0,
+           c->knnX.as<const double>(), (long)N, D, c->knnXT.as<double>());
+    launch(c, "sqdist64", k_sqdist_f64, dim3(blocks_for(N), (N + kSqRows - 1) / kSqRows), dim3(kThreads), 0,
+           c->knnX.as<const double>(), c->knnXT.as<const double>(), N, D, c->w.as<double>());
+    // computeGaussianPerplexity's rows (tsne.cpp:1519-1525), the list overload averaging (tsne.cpp:1474-1500)
+    const long threads = (long)N * kWave;
+    for (size_t i = 0; i < plist.size(); i++)
+        launch(c, "d2s_dense", k_d2s, dim3(blocks_for(threads)), dim3(kThreads), 0, c->w.as<const double>(),
+               c->val.as<double>(), N, N, plist[i], sigma, 1, (const double *)nullptr, c->xs.as<double>(),
+               (int *)nullptr, 1, i > 0 ? 1 : 0);
+    const long NN = (long)nn;
+    const int sblk = std::min(blocks_for(NN), 4096);
+    if (perplexity == 0)
+        launch(c, "scale_val", k_scale_val, dim3(sblk), dim3(kThreads), 0, c->val.as<double>(), NN, 1.0,
+               (double)plist.size());
+    // symmetrisation and normalisation (tsne.cpp:196-216)
+    const int T = (N + kSymTile - 1) / kSymTile;
+    launch(c, "exact_sym", k_exact_sym, dim3(T, T), dim3(256), 0, c->val.as<double>(), N);
+    launch(c, "dense_psum", k_partial_sum, dim3(kDenseSumBlocks), dim3(kThreads), 0, c->val.as<const double>(), NN,
+           c->klpart.as<double>());
+    launch(c, "dense_sum", k_sum_final, dim3(1), dim3(kThreads), 0, c->klpart.as<const double>(), kDenseSumBlocks,
+           0.0, c->zbuf.as<double>());
+    launch(c, "dense_div", k_div_by, dim3(sblk), dim3(kThreads), 0, c->val.as<double>(), NN,
+           c->zbuf.as<const double>());
+    HIPTRY(hipGetLastError());
+    double total = 0.0;
+    HIPTRY(hipMemcpyAsync(c->h_mm + 3, c->zbuf.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (P_out) HIPTRY(hipMemcpyAsync(P_out, c->val.p, sizeof(double) * nn, hipMemcpyDeviceToHost, c->stream));
+    TRY(sync_ctx(c));
+    total = c->h_mm[3];
+    release(c->w);  // the N x N distance matrix is not needed after the calibration
+    dense_P_set(c, N);
+    if (sum_P) *sum_P = total;
     return FITSNE_OK;
 }
 
@@ -1305,6 +1500,15 @@ extern "C" int fitsne_batch_step(fitsne_batch *b, const fitsne_step_params *prm)
 extern "C" int fitsne_kl(fitsne_ctx *c, double *C) {
     if (!c || !C) return fail(FITSNE_E_ARG, "null argument");
     TRY(check_graph(c));
+    if (c->precision == FITSNE_FP64_EXACT) {
+        if (c->N <= 0) return fail(FITSNE_E_ARG, "P and Y must be set before the KL divergence");
+        HIPTRY(hipSetDevice(c->device));
+        TRY(kl_exact(c));
+        HIPTRY(hipMemcpyAsync(c->h_mm + 3, c->mean.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        TRY(sync_ctx(c));
+        *C = c->h_mm[3];
+        return FITSNE_OK;
+    }
     if (!c->have_grad) return fail(FITSNE_E_ARG, "KL needs the Z of a previous gradient");
     HIPTRY(hipSetDevice(c->device));
     const int sw = sw_for(c->nnz, c->N);
@@ -1768,7 +1972,7 @@ extern "C" int fitsne_distances2similarities_batch(const double *D, double *P, i
     const long threads = (long)n_points * kWave;
     launch(c, "d2s", k_d2s, dim3(blocks_for(threads)), dim3(kThreads), 0, c->w.as<const double>(), c->V.as<double>(),
            n_points, K, perplexity, sigma, ifSquared, beta0 ? c->ys.as<const double>() : (const double *)nullptr,
-           c->xs.as<double>(), c->box.as<int>());
+           c->xs.as<double>(), c->box.as<int>(), 0, 0);
     HIPTRY(hipMemcpyAsync(P, c->V.p, sizeof(double) * nk, hipMemcpyDeviceToHost, c->stream));
     HIPTRY(hipMemcpyAsync(beta, c->xs.p, sizeof(double) * n_points, hipMemcpyDeviceToHost, c->stream));
     if (iters) HIPTRY(hipMemcpyAsync(iters, c->box.p, sizeof(int) * n_points, hipMemcpyDeviceToHost, c->stream));

@@ -1408,7 +1408,7 @@ __global__ __launch_bounds__(kThreads) void k_center(const DevGeo *__restrict__ 
     double mx = -INFINITY, mn = INFINITY;
     const double m0 = mean[0], m1 = D == 2 ? mean[1] : 0.0;
     for (long e = (long)blockIdx.x * kThreads + threadIdx.x; e < n; e += (long)gridDim.x * kThreads) {
-        const double m = (D == 2 && (e & 1)) ? m1 : m0;
+        const double m = D > 2 ? mean[e % D] : (D == 2 && (e & 1)) ? m1 : m0;
         const TV v = (TV)((double)Y[e] - m);
         Y[e] = v;
         mx = fmax(mx, (double)v);
@@ -1660,7 +1660,7 @@ __global__ __launch_bounds__(kThreads) void k_update_f64(double *__restrict__ Y,
                                                          int N, int D, double momentum, double lr, double msn,
                                                          int plain_descent, double *__restrict__ part) {
     const int i = blockIdx.x * kThreads + threadIdx.x;
-    double ys[2] = {0, 0};
+    double ys[4] = {0, 0, 0, 0};  // D <= 4
     if (i < N) {
         if (plain_descent) {
             for (int d = 0; d < D; d++) {
@@ -1668,7 +1668,7 @@ __global__ __launch_bounds__(kThreads) void k_update_f64(double *__restrict__ Y,
                 ys[d] = Y[(long)i * D + d];
             }
         } else {
-            double u[2], step = 0;
+            double u[4], step = 0;
             for (int d = 0; d < D; d++) {
                 const long e = (long)i * D + d;
                 const double sd = dY[e] == .0 ? .0 : (dY[e] < .0 ? -1.0 : 1.0);
@@ -1747,10 +1747,13 @@ __global__ __launch_bounds__(kThreads) void k_kl(const TI *__restrict__ rowP, co
 // distances2similarities (tsne.cpp:1394-1469) for many points: one wavefront per point, lanes over
 // the K distances; the bisection steps are the reference's (beta doubling/halving until bracketed,
 // tol 1e-5, at most 200 evaluations).  Sums use a fixed wave reduction order.
+// self_col != 0: row pt holds all N distances of point pt (K = N) and its own entry is DBL_MIN instead
+// of exp(0), as the reference's `n >= 0` argument (tsne.cpp:1421, :1459).  accumulate != 0 adds the
+// row to P instead of storing it (the perplexity-list sum, tsne.cpp:1489-1493).
 __global__ __launch_bounds__(kThreads) void k_d2s(const double *__restrict__ Dist, double *__restrict__ P, int n,
                                                   int K, double perplexity, double sigma, int ifSquared,
                                                   const double *__restrict__ beta0, double *__restrict__ beta_out,
-                                                  int *__restrict__ iters_out) {
+                                                  int *__restrict__ iters_out, int self_col, int accumulate) {
     const long gid = (long)blockIdx.x * kThreads + threadIdx.x;
     const long pt = gid / kWave;
     const int lane = threadIdx.x % kWave;
@@ -1770,7 +1773,7 @@ __global__ __launch_bounds__(kThreads) void k_d2s(const double *__restrict__ Dis
             double s = 0, hs = 0;
             for (int m = lane; m < K; m += kWave) {
                 const double d = ifSquared ? Dp[m] : Dp[m] * Dp[m];
-                const double pv = exp(-beta * d);
+                const double pv = (self_col && m == pt) ? DBL_MIN : exp(-beta * d);
                 s += pv;
                 hs += beta * (d * pv);
             }
@@ -1797,7 +1800,7 @@ __global__ __launch_bounds__(kThreads) void k_d2s(const double *__restrict__ Dis
         double s = 0;
         for (int m = lane; m < K; m += kWave) {
             const double d = ifSquared ? Dp[m] : Dp[m] * Dp[m];
-            s += exp(-beta * d);
+            s += (self_col && m == pt) ? DBL_MIN : exp(-beta * d);
         }
         sum_P = DBL_MIN + wave_sum(s);
     }
@@ -1805,7 +1808,8 @@ __global__ __launch_bounds__(kThreads) void k_d2s(const double *__restrict__ Dis
     // at its iteration cap, the returned beta has already moved past it.
     for (int m = lane; m < K; m += kWave) {
         const double d = ifSquared ? Dp[m] : Dp[m] * Dp[m];
-        Pp[m] = exp(-beta_eval * d) / sum_P;
+        const double v = ((self_col && m == pt) ? DBL_MIN : exp(-beta_eval * d)) / sum_P;
+        Pp[m] = accumulate ? Pp[m] + v : v;
     }
     if (lane == 0) {
         beta_out[pt] = beta;

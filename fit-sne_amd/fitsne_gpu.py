@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libfitsne.so")
 
 OK, E_ARG, E_ALLOC, E_HIP, E_UNSUPPORTED, E_NODEVICE = 0, -1, -2, -3, -4, -5
 KERNEL_SQUARED_CAUCHY, KERNEL_GENERAL, KERNEL_SQUARED_GENERAL = 0, 1, 2
-FP64_PLAIN, FP32 = 0, 1
+FP64_PLAIN, FP32, FP64_EXACT = 0, 1, 2
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _u32 = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
@@ -64,6 +64,8 @@ SIGNATURES = {
     "fitsne_ctx_destroy": (_ci, [_vp]),
     "fitsne_set_P": (_ci, [_vp, _ci, _u32, _u32, _dp]),
     "fitsne_set_P_i64": (_ci, [_vp, _ci, _i64, _i32, _dp]),
+    "fitsne_exact_affinities": (_ci, [_vp, _dp, _ci, _ci, _cd, _cd, _ci, _vp, _vp, C.POINTER(_cd)]),
+    "fitsne_set_P_dense": (_ci, [_vp, _ci, _dp]),
     "fitsne_set_exaggeration": (_ci, [_vp, _cd]),
     "fitsne_set_reorder": (_ci, [_vp, _ci]),
     "fitsne_reorder": (_ci, [_vp]),
@@ -351,6 +353,25 @@ class Context:
             check(lib().fitsne_set_P(self.h, N, np.ascontiguousarray(row, dtype=np.uint32),
                                      np.ascontiguousarray(col, dtype=np.uint32), val))
         self.N = N
+
+    def exact_affinities(self, X, perplexity=30.0, sigma=-1.0, perplexity_list=None, download=True):
+        """Dense P of the exact mode built on the device (FP64_EXACT contexts).  Returns (P or None,
+        sum_P before normalisation)."""
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        N, D = X.shape
+        plist = None if perplexity_list is None else np.ascontiguousarray(perplexity_list, dtype=np.float64)
+        P = np.empty((N, N)) if download else None
+        s = _cd()
+        check(lib().fitsne_exact_affinities(self.h, X.reshape(-1), N, D, perplexity, sigma,
+                                            0 if plist is None else plist.size, _ptr(plist), _ptr(P),
+                                            C.byref(s)))
+        self.N = N
+        return P, s.value
+
+    def set_P_dense(self, P):
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        check(lib().fitsne_set_P_dense(self.h, P.shape[0], P.reshape(-1)))
+        self.N = P.shape[0]
 
     def set_exaggeration(self, f):
         check(lib().fitsne_set_exaggeration(self.h, f))

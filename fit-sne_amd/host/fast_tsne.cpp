@@ -8,10 +8,12 @@
 // perplexity calibration run on the GPU through libfitsne's C ABI; the host keeps the steps whose order
 // defines the result: data centring and scaling, the symmetrisation of P and its normalisation.
 //
-// Supported: nbody_algo = 2 (FFT interpolation) with theta != 0, no_dims 1 or 2.  knn_algo 1 (Annoy)
-// and 2 (VP tree) both use the exact neighbour search (Annoy's approximate search is not reproduced).
-// The exact (theta = 0) and Barnes-Hut paths are not part of this build and are rejected with a message.
-// FITSNE_PRECISION=fp64 in the environment selects the FP64 plain GPU path (default: the FP32 form).
+// Supported: nbody_algo = 2 (FFT interpolation) with theta != 0, no_dims 1 or 2, and the exact mode
+// (theta = 0, any nbody_algo, no_dims 1..4) with its dense P built on the GPU.  knn_algo 1 (Annoy) and
+// 2 (VP tree) both use the exact neighbour search (Annoy's approximate search is not reproduced).  The
+// Barnes-Hut path (nbody_algo = 1 with theta != 0) is not part of this build and is rejected with a
+// message.  FITSNE_PRECISION=fp64 in the environment selects the FP64 plain GPU path of the FFT mode
+// (default: the FP32 form); the exact mode is FP64.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -229,6 +231,57 @@ void write_file(const char *name, const std::vector<T> &v) {
     std::fclose(f);
 }
 
+// The data as TSNE::run prepares it before the similarities (tsne.cpp:153-165): centred, and scaled
+// by the largest absolute value when a perplexity is used.
+void prepare_data(const Params &p, std::vector<double> &X) {
+    zero_mean(X, p.n, p.d);
+    if (p.perplexity > 0 || !p.perplexity_list.empty()) {
+        std::printf("Using perplexity, so normalizing input data (to prevent numerical problems)\n");
+        double mx = 0.0;
+        for (double v : X) mx = std::max(mx, std::fabs(v));
+        for (double &v : X) v /= mx;
+    } else {
+        std::printf("Not using perplexity, so data are left un-normalized.\n");
+    }
+}
+
+// Exact mode (theta = 0, tsne.cpp:171-229): the dense P, built on the device by
+// fitsne_exact_affinities (calibrated rows, P + P^T, divided by the total) or read from P.dat, stays in
+// the context.  Returns the largest row sum of P when asked (the automatic early exaggeration).
+double exact_affinities(Params &p, std::vector<double> &X, fitsne_ctx *ctx, bool want_max_row) {
+    const int N = p.n, D = p.d;
+    const size_t nn = (size_t)N * N;
+    std::vector<double> P;
+    if (p.load_affinities == 1) {
+        std::printf("Loading exact input similarities from file...\n");
+        if (!read_file("P.dat", P, nn)) die("could not read P.dat");
+        check(fitsne_set_P_dense(ctx, N, P.data()), "upload P");
+    } else {
+        prepare_data(p, X);
+        std::printf("Theta set to 0, so running exact algorithm\n");
+        const bool need_host = p.load_affinities == 2 || want_max_row;
+        if (need_host) P.resize(nn);
+        double sum_P = 0.0;
+        const double *plist = p.perplexity_list.empty() ? nullptr : p.perplexity_list.data();
+        check(fitsne_exact_affinities(ctx, X.data(), N, D, p.perplexity, p.sigma, (int)p.perplexity_list.size(), plist,
+                                      need_host ? P.data() : nullptr, &sum_P),
+              "exact input similarities");
+        std::printf("Finished exact calculation of the P.  Sum_p: %lf \n", sum_P);
+        if (p.load_affinities == 2) {
+            std::printf("Saving exact input similarities to file...\n");
+            write_file("P.dat", P);
+        }
+    }
+    double mx = 0.0;
+    if (want_max_row)
+        for (int n = 0; n < N; n++) {
+            double s = 0.0;
+            for (int m = 0; m < N; m++) s += P[(size_t)n * N + m];
+            mx = std::max(mx, s);
+        }
+    return mx;
+}
+
 // Input similarities: kNN rows calibrated by perplexity (or a fixed sigma), symmetrised, normalised
 // (tsne.cpp:151-366).
 Csr affinities(Params &p, std::vector<double> &X) {
@@ -242,13 +295,7 @@ Csr affinities(Params &p, std::vector<double> &X) {
             die("could not read P_val.dat / P_col.dat");
         return P;
     }
-    zero_mean(X, N, D);
-    if (p.perplexity > 0 || !p.perplexity_list.empty()) {
-        std::printf("Using perplexity, so normalizing input data (to prevent numerical problems)\n");
-        double mx = 0.0;
-        for (double v : X) mx = std::max(mx, std::fabs(v));
-        for (double &v : X) v /= mx;
-    }
+    prepare_data(p, X);
     int K;
     double sigma;
     if (p.perplexity < 0) {
@@ -327,13 +374,36 @@ int main(int argc, char **argv) {
     if (!load_data(data_path, p, X, Y, have_init)) return 0;
     const int N = p.n, nd = p.no_dims;
     if (N - 1 < 3 * p.perplexity) die("Perplexity too large for the number of data points!");
-    if (p.theta == 0.0) die("exact t-SNE (theta = 0) is not part of this build; use the FFT path (theta > 0)");
-    if (p.nbody_algo != 2) die("only the FFT-accelerated interpolation (nbody_algo = 2) is part of this build");
-    if (p.knn_algo != 1 && p.knn_algo != 2) die("Invalid knn_algo param");
+    const bool exact = p.theta == 0.0;
+    if (!exact && p.nbody_algo != 2)
+        die("the Barnes-Hut approximation (nbody_algo = 1) is not part of this build; use nbody_algo = 2 or theta = 0");
+    if (!exact && p.knn_algo != 1 && p.knn_algo != 2) die("Invalid knn_algo param");
+    if (exact && nd > 4) die("the exact mode of this build supports no_dims <= 4");
     if (fitsne_device_count() < 1) die("no GPU device is visible");
 
-    Csr P = affinities(p, X);
-    std::printf("Input similarities computed (sparsity = %f)!\n", (double)P.row[N] / ((double)N * (double)N));
+    fitsne_ctx *ctx = nullptr;
+    double max_row_sum = 0.0;
+    const char *prec = std::getenv("FITSNE_PRECISION");
+    const int precision = exact ? FITSNE_FP64_EXACT
+                          : (prec && std::strcmp(prec, "fp64") == 0) ? FITSNE_FP64_PLAIN : FITSNE_FP32;
+    if (exact) {
+        check(fitsne_ctx_create(&ctx, 0, precision, nd, p.nterms, p.intervals_per_integer, p.min_num_intervals, p.df),
+              "context");
+        max_row_sum = exact_affinities(p, X, ctx, p.early_exag_coeff == 0);
+        std::printf("Input similarities computed \n");
+    } else {
+        Csr P = affinities(p, X);
+        std::printf("Input similarities computed (sparsity = %f)!\n", (double)P.row[N] / ((double)N * (double)N));
+        if (p.early_exag_coeff == 0)
+            for (int n = 0; n < N; n++) {
+                double s = 0.0;
+                for (uint32_t k = P.row[n]; k < P.row[n + 1]; k++) s += P.val[k];
+                max_row_sum = std::max(max_row_sum, s);
+            }
+        check(fitsne_ctx_create(&ctx, 0, precision, nd, p.nterms, p.intervals_per_integer, p.min_num_intervals, p.df),
+              "context");
+        check(fitsne_set_P(ctx, N, P.row.data(), P.col.data(), P.val.data()), "upload P");
+    }
 
     if (!have_init) {
         if (p.rand_seed >= 0) {
@@ -349,28 +419,18 @@ int main(int argc, char **argv) {
         std::printf("Using the given initialization.\n");
     }
 
+    // largest stable exaggeration (tsne.cpp:389-401; the reference's exact mode has no row_P there, the
+    // row sums of the dense P are its natural counterpart)
     double early = p.early_exag_coeff;
-    if (early == 0) {  // largest stable exaggeration (tsne.cpp:389-401)
-        double mx = 0.0;
-        for (int n = 0; n < N; n++) {
-            double s = 0.0;
-            for (uint32_t k = P.row[n]; k < P.row[n + 1]; k++) s += P.val[k];
-            mx = std::max(mx, s);
-        }
-        early = 1.0 / (p.learning_rate * mx);
-    }
+    if (early == 0) early = 1.0 / (p.learning_rate * max_row_sum);
     std::printf("Exaggerating Ps by %f\n", early);
 
-    const char *prec = std::getenv("FITSNE_PRECISION");
-    const int precision = (prec && std::strcmp(prec, "fp64") == 0) ? FITSNE_FP64_PLAIN : FITSNE_FP32;
-    fitsne_ctx *ctx = nullptr;
-    check(fitsne_ctx_create(&ctx, 0, precision, nd, p.nterms, p.intervals_per_integer, p.min_num_intervals, p.df),
-          "context");
-    check(fitsne_set_P(ctx, N, P.row.data(), P.col.data(), P.val.data()), "upload P");
     double exag = early;
     check(fitsne_set_exaggeration(ctx, exag), "exaggeration");
     check(fitsne_set_Y(ctx, Y.data()), "upload Y");
-    std::printf("Learning embedding (%s)...\n", precision == FITSNE_FP32 ? "FP32 form" : "FP64 plain");
+    std::printf("Learning embedding (%s)...\n", precision == FITSNE_FP32      ? "FP32 form"
+                                                 : precision == FITSNE_FP64_EXACT ? "exact, FP64"
+                                                                                  : "FP64 plain");
 
     std::vector<double> costs((size_t)p.max_iter, 0.0);
     double momentum = p.momentum;

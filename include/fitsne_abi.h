@@ -42,6 +42,7 @@ extern "C" {
 /* precisions of the resident tier */
 #define FITSNE_FP64_PLAIN 0 /* FP64 throughout, stages as in the reference               */
 #define FITSNE_FP32 1       /* FP32 grid/FFT/interpolation, FP64 Z/geometry (product form) */
+#define FITSNE_FP64_EXACT 2 /* exact t-SNE (theta = 0): dense P, O(N^2) gradient and KL, FP64 */
 
 /* Text of the last error on this thread (never NULL). */
 const char *fitsne_last_error(void);
@@ -166,9 +167,10 @@ typedef struct {
     int reserved;
 } fitsne_step_params;
 
-/* Create a context on `device` for an embedding of dimension no_dims (1 or 2) with the FFT settings
- * of the run (n_interpolation_points = the wrapper's nterms, intervals_per_integer,
- * min_num_intervals) and the kernel's degree of freedom df. */
+/* Create a context on `device` for an embedding of dimension no_dims (1 or 2; 1..4 for
+ * FITSNE_FP64_EXACT) with the FFT settings of the run (n_interpolation_points = the wrapper's nterms,
+ * intervals_per_integer, min_num_intervals; unused by the exact path) and the kernel's degree of
+ * freedom df. */
 int fitsne_ctx_create(fitsne_ctx **ctx, int device, int precision, int no_dims, int n_interpolation_points,
                       double intervals_per_integer, int min_num_intervals, double df);
 int fitsne_ctx_destroy(fitsne_ctx *ctx);
@@ -177,8 +179,24 @@ int fitsne_ctx_destroy(fitsne_ctx *ctx);
  * previous call.  The FP32 context stages val_P in FP32 once. */
 int fitsne_set_P(fitsne_ctx *ctx, int N, const uint32_t *row_P, const uint32_t *col_P, const double *val_P);
 int fitsne_set_P_i64(fitsne_ctx *ctx, int N, const int64_t *row_P, const int32_t *col_P, const double *val_P);
-/* Scalar multiplier on P (early/late exaggeration, tsne.cpp:389-412, :534-543).  The FP64 plain
- * context rescales its val_P copy in place like the reference; the FP32 context keeps a scalar. */
+/* Exact t-SNE (theta = 0), FITSNE_FP64_EXACT contexts only; no_dims 1..4.  P is the dense N x N
+ * matrix of the reference's exact mode (tsne.cpp:171-229), kept on the device.
+ * fitsne_exact_affinities replaces computeGaussianPerplexity's dense overload (tsne.cpp:1503-1530:
+ * squared distances of tsne.cpp:1832-1848, each row calibrated with its own entry set to DBL_MIN;
+ * perplexity > 0: that perplexity; perplexity == 0: the average over perplexity_list; perplexity < 0:
+ * the fixed width sigma) followed by the symmetrisation P[n][m] = P[m][n] = P[n][m] + P[m][n] and the
+ * division by the total (tsne.cpp:196-216).  X[N*D] is the caller's (already centred and scaled)
+ * data.  P_out (nullable) receives the N*N matrix (the P.dat contents); sum_P (nullable) the total
+ * before normalisation.  fitsne_set_P_dense uploads a dense P (the P.dat route, tsne.cpp:172-189).
+ * On such a context fitsne_gradient / fitsne_step compute computeExactGradient (tsne.cpp:1232-1282;
+ * sum_Q as there) and fitsne_kl computes evaluateError (tsne.cpp:1286-1326) at the resident Y. */
+int fitsne_exact_affinities(fitsne_ctx *ctx, const double *X, int N, int D, double perplexity, double sigma,
+                            int perplexity_list_length, const double *perplexity_list, double *P_out,
+                            double *sum_P);
+int fitsne_set_P_dense(fitsne_ctx *ctx, int N, const double *P);
+
+/* Scalar multiplier on P (early/late exaggeration, tsne.cpp:389-412, :534-543).  The FP64 plain and
+ * exact contexts rescale their copy of P in place like the reference; the FP32 context keeps a scalar. */
 int fitsne_set_exaggeration(fitsne_ctx *ctx, double factor);
 
 /* Spatial order (FP32 context): points are kept internally in the Morton order of their cell on a
@@ -219,7 +237,8 @@ int fitsne_batch_step(fitsne_batch *batch, const fitsne_step_params *prm);
 int fitsne_batch_destroy(fitsne_batch *batch);
 
 /* KL divergence at the resident Y with the Z of the last gradient (tsne.cpp:547-571 semantics:
- * stale Z, FLT_MIN guards, exaggerated P); the exaggeration correction is the caller's. */
+ * stale Z, FLT_MIN guards, exaggerated P); the exaggeration correction is the caller's.  An exact
+ * context evaluates evaluateError (tsne.cpp:1286-1326) with the Z of the current Y. */
 int fitsne_kl(fitsne_ctx *ctx, double *C);
 
 /* Wait for the context's stream. */

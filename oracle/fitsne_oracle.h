@@ -24,6 +24,12 @@
  *   orc_symmetrize            reference symmetrizeMatrix (oracle/_ref), bit-exact
  *   orc_gaussian_perplexity_exact_knn  reference VP-tree computeGaussianPerplexity (oracle/_ref)
  *   orc_zero_mean             reference zeroMean (oracle/_ref)
+ *   orc_squared_distances     reference computeSquaredEuclideanDistance (oracle/_ref), bit-exact
+ *   orc_gaussian_perplexity_dense  reference dense computeGaussianPerplexity (oracle/_ref), bit-exact
+ *   orc_exact_symmetrize_normalize  symmetry, unit total, brute-force (P + P^T) / sum on small inputs
+ *   orc_exact_gradient        reference computeExactGradient (oracle/_ref), bit-exact; finite differences
+ *                             of the exact KL
+ *   orc_evaluate_error_exact  reference evaluateError (oracle/_ref), bit-exact
  *   orc_update / orc_run_fft  elementwise closed forms of tsne.cpp:478-544 (parity pinned by formula
  *                             tests; the reference's TSNE::run cannot be linked without FFTW)
  */
@@ -150,6 +156,15 @@ void orc_rows_from_knn(const int32_t *idx, const double *dist, int N, int K, dou
 /* symmetrizeMatrix (tsne.cpp:1730-1828).  Returns nnz; outputs must hold 2*row_P[N] entries. */
 int orc_symmetrize(const uint32_t *row_P, const uint32_t *col_P, const double *val_P, int N,
                    uint32_t *sym_row_P, uint32_t *sym_col_P, double *sym_val_P);
+
+/* ---- exact t-SNE, dense P[N*N] row-major (tsne.cpp:171-229, :1232-1326, :1503-1530, :1832-1848) ---- */
+void orc_squared_distances(const double *X, int N, int D, double *DD);
+void orc_gaussian_perplexity_dense(const double *X, int N, int D, double perplexity, double sigma,
+                                   int perplexity_list_length, const double *perplexity_list, double *P,
+                                   unsigned nthreads);
+double orc_exact_symmetrize_normalize(double *P, int N);
+void orc_exact_gradient(const double *P, const double *Y, int N, int D, double df, double *dC, double *sum_Q);
+double orc_evaluate_error_exact(const double *P, const double *Y, int N, int D, double df);
 
 #ifdef __cplusplus
 }

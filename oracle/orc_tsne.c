@@ -665,3 +665,116 @@ int orc_symmetrize(const uint32_t *row_P, const uint32_t *col_P, const double *v
     free(row_counts);
     return no_elem;
 }
+
+/* ---- exact t-SNE (theta = 0): dense P ---- */
+
+/* tsne.cpp:1832-1848: the upper triangle in order d = 0..D-1 from 0.0, mirrored; zero diagonal. */
+void orc_squared_distances(const double *X, int N, int D, double *DD) {
+    for (long n = 0; n < N; n++) {
+        DD[n * N + n] = 0.0;
+        for (long m = n + 1; m < N; m++) {
+            double acc = 0.0;
+            for (int d = 0; d < D; d++) acc += (X[n * D + d] - X[m * D + d]) * (X[n * D + d] - X[m * D + d]);
+            DD[n * N + m] = acc;
+            DD[m * N + n] = acc;
+        }
+    }
+}
+
+typedef struct {
+    const double *DD;
+    double *P;
+    int N, L;
+    double perplexity, sigma;
+    const double *list;
+} gpd_ctx;
+
+static void gpd_body(int lo, int hi, void *c_) {
+    gpd_ctx *c = (gpd_ctx *)c_;
+    for (int n = lo; n < hi; n++)
+        orc_distances2similarities_list(c->DD + (long)n * c->N, c->P + (long)n * c->N, c->N, n, c->perplexity,
+                                        c->sigma, 1, c->L, c->list);
+}
+
+/* tsne.cpp:1503-1530 (computeGaussianPerplexity, dense overload): every row of the squared-distance
+ * matrix through distances2similarities with the row's own index as the self entry; the list overload
+ * is always called with ifSquared = true. */
+void orc_gaussian_perplexity_dense(const double *X, int N, int D, double perplexity, double sigma,
+                                   int perplexity_list_length, const double *perplexity_list, double *P,
+                                   unsigned nthreads) {
+    double *DD = (double *)malloc(sizeof(double) * (size_t)N * N);
+    orc_squared_distances(X, N, D, DD);
+    gpd_ctx c = {DD, P, N, perplexity_list_length, perplexity, sigma, perplexity_list};
+    orc_parallel_for(nthreads, N, gpd_body, &c);
+    free(DD);
+}
+
+/* tsne.cpp:196-216: P[n][m] = P[m][n] = P[n][m] + P[m][n] for n < m (diagonal untouched), then every
+ * entry divided by the total (summed in row-major order).  Returns the total. */
+double orc_exact_symmetrize_normalize(double *P, int N) {
+    for (long n = 0; n < N; n++)
+        for (long m = n + 1; m < N; m++) {
+            P[n * N + m] += P[m * N + n];
+            P[m * N + n] = P[n * N + m];
+        }
+    double sum_P = .0;
+    const long NN = (long)N * N;
+    for (long i = 0; i < NN; i++) sum_P += P[i];
+    for (long i = 0; i < NN; i++) P[i] /= sum_P;
+    return sum_P;
+}
+
+/* tsne.cpp:1232-1282: Q = 1/(1 + d2/df), Qpow = Q^df, sum_Q over n != m, and
+ * dC[n] = sum_{m != n} (Y[n] - Y[m]) (P[n][m] - Qpow/sum_Q) Q (no factor 4). */
+void orc_exact_gradient(const double *P, const double *Y, int N, int D, double df, double *dC, double *sum_Q_out) {
+    const long NN = (long)N * N;
+    double *DD = (double *)malloc(sizeof(double) * (size_t)NN);
+    double *Q = (double *)malloc(sizeof(double) * (size_t)NN);
+    double *Qpow = (double *)malloc(sizeof(double) * (size_t)NN);
+    for (long i = 0; i < (long)N * D; i++) dC[i] = 0.0;
+    orc_squared_distances(Y, N, D, DD);
+    double sum_Q = .0;
+    for (long n = 0; n < N; n++)
+        for (long m = 0; m < N; m++)
+            if (n != m) {
+                Q[n * N + m] = 1.0 / (1.0 + DD[n * N + m] / df);
+                Qpow[n * N + m] = pow(Q[n * N + m], df);
+                sum_Q += Qpow[n * N + m];
+            }
+    for (long n = 0; n < N; n++)
+        for (long m = 0; m < N; m++)
+            if (n != m) {
+                double mult = (P[n * N + m] - (Qpow[n * N + m] / sum_Q)) * Q[n * N + m];
+                for (int d = 0; d < D; d++) dC[n * D + d] += (Y[n * D + d] - Y[m * D + d]) * mult;
+            }
+    if (sum_Q_out) *sum_Q_out = sum_Q;
+    free(DD);
+    free(Q);
+    free(Qpow);
+}
+
+/* tsne.cpp:1286-1326: Q normalised by its own sum (started at DBL_MIN) with DBL_MIN on the diagonal;
+ * C = sum over all N^2 entries of P log((P + FLT_MIN) / (Q + FLT_MIN)). */
+double orc_evaluate_error_exact(const double *P, const double *Y, int N, int D, double df) {
+    const long NN = (long)N * N;
+    double *DD = (double *)malloc(sizeof(double) * (size_t)NN);
+    double *Q = (double *)malloc(sizeof(double) * (size_t)NN);
+    orc_squared_distances(Y, N, D, DD);
+    double sum_Q = DBL_MIN;
+    for (long n = 0; n < N; n++)
+        for (long m = 0; m < N; m++) {
+            if (n != m) {
+                Q[n * N + m] = 1.0 / (1.0 + DD[n * N + m] / df);
+                Q[n * N + m] = pow(Q[n * N + m], df);
+                sum_Q += Q[n * N + m];
+            } else {
+                Q[n * N + m] = DBL_MIN;
+            }
+        }
+    for (long i = 0; i < NN; i++) Q[i] /= sum_Q;
+    double C = .0;
+    for (long i = 0; i < NN; i++) C += P[i] * log((P[i] + FLT_MIN) / (Q[i] + FLT_MIN));
+    free(DD);
+    free(Q);
+    return C;
+}

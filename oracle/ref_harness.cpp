@@ -103,6 +103,19 @@ API double ref_evaluate_error_exact(double *P, double *Y, int N, int D, double d
     return t.evaluateError(P, Y, N, D, df);
 }
 
+/* tsne.cpp:1832-1848 */
+API void ref_squared_distances(double *X, int N, int D, double *DD) {
+    TSNE t;
+    t.computeSquaredEuclideanDistance(X, N, D, DD);
+}
+
+/* tsne.cpp:1503-1530 (dense overload used by the exact mode); P[N*N] */
+API void ref_gaussian_perplexity_dense(double *X, int N, int D, double *P, double perplexity, double sigma,
+                                       int list_len, double *list) {
+    TSNE t;
+    t.computeGaussianPerplexity(X, N, D, P, perplexity, sigma, list_len, list);
+}
+
 /* tsne.cpp:1851-1876 */
 API void ref_zero_mean(double *X, int N, int D) {
     TSNE t;

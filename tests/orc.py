@@ -75,6 +75,11 @@ def oracle():
         _bind(L, "orc_knn_exact", None, [dp, ci, ci, ci, i32p, dp, cu])
         _bind(L, "orc_rows_from_knn", None, [i32p, dp, ci, ci, cd, cd, ci, dp, u32p, u32p, dp])
         _bind(L, "orc_symmetrize", ci, [u32p, u32p, dp, ci, u32p, u32p, dp])
+        _bind(L, "orc_squared_distances", None, [dp, ci, ci, dp])
+        _bind(L, "orc_gaussian_perplexity_dense", None, [dp, ci, ci, cd, cd, ci, dp, dp, cu])
+        _bind(L, "orc_exact_symmetrize_normalize", cd, [dp, ci])
+        _bind(L, "orc_exact_gradient", None, [dp, dp, ci, ci, cd, dp, C.POINTER(cd)])
+        _bind(L, "orc_evaluate_error_exact", cd, [dp, dp, ci, ci, cd])
         _oracle = L
     return _oracle
 
@@ -97,6 +102,8 @@ def ref():
         _bind(L, "ref_exact_gradient", None, [dp, dp, ci, ci, dp, cd])
         _bind(L, "ref_evaluate_error_exact", cd, [dp, dp, ci, ci, cd])
         _bind(L, "ref_zero_mean", None, [dp, ci, ci])
+        _bind(L, "ref_squared_distances", None, [dp, ci, ci, dp])
+        _bind(L, "ref_gaussian_perplexity_dense", None, [dp, ci, ci, dp, cd, cd, ci, dp])
         _bind(L, "ref_gaussian_perplexity_vptree", None,
               [dp, ci, ci, cd, ci, cd, cu, ci, dp, cu, u32p, u32p, dp])
         _bind(L, "ref_load_data", ci, [C.c_char_p, i32p, dp, C.c_void_p, C.c_void_p, C.c_void_p])
@@ -297,3 +304,63 @@ def run_fft(P, Y0, **kw):
     rc = oracle().orc_run_fft(row, col, val, N, D, Y.reshape(-1), C.byref(prm), costs)
     assert rc == 0
     return Y, costs
+
+
+# ---------------------------------------------------------------- exact t-SNE (dense P)
+
+def squared_distances(X, lib="oracle"):
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    N, D = X.shape
+    DD = np.empty(N * N)
+    if lib == "oracle":
+        oracle().orc_squared_distances(X.reshape(-1), N, D, DD)
+    else:
+        ref().ref_squared_distances(X.reshape(-1).copy(), N, D, DD)
+    return DD.reshape(N, N)
+
+
+def gaussian_perplexity_dense(X, perplexity, sigma=-1.0, plist=None, lib="oracle", nthreads=8):
+    """Row-calibrated dense P before symmetrisation (tsne.cpp:1503-1530)."""
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    N, D = X.shape
+    P = np.empty(N * N)
+    L = 0 if plist is None else len(plist)
+    pl = np.ascontiguousarray(plist if plist is not None else np.zeros(1), dtype=np.float64)
+    if lib == "oracle":
+        oracle().orc_gaussian_perplexity_dense(X.reshape(-1), N, D, perplexity, sigma, L, pl, P, nthreads)
+    else:
+        ref().ref_gaussian_perplexity_dense(X.reshape(-1).copy(), N, D, P, perplexity, sigma, L, pl.copy())
+    return P.reshape(N, N)
+
+
+def exact_symmetrize_normalize(P):
+    P = np.ascontiguousarray(P, dtype=np.float64).copy()
+    s = oracle().orc_exact_symmetrize_normalize(P.reshape(-1), P.shape[0])
+    return P, s
+
+
+def exact_P(X, perplexity, sigma=-1.0, plist=None):
+    """The exact mode's P (tsne.cpp:171-229): calibrated rows, P + P^T, divided by the total."""
+    return exact_symmetrize_normalize(gaussian_perplexity_dense(X, perplexity, sigma, plist))
+
+
+def exact_gradient(P, Y, df=1.0, lib="oracle"):
+    P = np.ascontiguousarray(P, dtype=np.float64)
+    Y = np.ascontiguousarray(Y, dtype=np.float64)
+    N, D = Y.shape
+    dC = np.empty(N * D)
+    if lib == "oracle":
+        sq = C.c_double()
+        oracle().orc_exact_gradient(P.reshape(-1), Y.reshape(-1), N, D, df, dC, C.byref(sq))
+        return dC.reshape(N, D), sq.value
+    ref().ref_exact_gradient(P.reshape(-1).copy(), Y.reshape(-1).copy(), N, D, dC, df)
+    return dC.reshape(N, D), None
+
+
+def exact_kl(P, Y, df=1.0, lib="oracle"):
+    P = np.ascontiguousarray(P, dtype=np.float64)
+    Y = np.ascontiguousarray(Y, dtype=np.float64)
+    N, D = Y.shape
+    if lib == "oracle":
+        return oracle().orc_evaluate_error_exact(P.reshape(-1), Y.reshape(-1), N, D, df)
+    return ref().ref_evaluate_error_exact(P.reshape(-1).copy(), Y.reshape(-1).copy(), N, D, df)
