@@ -113,22 +113,24 @@ __device__ __forceinline__ double exact_qpow(double q, double df) {
     else return pow(q, df);
 }
 
-// One sweep over rows n0 .. n0 + kExRows - 1 of the pair matrix.  With P: att[n], rep[n] (D each) and
-// zrow[n] = sum_{m != n} q^df.  Without P (nullptr): zrow only.  Rows past N (last workgroup) are
-// computed on row N - 1 and discarded.  Grid: ceil(N / kExRows) workgroups of kThreads.
+// One sweep over rows n0 .. n0 + kExRows - 1 of the pair matrix.  GRAD: att[n], rep[n] (D each) and
+// zrow[n] = sum_{m != n} q^df; otherwise zrow only (P unused).  Rows past N (last workgroup) are
+// computed on row N - 1 and discarded.  The next column's Y and P values are loaded into a second
+// register buffer while the current column is computed, so the loads of consecutive columns overlap
+// (the P loads are unconditional: every index is in bounds, the self pair is masked afterwards).
 // DF1 = (df == 1).
-template <int D, bool DF1>
+// Grid: ceil(N / kExRows) workgroups of kThreads.
+template <int D, bool DF1, bool GRAD>
 __global__ __launch_bounds__(kThreads) void k_exact_rows(const double *__restrict__ P, const double *__restrict__ Y,
                                                          int N, double df, double *__restrict__ att,
                                                          double *__restrict__ rep, double *__restrict__ zrow) {
     const int n0 = blockIdx.x * kExRows;
-    const bool grad = P != nullptr;
     double yn[kExRows][D], a[kExRows][D], b[kExRows][D], s[kExRows];
-    long prow[kExRows];
+    const double *prow[kExRows];
 #pragma unroll
     for (int r = 0; r < kExRows; r++) {
         const int n = min(n0 + r, N - 1);
-        prow[r] = (long)n * N;
+        prow[r] = GRAD ? P + (long)n * N : nullptr;
 #pragma unroll
         for (int d = 0; d < D; d++) {
             yn[r][d] = Y[(long)n * D + d];
@@ -137,14 +139,23 @@ __global__ __launch_bounds__(kThreads) void k_exact_rows(const double *__restric
         }
         s[r] = 0.0;
     }
-#pragma unroll 2
-    for (int m = threadIdx.x; m < N; m += kThreads) {
-        double ym[D];
+    // two register buffers: the loads of column m + kThreads are in flight while column m is computed
+    double yA[D], pA[kExRows], yB[D], pB[kExRows];
+    // branch-free loads (a column past N reloads column N - 1 and is not used), so that the wait for
+    // one buffer never waits for the other
+    auto fetch = [&](int mm, double (&y)[D], double (&p)[kExRows]) {
+        const int mc = min(mm, N - 1);
 #pragma unroll
-        for (int d = 0; d < D; d++) ym[d] = Y[(long)m * D + d];
+        for (int d = 0; d < D; d++) y[d] = Y[(long)mc * D + d];
+        if constexpr (GRAD) {
+#pragma unroll
+            for (int r = 0; r < kExRows; r++) p[r] = prow[r][mc];
+        }
+    };
+    auto column = [&](int mm, const double (&ym)[D], const double (&pv)[kExRows]) {
 #pragma unroll
         for (int r = 0; r < kExRows; r++) {
-            const bool self = m == n0 + r;
+            const bool self = mm == n0 + r;
             double diff[D], d2 = 0.0;
             {
 #pragma clang fp contract(off)
@@ -157,8 +168,8 @@ __global__ __launch_bounds__(kThreads) void k_exact_rows(const double *__restric
             const double q = exact_q<DF1>(d2, df);
             const double qp = exact_qpow<DF1>(q, df);
             s[r] += self ? 0.0 : qp;
-            if (grad) {
-                const double pq = (self ? 0.0 : P[prow[r] + m]) * q;
+            if constexpr (GRAD) {
+                const double pq = (self ? 0.0 : pv[r]) * q;
                 const double qq = self ? 0.0 : qp * q;
 #pragma unroll
                 for (int d = 0; d < D; d++) {
@@ -167,19 +178,30 @@ __global__ __launch_bounds__(kThreads) void k_exact_rows(const double *__restric
                 }
             }
         }
+    };
+    int m = threadIdx.x;
+    fetch(m, yA, pA);
+    for (; m < N; m += 2 * kThreads) {
+        fetch(m + kThreads, yB, pB);
+        column(m, yA, pA);
+        if (m + kThreads >= N) break;
+        fetch(m + 2 * kThreads, yA, pA);
+        column(m + kThreads, yB, pB);
     }
     constexpr int V = 2 * D + 1;
     __shared__ double red[kThreads / kWave][kExRows * V];
     const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
 #pragma unroll
     for (int r = 0; r < kExRows; r++) {
+        if constexpr (GRAD) {
 #pragma unroll
-        for (int d = 0; d < D; d++) {
-            const double ta = grad ? wave_sum(a[r][d]) : 0.0;
-            const double tb = grad ? wave_sum(b[r][d]) : 0.0;
-            if (l == 0) {
-                red[w][r * V + d] = ta;
-                red[w][r * V + D + d] = tb;
+            for (int d = 0; d < D; d++) {
+                const double ta = wave_sum(a[r][d]);
+                const double tb = wave_sum(b[r][d]);
+                if (l == 0) {
+                    red[w][r * V + d] = ta;
+                    red[w][r * V + D + d] = tb;
+                }
             }
         }
         const double ts = wave_sum(s[r]);
@@ -187,14 +209,14 @@ __global__ __launch_bounds__(kThreads) void k_exact_rows(const double *__restric
     }
     __syncthreads();
     if (threadIdx.x < kExRows * V) {
-        double t = 0.0;
-        for (int i = 0; i < kThreads / kWave; i++) t += red[i][threadIdx.x];
         const int r = threadIdx.x / V, k = threadIdx.x % V;
         const int n = n0 + r;
-        if (n < N) {
+        if (n < N && (GRAD || k == 2 * D)) {
+            double t = 0.0;
+            for (int i = 0; i < kThreads / kWave; i++) t += red[i][threadIdx.x];
             if (k == 2 * D) zrow[n] = t;
-            else if (grad && k < D) att[(long)n * D + k] = t;
-            else if (grad) rep[(long)n * D + k - D] = t;
+            else if (k < D) att[(long)n * D + k] = t;
+            else rep[(long)n * D + k - D] = t;
         }
     }
 }
@@ -229,13 +251,17 @@ __global__ __launch_bounds__(kThreads) void k_exact_kl(const double *__restrict_
         for (int d = 0; d < D; d++) yn[r][d] = Y[(long)n * D + d];
     }
     double acc = 0.0;
-    for (int m = threadIdx.x; m < N; m += kThreads) {
-        double ym[D];
+    double yA[D], pA[kExRows], yB[D], pB[kExRows];
+    auto fetch = [&](int mm, double (&y)[D], double (&p)[kExRows]) {
+        const int mc = min(mm, N - 1);
 #pragma unroll
-        for (int d = 0; d < D; d++) ym[d] = Y[(long)m * D + d];
+        for (int d = 0; d < D; d++) y[d] = Y[(long)mc * D + d];
+#pragma unroll
+        for (int r = 0; r < kExRows; r++) p[r] = P[prow[r] + mc];
+    };
+    auto column = [&](int mm, const double (&ym)[D], const double (&pv)[kExRows]) {
 #pragma unroll
         for (int r = 0; r < kExRows; r++) {
-            if (n0 + r >= N) continue;
             double d2 = 0.0;
             {
 #pragma clang fp contract(off)
@@ -245,10 +271,20 @@ __global__ __launch_bounds__(kThreads) void k_exact_kl(const double *__restrict_
                     d2 += t * t;
                 }
             }
-            const double Q = (m == n0 + r ? DBL_MIN : exact_qpow<DF1>(exact_q<DF1>(d2, df), df)) / sQ;
-            const double p = P[prow[r] + m];
-            acc += p * log((p + fm) / (Q + fm));
+            const double Q = (mm == n0 + r ? DBL_MIN : exact_qpow<DF1>(exact_q<DF1>(d2, df), df)) / sQ;
+            const double p = pv[r];
+            const double c = p * log((p + fm) / (Q + fm));
+            acc += n0 + r < N ? c : 0.0;
         }
+    };
+    int m = threadIdx.x;
+    fetch(m, yA, pA);
+    for (; m < N; m += 2 * kThreads) {
+        fetch(m + kThreads, yB, pB);
+        column(m, yA, pA);
+        if (m + kThreads >= N) break;
+        fetch(m + 2 * kThreads, yA, pA);
+        column(m + kThreads, yB, pB);
     }
     __shared__ double sh[kThreads / kWave];
     acc = block_sum(acc, sh);

