@@ -172,6 +172,68 @@ def bench_c4(args, fg, synth):
     print(json.dumps(out), flush=True)
 
 
+def bench_exact(args, fg):
+    """Exact t-SNE (theta = 0) at N = 46,000 in 2D, the largest N whose N*N fits the reference's int
+    indexing (tsne.cpp:186, :1232-1330): dense P built on the device, then timed steps (exact gradient,
+    update, zeroMean).  Roofline of the gradient sweep: it reads the dense P once (8 N^2 bytes)."""
+    N = args.n if args.n != 1_000_000 else 46_000
+    rng = np.random.default_rng(20261202)
+    k = 20
+    X = rng.normal(0, 4, size=(k, 50))[rng.integers(0, k, N)] + rng.normal(size=(N, 50))
+    X -= X.mean(0)
+    X /= np.abs(X).max()
+    c = fg.Context(no_dims=2, precision=fg.FP64_EXACT)
+    log(f"exact: dense affinities N={N} on the device")
+    t0 = time.perf_counter()
+    c.exact_affinities(X, 30.0, download=False)
+    c.sync()
+    t_aff = time.perf_counter() - t0
+    c.set_Y(rng.normal(0, 10, size=(N, 2)))
+    prm = dict(momentum=0.8, learning_rate=max(200.0, N / 12.0), max_step_norm=5.0)
+    dt = timed_steps(c, args.steps, args.warmup, prm)
+    ms = 1e3 * dt / args.steps
+    stats = kernel_profile(c, max(args.profile_steps, 1), prm)
+    t0 = time.perf_counter()
+    kl = c.kl()
+    t_kl = time.perf_counter() - t0
+    c.close()
+    cnt, tot = stats.get("exact_rows", (0, 0.0))
+    rows_ms = tot / max(cnt, 1)
+    nbytes = 8.0 * N * N + 8.0 * N * 2 + 8.0 * N * 5
+    achieved = nbytes / (rows_ms * 1e-3) / 1e9 if rows_ms > 0 else 0.0
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import orc
+
+        ns = 6000
+        log(f"CPU baseline: oracle exact gradient at N={ns} (single thread, as computeExactGradient)")
+        Ps, _ = orc.exact_P(X[:ns] / np.abs(X[:ns]).max(), 30.0)
+        Ys = rng.normal(0, 10, size=(ns, 2))
+        t0 = time.perf_counter()
+        orc.exact_gradient(Ps, Ys)
+        t_cpu = time.perf_counter() - t0
+        cpu = {"value": 1.0 / (t_cpu * (N / ns) ** 2), "unit": "grad iters/s", "cores": 1, "kind": "port",
+               "sample": f"one oracle exact gradient (C restatement of computeExactGradient, serial) at N={ns} "
+                         f"from the same construction, {t_cpu:.2f} s, scaled by the pair count (N/{ns})^2"}
+    out = {
+        "metric": "exact (theta=0) gradient iterations/s at N=46000 2D, FP64",
+        "value": args.steps / dt, "unit": "grad iters/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded Gaussian mixture, generated in place)",
+        "config": {"workload": f"exact t-SNE: N={N} points, D=50 input, 2D embedding, perplexity 30, dense P"},
+        "roofline": {"bound": "hbm", "kernel": "k_exact_rows<2, true, true>", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes": nbytes, "avg_ms": rows_ms},
+        "cpu_baseline": cpu,
+        "affinities_s": t_aff,
+        "kl_ms": 1e3 * t_kl,
+        "kl": kl,
+        "kernels_ms": {k: round(v[1] / max(v[0], 1), 4) for k, v in sorted(stats.items(), key=lambda kv: -kv[1][1])},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -182,8 +244,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=5)
-    ap.add_argument("--config", default="c2", choices=["c2", "c4"],
-                    help="c2 (default): the headline step at N=1M 2D; c4: 64 batched embeddings")
+    ap.add_argument("--config", default="c2", choices=["c2", "c4", "exact"],
+                    help="c2 (default): the headline step at N=1M 2D; c4: 64 batched embeddings; "
+                         "exact: theta = 0 at N=46000 2D")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run one plain 64 MiB device copy (torch) as the reference kernel for "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE scaling")
@@ -198,6 +261,8 @@ def main():
 
     if args.config == "c4":
         return bench_c4(args, fg, synth)
+    if args.config == "exact":
+        return bench_exact(args, fg)
 
     if args.copy_calibration:
         import torch

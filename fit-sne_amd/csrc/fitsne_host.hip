@@ -14,6 +14,7 @@
 #include "../../include/fitsne_abi.h"
 #include "fitsne_kernels.hpp"
 #include "fitsne_exact.hpp"
+#include "fitsne_sym.hpp"
 #include "fitsne_knn.hpp"
 #include "fitsne_mlp.hpp"
 
@@ -132,6 +133,9 @@ struct fitsne_ctx {
     DBuf xs, ys, box, wx, wy, ch, w, V, potA, potB;
     // exact path (dense P in val): per-row attractive / repulsive sums and Z shares
     DBuf att, rep, zrow;
+    // symmetrisation of a CSR P (mirror tier)
+    DBuf sy_row, sy_col, sy_val, sy_erow, sy_keys, sy_keys2, sy_vals, sy_vals2, sy_partner, sy_cnt, sy_start,
+        sy_okeys, sy_okeys2, sy_tmp, sy_scol, sy_sval;
     // twiddles per FFT length
     std::map<int, DBuf> tw32, tw64;
     // min/max handoff from the last step
@@ -960,7 +964,10 @@ extern "C" int fitsne_ctx_destroy(fitsne_ctx *c) {
                    &c->dstep, &c->dtw, &c->dlive, &c->klpart, &c->gin, &c->gout, &c->spec, &c->ospec, &c->Trow, &c->KQ,
                    &c->K1, &c->KS, &c->KT, &c->zpart, &c->zbuf, &c->part, &c->mean, &c->mmpart, &c->mm, &c->xs,
                    &c->ys, &c->box, &c->wx, &c->wy, &c->ch, &c->w, &c->V, &c->potA, &c->potB,
-                   &c->inv_d, &c->colP2, &c->val2, &c->tmp, &c->fixgrid, &c->att, &c->rep, &c->zrow};
+                   &c->inv_d, &c->colP2, &c->val2, &c->tmp, &c->fixgrid, &c->att, &c->rep, &c->zrow,
+                   &c->sy_row, &c->sy_col, &c->sy_val, &c->sy_erow, &c->sy_keys, &c->sy_keys2, &c->sy_vals,
+                   &c->sy_vals2, &c->sy_partner, &c->sy_cnt, &c->sy_start, &c->sy_okeys, &c->sy_okeys2, &c->sy_tmp,
+                   &c->sy_scol, &c->sy_sval};
     for (DBuf *b : all) release(*b);
     for (auto &kv : c->tw32) release(kv.second);
     for (auto &kv : c->tw64) release(kv.second);
@@ -1977,6 +1984,95 @@ extern "C" int fitsne_distances2similarities_batch(const double *D, double *P, i
     HIPTRY(hipMemcpyAsync(beta, c->xs.p, sizeof(double) * n_points, hipMemcpyDeviceToHost, c->stream));
     if (iters) HIPTRY(hipMemcpyAsync(iters, c->box.p, sizeof(int) * n_points, hipMemcpyDeviceToHost, c->stream));
     return sync_ctx(c);
+}
+
+// Number of bits that hold every value in [0, v].
+static int bits_for(unsigned long long v) {
+    int b = 1;
+    while (b < 64 && (1ull << b) <= v) b++;
+    return b;
+}
+
+extern "C" int fitsne_symmetrize(const uint32_t *row_P, const uint32_t *col_P, const double *val_P, int N,
+                                 uint32_t *sym_row_P, uint32_t *sym_col_P, double *sym_val_P, long long *sym_nnz) {
+    if (!row_P || !col_P || !val_P || !sym_row_P || !sym_col_P || !sym_val_P || N <= 0)
+        return fail(FITSNE_E_ARG, "invalid symmetrisation arguments");
+    if (row_P[0] != 0) return fail(FITSNE_E_ARG, "row_P must start at 0");
+    for (int i = 0; i < N; i++)
+        if (row_P[i + 1] < row_P[i]) return fail(FITSNE_E_ARG, "row_P must be non-decreasing");
+    const long nnz = (long)row_P[N];
+    if (nnz > (long)(INT32_MAX / 2)) return fail(FITSNE_E_UNSUPPORTED, "more than 2^30 entries");
+    for (long k = 0; k < nnz; k++)
+        if (col_P[k] >= (uint32_t)N) return fail(FITSNE_E_ARG, "col_P entry out of range");
+    const int cbits = bits_for((unsigned long long)N - 1), rbits = bits_for((unsigned long long)N);
+    const int ebits = bits_for((unsigned long long)std::max<long>(nnz - 1, 0));
+    std::lock_guard<std::mutex> lk(g_mirror_mu);
+    fitsne_ctx *c;
+    TRY(mirror_ctx(&c, 1, 3, 1.0, 1, 1.0));
+    HIPTRY(hipSetDevice(c->device));
+    if (nnz == 0) {
+        for (int i = 0; i <= N; i++) sym_row_P[i] = 0;
+        if (sym_nnz) *sym_nnz = 0;
+        return FITSNE_OK;
+    }
+    const size_t ne = (size_t)nnz;
+    TRY(ensure(c->sy_row, sizeof(uint32_t) * (N + 1)));
+    TRY(ensure(c->sy_col, sizeof(uint32_t) * ne));
+    TRY(ensure(c->sy_val, sizeof(double) * ne));
+    TRY(ensure(c->sy_erow, sizeof(int) * ne));
+    TRY(ensure(c->sy_keys, sizeof(uint64_t) * ne));
+    TRY(ensure(c->sy_keys2, sizeof(uint64_t) * ne));
+    TRY(ensure(c->sy_vals, sizeof(uint32_t) * ne));
+    TRY(ensure(c->sy_vals2, sizeof(uint32_t) * ne));
+    TRY(ensure(c->sy_partner, sizeof(int64_t) * ne));
+    TRY(ensure(c->sy_cnt, sizeof(int) * N));
+    TRY(ensure(c->sy_start, sizeof(int) * (N + 1)));
+    TRY(ensure(c->sy_okeys, sizeof(uint64_t) * 2 * ne));
+    TRY(ensure(c->sy_okeys2, sizeof(uint64_t) * 2 * ne));
+    TRY(ensure(c->sy_scol, sizeof(uint32_t) * 2 * ne));
+    TRY(ensure(c->sy_sval, sizeof(double) * 2 * ne));
+    HIPTRY(hipMemcpyAsync(c->sy_row.p, row_P, sizeof(uint32_t) * (N + 1), hipMemcpyHostToDevice, c->stream));
+    HIPTRY(hipMemcpyAsync(c->sy_col.p, col_P, sizeof(uint32_t) * ne, hipMemcpyHostToDevice, c->stream));
+    HIPTRY(hipMemcpyAsync(c->sy_val.p, val_P, sizeof(double) * ne, hipMemcpyHostToDevice, c->stream));
+    HIPTRY(hipMemsetAsync(c->sy_cnt.p, 0, sizeof(int) * N, c->stream));
+    const int gblk = std::min(blocks_for(nnz), 8192);
+    launch(c, "sym_rows", k_entry_rows<uint32_t>, dim3(blocks_for((long)N * kWave)), dim3(kThreads), 0,
+           c->sy_row.as<const uint32_t>(), N, c->sy_erow.as<int>());
+    launch(c, "sym_keys", k_sym_keys, dim3(gblk), dim3(kThreads), 0, c->sy_erow.as<const int>(),
+           c->sy_col.as<const uint32_t>(), nnz, cbits, c->sy_keys.as<uint64_t>(), c->sy_vals.as<uint32_t>());
+    // radix sorts: temporary storage sized for the larger of the two
+    size_t t1 = 0, t2 = 0;
+    HIPTRY(rocprim::radix_sort_pairs(nullptr, t1, c->sy_keys.as<uint64_t>(), c->sy_keys2.as<uint64_t>(),
+                                     c->sy_vals.as<uint32_t>(), c->sy_vals2.as<uint32_t>(), ne, 0u,
+                                     (unsigned)(rbits + cbits), c->stream));
+    HIPTRY(rocprim::radix_sort_keys(nullptr, t2, c->sy_okeys.as<uint64_t>(), c->sy_okeys2.as<uint64_t>(), 2 * ne, 0u,
+                                    (unsigned)(rbits + ebits), c->stream));
+    TRY(ensure(c->sy_tmp, std::max<size_t>(std::max(t1, t2), 16)));
+    HIPTRY(rocprim::radix_sort_pairs(c->sy_tmp.p, t1, c->sy_keys.as<uint64_t>(), c->sy_keys2.as<uint64_t>(),
+                                     c->sy_vals.as<uint32_t>(), c->sy_vals2.as<uint32_t>(), ne, 0u,
+                                     (unsigned)(rbits + cbits), c->stream));
+    const uint64_t none = (uint64_t)N << ebits;
+    launch(c, "sym_place", k_sym_place<uint32_t>, dim3(gblk), dim3(kThreads), 0, c->sy_row.as<const uint32_t>(),
+           c->sy_erow.as<const int>(), c->sy_col.as<const uint32_t>(), c->sy_keys2.as<const uint64_t>(),
+           c->sy_vals2.as<const uint32_t>(), nnz, cbits, ebits, none, c->sy_partner.as<int64_t>(),
+           c->sy_cnt.as<int>(), c->sy_okeys.as<uint64_t>());
+    HIPTRY(rocprim::radix_sort_keys(c->sy_tmp.p, t2, c->sy_okeys.as<uint64_t>(), c->sy_okeys2.as<uint64_t>(), 2 * ne,
+                                    0u, (unsigned)(rbits + ebits), c->stream));
+    launch(c, "scan_counts", k_scan_counts, dim3(1), dim3(1024), 0, c->sy_cnt.as<int>(), c->sy_start.as<int>(), N);
+    launch(c, "sym_fill", k_sym_fill, dim3(std::min(blocks_for(2 * nnz), 8192)), dim3(kThreads), 0,
+           c->sy_okeys2.as<const uint64_t>(), c->sy_start.as<const int>() + N, ebits, c->sy_erow.as<const int>(),
+           c->sy_col.as<const uint32_t>(), c->sy_val.as<const double>(), c->sy_partner.as<const int64_t>(),
+           c->sy_scol.as<uint32_t>(), c->sy_sval.as<double>());
+    HIPTRY(hipGetLastError());
+    std::vector<int> start((size_t)N + 1);
+    HIPTRY(hipMemcpyAsync(start.data(), c->sy_start.p, sizeof(int) * (N + 1), hipMemcpyDeviceToHost, c->stream));
+    TRY(sync_ctx(c));
+    const long M = start[N];
+    for (int i = 0; i <= N; i++) sym_row_P[i] = (uint32_t)start[i];
+    HIPTRY(hipMemcpy(sym_col_P, c->sy_scol.p, sizeof(uint32_t) * (size_t)M, hipMemcpyDeviceToHost));
+    HIPTRY(hipMemcpy(sym_val_P, c->sy_sval.p, sizeof(double) * (size_t)M, hipMemcpyDeviceToHost));
+    if (sym_nnz) *sym_nnz = M;
+    return FITSNE_OK;
 }
 
 extern "C" int fitsne_knn_exact(const double *X, int N, int D, int K, int32_t *idx, double *dist) {

@@ -49,6 +49,7 @@ SIGNATURES = {
     "fitsne_evaluate_error_fft": (_ci, [_u32, _u32, _dp, _dp, _ci, _ci, _cd, _cd, C.POINTER(_cd)]),
     "fitsne_distances2similarities_batch": (_ci, [_dp, _dp, _ci, _ci, _cd, _cd, _ci, _vp, _dp, _vp]),
     "fitsne_knn_exact": (_ci, [_dp, _ci, _ci, _ci, _i32, _dp]),
+    "fitsne_symmetrize": (_ci, [_u32, _u32, _dp, _ci, _u32, _u32, _dp, C.POINTER(C.c_longlong)]),
     "fitsne_set_graph": (_ci, [_vp, _ci]),
     "fitsne_batch_create": (_ci, [C.POINTER(_vp), C.POINTER(_vp), _ci]),
     "fitsne_batch_step": (_ci, [_vp, C.POINTER(StepParams)]),
@@ -217,6 +218,21 @@ def knn_exact(X, K):
     dist = np.empty((N, K))
     check(lib().fitsne_knn_exact(X.reshape(-1), N, D, K, idx.reshape(-1), dist.reshape(-1)))
     return idx, dist
+
+
+def symmetrize(row, col, val):
+    """symmetrizeMatrix on the GPU: returns (sym_row, sym_col, sym_val) before normalisation."""
+    row = np.ascontiguousarray(row, dtype=np.uint32)
+    col = np.ascontiguousarray(col, dtype=np.uint32)
+    val = np.ascontiguousarray(val, dtype=np.float64)
+    N = row.size - 1
+    nnz = int(row[-1])
+    sr = np.empty(N + 1, dtype=np.uint32)
+    sc = np.empty(max(2 * nnz, 1), dtype=np.uint32)
+    sv = np.empty(max(2 * nnz, 1))
+    m = C.c_longlong()
+    check(lib().fitsne_symmetrize(row, col, val, N, sr, sc, sv, C.byref(m)))
+    return sr, sc[: m.value].copy(), sv[: m.value].copy()
 
 
 class Batch:

@@ -4,9 +4,9 @@
 //     fast_tsne 1.2.1 [data_path=data.dat] [result_path=result.dat] [nthreads=0]
 // reads the parameter/data file written by fast_tsne.py (layout of TSNE::load_data, tsne.cpp:1895-2022),
 // runs the optimisation with the semantics of TSNE::run (tsne.cpp:118-592) and writes the result file
-// (TSNE::save_data, tsne.cpp:2025-2036).  The gradient loop, the nearest-neighbour search and the
-// perplexity calibration run on the GPU through libfitsne's C ABI; the host keeps the steps whose order
-// defines the result: data centring and scaling, the symmetrisation of P and its normalisation.
+// (TSNE::save_data, tsne.cpp:2025-2036).  The gradient loop, the nearest-neighbour search, the
+// perplexity calibration and the symmetrisation of P run on the GPU through libfitsne's C ABI; the host
+// keeps the sums whose order defines the result: data centring and scaling, and the normalisation of P.
 //
 // Supported: nbody_algo = 2 (FFT interpolation) with theta != 0, no_dims 1 or 2, and the exact mode
 // (theta = 0, any nbody_algo, no_dims 1..4) with its dense P built on the GPU.  knn_algo 1 (Annoy) and
@@ -165,51 +165,19 @@ struct Csr {
     std::vector<double> val;
 };
 
-// (P + P^T) / 2 with the entry order of the reference's symmetrizeMatrix (tsne.cpp:1730-1828): the
-// pairs {i, j} are visited in the order of their first entry in row-major order (the entry in the
-// lower row when both (i, j) and (j, i) exist), and each visit appends the pair to both rows.
+// (P + P^T) / 2 with the entry order and values of the reference's symmetrizeMatrix
+// (tsne.cpp:1730-1828), computed on the GPU.
 Csr symmetrize(const Csr &P, int N) {
-    const std::vector<uint32_t> &row = P.row, &col = P.col;
-    // per row, its entries sorted by column, to find (j, n) given (n, j)
-    std::vector<std::pair<uint32_t, uint32_t>> byc(col.size());
-    for (int n = 0; n < N; n++) {
-        for (uint32_t k = row[n]; k < row[n + 1]; k++) byc[k] = {col[k], k};
-        std::sort(byc.begin() + row[n], byc.begin() + row[n + 1]);
-    }
-    auto find = [&](uint32_t j, uint32_t n) -> long {
-        auto b = byc.begin() + row[j], e = byc.begin() + row[j + 1];
-        auto it = std::lower_bound(b, e, std::make_pair(n, 0u));
-        return (it != e && it->first == n) ? (long)it->second : -1;
-    };
-    std::vector<long> partner(col.size());
-    std::vector<uint32_t> cnt(N, 0);
-    for (int n = 0; n < N; n++)
-        for (uint32_t k = row[n]; k < row[n + 1]; k++) {
-            const uint32_t j = col[k];
-            partner[k] = find(j, (uint32_t)n);
-            cnt[n]++;
-            if (partner[k] < 0) cnt[j]++;
-        }
     Csr S;
-    S.row.assign(N + 1, 0);
-    for (int n = 0; n < N; n++) S.row[n + 1] = S.row[n] + cnt[n];
-    S.col.resize(S.row[N]);
-    S.val.resize(S.row[N]);
-    std::vector<uint32_t> off(N, 0);
-    for (int n = 0; n < N; n++)
-        for (uint32_t k = row[n]; k < row[n + 1]; k++) {
-            const uint32_t j = col[k];
-            const long m = partner[k];
-            if (m >= 0 && (uint32_t)n > j) continue;  // the pair was placed from row j
-            const double v = m >= 0 ? P.val[k] + P.val[m] : P.val[k];
-            S.col[S.row[n] + off[n]] = j;
-            S.val[S.row[n] + off[n]] = v;
-            S.col[S.row[j] + off[j]] = (uint32_t)n;
-            S.val[S.row[j] + off[j]] = v;
-            off[n]++;
-            if (j != (uint32_t)n) off[j]++;
-        }
-    for (double &v : S.val) v /= 2.0;
+    const size_t cap = std::max<size_t>(2 * P.col.size(), 1);
+    S.row.resize((size_t)N + 1);
+    S.col.resize(cap);
+    S.val.resize(cap);
+    long long m = 0;
+    check(fitsne_symmetrize(P.row.data(), P.col.data(), P.val.data(), N, S.row.data(), S.col.data(), S.val.data(), &m),
+          "symmetrisation");
+    S.col.resize((size_t)m);
+    S.val.resize((size_t)m);
     return S;
 }
 

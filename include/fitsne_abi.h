@@ -125,6 +125,14 @@ int fitsne_distances2similarities_batch(const double *D, double *P, int n_points
  * K <= min(N - 1, 1000); X must be finite. */
 int fitsne_knn_exact(const double *X, int N, int D, int K, int32_t *idx, double *dist);
 
+/* symmetrizeMatrix (tsne.cpp:1730-1828) on the GPU: the same output rows, entry order and values
+ * ((P[n][c] + P[c][n]) / 2, or P[n][c] / 2 for an entry without partner).  Input: a CSR with N rows,
+ * row_P[N] < 2^30 entries, no repeated (row, col) entry.  sym_col_P / sym_val_P must hold 2 * row_P[N]
+ * entries; *sym_nnz (nullable) receives the output size sym_row_P[N].  The normalisation by the total
+ * (tsne.cpp:327-330) is the caller's, as in the reference. */
+int fitsne_symmetrize(const uint32_t *row_P, const uint32_t *col_P, const double *val_P, int N,
+                      uint32_t *sym_row_P, uint32_t *sym_col_P, double *sym_val_P, long long *sym_nnz);
+
 /* ====================================================================================== */
 /* Learned warm start of the perplexity calibration (no reference counterpart: the reference */
 /* starts every bisection of distances2similarities, tsne.cpp:1394-1469, at beta = 1).      */
