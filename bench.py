@@ -173,9 +173,10 @@ def bench_c4(args, fg, synth):
 
 
 def bench_exact(args, fg):
-    """Exact t-SNE (theta = 0) at N = 46,000 in 2D, the largest N whose N*N fits the reference's int
-    indexing (tsne.cpp:186, :1232-1330): dense P built on the device, then timed steps (exact gradient,
-    update, zeroMean).  Roofline of the gradient sweep: it reads the dense P once (8 N^2 bytes)."""
+    """Exact t-SNE (theta = 0) at N = 46,000 in 2D, about the largest N whose N*N fits the reference's
+    int arithmetic (tsne.cpp:194, :215-216, :1237-1245): dense P built on the device, then timed steps
+    (exact gradient, update, zeroMean).  Roofline of the gradient sweep: it reads the dense P once
+    (8 N^2 bytes)."""
     N = args.n if args.n != 1_000_000 else 46_000
     rng = np.random.default_rng(20261202)
     k = 20
