@@ -237,6 +237,13 @@ static int twiddles(fitsne_ctx *c, int n, const cx<T> **out) {
 
 static int blocks_for(long n, int per = kThreads) { return (int)((n + per - 1) / per); }
 
+// Number of bits that hold every value in [0, v].
+static int bits_for(unsigned long long v) {
+    int b = 1;
+    while (b < 64 && (1ull << b) <= v) b++;
+    return b;
+}
+
 // lanes per CSR row: each lane carries 4 entries per pass, so 32 lanes cover 128 entries at once
 static int sw_for(long long nnz, int N) {
     const double avg = N > 0 ? (double)nnz / N : 0;
@@ -1984,13 +1991,6 @@ extern "C" int fitsne_distances2similarities_batch(const double *D, double *P, i
     HIPTRY(hipMemcpyAsync(beta, c->xs.p, sizeof(double) * n_points, hipMemcpyDeviceToHost, c->stream));
     if (iters) HIPTRY(hipMemcpyAsync(iters, c->box.p, sizeof(int) * n_points, hipMemcpyDeviceToHost, c->stream));
     return sync_ctx(c);
-}
-
-// Number of bits that hold every value in [0, v].
-static int bits_for(unsigned long long v) {
-    int b = 1;
-    while (b < 64 && (1ull << b) <= v) b++;
-    return b;
 }
 
 extern "C" int fitsne_symmetrize(const uint32_t *row_P, const uint32_t *col_P, const double *val_P, int N,

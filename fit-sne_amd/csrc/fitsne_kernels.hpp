@@ -159,6 +159,14 @@ __device__ __forceinline__ long long fix_rn(double x) {
     return __double_as_longlong(x + M) - __double_as_longlong(M);
 }
 
+// Round to nearest (ties to even, as fix_rn) for any |x| < 2^63: fix_rn inside its range, the full
+// conversion outside it (points that the reference's min/max rule leaves outside the grid are
+// extrapolated and can carry large weights).
+__device__ __forceinline__ long long fix_rn_any(double x) {
+    if (fabs(x) < 2251799813685248.0) return fix_rn(x);  // 2^51
+    return (long long)rint(x);
+}
+
 // Per row block: entries per window, rounded up to a multiple of 4, into count[b * NW + w], and
 // max_i sum_j |p_ij| into bmax[b].
 __global__ __launch_bounds__(kAttThreads) void k_win_count(const int64_t *__restrict__ rowP,
@@ -574,16 +582,34 @@ __global__ __launch_bounds__(1024) void k_scan_counts(int *__restrict__ count, i
 // A workgroup takes kSpreadPts consecutive points (memory order; spatially ordered points keep this to
 // a few dozen boxes).  Each lane finds its point's box and computes the node-relative charges
 //   A = L,  c = L |e|^2,  b = L e    (e = y - node, in box widths)
-// for the p^D nodes of its box, each scaled by 2^38 and rounded to int64.  Runs of consecutive lanes in
-// one box are summed through LDS, each run is added into the workgroup's LDS table slot of its box
-// (open addressing, LDS atomics), and the table goes to the grid with one 64-bit global atomic per
-// (box, node, term).  A run whose box finds no free slot adds to the grid directly.  All sums are exact
-// integers, so the grid, and so the whole gradient, is bitwise repeatable whatever the order of the
-// points.  Grid layout: int64 [u][v][4] (A, c, b_x, b_y), u the x node (nbodyfft.cpp:137-138); the row
-// FFT reads it, converts it and clears it for the next iteration.
+// for the p^D nodes of its box.  Runs of consecutive lanes in one box are summed in FP32 by a segmented
+// scan in the VALU (DPP; a fixed order over fixed points), each run's sums are scaled by 2^38, rounded
+// to int64 and added into the workgroup's LDS table slot of its box (open addressing, LDS atomics), and
+// the table goes to the grid with one 64-bit global atomic per (box, node, term).  A run whose box finds
+// no free slot adds to the grid directly.  The sums across runs are exact integers, so the grid, and so
+// the whole gradient, is bitwise repeatable whatever the order in which workgroups and waves run.
+// Grid layout: int64 [u][v][4] (A, c, b_x, b_y), u the x node (nbodyfft.cpp:137-138); the row FFT
+// reads it, converts it and clears it for the next iteration.
 constexpr double kFixScale = 274877906944.0;  // 2^38
 constexpr int kSpreadThreads = 256;
 constexpr int kSpreadPts = 1024;
+
+// One step of a segmented inclusive scan over the 64 lanes: each lane adds the value that the DPP
+// control CTRL (row_shr:n, row_bcast:15 / :31 with row mask RMASK) brings from a lower lane, when that
+// lane is in the same run.  Lanes without a source get run id 0, which is never a real run.
+template <int CTRL, int RMASK, int NV, int NQ>
+__device__ __forceinline__ void seg_scan_step(float (&val)[NV][NQ], int run) {
+    const int src_run = __builtin_amdgcn_update_dpp(0, run, CTRL, RMASK, 0xf, false);
+    const bool take = src_run == run;
+#pragma unroll
+    for (int v = 0; v < NV; v++)
+#pragma unroll
+        for (int q = 0; q < NQ; q++) {
+            const float y = __int_as_float(
+                __builtin_amdgcn_update_dpp(0, __float_as_int(val[v][q]), CTRL, RMASK, 0xf, false));
+            val[v][q] += take ? y : 0.f;
+        }
+}
 
 template <int P, int D>
 constexpr int spread_slots() {
@@ -599,16 +625,11 @@ __global__ __launch_bounds__(kSpreadThreads) void k_spread_f32(const float *__re
     constexpr int NV = (D == 2 ? P * P : P);
     constexpr int NQ = (D == 2 ? 4 : 3);
     constexpr int NS = spread_slots<P, D>();
-    constexpr int WPB = kSpreadThreads / kWave;
     __shared__ int tkey[NS];
     __shared__ unsigned long long tval[NS][NV * NQ];
-    __shared__ long long sh[WPB][NV][kWave + 1];
-    __shared__ int hl[WPB][kWave + 1];
-    __shared__ int hb[WPB][kWave];
-    __shared__ int hs[WPB][kWave];
     if (!dg->live) return;
     const Geo &g = dg->g;
-    const int t = threadIdx.x, lane = t % kWave, wv = t / kWave;
+    const int t = threadIdx.x, lane = t % kWave;
     for (int o = t; o < NS; o += kSpreadThreads) tkey[o] = -1;
     for (int o = t; o < NS * NV * NQ; o += kSpreadThreads) (&tval[0][0])[o] = 0ull;
     __syncthreads();
@@ -616,13 +637,12 @@ __global__ __launch_bounds__(kSpreadThreads) void k_spread_f32(const float *__re
         const long i = (long)blockIdx.x * kSpreadPts + (long)chunk * kSpreadThreads + t;
         const bool ok = i < N;
         int b = -1 - lane;  // lanes past N: distinct negative keys, never a real run
-        long long val[NV][NQ];
+        float val[NV][NQ];
 #pragma unroll
         for (int v = 0; v < NV; v++)
 #pragma unroll
-            for (int q = 0; q < NQ; q++) val[v][q] = 0;
+            for (int q = 0; q < NQ; q++) val[v][q] = 0.f;
         if (ok) {
-            const float S = (float)kFixScale;
             float y[D];
             load_pt<float, D>(Y, i, y);
             double tx;
@@ -657,71 +677,65 @@ __global__ __launch_bounds__(kSpreadThreads) void k_spread_f32(const float *__re
 #pragma unroll
                     for (int c = 0; c < P; c++) {
                         const float w = Lx[a] * Ly[c];
-                        val[a * P + c][0] = (long long)(w * S);
-                        val[a * P + c][1] = (long long)(w * (dx[a] * dx[a] + dy[c] * dy[c]) * S);
-                        val[a * P + c][2] = (long long)(w * dx[a] * S);
-                        val[a * P + c][3] = (long long)(w * dy[c] * S);
+                        val[a * P + c][0] = w;
+                        val[a * P + c][1] = w * (dx[a] * dx[a] + dy[c] * dy[c]);
+                        val[a * P + c][2] = w * dx[a];
+                        val[a * P + c][3] = w * dy[c];
                     }
             } else {
 #pragma unroll
                 for (int a = 0; a < P; a++) {
-                    val[a][0] = (long long)(Lx[a] * S);
-                    val[a][1] = (long long)(Lx[a] * dx[a] * dx[a] * S);
-                    val[a][2] = (long long)(Lx[a] * dx[a] * S);
+                    val[a][0] = Lx[a];
+                    val[a][1] = Lx[a] * dx[a] * dx[a];
+                    val[a][2] = Lx[a] * dx[a];
                 }
             }
         }
-        // runs of equal boxes among consecutive lanes; each run head claims its box's table slot
+        // runs of equal boxes among consecutive lanes: a segmented inclusive scan over the wave (DPP row
+        // shifts and row broadcasts, in the VALU) leaves each run's sum in its last lane.  The wave's
+        // points and their order are fixed, so the float sums are bitwise repeatable.
         const int prev = __shfl_up(b, 1, kWave);
         const bool head = lane == 0 || b != prev;
         const unsigned long long heads = __ballot(head);
-        const int nruns = __popcll(heads);
-        if (head) {
-            const int r = __popcll(heads & ((1ull << lane) - 1ull));
-            hl[wv][r] = lane;
-            hb[wv][r] = b;
+        const int run = __popcll(heads & ((2ull << lane) - 1ull));  // 1-based run of this lane
+        const bool tail = lane == kWave - 1 || ((heads >> (lane + 1)) & 1ull);
+        seg_scan_step<0x111, 0xf, NV, NQ>(val, run);  // row_shr:1
+        seg_scan_step<0x112, 0xf, NV, NQ>(val, run);  // row_shr:2
+        seg_scan_step<0x114, 0xf, NV, NQ>(val, run);  // row_shr:4
+        seg_scan_step<0x118, 0xf, NV, NQ>(val, run);  // row_shr:8
+        seg_scan_step<0x142, 0xa, NV, NQ>(val, run);  // row_bcast:15 into rows 1 and 3
+        seg_scan_step<0x143, 0xc, NV, NQ>(val, run);  // row_bcast:31 into rows 2 and 3
+        // each run's last lane adds its totals, as 2^38 fixed point, into its box's table slot (open
+        // addressing), or into the grid when the table has no free slot
+        if (tail && b >= 0) {
             int slot = -1;
-            if (b >= 0) {
-                int h = b % NS;
-                for (int probe = 0; probe < NS; probe++) {
-                    const int k = atomicCAS(&tkey[h], -1, b);
-                    if (k == -1 || k == b) {
-                        slot = h;
-                        break;
+            int h = b % NS;
+            for (int probe = 0; probe < NS; probe++) {
+                const int k = atomicCAS(&tkey[h], -1, b);
+                if (k == -1 || k == b) {
+                    slot = h;
+                    break;
+                }
+                h = h + 1 == NS ? 0 : h + 1;
+            }
+            const int bx = D == 2 ? b % g.nb : b, by = D == 2 ? b / g.nb : 0;
+#pragma unroll
+            for (int v = 0; v < NV; v++)
+#pragma unroll
+                for (int q = 0; q < NQ; q++) {
+                    const long long sum = fix_rn_any((double)val[v][q] * kFixScale);
+                    if (sum == 0) continue;
+                    if (slot >= 0) {
+                        atomicAdd(&tval[slot][v * NQ + q], (unsigned long long)sum);
+                    } else {
+                        const long node =
+                            D == 2 ? (long)(bx * P + v / P) * g.n1d + by * P + v % P : (long)bx * P + v;
+                        atomicAdd(&grid[node * 4 + q], (unsigned long long)sum);
                     }
-                    h = h + 1 == NS ? 0 : h + 1;
                 }
-            }
-            hs[wv][r] = slot;
-        }
-        if (lane == 0) hl[wv][nruns] = kWave;
-        // one charge term per round keeps the LDS footprint small
-#pragma unroll
-        for (int q = 0; q < NQ; q++) {
-#pragma unroll
-            for (int v = 0; v < NV; v++) sh[wv][v][lane] = val[v][q];
-            __syncthreads();
-            for (int o = lane; o < nruns * NV; o += kWave) {
-                const int r = o / NV, v = o - r * NV;
-                const int box = hb[wv][r];
-                if (box < 0) continue;  // run of lanes past N
-                const int l0 = hl[wv][r], l1 = hl[wv][r + 1];
-                long long sum = 0;
-                for (int l = l0; l < l1; l++) sum += sh[wv][v][l];
-                if (sum == 0) continue;
-                const int slot = hs[wv][r];
-                if (slot >= 0) {
-                    atomicAdd(&tval[slot][v * NQ + q], (unsigned long long)sum);
-                } else {
-                    const int bx = D == 2 ? box % g.nb : box, by = D == 2 ? box / g.nb : 0;
-                    const long node =
-                        D == 2 ? (long)(bx * P + v / P) * g.n1d + by * P + v % P : (long)bx * P + v;
-                    atomicAdd(&grid[node * 4 + q], (unsigned long long)sum);
-                }
-            }
-            __syncthreads();
         }
     }
+    __syncthreads();
     // table -> grid
     for (int o = t; o < NS * NV * NQ; o += kSpreadThreads) {
         const int slot = o / (NV * NQ), r = o - slot * (NV * NQ);
