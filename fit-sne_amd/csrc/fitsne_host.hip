@@ -101,6 +101,10 @@ struct fitsne_ctx {
     int D = 2, p = 3, mni = 50;
     double ipi = 1.0, df = 1.0;
     hipStream_t stream = nullptr;
+    // second stream of the host-planned FP32 step: the attractive term runs on it beside the grid stages
+    hipStream_t astream = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+    bool join_pending = false;
 
     int N = 0;
     long long nnz = 0;
@@ -215,6 +219,7 @@ static int flush_timing(fitsne_ctx *c) {
 static int sync_ctx(fitsne_ctx *c) {
     HIPTRY(hipGetLastError());
     HIPTRY(hipStreamSynchronize(c->stream));
+    if (c->astream) HIPTRY(hipStreamSynchronize(c->astream));
     return flush_timing(c);
 }
 
@@ -544,6 +549,10 @@ static int launch_f32_stages(fitsne_ctx *c, const cx<float> *tw, const StageDims
                c->zbuf.as<double>());
     }
     const int nblk = blocks_for(N);
+    if (c->join_pending) {
+        HIPTRY(hipStreamWaitEvent(c->stream, c->join_ev, 0));
+        c->join_pending = false;
+    }
     launch(c, "interp_f32", pick_interp_f32<D>(p), dim3(nblk), dim3(kThreads), 0, c->Y.as<float>(),
            c->pos.as<const float>(), c->gout.as<const float4>(), dg, c->zbuf.as<const double>(),
            c->dstep.as<const StepDev>(), c->dC.as<float>(), c->uY.as<float>(), c->gains.as<float>(),
@@ -607,7 +616,19 @@ static int pipeline_f32(fitsne_ctx *c, int mode, const fitsne_step_params *sp) {
     TRY(put_step(c, mode, sp));
     const DevGeo *live;
     TRY(live_record(c, &live));
-    TRY((launch_attractive<float, D>(c, (float)c->exag, c->pos.as<float>(), live)));
+    // the attractive term on the second stream (after everything queued so far), beside the grid
+    // stages; the interpolation waits for it
+    HIPTRY(hipEventRecord(c->fork_ev, c->stream));
+    HIPTRY(hipStreamWaitEvent(c->astream, c->fork_ev, 0));
+    {
+        hipStream_t main = c->stream;
+        c->stream = c->astream;
+        const int rc = launch_attractive<float, D>(c, (float)c->exag, c->pos.as<float>(), live);
+        c->stream = main;
+        TRY(rc);
+    }
+    HIPTRY(hipEventRecord(c->join_ev, c->astream));
+    c->join_pending = true;
     double mn, mx;
     TRY(current_minmax<float>(c, &mn, &mx));
     Geo g;
@@ -952,6 +973,9 @@ extern "C" int fitsne_ctx_create(fitsne_ctx **out, int device, int precision, in
     c->mni = min_num_intervals;
     c->df = df;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->astream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming) != hipSuccess ||
         hipHostMalloc((void **)&c->h_mm, sizeof(double) * 4, hipHostMallocDefault) != hipSuccess ||
         hipEventCreateWithFlags(&c->mm_event, hipEventDisableTiming) != hipSuccess) {
         delete c;
@@ -965,6 +989,7 @@ extern "C" int fitsne_ctx_destroy(fitsne_ctx *c) {
     if (!c) return FITSNE_OK;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->astream) (void)hipStreamSynchronize(c->astream);
     drop_graph(c);
     DBuf *all[] = {&c->rowP, &c->colP, &c->val, &c->Y, &c->uY, &c->gains, &c->pos, &c->dC, &c->wcount, &c->wstart,
                    &c->wkey, &c->wval, &c->wbmax, &c->knnX, &c->knnXT, &c->knnI, &c->knnD, &c->dgeo,
@@ -984,6 +1009,9 @@ extern "C" int fitsne_ctx_destroy(fitsne_ctx *c) {
         (void)hipEventDestroy(p.b);
     }
     if (c->mm_event) (void)hipEventDestroy(c->mm_event);
+    if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
+    if (c->join_ev) (void)hipEventDestroy(c->join_ev);
+    if (c->astream) (void)hipStreamDestroy(c->astream);
     if (c->h_mm) (void)hipHostFree(c->h_mm);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
