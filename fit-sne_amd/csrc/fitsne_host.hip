@@ -15,6 +15,7 @@
 #include "fitsne_kernels.hpp"
 #include "fitsne_exact.hpp"
 #include "fitsne_sym.hpp"
+#include "fitsne_bh.hpp"
 #include "fitsne_knn.hpp"
 #include "fitsne_mlp.hpp"
 
@@ -137,6 +138,12 @@ struct fitsne_ctx {
     DBuf xs, ys, box, wx, wy, ch, w, V, potA, potB;
     // exact path (dense P in val): per-row attractive / repulsive sums and Z shares
     DBuf att, rep, zrow;
+    // Barnes-Hut: the host tree of the current Y and its device copy (pre-order), theta
+    double theta = 0.5;
+    BhTree bh_tree;
+    BhFlat bh_flat;
+    std::vector<double> bh_y;
+    DBuf bh_com, bh_maxw, bh_cum, bh_leafpt, bh_isleaf, bh_skip, bh_neg, bh_Q;
     // symmetrisation of a CSR P (mirror tier)
     DBuf sy_row, sy_col, sy_val, sy_erow, sy_keys, sy_keys2, sy_vals, sy_vals2, sy_partner, sy_cnt, sy_start,
         sy_okeys, sy_okeys2, sy_tmp, sy_scol, sy_sval;
@@ -900,10 +907,115 @@ static int kl_exact(fitsne_ctx *c) {
     }
 }
 
+// ============================================================================ Barnes-Hut (nbody_algo = 1)
+
+// The SPTree of the resident Y, built on the host in insertion order and uploaded in pre-order.
+static int bh_tree_of_Y(fitsne_ctx *c) {
+    const int N = c->N, D = c->D;
+    c->bh_y.resize((size_t)N * D);
+    HIPTRY(hipMemcpyAsync(c->bh_y.data(), c->Y.p, sizeof(double) * (size_t)N * D, hipMemcpyDeviceToHost, c->stream));
+    TRY(sync_ctx(c));
+    c->bh_tree.build(c->bh_y.data(), N, D);
+    c->bh_flat.flatten(c->bh_tree);
+    const BhFlat &f = c->bh_flat;
+    const size_t n = f.cum.size();
+    TRY(ensure(c->bh_com, sizeof(double) * n * D));
+    TRY(ensure(c->bh_maxw, sizeof(double) * n));
+    TRY(ensure(c->bh_cum, sizeof(unsigned) * n));
+    TRY(ensure(c->bh_leafpt, sizeof(int) * n));
+    TRY(ensure(c->bh_isleaf, sizeof(int) * n));
+    TRY(ensure(c->bh_skip, sizeof(int) * n));
+    HIPTRY(hipMemcpyAsync(c->bh_com.p, f.com.data(), sizeof(double) * n * D, hipMemcpyHostToDevice, c->stream));
+    HIPTRY(hipMemcpyAsync(c->bh_maxw.p, f.maxw.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    HIPTRY(hipMemcpyAsync(c->bh_cum.p, f.cum.data(), sizeof(unsigned) * n, hipMemcpyHostToDevice, c->stream));
+    HIPTRY(hipMemcpyAsync(c->bh_leafpt.p, f.leafpt.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
+    HIPTRY(hipMemcpyAsync(c->bh_isleaf.p, f.isleaf.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
+    HIPTRY(hipMemcpyAsync(c->bh_skip.p, f.skip.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
+    // the host vectors stay alive until the next build; the copies are ordered before the walk
+    return FITSNE_OK;
+}
+
+template <int D>
+static int bh_walk(fitsne_ctx *c, bool forces) {
+    const int N = c->N;
+    TRY(ensure(c->bh_Q, sizeof(double) * (size_t)N));
+    if (forces) TRY(ensure(c->bh_neg, sizeof(double) * (size_t)N * D));
+    launch(c, "bh_nonedge", k_bh_nonedge<D>, dim3(blocks_for(N)), dim3(kThreads), 0, c->Y.as<const double>(), N,
+           c->bh_com.as<const double>(), c->bh_maxw.as<const double>(), c->bh_cum.as<const unsigned>(),
+           c->bh_leafpt.as<const int>(), c->bh_isleaf.as<const int>(), c->bh_skip.as<const int>(), c->theta,
+           forces ? c->bh_neg.as<double>() : (double *)nullptr, c->bh_Q.as<double>());
+    return FITSNE_OK;
+}
+
+// computeGradient (tsne.cpp:596-640) at the resident Y; mode 1 adds the update and zeroMean.
+template <int D>
+static int pipeline_bh(fitsne_ctx *c, int mode, const fitsne_step_params *sp) {
+    const int N = c->N;
+    const long nd = (long)N * D;
+    TRY(bh_tree_of_Y(c));
+    TRY(ensure(c->pos, sizeof(double) * (size_t)nd));
+    TRY(ensure(c->dC, sizeof(double) * (size_t)nd));
+    TRY(ensure(c->zbuf, sizeof(double) * 4));
+    launch(c, "bh_edge", k_bh_edge<D>, dim3(blocks_for(N)), dim3(kThreads), 0, c->rowP.as<const int64_t>(),
+           c->colP.as<const uint32_t>(), c->val.as<const double>(), c->Y.as<const double>(), N, c->pos.as<double>());
+    TRY(bh_walk<D>(c, true));
+    launch(c, "bh_sumq", k_seq_sum, dim3(1), dim3(kWave), 0, c->bh_Q.as<const double>(), N, c->zbuf.as<double>());
+    launch(c, "bh_combine", k_bh_combine, dim3(std::min(blocks_for(nd), 4096)), dim3(kThreads), 0,
+           c->pos.as<const double>(), c->bh_neg.as<const double>(), c->zbuf.as<const double>(), nd, c->dC.as<double>());
+    if (mode == 1) {
+        const DevGeo *live;
+        TRY(live_record(c, &live));
+        const int nblk = blocks_for(N);
+        TRY(ensure(c->part, sizeof(double) * (size_t)nblk * D));
+        TRY(ensure(c->mean, sizeof(double) * kExMaxDims));
+        const int cblk = std::min(blocks_for(nd), 1024);
+        TRY(ensure(c->mmpart, sizeof(double) * 2 * cblk));
+        launch(c, "update64", k_update_f64, dim3(nblk), dim3(kThreads), 0, c->Y.as<double>(), c->uY.as<double>(),
+               c->gains.as<double>(), c->dC.as<const double>(), N, D, sp->momentum, sp->learning_rate,
+               sp->max_step_norm, sp->plain_descent, c->part.as<double>());
+        launch(c, "mean_final", k_mean_final, dim3(1), dim3(kThreads), 0, live, c->part.as<const double>(), nblk, D,
+               (double)N, c->mean.as<double>());
+        launch(c, "center64", k_center<double>, dim3(cblk), dim3(kThreads), 0, live, c->Y.as<double>(), nd, D, -1L,
+               c->mean.as<const double>(), c->mmpart.as<double>());
+    }
+    c->have_grad = true;
+    HIPTRY(hipGetLastError());
+    return FITSNE_OK;
+}
+
+// evaluateError, Barnes-Hut form (tsne.cpp:1359-1390): Z of the current Y from the tree (summed over the
+// points in index order; the reference keeps one running sum across the points), then the KL over the
+// edges with q = 1 / (1 + d^2) / Z.  The result goes to c->mean[0].
+template <int D>
+static int kl_bh_d(fitsne_ctx *c) {
+    const int N = c->N;
+    TRY(bh_tree_of_Y(c));
+    TRY(bh_walk<D>(c, false));
+    TRY(ensure(c->mean, sizeof(double) * kExMaxDims));
+    launch(c, "bh_sumq", k_seq_sum, dim3(1), dim3(kWave), 0, c->bh_Q.as<const double>(), N, c->mean.as<double>() + 2);
+    const long threads = (long)N * 32;
+    const int nblk = blocks_for(threads);
+    TRY(ensure(c->klpart, sizeof(double) * (size_t)std::max(nblk, 2)));
+    launch(c, "kl", k_kl<double, int64_t, uint32_t, D, 32>, dim3(nblk), dim3(kThreads), 0, c->rowP.as<const int64_t>(),
+           c->colP.as<const uint32_t>(), c->val.as<const double>(), 1.0, c->Y.as<const double>(), N, 1.0,
+           c->mean.as<const double>() + 2, c->klpart.as<double>());
+    launch(c, "kl_sum", k_sum_final, dim3(1), dim3(kThreads), 0, c->klpart.as<const double>(), nblk, 0.0,
+           c->mean.as<double>());
+    HIPTRY(hipGetLastError());
+    return FITSNE_OK;
+}
+
 static int run_pipeline(fitsne_ctx *c, int mode, const fitsne_step_params *sp) {
     if (c->N <= 0) return fail(FITSNE_E_ARG, "P and Y must be set before computing a gradient");
     HIPTRY(hipSetDevice(c->device));
     if (c->precision == FITSNE_FP32) return c->D == 2 ? pipeline_f32<2>(c, mode, sp) : pipeline_f32<1>(c, mode, sp);
+    if (c->precision == FITSNE_FP64_BH) {
+        switch (c->D) {
+        case 1: return pipeline_bh<1>(c, mode, sp);
+        case 2: return pipeline_bh<2>(c, mode, sp);
+        default: return pipeline_bh<3>(c, mode, sp);
+        }
+    }
     if (c->precision == FITSNE_FP64_EXACT) {
         switch (c->D) {
         case 1: return pipeline_exact<1>(c, mode, sp);
@@ -950,9 +1062,14 @@ extern "C" int fitsne_ctx_create(fitsne_ctx **out, int device, int precision, in
                                  double intervals_per_integer, int min_num_intervals, double df) {
     if (!out) return fail(FITSNE_E_ARG, "ctx out-pointer is null");
     *out = nullptr;
-    if (precision != FITSNE_FP32 && precision != FITSNE_FP64_PLAIN && precision != FITSNE_FP64_EXACT)
+    if (precision != FITSNE_FP32 && precision != FITSNE_FP64_PLAIN && precision != FITSNE_FP64_EXACT &&
+        precision != FITSNE_FP64_BH)
         return fail(FITSNE_E_ARG, "unknown precision");
-    if (precision == FITSNE_FP64_EXACT) {
+    if (precision == FITSNE_FP64_BH) {
+        if (no_dims < 1) return fail(FITSNE_E_ARG, "no_dims must be positive");
+        if (no_dims > kBhMaxDims)
+            return fail(FITSNE_E_UNSUPPORTED, "the Barnes-Hut path supports no_dims <= 3 in this build");
+    } else if (precision == FITSNE_FP64_EXACT) {
         if (no_dims < 1) return fail(FITSNE_E_ARG, "no_dims must be positive");
         if (no_dims > kExMaxDims) return fail(FITSNE_E_UNSUPPORTED, "the exact path supports no_dims <= 4 in this build");
     } else if (no_dims != 1 && no_dims != 2) {
@@ -999,7 +1116,8 @@ extern "C" int fitsne_ctx_destroy(fitsne_ctx *c) {
                    &c->inv_d, &c->colP2, &c->val2, &c->tmp, &c->fixgrid, &c->att, &c->rep, &c->zrow,
                    &c->sy_row, &c->sy_col, &c->sy_val, &c->sy_erow, &c->sy_keys, &c->sy_keys2, &c->sy_vals,
                    &c->sy_vals2, &c->sy_partner, &c->sy_cnt, &c->sy_start, &c->sy_okeys, &c->sy_okeys2, &c->sy_tmp,
-                   &c->sy_scol, &c->sy_sval};
+                   &c->sy_scol, &c->sy_sval, &c->bh_com, &c->bh_maxw, &c->bh_cum, &c->bh_leafpt, &c->bh_isleaf,
+                   &c->bh_skip, &c->bh_neg, &c->bh_Q};
     for (DBuf *b : all) release(*b);
     for (auto &kv : c->tw32) release(kv.second);
     for (auto &kv : c->tw64) release(kv.second);
@@ -1089,7 +1207,7 @@ extern "C" int fitsne_set_exaggeration(fitsne_ctx *c, double factor) {
     if (!c || !(factor > 0)) return fail(FITSNE_E_ARG, "exaggeration must be positive");
     if (factor == c->exag) return FITSNE_OK;
     HIPTRY(hipSetDevice(c->device));
-    if ((c->precision == FITSNE_FP64_PLAIN || c->precision == FITSNE_FP64_EXACT) && c->nnz > 0)
+    if (c->precision != FITSNE_FP32 && c->nnz > 0)
         launch(c, "scale_val", k_scale_val, dim3(std::min(blocks_for(c->nnz), 4096)), dim3(kThreads), 0,
                c->val.as<double>(), (long)c->nnz, factor, c->exag);
     c->exag = factor;
@@ -1293,6 +1411,14 @@ extern "C" int fitsne_reorder(fitsne_ctx *c) {
     if (c->N <= 0 || !c->Y.p) return fail(FITSNE_E_ARG, "set P and Y before reordering");
     HIPTRY(hipSetDevice(c->device));
     return reorder_points(c);
+}
+
+extern "C" int fitsne_set_theta(fitsne_ctx *c, double theta) {
+    if (!c) return fail(FITSNE_E_ARG, "null context");
+    if (c->precision != FITSNE_FP64_BH) return fail(FITSNE_E_ARG, "theta is a parameter of the Barnes-Hut context");
+    if (!(theta > 0)) return fail(FITSNE_E_ARG, "theta must be positive (theta = 0 is the exact context)");
+    c->theta = theta;
+    return FITSNE_OK;
 }
 
 extern "C" int fitsne_set_reorder(fitsne_ctx *c, int enable) {
@@ -1542,10 +1668,11 @@ extern "C" int fitsne_batch_step(fitsne_batch *b, const fitsne_step_params *prm)
 extern "C" int fitsne_kl(fitsne_ctx *c, double *C) {
     if (!c || !C) return fail(FITSNE_E_ARG, "null argument");
     TRY(check_graph(c));
-    if (c->precision == FITSNE_FP64_EXACT) {
+    if (c->precision == FITSNE_FP64_EXACT || c->precision == FITSNE_FP64_BH) {
         if (c->N <= 0) return fail(FITSNE_E_ARG, "P and Y must be set before the KL divergence");
         HIPTRY(hipSetDevice(c->device));
-        TRY(kl_exact(c));
+        if (c->precision == FITSNE_FP64_EXACT) TRY(kl_exact(c));
+        else TRY(c->D == 1 ? kl_bh_d<1>(c) : c->D == 2 ? kl_bh_d<2>(c) : kl_bh_d<3>(c));
         HIPTRY(hipMemcpyAsync(c->h_mm + 3, c->mean.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
         TRY(sync_ctx(c));
         *C = c->h_mm[3];

@@ -49,7 +49,8 @@ __device__ __forceinline__ void load_pt(const TV *__restrict__ Y, long i, TV (&y
         y[0] = v.x;
         y[1] = v.y;
     } else {
-        y[0] = Y[i];
+#pragma unroll
+        for (int d = 0; d < D; d++) y[d] = Y[i * D + d];
     }
 }
 

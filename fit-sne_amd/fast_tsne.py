@@ -108,11 +108,10 @@ def fast_tsne(X, theta=0.5, perplexity=30, map_dims=2, max_iter=750, stop_early_
               start_late_exag_iter="auto", late_exag_coeff=-1, nterms=3, intervals_per_integer=1,
               min_num_intervals=50, seed=-1, initialization="pca", load_affinities=None, perplexity_list=None, df=1,
               return_loss=False, nthreads=-1, max_step_norm=5):
-    """Run t-SNE on the GPU: FFT-accelerated interpolation, or exact t-SNE with theta = 0.  Parameters and
-    return value as the reference's fast_tsne (its docstring describes each option); the Barnes-Hut
-    approximation (nbody_algo = 'Barnes-Hut' with theta != 0) is not part of this build and makes the
-    program exit with an error.  As in the reference, an exact run with map_dims > 2 needs
-    nbody_algo = 'Barnes-Hut' (the FFT setting limits map_dims to 1 or 2 before theta is looked at)."""
+    """Run t-SNE on the GPU: FFT-accelerated interpolation (nbody_algo = 'FFT'), the Barnes-Hut
+    approximation (nbody_algo = 'Barnes-Hut'), or exact t-SNE with theta = 0.  Parameters and return
+    value as the reference's fast_tsne (its docstring describes each option).  As in the reference, a run
+    with map_dims > 2 needs nbody_algo = 'Barnes-Hut' (the FFT setting limits map_dims to 1 or 2)."""
     X = np.array(X).astype(float)
     r = _resolve(X, theta, perplexity, map_dims, early_exag_coeff, learning_rate, start_late_exag_iter,
                  stop_early_exag_iter, late_exag_coeff, initialization, seed, perplexity_list, sigma, K, search_k,

@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libfitsne.so")
 
 OK, E_ARG, E_ALLOC, E_HIP, E_UNSUPPORTED, E_NODEVICE = 0, -1, -2, -3, -4, -5
 KERNEL_SQUARED_CAUCHY, KERNEL_GENERAL, KERNEL_SQUARED_GENERAL = 0, 1, 2
-FP64_PLAIN, FP32, FP64_EXACT = 0, 1, 2
+FP64_PLAIN, FP32, FP64_EXACT, FP64_BH = 0, 1, 2, 3
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _u32 = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
@@ -67,6 +67,7 @@ SIGNATURES = {
     "fitsne_set_P_i64": (_ci, [_vp, _ci, _i64, _i32, _dp]),
     "fitsne_exact_affinities": (_ci, [_vp, _dp, _ci, _ci, _cd, _cd, _ci, _vp, _vp, C.POINTER(_cd)]),
     "fitsne_set_P_dense": (_ci, [_vp, _ci, _dp]),
+    "fitsne_set_theta": (_ci, [_vp, _cd]),
     "fitsne_set_exaggeration": (_ci, [_vp, _cd]),
     "fitsne_set_reorder": (_ci, [_vp, _ci]),
     "fitsne_reorder": (_ci, [_vp]),
@@ -388,6 +389,10 @@ class Context:
         P = np.ascontiguousarray(P, dtype=np.float64)
         check(lib().fitsne_set_P_dense(self.h, P.shape[0], P.reshape(-1)))
         self.N = P.shape[0]
+
+    def set_theta(self, theta):
+        """Barnes-Hut opening parameter (FP64_BH contexts)."""
+        check(lib().fitsne_set_theta(self.h, theta))
 
     def set_exaggeration(self, f):
         check(lib().fitsne_set_exaggeration(self.h, f))

@@ -8,12 +8,12 @@
 // perplexity calibration and the symmetrisation of P run on the GPU through libfitsne's C ABI; the host
 // keeps the sums whose order defines the result: data centring and scaling, and the normalisation of P.
 //
-// Supported: nbody_algo = 2 (FFT interpolation) with theta != 0, no_dims 1 or 2, and the exact mode
-// (theta = 0, any nbody_algo, no_dims 1..4) with its dense P built on the GPU.  knn_algo 1 (Annoy) and
-// 2 (VP tree) both use the exact neighbour search (Annoy's approximate search is not reproduced).  The
-// Barnes-Hut path (nbody_algo = 1 with theta != 0) is not part of this build and is rejected with a
-// message.  FITSNE_PRECISION=fp64 in the environment selects the FP64 plain GPU path of the FFT mode
-// (default: the FP32 form); the exact mode is FP64.
+// Supported: nbody_algo = 2 (FFT interpolation) with theta != 0, no_dims 1 or 2; nbody_algo = 1
+// (Barnes-Hut) with theta != 0, no_dims 1..3; and the exact mode (theta = 0, any nbody_algo, no_dims
+// 1..4) with its dense P built on the GPU.  knn_algo 1 (Annoy) and 2 (VP tree) both use the exact
+// neighbour search (Annoy's approximate search is not reproduced).  FITSNE_PRECISION=fp64 in the
+// environment selects the FP64 plain GPU path of the FFT mode (default: the FP32 form); the exact and
+// Barnes-Hut modes are FP64.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -343,16 +343,18 @@ int main(int argc, char **argv) {
     const int N = p.n, nd = p.no_dims;
     if (N - 1 < 3 * p.perplexity) die("Perplexity too large for the number of data points!");
     const bool exact = p.theta == 0.0;
-    if (!exact && p.nbody_algo != 2)
-        die("the Barnes-Hut approximation (nbody_algo = 1) is not part of this build; use nbody_algo = 2 or theta = 0");
+    const bool bh = !exact && p.nbody_algo == 1;
+    if (!exact && p.nbody_algo != 1 && p.nbody_algo != 2) die("Error: Undefined algorithm");
     if (!exact && p.knn_algo != 1 && p.knn_algo != 2) die("Invalid knn_algo param");
     if (exact && nd > 4) die("the exact mode of this build supports no_dims <= 4");
+    if (bh && nd > 3) die("the Barnes-Hut mode of this build supports no_dims <= 3");
     if (fitsne_device_count() < 1) die("no GPU device is visible");
 
     fitsne_ctx *ctx = nullptr;
     double max_row_sum = 0.0;
     const char *prec = std::getenv("FITSNE_PRECISION");
     const int precision = exact ? FITSNE_FP64_EXACT
+                          : bh ? FITSNE_FP64_BH
                           : (prec && std::strcmp(prec, "fp64") == 0) ? FITSNE_FP64_PLAIN : FITSNE_FP32;
     if (exact) {
         check(fitsne_ctx_create(&ctx, 0, precision, nd, p.nterms, p.intervals_per_integer, p.min_num_intervals, p.df),
@@ -371,6 +373,10 @@ int main(int argc, char **argv) {
         check(fitsne_ctx_create(&ctx, 0, precision, nd, p.nterms, p.intervals_per_integer, p.min_num_intervals, p.df),
               "context");
         check(fitsne_set_P(ctx, N, P.row.data(), P.col.data(), P.val.data()), "upload P");
+        if (bh) {
+            std::printf("Using the Barnes-Hut approximation.\n");
+            check(fitsne_set_theta(ctx, p.theta), "theta");
+        }
     }
 
     if (!have_init) {
@@ -396,8 +402,9 @@ int main(int argc, char **argv) {
     double exag = early;
     check(fitsne_set_exaggeration(ctx, exag), "exaggeration");
     check(fitsne_set_Y(ctx, Y.data()), "upload Y");
-    std::printf("Learning embedding (%s)...\n", precision == FITSNE_FP32      ? "FP32 form"
+    std::printf("Learning embedding (%s)...\n", precision == FITSNE_FP32        ? "FP32 form"
                                                  : precision == FITSNE_FP64_EXACT ? "exact, FP64"
+                                                 : precision == FITSNE_FP64_BH    ? "Barnes-Hut, FP64"
                                                                                   : "FP64 plain");
 
     std::vector<double> costs((size_t)p.max_iter, 0.0);

@@ -43,6 +43,7 @@ extern "C" {
 #define FITSNE_FP64_PLAIN 0 /* FP64 throughout, stages as in the reference               */
 #define FITSNE_FP32 1       /* FP32 grid/FFT/interpolation, FP64 Z/geometry (product form) */
 #define FITSNE_FP64_EXACT 2 /* exact t-SNE (theta = 0): dense P, O(N^2) gradient and KL, FP64 */
+#define FITSNE_FP64_BH 3    /* Barnes-Hut (nbody_algo = 1): the reference's SPTree, FP64       */
 
 /* Text of the last error on this thread (never NULL). */
 const char *fitsne_last_error(void);
@@ -176,7 +177,7 @@ typedef struct {
 } fitsne_step_params;
 
 /* Create a context on `device` for an embedding of dimension no_dims (1 or 2; 1..4 for
- * FITSNE_FP64_EXACT) with the FFT settings of the run (n_interpolation_points = the wrapper's nterms,
+ * FITSNE_FP64_EXACT, 1..3 for FITSNE_FP64_BH) with the FFT settings of the run (n_interpolation_points = the wrapper's nterms,
  * intervals_per_integer, min_num_intervals; unused by the exact path) and the kernel's degree of
  * freedom df. */
 int fitsne_ctx_create(fitsne_ctx **ctx, int device, int precision, int no_dims, int n_interpolation_points,
@@ -203,8 +204,17 @@ int fitsne_exact_affinities(fitsne_ctx *ctx, const double *X, int N, int D, doub
                             double *sum_P);
 int fitsne_set_P_dense(fitsne_ctx *ctx, int N, const double *P);
 
-/* Scalar multiplier on P (early/late exaggeration, tsne.cpp:389-412, :534-543).  The FP64 plain and
- * exact contexts rescale their copy of P in place like the reference; the FP32 context keeps a scalar. */
+/* Barnes-Hut (nbody_algo = 1, theta > 0), FITSNE_FP64_BH contexts only; no_dims 1..3; CSR P as
+ * fitsne_set_P.  fitsne_gradient / fitsne_step compute computeGradient (tsne.cpp:596-640): the SPTree of
+ * sptree.cpp built each iteration on the host in the reference's insertion order, its walk
+ * (computeNonEdgeForces) and the edge forces on the GPU, bitwise the reference's result.  fitsne_kl
+ * computes the Barnes-Hut evaluateError (tsne.cpp:1359-1390) at the current Y.  The kernel is the
+ * Cauchy one whatever df (the reference's Barnes-Hut path has no df).  Default theta 0.5. */
+int fitsne_set_theta(fitsne_ctx *ctx, double theta);
+
+/* Scalar multiplier on P (early/late exaggeration, tsne.cpp:389-412, :534-543).  The FP64 contexts
+ * (plain, exact, Barnes-Hut) rescale their copy of P in place like the reference; the FP32 context keeps
+ * a scalar. */
 int fitsne_set_exaggeration(fitsne_ctx *ctx, double factor);
 
 /* Spatial order (FP32 context): points are kept internally in the Morton order of their cell on a

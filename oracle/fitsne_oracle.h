@@ -30,6 +30,8 @@
  *   orc_exact_gradient        reference computeExactGradient (oracle/_ref), bit-exact; finite differences
  *                             of the exact KL
  *   orc_evaluate_error_exact  reference evaluateError (oracle/_ref), bit-exact
+ *   orc_bh_gradient / orc_bh_error  reference computeGradient (SPTree) / evaluateError (oracle/_ref),
+ *                             bit-exact; the exact O(N^2) gradient as theta -> 0
  *   orc_update / orc_run_fft  elementwise closed forms of tsne.cpp:478-544 (parity pinned by formula
  *                             tests; the reference's TSNE::run cannot be linked without FFTW)
  */
@@ -165,6 +167,12 @@ void orc_gaussian_perplexity_dense(const double *X, int N, int D, double perplex
 double orc_exact_symmetrize_normalize(double *P, int N);
 void orc_exact_gradient(const double *P, const double *Y, int N, int D, double df, double *dC, double *sum_Q);
 double orc_evaluate_error_exact(const double *P, const double *Y, int N, int D, double df);
+
+/* ---- Barnes-Hut (nbody_algo = 1; sptree.cpp, tsne.cpp:596-640, :1359-1390), D <= 3 ---- */
+void orc_bh_gradient(const uint32_t *row_P, const uint32_t *col_P, const double *val_P, const double *Y, int N,
+                     int D, double theta, double *dC, double *sum_Q);
+double orc_bh_error(const uint32_t *row_P, const uint32_t *col_P, const double *val_P, const double *Y, int N,
+                    int D, double theta);
 
 #ifdef __cplusplus
 }

@@ -778,3 +778,219 @@ double orc_evaluate_error_exact(const double *P, const double *Y, int N, int D, 
     free(Q);
     return C;
 }
+
+/* ---- Barnes-Hut (nbody_algo = 1): SPTree (sptree.cpp) and computeGradient / evaluateError ---- */
+
+/* A flattened SPTree: node k has center c[k*D+d] and half-width w[k*D+d] (the reference's Cell
+ * "corner" and "width"), the online center of mass m[k*D+d], cum_size, at most one stored point, and
+ * 2^D children allocated together at first_child[k] (or -1). */
+typedef struct {
+    int D, nc;
+    long n, cap;
+    double *c, *w, *m;
+    unsigned *cum, *size, *pt;
+    int *leaf;
+    long *first_child;
+    const double *Y;
+} bh_tree;
+
+static long bh_new_node(bh_tree *t, const double *center, const double *hw) {
+    if (t->n == t->cap) {
+        t->cap = t->cap ? 2 * t->cap : 1024;
+        t->c = (double *)realloc(t->c, sizeof(double) * t->cap * t->D);
+        t->w = (double *)realloc(t->w, sizeof(double) * t->cap * t->D);
+        t->m = (double *)realloc(t->m, sizeof(double) * t->cap * t->D);
+        t->cum = (unsigned *)realloc(t->cum, sizeof(unsigned) * t->cap);
+        t->size = (unsigned *)realloc(t->size, sizeof(unsigned) * t->cap);
+        t->pt = (unsigned *)realloc(t->pt, sizeof(unsigned) * t->cap);
+        t->leaf = (int *)realloc(t->leaf, sizeof(int) * t->cap);
+        t->first_child = (long *)realloc(t->first_child, sizeof(long) * t->cap);
+    }
+    long k = t->n++;
+    for (int d = 0; d < t->D; d++) {
+        t->c[k * t->D + d] = center[d];
+        t->w[k * t->D + d] = hw[d];
+        t->m[k * t->D + d] = .0;
+    }
+    t->cum[k] = 0;
+    t->size[k] = 0;
+    t->pt[k] = 0;
+    t->leaf[k] = 1;
+    t->first_child[k] = -1;
+    return k;
+}
+
+static int bh_insert(bh_tree *t, long k, unsigned i);
+
+/* sptree.cpp:235-262 */
+static void bh_subdivide(bh_tree *t, long k) {
+    const int D = t->D;
+    double nc[8], nw[8];
+    long first = -1;
+    for (int ch = 0; ch < t->nc; ch++) {
+        unsigned div = 1;
+        for (int d = 0; d < D; d++) {
+            nw[d] = .5 * t->w[k * D + d];
+            if ((ch / div) % 2 == 1) nc[d] = t->c[k * D + d] - .5 * t->w[k * D + d];
+            else nc[d] = t->c[k * D + d] + .5 * t->w[k * D + d];
+            div *= 2;
+        }
+        long id = bh_new_node(t, nc, nw);
+        if (ch == 0) first = id;
+    }
+    t->first_child[k] = first;
+    for (unsigned s = 0; s < t->size[k]; s++) {
+        int ok = 0;
+        for (int ch = 0; ch < t->nc; ch++)
+            if (!ok) ok = bh_insert(t, first + ch, t->pt[k]);
+    }
+    t->size[k] = 0;
+    t->leaf[k] = 0;
+}
+
+/* sptree.cpp:189-232 */
+static int bh_insert(bh_tree *t, long k, unsigned i) {
+    const int D = t->D;
+    const double *p = t->Y + (long)i * D;
+    for (int d = 0; d < D; d++) {
+        if (t->c[k * D + d] - t->w[k * D + d] > p[d]) return 0;
+        if (t->c[k * D + d] + t->w[k * D + d] < p[d]) return 0;
+    }
+    t->cum[k]++;
+    double mult1 = (double)(t->cum[k] - 1) / (double)t->cum[k];
+    double mult2 = 1.0 / (double)t->cum[k];
+    for (int d = 0; d < D; d++) t->m[k * D + d] *= mult1;
+    for (int d = 0; d < D; d++) t->m[k * D + d] += mult2 * p[d];
+    if (t->leaf[k] && t->size[k] < 1) {
+        t->pt[k] = i;
+        t->size[k] = 1;
+        return 1;
+    }
+    int dup = 0;
+    for (unsigned s = 0; s < t->size[k]; s++) {
+        int same = 1;
+        for (int d = 0; d < D; d++)
+            if (p[d] != t->Y[(long)t->pt[k] * D + d]) {
+                same = 0;
+                break;
+            }
+        dup = dup | same;
+    }
+    if (dup) return 1;
+    if (t->leaf[k]) bh_subdivide(t, k);
+    for (int ch = 0; ch < t->nc; ch++)
+        if (bh_insert(t, t->first_child[k] + ch, i)) return 1;
+    return 0;
+}
+
+/* sptree.cpp:57-84 (root cell: the mean, half-widths max(max - mean, mean - min) + 1e-5) and :277 */
+static void bh_build(bh_tree *t, const double *Y, int N, int D) {
+    memset(t, 0, sizeof(*t));
+    t->D = D;
+    t->nc = 1 << D;
+    t->Y = Y;
+    double mean[8] = {0}, mn[8], mx[8], w[8];
+    for (int d = 0; d < D; d++) {
+        mn[d] = DBL_MAX;
+        mx[d] = -DBL_MAX;
+    }
+    for (long n = 0; n < N; n++)
+        for (int d = 0; d < D; d++) {
+            mean[d] += Y[n * D + d];
+            if (Y[n * D + d] < mn[d]) mn[d] = Y[n * D + d];
+            if (Y[n * D + d] > mx[d]) mx[d] = Y[n * D + d];
+        }
+    for (int d = 0; d < D; d++) mean[d] /= (double)N;
+    for (int d = 0; d < D; d++) w[d] = fmax(mx[d] - mean[d], mean[d] - mn[d]) + 1e-5;
+    bh_new_node(t, mean, w);
+    for (unsigned i = 0; i < (unsigned)N; i++) bh_insert(t, 0, i);
+}
+
+static void bh_free(bh_tree *t) {
+    free(t->c);
+    free(t->w);
+    free(t->m);
+    free(t->cum);
+    free(t->size);
+    free(t->pt);
+    free(t->leaf);
+    free(t->first_child);
+}
+
+/* sptree.cpp:322-354 */
+static void bh_nonedge(const bh_tree *t, long k, unsigned i, double theta, double *neg_f, double *sum_Q) {
+    const int D = t->D;
+    if (t->cum[k] == 0 || (t->leaf[k] && t->size[k] == 1 && t->pt[k] == i)) return;
+    double Dd = .0;
+    const double *y = t->Y + (long)i * D;
+    for (int d = 0; d < D; d++) Dd += (y[d] - t->m[k * D + d]) * (y[d] - t->m[k * D + d]);
+    double max_width = 0.0, cur;
+    for (int d = 0; d < D; d++) {
+        cur = t->w[k * D + d];
+        max_width = (max_width > cur) ? max_width : cur;
+    }
+    if (t->leaf[k] || max_width / sqrt(Dd) < theta) {
+        Dd = 1.0 / (1.0 + Dd);
+        double mult = t->cum[k] * Dd;
+        *sum_Q += mult;
+        mult *= Dd;
+        for (int d = 0; d < D; d++) neg_f[d] += mult * (y[d] - t->m[k * D + d]);
+    } else {
+        for (int ch = 0; ch < t->nc; ch++) bh_nonedge(t, t->first_child[k] + ch, i, theta, neg_f, sum_Q);
+    }
+}
+
+/* tsne.cpp:596-640 with sptree.cpp:358-376 (edge forces) and nthreads = 1 */
+void orc_bh_gradient(const uint32_t *row_P, const uint32_t *col_P, const double *val_P, const double *Y, int N,
+                     int D, double theta, double *dC, double *sum_Q_out) {
+    bh_tree t;
+    bh_build(&t, Y, N, D);
+    double *pos_f = (double *)calloc((size_t)N * D, sizeof(double));
+    double *neg_f = (double *)calloc((size_t)N * D, sizeof(double));
+    double *Q = (double *)calloc((size_t)N, sizeof(double));
+    for (long n = 0; n < N; n++) {
+        for (uint32_t e = row_P[n]; e < row_P[n + 1]; e++) {
+            double Dd = 1.0;
+            long m = col_P[e];
+            for (int d = 0; d < D; d++) Dd += (Y[n * D + d] - Y[m * D + d]) * (Y[n * D + d] - Y[m * D + d]);
+            Dd = val_P[e] / Dd;
+            for (int d = 0; d < D; d++) pos_f[n * D + d] += Dd * (Y[n * D + d] - Y[m * D + d]);
+        }
+    }
+    for (unsigned n = 0; n < (unsigned)N; n++) bh_nonedge(&t, 0, n, theta, neg_f + (long)n * D, Q + n);
+    double sum_Q = .0;
+    for (long n = 0; n < N; n++) sum_Q += Q[n];
+    for (long e = 0; e < (long)N * D; e++) dC[e] = pos_f[e] - (neg_f[e] / sum_Q);
+    if (sum_Q_out) *sum_Q_out = sum_Q;
+    free(pos_f);
+    free(neg_f);
+    free(Q);
+    bh_free(&t);
+}
+
+/* tsne.cpp:1359-1390 with nthreads = 1: sum_Q as one running sum over all points, then the KL over
+ * the edges, rows in order. */
+double orc_bh_error(const uint32_t *row_P, const uint32_t *col_P, const double *val_P, const double *Y, int N,
+                    int D, double theta) {
+    bh_tree t;
+    bh_build(&t, Y, N, D);
+    double buff[8] = {0};
+    double sum_Q = .0;
+    for (unsigned n = 0; n < (unsigned)N; n++) bh_nonedge(&t, 0, n, theta, buff, &sum_Q);
+    double C = .0;
+    for (long n = 0; n < N; n++) {
+        double temp = 0;
+        for (uint32_t e = row_P[n]; e < row_P[n + 1]; e++) {
+            double Qv = .0;
+            long m = col_P[e];
+            for (int d = 0; d < D; d++) buff[d] = Y[n * D + d];
+            for (int d = 0; d < D; d++) buff[d] -= Y[m * D + d];
+            for (int d = 0; d < D; d++) Qv += buff[d] * buff[d];
+            Qv = (1.0 / (1.0 + Qv)) / sum_Q;
+            temp += val_P[e] * log((val_P[e] + FLT_MIN) / (Qv + FLT_MIN));
+        }
+        C += temp;
+    }
+    bh_free(&t);
+    return C;
+}

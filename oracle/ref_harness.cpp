@@ -1,6 +1,7 @@
 /*
- * ref_harness.cpp -- C entry points into the reference's OWN source (FIt-SNE 1.2.1, compiled from
- * /root/reference/src by oracle/Makefile into oracle/_ref/libfitsne_ref.so).  TEST INFRASTRUCTURE.
+ * ref_harness.cpp -- C entry points into the reference's OWN source (FIt-SNE 1.2.1: tsne.cpp,
+ * nbodyfft.cpp, sptree.cpp, compiled from /root/reference/src by oracle/Makefile into
+ * oracle/_ref/libfitsne_ref.so).  TEST INFRASTRUCTURE.
  *
  * Only reference functions that never reach FFTW are exposed: FFTW is absent from this image, so the
  * reference's FFT path (precompute_2d, n_body_fft_2d, precompute, nbodyfft, computeFftGradient*,
@@ -114,6 +115,19 @@ API void ref_gaussian_perplexity_dense(double *X, int N, int D, double *P, doubl
                                        int list_len, double *list) {
     TSNE t;
     t.computeGaussianPerplexity(X, N, D, P, perplexity, sigma, list_len, list);
+}
+
+/* tsne.cpp:596-640 (Barnes-Hut gradient over the reference's SPTree), nthreads = 1 */
+API void ref_bh_gradient(unsigned *row_P, unsigned *col_P, double *val_P, double *Y, int N, int D, double theta,
+                         double *dC) {
+    TSNE t;
+    t.computeGradient(nullptr, row_P, col_P, val_P, Y, N, D, dC, theta, 1);
+}
+
+/* tsne.cpp:1359-1390 (Barnes-Hut KL), nthreads = 1 */
+API double ref_bh_error(unsigned *row_P, unsigned *col_P, double *val_P, double *Y, int N, int D, double theta) {
+    TSNE t;
+    return t.evaluateError(row_P, col_P, val_P, Y, N, D, theta, 1);
 }
 
 /* tsne.cpp:1851-1876 */

@@ -80,6 +80,8 @@ def oracle():
         _bind(L, "orc_exact_symmetrize_normalize", cd, [dp, ci])
         _bind(L, "orc_exact_gradient", None, [dp, dp, ci, ci, cd, dp, C.POINTER(cd)])
         _bind(L, "orc_evaluate_error_exact", cd, [dp, dp, ci, ci, cd])
+        _bind(L, "orc_bh_gradient", None, [u32p, u32p, dp, dp, ci, ci, cd, dp, C.POINTER(cd)])
+        _bind(L, "orc_bh_error", cd, [u32p, u32p, dp, dp, ci, ci, cd])
         _oracle = L
     return _oracle
 
@@ -104,6 +106,8 @@ def ref():
         _bind(L, "ref_zero_mean", None, [dp, ci, ci])
         _bind(L, "ref_squared_distances", None, [dp, ci, ci, dp])
         _bind(L, "ref_gaussian_perplexity_dense", None, [dp, ci, ci, dp, cd, cd, ci, dp])
+        _bind(L, "ref_bh_gradient", None, [u32p, u32p, dp, dp, ci, ci, cd, dp])
+        _bind(L, "ref_bh_error", cd, [u32p, u32p, dp, dp, ci, ci, cd])
         _bind(L, "ref_gaussian_perplexity_vptree", None,
               [dp, ci, ci, cd, ci, cd, cu, ci, dp, cu, u32p, u32p, dp])
         _bind(L, "ref_load_data", ci, [C.c_char_p, i32p, dp, C.c_void_p, C.c_void_p, C.c_void_p])
@@ -364,3 +368,33 @@ def exact_kl(P, Y, df=1.0, lib="oracle"):
     if lib == "oracle":
         return oracle().orc_evaluate_error_exact(P.reshape(-1), Y.reshape(-1), N, D, df)
     return ref().ref_evaluate_error_exact(P.reshape(-1).copy(), Y.reshape(-1).copy(), N, D, df)
+
+
+# ---------------------------------------------------------------- Barnes-Hut (SPTree)
+
+def _u32csr(P):
+    row, col, val = P
+    return (np.ascontiguousarray(row, dtype=np.uint32), np.ascontiguousarray(col, dtype=np.uint32),
+            np.ascontiguousarray(val, dtype=np.float64))
+
+
+def bh_gradient(P, Y, theta, lib="oracle"):
+    row, col, val = _u32csr(P)
+    Y = np.ascontiguousarray(Y, dtype=np.float64)
+    N, D = Y.shape
+    dC = np.empty(N * D)
+    if lib == "oracle":
+        sq = C.c_double()
+        oracle().orc_bh_gradient(row, col, val, Y.reshape(-1), N, D, theta, dC, C.byref(sq))
+        return dC.reshape(N, D), sq.value
+    ref().ref_bh_gradient(row.copy(), col.copy(), val.copy(), Y.reshape(-1).copy(), N, D, theta, dC)
+    return dC.reshape(N, D), None
+
+
+def bh_error(P, Y, theta, lib="oracle"):
+    row, col, val = _u32csr(P)
+    Y = np.ascontiguousarray(Y, dtype=np.float64)
+    N, D = Y.shape
+    if lib == "oracle":
+        return oracle().orc_bh_error(row, col, val, Y.reshape(-1), N, D, theta)
+    return ref().ref_bh_error(row.copy(), col.copy(), val.copy(), Y.reshape(-1).copy(), N, D, theta)

@@ -149,23 +149,23 @@ def test_binary_reads_the_data_file(tmp_path):
         assert "no GPU device is visible" in r.stderr
 
 
-def test_binary_rejects_paths_outside_this_build(tmp_path):
+def test_binary_rejects_invalid_settings(tmp_path):
     X = np.random.default_rng(3).normal(size=(40, 3))
-    _write(tmp_path / "d.dat", X, "random", perplexity=5.0, nbody_algo=1)
+    _write(tmp_path / "d.dat", X, "random", perplexity=5.0, nbody_algo=3)
     r = _run_binary([ft.VERSION, "d.dat", "r.dat"], tmp_path)
-    assert r.returncode != 0 and "Barnes-Hut" in r.stderr
+    assert r.returncode != 0 and "Undefined algorithm" in r.stderr
     # the FFT setting limits no_dims to 1 or 2 whatever theta is (tsne.cpp:1954-1957)
     _write(tmp_path / "d.dat", X, "random", perplexity=5.0, theta=0.0, map_dims=3)
     r = _run_binary([ft.VERSION, "d.dat", "r.dat"], tmp_path)
     assert r.returncode != 0 and "supports only 1 or 2 output dimensions" in r.stderr
 
 
-def test_binary_accepts_the_exact_mode(tmp_path):
-    """theta = 0 runs the exact mode (with nbody_algo = 1 also for 3 output dimensions); without a GPU the
-    program stops before computing."""
+def test_binary_accepts_the_exact_and_barnes_hut_modes(tmp_path):
+    """theta = 0 runs the exact mode, nbody_algo = 1 with theta > 0 the Barnes-Hut mode (both also in 3
+    output dimensions); without a GPU the program stops before computing."""
     X = np.random.default_rng(4).normal(size=(40, 3))
-    for dims, algo in [(2, 2), (3, 1)]:
-        _write(tmp_path / "d.dat", X, "random", perplexity=5.0, theta=0.0, map_dims=dims, nbody_algo=algo,
+    for dims, algo, theta in [(2, 2, 0.0), (3, 1, 0.0), (3, 1, 0.5), (1, 1, 0.5)]:
+        _write(tmp_path / "d.dat", X, "random", perplexity=5.0, theta=theta, map_dims=dims, nbody_algo=algo,
                max_iter=3)
         r = _run_binary([ft.VERSION, "d.dat", "r.dat"], tmp_path)
         if r.returncode != 0:
