@@ -235,6 +235,50 @@ def bench_exact(args, fg):
     print(json.dumps(out), flush=True)
 
 
+def bench_bh(args, fg, synth):
+    """Barnes-Hut (nbody_algo = 1, theta = 0.5) at N = 100,000 in 2D: each step builds the reference's
+    SPTree on the host in insertion order, then walks it and sums the edge forces on the GPU (bitwise the
+    reference's gradient).  The walk is irregular (pointer chasing over a tree of ~2N cells), so it has no
+    meaningful bandwidth roofline; the line reports the step and its two halves."""
+    N = args.n if args.n != 1_000_000 else 100_000
+    P, Y, _ = synth.small_mixture_workload(N=N, D=2, k=20, per_point=30, span=60.0, seed=20261202)
+    theta = 0.5
+    c = fg.Context(no_dims=2, precision=fg.FP64_BH)
+    c.set_theta(theta)
+    c.set_P(*P)
+    c.set_Y(Y)
+    prm = dict(momentum=0.8, learning_rate=max(200.0, N / 12.0), max_step_norm=5.0)
+    dt = timed_steps(c, args.steps, args.warmup, prm)
+    stats = kernel_profile(c, max(args.profile_steps, 1), prm)
+    c.close()
+    gpu_ms = sum(v[1] for v in stats.values()) / max(args.profile_steps, 1)
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import orc
+
+        log(f"CPU baseline: oracle Barnes-Hut gradient at N={N} (serial, the reference's SPTree)")
+        t0 = time.perf_counter()
+        orc.bh_gradient(P, Y, theta)
+        t_cpu = time.perf_counter() - t0
+        cpu = {"value": 1.0 / t_cpu, "unit": "grad iters/s", "cores": 1, "kind": "port",
+               "sample": f"one gradient of the same N={N} workload by the oracle's restatement of "
+                         f"computeGradient (tree build and walk, one thread), {t_cpu:.2f} s"}
+    out = {
+        "metric": "Barnes-Hut (theta=0.5) gradient iterations/s at N=100000 2D, FP64",
+        "value": args.steps / dt, "unit": "grad iters/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * dt / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (seeded Gaussian mixture, generated in place)",
+        "config": {"workload": f"Barnes-Hut t-SNE: N={N} 2D, theta={theta}, ~60 nnz/row symmetric P"},
+        "roofline": None,
+        "gpu_ms_per_step": gpu_ms,
+        "host_tree_ms_per_step": 1e3 * dt / args.steps - gpu_ms,
+        "cpu_baseline": cpu,
+        "kernels_ms": {k: round(v[1] / max(v[0], 1), 4) for k, v in sorted(stats.items(), key=lambda kv: -kv[1][1])},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -245,9 +289,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=5)
-    ap.add_argument("--config", default="c2", choices=["c2", "c4", "exact"],
+    ap.add_argument("--config", default="c2", choices=["c2", "c4", "exact", "bh"],
                     help="c2 (default): the headline step at N=1M 2D; c4: 64 batched embeddings; "
-                         "exact: theta = 0 at N=46000 2D")
+                         "exact: theta = 0 at N=46000 2D; bh: Barnes-Hut at N=100000 2D")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run one plain 64 MiB device copy (torch) as the reference kernel for "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE scaling")
@@ -264,6 +308,8 @@ def main():
         return bench_c4(args, fg, synth)
     if args.config == "exact":
         return bench_exact(args, fg)
+    if args.config == "bh":
+        return bench_bh(args, fg, synth)
 
     if args.copy_calibration:
         import torch

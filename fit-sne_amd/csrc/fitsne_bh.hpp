@@ -28,93 +28,108 @@ constexpr int kBhMaxDims = 3;
 
 // ---------------------------------------------------------------- host: the tree
 
+// One cell: center and half-width (the reference's Cell corner / width), online center of mass,
+// cum_size, at most one stored point, and the first of its 2^D consecutive children.
+template <int D>
+struct BhNode {
+    double c[D], w[D], m[D];
+    unsigned cum, pt;
+    int first;
+    unsigned char size, leaf;
+};
+
+template <int D>
 struct BhTree {
-    int D = 2, nc = 4;
+    static constexpr int NC = 1 << D;
     const double *Y = nullptr;
-    std::vector<double> c, w, m;  // per node: cell center, half-width, center of mass (D each)
-    std::vector<unsigned> cum, size, pt;
-    std::vector<char> leaf;
-    std::vector<int> first;  // first of the node's 2^D consecutive children, or -1
+    std::vector<BhNode<D>> nodes;
 
     int add(const double *center, const double *hw) {
-        const int k = (int)cum.size();
+        BhNode<D> n;
         for (int d = 0; d < D; d++) {
-            c.push_back(center[d]);
-            w.push_back(hw[d]);
-            m.push_back(0.0);
+            n.c[d] = center[d];
+            n.w[d] = hw[d];
+            n.m[d] = 0.0;
         }
-        cum.push_back(0);
-        size.push_back(0);
-        pt.push_back(0);
-        leaf.push_back(1);
-        first.push_back(-1);
-        return k;
+        n.cum = 0;
+        n.pt = 0;
+        n.first = -1;
+        n.size = 0;
+        n.leaf = 1;
+        nodes.push_back(n);
+        return (int)nodes.size() - 1;
     }
 
-    // sptree.cpp:235-262: children i take the upper half of dimension d when bit d of i is 0
+    // sptree.cpp:235-262: child ch takes the lower half of dimension d when bit d of ch is 1
     void subdivide(int k) {
-        double nc_[kBhMaxDims], nw[kBhMaxDims];
+        double nc[D], nw[D];
         int f = -1;
-        for (int ch = 0; ch < nc; ch++) {
+        for (int ch = 0; ch < NC; ch++) {
             unsigned div = 1;
+            const BhNode<D> &p = nodes[k];
             for (int d = 0; d < D; d++) {
-                nw[d] = .5 * w[k * D + d];
-                nc_[d] = ((unsigned)ch / div) % 2 == 1 ? c[k * D + d] - .5 * w[k * D + d] : c[k * D + d] + .5 * w[k * D + d];
+                nw[d] = .5 * p.w[d];
+                nc[d] = ((unsigned)ch / div) % 2 == 1 ? p.c[d] - .5 * p.w[d] : p.c[d] + .5 * p.w[d];
                 div *= 2;
             }
-            const int id = add(nc_, nw);
+            const int id = add(nc, nw);  // may move nodes[]
             if (ch == 0) f = id;
         }
-        first[k] = f;
-        if (size[k] == 1) {
-            for (int ch = 0; ch < nc; ch++)
-                if (insert(f + ch, pt[k])) break;
+        nodes[k].first = f;
+        if (nodes[k].size == 1) {
+            const unsigned q = nodes[k].pt;
+            for (int ch = 0; ch < NC; ch++)
+                if (insert(f + ch, q)) break;
         }
-        size[k] = 0;
-        leaf[k] = 0;
+        nodes[k].size = 0;
+        nodes[k].leaf = 0;
     }
 
     // sptree.cpp:189-232
     bool insert(int k, unsigned i) {
-        const double *p = Y + (long)i * D;
-        for (int d = 0; d < D; d++)
-            if (c[k * D + d] - w[k * D + d] > p[d] || c[k * D + d] + w[k * D + d] < p[d]) return false;
-        cum[k]++;
-        const double mult1 = (double)(cum[k] - 1) / (double)cum[k], mult2 = 1.0 / (double)cum[k];
-        for (int d = 0; d < D; d++) m[k * D + d] *= mult1;
-        for (int d = 0; d < D; d++) m[k * D + d] += mult2 * p[d];
-        if (leaf[k] && size[k] < 1) {
-            pt[k] = i;
-            size[k] = 1;
-            return true;
-        }
-        if (size[k] == 1) {
-            bool same = true;
+        const double *p = Y + (size_t)i * D;
+        {
+            BhNode<D> &n = nodes[k];
             for (int d = 0; d < D; d++)
-                if (p[d] != Y[(long)pt[k] * D + d]) {
-                    same = false;
-                    break;
-                }
-            if (same) return true;
+                if (n.c[d] - n.w[d] > p[d] || n.c[d] + n.w[d] < p[d]) return false;
+            n.cum++;
+            const double mult1 = (double)(n.cum - 1) / (double)n.cum, mult2 = 1.0 / (double)n.cum;
+            for (int d = 0; d < D; d++) n.m[d] *= mult1;
+            for (int d = 0; d < D; d++) n.m[d] += mult2 * p[d];
+            if (n.leaf && n.size < 1) {
+                n.pt = i;
+                n.size = 1;
+                return true;
+            }
+            if (n.size == 1) {
+                bool same = true;
+                for (int d = 0; d < D; d++)
+                    if (p[d] != Y[(size_t)n.pt * D + d]) {
+                        same = false;
+                        break;
+                    }
+                if (same) return true;
+            }
         }
-        if (leaf[k]) subdivide(k);
-        for (int ch = 0; ch < nc; ch++)
-            if (insert(first[k] + ch, i)) return true;
+        if (nodes[k].leaf) subdivide(k);
+        const int f = nodes[k].first;
+        for (int ch = 0; ch < NC; ch++)
+            if (insert(f + ch, i)) return true;
         return false;
     }
 
     // sptree.cpp:57-84 and :275-277
-    void build(const double *Yin, int N, int Din) {
-        D = Din;
-        nc = 1 << D;
+    void build(const double *Yin, int N) {
         Y = Yin;
-        c.clear(), w.clear(), m.clear(), cum.clear(), size.clear(), pt.clear(), leaf.clear(), first.clear();
-        double mean[kBhMaxDims] = {0, 0, 0}, mn[kBhMaxDims], mx[kBhMaxDims], hw[kBhMaxDims];
+        nodes.clear();
+        nodes.reserve((size_t)3 * N + 16);
+        double mean[D], mn[D], mx[D], hw[D];
         for (int d = 0; d < D; d++) {
+            mean[d] = 0.0;
             mn[d] = DBL_MAX;
             mx[d] = -DBL_MAX;
         }
-        for (long n = 0; n < N; n++)
+        for (size_t n = 0; n < (size_t)N; n++)
             for (int d = 0; d < D; d++) {
                 const double v = Y[n * D + d];
                 mean[d] += v;
@@ -128,61 +143,52 @@ struct BhTree {
     }
 };
 
-// The walk's view of the tree in depth-first pre-order.
-struct BhFlat {
-    std::vector<double> com, maxw;
-    std::vector<unsigned> cum;
-    std::vector<int> leafpt, isleaf, skip;
-    int depth = 0;
+// The walk's view of the tree in depth-first pre-order, written into one block of memory (the host
+// side of a single copy): com[n*D] and maxw[n] (double), then cum[n], leafpt[n], isleaf[n], skip[n]
+// (32-bit).  bh_flat_bytes gives its size for n nodes.
+inline size_t bh_flat_bytes(size_t n, int D) { return n * (8 * (size_t)D + 8 + 16); }
 
-    void flatten(const BhTree &t) {
-        const int n = (int)t.cum.size(), D = t.D;
-        com.resize((size_t)n * D);
-        maxw.resize(n);
-        cum.resize(n);
-        leafpt.resize(n);
-        isleaf.resize(n);
-        skip.resize(n);
-        // iterative pre-order: (tree node, depth); a node's slot is taken when it is popped, and its skip
-        // is filled when the walk has passed its subtree
-        std::vector<std::pair<int, int>> st;
-        std::vector<int> open;  // slots whose subtree is still being laid out, with their depth
-        std::vector<int> open_depth;
-        st.push_back({0, 0});
-        int next = 0;
-        depth = 0;
-        while (!st.empty()) {
-            const int k = st.back().first, dep = st.back().second;
-            st.pop_back();
-            while (!open.empty() && open_depth.back() >= dep) {
-                skip[open.back()] = next;
-                open.pop_back();
-                open_depth.pop_back();
-            }
-            const int s = next++;
-            for (int d = 0; d < D; d++) com[(size_t)s * D + d] = t.m[(size_t)k * D + d];
-            double mw = 0.0;
-            for (int d = 0; d < D; d++) {
-                const double cur = t.w[(size_t)k * D + d];
-                mw = mw > cur ? mw : cur;
-            }
-            maxw[s] = mw;
-            cum[s] = t.cum[k];
-            isleaf[s] = t.leaf[k] ? 1 : 0;
-            leafpt[s] = (t.leaf[k] && t.size[k] == 1) ? (int)t.pt[k] : -1;
-            if (dep + 1 > depth) depth = dep + 1;
-            open.push_back(s);
-            open_depth.push_back(dep);
-            if (!t.leaf[k])
-                for (int ch = t.nc - 1; ch >= 0; ch--) st.push_back({t.first[k] + ch, dep + 1});
-        }
-        while (!open.empty()) {
-            skip[open.back()] = -1;
-            open.pop_back();
-            open_depth.pop_back();
-        }
+struct BhFlatView {
+    double *com, *maxw;
+    unsigned *cum;
+    int *leafpt, *isleaf, *skip;
+    BhFlatView(void *base, size_t n, int D) {
+        char *p = static_cast<char *>(base);
+        com = reinterpret_cast<double *>(p);
+        maxw = com + n * D;
+        cum = reinterpret_cast<unsigned *>(maxw + n);
+        leafpt = reinterpret_cast<int *>(cum + n);
+        isleaf = leafpt + n;
+        skip = isleaf + n;
     }
 };
+
+// Lay out node k's subtree from slot s in pre-order; returns the slot after it (the subtree's skip).
+template <int D>
+int bh_lay(const BhTree<D> &t, const BhFlatView &out, int k, int s) {
+    const BhNode<D> &n = t.nodes[k];
+    double mw = 0.0;
+    for (int d = 0; d < D; d++) {
+        out.com[(size_t)s * D + d] = n.m[d];
+        mw = mw > n.w[d] ? mw : n.w[d];
+    }
+    out.maxw[s] = mw;
+    out.cum[s] = n.cum;
+    out.isleaf[s] = n.leaf ? 1 : 0;
+    out.leafpt[s] = (n.leaf && n.size == 1) ? (int)n.pt : -1;
+    int next = s + 1;
+    if (!n.leaf)
+        for (int ch = 0; ch < BhTree<D>::NC; ch++) next = bh_lay(t, out, n.first + ch, next);
+    out.skip[s] = next;
+    return next;
+}
+
+template <int D>
+void bh_flatten(const BhTree<D> &t, const BhFlatView &out) {
+    const int n = bh_lay(t, out, 0, 0);
+    for (int s = 0; s < n; s++)
+        if (out.skip[s] == n) out.skip[s] = -1;  // the end of the walk
+}
 
 // ---------------------------------------------------------------- device
 
@@ -263,13 +269,24 @@ __global__ __launch_bounds__(kThreads) void k_bh_edge(const int64_t *__restrict_
     for (int d = 0; d < D; d++) pos_f[(long)n * D + d] = pf[d];
 }
 
-// sum_Q = sum of Q[i] in index order (tsne.cpp:612-614), one thread; out[0] = sum_Q.
-__global__ void k_seq_sum(const double *__restrict__ v, int n, double *__restrict__ out) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+// sum_Q = sum of Q[i] in index order (tsne.cpp:612-614): one workgroup stages chunks of v in LDS and
+// its first thread adds them in order; out[0] = sum_Q.
+constexpr int kSeqChunk = 8192;
+__global__ __launch_bounds__(1024) void k_seq_sum(const double *__restrict__ v, int n, double *__restrict__ out) {
+    __shared__ double buf[kSeqChunk];
     double s = .0;
-    for (int i = 0; i < n; i++) s += v[i];
-    out[0] = s;
-    out[1] = 1.0 / s;
+    for (int base = 0; base < n; base += kSeqChunk) {
+        const int len = min(kSeqChunk, n - base);
+        __syncthreads();
+        for (int i = threadIdx.x; i < len; i += blockDim.x) buf[i] = v[base + i];
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int i = 0; i < len; i++) s += buf[i];
+    }
+    if (threadIdx.x == 0) {
+        out[0] = s;
+        out[1] = 1.0 / s;
+    }
 }
 
 // dC = pos_f - neg_f / sum_Q (tsne.cpp:626-627).

@@ -140,10 +140,14 @@ struct fitsne_ctx {
     DBuf att, rep, zrow;
     // Barnes-Hut: the host tree of the current Y and its device copy (pre-order), theta
     double theta = 0.5;
-    BhTree bh_tree;
-    BhFlat bh_flat;
+    BhTree<1> bh_tree1;
+    BhTree<2> bh_tree2;
+    BhTree<3> bh_tree3;
     std::vector<double> bh_y;
-    DBuf bh_com, bh_maxw, bh_cum, bh_leafpt, bh_isleaf, bh_skip, bh_neg, bh_Q;
+    void *bh_pin = nullptr;  // pinned host memory: Y, then the flattened tree
+    size_t bh_pin_cap = 0;
+    size_t bh_nodes = 0;
+    DBuf bh_tree_d, bh_neg, bh_Q;
     // symmetrisation of a CSR P (mirror tier)
     DBuf sy_row, sy_col, sy_val, sy_erow, sy_keys, sy_keys2, sy_vals, sy_vals2, sy_partner, sy_cnt, sy_start,
         sy_okeys, sy_okeys2, sy_tmp, sy_scol, sy_sval;
@@ -909,29 +913,48 @@ static int kl_exact(fitsne_ctx *c) {
 
 // ============================================================================ Barnes-Hut (nbody_algo = 1)
 
-// The SPTree of the resident Y, built on the host in insertion order and uploaded in pre-order.
-static int bh_tree_of_Y(fitsne_ctx *c) {
-    const int N = c->N, D = c->D;
-    c->bh_y.resize((size_t)N * D);
-    HIPTRY(hipMemcpyAsync(c->bh_y.data(), c->Y.p, sizeof(double) * (size_t)N * D, hipMemcpyDeviceToHost, c->stream));
+// Pinned host memory of at least `bytes` for the Barnes-Hut transfers.
+static int bh_pinned(fitsne_ctx *c, size_t bytes) {
+    if (c->bh_pin_cap >= bytes) return FITSNE_OK;
     TRY(sync_ctx(c));
-    c->bh_tree.build(c->bh_y.data(), N, D);
-    c->bh_flat.flatten(c->bh_tree);
-    const BhFlat &f = c->bh_flat;
-    const size_t n = f.cum.size();
-    TRY(ensure(c->bh_com, sizeof(double) * n * D));
-    TRY(ensure(c->bh_maxw, sizeof(double) * n));
-    TRY(ensure(c->bh_cum, sizeof(unsigned) * n));
-    TRY(ensure(c->bh_leafpt, sizeof(int) * n));
-    TRY(ensure(c->bh_isleaf, sizeof(int) * n));
-    TRY(ensure(c->bh_skip, sizeof(int) * n));
-    HIPTRY(hipMemcpyAsync(c->bh_com.p, f.com.data(), sizeof(double) * n * D, hipMemcpyHostToDevice, c->stream));
-    HIPTRY(hipMemcpyAsync(c->bh_maxw.p, f.maxw.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
-    HIPTRY(hipMemcpyAsync(c->bh_cum.p, f.cum.data(), sizeof(unsigned) * n, hipMemcpyHostToDevice, c->stream));
-    HIPTRY(hipMemcpyAsync(c->bh_leafpt.p, f.leafpt.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
-    HIPTRY(hipMemcpyAsync(c->bh_isleaf.p, f.isleaf.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
-    HIPTRY(hipMemcpyAsync(c->bh_skip.p, f.skip.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
-    // the host vectors stay alive until the next build; the copies are ordered before the walk
+    if (c->bh_pin) (void)hipHostFree(c->bh_pin);
+    c->bh_pin = nullptr;
+    c->bh_pin_cap = 0;
+    const size_t cap = bytes + bytes / 4;
+    if (hipHostMalloc(&c->bh_pin, cap, hipHostMallocDefault) != hipSuccess) {
+        c->bh_pin = nullptr;
+        return fail(FITSNE_E_ALLOC, "pinned host memory for the Barnes-Hut tree");
+    }
+    c->bh_pin_cap = cap;
+    return FITSNE_OK;
+}
+
+template <int D>
+static BhTree<D> &bh_tree_for(fitsne_ctx *c) {
+    if constexpr (D == 1) return c->bh_tree1;
+    else if constexpr (D == 2) return c->bh_tree2;
+    else return c->bh_tree3;
+}
+
+// The SPTree of the resident Y, built on the host in insertion order (from a cached host copy of Y) and
+// uploaded in pre-order, one copy each way through pinned memory.
+template <int D>
+static int bh_tree_of_Y(fitsne_ctx *c) {
+    const int N = c->N;
+    const size_t ybytes = sizeof(double) * (size_t)N * D;
+    TRY(bh_pinned(c, ybytes));
+    HIPTRY(hipMemcpyAsync(c->bh_pin, c->Y.p, ybytes, hipMemcpyDeviceToHost, c->stream));
+    TRY(sync_ctx(c));
+    c->bh_y.resize((size_t)N * D);
+    std::memcpy(c->bh_y.data(), c->bh_pin, ybytes);
+    BhTree<D> &t = bh_tree_for<D>(c);
+    t.build(c->bh_y.data(), N);
+    const size_t n = t.nodes.size(), tbytes = bh_flat_bytes(n, D);
+    TRY(bh_pinned(c, tbytes));
+    bh_flatten(t, BhFlatView(c->bh_pin, n, D));
+    TRY(ensure(c->bh_tree_d, tbytes));
+    HIPTRY(hipMemcpyAsync(c->bh_tree_d.p, c->bh_pin, tbytes, hipMemcpyHostToDevice, c->stream));
+    c->bh_nodes = n;
     return FITSNE_OK;
 }
 
@@ -940,10 +963,11 @@ static int bh_walk(fitsne_ctx *c, bool forces) {
     const int N = c->N;
     TRY(ensure(c->bh_Q, sizeof(double) * (size_t)N));
     if (forces) TRY(ensure(c->bh_neg, sizeof(double) * (size_t)N * D));
+    const BhFlatView v(c->bh_tree_d.p, c->bh_nodes, D);
     launch(c, "bh_nonedge", k_bh_nonedge<D>, dim3(blocks_for(N)), dim3(kThreads), 0, c->Y.as<const double>(), N,
-           c->bh_com.as<const double>(), c->bh_maxw.as<const double>(), c->bh_cum.as<const unsigned>(),
-           c->bh_leafpt.as<const int>(), c->bh_isleaf.as<const int>(), c->bh_skip.as<const int>(), c->theta,
-           forces ? c->bh_neg.as<double>() : (double *)nullptr, c->bh_Q.as<double>());
+           (const double *)v.com, (const double *)v.maxw, (const unsigned *)v.cum, (const int *)v.leafpt,
+           (const int *)v.isleaf, (const int *)v.skip, c->theta, forces ? c->bh_neg.as<double>() : (double *)nullptr,
+           c->bh_Q.as<double>());
     return FITSNE_OK;
 }
 
@@ -952,14 +976,14 @@ template <int D>
 static int pipeline_bh(fitsne_ctx *c, int mode, const fitsne_step_params *sp) {
     const int N = c->N;
     const long nd = (long)N * D;
-    TRY(bh_tree_of_Y(c));
+    TRY(bh_tree_of_Y<D>(c));
     TRY(ensure(c->pos, sizeof(double) * (size_t)nd));
     TRY(ensure(c->dC, sizeof(double) * (size_t)nd));
     TRY(ensure(c->zbuf, sizeof(double) * 4));
     launch(c, "bh_edge", k_bh_edge<D>, dim3(blocks_for(N)), dim3(kThreads), 0, c->rowP.as<const int64_t>(),
            c->colP.as<const uint32_t>(), c->val.as<const double>(), c->Y.as<const double>(), N, c->pos.as<double>());
     TRY(bh_walk<D>(c, true));
-    launch(c, "bh_sumq", k_seq_sum, dim3(1), dim3(kWave), 0, c->bh_Q.as<const double>(), N, c->zbuf.as<double>());
+    launch(c, "bh_sumq", k_seq_sum, dim3(1), dim3(1024), 0, c->bh_Q.as<const double>(), N, c->zbuf.as<double>());
     launch(c, "bh_combine", k_bh_combine, dim3(std::min(blocks_for(nd), 4096)), dim3(kThreads), 0,
            c->pos.as<const double>(), c->bh_neg.as<const double>(), c->zbuf.as<const double>(), nd, c->dC.as<double>());
     if (mode == 1) {
@@ -989,10 +1013,10 @@ static int pipeline_bh(fitsne_ctx *c, int mode, const fitsne_step_params *sp) {
 template <int D>
 static int kl_bh_d(fitsne_ctx *c) {
     const int N = c->N;
-    TRY(bh_tree_of_Y(c));
+    TRY(bh_tree_of_Y<D>(c));
     TRY(bh_walk<D>(c, false));
     TRY(ensure(c->mean, sizeof(double) * kExMaxDims));
-    launch(c, "bh_sumq", k_seq_sum, dim3(1), dim3(kWave), 0, c->bh_Q.as<const double>(), N, c->mean.as<double>() + 2);
+    launch(c, "bh_sumq", k_seq_sum, dim3(1), dim3(1024), 0, c->bh_Q.as<const double>(), N, c->mean.as<double>() + 2);
     const long threads = (long)N * 32;
     const int nblk = blocks_for(threads);
     TRY(ensure(c->klpart, sizeof(double) * (size_t)std::max(nblk, 2)));
@@ -1116,8 +1140,7 @@ extern "C" int fitsne_ctx_destroy(fitsne_ctx *c) {
                    &c->inv_d, &c->colP2, &c->val2, &c->tmp, &c->fixgrid, &c->att, &c->rep, &c->zrow,
                    &c->sy_row, &c->sy_col, &c->sy_val, &c->sy_erow, &c->sy_keys, &c->sy_keys2, &c->sy_vals,
                    &c->sy_vals2, &c->sy_partner, &c->sy_cnt, &c->sy_start, &c->sy_okeys, &c->sy_okeys2, &c->sy_tmp,
-                   &c->sy_scol, &c->sy_sval, &c->bh_com, &c->bh_maxw, &c->bh_cum, &c->bh_leafpt, &c->bh_isleaf,
-                   &c->bh_skip, &c->bh_neg, &c->bh_Q};
+                   &c->sy_scol, &c->sy_sval, &c->bh_tree_d, &c->bh_neg, &c->bh_Q};
     for (DBuf *b : all) release(*b);
     for (auto &kv : c->tw32) release(kv.second);
     for (auto &kv : c->tw64) release(kv.second);
@@ -1131,6 +1154,7 @@ extern "C" int fitsne_ctx_destroy(fitsne_ctx *c) {
     if (c->join_ev) (void)hipEventDestroy(c->join_ev);
     if (c->astream) (void)hipStreamDestroy(c->astream);
     if (c->h_mm) (void)hipHostFree(c->h_mm);
+    if (c->bh_pin) (void)hipHostFree(c->bh_pin);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return FITSNE_OK;
