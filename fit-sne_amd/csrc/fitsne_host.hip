@@ -327,8 +327,12 @@ static int current_minmax(fitsne_ctx *c, double *mn, double *mx) {
 }
 
 static int geometry(fitsne_ctx *c, double mn, double mx, Geo *g) {
+    // Y is finite on upload; the bounds are not when the optimiser has diverged, or in 2D when the
+    // coordinates increase strictly in input order: the reference's `else if` rule (tsne.cpp:1045-1048)
+    // then never assigns its minimum and its grid would be built from an infinite span.
     if (!(std::isfinite(mn) && std::isfinite(mx)))
-        return fail(FITSNE_E_ARG, "embedding contains non-finite coordinates");
+        return fail(FITSNE_E_ARG, "grid bounds are not finite: non-finite coordinates, or (2D) coordinates that "
+                                  "increase strictly in input order, which leave the min rule without a minimum");
     const int nb = c->D == 2 ? n_boxes_2d(mn, mx, c->ipi, c->mni) : n_boxes_1d(mn, mx, c->ipi, c->mni);
     if (nb < 1) return fail(FITSNE_E_ARG, "n_boxes < 1");
     *g = make_geo(c->D, c->p, mn, mx, nb);
@@ -1457,9 +1461,11 @@ constexpr int kReorderMinN = 65536;
 extern "C" int fitsne_set_Y(fitsne_ctx *c, const double *Y) {
     if (!c || !Y) return fail(FITSNE_E_ARG, "null argument");
     if (c->N <= 0) return fail(FITSNE_E_ARG, "set P before Y");
+    const size_t n = (size_t)c->N * c->D;
+    for (size_t e = 0; e < n; e++)
+        if (!std::isfinite(Y[e])) return fail(FITSNE_E_ARG, "Y contains non-finite values");
     HIPTRY(hipSetDevice(c->device));
     TRY(sync_ctx(c));
-    const size_t n = (size_t)c->N * c->D;
     const int D = c->D;
     if (c->precision == FITSNE_FP32) {
         std::vector<float> y(n), z(n, 0.f), one(n, 1.f);
