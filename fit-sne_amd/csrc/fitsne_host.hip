@@ -118,7 +118,7 @@ struct fitsne_ctx {
     // exact kNN (affinity build)
     DBuf knnX, knnXT, knnI, knnD;
     // windowed copy of P for the FP32 attractive term (k_attr_win), built on first use
-    DBuf wcount, wstart, wkey, wval, wbmax;
+    DBuf wcount, wstart, wpure, wkey, wval, wbmax;
     bool win_ok = false;
     int win_R = 0, win_NW = 0;
     // grids, spectra
@@ -443,19 +443,21 @@ static int build_windows(fitsne_ctx *c) {
     const int NB = (N + R - 1) / R, NW = (N + kWin - 1) / kWin;
     const long nbw = (long)NB * NW;
     const long long cap = c->nnz + 3ll * nbw;  // every bucket padded to a multiple of 4 entries
-    if (c->nnz <= 0 || NW > 16384 || nbw >= (1l << 28) || cap >= (1ll << 31) - 1) return FITSNE_OK;
+    if (c->nnz <= 0 || NW > 8192 || nbw >= (1l << 28) || cap >= (1ll << 31) - 1) return FITSNE_OK;
     TRY(ensure(c->wcount, sizeof(int) * nbw));
+    TRY(ensure(c->wpure, sizeof(int) * nbw));
     TRY(ensure(c->wstart, sizeof(int) * (nbw + 1)));
     TRY(ensure(c->wbmax, sizeof(float) * NB));
     TRY(ensure(c->wkey, sizeof(uint32_t) * (size_t)cap));
     TRY(ensure(c->wval, sizeof(float) * (size_t)cap));
-    const size_t lds = sizeof(int) * (size_t)NW;
+    const size_t lds = sizeof(int) * 2 * (size_t)NW;
     launch(c, "win_count", k_win_count, dim3(NB), dim3(kAttThreads), lds, c->rowP.as<const int64_t>(),
-           c->colP.as<const uint32_t>(), c->val.as<const float>(), N, R, NW, c->wcount.as<int>(), c->wbmax.as<float>());
+           c->colP.as<const uint32_t>(), c->val.as<const float>(), N, R, NW, c->wcount.as<int>(), c->wpure.as<int>(),
+           c->wbmax.as<float>());
     launch(c, "scan_counts", k_scan_counts, dim3(1), dim3(1024), 0, c->wcount.as<int>(), c->wstart.as<int>(), (int)nbw);
     launch(c, "win_fill", k_win_fill, dim3(NB), dim3(kAttThreads), lds, c->rowP.as<const int64_t>(),
            c->colP.as<const uint32_t>(), c->val.as<const float>(), N, R, NW, c->wstart.as<const int>(),
-           c->wkey.as<uint32_t>(), c->wval.as<float>());
+           c->wpure.as<const int>(), c->wkey.as<uint32_t>(), c->wval.as<float>());
     HIPTRY(hipGetLastError());
     c->win_R = R;
     c->win_NW = NW;
@@ -486,7 +488,8 @@ static int launch_attractive(fitsne_ctx *c, TV scale, TV *pos, const DevGeo *dg 
         if (c->win_ok) {
             const int NB = (c->N + c->win_R - 1) / c->win_R;
             launch(c, "attractive", k_attr_win<D>, dim3(8 * ((NB + 7) / 8)), dim3(kAttThreads), 0,
-                   c->wstart.as<const int>(), c->wkey.as<const uint32_t>(), c->wval.as<const float>(),
+                   c->wstart.as<const int>(), c->wpure.as<const int>(), c->wkey.as<const uint32_t>(),
+                   c->wval.as<const float>(),
                    c->wbmax.as<const float>(), c->Y.as<const float>(), c->N, c->win_R, c->win_NW, inv_df, df1,
                    (float)(0.5 * std::sqrt(c->df)), c->dstep.as<const StepDev>(), dg, pos);
             return FITSNE_OK;
@@ -1136,7 +1139,7 @@ extern "C" int fitsne_ctx_destroy(fitsne_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->astream) (void)hipStreamSynchronize(c->astream);
     drop_graph(c);
-    DBuf *all[] = {&c->rowP, &c->colP, &c->val, &c->Y, &c->uY, &c->gains, &c->pos, &c->dC, &c->wcount, &c->wstart,
+    DBuf *all[] = {&c->rowP, &c->colP, &c->val, &c->Y, &c->uY, &c->gains, &c->pos, &c->dC, &c->wcount, &c->wstart, &c->wpure,
                    &c->wkey, &c->wval, &c->wbmax, &c->knnX, &c->knnXT, &c->knnI, &c->knnD, &c->dgeo,
                    &c->dstep, &c->dtw, &c->dlive, &c->klpart, &c->gin, &c->gout, &c->spec, &c->ospec, &c->Trow, &c->KQ,
                    &c->K1, &c->KS, &c->KT, &c->zpart, &c->zbuf, &c->part, &c->mean, &c->mmpart, &c->mm, &c->xs,

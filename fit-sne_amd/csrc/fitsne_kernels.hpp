@@ -138,15 +138,21 @@ __global__ __launch_bounds__(kThreads) void k_attractive(const TI *__restrict__ 
 // scattered inside them, so the CSR kernel's gathers are one L2 request per entry.  This layout works
 // from LDS instead.  Rows are cut into blocks of R (<= kAttRows) consecutive points and columns into
 // windows of kWin points.  The entries of row block b whose column falls in window w form bucket (b, w);
-// buckets are stored contiguously, b-major then w, each padded to a multiple of 4 entries (padding:
-// key 0, value 0), each entry as
+// buckets are stored contiguously, b-major then w, each entry as
 //   key = (row - b R) << 16 | (col - w kWin)   (u32)   and   p_ij (f32).
+// A bucket is a pure section of row-pure quads (4 consecutive entries of one row) followed by a mixed
+// section padded to a multiple of 4 entries (padding: key 0, value 0).  Each 64-entry chunk of a row's
+// CSR range is split by window into groups; a group of n entries puts 4 floor(n / 4) of them in the
+// pure section and the other n mod 4 in the mixed one.
 // The workgroup of row block b holds its rows' Y and one int64 accumulator per row and coordinate in
 // LDS, and walks its non-empty buckets: window w of Y sits in LDS (the next one is loaded into registers
 // meanwhile), every lane takes 4 consecutive entries per 16-byte load, gathers both points from LDS and
 // adds
 //   round(p_ij q_ij (y_i - y_j) S_b)
-// to the row's accumulator (LDS 64-bit atomics; a lane first merges its consecutive entries of one row).
+// to the row's accumulator (LDS 64-bit atomics).  A pure quad reads its row once and issues one atomic
+// per coordinate; a mixed quad merges its consecutive entries of one row and issues one per run.  Most
+// entries are in pure quads, so the per-run atomics of a wave (issued whenever any lane changes row)
+// are confined to the short mixed sections.
 // S_b = 2^(50 - e), 2^e > max over the block's rows of sum_j p_ij * sqrt(df)/2, bounds every partial
 // sum by 2^50 since |q d| <= sqrt(df)/2: the integer sums are exact, so the forces do not depend on the
 // order of the entries and are bitwise repeatable.  Same formula as k_attractive (tsne.cpp:1121-1137).
@@ -174,58 +180,108 @@ __device__ __forceinline__ long long fix_rn_any(double x) {
     return (long long)rint(x);
 }
 
-// Per row block: entries per window, rounded up to a multiple of 4, into count[b * NW + w], and
-// max_i sum_j |p_ij| into bmax[b].
+// The window groups of one 64-entry chunk of a row (entries k0 + lane < k1), in a fixed order: the
+// wave-uniform loop takes the window of the lowest remaining lane, the lanes that share it form the
+// group.  f(w, m) runs on every lane for each group (m: its lanes).
+template <typename F>
+__device__ __forceinline__ void window_groups(bool valid, int w, F &&f) {
+    unsigned long long left = __ballot(valid);
+    while (left) {
+        const int leader = __ffsll((long long)left) - 1;
+        const int wl = __shfl(w, leader, kWave);
+        const unsigned long long m = __ballot(valid && w == wl);
+        f(wl, m, leader);
+        left &= ~m;
+    }
+}
+
+// Per row block: the size of every bucket (pure entries + mixed entries rounded up to a multiple of 4)
+// into count[b * NW + w], its pure entries into pure[b * NW + w], and max_i sum_j |p_ij| into bmax[b].
 __global__ __launch_bounds__(kAttThreads) void k_win_count(const int64_t *__restrict__ rowP,
                                                            const uint32_t *__restrict__ colP,
                                                            const float *__restrict__ val, int N, int R, int NW,
-                                                           int *__restrict__ count, float *__restrict__ bmax) {
-    extern __shared__ int hist[];
+                                                           int *__restrict__ count, int *__restrict__ pure,
+                                                           float *__restrict__ bmax) {
+    extern __shared__ int hist[];  // [0, NW): pure entries, [NW, 2 NW): mixed entries
     __shared__ unsigned int smax;
     const int b = blockIdx.x, t = threadIdx.x, lane = t % kWave, wv = t / kWave;
     const int r0 = b * R, r1 = min(N, r0 + R);
-    for (int w = t; w < NW; w += kAttThreads) hist[w] = 0;
+    for (int w = t; w < 2 * NW; w += kAttThreads) hist[w] = 0;
     if (t == 0) smax = 0u;
     __syncthreads();
     for (int r = r0 + wv; r < r1; r += kAttThreads / kWave) {
         float sum = 0.f;
-        for (long k = rowP[r] + lane; k < rowP[r + 1]; k += kWave) {
-            atomicAdd(&hist[colP[k] / kWin], 1);
-            sum += fabsf(val[k]);
+        const long k1 = rowP[r + 1];
+        for (long k0 = rowP[r]; k0 < k1; k0 += kWave) {
+            const long k = k0 + lane;
+            const bool valid = k < k1;
+            const int w = valid ? (int)(colP[k] / kWin) : -1;
+            if (valid) sum += fabsf(val[k]);
+            window_groups(valid, w, [&](int wl, unsigned long long m, int leader) {
+                if (lane == leader) {
+                    const int n = __popcll(m);
+                    atomicAdd(&hist[wl], n & ~3);
+                    atomicAdd(&hist[NW + wl], n & 3);
+                }
+            });
         }
         sum = group_sum<float, kWave>(sum);
         if (lane == 0) atomicMax(&smax, __float_as_uint(sum));  // non-negative floats order as integers
     }
     __syncthreads();
-    for (int w = t; w < NW; w += kAttThreads) count[(long)b * NW + w] = (hist[w] + 3) & ~3;
+    for (int w = t; w < NW; w += kAttThreads) {
+        count[(long)b * NW + w] = hist[w] + ((hist[NW + w] + 3) & ~3);
+        pure[(long)b * NW + w] = hist[w];
+    }
     if (t == 0) bmax[b] = __uint_as_float(smax);
 }
 
 // Scatter the entries into their buckets (start[] from the exclusive scan of the counts) and write the
-// padding.  The order inside a bucket is whatever the LDS cursors give; the sums do not depend on it.
+// padding.  The groups are those of k_win_count; a group reserves its pure and mixed slots with one
+// LDS atomic each, so its pure quads are row-pure.  The order of the groups inside a bucket is whatever
+// the cursors give; the sums do not depend on it.
 __global__ __launch_bounds__(kAttThreads) void k_win_fill(const int64_t *__restrict__ rowP,
                                                           const uint32_t *__restrict__ colP,
                                                           const float *__restrict__ val, int N, int R, int NW,
-                                                          const int *__restrict__ start, uint32_t *__restrict__ key,
+                                                          const int *__restrict__ start,
+                                                          const int *__restrict__ pure, uint32_t *__restrict__ key,
                                                           float *__restrict__ wval) {
-    extern __shared__ int cur[];
+    extern __shared__ int cur[];  // [0, NW): pure cursors, [NW, 2 NW): mixed cursors
     const int b = blockIdx.x, t = threadIdx.x, lane = t % kWave, wv = t / kWave;
     const int r0 = b * R, r1 = min(N, r0 + R);
-    for (int w = t; w < NW; w += kAttThreads) cur[w] = 0;
+    for (int w = t; w < 2 * NW; w += kAttThreads) cur[w] = 0;
     __syncthreads();
     const int *st = start + (long)b * NW;
+    const int *pu = pure + (long)b * NW;
+    const unsigned long long below = (1ull << lane) - 1ull;
     for (int r = r0 + wv; r < r1; r += kAttThreads / kWave) {
-        for (long k = rowP[r] + lane; k < rowP[r + 1]; k += kWave) {
-            const uint32_t j = colP[k];
-            const int w = (int)(j / kWin);
-            const int at = st[w] + atomicAdd(&cur[w], 1);
-            key[at] = (uint32_t)(r - r0) << 16 | (j - (uint32_t)w * kWin);
-            wval[at] = val[k];
+        const long k1 = rowP[r + 1];
+        for (long k0 = rowP[r]; k0 < k1; k0 += kWave) {
+            const long k = k0 + lane;
+            const bool valid = k < k1;
+            const uint32_t j = valid ? colP[k] : 0u;
+            const int w = valid ? (int)(j / kWin) : -1;
+            window_groups(valid, w, [&](int wl, unsigned long long m, int leader) {
+                const int n = __popcll(m), np = n & ~3;
+                int bp = 0, bm = 0;
+                if (lane == leader) {
+                    bp = atomicAdd(&cur[wl], np);
+                    bm = atomicAdd(&cur[NW + wl], n & 3);
+                }
+                bp = __shfl(bp, leader, kWave);
+                bm = __shfl(bm, leader, kWave);
+                if (valid && w == wl) {
+                    const int rank = __popcll(m & below);
+                    const int at = rank < np ? st[wl] + bp + rank : st[wl] + pu[wl] + bm + (rank - np);
+                    key[at] = (uint32_t)(r - r0) << 16 | (j - (uint32_t)wl * kWin);
+                    wval[at] = val[k];
+                }
+            });
         }
     }
     __syncthreads();
     for (int w = t; w < NW; w += kAttThreads)
-        for (int at = st[w] + cur[w]; at < st[w + 1]; at++) {
+        for (int at = st[w] + pu[w] + cur[NW + w]; at < st[w + 1]; at++) {
             key[at] = 0u;
             wval[at] = 0.f;
         }
@@ -234,6 +290,39 @@ __global__ __launch_bounds__(kAttThreads) void k_win_fill(const int64_t *__restr
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
+// Four entries of one row (a quad of the pure section).
+template <int D>
+__device__ __forceinline__ void attr_quad_pure(const u32x4 &kq, const f32x4 &vq4, const float *yr, const float *yw,
+                                               unsigned long long *acc, float inv_df, int df_one, double S) {
+    const uint32_t kk[4] = {kq.x, kq.y, kq.z, kq.w};
+    const float v[4] = {vq4.x, vq4.y, vq4.z, vq4.w};
+    const int il = (int)(kk[0] >> 16);
+    float yi[D];
+    long long run[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        yi[d] = yr[il * D + d];
+        run[d] = 0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const int jl = (int)(kk[u] & 0xffffu);
+        float diff[D], d2 = 0.f;
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            diff[d] = yi[d] - yw[jl * D + d];
+            d2 += diff[d] * diff[d];
+        }
+        const float q = __builtin_amdgcn_rcpf(df_one ? 1.f + d2 : 1.f + d2 * inv_df);
+        const float vq = v[u] * q;
+#pragma unroll
+        for (int d = 0; d < D; d++) run[d] += fix_rn_fma((double)(vq * diff[d]), S);
+    }
+#pragma unroll
+    for (int d = 0; d < D; d++) atomicAdd(&acc[il * D + d], (unsigned long long)run[d]);
+}
+
+// Four consecutive entries of the mixed section (any rows).
 template <int D>
 __device__ __forceinline__ void attr_quad(const u32x4 &kq, const f32x4 &vq4, const float *yr, const float *yw,
                                           unsigned long long *acc, float inv_df, int df_one, double S) {
@@ -273,6 +362,7 @@ __device__ __forceinline__ void attr_quad(const u32x4 &kq, const f32x4 &vq4, con
 
 template <int D>
 __global__ __launch_bounds__(kAttThreads) void k_attr_win(const int *__restrict__ start,
+                                                          const int *__restrict__ pure,
                                                           const uint32_t *__restrict__ key,
                                                           const float *__restrict__ wval,
                                                           const float *__restrict__ bmax,
@@ -300,6 +390,7 @@ __global__ __launch_bounds__(kAttThreads) void k_attr_win(const int *__restrict_
     if (bound > 0.0) (void)frexp(bound, &e);  // bound < 2^e
     const double S = ldexp(1.0, 50 - e), invS = ldexp(1.0, e - 50);
     const int *st = start + (long)b * NW;
+    const int *pu = pure + (long)b * NW;
     const u32x4 *K4 = reinterpret_cast<const u32x4 *>(key);
     const f32x4 *V4 = reinterpret_cast<const f32x4 *>(wval);
     int w = 0;
@@ -316,7 +407,7 @@ __global__ __launch_bounds__(kAttThreads) void k_attr_win(const int *__restrict_
     };
     if (w < NW) fetch(w);
     while (w < NW) {
-        const int q0 = st[w] >> 2, q1 = st[w + 1] >> 2;
+        const int q0 = st[w] >> 2, qm = (st[w] + pu[w]) >> 2, q1 = st[w + 1] >> 2;
         int wn = w + 1;
         while (wn < NW && st[wn] == st[wn + 1]) wn++;
         __syncthreads();  // the previous window's readers are done
@@ -334,8 +425,12 @@ __global__ __launch_bounds__(kAttThreads) void k_attr_win(const int *__restrict_
                 kb = __builtin_nontemporal_load(&K4[qq + kAttThreads]);
                 vb = __builtin_nontemporal_load(&V4[qq + kAttThreads]);
             }
-            attr_quad<D>(ka, va, yr, yw, acc, inv_df, df_one, S);
-            if (two) attr_quad<D>(kb, vb, yr, yw, acc, inv_df, df_one, S);
+            if (qq < qm) attr_quad_pure<D>(ka, va, yr, yw, acc, inv_df, df_one, S);
+            else attr_quad<D>(ka, va, yr, yw, acc, inv_df, df_one, S);
+            if (two) {
+                if (qq + kAttThreads < qm) attr_quad_pure<D>(kb, vb, yr, yw, acc, inv_df, df_one, S);
+                else attr_quad<D>(kb, vb, yr, yw, acc, inv_df, df_one, S);
+            }
         }
         w = wn;
     }
