@@ -46,6 +46,7 @@ struct Geo {
     double h;        // node spacing (1 / p) * bw0
     double sp[kMaxP];   // in-box node positions (nbodyfft.cpp:30-34)
     double den[kMaxP];  // Lagrange denominators (nbodyfft.cpp:313-321)
+    float rdenf[kMaxP]; // 1 / den rounded once to FP32 (the FP32 form multiplies instead of dividing)
     int nb;          // boxes per dimension
     int n1d;         // nodes per dimension = p * nb
     int Mp;          // FFT length (2^a 3^b >= 2 n1d - 1)
@@ -137,6 +138,7 @@ __host__ __device__ inline void fill_spacings(Geo &g, int p) {
         g.den[i] = 1;
         for (int j = 0; j < p; j++)
             if (i != j) g.den[i] *= g.sp[i] - g.sp[j];
+        g.rdenf[i] = (float)(1.0 / g.den[i]);
     }
 }
 

@@ -105,6 +105,7 @@ struct fitsne_ctx {
     // second stream of the host-planned FP32 step: the attractive term runs on it beside the grid stages
     hipStream_t astream = nullptr;
     hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+    hipEvent_t geo_ev = nullptr, kern_ev = nullptr;  // kernel spectra on astream: after the geometry, before cols
     bool join_pending = false;
 
     int N = 0;
@@ -480,7 +481,7 @@ static int live_record(fitsne_ctx *c, const DevGeo **out) {
 // FP32: the windowed kernel reads the P multiplier from the step record (scale is for the CSR kernel)
 // and skips the step when the geometry record dg says so.
 template <typename TV, int D>
-static int launch_attractive(fitsne_ctx *c, TV scale, TV *pos, const DevGeo *dg = nullptr) {
+static int launch_attractive(fitsne_ctx *c, TV scale, TV *pos, const DevGeo *dg = nullptr, bool captured = false) {
     const TV inv_df = (TV)(1.0 / c->df);
     const int df1 = c->df == 1.0;
     if constexpr (std::is_same<TV, float>::value) {
@@ -491,7 +492,8 @@ static int launch_attractive(fitsne_ctx *c, TV scale, TV *pos, const DevGeo *dg 
                    c->wstart.as<const int>(), c->wpure.as<const int>(), c->wkey.as<const uint32_t>(),
                    c->wval.as<const float>(),
                    c->wbmax.as<const float>(), c->Y.as<const float>(), c->N, c->win_R, c->win_NW, inv_df, df1,
-                   (float)(0.5 * std::sqrt(c->df)), c->dstep.as<const StepDev>(), dg, pos);
+                   (float)(0.5 * std::sqrt(c->df)), c->dstep.as<const StepDev>(), captured ? -1.f : (float)scale,
+                   dg, pos);
             return FITSNE_OK;
         }
     }
@@ -527,28 +529,44 @@ struct StageDims {
 };
 
 // Spread, FFT convolution, Z, interpolation (+ update, zeroMean) of the FP32 form, reading geometry,
-// plan and step parameters from the context's device records.
+// plan and step parameters from the context's device records.  side: the kernel spectra go on the
+// second stream (behind the attractive kernel there), after the geometry record, so that they run
+// beside the spread and the forward row pass; the column pass waits for them.
 template <int D>
-static int launch_f32_stages(fitsne_ctx *c, const cx<float> *tw, const StageDims &sd, bool kern, int mode) {
+static int launch_f32_stages(fitsne_ctx *c, const cx<float> *tw, const StageDims &sd, bool kern, int mode,
+                             bool side) {
     const int N = c->N, p = c->p;
     const bool df1 = c->df == 1.0;
     const DevGeo *dg = c->dgeo.as<const DevGeo>();
-    launch(c, "spread_f32", pick_spread_f32<D>(p), dim3((N + kSpreadPts - 1) / kSpreadPts), dim3(kSpreadThreads), 0,
-           c->Y.as<const float>(), N, dg, c->fixgrid.as<unsigned long long>());
     const int kind_q = df1 ? FITSNE_KERNEL_SQUARED_CAUCHY : FITSNE_KERNEL_SQUARED_GENERAL;
     const int kind_1 = FITSNE_KERNEL_GENERAL;
+    auto kernel_spectra = [&]() {
+        const size_t Mp = (size_t)sd.Mp;
+        launch(c, "kern_rows", k_kern_rows<float, kKernThreads, kKernTot>, dim3(sd.gr_rows), dim3(kKernThreads),
+               sizeof(cx<float>) * 2 * Mp, dg, tw, kind_q, kind_1, c->df, c->Trow.as<cx<float>>());
+        launch(c, "kern_cols", k_kern_cols<float, kKernThreads, kKernTot>, dim3(sd.gr_cols), dim3(kKernThreads),
+               sizeof(cx<float>) * 2 * Mp, dg, tw, c->Trow.as<const cx<float>>(), 1, c->KQ.as<float>(),
+               c->K1.as<float>(), c->KS.as<float>(), c->KT.as<float>());
+    };
+    const bool kside = D == 2 && kern && side;
+    if (kside) {
+        HIPTRY(hipEventRecord(c->geo_ev, c->stream));
+        HIPTRY(hipStreamWaitEvent(c->astream, c->geo_ev, 0));
+        hipStream_t main = c->stream;
+        c->stream = c->astream;
+        kernel_spectra();
+        c->stream = main;
+        HIPTRY(hipEventRecord(c->kern_ev, c->astream));
+    }
+    launch(c, "spread_f32", pick_spread_f32<D>(p), dim3((N + kSpreadPts - 1) / kSpreadPts), dim3(kSpreadThreads), 0,
+           c->Y.as<const float>(), N, dg, c->fixgrid.as<unsigned long long>());
     if constexpr (D == 2) {
         const size_t Mp = (size_t)sd.Mp;
-        if (kern) {
-            launch(c, "kern_rows", k_kern_rows<float, kKernThreads, kKernTot>, dim3(sd.gr_rows), dim3(kKernThreads),
-                   sizeof(cx<float>) * 2 * Mp, dg, tw, kind_q, kind_1, c->df, c->Trow.as<cx<float>>());
-            launch(c, "kern_cols", k_kern_cols<float, kKernThreads, kKernTot>, dim3(sd.gr_cols), dim3(kKernThreads),
-                   sizeof(cx<float>) * 2 * Mp, dg, tw, c->Trow.as<const cx<float>>(), 1, c->KQ.as<float>(),
-                   c->K1.as<float>(), c->KS.as<float>(), c->KT.as<float>());
-        }
+        if (kern && !kside) kernel_spectra();
         launch(c, "rows_fwd", k_rows_fwd<float, kRowThreads, kRowTot>, dim3(sd.gr_rows), dim3(kRowThreads),
                sizeof(cx<float>) * 3 * Mp, dg, tw, (const float *)nullptr, 4, 4, c->spec.as<cx<float>>(),
                c->fixgrid.as<unsigned long long>());
+        if (kside) HIPTRY(hipStreamWaitEvent(c->stream, c->kern_ev, 0));
         launch(c, "cols_f32", k_cols_f32<kColThreads, kColTot32>, dim3(sd.gr_cols), dim3(kColThreads),
                sizeof(cx<float>) * 5 * Mp, dg, tw, c->spec.as<const cx<float>>(), c->KQ.as<const float>(),
                c->K1.as<const float>(), c->KS.as<const float>(), c->KT.as<const float>(), c->ospec.as<cx<float>>(),
@@ -631,7 +649,7 @@ static int ensure_f32_buffers(fitsne_ctx *c, int n1d, int Mp) {
 template <int D>
 static int pipeline_f32(fitsne_ctx *c, int mode, const fitsne_step_params *sp) {
     TRY(ensure(c->pos, sizeof(float) * (size_t)c->N * D));
-    TRY(put_step(c, mode, sp));
+    TRY(ensure(c->dstep, sizeof(StepDev)));
     const DevGeo *live;
     TRY(live_record(c, &live));
     // the attractive term on the second stream (after everything queued so far), beside the grid
@@ -647,6 +665,9 @@ static int pipeline_f32(fitsne_ctx *c, int mode, const fitsne_step_params *sp) {
     }
     HIPTRY(hipEventRecord(c->join_ev, c->astream));
     c->join_pending = true;
+    // the step record (read by the interpolation; the attractive kernel took the exaggeration as an
+    // argument) is written beside the attractive kernel
+    TRY(put_step(c, mode, sp));
     double mn, mx;
     TRY(current_minmax<float>(c, &mn, &mx));
     Geo g;
@@ -663,7 +684,7 @@ static int pipeline_f32(fitsne_ctx *c, int mode, const fitsne_step_params *sp) {
         c->kc_nb = g.nb;
     }
     const StageDims sd{g.Mp, 8 * ((g.n1d + 7) / 8), 8 * ((g.H + 7) / 8), blocks_for(g.n1d)};
-    TRY(launch_f32_stages<D>(c, tw, sd, kern, mode));
+    TRY(launch_f32_stages<D>(c, tw, sd, kern, mode, true));
     if (mode == 1) {
         const int cblk = std::min(blocks_for((long)c->N * D), 1024);
         launch(c, "minmax_final", k_minmax_final<float>, dim3(1), dim3(kWave), 0, c->mmpart.as<const double>(), cblk,
@@ -692,11 +713,11 @@ static int launch_step_dev(fitsne_ctx *c) {
            (long)c->N * D, D, c->p, c->ipi, c->mni, n1d_max, mp_max, c->dgeo.as<DevGeo>());
     launch(c, "twiddles", k_twiddles_dev<float>, dim3(8), dim3(256), 0, c->dgeo.as<const DevGeo>(),
            c->dtw.as<cx<float>>());
-    TRY((launch_attractive<float, D>(c, 1.f, c->pos.as<float>(), c->dgeo.as<const DevGeo>())));
+    TRY((launch_attractive<float, D>(c, 1.f, c->pos.as<float>(), c->dgeo.as<const DevGeo>(), true)));
     const int H_max = mp_max / 2 + 1;
     const StageDims sd{mp_max, std::min(8 * ((n1d_max + 7) / 8), 512), std::min(8 * ((H_max + 7) / 8), 512),
                        std::min(blocks_for(n1d_max), 64)};
-    TRY(launch_f32_stages<D>(c, c->dtw.as<const cx<float>>(), sd, D == 2, 1));
+    TRY(launch_f32_stages<D>(c, c->dtw.as<const cx<float>>(), sd, D == 2, 1, false));
     return FITSNE_OK;
 }
 
@@ -1124,6 +1145,8 @@ extern "C" int fitsne_ctx_create(fitsne_ctx **out, int device, int precision, in
         hipStreamCreateWithFlags(&c->astream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->geo_ev, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->kern_ev, hipEventDisableTiming) != hipSuccess ||
         hipHostMalloc((void **)&c->h_mm, sizeof(double) * 4, hipHostMallocDefault) != hipSuccess ||
         hipEventCreateWithFlags(&c->mm_event, hipEventDisableTiming) != hipSuccess) {
         delete c;
@@ -1159,6 +1182,8 @@ extern "C" int fitsne_ctx_destroy(fitsne_ctx *c) {
     if (c->mm_event) (void)hipEventDestroy(c->mm_event);
     if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
     if (c->join_ev) (void)hipEventDestroy(c->join_ev);
+    if (c->geo_ev) (void)hipEventDestroy(c->geo_ev);
+    if (c->kern_ev) (void)hipEventDestroy(c->kern_ev);
     if (c->astream) (void)hipStreamDestroy(c->astream);
     if (c->h_mm) (void)hipHostFree(c->h_mm);
     if (c->bh_pin) (void)hipHostFree(c->bh_pin);

@@ -368,11 +368,11 @@ __global__ __launch_bounds__(kAttThreads) void k_attr_win(const int *__restrict_
                                                           const float *__restrict__ bmax,
                                                           const float *__restrict__ Y, int N, int R, int NW,
                                                           float inv_df, int df_one, float mdf,
-                                                          const StepDev *__restrict__ sd,
+                                                          const StepDev *__restrict__ sd, float exag,
                                                           const DevGeo *__restrict__ dg, float *__restrict__ pos) {
     constexpr int WL = kWin * D / kAttThreads;  // window floats per thread
     if (!dg->live) return;
-    const float scale = sd->exag;
+    const float scale = exag >= 0.f ? exag : sd->exag;  // exag < 0: the step record's (captured steps)
     __shared__ float yw[kWin * D];
     __shared__ float yr[kAttRows * D];
     __shared__ unsigned long long acc[kAttRows * D];
@@ -475,7 +475,19 @@ template <typename TV>
 __global__ void k_minmax_final(const double *__restrict__ part, int nblk, const TV *__restrict__ Y, long first,
                                double *__restrict__ out) {
     double mx = -INFINITY, mn = INFINITY;
-    for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+    const int T = blockDim.x;
+    int b = threadIdx.x;
+    for (; b + 7 * T < nblk; b += 8 * T) {  // loads issued eight at a time
+        double2 v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = reinterpret_cast<const double2 *>(part)[b + u * T];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            mx = fmax(mx, v[u].x);
+            mn = fmin(mn, v[u].y);
+        }
+    }
+    for (; b < nblk; b += T) {
         mx = fmax(mx, part[2 * b]);
         mn = fmin(mn, part[2 * b + 1]);
     }
@@ -759,7 +771,7 @@ __global__ __launch_bounds__(kSpreadThreads) void k_spread_f32(const float *__re
 #pragma unroll
                 for (int c = 0; c < P; c++)
                     if (c != a) v *= dx[c];
-                Lx[a] = v / (float)g.den[a];
+                Lx[a] = v * g.rdenf[a];
             }
             if constexpr (D == 2) {
                 double ty;
@@ -773,7 +785,7 @@ __global__ __launch_bounds__(kSpreadThreads) void k_spread_f32(const float *__re
 #pragma unroll
                     for (int c = 0; c < P; c++)
                         if (c != a) v *= dy[c];
-                    Ly[a] = v / (float)g.den[a];
+                    Ly[a] = v * g.rdenf[a];
                 }
 #pragma unroll
                 for (int a = 0; a < P; a++)
@@ -1387,8 +1399,16 @@ __global__ __launch_bounds__(kThreads) void k_interp_f32(float *__restrict__ Y, 
 #pragma unroll
     for (int d = 0; d < D; d++) ysum[d] = 0.0;
     if (i < N) {
-        float y[D];
+        // every per-point load first, so that they are in flight together
+        const bool mom = up.mode != 0 && !up.plain_descent;
+        float y[D], pi[D], u[D], gn[D];
         load_pt<float, D>(Y, i, y);
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            pi[d] = pos[(long)i * D + d];
+            u[d] = mom ? uY[(long)i * D + d] : 0.f;
+            gn[d] = mom ? gains[(long)i * D + d] : 0.f;
+        }
         const float zinv = (float)zbuf[1];
         float Lx[P], ex[P], Ly[P], ey[P];
         int bx, by = 0;
@@ -1404,7 +1424,7 @@ __global__ __launch_bounds__(kThreads) void k_interp_f32(float *__restrict__ Y, 
 #pragma unroll
                 for (int c = 0; c < P; c++)
                     if (c != a) v *= dt[c];
-                Lx[a] = v / (float)g.den[a];
+                Lx[a] = v * g.rdenf[a];
                 ex[a] = dt[a] * (float)g.bw0;
             }
         }
@@ -1420,7 +1440,7 @@ __global__ __launch_bounds__(kThreads) void k_interp_f32(float *__restrict__ Y, 
 #pragma unroll
                 for (int c = 0; c < P; c++)
                     if (c != a) v *= dt[c];
-                Ly[a] = v / (float)g.den[a];
+                Ly[a] = v * g.rdenf[a];
                 ey[a] = dt[a] * (float)g.bw0;
             }
         }
@@ -1448,7 +1468,7 @@ __global__ __launch_bounds__(kThreads) void k_interp_f32(float *__restrict__ Y, 
         }
         float dy[D];
 #pragma unroll
-        for (int d = 0; d < D; d++) dy[d] = pos[(long)i * D + d] - neg[d] * zinv;
+        for (int d = 0; d < D; d++) dy[d] = pi[d] - neg[d] * zinv;
         if (up.mode == 0) {
 #pragma unroll
             for (int d = 0; d < D; d++) dC[(long)i * D + d] = dy[d];
@@ -1458,11 +1478,9 @@ __global__ __launch_bounds__(kThreads) void k_interp_f32(float *__restrict__ Y, 
 #pragma unroll
                 for (int d = 0; d < D; d++) yn[d] = y[d] - dy[d];
             } else {
-                float u[D], gn[D], nrm = 0.f;
+                float nrm = 0.f;
 #pragma unroll
                 for (int d = 0; d < D; d++) {
-                    u[d] = uY[(long)i * D + d];
-                    gn[d] = gains[(long)i * D + d];
                     const float sd = dy[d] == 0.f ? 0.f : (dy[d] < 0.f ? -1.f : 1.f);
                     const float su = u[d] == 0.f ? 0.f : (u[d] < 0.f ? -1.f : 1.f);
                     gn[d] = sd != su ? gn[d] + 0.2f : gn[d] * 0.8f;
@@ -1506,9 +1524,19 @@ __global__ void k_mean_final(const DevGeo *__restrict__ dg, const double *__rest
                              double N, double *__restrict__ mean) {
     if (!dg->live) return;
     __shared__ double sh[kThreads / kWave];
+    const int T = blockDim.x;
     for (int d = 0; d < D; d++) {
+        // each thread adds part[t], part[t + T], ... in that order; the loads are issued eight at a time
         double s = 0;
-        for (int b = threadIdx.x; b < nblk; b += blockDim.x) s += part[b * D + d];
+        int b = threadIdx.x;
+        for (; b + 7 * T < nblk; b += 8 * T) {
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) v[u] = part[(long)(b + u * T) * D + d];
+#pragma unroll
+            for (int u = 0; u < 8; u++) s += v[u];
+        }
+        for (; b < nblk; b += T) s += part[(long)b * D + d];
         s = block_sum(s, sh);
         if (threadIdx.x == 0) mean[d] = s / N;
         __syncthreads();
