@@ -117,7 +117,17 @@ __device__ __forceinline__ void dft_small(cx<T> (&v)[R], bool inv) {
     }
 }
 
-template <typename T, int R, int NTHR, int MAXPER>
+// LDS position of element e of the lines.  With PAD one slot is skipped after every 16 elements: the
+// first stage writes R consecutive elements per lane with lanes R elements apart, which without the
+// skip puts every lane of a write in the same bank (R = 16); with it they spread over the banks.  All
+// other accesses are consecutive across lanes either way.
+template <bool PAD>
+__host__ __device__ __forceinline__ int lp(int e) { return PAD ? e + (e >> 4) : e; }
+// LDS elements taken by n elements of lines
+template <bool PAD>
+__host__ __device__ __forceinline__ int lspan(int n) { return PAD ? n + (n >> 4) + 1 : n; }
+
+template <typename T, int R, int NTHR, int MAXPER, bool PAD>
 __device__ __forceinline__ void fft_stage(cx<T> *buf, int n, int lines, int stride, int Ns,
                                           const cx<T> *__restrict__ tw, bool inv) {
     const int nbf = n / R;
@@ -129,9 +139,9 @@ __device__ __forceinline__ void fft_stage(cx<T> *buf, int n, int lines, int stri
         const int q = threadIdx.x + c * NTHR;
         if (q < total) {
             const int l = q / nbf, j = q - l * nbf;
-            const cx<T> *src = buf + l * stride + j;
+            const int src = l * stride + j;
 #pragma unroll
-            for (int r = 0; r < R; r++) v[c][r] = src[r * nbf];
+            for (int r = 0; r < R; r++) v[c][r] = buf[lp<PAD>(src + r * nbf)];
         }
     }
     __syncthreads();
@@ -150,16 +160,16 @@ __device__ __forceinline__ void fft_stage(cx<T> *buf, int n, int lines, int stri
                 }
             }
             dft_small<R, T>(v[c], inv);
-            cx<T> *dst = buf + l * stride + (j - jm) * R + jm;
+            const int dst = l * stride + (j - jm) * R + jm;
 #pragma unroll
-            for (int r = 0; r < R; r++) dst[r * Ns] = v[c][r];
+            for (int r = 0; r < R; r++) buf[lp<PAD>(dst + r * Ns)] = v[c][r];
         }
     }
     __syncthreads();
 }
 
 // Transform `lines` lines of length plan.n in place.  Requires plan.n * lines <= MAXTOT.
-template <typename T, int NTHR, int MAXTOT>
+template <typename T, int NTHR, int MAXTOT, bool PAD = false>
 __device__ void fft_lines(cx<T> *buf, const FftPlan &plan, int lines, int stride, const cx<T> *__restrict__ tw,
                           bool inv) {
     constexpr int P16 = (MAXTOT / 16 + NTHR - 1) / NTHR;
@@ -172,12 +182,12 @@ __device__ void fft_lines(cx<T> *buf, const FftPlan &plan, int lines, int stride
     for (int s = 0; s < plan.nst; s++) {
         const int R = plan.rad[s];
         switch (R) {
-        case 16: fft_stage<T, 16, NTHR, P16>(buf, plan.n, lines, stride, Ns, tw, inv); break;
-        case 8: fft_stage<T, 8, NTHR, P8>(buf, plan.n, lines, stride, Ns, tw, inv); break;
-        case 4: fft_stage<T, 4, NTHR, P4>(buf, plan.n, lines, stride, Ns, tw, inv); break;
-        case 2: fft_stage<T, 2, NTHR, P2>(buf, plan.n, lines, stride, Ns, tw, inv); break;
-        case 9: fft_stage<T, 9, NTHR, P9>(buf, plan.n, lines, stride, Ns, tw, inv); break;
-        default: fft_stage<T, 3, NTHR, P3>(buf, plan.n, lines, stride, Ns, tw, inv); break;
+        case 16: fft_stage<T, 16, NTHR, P16, PAD>(buf, plan.n, lines, stride, Ns, tw, inv); break;
+        case 8: fft_stage<T, 8, NTHR, P8, PAD>(buf, plan.n, lines, stride, Ns, tw, inv); break;
+        case 4: fft_stage<T, 4, NTHR, P4, PAD>(buf, plan.n, lines, stride, Ns, tw, inv); break;
+        case 2: fft_stage<T, 2, NTHR, P2, PAD>(buf, plan.n, lines, stride, Ns, tw, inv); break;
+        case 9: fft_stage<T, 9, NTHR, P9, PAD>(buf, plan.n, lines, stride, Ns, tw, inv); break;
+        default: fft_stage<T, 3, NTHR, P3, PAD>(buf, plan.n, lines, stride, Ns, tw, inv); break;
         }
         Ns *= R;
     }

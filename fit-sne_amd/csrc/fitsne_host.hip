@@ -542,10 +542,10 @@ static int launch_f32_stages(fitsne_ctx *c, const cx<float> *tw, const StageDims
     const int kind_1 = FITSNE_KERNEL_GENERAL;
     auto kernel_spectra = [&]() {
         const size_t Mp = (size_t)sd.Mp;
-        launch(c, "kern_rows", k_kern_rows<float, kKernThreads, kKernTot>, dim3(sd.gr_rows), dim3(kKernThreads),
-               sizeof(cx<float>) * 2 * Mp, dg, tw, kind_q, kind_1, c->df, c->Trow.as<cx<float>>());
-        launch(c, "kern_cols", k_kern_cols<float, kKernThreads, kKernTot>, dim3(sd.gr_cols), dim3(kKernThreads),
-               sizeof(cx<float>) * 2 * Mp, dg, tw, c->Trow.as<const cx<float>>(), 1, c->KQ.as<float>(),
+        launch(c, "kern_rows", k_kern_rows<float, kKernThreads, kKernTot, true>, dim3(sd.gr_rows), dim3(kKernThreads),
+               sizeof(cx<float>) * (lspan<true>((int)Mp) + Mp), dg, tw, kind_q, kind_1, c->df, c->Trow.as<cx<float>>());
+        launch(c, "kern_cols", k_kern_cols<float, kKernThreads, kKernTot, true>, dim3(sd.gr_cols), dim3(kKernThreads),
+               sizeof(cx<float>) * (lspan<true>((int)Mp) + Mp), dg, tw, c->Trow.as<const cx<float>>(), 1, c->KQ.as<float>(),
                c->K1.as<float>(), c->KS.as<float>(), c->KT.as<float>());
     };
     const bool kside = D == 2 && kern && side;
@@ -563,18 +563,18 @@ static int launch_f32_stages(fitsne_ctx *c, const cx<float> *tw, const StageDims
     if constexpr (D == 2) {
         const size_t Mp = (size_t)sd.Mp;
         if (kern && !kside) kernel_spectra();
-        launch(c, "rows_fwd", k_rows_fwd<float, kRowThreads, kRowTot>, dim3(sd.gr_rows), dim3(kRowThreads),
-               sizeof(cx<float>) * 3 * Mp, dg, tw, (const float *)nullptr, 4, 4, c->spec.as<cx<float>>(),
+        launch(c, "rows_fwd", k_rows_fwd<float, kRowThreads, kRowTot, true>, dim3(sd.gr_rows), dim3(kRowThreads),
+               sizeof(cx<float>) * (lspan<true>(2 * (int)Mp) + Mp), dg, tw, (const float *)nullptr, 4, 4, c->spec.as<cx<float>>(),
                c->fixgrid.as<unsigned long long>());
         if (kside) HIPTRY(hipStreamWaitEvent(c->stream, c->kern_ev, 0));
         launch(c, "cols_f32", k_cols_f32<kColThreads, kColTot32>, dim3(sd.gr_cols), dim3(kColThreads),
-               sizeof(cx<float>) * 5 * Mp, dg, tw, c->spec.as<const cx<float>>(), c->KQ.as<const float>(),
+               sizeof(cx<float>) * (lspan<true>(4 * (int)Mp) + Mp), dg, tw, c->spec.as<const cx<float>>(), c->KQ.as<const float>(),
                c->K1.as<const float>(), c->KS.as<const float>(), c->KT.as<const float>(), c->ospec.as<cx<float>>(),
                c->zpart.as<double>());
         launch(c, "zfinal", k_zfinal, dim3(1), dim3(kThreads), 0, dg, c->zpart.as<const double>(), (int)df1, (double)N,
                c->zbuf.as<double>());
-        launch(c, "rows_inv", k_rows_inv<float, kRowThreads, kRowTot>, dim3(sd.gr_rows), dim3(kRowThreads),
-               sizeof(cx<float>) * 3 * Mp, dg, tw, c->ospec.as<const cx<float>>(), 3, c->gout.as<float>(), 4);
+        launch(c, "rows_inv", k_rows_inv<float, kRowThreads, kRowTot, true>, dim3(sd.gr_rows), dim3(kRowThreads),
+               sizeof(cx<float>) * (lspan<true>(2 * (int)Mp) + Mp), dg, tw, c->ospec.as<const cx<float>>(), 3, c->gout.as<float>(), 4);
     } else {
         launch(c, "grid_to_f32", k_grid_to_f32, dim3(sd.gr_nodes), dim3(kThreads), 0, dg,
                c->fixgrid.as<unsigned long long>(), c->gin.as<float4>());

@@ -894,7 +894,7 @@ __global__ __launch_bounds__(kThreads) void k_grid_to_f32(const DevGeo *__restri
 // K_1(a, |b|) over b in (-n1d, n1d) wrapped into Mp, packed as one complex line; its DFT is
 // T_q + i T_1 (both real).  Trow[k][a] = (T_q(a, k), T_1(a, k)), k in [0, H).  Runs only when the
 // step marks the spectra stale.
-template <typename T, int NTHR, int MAXTOT>
+template <typename T, int NTHR, int MAXTOT, bool PAD = false>
 __global__ __launch_bounds__(NTHR) void k_kern_rows(const DevGeo *__restrict__ dg, const cx<T> *__restrict__ tw,
                                                     int kind_q, int kind_1, double df, cx<T> *__restrict__ Trow) {
     if (!dg->live || !dg->kspec_stale) return;
@@ -902,25 +902,45 @@ __global__ __launch_bounds__(NTHR) void k_kern_rows(const DevGeo *__restrict__ d
     extern __shared__ unsigned char smem_raw[];
     cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
     const int Mp = g.Mp, n1d = g.n1d;
-    const cx<T> *twl = stage_twiddles(buf + Mp, tw, Mp);
+    const cx<T> *twl = stage_twiddles(buf + lspan<PAD>(Mp), tw, Mp);
     const int npad = 8 * ((n1d + 7) / 8);
     for (int slot = blockIdx.x; slot < npad; slot += gridDim.x) {
         const int a = xcd_slot_row(slot, n1d);
         if (a >= n1d) continue;
         const double du = a * g.h;
-        for (int b = threadIdx.x; b < Mp; b += NTHR) buf[b] = cx<T>{(T)0, (T)0};
+        for (int b = threadIdx.x; b < Mp; b += NTHR) buf[lp<PAD>(b)] = cx<T>{(T)0, (T)0};
         __syncthreads();
         for (int b = threadIdx.x; b < n1d; b += NTHR) {
             const double dv = b * g.h;
             const double r2 = du * du + dv * dv;
             cx<T> val{kernel_value<T>(kind_q, r2, df), kernel_value<T>(kind_1, r2, df)};
-            buf[b] = val;
-            if (b > 0) buf[Mp - b] = val;
+            buf[lp<PAD>(b)] = val;
+            if (b > 0) buf[lp<PAD>(Mp - b)] = val;
         }
         __syncthreads();
-        fft_lines<T, NTHR, MAXTOT>(buf, dg->pl, 1, Mp, twl, false);
-        for (int k = threadIdx.x; k < g.H; k += NTHR) Trow[(long)k * n1d + a] = buf[k];
+        fft_lines<T, NTHR, MAXTOT, PAD>(buf, dg->pl, 1, Mp, twl, false);
+        for (int k = threadIdx.x; k < g.H; k += NTHR) Trow[(long)k * n1d + a] = buf[lp<PAD>(k)];
         __syncthreads();
+    }
+}
+
+// buf[q] = f(q) for q < total, the loads of a thread issued eight at a time before their LDS writes, so
+// a workgroup waits for memory once per eight elements per thread instead of once per element.
+template <int NTHR, bool PAD, typename V, typename F>
+__device__ __forceinline__ void lds_fill(V *buf, int total, F &&f) {
+    constexpr int B = 8;
+    for (int q0 = threadIdx.x; q0 < total; q0 += B * NTHR) {
+        V v[B];
+#pragma unroll
+        for (int s = 0; s < B; s++) {
+            const int q = q0 + s * NTHR;
+            if (q < total) v[s] = f(q);
+        }
+#pragma unroll
+        for (int s = 0; s < B; s++) {
+            const int q = q0 + s * NTHR;
+            if (q < total) buf[lp<PAD>(q)] = v[s];
+        }
     }
 }
 
@@ -931,7 +951,7 @@ __global__ __launch_bounds__(NTHR) void k_kern_rows(const DevGeo *__restrict__ d
 // kernel K_q(a,b) a h is -i S(m,k) and that of K_q(a,b) b h is -i S(k,m) (K_q symmetric in a, b).
 // Stored for m, k in [0, H):  KQ[k][m] = C_q(m,k), K1[k][m] = C_1(m,k), KS[k][m] = S(m,k),
 // KT[k][m] = S(k,m) (written by column m's workgroup).
-template <typename T, int NTHR, int MAXTOT>
+template <typename T, int NTHR, int MAXTOT, bool PAD = false>
 __global__ __launch_bounds__(NTHR) void k_kern_cols(const DevGeo *__restrict__ dg, const cx<T> *__restrict__ tw,
                                                     const cx<T> *__restrict__ Trow, int need_s,
                                                     T *__restrict__ KQ, T *__restrict__ K1, T *__restrict__ KS,
@@ -941,28 +961,25 @@ __global__ __launch_bounds__(NTHR) void k_kern_cols(const DevGeo *__restrict__ d
     extern __shared__ unsigned char smem_raw[];
     cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
     const int Mp = g.Mp, n1d = g.n1d, H = g.H;
-    const cx<T> *twl = stage_twiddles(buf + Mp, tw, Mp);
+    const cx<T> *twl = stage_twiddles(buf + lspan<PAD>(Mp), tw, Mp);
     const int npad = 8 * ((H + 7) / 8);
     for (int slot = blockIdx.x; slot < npad; slot += gridDim.x) {
         const int k = xcd_slot_row(slot, H);
         if (k >= H) continue;
-        for (int a = threadIdx.x; a < Mp; a += NTHR) {
+        lds_fill<NTHR, PAD>(buf, Mp, [&](int a) {
             int as;
             if (a < n1d) as = a;
             else if (a > Mp - n1d) as = a - Mp;
-            else {
-                buf[a] = cx<T>{(T)0, (T)0};
-                continue;
-            }
+            else return cx<T>{(T)0, (T)0};
             const cx<T> t = Trow[(long)k * n1d + (as < 0 ? -as : as)];
             const T w = need_s ? (T)1 + (T)(as * g.h) : (T)1;
-            buf[a] = cx<T>{t.x * w, t.y};
-        }
+            return cx<T>{t.x * w, t.y};
+        });
         __syncthreads();
-        fft_lines<T, NTHR, MAXTOT>(buf, dg->pl, 1, Mp, twl, false);
+        fft_lines<T, NTHR, MAXTOT, PAD>(buf, dg->pl, 1, Mp, twl, false);
         for (int m = threadIdx.x; m < H; m += NTHR) {
-            const cx<T> z = buf[m];
-            const T im_neg = buf[m == 0 ? 0 : Mp - m].y;
+            const cx<T> z = buf[lp<PAD>(m)];
+            const T im_neg = buf[lp<PAD>(m == 0 ? 0 : Mp - m)].y;
             KQ[(long)k * H + m] = z.x;
             K1[(long)k * H + m] = (z.y + im_neg) * (T)0.5;
             if (need_s) {
@@ -980,7 +997,7 @@ __global__ __launch_bounds__(NTHR) void k_kern_cols(const DevGeo *__restrict__ d
 // spectra unpacked from the packed line: A = (Z(k) + conj Z(-k)) / 2, B = -i (Z(k) - conj Z(-k)) / 2).
 // With fixgrid != null the input is the exact int64 grid of k_spread_f32 ([u][v][4] = A, c, b_x, b_y in
 // 2^-38 units, offsets in box widths); it is converted and cleared here (each row read once).
-template <typename T, int NTHR, int MAXTOT>
+template <typename T, int NTHR, int MAXTOT, bool PAD = false>
 __global__ __launch_bounds__(NTHR) void k_rows_fwd(const DevGeo *__restrict__ dg, const cx<T> *__restrict__ tw,
                                                    const T *__restrict__ grid, int sin, int nterms,
                                                    cx<T> *__restrict__ spec,
@@ -991,7 +1008,7 @@ __global__ __launch_bounds__(NTHR) void k_rows_fwd(const DevGeo *__restrict__ dg
     cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
     const int Mp = g.Mp, n1d = g.n1d, H = g.H;
     const int lines = (nterms + 1) / 2;
-    const cx<T> *twl = stage_twiddles(buf + lines * Mp, tw, Mp);
+    const cx<T> *twl = stage_twiddles(buf + lspan<PAD>(lines * Mp), tw, Mp);
     const double inv = 1.0 / kFixScale;
     const double unit[4] = {inv, inv * g.bw0 * g.bw0, inv * g.bw0, inv * g.bw0};
     const long plane = (long)H * n1d;
@@ -999,7 +1016,7 @@ __global__ __launch_bounds__(NTHR) void k_rows_fwd(const DevGeo *__restrict__ dg
     for (int slot = blockIdx.x; slot < npad; slot += gridDim.x) {
         const int u = xcd_slot_row(slot, n1d);
         if (u >= n1d) continue;
-        for (int q = threadIdx.x; q < lines * Mp; q += NTHR) {
+        lds_fill<NTHR, PAD>(buf, lines * Mp, [&](int q) {
             const int l = q / Mp, v = q - l * Mp;
             cx<T> z{(T)0, (T)0};
             if (v < n1d) {
@@ -1015,14 +1032,14 @@ __global__ __launch_bounds__(NTHR) void k_rows_fwd(const DevGeo *__restrict__ dg
                     if (2 * l + 1 < nterms) z.y = src[1];
                 }
             }
-            buf[q] = z;
-        }
+            return z;
+        });
         __syncthreads();
-        fft_lines<T, NTHR, MAXTOT>(buf, dg->pl, lines, Mp, twl, false);
+        fft_lines<T, NTHR, MAXTOT, PAD>(buf, dg->pl, lines, Mp, twl, false);
         for (int q = threadIdx.x; q < lines * H; q += NTHR) {
             const int l = q / H, k = q - l * H;
-            const cx<T> Z = buf[l * Mp + k];
-            const cx<T> Zc = conj(buf[l * Mp + (k == 0 ? 0 : Mp - k)]);
+            const cx<T> Z = buf[lp<PAD>(l * Mp + k)];
+            const cx<T> Zc = conj(buf[lp<PAD>(l * Mp + (k == 0 ? 0 : Mp - k))]);
             spec[(2 * l) * plane + (long)k * n1d + u] = (T)0.5 * (Z + Zc);
             if (2 * l + 1 < nterms) spec[(2 * l + 1) * plane + (long)k * n1d + u] = (T)0.5 * mul_negi(Z - Zc);
         }
@@ -1048,17 +1065,17 @@ __global__ __launch_bounds__(NTHR) void k_cols_f32(const DevGeo *__restrict__ dg
     __shared__ double shz[NTHR / kWave];
     const int Mp = g.Mp, n1d = g.n1d, H = g.H;
     const long plane = (long)H * n1d;
-    const cx<float> *twl = stage_twiddles(buf + 4 * Mp, tw, Mp);
+    const cx<float> *twl = stage_twiddles(buf + lspan<true>(4 * Mp), tw, Mp);
     const int npad = 8 * ((H + 7) / 8);
     for (int slot = blockIdx.x; slot < npad; slot += gridDim.x) {
         const int k = xcd_slot_row(slot, H);
         if (k >= H) continue;
-        for (int q = threadIdx.x; q < 4 * Mp; q += NTHR) {
+        lds_fill<NTHR, true>(buf, 4 * Mp, [&](int q) {
             const int t = q / Mp, u = q - t * Mp;
-            buf[q] = u < n1d ? spec[t * plane + (long)k * n1d + u] : cx<float>{0.f, 0.f};
-        }
+            return u < n1d ? spec[t * plane + (long)k * n1d + u] : cx<float>{0.f, 0.f};
+        });
         __syncthreads();
-        fft_lines<float, NTHR, MAXTOT>(buf, dg->pl, 4, Mp, twl, false);
+        fft_lines<float, NTHR, MAXTOT, true>(buf, dg->pl, 4, Mp, twl, false);
         const double wk = (k == 0 || 2 * k == Mp) ? 1.0 : 2.0;
         double z0 = 0, z1 = 0, z2 = 0, z3 = 0;
         for (int m = threadIdx.x; m < Mp; m += NTHR) {
@@ -1066,11 +1083,12 @@ __global__ __launch_bounds__(NTHR) void k_cols_f32(const DevGeo *__restrict__ dg
             const float sg = m <= Mp / 2 ? 1.f : -1.f;
             const float cq = KQ[(long)k * H + mb], c1 = K1[(long)k * H + mb];
             const float sx = sg * KS[(long)k * H + mb], sy = KT[(long)k * H + mb];
-            const cx<float> A = buf[m], C = buf[Mp + m], BX = buf[2 * Mp + m], BY = buf[3 * Mp + m];
+            const cx<float> A = buf[lp<true>(m)], C = buf[lp<true>(Mp + m)], BX = buf[lp<true>(2 * Mp + m)],
+                            BY = buf[lp<true>(3 * Mp + m)];
             const cx<float> KxA = mul_negi(sx * A), KyA = mul_negi(sy * A);
-            buf[m] = cq * A;
-            buf[Mp + m] = KxA - cq * BX;
-            buf[2 * Mp + m] = KyA - cq * BY;
+            buf[lp<true>(m)] = cq * A;
+            buf[lp<true>(Mp + m)] = KxA - cq * BX;
+            buf[lp<true>(2 * Mp + m)] = KyA - cq * BY;
             const double a2 = (double)A.x * A.x + (double)A.y * A.y;
             z0 += (double)c1 * a2;
             z1 += (double)cq * ((double)C.x * A.x + (double)C.y * A.y);
@@ -1080,10 +1098,10 @@ __global__ __launch_bounds__(NTHR) void k_cols_f32(const DevGeo *__restrict__ dg
             z3 += (double)cq * ((double)BX.x * BX.x + (double)BX.y * BX.y + (double)BY.x * BY.x + (double)BY.y * BY.y);
         }
         __syncthreads();
-        fft_lines<float, NTHR, MAXTOT>(buf, dg->pl, 3, Mp, twl, true);
+        fft_lines<float, NTHR, MAXTOT, true>(buf, dg->pl, 3, Mp, twl, true);
         for (int q = threadIdx.x; q < 3 * n1d; q += NTHR) {
             const int t = q / n1d, u = q - t * n1d;
-            ospec[t * plane + (long)k * n1d + u] = buf[t * Mp + u];
+            ospec[t * plane + (long)k * n1d + u] = buf[lp<true>(t * Mp + u)];
         }
         z0 = block_sum(z0 * wk, shz);
         z1 = block_sum(z1 * wk, shz);
@@ -1144,7 +1162,7 @@ __global__ __launch_bounds__(NTHR) void k_cols_plain(const DevGeo *__restrict__ 
 // Row FFT, inverse.  ospec[t][k][u], k in [0, H): half spectra over v of real rows.  Terms 2l, 2l+1 are
 // rebuilt into one Hermitian-extended complex line (self-conjugate bins taken real), inverse-
 // transformed, scaled by 1 / Mp^2, and written to out[u][v][sout + ...].
-template <typename T, int NTHR, int MAXTOT>
+template <typename T, int NTHR, int MAXTOT, bool PAD = false>
 __global__ __launch_bounds__(NTHR) void k_rows_inv(const DevGeo *__restrict__ dg, const cx<T> *__restrict__ tw,
                                                    const cx<T> *__restrict__ ospec, int nterms,
                                                    T *__restrict__ out, int sout) {
@@ -1156,12 +1174,12 @@ __global__ __launch_bounds__(NTHR) void k_rows_inv(const DevGeo *__restrict__ dg
     const T scale = (T)(1.0 / ((double)Mp * Mp));
     const long plane = (long)H * n1d;
     const int lines = (nterms + 1) / 2;
-    const cx<T> *twl = stage_twiddles(buf + lines * Mp, tw, Mp);
+    const cx<T> *twl = stage_twiddles(buf + lspan<PAD>(lines * Mp), tw, Mp);
     const int npad = 8 * ((n1d + 7) / 8);
     for (int slot = blockIdx.x; slot < npad; slot += gridDim.x) {
         const int u = xcd_slot_row(slot, n1d);
         if (u >= n1d) continue;
-        for (int q = threadIdx.x; q < lines * Mp; q += NTHR) {
+        lds_fill<NTHR, PAD>(buf, lines * Mp, [&](int q) {
             const int l = q / Mp, k = q - l * Mp;
             const bool lo = k < H;
             const int kk = lo ? k : Mp - k;
@@ -1175,13 +1193,13 @@ __global__ __launch_bounds__(NTHR) void k_rows_inv(const DevGeo *__restrict__ dg
                 f1 = conj(f1);
                 f2 = conj(f2);
             }
-            buf[q] = f1 + mul_i(f2);
-        }
+            return f1 + mul_i(f2);
+        });
         __syncthreads();
-        fft_lines<T, NTHR, MAXTOT>(buf, dg->pl, lines, Mp, twl, true);
+        fft_lines<T, NTHR, MAXTOT, PAD>(buf, dg->pl, lines, Mp, twl, true);
         for (int q = threadIdx.x; q < lines * n1d; q += NTHR) {
             const int l = q / n1d, v = q - l * n1d;
-            const cx<T> z = buf[l * Mp + v];
+            const cx<T> z = buf[lp<PAD>(l * Mp + v)];
             T *dst = out + ((long)u * n1d + v) * sout + 2 * l;
             dst[0] = z.x * scale;
             if (2 * l + 1 < nterms) dst[1] = z.y * scale;
