@@ -531,7 +531,8 @@ struct StageDims {
 // Spread, FFT convolution, Z, interpolation (+ update, zeroMean) of the FP32 form, reading geometry,
 // plan and step parameters from the context's device records.  side: the kernel spectra go on the
 // second stream (behind the attractive kernel there), after the geometry record, so that they run
-// beside the spread and the forward row pass; the column pass waits for them.
+// beside the spread and the forward row pass; the column pass waits for them.  (Measured at N = 1M:
+// this beats computing them on the main stream before or after the spread by ~0.015 ms per step.)
 template <int D>
 static int launch_f32_stages(fitsne_ctx *c, const cx<float> *tw, const StageDims &sd, bool kern, int mode,
                              bool side) {
