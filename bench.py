@@ -108,6 +108,7 @@ def bench_c4(args, fg, synth):
             c = fg.Context(no_dims=2, precision=fg.FP32)
             c.set_P(*P)
             c.set_Y(Y)
+            c.autotune()  # as the batch does for its contexts
             c.N = Y.shape[0]
             cs.append(c)
         return cs
@@ -334,6 +335,8 @@ def main():
     c32 = fg.Context(no_dims=2, precision=fg.FP32)
     c32.set_P(*P)
     c32.set_Y(Y)
+    spread_pts = c32.autotune()
+    log(f"autotuner: {spread_pts} points per spread workgroup")
     g32, z32 = c32.gradient()
     geo = c32.last_geometry()
     log(f"geometry: n_boxes={geo['n_boxes']} n1d={geo['n1d']} fft_len={geo['fft_len']} nnz={geo['nnz']}")
@@ -400,7 +403,8 @@ def main():
         "config": {"workload": "C2: N=1,000,000 2D, df=1, symmetric P ~90 nnz/row, Y span ~[-80,80]",
                    "N": N, "nnz": nnz, "n_boxes_per_dim": geo["n_boxes"], "n1d": geo["n1d"],
                    "fft_len": geo["fft_len"], "n_boxes_end": geo_end["n_boxes"], "learning_rate": lr,
-                   "momentum": 0.8, "max_step_norm": 5.0, "parallelism": "single GPU"},
+                   "momentum": 0.8, "max_step_norm": 5.0, "parallelism": "single GPU",
+                   "spread_points_per_workgroup": spread_pts},
         "roofline": {"bound": "hbm", "kernel": ATTR_KERNEL, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(ATTR_KERNEL),

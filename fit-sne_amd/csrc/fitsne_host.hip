@@ -104,6 +104,7 @@ struct fitsne_ctx {
     hipStream_t stream = nullptr;
     // second stream of the host-planned FP32 step: the attractive term runs on it beside the grid stages
     hipStream_t astream = nullptr;
+    int spread_pts = kSpreadPts;  // points per spread workgroup (fitsne_autotune); results do not depend on it
     hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     hipEvent_t geo_ev = nullptr, kern_ev = nullptr;  // kernel spectra on astream: after the geometry, before cols
     bool join_pending = false;
@@ -559,8 +560,9 @@ static int launch_f32_stages(fitsne_ctx *c, const cx<float> *tw, const StageDims
         c->stream = main;
         HIPTRY(hipEventRecord(c->kern_ev, c->astream));
     }
-    launch(c, "spread_f32", pick_spread_f32<D>(p), dim3((N + kSpreadPts - 1) / kSpreadPts), dim3(kSpreadThreads), 0,
-           c->Y.as<const float>(), N, dg, c->fixgrid.as<unsigned long long>());
+    const int pts = c->spread_pts;
+    launch(c, "spread_f32", pick_spread_f32<D>(p), dim3((N + pts - 1) / pts), dim3(kSpreadThreads), 0,
+           c->Y.as<const float>(), N, pts, dg, c->fixgrid.as<unsigned long long>());
     if constexpr (D == 2) {
         const size_t Mp = (size_t)sd.Mp;
         if (kern && !kside) kernel_spectra();
@@ -655,6 +657,10 @@ static int pipeline_f32(fitsne_ctx *c, int mode, const fitsne_step_params *sp) {
     TRY(live_record(c, &live));
     // the attractive term on the second stream (after everything queued so far), beside the grid
     // stages; the interpolation waits for it
+    // The second stream is made on first use: contexts that only run captured or batched steps never
+    // create it, so their streams alone share the device's hardware queues (a second stream per context
+    // measured a third fewer batched iterations).
+    if (!c->astream) HIPTRY(hipStreamCreateWithFlags(&c->astream, hipStreamNonBlocking));
     HIPTRY(hipEventRecord(c->fork_ev, c->stream));
     HIPTRY(hipStreamWaitEvent(c->astream, c->fork_ev, 0));
     {
@@ -719,6 +725,64 @@ static int launch_step_dev(fitsne_ctx *c) {
     const StageDims sd{mp_max, std::min(8 * ((n1d_max + 7) / 8), 512), std::min(8 * ((H_max + 7) / 8), 512),
                        std::min(blocks_for(n1d_max), 64)};
     TRY(launch_f32_stages<D>(c, c->dtw.as<const cx<float>>(), sd, D == 2, 1, false));
+    return FITSNE_OK;
+}
+
+// ============================================================================ autotuner
+// Points per spread workgroup, chosen by timing: each candidate runs the spread on the context's current
+// Y and geometry on its own (best of three launches), and the fastest is kept.  The grid, and so every
+// result, is the same for every candidate (the spread's waves are the same aligned groups of 64 points);
+// only the time differs: small problems want more, smaller workgroups, large ones fewer table flushes.
+constexpr int kSpreadPtsCand[] = {256, 512, 1024, 2048};
+
+template <int D>
+static int tune_spread(fitsne_ctx *c) {
+    const int N = c->N;
+    double mn, mx;
+    TRY(current_minmax<float>(c, &mn, &mx));
+    Geo g;
+    TRY(geometry(c, mn, mx, &g));
+    TRY(ensure_f32_buffers<D>(c, g.n1d, g.Mp));
+    TRY(put_geo(c, g, true));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPTRY(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return fail(FITSNE_E_HIP, "event creation failed");
+    }
+    const long nw = (long)(c->fixgrid.cap / sizeof(unsigned long long));
+    float best_ms = INFINITY;
+    int best = c->spread_pts;
+    int rc = FITSNE_OK;
+    for (int pts : kSpreadPtsCand) {
+        float t = INFINITY;
+        for (int rep = 0; rep < 3 && rc == FITSNE_OK; rep++) {
+            if (hipEventRecord(e0, c->stream) != hipSuccess) rc = fail(FITSNE_E_HIP, "event record failed");
+            launch(c, "spread_f32", pick_spread_f32<D>(c->p), dim3((N + pts - 1) / pts), dim3(kSpreadThreads), 0,
+                   c->Y.as<const float>(), N, pts, c->dgeo.as<const DevGeo>(), c->fixgrid.as<unsigned long long>());
+            if (hipEventRecord(e1, c->stream) != hipSuccess) rc = fail(FITSNE_E_HIP, "event record failed");
+            // the grid is zero between iterations: clear what the trial added
+            launch(c, "fill", k_fill<unsigned long long>, dim3(std::min(blocks_for(nw), 2048)), dim3(kThreads), 0,
+                   c->fixgrid.as<unsigned long long>(), nw, 0ull);
+            float ms = INFINITY;
+            if (rc == FITSNE_OK && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                rc = fail(FITSNE_E_HIP, "event timing failed");
+            t = std::min(t, ms);
+        }
+        if (rc != FITSNE_OK) break;
+        if (t < best_ms) {
+            best_ms = t;
+            best = pts;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    TRY(rc);
+    TRY(sync_ctx(c));
+    if (best != c->spread_pts) {
+        c->spread_pts = best;
+        drop_graph(c);
+    }
     return FITSNE_OK;
 }
 
@@ -1143,7 +1207,6 @@ extern "C" int fitsne_ctx_create(fitsne_ctx **out, int device, int precision, in
     c->mni = min_num_intervals;
     c->df = df;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipStreamCreateWithFlags(&c->astream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->geo_ev, hipEventDisableTiming) != hipSuccess ||
@@ -1650,6 +1713,31 @@ static int check_graph(fitsne_ctx *c) {
     return FITSNE_OK;
 }
 
+extern "C" int fitsne_autotune(fitsne_ctx *c, int *spread_points) {
+    if (!c) return fail(FITSNE_E_ARG, "null argument");
+    if (c->precision != FITSNE_FP32) return fail(FITSNE_E_ARG, "the autotuner is for FP32 contexts");
+    if (c->N <= 0 || !c->Y.p) return fail(FITSNE_E_ARG, "set P and Y before tuning");
+    HIPTRY(hipSetDevice(c->device));
+    TRY(check_graph(c));
+    TRY(sync_ctx(c));
+    TRY(c->D == 2 ? tune_spread<2>(c) : tune_spread<1>(c));
+    if (spread_points) *spread_points = c->spread_pts;
+    return FITSNE_OK;
+}
+
+extern "C" int fitsne_set_spread_points(fitsne_ctx *c, int points) {
+    if (!c) return fail(FITSNE_E_ARG, "null argument");
+    if (points < kSpreadThreads || points > 8192 || points % kSpreadThreads != 0)
+        return fail(FITSNE_E_ARG, "spread points per workgroup must be a multiple of 256 in [256, 8192]");
+    HIPTRY(hipSetDevice(c->device));
+    TRY(sync_ctx(c));
+    if (points != c->spread_pts) {
+        c->spread_pts = points;
+        drop_graph(c);
+    }
+    return FITSNE_OK;
+}
+
 extern "C" int fitsne_set_graph(fitsne_ctx *c, int enable) {
     if (!c) return fail(FITSNE_E_ARG, "null context");
     if (enable && c->precision != FITSNE_FP32) return fail(FITSNE_E_UNSUPPORTED, "captured steps are for the FP32 form");
@@ -1697,6 +1785,8 @@ extern "C" int fitsne_batch_create(fitsne_batch **out, fitsne_ctx **ctxs, int n)
         for (int j = 0; j < i; j++)
             if (ctxs[j] == ctxs[i]) return fail(FITSNE_E_ARG, "a context appears twice in the batch");
     }
+    // every problem gets its own spread workgroup size (results do not depend on it)
+    for (int i = 0; i < n; i++) TRY(fitsne_autotune(ctxs[i], nullptr));
     auto *b = new fitsne_batch();
     b->ctx.assign(ctxs, ctxs + n);
     *out = b;

@@ -694,20 +694,21 @@ __global__ __launch_bounds__(1024) void k_scan_counts(int *__restrict__ count, i
 }
 
 // ---------------------------------------------------------------- spread, FP32 form
-// A workgroup takes kSpreadPts consecutive points (memory order; spatially ordered points keep this to
-// a few dozen boxes).  Each lane finds its point's box and computes the node-relative charges
+// A workgroup takes `pts` consecutive points (a multiple of kSpreadThreads; memory order; spatially
+// ordered points keep this to a few dozen boxes).  Each lane finds its point's box and computes the node-relative charges
 //   A = L,  c = L |e|^2,  b = L e    (e = y - node, in box widths)
 // for the p^D nodes of its box.  Runs of consecutive lanes in one box are summed in FP32 by a segmented
 // scan in the VALU (DPP; a fixed order over fixed points), each run's sums are scaled by 2^38, rounded
 // to int64 and added into the workgroup's LDS table slot of its box (open addressing, LDS atomics), and
 // the table goes to the grid with one 64-bit global atomic per (box, node, term).  A run whose box finds
 // no free slot adds to the grid directly.  The sums across runs are exact integers, so the grid, and so
-// the whole gradient, is bitwise repeatable whatever the order in which workgroups and waves run.
+// the whole gradient, is bitwise repeatable whatever the order in which workgroups and waves run, and
+// whatever `pts`: the waves are the same aligned groups of 64 points for every choice.
 // Grid layout: int64 [u][v][4] (A, c, b_x, b_y), u the x node (nbodyfft.cpp:137-138); the row FFT
 // reads it, converts it and clears it for the next iteration.
 constexpr double kFixScale = 274877906944.0;  // 2^38
 constexpr int kSpreadThreads = 256;
-constexpr int kSpreadPts = 1024;
+constexpr int kSpreadPts = 1024;  // default points per workgroup (the autotuner may choose another)
 
 // One step of a segmented inclusive scan over the 64 lanes: each lane adds the value that the DPP
 // control CTRL (row_shr:n, row_bcast:15 / :31 with row mask RMASK) brings from a lower lane, when that
@@ -734,7 +735,7 @@ constexpr int spread_slots() {
 }
 
 template <int P, int D>
-__global__ __launch_bounds__(kSpreadThreads) void k_spread_f32(const float *__restrict__ Y, int N,
+__global__ __launch_bounds__(kSpreadThreads) void k_spread_f32(const float *__restrict__ Y, int N, int pts,
                                                                const DevGeo *__restrict__ dg,
                                                                unsigned long long *__restrict__ grid) {
     constexpr int NV = (D == 2 ? P * P : P);
@@ -748,8 +749,8 @@ __global__ __launch_bounds__(kSpreadThreads) void k_spread_f32(const float *__re
     for (int o = t; o < NS; o += kSpreadThreads) tkey[o] = -1;
     for (int o = t; o < NS * NV * NQ; o += kSpreadThreads) (&tval[0][0])[o] = 0ull;
     __syncthreads();
-    for (int chunk = 0; chunk < kSpreadPts / kSpreadThreads; chunk++) {
-        const long i = (long)blockIdx.x * kSpreadPts + (long)chunk * kSpreadThreads + t;
+    for (int chunk = 0; chunk < pts / kSpreadThreads; chunk++) {
+        const long i = (long)blockIdx.x * pts + (long)chunk * kSpreadThreads + t;
         const bool ok = i < N;
         int b = -1 - lane;  // lanes past N: distinct negative keys, never a real run
         float val[NV][NQ];

@@ -51,6 +51,8 @@ SIGNATURES = {
     "fitsne_knn_exact": (_ci, [_dp, _ci, _ci, _ci, _i32, _dp]),
     "fitsne_symmetrize": (_ci, [_u32, _u32, _dp, _ci, _u32, _u32, _dp, C.POINTER(C.c_longlong)]),
     "fitsne_set_graph": (_ci, [_vp, _ci]),
+    "fitsne_autotune": (_ci, [_vp, C.POINTER(_ci)]),
+    "fitsne_set_spread_points": (_ci, [_vp, _ci]),
     "fitsne_batch_create": (_ci, [C.POINTER(_vp), C.POINTER(_vp), _ci]),
     "fitsne_batch_step": (_ci, [_vp, C.POINTER(StepParams)]),
     "fitsne_batch_destroy": (_ci, [_vp]),
@@ -403,6 +405,16 @@ class Context:
     def set_graph(self, enable):
         """Replay steps as one captured graph with device-planned geometry (FP32 form)."""
         check(lib().fitsne_set_graph(self.h, int(enable)))
+
+    def autotune(self):
+        """Time the spread's points per workgroup on the current Y and keep the fastest; returns it."""
+        out = C.c_int(0)
+        check(lib().fitsne_autotune(self.h, C.byref(out)))
+        return out.value
+
+    def set_spread_points(self, points):
+        """Points per spread workgroup (a multiple of 256 in [256, 8192]); results do not depend on it."""
+        check(lib().fitsne_set_spread_points(self.h, int(points)))
 
     def reorder(self):
         check(lib().fitsne_reorder(self.h))

@@ -245,10 +245,19 @@ int fitsne_step(fitsne_ctx *ctx, const fitsne_step_params *prm, double *sum_Q);
  * FITSNE_E_UNSUPPORTED.  Re-captured after set_P / set_Y / reorder. */
 int fitsne_set_graph(fitsne_ctx *ctx, int enable);
 
+/* Autotuner (no reference counterpart: the reference's spread is a serial loop).  Times the FP32
+ * spread's points per workgroup (256, 512, 1024, 2048) on the context's current Y and geometry and
+ * keeps the fastest; every result is bitwise the same for any choice.  spread_points (may be null)
+ * receives the choice.  FP32 contexts with P and Y set; fitsne_batch_create runs it for each context. */
+int fitsne_autotune(fitsne_ctx *ctx, int *spread_points);
+/* Set the spread's points per workgroup directly (a multiple of 256 in [256, 8192]; default 1024). */
+int fitsne_set_spread_points(fitsne_ctx *ctx, int points);
+
 /* Batched embeddings (no reference counterpart): one optimisation step of every context of the batch
  * (FP32 contexts on one device, each with its own P and Y): each context's captured step is launched
  * on its own stream, so the embeddings' steps overlap on the device.  prm[n] holds each context's
- * parameters.  Results are bitwise those of stepping each context alone. */
+ * parameters.  Results are bitwise those of stepping each context alone.  Creating a batch runs the
+ * autotuner on each context. */
 typedef struct fitsne_batch fitsne_batch;
 int fitsne_batch_create(fitsne_batch **batch, fitsne_ctx **ctxs, int n);
 int fitsne_batch_step(fitsne_batch *batch, const fitsne_step_params *prm);

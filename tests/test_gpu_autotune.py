@@ -1,0 +1,63 @@
+"""The autotuner's knob, the spread's points per workgroup, leaves every result bitwise unchanged.
+
+The spread's waves are the same aligned groups of 64 points for any workgroup size, and everything
+after the in-wave sums is exact integer arithmetic, so the FP32 gradient, Z and a run of optimiser
+steps (host-planned and captured) must be identical for 256, 512, 1024 and 2048 points per
+workgroup.  The autotuner must return one of those and leave the results as they were.
+"""
+import numpy as np
+import pytest
+
+import fitsne_gpu as fg
+from datagen import blobs, random_csr
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(P, Y, pts, D, graph=False, steps=4):
+    c = fg.Context(no_dims=D, precision=fg.FP32)
+    c.set_P(*P)
+    c.set_Y(Y)
+    if pts is None:
+        chosen = c.autotune()
+    else:
+        c.set_spread_points(pts)
+        chosen = pts
+    g, z = c.gradient()
+    if graph:
+        c.set_graph(True)
+    for _ in range(steps):
+        c.step(momentum=0.5, learning_rate=200.0)
+    Yout = c.get_Y()
+    c.close()
+    return chosen, g, z, Yout
+
+
+@pytest.mark.parametrize("D,N", [(2, 30_000), (1, 20_000)])
+def test_spread_workgroup_size_does_not_change_results(D, N):
+    Y = blobs(N, D, 6, seed=31 + D, span=40.0)
+    P = random_csr(N, 15, seed=5)
+    print(f"[autotune test] D={D} N={N}", flush=True)
+    _, g0, z0, y0 = _run(P, Y, 1024, D)
+    for pts in (256, 512, 2048):
+        _, g, z, y = _run(P, Y, pts, D)
+        np.testing.assert_array_equal(g, g0)
+        assert z == z0
+        np.testing.assert_array_equal(y, y0)
+    _, g, z, y = _run(P, Y, 256, D, graph=True)
+    _, gr, zr, yr = _run(P, Y, 1024, D, graph=True)
+    np.testing.assert_array_equal(y, yr)
+    chosen, g, z, y = _run(P, Y, None, D)
+    assert chosen in (256, 512, 1024, 2048)
+    np.testing.assert_array_equal(g, g0)
+    np.testing.assert_array_equal(y, y0)
+
+
+def test_spread_points_are_validated():
+    c = fg.Context(no_dims=2, precision=fg.FP32)
+    for bad in (0, 100, 300, 16384):
+        with pytest.raises(fg.FitsneError):
+            c.set_spread_points(bad)
+    with pytest.raises(fg.FitsneError):
+        c.autotune()  # no P and Y yet
+    c.close()
