@@ -4,6 +4,8 @@
   neighbours; its distance is vptree.h:97-107): same indices, same distances.
 * bin/fast_tsne with load_affinities='save' writes P_row/P_col/P_val.dat; they must equal the oracle's
   computeGaussianPerplexity + symmetrizeMatrix + normalisation (tsne.cpp:151-330) on the same data.
+* The optimisation schedule (early and late exaggeration, momentum switch, no_momentum_during_exag, 1D
+  with df != 1): the program (FP64 plain path) against the oracle's run of the same loop, 1e-9.
 * C1 end to end (N = 10,000, 1000 iterations): the program's final KL within 0.5% of the oracle's run
   of TSNE::run on the same data and initialisation (tests/golden/c1_oracle_kl.json, made by
   tests/golden/make_c1_golden.py), in the FP32 form and the FP64 plain path.
@@ -86,3 +88,33 @@ def test_c1_final_kl_within_half_percent_of_the_oracle(tmp_path, precision):
     _note(f"{precision}: KL {kl[[0, 4, 9, -1]]} vs oracle {np.array(gold['kl'])[[0, 4, 9, -1]]}")
     assert np.isfinite(Y).all() and Y.shape == (10000, 2)
     assert abs(kl[-1] - gold["kl"][-1]) / gold["kl"][-1] < 5e-3
+
+
+@pytest.mark.parametrize("dims,df,nomom", [(1, 0.6, 0), (2, 1.0, 1)])
+def test_program_schedule_tracks_the_oracle_run(tmp_path, dims, df, nomom):
+    """The program's optimisation schedule (tsne.cpp:389-577) against the oracle's run of the same loop
+    (orc_run_fft) on the same P and initial Y, with the FP64 plain path: early exaggeration until
+    iteration 30, the momentum switch at 30, late exaggeration x3 from 60, in 1D with df = 0.6 and in 2D
+    with no_momentum_during_exag.  Any switch a step early or late moves the KL entries far more than
+    the bound."""
+    N = 1500
+    X, _ = synth.c1_data(N=N, D=30, k=5, seed=9)
+    Y0 = np.random.default_rng(3).normal(0, 1e-4, size=(N, dims))
+    sched = dict(stop_early_exag_iter=30, mom_switch_iter=30, start_late_exag_iter=60, late_exag_coeff=3.0,
+                 early_exag_coeff=12.0, learning_rate=200.0, max_step_norm=5.0)
+    _note(f"program schedule N={N} dims={dims} df={df} no_momentum_during_exag={nomom}")
+    Y, loss = _run_program(tmp_path, X, Y0, max_iter=120, precision="fp64", map_dims=dims, df=df,
+                           no_momentum_during_exag=bool(nomom), **sched)
+    P = c1.c1_affinities(X)
+    sched["stop_lying_iter"] = sched.pop("stop_early_exag_iter")
+    Yr, costs = orc.run_fft(P, Y0, max_iter=120, df=df, no_momentum_during_exag=nomom, **sched)
+    # the result reader keeps the multiples of 50 only (uncomputed entries NaN, as the reference's)
+    at = np.nonzero(np.isfinite(loss))[0]
+    np.testing.assert_array_equal(at, [49, 99])
+    assert (costs[at] > 0).all()
+    r_kl = np.abs(loss[at] - costs[at]) / costs[at]
+    r_y = np.linalg.norm(Y.reshape(Yr.shape) - Yr) / np.linalg.norm(Yr)
+    _note(f"KL entries at {list(at)}: max rel diff {r_kl.max():.2e}; final Y rel diff {r_y:.2e}")
+    # measured 1e-13 (KL) and 6e-12 (Y): the plain path's gradient agrees with the oracle's to ~1e-12
+    assert r_kl.max() <= 1e-9
+    assert r_y <= 1e-9
