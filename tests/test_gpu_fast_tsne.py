@@ -4,6 +4,9 @@
   neighbours; its distance is vptree.h:97-107): same indices, same distances.
 * bin/fast_tsne with load_affinities='save' writes P_row/P_col/P_val.dat; they must equal the oracle's
   computeGaussianPerplexity + symmetrizeMatrix + normalisation (tsne.cpp:151-330) on the same data.
+* The affinity modes: a perplexity list (K = 3 max, rows averaged) and a fixed sigma with K neighbours on
+  un-normalised data, against the oracle (same rows and columns, values 1e-12); load_affinities='load'
+  reproduces the saving run bit for bit.
 * The optimisation schedule (early and late exaggeration, momentum switch, no_momentum_during_exag, 1D
   with df != 1): the program (FP64 plain path) against the oracle's run of the same loop, 1e-9.
 * C1 end to end (N = 10,000, 1000 iterations): the program's final KL within 0.5% of the oracle's run
@@ -118,3 +121,49 @@ def test_program_schedule_tracks_the_oracle_run(tmp_path, dims, df, nomom):
     # measured 1e-13 (KL) and 6e-12 (Y): the plain path's gradient agrees with the oracle's to ~1e-12
     assert r_kl.max() <= 1e-9
     assert r_y <= 1e-9
+
+
+def _oracle_P_modes(X, K, perplexity, sigma=-1.0, plist=None):
+    """The oracle's computeGaussianPerplexity (list and fixed-sigma modes, tsne.cpp:1643-1726 with the
+    list overload :1474-1500) + symmetrizeMatrix + normalisation on the program's prepared data: X
+    zero-mean, and scaled by its largest magnitude only when a perplexity is used (tsne.cpp:153-164)."""
+    N, D = X.shape
+    Xc = np.ascontiguousarray(X, dtype=np.float64).copy()
+    orc.oracle().orc_zero_mean(Xc.reshape(-1), N, D)
+    if perplexity > 0 or plist is not None:
+        Xc /= np.abs(Xc).max()
+    idx, dist = orc.knn_exact(Xc, K, nthreads=os.cpu_count() or 1)
+    row, col, val = orc.rows_from_knn(idx, dist, perplexity, sigma=sigma, plist=plist)
+    return orc.normalize(*orc.symmetrize(row, col, val))
+
+
+@pytest.mark.parametrize("mode", ["perplexity_list", "sigma"])
+def test_program_affinity_modes_match_the_oracle(tmp_path, mode):
+    X, _ = synth.c1_data(N=1800, D=30, k=6, seed=7)
+    Y0 = c1.c1_init(X)
+    if mode == "perplexity_list":
+        kw = dict(perplexity_list=[10, 25])
+        want = _oracle_P_modes(X, 75, 0.0, plist=np.array([10.0, 25.0]))
+    else:
+        # the data stay un-normalised in this mode: within-cluster squared distances are ~60
+        kw = dict(sigma=5.0, K=40)
+        want = _oracle_P_modes(X, 40, -1.0, sigma=5.0)
+    _note(f"program N=1800 affinities, {mode}")
+    _run_program(tmp_path, X, Y0, max_iter=2, load_affinities="save", **kw)
+    row = np.fromfile(tmp_path / "P_row.dat", dtype=np.uint32)
+    col = np.fromfile(tmp_path / "P_col.dat", dtype=np.uint32)
+    val = np.fromfile(tmp_path / "P_val.dat", dtype=np.float64)
+    np.testing.assert_array_equal(row, want[0])
+    np.testing.assert_array_equal(col, want[1])
+    np.testing.assert_allclose(val, want[2], rtol=1e-12, atol=0)
+
+
+def test_program_loaded_affinities_reproduce_the_run(tmp_path):
+    """load_affinities='load' reads P_row/P_col/P_val.dat (tsne.cpp:232-265) instead of building P: a run
+    on those files is the run that saved them, bit for bit (the FP32 step is deterministic)."""
+    X, _ = synth.c1_data(N=1500, D=30, k=5, seed=11)
+    Y0 = c1.c1_init(X)
+    _note("program N=1500 save, then load")
+    Y1 = _run_program(tmp_path, X, Y0, max_iter=40, load_affinities="save")[0]
+    Y2 = _run_program(tmp_path, X, Y0, max_iter=40, load_affinities="load")[0]
+    np.testing.assert_array_equal(Y1, Y2)
