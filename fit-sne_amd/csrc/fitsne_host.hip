@@ -574,10 +574,10 @@ static int launch_f32_stages(fitsne_ctx *c, const cx<float> *tw, const StageDims
                sizeof(cx<float>) * (lspan<true>(4 * (int)Mp) + Mp), dg, tw, c->spec.as<const cx<float>>(), c->KQ.as<const float>(),
                c->K1.as<const float>(), c->KS.as<const float>(), c->KT.as<const float>(), c->ospec.as<cx<float>>(),
                c->zpart.as<double>());
-        launch(c, "zfinal", k_zfinal, dim3(1), dim3(kThreads), 0, dg, c->zpart.as<const double>(), (int)df1, (double)N,
-               c->zbuf.as<double>());
+        // the final Z is computed by the inverse row pass's first workgroup
         launch(c, "rows_inv", k_rows_inv<float, kRowThreads, kRowTot, true>, dim3(sd.gr_rows), dim3(kRowThreads),
-               sizeof(cx<float>) * (lspan<true>(2 * (int)Mp) + Mp), dg, tw, c->ospec.as<const cx<float>>(), 3, c->gout.as<float>(), 4);
+               sizeof(cx<float>) * (lspan<true>(2 * (int)Mp) + Mp), dg, tw, c->ospec.as<const cx<float>>(), 3, c->gout.as<float>(), 4,
+               c->zpart.as<const double>(), (int)df1, (double)N, c->zbuf.as<double>());
     } else {
         launch(c, "grid_to_f32", k_grid_to_f32, dim3(sd.gr_nodes), dim3(kThreads), 0, dg,
                c->fixgrid.as<unsigned long long>(), c->gin.as<float4>());
@@ -820,7 +820,7 @@ static int plain_pass(fitsne_ctx *c, const Geo &g, const cx<double> *tw, int set
                c->ospec.as<cx<double>>());
         launch(c, "rows_inv64", k_rows_inv<double, kRowThreads, kRowTot>, dim3(gr_rows), dim3(kRowThreads),
                sizeof(cx<double>) * (lines + 1) * g.Mp, c->dgeo.as<const DevGeo>(), tw, c->ospec.as<const cx<double>>(), nt,
-               c->V.as<double>(), nt);
+               c->V.as<double>(), nt, (const double *)nullptr, 0, 0.0, (double *)nullptr);
     } else {
         launch(c, "conv1d_f64", k_conv1d<double, kConv1dThreads, kMaxTot64>, dim3(1), dim3(kConv1dThreads),
                sizeof(cx<double>) * g.Mp, c->dgeo.as<const DevGeo>(), tw, kind1d, kind1d, c->df, 0, c->w.as<const double>(), nt, nt, use_k1,
@@ -2123,7 +2123,7 @@ extern "C" int fitsne_n_body_fft_2d(int N, int n_terms, const double *xs, const 
            c->ospec.as<cx<double>>());
     launch(c, "rows_inv64", k_rows_inv<double, kRowThreads, kRowTot>, dim3(gr_rows), dim3(kRowThreads),
            sizeof(cx<double>) * (lines + 1) * g.Mp, c->dgeo.as<const DevGeo>(), tw, c->ospec.as<const cx<double>>(), nt,
-           c->V.as<double>(), nt);
+           c->V.as<double>(), nt, (const double *)nullptr, 0, 0.0, (double *)nullptr);
     launch(c, "interp_f64", pick_interp_f64<2>(p), dim3(blocks_for(N)), dim3(kThreads), 0, c->box.as<const int>(),
            c->wx.as<const double>(), c->wy.as<const double>(), c->V.as<const double>(), nt, N, g, c->potA.as<double>());
     HIPTRY(hipMemcpyAsync(potentialQij, c->potA.p, sizeof(double) * (size_t)N * n_terms, hipMemcpyDeviceToHost,

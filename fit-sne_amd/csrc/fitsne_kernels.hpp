@@ -1164,10 +1164,17 @@ __global__ __launch_bounds__(NTHR) void k_cols_plain(const DevGeo *__restrict__ 
 // rebuilt into one Hermitian-extended complex line (self-conjugate bins taken real), inverse-
 // transformed, scaled by 1 / Mp^2, and written to out[u][v][sout + ...].
 template <typename T, int NTHR, int MAXTOT, bool PAD = false>
+// zpart != null: the first workgroup also computes the FP32 form's final Z (k_zfinal's reduction and
+// order; NTHR == kThreads) before its lines, one launch fewer on the step's critical path.
 __global__ __launch_bounds__(NTHR) void k_rows_inv(const DevGeo *__restrict__ dg, const cx<T> *__restrict__ tw,
                                                    const cx<T> *__restrict__ ospec, int nterms,
-                                                   T *__restrict__ out, int sout) {
+                                                   T *__restrict__ out, int sout, const double *__restrict__ zpart,
+                                                   int df_one, double Nz, double *__restrict__ zout) {
     if (!dg->live) return;
+    if (zpart && blockIdx.x == 0) {
+        __shared__ double shz[NTHR / kWave];
+        zfinal_block(dg, zpart, df_one, Nz, zout, shz);
+    }
     const Geo &g = dg->g;
     extern __shared__ unsigned char smem_raw[];
     cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
@@ -1380,13 +1387,11 @@ __global__ __launch_bounds__(NTHR) void k_kspec1d(const DevGeo *__restrict__ dg,
 
 // Final Z of the FP32 form from the per-column Parseval partials, in FP64 and a fixed order:
 //   df = 1:  Z = (z0 + 2 z1 + 4 z2 - 2 z3) / Mp^D - N;   df != 1:  Z = z0 / Mp^D - N.
-__global__ void k_zfinal(const DevGeo *__restrict__ dg, const double *__restrict__ zpart, int df_one, double N,
-                         double *__restrict__ zout) {
-    if (!dg->live) return;
+__device__ __forceinline__ void zfinal_block(const DevGeo *__restrict__ dg, const double *__restrict__ zpart,
+                                             int df_one, double N, double *__restrict__ zout, double *sh) {
     const Geo &g = dg->g;
     const int ncols = g.D == 2 ? g.H : 1;
     const double norm = g.D == 2 ? 1.0 / ((double)g.Mp * g.Mp) : 1.0 / (double)g.Mp;
-    __shared__ double sh[kThreads / kWave];
     double s = 0;
     for (int k = threadIdx.x; k < ncols; k += blockDim.x) {
         const double *z = zpart + 4 * k;
@@ -1398,6 +1403,13 @@ __global__ void k_zfinal(const DevGeo *__restrict__ dg, const double *__restrict
         zout[0] = Z;
         zout[1] = 1.0 / Z;
     }
+}
+
+__global__ void k_zfinal(const DevGeo *__restrict__ dg, const double *__restrict__ zpart, int df_one, double N,
+                         double *__restrict__ zout) {
+    if (!dg->live) return;
+    __shared__ double sh[kThreads / kWave];
+    zfinal_block(dg, zpart, df_one, N, zout, sh);
 }
 
 // ---------------------------------------------------------------- interpolation + update, FP32 form
