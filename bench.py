@@ -298,8 +298,8 @@ def main():
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE scaling")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one MI355X; multi-GPU runs are not part of this build (see DESIGN.md for how "
-              "the path would shard)", file=sys.stderr)
+        print("bench.py measures one MI355X: runs on several GPUs are not supported (the gradient path is "
+              "single-GPU)", file=sys.stderr)
         sys.exit(2)
 
     import fitsne_gpu as fg
