@@ -159,6 +159,10 @@ __global__ __launch_bounds__(kThreads) void k_attractive(const TI *__restrict__ 
 constexpr int kWin = 8192;
 constexpr int kAttRows = 4096;
 constexpr int kAttThreads = 1024;
+// A bucket of fewer quads is processed from a flat list before the windows, with its columns' Y read
+// from global memory: staging a whole window (64 KB and two barriers) for a few entries cost more.
+constexpr int kSmallQuads = 512;
+static_assert((kWin * 2 - 1) / 2 * 2 + 2 <= kWin * 2 && (kWin - 1) / 2 * 2 + 2 <= kWin, "small-bucket list fits");
 
 // round-to-nearest int64 of x for |x| < 2^51: adding 1.5 * 2^52 leaves the integer in the low mantissa bits
 __device__ __forceinline__ long long fix_rn(double x) {
@@ -393,8 +397,67 @@ __global__ __launch_bounds__(kAttThreads) void k_attr_win(const int *__restrict_
     const int *pu = pure + (long)b * NW;
     const u32x4 *K4 = reinterpret_cast<const u32x4 *>(key);
     const f32x4 *V4 = reinterpret_cast<const f32x4 *>(wval);
+    // Small buckets: listed (window numbers, then the exclusive prefix of their quads) in the window
+    // buffer, which is free until the first window is staged, and processed as one flat range of quads.
+    // If more than fit are found, none is listed and every bucket is staged.
+    constexpr int kSmCap = (kWin * D - 1) / 2;  // sm_w[kSmCap], sm_pre[kSmCap + 1], count: kWin * D ints
+    int *sm_w = reinterpret_cast<int *>(yw);
+    int *sm_pre = sm_w + kSmCap;
+    int *sm_n = sm_pre + kSmCap + 1;
+    if (t == 0) *sm_n = 0;
+    __syncthreads();  // also: yr and acc are initialised
+    for (int ww = t; ww < NW; ww += kAttThreads) {
+        const int nq = (st[ww + 1] - st[ww]) >> 2;
+        if (nq > 0 && nq < kSmallQuads) {
+            const int slot = atomicAdd(sm_n, 1);
+            if (slot < kSmCap) sm_w[slot] = ww;
+        }
+    }
+    __syncthreads();
+    const int ns = *sm_n;
+    const bool flat = ns <= kSmCap;
+    if (flat && ns > 0) {
+        if (t < kWave) {  // one wave: exclusive prefix of the listed buckets' quads (list order)
+            int carry = 0;
+            for (int i0 = 0; i0 < ns; i0 += kWave) {
+                const int i = i0 + t;
+                const int nq = i < ns ? (st[sm_w[i] + 1] - st[sm_w[i]]) >> 2 : 0;
+                int incl = nq;
+                for (int o = 1; o < kWave; o <<= 1) {
+                    const int y = __shfl_up(incl, o, kWave);
+                    if (t >= o) incl += y;
+                }
+                if (i < ns) sm_pre[i] = carry + incl - nq;
+                carry += __shfl(incl, kWave - 1, kWave);
+            }
+            if (t == 0) sm_pre[ns] = carry;
+        }
+        __syncthreads();
+        const int total = sm_pre[ns];
+        for (int qi = t; qi < total; qi += kAttThreads) {
+            int lo = 0, hi = ns - 1;  // the listed bucket holding quad qi (prefixes strictly increase)
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (sm_pre[mid] <= qi) lo = mid;
+                else hi = mid - 1;
+            }
+            const int ww = sm_w[lo];
+            const int qq = (st[ww] >> 2) + (qi - sm_pre[lo]);
+            const int qm = (st[ww] + pu[ww]) >> 2;
+            const float *ywg = Y + (long)ww * kWin * D;  // the window's Y in global memory
+            const u32x4 ka = __builtin_nontemporal_load(&K4[qq]);
+            const f32x4 va = __builtin_nontemporal_load(&V4[qq]);
+            if (qq < qm) attr_quad_pure<D>(ka, va, yr, ywg, acc, inv_df, df_one, S);
+            else attr_quad<D>(ka, va, yr, ywg, acc, inv_df, df_one, S);
+        }
+    }
+    __syncthreads();  // the list lives in the window buffer
+    auto staged = [&](int ww) {
+        const int nq = (st[ww + 1] - st[ww]) >> 2;
+        return nq > 0 && !(flat && nq < kSmallQuads);
+    };
     int w = 0;
-    while (w < NW && st[w] == st[w + 1]) w++;
+    while (w < NW && !staged(w)) w++;
     float pre[WL];
     auto fetch = [&](int ww) {
         const long base = (long)ww * kWin * D;
@@ -409,7 +472,7 @@ __global__ __launch_bounds__(kAttThreads) void k_attr_win(const int *__restrict_
     while (w < NW) {
         const int q0 = st[w] >> 2, qm = (st[w] + pu[w]) >> 2, q1 = st[w + 1] >> 2;
         int wn = w + 1;
-        while (wn < NW && st[wn] == st[wn + 1]) wn++;
+        while (wn < NW && !staged(wn)) wn++;
         __syncthreads();  // the previous window's readers are done
 #pragma unroll
         for (int u = 0; u < WL; u++) yw[t + u * kAttThreads] = pre[u];
