@@ -77,8 +77,8 @@ def measured_traffic(kernel):
     return kib * 1024.0
 
 
-def cpu_baseline(P, Y32, calls):
-    """The oracle's gradient (FP64 C restatement of computeFftGradient, PARALLEL_FOR threading kept)
+def cpu_baseline(P, Y32, calls, df=1.0):
+    """The oracle's gradient (FP64 C restatement of computeFftGradient*, PARALLEL_FOR threading kept)
     on the same Y and P; returns (iters/s, threads, dC, sum_Q)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import orc
@@ -89,7 +89,7 @@ def cpu_baseline(P, Y32, calls):
     g = zq = None
     t0 = time.perf_counter()
     for i in range(calls):
-        g, zq = orc.gradient(Pu, Y32, nthreads=nthreads)
+        g, zq = orc.gradient(Pu, Y32, nthreads=nthreads, df=df)
         log(f"cpu baseline gradient {i + 1}/{calls} done ({time.perf_counter() - t0:.1f}s)")
     dt = time.perf_counter() - t0
     return calls / dt, nthreads, g, zq
@@ -236,6 +236,74 @@ def bench_exact(args, fg):
     print(json.dumps(out), flush=True)
 
 
+def bench_c3(args, fg, synth):
+    """C3: the 1D heavy-tailed gradient (computeFftGradientOneDVariableDf) at N = 2,000,000, df = 0.5,
+    late exaggeration 4, span about [-300, 300] (600 boxes, FFT length 3888), FP32 form against the FP64
+    plain path on the same Y and P; the attractive kernel's roofline as for C2."""
+    N = args.n if args.n != 1_000_000 else 2_000_000
+    log(f"generating the C3 workload N={N}")
+    P, Y, _ = synth.c3_workload(N)
+    nnz = int(P[0][-1])
+    df = 0.5
+    lr = max(200.0, N / 12.0)
+    prm = dict(momentum=0.8, learning_rate=lr, max_step_norm=5.0)
+    c32 = fg.Context(no_dims=1, precision=fg.FP32, df=df)
+    c32.set_P(*P)
+    c32.set_Y(Y)
+    c32.set_exaggeration(4.0)
+    spread_pts = c32.autotune()
+    g32, z32 = c32.gradient()
+    geo = c32.last_geometry()
+    log(f"geometry: n_boxes={geo['n_boxes']} n1d={geo['n1d']} fft_len={geo['fft_len']} nnz={nnz}")
+    res64, g64, z64 = {}, None, None
+    if not args.no_fp64:
+        c64 = fg.Context(no_dims=1, precision=fg.FP64_PLAIN, df=df)
+        c64.set_P(*P)
+        c64.set_Y(Y.astype(np.float32).astype(np.float64))
+        c64.set_exaggeration(4.0)
+        g64, z64 = c64.gradient()
+        dt64 = timed_steps(c64, args.steps, args.warmup, prm)
+        res64 = {"ms_per_step": 1e3 * dt64 / args.steps, "iters_per_s": args.steps / dt64}
+        c64.close()
+    dt = timed_steps(c32, args.steps, args.warmup, prm)
+    stats = kernel_profile(c32, max(args.profile_steps, 1), prm)
+    c32.close()
+    cnt, tot = stats.get("attractive", (0, 0.0))
+    att_ms = tot / max(cnt, 1)
+    att_bytes = 8 * (N + 1) + nnz * (4 + 4) + 2 * N * 4
+    achieved = att_bytes / (att_ms * 1e-3) / 1e9 if att_ms > 0 else 0.0
+    acc = {}
+    if g64 is not None:
+        acc["fp32_vs_fp64_gpu_grad_rel"] = rel(g32, g64)
+        acc["fp32_vs_fp64_gpu_Z_rel"] = abs(z32 - z64) / abs(z64)
+    cpu = None
+    if not args.no_cpu:
+        log("CPU baseline: one oracle gradient call at N={N} (1D, df = 0.5)".format(N=N))
+        v, nthr, _, _ = cpu_baseline(P, Y.astype(np.float32).astype(np.float64), 1, df=df)
+        cpu = {"value": v, "unit": "grad iters/s", "cores": nthr, "kind": "port",
+               "sample": f"one gradient of the same N={N} 1D workload (same Y, P, df = 0.5) by the oracle's C "
+                         f"restatement of computeFftGradientOneDVariableDf, {nthr} threads for the reference's "
+                         f"PARALLEL_FOR stages, serial spread/FFT as in the reference"}
+    out = {
+        "metric": "1D gradient iterations/s at N=2M, df=0.5 (C3), FP32 form",
+        "value": args.steps / dt, "unit": "grad iters/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * dt / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (seeded Gaussian mixture, generated in place)",
+        "config": {"workload": f"C3: N={N} 1D, df=0.5, late exaggeration 4, ~{nnz / N:.0f} nnz/row, span ~[-300, 300]",
+                   "n_boxes": geo["n_boxes"], "n1d": geo["n1d"], "fft_len": geo["fft_len"],
+                   "spread_points_per_workgroup": spread_pts},
+        "roofline": {"bound": "hbm", "kernel": "k_attr_win<1>", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes": att_bytes, "avg_ms": att_ms},
+        "cpu_baseline": cpu,
+        "fp64_plain": res64,
+        "speedup_vs_fp64": (res64["ms_per_step"] / (1e3 * dt / args.steps)) if res64 else None,
+        "accuracy": acc,
+        "kernels_ms": {k: round(v[1] / max(v[0], 1), 4) for k, v in sorted(stats.items(), key=lambda kv: -kv[1][1])},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def bench_bh(args, fg, synth):
     """Barnes-Hut (nbody_algo = 1, theta = 0.5) at N = 100,000 in 2D: each step builds the reference's
     SPTree on the host in insertion order, then walks it and sums the edge forces on the GPU (bitwise the
@@ -290,9 +358,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=5)
-    ap.add_argument("--config", default="c2", choices=["c2", "c4", "exact", "bh"],
-                    help="c2 (default): the headline step at N=1M 2D; c4: 64 batched embeddings; "
-                         "exact: theta = 0 at N=46000 2D; bh: Barnes-Hut at N=100000 2D")
+    ap.add_argument("--config", default="c2", choices=["c2", "c3", "c4", "exact", "bh"],
+                    help="c2 (default): the headline step at N=1M 2D; c3: 1D df=0.5 at N=2M; c4: 64 batched "
+                         "embeddings; exact: theta = 0 at N=46000 2D; bh: Barnes-Hut at N=100000 2D")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run one plain 64 MiB device copy (torch) as the reference kernel for "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE scaling")
@@ -307,6 +375,8 @@ def main():
 
     if args.config == "c4":
         return bench_c4(args, fg, synth)
+    if args.config == "c3":
+        return bench_c3(args, fg, synth)
     if args.config == "exact":
         return bench_exact(args, fg)
     if args.config == "bh":
