@@ -1619,18 +1619,38 @@ __global__ void k_mean_final(const DevGeo *__restrict__ dg, const double *__rest
     if (!dg->live) return;
     __shared__ double sh[kThreads / kWave];
     const int T = blockDim.x;
+    // Each thread adds part[t], part[t + T], ... of a coordinate in that order.  The loads are issued
+    // eight at a time; those past the end read as +0.0, which leaves the sum unchanged (it starts at
+    // +0.0, so it is never -0.0).  In 2D both coordinates come from one 16-byte load.
+    if (D == 2) {
+        const double2 *p2 = reinterpret_cast<const double2 *>(part);
+        double s0 = 0, s1 = 0;
+        for (int b = threadIdx.x; b < nblk; b += 8 * T) {
+            double2 v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) v[u] = b + u * T < nblk ? p2[b + u * T] : make_double2(0.0, 0.0);
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                s0 += v[u].x;
+                s1 += v[u].y;
+            }
+        }
+        s0 = block_sum(s0, sh);
+        if (threadIdx.x == 0) mean[0] = s0 / N;
+        __syncthreads();
+        s1 = block_sum(s1, sh);
+        if (threadIdx.x == 0) mean[1] = s1 / N;
+        return;
+    }
     for (int d = 0; d < D; d++) {
-        // each thread adds part[t], part[t + T], ... in that order; the loads are issued eight at a time
         double s = 0;
-        int b = threadIdx.x;
-        for (; b + 7 * T < nblk; b += 8 * T) {
+        for (int b = threadIdx.x; b < nblk; b += 8 * T) {
             double v[8];
 #pragma unroll
-            for (int u = 0; u < 8; u++) v[u] = part[(long)(b + u * T) * D + d];
+            for (int u = 0; u < 8; u++) v[u] = b + u * T < nblk ? part[(long)(b + u * T) * D + d] : 0.0;
 #pragma unroll
             for (int u = 0; u < 8; u++) s += v[u];
         }
-        for (; b < nblk; b += T) s += part[(long)b * D + d];
         s = block_sum(s, sh);
         if (threadIdx.x == 0) mean[d] = s / N;
         __syncthreads();
