@@ -364,8 +364,11 @@ __device__ __forceinline__ void attr_quad(const u32x4 &kq, const f32x4 &vq4, con
     for (int d = 0; d < D; d++) atomicAdd(&acc[cur * D + d], (unsigned long long)run[d]);
 }
 
+// In 1D a workgroup holds 80 KB of LDS, so two share a CU if each wave keeps to 64 VGPRs (8 waves per
+// SIMD); without the request the compiler used 70 and the blocks ran in two rounds.  In 2D the LDS
+// (160 KB) allows one workgroup per CU, 4 waves per SIMD.
 template <int D>
-__global__ __launch_bounds__(kAttThreads) void k_attr_win(const int *__restrict__ start,
+__global__ __launch_bounds__(kAttThreads) __attribute__((amdgpu_waves_per_eu(D == 1 ? 8 : 4))) void k_attr_win(const int *__restrict__ start,
                                                           const int *__restrict__ pure,
                                                           const uint32_t *__restrict__ key,
                                                           const float *__restrict__ wval,
