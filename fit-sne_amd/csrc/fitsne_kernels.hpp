@@ -793,15 +793,21 @@ __device__ __forceinline__ void seg_scan_step(float (&val)[NV][NQ], int run) {
         }
 }
 
+// Table slots of the spread: the table (sums and keys) stays within 40 KB, so that four workgroups fit
+// in a CU's LDS.
 template <int P, int D>
 constexpr int spread_slots() {
-    constexpr int per = (D == 2 ? P * P * 4 : P * 3) * 8;  // bytes of one table slot
+    constexpr int per = (D == 2 ? P * P * 4 : P * 3) * 8 + 4;  // bytes of one table slot: sums and key
     constexpr int s = (40 * 1024) / per;
     return s >= 256 ? 256 : s;
 }
 
+// In 2D with p <= 3 the registers are held to 128 per lane (4 waves per SIMD), so that, with the table
+// under 40 KB, four workgroups share a CU (three at the 150 registers the compiler chose on its own:
+// measured 0.355 against 0.362 ms per step at N = 1M, results unchanged).  For p = 4, 5 the limit would
+// spill most of the kernel's values, so it is not requested.
 template <int P, int D>
-__global__ __launch_bounds__(kSpreadThreads) void k_spread_f32(const float *__restrict__ Y, int N, int pts,
+__global__ __launch_bounds__(kSpreadThreads) __attribute__((amdgpu_waves_per_eu(D == 2 && P <= 3 ? 4 : 1))) void k_spread_f32(const float *__restrict__ Y, int N, int pts,
                                                                const DevGeo *__restrict__ dg,
                                                                unsigned long long *__restrict__ grid) {
     constexpr int NV = (D == 2 ? P * P : P);
