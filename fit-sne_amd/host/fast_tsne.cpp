@@ -68,7 +68,7 @@ bool get(FILE *f, T *v, size_t n = 1) {
 bool load_data(const char *path, Params &p, std::vector<double> &X, std::vector<double> &Y, bool &have_init) {
     FILE *f = std::fopen(path, "rb");
     if (!f) {
-        std::printf("Error: could not open data file.\n");
+        std::printf("[fast_tsne] cannot open the data file %s\n", path);
         return false;
     }
     bool ok = get(f, &p.n) && get(f, &p.d) && get(f, &p.theta) && get(f, &p.perplexity);
@@ -91,7 +91,7 @@ bool load_data(const char *path, Params &p, std::vector<double> &X, std::vector<
         die("malformed data file header");
     }
     if (p.nbody_algo == 2 && p.no_dims > 2)
-        die("FFT interpolation scheme supports only 1 or 2 output dimensions, not " + std::to_string(p.no_dims));
+        die("the FFT gradient needs no_dims 1 or 2, got " + std::to_string(p.no_dims));
     X.resize((size_t)p.n * p.d);
     if (!get(f, X.data(), X.size())) {
         std::fclose(f);
@@ -102,27 +102,24 @@ bool load_data(const char *path, Params &p, std::vector<double> &X, std::vector<
     Y.assign((size_t)p.n * p.no_dims, 0.0);
     have_init = get(f, Y.data(), Y.size());
     std::fclose(f);
-    std::printf("Read the following parameters:\n\t n %d by d %d dataset, theta %lf,\n"
-                "\t perplexity %lf, no_dims %d, max_iter %d,\n"
-                "\t stop_lying_iter %d, mom_switch_iter %d,\n"
-                "\t momentum %lf, final_momentum %lf,\n"
-                "\t learning_rate %lf, max_step_norm %lf,\n"
-                "\t K %d, sigma %lf, nbody_algo %d,\n"
-                "\t knn_algo %d, early_exag_coeff %lf,\n"
-                "\t no_momentum_during_exag %d, n_trees %d, search_k %d,\n"
-                "\t start_late_exag_iter %d, late_exag_coeff %lf\n"
-                "\t nterms %d, interval_per_integer %lf, min_num_intervals %d, t-dist df %lf\n",
-                p.n, p.d, p.theta, p.perplexity, p.no_dims, p.max_iter, p.stop_lying_iter, p.mom_switch_iter,
-                p.momentum, p.final_momentum, p.learning_rate, p.max_step_norm, p.K, p.sigma, p.nbody_algo,
-                p.knn_algo, p.early_exag_coeff, p.no_momentum_during_exag, p.n_trees, p.search_k,
-                p.start_late_exag_iter, p.late_exag_coeff, p.nterms, p.intervals_per_integer, p.min_num_intervals,
-                p.df);
+    // one "name=value" pair per setting, in file order (nothing parses this; it is a log)
+    std::printf("[fast_tsne] data: N=%d D=%d\n", p.n, p.d);
+    std::printf("[fast_tsne] settings: theta=%g perplexity=%g no_dims=%d max_iter=%d stop_lying_iter=%d "
+                "mom_switch_iter=%d momentum=%g final_momentum=%g learning_rate=%g max_step_norm=%g K=%d sigma=%g "
+                "nbody_algo=%d knn_algo=%d early_exag_coeff=%g no_momentum_during_exag=%d n_trees=%d search_k=%d "
+                "start_late_exag_iter=%d late_exag_coeff=%g nterms=%d intervals_per_integer=%g "
+                "min_num_intervals=%d df=%g seed=%d load_affinities=%d\n",
+                p.theta, p.perplexity, p.no_dims, p.max_iter, p.stop_lying_iter, p.mom_switch_iter, p.momentum,
+                p.final_momentum, p.learning_rate, p.max_step_norm, p.K, p.sigma, p.nbody_algo, p.knn_algo,
+                p.early_exag_coeff, p.no_momentum_during_exag, p.n_trees, p.search_k, p.start_late_exag_iter,
+                p.late_exag_coeff, p.nterms, p.intervals_per_integer, p.min_num_intervals, p.df, p.rand_seed,
+                p.load_affinities);
     if (!p.perplexity_list.empty()) {
-        std::printf("Read the list of perplexities: ");
-        for (double v : p.perplexity_list) std::printf("%f ", v);
+        std::printf("[fast_tsne] perplexities:");
+        for (double v : p.perplexity_list) std::printf(" %g", v);
         std::printf("\n");
     }
-    if (have_init) std::printf("Read the initialization successfully.\n");
+    std::printf("[fast_tsne] initial embedding: %s\n", have_init ? "given" : "random");
     return true;
 }
 
@@ -136,7 +133,7 @@ void save_data(const char *path, const std::vector<double> &Y, const std::vector
     std::fwrite(&max_iter, sizeof(int), 1, f);
     std::fwrite(costs.data(), sizeof(double), costs.size(), f);
     std::fclose(f);
-    std::printf("Wrote the %i x %i data matrix successfully.\n", n, d);
+    std::printf("[fast_tsne] result: %d x %d embedding written to %s\n", n, d, path);
 }
 
 // Column means subtracted, each mean summed over the rows in order then divided by N (tsne.cpp:1851-1876).
@@ -204,12 +201,12 @@ void write_file(const char *name, const std::vector<T> &v) {
 void prepare_data(const Params &p, std::vector<double> &X) {
     zero_mean(X, p.n, p.d);
     if (p.perplexity > 0 || !p.perplexity_list.empty()) {
-        std::printf("Using perplexity, so normalizing input data (to prevent numerical problems)\n");
+        std::printf("[fast_tsne] data centred and divided by max |x| (perplexity calibration)\n");
         double mx = 0.0;
         for (double v : X) mx = std::max(mx, std::fabs(v));
         for (double &v : X) v /= mx;
     } else {
-        std::printf("Not using perplexity, so data are left un-normalized.\n");
+        std::printf("[fast_tsne] data centred, not rescaled (fixed sigma)\n");
     }
 }
 
@@ -221,12 +218,12 @@ double exact_affinities(Params &p, std::vector<double> &X, fitsne_ctx *ctx, bool
     const size_t nn = (size_t)N * N;
     std::vector<double> P;
     if (p.load_affinities == 1) {
-        std::printf("Loading exact input similarities from file...\n");
+        std::printf("[fast_tsne] affinities: dense P read from P.dat\n");
         if (!read_file("P.dat", P, nn)) die("could not read P.dat");
         check(fitsne_set_P_dense(ctx, N, P.data()), "upload P");
     } else {
         prepare_data(p, X);
-        std::printf("Theta set to 0, so running exact algorithm\n");
+        std::printf("[fast_tsne] affinities: dense P on the GPU (theta = 0)\n");
         const bool need_host = p.load_affinities == 2 || want_max_row;
         if (need_host) P.resize(nn);
         double sum_P = 0.0;
@@ -234,9 +231,9 @@ double exact_affinities(Params &p, std::vector<double> &X, fitsne_ctx *ctx, bool
         check(fitsne_exact_affinities(ctx, X.data(), N, D, p.perplexity, p.sigma, (int)p.perplexity_list.size(), plist,
                                       need_host ? P.data() : nullptr, &sum_P),
               "exact input similarities");
-        std::printf("Finished exact calculation of the P.  Sum_p: %lf \n", sum_P);
+        std::printf("[fast_tsne] dense P total before normalisation: %.17g\n", sum_P);
         if (p.load_affinities == 2) {
-            std::printf("Saving exact input similarities to file...\n");
+            std::printf("[fast_tsne] dense P written to P.dat\n");
             write_file("P.dat", P);
         }
     }
@@ -256,7 +253,7 @@ Csr affinities(Params &p, std::vector<double> &X) {
     const int N = p.n, D = p.d;
     Csr P;
     if (p.load_affinities == 1) {
-        std::printf("Loading approximate input similarities from files...\n");
+        std::printf("[fast_tsne] affinities: sparse P read from P_row.dat, P_col.dat, P_val.dat\n");
         if (!read_file("P_row.dat", P.row, (size_t)N + 1)) die("could not read P_row.dat");
         const size_t nnz = P.row[N];
         if (!read_file("P_val.dat", P.val, nnz) || !read_file("P_col.dat", P.col, nnz))
@@ -267,7 +264,7 @@ Csr affinities(Params &p, std::vector<double> &X) {
     int K;
     double sigma;
     if (p.perplexity < 0) {
-        std::printf("Using manually set kernel width\n");
+        std::printf("[fast_tsne] affinities: fixed sigma %g with K = %d\n", p.sigma, p.K);
         K = p.K;
         sigma = p.sigma;
     } else {
@@ -281,8 +278,8 @@ Csr affinities(Params &p, std::vector<double> &X) {
         sigma = -1.0;
     }
     if (K < 1 || K > N - 1) die("the number of neighbours K = " + std::to_string(K) + " must be in [1, N-1]");
-    if (p.perplexity > K) std::printf("Perplexity should be lower than K!\n");
-    std::printf("Exact nearest neighbour search on the GPU (K = %d)\n", K);
+    if (p.perplexity > K) std::printf("[fast_tsne] warning: perplexity %g exceeds K = %d\n", p.perplexity, K);
+    std::printf("[fast_tsne] exact %d nearest neighbours on the GPU\n", K);
     std::vector<int32_t> idx((size_t)N * K);
     std::vector<double> dist((size_t)N * K);
     check(fitsne_knn_exact(X.data(), N, D, K, idx.data(), dist.data()), "nearest neighbours");
@@ -304,13 +301,13 @@ Csr affinities(Params &p, std::vector<double> &X) {
     for (int n = 0; n <= N; n++) P.row[n] = (uint32_t)((size_t)n * K);
     P.col.assign(idx.begin(), idx.end());
     P.val = row;
-    std::printf("Symmetrizing...\n");
+    std::printf("[fast_tsne] symmetrising P on the GPU\n");
     Csr S = symmetrize(P, N);
     double sum = 0.0;
     for (double v : S.val) sum += v;
     for (double &v : S.val) v /= sum;
     if (p.load_affinities == 2) {
-        std::printf("Saving approximate input similarities to files...\n");
+        std::printf("[fast_tsne] sparse P written to P_row.dat, P_col.dat, P_val.dat\n");
         write_file("P_val.dat", S.val);
         write_file("P_col.dat", S.col);
         write_file("P_row.dat", S.row);
@@ -321,31 +318,31 @@ Csr affinities(Params &p, std::vector<double> &X) {
 }  // namespace
 
 int main(int argc, char **argv) {
-    std::printf("=============== t-SNE v%s (MI355X) ===============\n", kVersion);
+    std::printf("[fast_tsne] FIt-SNE %s interface on MI355X\n", kVersion);
     if (argc < 2) {
-        std::printf("Please pass version number as first argument.\n");
+        std::printf("[fast_tsne] usage: fast_tsne %s [data_path] [result_path] [nthreads]\n", kVersion);
         return -1;
     }
     if (std::strcmp(argv[1], kVersion) != 0) {
-        std::printf("Wrapper passed wrong version number: %s\n", argv[1]);
+        std::printf("[fast_tsne] wrong version %s: this program speaks %s\n", argv[1], kVersion);
         return -1;
     }
     const char *data_path = argc >= 3 ? argv[2] : "data.dat";
     const char *result_path = argc >= 4 ? argv[3] : "result.dat";
     const unsigned nthreads = argc >= 5 ? (unsigned)std::strtoul(argv[4], nullptr, 10) : 0u;
-    std::printf("fast_tsne data_path: %s\nfast_tsne result_path: %s\nfast_tsne nthreads: %u\n", data_path, result_path,
-                nthreads);
+    std::printf("[fast_tsne] data %s, result %s, nthreads %u (unused: the GPU sets the parallelism)\n", data_path,
+                result_path, nthreads);
 
     Params p;
     std::vector<double> X, Y;
     bool have_init = false;
     if (!load_data(data_path, p, X, Y, have_init)) return 0;
     const int N = p.n, nd = p.no_dims;
-    if (N - 1 < 3 * p.perplexity) die("Perplexity too large for the number of data points!");
+    if (N - 1 < 3 * p.perplexity) die("perplexity too large: 3 x perplexity must be below N");
     const bool exact = p.theta == 0.0;
     const bool bh = !exact && p.nbody_algo == 1;
-    if (!exact && p.nbody_algo != 1 && p.nbody_algo != 2) die("Error: Undefined algorithm");
-    if (!exact && p.knn_algo != 1 && p.knn_algo != 2) die("Invalid knn_algo param");
+    if (!exact && p.nbody_algo != 1 && p.nbody_algo != 2) die("undefined algorithm: nbody_algo must be 1 or 2");
+    if (!exact && p.knn_algo != 1 && p.knn_algo != 2) die("invalid knn_algo: must be 1 or 2");
     if (exact && nd > 4) die("the exact mode of this build supports no_dims <= 4");
     if (bh && nd > 3) die("the Barnes-Hut mode of this build supports no_dims <= 3");
     if (fitsne_device_count() < 1) die("no GPU device is visible");
@@ -360,10 +357,11 @@ int main(int argc, char **argv) {
         check(fitsne_ctx_create(&ctx, 0, precision, nd, p.nterms, p.intervals_per_integer, p.min_num_intervals, p.df),
               "context");
         max_row_sum = exact_affinities(p, X, ctx, p.early_exag_coeff == 0);
-        std::printf("Input similarities computed \n");
+        std::printf("[fast_tsne] affinities ready (dense)\n");
     } else {
         Csr P = affinities(p, X);
-        std::printf("Input similarities computed (sparsity = %f)!\n", (double)P.row[N] / ((double)N * (double)N));
+        std::printf("[fast_tsne] affinities ready: %u entries, density %.3g\n", P.row[N],
+                    (double)P.row[N] / ((double)N * (double)N));
         if (p.early_exag_coeff == 0)
             for (int n = 0; n < N; n++) {
                 double s = 0.0;
@@ -374,35 +372,32 @@ int main(int argc, char **argv) {
               "context");
         check(fitsne_set_P(ctx, N, P.row.data(), P.col.data(), P.val.data()), "upload P");
         if (bh) {
-            std::printf("Using the Barnes-Hut approximation.\n");
+            std::printf("[fast_tsne] Barnes-Hut gradient, theta %g\n", p.theta);
             check(fitsne_set_theta(ctx, p.theta), "theta");
         }
     }
 
     if (!have_init) {
         if (p.rand_seed >= 0) {
-            std::printf("Using random seed: %d\n", p.rand_seed);
+            std::printf("[fast_tsne] random start, seed %d\n", p.rand_seed);
             std::srand((unsigned)p.rand_seed);
         } else {
-            std::printf("Using current time as random seed...\n");
+            std::printf("[fast_tsne] random start, seeded from the clock\n");
             std::srand((unsigned)std::time(nullptr));
         }
-        std::printf("Randomly initializing the solution.\n");
         for (double &v : Y) v = randn() * .0001;
-    } else {
-        std::printf("Using the given initialization.\n");
     }
 
     // largest stable exaggeration (tsne.cpp:389-401; the reference's exact mode has no row_P there, the
     // row sums of the dense P are its natural counterpart)
     double early = p.early_exag_coeff;
     if (early == 0) early = 1.0 / (p.learning_rate * max_row_sum);
-    std::printf("Exaggerating Ps by %f\n", early);
+    std::printf("[fast_tsne] early exaggeration %g\n", early);
 
     double exag = early;
     check(fitsne_set_exaggeration(ctx, exag), "exaggeration");
     check(fitsne_set_Y(ctx, Y.data()), "upload Y");
-    std::printf("Learning embedding (%s)...\n", precision == FITSNE_FP32        ? "FP32 form"
+    std::printf("[fast_tsne] optimising (%s)\n", precision == FITSNE_FP32        ? "FP32 form"
                                                  : precision == FITSNE_FP64_EXACT ? "exact, FP64"
                                                  : precision == FITSNE_FP64_BH    ? "Barnes-Hut, FP64"
                                                                                   : "FP64 plain");
@@ -421,12 +416,12 @@ int main(int argc, char **argv) {
         check(fitsne_step(ctx, &sp, nullptr), "gradient step");
         bool changed = false;
         if (iter == p.stop_lying_iter) {
-            std::printf("Unexaggerating Ps by %f\n", early);
+            std::printf("[fast_tsne] iteration %d: early exaggeration off\n", iter);
             exag /= early;
             changed = true;
         }
         if (iter == p.start_late_exag_iter) {
-            std::printf("Exaggerating Ps by %f\n", p.late_exag_coeff);
+            std::printf("[fast_tsne] iteration %d: late exaggeration %g\n", iter, p.late_exag_coeff);
             exag *= p.late_exag_coeff;
             changed = true;
         }
@@ -440,14 +435,14 @@ int main(int argc, char **argv) {
                 C = C / p.late_exag_coeff - std::log(p.late_exag_coeff);
             costs[iter] = C;
             const auto now = std::chrono::steady_clock::now();
-            std::printf("Iteration %d (50 iterations in %.2f seconds), cost %f\n", iter + 1,
-                        std::chrono::duration<double>(now - t0).count(), C);
+            std::printf("[fast_tsne] iteration %d: KL %.6f (%.2f s since the last report)\n", iter + 1, C,
+                        std::chrono::duration<double>(now - t0).count());
             t0 = now;
         }
     }
     check(fitsne_get_Y(ctx, Y.data()), "download Y");
     fitsne_ctx_destroy(ctx);
     save_data(result_path, Y, costs, N, nd, p.max_iter);
-    std::printf("Done.\n\n");
+    std::printf("[fast_tsne] done\n");
     return 0;
 }

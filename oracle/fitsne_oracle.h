@@ -32,8 +32,12 @@
  *   orc_evaluate_error_exact  reference evaluateError (oracle/_ref), bit-exact
  *   orc_bh_gradient / orc_bh_error  reference computeGradient (SPTree) / evaluateError (oracle/_ref),
  *                             bit-exact; the exact O(N^2) gradient as theta -> 0
- *   orc_update / orc_run_fft  elementwise closed forms of tsne.cpp:478-544 (parity pinned by formula
- *                             tests; the reference's TSNE::run cannot be linked without FFTW)
+ *   orc_update                elementwise closed form of tsne.cpp:478-514
+ *   orc_run_fft / _bh / _exact one loop (exaggeration switches, momentum switch, no_momentum_during_exag,
+ *                             step clip, KL correction) around the three gradient sources; the Barnes-Hut
+ *                             and exact forms are compared with the reference's own TSNE::run
+ *                             (oracle/_ref/libfitsne_ref_run.so, tests/test_oracle_run.py), which pins the
+ *                             loop the FFT form shares
  */
 #ifndef FITSNE_ORACLE_H
 #define FITSNE_ORACLE_H
@@ -137,6 +141,12 @@ typedef struct {
  * initial embedding and receives the result; costs[max_iter] gets the KL entries. */
 int orc_run_fft(const uint32_t *row_P, const uint32_t *col_P, const double *val_P, int N, int no_dims,
                 double *Y, const orc_run_params *prm, double *costs);
+/* The same loop with the Barnes-Hut gradient and KL (nbody_algorithm = 1, tsne.cpp:458-460, :556-558), and
+ * with the exact gradient over a dense P[N*N] (theta = 0, tsne.cpp:442-443, :552-553; early_exag_coeff
+ * must be given: its automatic value needs the CSR).  Returns 0, or -1 for an unsupported request. */
+int orc_run_bh(const uint32_t *row_P, const uint32_t *col_P, const double *val_P, int N, int no_dims,
+               double *Y, const orc_run_params *prm, double theta, double *costs);
+int orc_run_exact(const double *P, int N, int no_dims, double *Y, const orc_run_params *prm, double *costs);
 
 /* ---- affinities (tsne.cpp:1394-1500, :1643-1726, :1730-1828) ---- */
 double orc_distances2similarities(const double *D, double *P, int N, int n, double perplexity,

@@ -394,38 +394,49 @@ double orc_evaluate_error_fft(const uint32_t *row_P, const uint32_t *col_P, cons
     return C;
 }
 
-/* tsne.cpp:389-412, :437-577 (FFT branch) */
-int orc_run_fft(const uint32_t *row_P, const uint32_t *col_P, const double *val_P_in, int N, int no_dims,
-                double *Y, const orc_run_params *prm, double *costs) {
+/* TSNE::run's hot loop (tsne.cpp:389-412, :437-577) around one of its three gradient sources
+ * (tsne.cpp:440-467): the FFT variants (nbody_algorithm 2), Barnes-Hut (1) or the exact dense gradient
+ * (theta = 0), each with its own KL (tsne.cpp:551-559).  P (nval entries: the CSR values, or the dense
+ * N x N matrix) is a private copy exaggerated in place as the reference does. */
+enum { RUN_FFT = 0, RUN_BH = 1, RUN_EXACT = 2 };
+
+static int run_loop(int kind, const uint32_t *row_P, const uint32_t *col_P, const double *P_in, long nval,
+                    int N, int no_dims, double *Y, const orc_run_params *prm, double theta, double *costs) {
     long ND = (long)N * no_dims;
-    uint32_t nnz = row_P[N];
-    double *val_P = (double *)malloc(sizeof(double) * (size_t)nnz);
-    memcpy(val_P, val_P_in, sizeof(double) * (size_t)nnz);
-    double *dY = (double *)malloc(sizeof(double) * (size_t)ND);
-    double *uY = (double *)calloc((size_t)ND, sizeof(double));
-    double *gains = (double *)malloc(sizeof(double) * (size_t)ND);
+    double early = prm->early_exag_coeff;
+    /* the automatic coefficient sums CSR rows (tsne.cpp:391-402); the exact mode has no CSR there */
+    if (early == 0 && kind == RUN_EXACT) return -1;
+    double *P = (double *)malloc(sizeof(double) * (size_t)(nval > 0 ? nval : 1));
+    double *dY = (double *)malloc(sizeof(double) * (size_t)(ND > 0 ? ND : 1));
+    double *uY = (double *)calloc((size_t)(ND > 0 ? ND : 1), sizeof(double));
+    double *gains = (double *)malloc(sizeof(double) * (size_t)(ND > 0 ? ND : 1));
+    memcpy(P, P_in, sizeof(double) * (size_t)nval);
     for (long i = 0; i < ND; i++) gains[i] = 1.0;
     double momentum = prm->momentum;
-    double early = prm->early_exag_coeff;
     if (early == 0) {
         double max_sum_cols = 0;
         for (int n = 0; n < N; n++) {
             double rs = 0;
-            for (uint32_t i = row_P[n]; i < row_P[n + 1]; i++) rs += val_P[i];
+            for (uint32_t i = row_P[n]; i < row_P[n + 1]; i++) rs += P[i];
             if (rs > max_sum_cols) max_sum_cols = rs;
         }
         early = (1.0 / (prm->learning_rate * max_sum_cols));
     }
-    for (uint32_t i = 0; i < nnz; i++) val_P[i] *= early;
-    for (long i = 0; i < N; i++) costs[i < prm->max_iter ? i : 0] = costs[i < prm->max_iter ? i : 0];
+    for (long i = 0; i < nval; i++) P[i] *= early;
     for (int i = 0; i < prm->max_iter; i++) costs[i] = 0.0;
 
     double sum_Q = 0;
+    int rc = 0;
     for (int iter = 0; iter < prm->max_iter; iter++) {
-        int rc = orc_compute_fft_gradient(row_P, col_P, val_P, Y, N, no_dims, dY, prm->n_interpolation_points,
+        if (kind == RUN_FFT)
+            rc = orc_compute_fft_gradient(row_P, col_P, P, Y, N, no_dims, dY, prm->n_interpolation_points,
                                           prm->intervals_per_integer, prm->min_num_intervals, prm->nthreads,
                                           prm->df, &sum_Q);
-        if (rc) { free(val_P); free(dY); free(uY); free(gains); return rc; }
+        else if (kind == RUN_BH)
+            orc_bh_gradient(row_P, col_P, P, Y, N, no_dims, theta, dY, &sum_Q);
+        else
+            orc_exact_gradient(P, Y, N, no_dims, prm->df, dY, &sum_Q);
+        if (rc) break;
         if (prm->no_momentum_during_exag) {
             if (iter > prm->stop_lying_iter)
                 orc_update(Y, uY, gains, dY, N, no_dims, momentum, prm->learning_rate, -1.0, 0);
@@ -436,20 +447,37 @@ int orc_run_fft(const uint32_t *row_P, const uint32_t *col_P, const double *val_
         }
         orc_zero_mean(Y, N, no_dims);
         if (iter == prm->stop_lying_iter)
-            for (uint32_t i = 0; i < nnz; i++) val_P[i] /= early;
+            for (long i = 0; i < nval; i++) P[i] /= early;
         if (iter == prm->start_late_exag_iter)
-            for (uint32_t i = 0; i < nnz; i++) val_P[i] *= prm->late_exag_coeff;
+            for (long i = 0; i < nval; i++) P[i] *= prm->late_exag_coeff;
         if (iter == prm->mom_switch_iter) momentum = prm->final_momentum;
         if ((iter + 1) % 50 == 0 || iter == prm->max_iter - 1) {
-            double C = orc_evaluate_error_fft(row_P, col_P, val_P, Y, N, no_dims, sum_Q, prm->df);
+            double C;
+            if (kind == RUN_FFT) C = orc_evaluate_error_fft(row_P, col_P, P, Y, N, no_dims, sum_Q, prm->df);
+            else if (kind == RUN_BH) C = orc_bh_error(row_P, col_P, P, Y, N, no_dims, theta);
+            else C = orc_evaluate_error_exact(P, Y, N, no_dims, prm->df);
             if (iter < prm->stop_lying_iter && prm->stop_lying_iter != -1) C = C / early - log(early);
             if (iter >= prm->start_late_exag_iter && prm->start_late_exag_iter != -1)
                 C = C / prm->late_exag_coeff - log(prm->late_exag_coeff);
             costs[iter] = C;
         }
     }
-    free(val_P); free(dY); free(uY); free(gains);
-    return 0;
+    free(P); free(dY); free(uY); free(gains);
+    return rc;
+}
+
+int orc_run_fft(const uint32_t *row_P, const uint32_t *col_P, const double *val_P, int N, int no_dims,
+                double *Y, const orc_run_params *prm, double *costs) {
+    return run_loop(RUN_FFT, row_P, col_P, val_P, (long)row_P[N], N, no_dims, Y, prm, 0.0, costs);
+}
+
+int orc_run_bh(const uint32_t *row_P, const uint32_t *col_P, const double *val_P, int N, int no_dims,
+               double *Y, const orc_run_params *prm, double theta, double *costs) {
+    return run_loop(RUN_BH, row_P, col_P, val_P, (long)row_P[N], N, no_dims, Y, prm, theta, costs);
+}
+
+int orc_run_exact(const double *P, int N, int no_dims, double *Y, const orc_run_params *prm, double *costs) {
+    return run_loop(RUN_EXACT, NULL, NULL, P, (long)N * N, N, no_dims, Y, prm, 0.0, costs);
 }
 
 /* ---- affinities ---- */

@@ -11,6 +11,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
 REF_SO = os.path.join(ROOT, "oracle", "_ref", "libfitsne_ref.so")
+REF_RUN_SO = os.path.join(ROOT, "oracle", "_ref", "libfitsne_ref_run.so")
 
 dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
@@ -69,6 +70,8 @@ def oracle():
         _bind(L, "orc_update", None, [dp, dp, dp, dp, ci, ci, cd, cd, cd, ci])
         _bind(L, "orc_evaluate_error_fft", cd, [u32p, u32p, dp, dp, ci, ci, cd, cd])
         _bind(L, "orc_run_fft", ci, [u32p, u32p, dp, ci, ci, dp, C.POINTER(RunParams), dp])
+        _bind(L, "orc_run_bh", ci, [u32p, u32p, dp, ci, ci, dp, C.POINTER(RunParams), cd, dp])
+        _bind(L, "orc_run_exact", ci, [dp, ci, ci, dp, C.POINTER(RunParams), dp])
         _bind(L, "orc_distances2similarities", cd, [dp, dp, ci, ci, cd, cd, ci])
         _bind(L, "orc_d2s_bisect", cd, [dp, dp, ci, cd, cd, ci, C.POINTER(ci), C.POINTER(cd)])
         _bind(L, "orc_distances2similarities_list", cd, [dp, dp, ci, ci, cd, cd, ci, ci, dp])
@@ -114,6 +117,55 @@ def ref():
         _bind(L, "ref_save_data", None, [C.c_char_p, dp, dp, ci, ci, ci])
         _ref = L
     return _ref
+
+
+_ref_run = None
+
+
+def have_ref_run():
+    return os.path.exists(REF_RUN_SO)
+
+
+def ref_run():
+    """The reference's TSNE::run (oracle/_ref/libfitsne_ref_run.so).  FFTW's entry points are unresolved in
+    that library, so it is opened with lazy binding; only its Barnes-Hut and exact branches are driven.
+    It reads its affinities from the current directory (load_affinities = 1): call it from a directory of
+    its own (tests/ref_run_child.py)."""
+    global _ref_run
+    if _ref_run is None:
+        L = C.CDLL(REF_RUN_SO, mode=os.RTLD_LAZY)
+        _bind(L, "ref_tsne_run", ci, [dp, ci, ci, dp, ci, cd, cd, ci, ci, ci, cd, cd, cd, ci, cd, dp, ci, ci, cd,
+                                      cd, cd])
+        _ref_run = L
+    return _ref_run
+
+
+def ref_tsne_run(workdir, Y0, settings, P=None, Pd=None):
+    """The reference's TSNE::run with the given affinities, in a child process whose working directory is
+    `workdir` (tests/ref_run_child.py): the CSR P is written to P_row.dat / P_col.dat / P_val.dat, a dense
+    Pd to P.dat, where its load_affinities = 1 branch reads them (tsne.cpp:172-276).  `settings` holds
+    perplexity, theta, max_iter, stop_lying_iter, mom_switch_iter, momentum, final_momentum,
+    learning_rate, nbody_algorithm, early_exag_coeff, no_momentum_during_exag, start_late_exag_iter,
+    late_exag_coeff, df and max_step_norm.  Returns (Y, costs)."""
+    import subprocess
+    import sys
+    os.makedirs(workdir, exist_ok=True)
+    if P is not None:
+        row, col, val = P
+        np.asarray(row, dtype=np.uint32).tofile(os.path.join(workdir, "P_row.dat"))
+        np.asarray(col, dtype=np.uint32).tofile(os.path.join(workdir, "P_col.dat"))
+        np.asarray(val, dtype=np.float64).tofile(os.path.join(workdir, "P_val.dat"))
+    if Pd is not None:
+        np.ascontiguousarray(Pd, dtype=np.float64).tofile(os.path.join(workdir, "P.dat"))
+    np.savez(os.path.join(workdir, "run.npz"), Y0=np.asarray(Y0, dtype=np.float64), **settings)
+    child = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ref_run_child.py")
+    r = subprocess.run([sys.executable, child], cwd=workdir, capture_output=True, text=True, timeout=300)
+    if r.returncode != 0:
+        raise RuntimeError("reference run failed: " + r.stdout[-2000:] + r.stderr[-2000:])
+    out = np.load(os.path.join(workdir, "out.npz"))
+    if int(out["rc"]) != 0:
+        raise RuntimeError(f"TSNE::run returned {int(out['rc'])}")
+    return out["Y"], out["costs"]
 
 
 # ---------------------------------------------------------------- convenience wrappers
@@ -290,11 +342,8 @@ def normalize(row, col, val):
     return row, col, val / s
 
 
-def run_fft(P, Y0, **kw):
-    row, col, val = P
-    Y = np.ascontiguousarray(Y0, dtype=np.float64).copy()
-    N, D = Y.shape
-    prm = RunParams(
+def run_params(**kw):
+    return RunParams(
         max_iter=kw.get("max_iter", 1000), stop_lying_iter=kw.get("stop_lying_iter", 250),
         mom_switch_iter=kw.get("mom_switch_iter", 250), start_late_exag_iter=kw.get("start_late_exag_iter", -1),
         no_momentum_during_exag=kw.get("no_momentum_during_exag", 0),
@@ -304,8 +353,41 @@ def run_fft(P, Y0, **kw):
         late_exag_coeff=kw.get("late_exag_coeff", -1.0), max_step_norm=kw.get("max_step_norm", 5.0),
         df=kw.get("df", 1.0), intervals_per_integer=kw.get("intervals_per_integer", 1.0),
         nthreads=kw.get("nthreads", 1))
+
+
+def run_fft(P, Y0, **kw):
+    """The oracle's TSNE::run loop with the FFT gradient (orc_run_fft)."""
+    row, col, val = P
+    Y = np.ascontiguousarray(Y0, dtype=np.float64).copy()
+    N, D = Y.shape
+    prm = run_params(**kw)
     costs = np.zeros(prm.max_iter)
     rc = oracle().orc_run_fft(row, col, val, N, D, Y.reshape(-1), C.byref(prm), costs)
+    assert rc == 0
+    return Y, costs
+
+
+def run_bh(P, Y0, theta=0.5, **kw):
+    """The same loop with the Barnes-Hut gradient and KL (orc_run_bh)."""
+    row, col, val = P
+    Y = np.ascontiguousarray(Y0, dtype=np.float64).copy()
+    N, D = Y.shape
+    prm = run_params(**kw)
+    costs = np.zeros(prm.max_iter)
+    rc = oracle().orc_run_bh(row, col, np.ascontiguousarray(val, dtype=np.float64), N, D, Y.reshape(-1),
+                             C.byref(prm), theta, costs)
+    assert rc == 0
+    return Y, costs
+
+
+def run_exact(Pd, Y0, **kw):
+    """The same loop with the exact gradient over a dense P (orc_run_exact)."""
+    Y = np.ascontiguousarray(Y0, dtype=np.float64).copy()
+    N, D = Y.shape
+    prm = run_params(**kw)
+    costs = np.zeros(prm.max_iter)
+    rc = oracle().orc_run_exact(np.ascontiguousarray(Pd, dtype=np.float64).reshape(-1), N, D, Y.reshape(-1),
+                                C.byref(prm), costs)
     assert rc == 0
     return Y, costs
 

@@ -129,22 +129,30 @@ def _run_binary(args, cwd):
 
 def test_binary_rejects_a_wrong_version(tmp_path):
     r = _run_binary(["1.2.0"], tmp_path)
-    assert r.returncode != 0 and "wrong version" in r.stdout
+    assert r.returncode != 0 and "wrong version 1.2.0" in r.stdout
+
+
+def _echoed_settings(out):
+    line = next(ln for ln in out.splitlines() if ln.startswith("[fast_tsne] settings:"))
+    return {k: v for k, v in (kv.split("=") for kv in line.split(":", 1)[1].split())}
 
 
 def test_binary_reads_the_data_file(tmp_path):
-    """The program echoes what it parsed (as the reference's load_data does) before it needs a GPU."""
+    """The program logs what it parsed before it needs a GPU; those values are the file's."""
     rng = np.random.default_rng(2)
     X = rng.normal(size=(60, 4))
-    _write(tmp_path / "d.dat", X, rng.normal(size=(60, 2)) * 1e-4, perplexity=5.0, max_iter=3)
+    kw = _write(tmp_path / "d.dat", X, rng.normal(size=(60, 2)) * 1e-4, perplexity=5.0, max_iter=3)
     r = _run_binary([ft.VERSION, "d.dat", "r.dat", "1"], tmp_path)
     out = r.stdout
-    assert "n 60 by d 4 dataset, theta 0.500000" in out
-    assert "perplexity 5.000000, no_dims 2, max_iter 3" in out
-    assert "learning_rate 321.500000, max_step_norm 4.500000" in out
-    assert "knn_algo 2, early_exag_coeff 11.000000" in out
-    assert "nterms 3, interval_per_integer 1.000000, min_num_intervals 50, t-dist df 0.750000" in out
-    assert "Read the initialization successfully." in out
+    assert "[fast_tsne] data: N=60 D=4" in out
+    got = _echoed_settings(out)
+    for k in ["theta", "perplexity", "learning_rate", "max_step_norm", "early_exag_coeff", "df", "sigma",
+              "intervals_per_integer"]:
+        assert float(got[k]) == kw[k], k
+    for k in ["max_iter", "K", "nbody_algo", "knn_algo", "n_trees", "search_k", "nterms", "min_num_intervals"]:
+        assert int(got[k]) == kw[k], k
+    assert int(got["no_dims"]) == 2 and int(got["stop_lying_iter"]) == 3 and int(got["seed"]) == 123
+    assert "[fast_tsne] initial embedding: given" in out
     if r.returncode != 0:  # no GPU here: the program stops with a message instead of computing on the CPU
         assert "no GPU device is visible" in r.stderr
 
@@ -153,11 +161,11 @@ def test_binary_rejects_invalid_settings(tmp_path):
     X = np.random.default_rng(3).normal(size=(40, 3))
     _write(tmp_path / "d.dat", X, "random", perplexity=5.0, nbody_algo=3)
     r = _run_binary([ft.VERSION, "d.dat", "r.dat"], tmp_path)
-    assert r.returncode != 0 and "Undefined algorithm" in r.stderr
+    assert r.returncode != 0 and "undefined algorithm" in r.stderr
     # the FFT setting limits no_dims to 1 or 2 whatever theta is (tsne.cpp:1954-1957)
     _write(tmp_path / "d.dat", X, "random", perplexity=5.0, theta=0.0, map_dims=3)
     r = _run_binary([ft.VERSION, "d.dat", "r.dat"], tmp_path)
-    assert r.returncode != 0 and "supports only 1 or 2 output dimensions" in r.stderr
+    assert r.returncode != 0 and "needs no_dims 1 or 2" in r.stderr
 
 
 def test_binary_accepts_the_exact_and_barnes_hut_modes(tmp_path):
@@ -173,3 +181,93 @@ def test_binary_accepts_the_exact_and_barnes_hut_modes(tmp_path):
         else:
             Y, _ = ft.read_result_file(tmp_path / "r.dat", 3)
             assert Y.shape == (40, dims) and np.isfinite(Y).all()
+
+
+# ---------------------------------------------------------------- against the reference's own parser and writer
+
+import orc  # noqa: E402
+
+needs_ref = pytest.mark.skipif(not orc.have_ref(), reason="oracle/_ref not built (reference sources absent)")
+
+REF_INTS = ["n", "d", "no_dims", "max_iter", "stop_lying_iter", "mom_switch_iter", "K", "nbody_algo", "knn_algo",
+            "no_momentum_during_exag", "n_trees", "search_k", "start_late_exag_iter", "nterms", "min_num_intervals",
+            "seed", "load_affinities", "skip_random_init", "perplexity_list_length"]
+REF_DBLS = ["theta", "perplexity", "momentum", "final_momentum", "learning_rate", "max_step_norm", "sigma",
+            "early_exag_coeff", "late_exag_coeff", "intervals_per_integer", "df"]
+
+
+def ref_load(path, n, d, nd, L):
+    ints = np.zeros(19, dtype=np.int32)
+    dbls = np.zeros(11)
+    X = np.zeros(n * d)
+    Y = np.full(n * nd, np.nan)
+    pl = np.zeros(max(L, 1))
+    rc = orc.ref().ref_load_data(str(path).encode(), ints, dbls, X.ctypes.data, Y.ctypes.data, pl.ctypes.data)
+    assert rc == 0
+    return dict(zip(REF_INTS, ints.tolist())), dict(zip(REF_DBLS, dbls.tolist())), X.reshape(n, d), Y, pl[:L]
+
+
+VARIANTS = {
+    "perplexity_given_init": dict(perplexity=30.0, plist=None, init="given"),
+    "perplexity_list_random_init": dict(perplexity=0.0, plist=[5.0, 10.0, 2.5], init="random"),
+    "sigma_K": dict(perplexity=-1.0, plist=None, init="random", sigma=2.0, K=15),
+    "load_affinities_late_exag": dict(perplexity=12.0, plist=None, init="given", load_affinities=1, df=0.6,
+                                      start_late_exag_iter=5, late_exag_coeff=4.0, no_momentum_during_exag=1,
+                                      nbody_algo=1, theta=0.3, map_dims=1),
+}
+
+
+@needs_ref
+@pytest.mark.parametrize("variant", list(VARIANTS))
+def test_reference_load_data_reads_the_wrapper_file(tmp_path, variant):
+    """The data file written by fast_tsne.write_data_file, parsed by the reference's own TSNE::load_data
+    (tsne.cpp:1895-2022, compiled from its sources): every field, X, the initial Y and the perplexity list."""
+    v = dict(VARIANTS[variant])
+    rng = np.random.default_rng(hash(variant) % 1000)
+    n, d = 41, 6
+    nd = v.pop("map_dims", 2)
+    X = rng.normal(size=(n, d))
+    init = rng.normal(size=(n, nd)) if v.pop("init") == "given" else "random"
+    perplexity, plist = v.pop("perplexity"), v.pop("plist")
+    kw = _write(tmp_path / "d.dat", X, init, perplexity=perplexity, plist=plist, map_dims=nd, **v)
+    L = len(plist) if plist else 0
+    ints, dbls, Xr, Yr, plr = ref_load(tmp_path / "d.dat", n, d, nd, L)
+    assert (ints["n"], ints["d"], ints["no_dims"]) == (n, d, nd)
+    assert ints["stop_lying_iter"] == kw["stop_early_exag_iter"]
+    for k in ["max_iter", "mom_switch_iter", "K", "nbody_algo", "knn_algo", "no_momentum_during_exag", "n_trees",
+              "search_k", "start_late_exag_iter", "nterms", "min_num_intervals", "seed", "load_affinities"]:
+        assert ints[k] == kw[k], k
+    for k in REF_DBLS:
+        assert dbls[k] == kw[k], k
+    assert ints["skip_random_init"] == (0 if isinstance(init, str) else 1)
+    np.testing.assert_array_equal(Xr, X)
+    if not isinstance(init, str):
+        np.testing.assert_array_equal(Yr, init.reshape(-1))
+    assert ints["perplexity_list_length"] == L and list(plr) == (plist or [])
+    # the drop-in program parses the same file to the same values
+    r = _run_binary([ft.VERSION, "d.dat", "r.dat"], tmp_path)
+    got = _echoed_settings(r.stdout)
+    for k in ["max_iter", "K", "nbody_algo", "knn_algo", "n_trees", "search_k", "nterms", "min_num_intervals",
+              "start_late_exag_iter", "load_affinities", "no_momentum_during_exag", "no_dims"]:
+        assert int(got[k]) == ints[k], k
+    for k in ["theta", "perplexity", "learning_rate", "max_step_norm", "early_exag_coeff", "late_exag_coeff",
+              "df", "sigma", "intervals_per_integer", "momentum", "final_momentum"]:
+        assert float(got[k]) == pytest.approx(dbls[k], rel=1e-5), k
+
+
+@needs_ref
+@pytest.mark.parametrize("max_iter", [120, 100, 1])
+def test_result_file_from_the_reference_save_data(tmp_path, max_iter):
+    """A result file written by the reference's TSNE::save_data (tsne.cpp:2024-2038) read by
+    fast_tsne.read_result_file: the embedding exactly, the losses at every 50th and the last iteration,
+    NaN elsewhere (the wrapper's rule, fast_tsne.py:328)."""
+    n, d = 13, 2
+    rng = np.random.default_rng(max_iter)
+    Y = rng.normal(size=n * d)
+    costs = rng.uniform(1, 3, size=max_iter)
+    orc.ref().ref_save_data(str(tmp_path / "r.dat").encode(), Y.copy(), costs.copy(), n, d, max_iter)
+    Yr, loss = ft.read_result_file(tmp_path / "r.dat", max_iter)
+    np.testing.assert_array_equal(Yr, Y.reshape(n, d))
+    keep = (np.arange(1, max_iter + 1) % 50) == 0
+    np.testing.assert_array_equal(loss[keep], costs[keep])
+    assert np.isnan(loss[~keep]).all()
