@@ -20,6 +20,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "fit-sne_amd"))
 
+from accuracy import gradient_errors  # noqa: E402
+
 METRIC = "grad iters/sec at N=1M 2D, FP32 vs plain FP64 GPU @ rel err≤1e-4; %HBM peak"
 HBM_PEAK_GBS = 8000.0
 # the FP32 form's attractive kernel in 2D (bucketed column windows); it dominates the step
@@ -77,27 +79,52 @@ def measured_traffic(kernel):
     return kib * 1024.0
 
 
-def cpu_baseline(P, Y32, calls, df=1.0):
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_threads():
+    """Threads the CPU baseline uses: the cores this process may run on."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_baseline(P, Y32, calls, df=1.0, warmup=1):
     """The oracle's gradient (FP64 C restatement of computeFftGradient*, PARALLEL_FOR threading kept)
-    on the same Y and P; returns (iters/s, threads, dC, sum_Q)."""
+    on the same Y and P: `warmup` untimed calls, then the median of `calls` timed ones; returns
+    (iters/s, threads, dC, sum_Q, median seconds)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import orc
 
     row, col, val = P
     Pu = (row.astype(np.uint32), col.astype(np.uint32), val)
-    nthreads = os.cpu_count() or 1
+    nthreads = host_threads()
     g = zq = None
-    t0 = time.perf_counter()
-    for i in range(calls):
+    for _ in range(warmup):
         g, zq = orc.gradient(Pu, Y32, nthreads=nthreads, df=df)
-        log(f"cpu baseline gradient {i + 1}/{calls} done ({time.perf_counter() - t0:.1f}s)")
-    dt = time.perf_counter() - t0
-    return calls / dt, nthreads, g, zq
+    times = []
+    for i in range(calls):
+        t0 = time.perf_counter()
+        g, zq = orc.gradient(Pu, Y32, nthreads=nthreads, df=df)
+        times.append(time.perf_counter() - t0)
+        log(f"cpu baseline gradient {i + 1}/{calls}: {times[-1]:.3f}s")
+    med = float(np.median(times))
+    return 1.0 / med, nthreads, g, zq, med
 
 
-def bench_c4(args, fg, synth):
-    """C4: 64 independent 2D embeddings, one batched graph launch per iteration, against the same 64
-    problems run one after another with the host-planned step (same kernels; results bitwise equal)."""
+def c4_measure(args, fg, synth, one_by_one=True):
+    """C4: 64 independent 2D embeddings stepped together (fitsne_batch_step: every stage of every problem in
+    one launch, the whole step one graph launch), against the same 64 problems run one after another with
+    the host-planned step (same kernels; results bitwise equal), and optionally each problem's own captured
+    step one after another."""
     probs = synth.c4_problems()
     sizes = [Y.shape[0] for (_, Y) in probs]
     prm = [(0.8, max(200.0, n / 12.0), 5.0) for n in sizes]
@@ -125,51 +152,65 @@ def bench_c4(args, fg, synth):
             c.step(m, lr, msn)
         c.sync()
     t_seq = time.perf_counter() - t0
-    log("C4 captured, one problem after another (device-planned steps, no concurrency)")
-    cap = make()
-    for c, (m, lr, msn) in zip(cap, prm):
-        c.set_graph(True)
-        for _ in range(args.warmup):
-            c.step(m, lr, msn)
-        c.sync()
-    t0 = time.perf_counter()
-    for c, (m, lr, msn) in zip(cap, prm):
-        for _ in range(args.steps):
-            c.step(m, lr, msn)
-        c.sync()
-    t_cap = time.perf_counter() - t0
-    for c in cap:
-        c.close()
-    log("C4 batched: every problem's captured step on its own stream, each iteration")
+    t_cap = None
+    if one_by_one:
+        log("C4 captured, one problem after another (device-planned steps, no concurrency)")
+        cap = make()
+        for c, (m, lr, msn) in zip(cap, prm):
+            c.set_graph(True)
+            for _ in range(args.warmup):
+                c.step(m, lr, msn)
+            c.sync()
+        t0 = time.perf_counter()
+        for c, (m, lr, msn) in zip(cap, prm):
+            for _ in range(args.steps):
+                c.step(m, lr, msn)
+            c.sync()
+        t_cap = time.perf_counter() - t0
+        for c in cap:
+            c.close()
+    log("C4 batched: one launch per stage for all 64 problems, one graph launch per iteration")
     bat = make()
     batch = fg.Batch(bat)
     for _ in range(args.warmup):
         batch.step(prm)
-    for c in bat:
-        c.sync()
+    bat[0].sync()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         batch.step(prm)
-    for c in bat:
-        c.sync()
+    bat[0].sync()
     t_bat = time.perf_counter() - t0
     same = all(np.array_equal(a.get_Y(), b.get_Y()) for a, b in zip(seq, bat))
+    batch.close()
+    for c in seq + bat:
+        c.close()
     n = len(sizes)
-    value = n * args.steps / t_bat
     out = {
-        "metric": "aggregate gradient iterations/s over 64 batched 2D embeddings (C4)",
-        "value": value, "unit": "problem-iters/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": 1e3 * t_bat / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32", "data": "synthetic (seeded Gaussian mixtures, generated in place)",
-        "config": {"workload": "C4: 64 problems, N log-uniform in [2000, 200000], 2D, df=1",
-                   "N_total": int(sum(sizes)), "N_min": int(min(sizes)), "N_max": int(max(sizes))},
+        "value": n * args.steps / t_bat, "unit": "problem-iters/s", "ms_per_step": 1e3 * t_bat / args.steps,
+        "workload": "C4: 64 problems, N log-uniform in [2000, 200000], 2D, df=1",
+        "N_total": int(sum(sizes)), "N_min": int(min(sizes)), "N_max": int(max(sizes)),
         "sequential": {"value": n * args.steps / t_seq, "unit": "problem-iters/s",
                        "ms_per_round": 1e3 * t_seq / args.steps},
-        "captured_one_by_one": {"value": n * args.steps / t_cap, "unit": "problem-iters/s",
-                                "ms_per_round": 1e3 * t_cap / args.steps},
         "speedup_vs_sequential": t_seq / t_bat,
         "bitwise_equal_to_sequential": bool(same),
     }
+    if t_cap is not None:
+        out["captured_one_by_one"] = {"value": n * args.steps / t_cap, "unit": "problem-iters/s",
+                                      "ms_per_round": 1e3 * t_cap / args.steps}
+    return out
+
+
+def bench_c4(args, fg, synth):
+    r = c4_measure(args, fg, synth)
+    out = {
+        "metric": "aggregate gradient iterations/s over 64 batched 2D embeddings (C4)",
+        "value": r["value"], "unit": "problem-iters/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (seeded Gaussian mixtures, generated in place)",
+        "config": {"workload": r["workload"], "N_total": r["N_total"], "N_min": r["N_min"], "N_max": r["N_max"]},
+    }
+    out.update({k: r[k] for k in ("sequential", "captured_one_by_one", "speedup_vs_sequential",
+                                  "bitwise_equal_to_sequential") if k in r})
     print(json.dumps(out), flush=True)
 
 
@@ -276,14 +317,16 @@ def bench_c3(args, fg, synth):
     if g64 is not None:
         acc["fp32_vs_fp64_gpu_grad_rel"] = rel(g32, g64)
         acc["fp32_vs_fp64_gpu_Z_rel"] = abs(z32 - z64) / abs(z64)
+        acc["fp32_vs_fp64_gpu_per_point"] = gradient_errors(g32, g64)
     cpu = None
     if not args.no_cpu:
         log("CPU baseline: one oracle gradient call at N={N} (1D, df = 0.5)".format(N=N))
-        v, nthr, _, _ = cpu_baseline(P, Y.astype(np.float32).astype(np.float64), 1, df=df)
-        cpu = {"value": v, "unit": "grad iters/s", "cores": nthr, "kind": "port",
-               "sample": f"one gradient of the same N={N} 1D workload (same Y, P, df = 0.5) by the oracle's C "
-                         f"restatement of computeFftGradientOneDVariableDf, {nthr} threads for the reference's "
-                         f"PARALLEL_FOR stages, serial spread/FFT as in the reference"}
+        v, nthr, _, _, med = cpu_baseline(P, Y.astype(np.float32).astype(np.float64), 3, df=df)
+        cpu = {"value": v, "unit": "grad iters/s", "cores": nthr, "kind": "port", "cpu_model": cpu_model(),
+               "sample": f"median of 3 gradients ({med:.2f} s each, after one untimed) of the same N={N} 1D "
+                         f"workload (same Y, P, df = 0.5) by the oracle's C restatement of "
+                         f"computeFftGradientOneDVariableDf, {nthr} threads for the reference's PARALLEL_FOR "
+                         f"stages, serial spread/FFT as in the reference"}
     out = {
         "metric": "1D gradient iterations/s at N=2M, df=0.5 (C3), FP32 form",
         "value": args.steps / dt, "unit": "grad iters/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -354,9 +397,10 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=1_000_000)
-    ap.add_argument("--cpu-calls", type=int, default=3)
+    ap.add_argument("--cpu-calls", type=int, default=9, help="timed oracle calls (after one untimed) for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-c4", action="store_true", help="leave the batched C4 figure out of the default line")
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--config", default="c2", choices=["c2", "c3", "c4", "exact", "bh"],
                     help="c2 (default): the headline step at N=1M 2D; c3: 1D df=0.5 at N=2M; c4: 64 batched "
@@ -443,19 +487,26 @@ def main():
     if g64 is not None:
         acc["fp32_vs_fp64_gpu_grad_rel"] = rel(g32, g64)
         acc["fp32_vs_fp64_gpu_Z_rel"] = abs(z32 - z64) / abs(z64)
+        acc["fp32_vs_fp64_gpu_per_point"] = gradient_errors(g32, g64)
     cpu = None
     if not args.no_cpu:
-        log(f"CPU baseline: {args.cpu_calls} oracle gradient calls at N={N}")
+        log(f"CPU baseline: 1 + {args.cpu_calls} oracle gradient calls at N={N}")
         Y32 = Y.astype(np.float32).astype(np.float64)
-        v, nthr, gref, zref = cpu_baseline(P, Y32, args.cpu_calls)
-        cpu = {"value": v, "unit": "grad iters/s", "cores": nthr, "kind": "port",
-               "sample": f"{args.cpu_calls} full gradient evaluations of the same N={N} workload (same Y, P) by the "
-                         f"oracle's C restatement of computeFftGradient, {nthr} threads for the reference's "
-                         f"PARALLEL_FOR stages, serial spread/FFT as in the reference, own FFT (no FFTW)"}
+        v, nthr, gref, zref, med = cpu_baseline(P, Y32, args.cpu_calls)
+        cpu = {"value": v, "unit": "grad iters/s", "cores": nthr, "kind": "port", "cpu_model": cpu_model(),
+               "sample": f"median of {args.cpu_calls} full gradient evaluations ({med:.3f} s each) after one "
+                         f"untimed call, on the same N={N} workload (same Y, P), by the oracle's C restatement "
+                         f"of computeFftGradient: {nthr} threads for the reference's PARALLEL_FOR stages, "
+                         f"serial spread/FFT as in the reference, own FFT (no FFTW on the box)"}
         acc["fp32_vs_oracle_grad_rel"] = rel(g32, gref)
         acc["fp32_vs_oracle_Z_rel"] = abs(z32 - zref) / abs(zref)
+        acc["fp32_vs_oracle_per_point"] = gradient_errors(g32, gref)
         if g64 is not None:
             acc["fp64_vs_oracle_grad_rel_at_fp32_rounded_Y"] = rel(g64, gref)
+
+    c4 = None
+    if not args.no_c4:
+        c4 = c4_measure(args, fg, synth, one_by_one=False)
 
     out = {
         "metric": METRIC,
@@ -484,6 +535,7 @@ def main():
         "speedup_vs_fp64": (res64["ms_per_step"] / ms) if res64 else None,
         "accuracy": acc,
         "kernels_ms": kernels_ms,
+        "c4_batched": c4,
     }
     print(json.dumps(out), flush=True)
 

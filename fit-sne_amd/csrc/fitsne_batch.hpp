@@ -1,0 +1,267 @@
+// fitsne_batch.hpp -- one step of many independent 2D embeddings (FP32 form) with one launch per stage.
+//
+// Each stage kernel covers every problem of the batch.  Stages whose work is fixed by the problem's N
+// (attractive term, spread, interpolation, centring) map workgroups to problems through a prefix of
+// per-problem workgroup counts computed once on the host.  The FFT passes, whose line counts follow each
+// problem's grid and so are known only after the step's geometry on the device, are persistent: a
+// fixed number of workgroups takes (problem, line) items from a per-stage counter, over a prefix of
+// line counts that the scheduling kernel writes after the geometry.  Every line, row block and point
+// group is computed by the same device code as a single context's step, so each embedding of the batch
+// is bitwise the one its own context would reach.
+#pragma once
+
+#include "fitsne_kernels.hpp"
+
+namespace fitsne {
+
+// One problem of a batch: its records and device buffers (all owned by its context).
+struct BatchProb {
+    DevGeo *dg;
+    const StepDev *sd;
+    float *Y, *pos, *dC, *uY, *gains;
+    int N;
+    int spread_pts;  // points per spread workgroup
+    int cblk;        // workgroups of the centring (and so min / max partials)
+    long first;      // flat index of the caller's first coordinate in the internal order
+    const int *inv;  // caller -> internal order (null when unpermuted), for the 2D min rule replay
+    const int *wstart, *wpure;
+    const uint32_t *wkey;
+    const float *wval, *wbmax;
+    int win_R, win_NW;
+    unsigned long long *fixgrid;
+    cx<float> *tw, *Trow, *spec, *ospec;
+    float *KQ, *K1, *KS, *KT;
+    float4 *gout;
+    double *zpart, *zbuf, *part, *mean, *mmpart;
+};
+
+// FFT passes fed from per-stage work counters.
+enum { kBsKernRows = 0, kBsKernCols, kBsRowsFwd, kBsCols, kBsRowsInv, kBsStages };
+
+struct BatchSched {
+    int ctr[kBsStages];    // next item of each pass (cleared by k_batch_sched every step)
+    int total[kBsStages];  // items of each pass this step
+};
+
+// The problem of workgroup / item x: the largest p with pref[p] <= x (pref non-decreasing, x < pref[n]).
+__device__ __forceinline__ int batch_find(const int *__restrict__ pref, int nprob, int x) {
+    int lo = 0, hi = nprob - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pref[mid] <= x) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Geometry (and twiddles when the FFT length changed) of every problem: workgroup p of 1024 threads.
+__global__ __launch_bounds__(1024) void kb_geometry(const BatchProb *__restrict__ bp, int D, int p, double ipi, int mni,
+                                                    int n1d_max, int mp_max) {
+    const BatchProb &b = bp[blockIdx.x];
+    geometry_block<float>(b.mmpart, b.cblk, b.Y, b.first, b.inv, (long)b.N * D, D, p, ipi, mni, n1d_max, mp_max,
+                          b.dg, b.tw);
+}
+
+// Line counts of the FFT passes per problem and their exclusive prefixes (pref[s * (nprob + 1) + p]),
+// the totals, and cleared counters.  One workgroup of 1024 threads; problems in chunks of 1024.
+__global__ __launch_bounds__(1024) void kb_sched(const BatchProb *__restrict__ bp, int nprob,
+                                                 BatchSched *__restrict__ sch, int *__restrict__ pref) {
+    __shared__ int scan[kBsStages][1024];
+    __shared__ int carry[kBsStages];
+    const int t = threadIdx.x;
+    if (t < kBsStages) carry[t] = 0;
+    for (int p0 = 0; p0 < nprob; p0 += 1024) {
+        const int p = p0 + t;
+        int cnt[kBsStages] = {0, 0, 0, 0, 0};
+        if (p < nprob) {
+            const DevGeo *dg = bp[p].dg;
+            if (dg->live) {
+                const int n1d = dg->g.n1d, H = dg->g.H;
+                if (dg->kspec_stale) {
+                    cnt[kBsKernRows] = n1d;
+                    cnt[kBsKernCols] = H;
+                }
+                cnt[kBsRowsFwd] = n1d;
+                cnt[kBsCols] = H;
+                cnt[kBsRowsInv] = n1d;
+            }
+        }
+        __syncthreads();  // the previous chunk's carry has been read
+#pragma unroll
+        for (int s = 0; s < kBsStages; s++) scan[s][t] = cnt[s];
+        __syncthreads();
+        for (int o = 1; o < 1024; o <<= 1) {  // inclusive Hillis-Steele scan
+            int v[kBsStages];
+#pragma unroll
+            for (int s = 0; s < kBsStages; s++) v[s] = t >= o ? scan[s][t - o] : 0;
+            __syncthreads();
+#pragma unroll
+            for (int s = 0; s < kBsStages; s++) scan[s][t] += v[s];
+            __syncthreads();
+        }
+        if (p < nprob) {
+#pragma unroll
+            for (int s = 0; s < kBsStages; s++) pref[s * (nprob + 1) + p] = carry[s] + scan[s][t] - cnt[s];
+        }
+        __syncthreads();
+        if (t < kBsStages) carry[t] += scan[t][1023];
+    }
+    __syncthreads();
+    if (t < kBsStages) {
+        pref[t * (nprob + 1) + nprob] = carry[t];
+        sch->total[t] = carry[t];
+        sch->ctr[t] = 0;
+    }
+}
+
+// Workgroup loop of a persistent FFT pass: takes items of stage S until they run out and calls
+// line(problem, line index, problem changed).  Every call of `line` ends with a barrier, so the item
+// slot and the staged twiddles are free again when the next item is taken.
+template <int S, typename F>
+__device__ __forceinline__ void batch_items(BatchSched *__restrict__ sch, const int *__restrict__ pref, int nprob,
+                                            F &&line) {
+    __shared__ int s_item;
+    const int *pf = pref + S * (nprob + 1);
+    const int total = sch->total[S];
+    int cur = -1;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(&sch->ctr[S], 1);
+        __syncthreads();
+        const int it = s_item;
+        if (it >= total) break;
+        const int p = batch_find(pf, nprob, it);
+        line(p, it - pf[p], p != cur);
+        cur = p;
+    }
+}
+
+template <int NTHR, int MAXTOT>
+__global__ __launch_bounds__(NTHR) void kb_kern_rows(const BatchProb *__restrict__ bp, int nprob,
+                                                     BatchSched *__restrict__ sch, const int *__restrict__ pref,
+                                                     int kind_q, int kind_1, double df) {
+    extern __shared__ unsigned char smem_raw[];
+    cx<float> *buf = reinterpret_cast<cx<float> *>(smem_raw);
+    const cx<float> *twl = nullptr;
+    batch_items<kBsKernRows>(sch, pref, nprob, [&](int p, int a, bool fresh) {
+        const BatchProb &b = bp[p];
+        const int Mp = b.dg->g.Mp;
+        if (fresh) twl = stage_twiddles(buf + lspan<true>(Mp), b.tw, Mp);
+        kern_rows_line<float, NTHR, MAXTOT, true>(b.dg, buf, twl, kind_q, kind_1, df, b.Trow, a);
+    });
+}
+
+template <int NTHR, int MAXTOT>
+__global__ __launch_bounds__(NTHR) void kb_kern_cols(const BatchProb *__restrict__ bp, int nprob,
+                                                     BatchSched *__restrict__ sch, const int *__restrict__ pref) {
+    extern __shared__ unsigned char smem_raw[];
+    cx<float> *buf = reinterpret_cast<cx<float> *>(smem_raw);
+    const cx<float> *twl = nullptr;
+    batch_items<kBsKernCols>(sch, pref, nprob, [&](int p, int k, bool fresh) {
+        const BatchProb &b = bp[p];
+        const int Mp = b.dg->g.Mp;
+        if (fresh) twl = stage_twiddles(buf + lspan<true>(Mp), b.tw, Mp);
+        kern_cols_line<float, NTHR, MAXTOT, true>(b.dg, buf, twl, b.Trow, 1, b.KQ, b.K1, b.KS, b.KT, k);
+    });
+}
+
+template <int NTHR, int MAXTOT>
+__global__ __launch_bounds__(NTHR) void kb_rows_fwd(const BatchProb *__restrict__ bp, int nprob,
+                                                    BatchSched *__restrict__ sch, const int *__restrict__ pref) {
+    extern __shared__ unsigned char smem_raw[];
+    cx<float> *buf = reinterpret_cast<cx<float> *>(smem_raw);
+    const cx<float> *twl = nullptr;
+    batch_items<kBsRowsFwd>(sch, pref, nprob, [&](int p, int u, bool fresh) {
+        const BatchProb &b = bp[p];
+        const int Mp = b.dg->g.Mp;
+        if (fresh) twl = stage_twiddles(buf + lspan<true>(2 * Mp), b.tw, Mp);
+        rows_fwd_line<float, NTHR, MAXTOT, true>(b.dg, buf, twl, (const float *)nullptr, 4, 4, b.spec, b.fixgrid, u);
+    });
+}
+
+template <int NTHR, int MAXTOT>
+__global__ __launch_bounds__(NTHR) void kb_cols(const BatchProb *__restrict__ bp, int nprob,
+                                                BatchSched *__restrict__ sch, const int *__restrict__ pref) {
+    extern __shared__ unsigned char smem_raw[];
+    cx<float> *buf = reinterpret_cast<cx<float> *>(smem_raw);
+    __shared__ double shz[NTHR / kWave];
+    const cx<float> *twl = nullptr;
+    batch_items<kBsCols>(sch, pref, nprob, [&](int p, int k, bool fresh) {
+        const BatchProb &b = bp[p];
+        const int Mp = b.dg->g.Mp;
+        if (fresh) twl = stage_twiddles(buf + lspan<true>(4 * Mp), b.tw, Mp);
+        cols_f32_line<NTHR, MAXTOT>(b.dg, buf, twl, b.spec, b.KQ, b.K1, b.KS, b.KT, b.ospec, b.zpart, shz, k);
+    });
+}
+
+template <int NTHR, int MAXTOT>
+__global__ __launch_bounds__(NTHR) void kb_rows_inv(const BatchProb *__restrict__ bp, int nprob,
+                                                    BatchSched *__restrict__ sch, const int *__restrict__ pref) {
+    extern __shared__ unsigned char smem_raw[];
+    cx<float> *buf = reinterpret_cast<cx<float> *>(smem_raw);
+    const cx<float> *twl = nullptr;
+    batch_items<kBsRowsInv>(sch, pref, nprob, [&](int p, int u, bool fresh) {
+        const BatchProb &b = bp[p];
+        const int Mp = b.dg->g.Mp;
+        if (fresh) twl = stage_twiddles(buf + lspan<true>(2 * Mp), b.tw, Mp);
+        rows_inv_line<float, NTHR, MAXTOT, true>(b.dg, buf, twl, b.ospec, 3, reinterpret_cast<float *>(b.gout), 4, u);
+    });
+}
+
+// Final Z of every problem: workgroup p of kThreads (the reduction of the inverse row pass's first
+// workgroup in a single context's step).
+__global__ __launch_bounds__(kThreads) void kb_zfinal(const BatchProb *__restrict__ bp, int df_one) {
+    const BatchProb &b = bp[blockIdx.x];
+    if (!b.dg->live) return;
+    __shared__ double sh[kThreads / kWave];
+    zfinal_block(b.dg, b.zpart, df_one, (double)b.N, b.zbuf, sh);
+}
+
+// Attractive term: workgroups of problem p are [pref[p], pref[p + 1]), one per row block.
+__global__ __launch_bounds__(kAttThreads) __attribute__((amdgpu_waves_per_eu(4))) void kb_attr(
+    const BatchProb *__restrict__ bp, int nprob, const int *__restrict__ pref, float inv_df, int df_one, float mdf) {
+    __shared__ AttrLds<2> L;
+    const int p = batch_find(pref, nprob, blockIdx.x);
+    const BatchProb &b = bp[p];
+    if (!b.dg->live) return;
+    attr_win_block<2>(b.wstart, b.wpure, b.wkey, b.wval, b.wbmax, b.Y, b.N, b.win_R, b.win_NW, inv_df, df_one, mdf,
+                      b.sd->exag, b.pos, blockIdx.x - pref[p], L);
+}
+
+template <int P>
+__global__ __launch_bounds__(kSpreadThreads) __attribute__((amdgpu_waves_per_eu(P <= 3 ? 4 : 1))) void kb_spread(
+    const BatchProb *__restrict__ bp, int nprob, const int *__restrict__ pref) {
+    __shared__ SpreadLds<P, 2> L;
+    const int p = batch_find(pref, nprob, blockIdx.x);
+    const BatchProb &b = bp[p];
+    if (!b.dg->live) return;
+    spread_block<P, 2>(b.Y, b.N, b.spread_pts, b.dg, b.fixgrid, blockIdx.x - pref[p], L);
+}
+
+template <int P>
+__global__ __launch_bounds__(kThreads) void kb_interp(const BatchProb *__restrict__ bp, int nprob,
+                                                      const int *__restrict__ pref) {
+    __shared__ double sh[kThreads / kWave];
+    const int p = batch_find(pref, nprob, blockIdx.x);
+    const BatchProb &b = bp[p];
+    if (!b.dg->live) return;
+    interp_block<P, 2>(b.Y, b.pos, b.gout, b.dg, b.zbuf, b.sd, b.dC, b.uY, b.gains, b.part, b.N, blockIdx.x - pref[p],
+                       sh);
+}
+
+// Mean of every problem's updated Y: workgroup p of kThreads.
+__global__ __launch_bounds__(kThreads) void kb_mean(const BatchProb *__restrict__ bp) {
+    const BatchProb &b = bp[blockIdx.x];
+    if (!b.dg->live) return;
+    __shared__ double sh[kThreads / kWave];
+    mean_block(b.part, (b.N + kThreads - 1) / kThreads, 2, (double)b.N, b.mean, sh);
+}
+
+__global__ __launch_bounds__(kThreads) void kb_center(const BatchProb *__restrict__ bp, int nprob,
+                                                      const int *__restrict__ pref) {
+    const int p = batch_find(pref, nprob, blockIdx.x);
+    const BatchProb &b = bp[p];
+    if (!b.dg->live) return;
+    center_block<float>(b.Y, (long)b.N * 2, 2, b.first, b.mean, b.mmpart, blockIdx.x - pref[p], b.cblk);
+}
+
+}  // namespace fitsne

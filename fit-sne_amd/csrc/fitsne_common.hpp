@@ -54,6 +54,7 @@ struct Geo {
     int nbox;        // nb^D
     int D;
     int p;
+    double fix;      // fixed-point scale of the FP32 spread's integer grid (spread_fix_scale)
 };
 
 // Stage radices of one FFT length (Stockham).
@@ -160,6 +161,42 @@ __host__ __device__ inline Geo make_geo(int D, int p, double mn, double mx, int 
     g.H = g.Mp / 2 + 1;
     g.nbox = D == 2 ? nb * nb : nb;
     return g;
+}
+
+// Largest |L_j(t)| over the nodes j, at t (in box widths).
+__host__ __device__ inline double lagrange_absmax(const Geo &g, double t) {
+    double m = 0;
+    for (int j = 0; j < g.p; j++) {
+        double v = 1;
+        for (int k = 0; k < g.p; k++)
+            if (k != j) v *= t - g.sp[k];
+        m = fmax(m, fabs(v / g.den[j]));
+    }
+    return m;
+}
+
+// Fixed-point scale of the FP32 spread's int64 grid: each run of points' sums is rounded to an integer
+// at this scale and the integers add exactly, so no node's total may reach 2^62.  A bound on that total:
+// every point of the step in one node at the largest in-box Lagrange weight (sampled on [0, 1] at 257
+// points, with a 10% margin that covers the variation of these low-degree polynomials between samples),
+// times D for the |e|^2 moment, plus the one point the 2D min rule can leave outside the grid (the first
+// x coordinate, `gap` box widths below box 0, with its extrapolated weights).  2^38 while that bound
+// stays below 2^24; above, the largest power of two that keeps the total under 2^62.  0 (refuse the
+// step) when that is below 2^20: the outlier's weights would swamp every other point's resolution.
+__host__ __device__ inline double spread_fix_scale(const Geo &g, double n_points, double gap) {
+    double lb = 0;
+    for (int s = 0; s <= 256; s++) lb = fmax(lb, lagrange_absmax(g, s / 256.0));
+    lb *= 1.1;
+    double bound = n_points * (g.D == 2 ? lb * lb * 2 : lb);
+    if (g.D == 2 && gap > 0) bound += 1.1 * lagrange_absmax(g, -gap) * lb * ((1 + gap) * (1 + gap) + 1);
+    if (!(bound < 16777216.0)) {          // 2^24
+        if (!(bound < 4.0e18)) return 0;  // also a NaN bound
+        int e = 0;
+        (void)frexp(bound, &e);  // bound < 2^e
+        if (62 - e < 20) return 0;
+        return ldexp(1.0, 62 - e);
+    }
+    return 274877906944.0;  // 2^38
 }
 
 // Workgroup-to-row map over npad = 8 ceil(n / 8) slots: consecutive workgroups go to different XCDs

@@ -13,6 +13,7 @@
 
 #include "../../include/fitsne_abi.h"
 #include "fitsne_kernels.hpp"
+#include "fitsne_batch.hpp"
 #include "fitsne_exact.hpp"
 #include "fitsne_sym.hpp"
 #include "fitsne_bh.hpp"
@@ -102,6 +103,9 @@ struct fitsne_ctx {
     int D = 2, p = 3, mni = 50;
     double ipi = 1.0, df = 1.0;
     hipStream_t stream = nullptr;
+    hipStream_t own_stream = nullptr;  // the context's stream; `stream` is a batch's while it is a member
+    struct fitsne_batch *batch = nullptr;  // the fused batch this context belongs to
+    long layout_gen = 0;  // bumped whenever device buffers or launch shapes may have changed
     // second stream of the host-planned FP32 step: the attractive term runs on it beside the grid stages
     hipStream_t astream = nullptr;
     int spread_pts = kSpreadPts;  // points per spread workgroup (fitsne_autotune); results do not depend on it
@@ -160,6 +164,7 @@ struct fitsne_ctx {
     hipEvent_t mm_event = nullptr;
     bool mm_pending = false, mm_valid = false;
     double mm_min = 0, mm_max = 0;
+    double mm_first = 0;  // the caller's first coordinate (xs[0]) at the last min / max
     // kernel-spectrum cache key
     bool kcache = false;
     double kc_mn = 0, kc_mx = 0;
@@ -238,6 +243,7 @@ static int sync_ctx(fitsne_ctx *c) {
 
 static void drop_graph(fitsne_ctx *c);
 static int check_graph(fitsne_ctx *c);
+static int prepare_captured(fitsne_ctx *c);
 
 template <typename T>
 static int twiddles(fitsne_ctx *c, int n, const cx<T> **out) {
@@ -315,11 +321,13 @@ static int current_minmax(fitsne_ctx *c, double *mn, double *mx) {
         c->mm_max = c->h_mm[0];
         c->mm_min = c->h_mm[1];
         double y0 = c->h_mm[2];
+        c->mm_first = y0;
         TRY(resolve_quirk<TV>(c, y0, &c->mm_min, &c->mm_max));
     }
     if (!c->mm_valid) {
         TRY(minmax_now<TV>(c, &c->mm_min, &c->mm_max));
         double y0 = c->h_mm[2];
+        c->mm_first = y0;
         TRY(resolve_quirk<TV>(c, y0, &c->mm_min, &c->mm_max));
         c->mm_valid = true;
     }
@@ -339,6 +347,16 @@ static int geometry(fitsne_ctx *c, double mn, double mx, Geo *g) {
     if (nb < 1) return fail(FITSNE_E_ARG, "n_boxes < 1");
     *g = make_geo(c->D, c->p, mn, mx, nb);
     const bool f32 = c->precision == FITSNE_FP32;
+    if (f32) {
+        // the first x coordinate lies below the grid when the 2D min rule skipped it (extrapolated)
+        const double gap = c->D == 2 && c->mm_first < mn ? (mn - c->mm_first) / g->bw : 0.0;
+        g->fix = spread_fix_scale(*g, (double)c->N, gap);
+        if (g->fix == 0)
+            return fail(FITSNE_E_UNSUPPORTED,
+                        "the first point lies " + std::to_string(gap) + " box widths below the grid (the 2D min "
+                        "rule leaves it out); its extrapolated spread weights exceed the FP32 form's "
+                        "fixed-point range: use the FP64 plain path for this embedding");
+    }
     if (c->D == 2) {
         const int lim = kMaxLen2d;
         if (g->Mp > lim)
@@ -1216,12 +1234,14 @@ extern "C" int fitsne_ctx_create(fitsne_ctx **out, int device, int precision, in
         delete c;
         return fail(FITSNE_E_HIP, "stream/event/pinned allocation failed");
     }
+    c->own_stream = c->stream;
     *out = c;
     return FITSNE_OK;
 }
 
 extern "C" int fitsne_ctx_destroy(fitsne_ctx *c) {
     if (!c) return FITSNE_OK;
+    if (c->batch) return fail(FITSNE_E_ARG, "destroy the batch that holds this context first");
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->astream) (void)hipStreamSynchronize(c->astream);
@@ -1251,7 +1271,7 @@ extern "C" int fitsne_ctx_destroy(fitsne_ctx *c) {
     if (c->astream) (void)hipStreamDestroy(c->astream);
     if (c->h_mm) (void)hipHostFree(c->h_mm);
     if (c->bh_pin) (void)hipHostFree(c->bh_pin);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
     delete c;
     return FITSNE_OK;
 }
@@ -1580,6 +1600,7 @@ extern "C" int fitsne_set_Y(fitsne_ctx *c, const double *Y) {
     }
     c->mm_valid = c->mm_pending = false;
     c->have_grad = false;
+    c->kcache = false;
     drop_graph(c);
     c->mm_parts_valid = false;
     if (c->precision == FITSNE_FP32 && c->reorder_on_set_Y && c->N >= kReorderMinN) TRY(reorder_points(c));
@@ -1627,6 +1648,7 @@ extern "C" int fitsne_gradient(fitsne_ctx *c, double *dC, double *sum_Q) {
 // ============================================================================ captured steps
 
 static void drop_graph(fitsne_ctx *c) {
+    c->layout_gen++;
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
     c->gexec = nullptr;
@@ -1637,19 +1659,12 @@ static void drop_graph(fitsne_ctx *c) {
 // buffers are allocated first at the build's largest grid; nothing is allocated during capture.
 static int capture_step(fitsne_ctx *c) {
     drop_graph(c);
+    c->kcache = false;
     if (c->precision != FITSNE_FP32) return fail(FITSNE_E_UNSUPPORTED, "captured steps are for the FP32 form");
     if (c->N <= 0) return fail(FITSNE_E_ARG, "P and Y must be set before a step");
     HIPTRY(hipSetDevice(c->device));
-    if (!c->win_ok) TRY(build_windows(c));
-    if (!c->win_ok) return fail(FITSNE_E_UNSUPPORTED, "captured steps need the windowed attractive layout");
     const int D = c->D;
-    TRY(ensure(c->dgeo, sizeof(DevGeo)));
-    TRY(ensure(c->dstep, sizeof(StepDev)));
-    TRY(ensure(c->dtw, sizeof(cx<float>) * (size_t)(D == 2 ? kCapMp2 : kCapMp1)));
-    TRY(D == 2 ? ensure_f32_buffers<2>(c, kCapN1d2, kCapMp2) : ensure_f32_buffers<1>(c, kCapN1d1, kCapMp1));
-    DevGeo v{};  // no spectra or twiddles cached yet
-    v.live = 1;
-    launch(c, "set_geo", k_set_geo, dim3(1), dim3(kWave), 0, v, c->dgeo.as<DevGeo>());
+    TRY(prepare_captured(c));
     TRY(sync_ctx(c));
     const bool timing = c->timing;
     c->timing = false;
@@ -1688,11 +1703,13 @@ static int ensure_partials(fitsne_ctx *c) {
 }
 
 // After captured steps: the host's min / max must be recomputed, and a refused step is an error.
+// The captured step rewrites the kernel-spectrum buffers, so the host-planned step's cache of them is void.
 static void after_graph_steps(fitsne_ctx *c) {
     c->graph_ran = true;
     c->have_grad = true;
     c->mm_valid = false;
     c->mm_pending = false;
+    c->kcache = false;
 }
 
 static int check_graph(fitsne_ctx *c) {
@@ -1704,6 +1721,12 @@ static int check_graph(fitsne_ctx *c) {
     c->last = v.g;
     c->last_min = v.mn;
     c->last_max = v.mx;
+    if (v.err == 2) {
+        launch(c, "clear_err", k_clear_err, dim3(1), dim3(kWave), 0, c->dgeo.as<DevGeo>());
+        return fail(FITSNE_E_UNSUPPORTED, "a captured step's first point lay so far below the grid (the 2D min rule "
+                                          "leaves it out) that the spread had no usable fixed-point scale; the "
+                                          "embedding was left at the previous step");
+    }
     if (v.err) {
         launch(c, "clear_err", k_clear_err, dim3(1), dim3(kWave), 0, c->dgeo.as<DevGeo>());
         return fail(FITSNE_E_UNSUPPORTED, "a captured step's grid exceeded this build's limits (FFT length " +
@@ -1741,6 +1764,10 @@ extern "C" int fitsne_set_spread_points(fitsne_ctx *c, int points) {
 extern "C" int fitsne_set_graph(fitsne_ctx *c, int enable) {
     if (!c) return fail(FITSNE_E_ARG, "null context");
     if (enable && c->precision != FITSNE_FP32) return fail(FITSNE_E_UNSUPPORTED, "captured steps are for the FP32 form");
+    // a captured step may still run: wait for it (and report a refused one) before its graph goes
+    HIPTRY(hipSetDevice(c->device));
+    TRY(sync_ctx(c));
+    TRY(check_graph(c));
     drop_graph(c);
     c->use_graph = enable != 0;
     return FITSNE_OK;
@@ -1769,12 +1796,234 @@ extern "C" int fitsne_step(fitsne_ctx *c, const fitsne_step_params *prm, double 
 
 // ============================================================================ batched embeddings
 
-// One step of every embedding of the batch: each context's captured step is launched on that
-// context's own stream, so the steps of different embeddings overlap on the device (one graph launch
-// per embedding and iteration, no host round trip).
+// A batch steps many FP32 contexts together.  When they are all 2D with the same interpolation settings
+// and df (the usual seed or perplexity sweep), the batch is fused: one captured graph whose every stage is
+// one launch over all problems (fitsne_batch.hpp), on the batch's stream, which the members use as their
+// own while they belong to it.  Otherwise each member's captured step is launched on its own stream.
+// The step parameters of each iteration go to the device through a ring of pinned slots: a slot is
+// refilled only once the copy issued from it kStepSlots iterations earlier has completed.
+constexpr int kStepSlots = 4;
+
 struct fitsne_batch {
     std::vector<fitsne_ctx *> ctx;
+    bool fused = false;
+    int device = 0, p = 3, mni = 50;
+    double df = 1.0, ipi = 1.0;
+    hipStream_t stream = nullptr, astream = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+    hipEvent_t slot_ev[kStepSlots] = {};
+    bool slot_used[kStepSlots] = {};
+    StepDev *h_steps = nullptr;  // pinned, kStepSlots x n
+    int slot = 0;
+    DBuf dprob, dsteps, dsched, dpref, dspre;
+    int tot_attr = 0, tot_spread = 0, tot_interp = 0, tot_center = 0;
+    int g_kern = 0, g_rows = 0, g_cols = 0;
+    std::vector<long> gens;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
 };
+
+static void batch_drop_graph(fitsne_batch *b) {
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->gexec = nullptr;
+    b->graph = nullptr;
+}
+
+// Buffers of a device-planned step at the build's largest 2D grid, and a fresh geometry record (no
+// kernel spectra or twiddles cached): what a captured step of this context, alone or in a batch, needs.
+static int prepare_captured(fitsne_ctx *c) {
+    if (!c->win_ok) TRY(build_windows(c));
+    if (!c->win_ok) return fail(FITSNE_E_UNSUPPORTED, "captured steps need the windowed attractive layout");
+    const int D = c->D;
+    TRY(ensure(c->dgeo, sizeof(DevGeo)));
+    TRY(ensure(c->dstep, sizeof(StepDev)));
+    TRY(ensure(c->dtw, sizeof(cx<float>) * (size_t)(D == 2 ? kCapMp2 : kCapMp1)));
+    TRY(D == 2 ? ensure_f32_buffers<2>(c, kCapN1d2, kCapMp2) : ensure_f32_buffers<1>(c, kCapN1d1, kCapMp1));
+    DevGeo v{};
+    v.live = 1;
+    launch(c, "set_geo", k_set_geo, dim3(1), dim3(kWave), 0, v, c->dgeo.as<DevGeo>());
+    return FITSNE_OK;
+}
+
+template <typename K>
+static int persistent_blocks(K kernel, int threads, size_t lds, int cus, int *out) {
+    int per = 0;
+    HIPTRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void *>(kernel), threads, lds));
+    *out = std::max(1, per) * cus;
+    return FITSNE_OK;
+}
+
+template <int P>
+static int batch_capture_p(fitsne_batch *b, const BatchProb *bp, BatchSched *sch, int *pref, const int *spre) {
+    const int n = (int)b->ctx.size();
+    const bool df1 = b->df == 1.0;
+    const int kind_q = df1 ? FITSNE_KERNEL_SQUARED_CAUCHY : FITSNE_KERNEL_SQUARED_GENERAL;
+    const int kind_1 = FITSNE_KERNEL_GENERAL;
+    const size_t Mp = (size_t)kCapMp2;
+    const size_t lds_k = sizeof(cx<float>) * (lspan<true>((int)Mp) + Mp);
+    const size_t lds_r = sizeof(cx<float>) * (lspan<true>(2 * (int)Mp) + Mp);
+    const size_t lds_c = sizeof(cx<float>) * (lspan<true>(4 * (int)Mp) + Mp);
+    const int *apre = spre, *sppre = spre + (n + 1), *ipre = spre + 2 * (n + 1), *cpre = spre + 3 * (n + 1);
+    hipStream_t st = b->stream;
+    kb_geometry<<<dim3(n), dim3(1024), 0, st>>>(bp, 2, P, b->ipi, b->mni, kCapN1d2, kCapMp2);
+    HIPTRY(hipEventRecord(b->fork_ev, st));
+    HIPTRY(hipStreamWaitEvent(b->astream, b->fork_ev, 0));
+    kb_attr<<<dim3(b->tot_attr), dim3(kAttThreads), 0, b->astream>>>(bp, n, apre, (float)(1.0 / b->df), (int)df1,
+                                                                      (float)(0.5 * std::sqrt(b->df)));
+    HIPTRY(hipEventRecord(b->join_ev, b->astream));
+    kb_sched<<<dim3(1), dim3(1024), 0, st>>>(bp, n, sch, pref);
+    kb_kern_rows<kKernThreads, kKernTot><<<dim3(b->g_kern), dim3(kKernThreads), lds_k, st>>>(bp, n, sch, pref, kind_q,
+                                                                                              kind_1, b->df);
+    kb_kern_cols<kKernThreads, kKernTot><<<dim3(b->g_kern), dim3(kKernThreads), lds_k, st>>>(bp, n, sch, pref);
+    kb_spread<P><<<dim3(b->tot_spread), dim3(kSpreadThreads), 0, st>>>(bp, n, sppre);
+    kb_rows_fwd<kRowThreads, kRowTot><<<dim3(b->g_rows), dim3(kRowThreads), lds_r, st>>>(bp, n, sch, pref);
+    kb_cols<kColThreads, kColTot32><<<dim3(b->g_cols), dim3(kColThreads), lds_c, st>>>(bp, n, sch, pref);
+    kb_rows_inv<kRowThreads, kRowTot><<<dim3(b->g_rows), dim3(kRowThreads), lds_r, st>>>(bp, n, sch, pref);
+    kb_zfinal<<<dim3(n), dim3(kThreads), 0, st>>>(bp, (int)df1);
+    HIPTRY(hipStreamWaitEvent(st, b->join_ev, 0));
+    kb_interp<P><<<dim3(b->tot_interp), dim3(kThreads), 0, st>>>(bp, n, ipre);
+    kb_mean<<<dim3(n), dim3(kThreads), 0, st>>>(bp);
+    kb_center<<<dim3(b->tot_center), dim3(kThreads), 0, st>>>(bp, n, cpre);
+    return FITSNE_OK;
+}
+
+// (Re)build the fused batch: every member's buffers and records, the problem table, the workgroup
+// prefixes of the fixed-size stages, the persistent passes' sizes, and the captured graph.
+static int batch_prepare(fitsne_batch *b) {
+    batch_drop_graph(b);
+    const int n = (int)b->ctx.size();
+    HIPTRY(hipSetDevice(b->device));
+    HIPTRY(hipStreamSynchronize(b->stream));
+    for (fitsne_ctx *c : b->ctx) {
+        if (c->N <= 0 || !c->Y.p) return fail(FITSNE_E_ARG, "P and Y must be set before a batched step");
+        TRY(prepare_captured(c));
+    }
+    TRY(ensure(b->dsteps, sizeof(StepDev) * n));
+    std::vector<BatchProb> bp(n);
+    std::vector<int> spre(4 * (size_t)(n + 1), 0);
+    int *apre = spre.data(), *sppre = apre + (n + 1), *ipre = apre + 2 * (n + 1), *cpre = apre + 3 * (n + 1);
+    for (int i = 0; i < n; i++) {
+        fitsne_ctx *c = b->ctx[i];
+        BatchProb &q = bp[i];
+        q.dg = c->dgeo.as<DevGeo>();
+        q.sd = b->dsteps.as<const StepDev>() + i;
+        q.Y = c->Y.as<float>();
+        q.pos = c->pos.as<float>();
+        q.dC = c->dC.as<float>();
+        q.uY = c->uY.as<float>();
+        q.gains = c->gains.as<float>();
+        q.N = c->N;
+        q.spread_pts = c->spread_pts;
+        q.cblk = std::min(blocks_for((long)c->N * 2), 1024);
+        q.first = first_coord(c);
+        q.inv = c->permuted ? c->inv_d.as<const int>() : nullptr;
+        q.wstart = c->wstart.as<const int>();
+        q.wpure = c->wpure.as<const int>();
+        q.wkey = c->wkey.as<const uint32_t>();
+        q.wval = c->wval.as<const float>();
+        q.wbmax = c->wbmax.as<const float>();
+        q.win_R = c->win_R;
+        q.win_NW = c->win_NW;
+        q.fixgrid = c->fixgrid.as<unsigned long long>();
+        q.tw = c->dtw.as<cx<float>>();
+        q.Trow = c->Trow.as<cx<float>>();
+        q.spec = c->spec.as<cx<float>>();
+        q.ospec = c->ospec.as<cx<float>>();
+        q.KQ = c->KQ.as<float>();
+        q.K1 = c->K1.as<float>();
+        q.KS = c->KS.as<float>();
+        q.KT = c->KT.as<float>();
+        q.gout = c->gout.as<float4>();
+        q.zpart = c->zpart.as<double>();
+        q.zbuf = c->zbuf.as<double>();
+        q.part = c->part.as<double>();
+        q.mean = c->mean.as<double>();
+        q.mmpart = c->mmpart.as<double>();
+        const long a = (c->N + c->win_R - 1) / c->win_R, sp = (c->N + c->spread_pts - 1) / c->spread_pts;
+        const long ip = blocks_for(c->N), ce = q.cblk;
+        if ((long)apre[i] + a > INT32_MAX / 2 || (long)sppre[i] + sp > INT32_MAX / 2 ||
+            (long)ipre[i] + ip > INT32_MAX / 2)
+            return fail(FITSNE_E_UNSUPPORTED, "batch too large for one launch per stage");
+        apre[i + 1] = apre[i] + (int)a;
+        sppre[i + 1] = sppre[i] + (int)sp;
+        ipre[i + 1] = ipre[i] + (int)ip;
+        cpre[i + 1] = cpre[i] + (int)ce;
+    }
+    b->tot_attr = apre[n];
+    b->tot_spread = sppre[n];
+    b->tot_interp = ipre[n];
+    b->tot_center = cpre[n];
+    TRY(ensure(b->dprob, sizeof(BatchProb) * n));
+    TRY(ensure(b->dspre, sizeof(int) * spre.size()));
+    TRY(ensure(b->dsched, sizeof(BatchSched)));
+    TRY(ensure(b->dpref, sizeof(int) * kBsStages * (size_t)(n + 1)));
+    HIPTRY(hipMemcpyAsync(b->dprob.p, bp.data(), sizeof(BatchProb) * n, hipMemcpyHostToDevice, b->stream));
+    HIPTRY(hipMemcpyAsync(b->dspre.p, spre.data(), sizeof(int) * spre.size(), hipMemcpyHostToDevice, b->stream));
+    HIPTRY(hipStreamSynchronize(b->stream));  // the host vectors go out of scope
+    int cus = 0;
+    HIPTRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device));
+    const size_t Mp = (size_t)kCapMp2;
+    TRY(persistent_blocks(kb_kern_rows<kKernThreads, kKernTot>, kKernThreads,
+                          sizeof(cx<float>) * (lspan<true>((int)Mp) + Mp), cus, &b->g_kern));
+    TRY(persistent_blocks(kb_rows_fwd<kRowThreads, kRowTot>, kRowThreads,
+                          sizeof(cx<float>) * (lspan<true>(2 * (int)Mp) + Mp), cus, &b->g_rows));
+    TRY(persistent_blocks(kb_cols<kColThreads, kColTot32>, kColThreads,
+                          sizeof(cx<float>) * (lspan<true>(4 * (int)Mp) + Mp), cus, &b->g_cols));
+    const BatchProb *dbp = b->dprob.as<const BatchProb>();
+    auto *sch = b->dsched.as<BatchSched>();
+    int *pref = b->dpref.as<int>();
+    const int *dspre = b->dspre.as<const int>();
+    HIPTRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int rc;
+    switch (b->p) {
+    case 2: rc = batch_capture_p<2>(b, dbp, sch, pref, dspre); break;
+    case 3: rc = batch_capture_p<3>(b, dbp, sch, pref, dspre); break;
+    case 4: rc = batch_capture_p<4>(b, dbp, sch, pref, dspre); break;
+    default: rc = batch_capture_p<5>(b, dbp, sch, pref, dspre); break;
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (rc != FITSNE_OK || e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        if (rc != FITSNE_OK) return rc;
+        HIPTRY(e);
+    }
+    hipGraphExec_t x = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        HIPTRY(ie);
+    }
+    b->graph = g;
+    b->gexec = x;
+    b->gens.resize(n);
+    for (int i = 0; i < n; i++) b->gens[i] = b->ctx[i]->layout_gen;
+    return FITSNE_OK;
+}
+
+static void batch_release(fitsne_batch *b) {
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->astream) (void)hipStreamSynchronize(b->astream);
+    for (fitsne_ctx *c : b->ctx)
+        if (c->batch == b) {
+            c->stream = c->own_stream;
+            c->batch = nullptr;
+        }
+    batch_drop_graph(b);
+    release(b->dprob);
+    release(b->dsteps);
+    release(b->dsched);
+    release(b->dpref);
+    release(b->dspre);
+    for (auto &e : b->slot_ev)
+        if (e) (void)hipEventDestroy(e);
+    if (b->fork_ev) (void)hipEventDestroy(b->fork_ev);
+    if (b->join_ev) (void)hipEventDestroy(b->join_ev);
+    if (b->h_steps) (void)hipHostFree(b->h_steps);
+    if (b->astream) (void)hipStreamDestroy(b->astream);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+}
 
 extern "C" int fitsne_batch_create(fitsne_batch **out, fitsne_ctx **ctxs, int n) {
     if (!out || !ctxs || n <= 0) return fail(FITSNE_E_ARG, "invalid batch");
@@ -1782,6 +2031,7 @@ extern "C" int fitsne_batch_create(fitsne_batch **out, fitsne_ctx **ctxs, int n)
     for (int i = 0; i < n; i++) {
         if (!ctxs[i] || ctxs[i]->precision != FITSNE_FP32 || ctxs[i]->device != ctxs[0]->device)
             return fail(FITSNE_E_ARG, "a batch holds FP32 contexts on one device");
+        if (ctxs[i]->batch) return fail(FITSNE_E_ARG, "a context belongs to another batch");
         for (int j = 0; j < i; j++)
             if (ctxs[j] == ctxs[i]) return fail(FITSNE_E_ARG, "a context appears twice in the batch");
     }
@@ -1789,11 +2039,59 @@ extern "C" int fitsne_batch_create(fitsne_batch **out, fitsne_ctx **ctxs, int n)
     for (int i = 0; i < n; i++) TRY(fitsne_autotune(ctxs[i], nullptr));
     auto *b = new fitsne_batch();
     b->ctx.assign(ctxs, ctxs + n);
+    const fitsne_ctx *c0 = ctxs[0];
+    b->device = c0->device;
+    b->p = c0->p;
+    b->df = c0->df;
+    b->ipi = c0->ipi;
+    b->mni = c0->mni;
+    bool fused = true;
+    for (int i = 0; i < n; i++) {
+        const fitsne_ctx *c = ctxs[i];
+        fused = fused && c->D == 2 && c->p == b->p && c->df == b->df && c->ipi == b->ipi && c->mni == b->mni &&
+                c->N > 0 && c->nnz > 0;
+    }
+    if (!fused) {
+        *out = b;
+        return FITSNE_OK;
+    }
+    b->fused = true;
+    int rc = FITSNE_OK;
+    if (hipSetDevice(b->device) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&b->astream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->join_ev, hipEventDisableTiming) != hipSuccess ||
+        hipHostMalloc((void **)&b->h_steps, sizeof(StepDev) * kStepSlots * (size_t)n, hipHostMallocDefault) != hipSuccess)
+        rc = fail(FITSNE_E_HIP, "batch stream/event/pinned allocation failed");
+    for (int s = 0; s < kStepSlots && rc == FITSNE_OK; s++)
+        if (hipEventCreateWithFlags(&b->slot_ev[s], hipEventDisableTiming) != hipSuccess)
+            rc = fail(FITSNE_E_HIP, "batch event creation failed");
+    if (rc == FITSNE_OK) {
+        // the members use the batch's stream from now on, so their own calls stay ordered with its steps
+        for (fitsne_ctx *c : b->ctx) {
+            if (sync_ctx(c) != FITSNE_OK) rc = FITSNE_E_HIP;
+            c->stream = b->stream;
+            c->batch = b;
+        }
+    }
+    if (rc == FITSNE_OK) rc = batch_prepare(b);
+    if (rc != FITSNE_OK) {
+        const std::string msg = g_err;
+        batch_release(b);
+        delete b;
+        g_err = msg;
+        return rc;
+    }
     *out = b;
     return FITSNE_OK;
 }
 
 extern "C" int fitsne_batch_destroy(fitsne_batch *b) {
+    if (!b) return FITSNE_OK;
+    if (b->fused) {
+        (void)hipSetDevice(b->device);
+        batch_release(b);
+    }
     delete b;
     return FITSNE_OK;
 }
@@ -1801,16 +2099,43 @@ extern "C" int fitsne_batch_destroy(fitsne_batch *b) {
 extern "C" int fitsne_batch_step(fitsne_batch *b, const fitsne_step_params *prm) {
     if (!b || !prm) return fail(FITSNE_E_ARG, "null argument");
     const int n = (int)b->ctx.size();
-    for (int i = 0; i < n; i++) {
-        fitsne_ctx *c = b->ctx[i];
-        HIPTRY(hipSetDevice(c->device));
-        c->use_graph = true;
-        if (!c->gexec) TRY(capture_step(c));
-        TRY(ensure_partials(c));
-        TRY(put_step(c, 1, &prm[i]));
-        HIPTRY(hipGraphLaunch(c->gexec, c->stream));
-        after_graph_steps(c);
+    if (!b->fused) {
+        for (int i = 0; i < n; i++) {
+            fitsne_ctx *c = b->ctx[i];
+            HIPTRY(hipSetDevice(c->device));
+            c->use_graph = true;
+            if (!c->gexec) TRY(capture_step(c));
+            TRY(ensure_partials(c));
+            TRY(put_step(c, 1, &prm[i]));
+            HIPTRY(hipGraphLaunch(c->gexec, c->stream));
+            after_graph_steps(c);
+        }
+        return FITSNE_OK;
     }
+    HIPTRY(hipSetDevice(b->device));
+    bool stale = !b->gexec;
+    for (int i = 0; i < n && !stale; i++) stale = b->ctx[i]->layout_gen != b->gens[i];
+    if (stale) TRY(batch_prepare(b));
+    for (fitsne_ctx *c : b->ctx) TRY(ensure_partials(c));
+    const int s = b->slot;
+    if (b->slot_used[s]) HIPTRY(hipEventSynchronize(b->slot_ev[s]));
+    StepDev *h = b->h_steps + (size_t)s * n;
+    for (int i = 0; i < n; i++) {
+        StepDev v{};
+        v.mode = 1;
+        v.exag = (float)b->ctx[i]->exag;
+        v.momentum = (float)prm[i].momentum;
+        v.lr = (float)prm[i].learning_rate;
+        v.max_step_norm = (float)prm[i].max_step_norm;
+        v.plain_descent = prm[i].plain_descent;
+        h[i] = v;
+    }
+    HIPTRY(hipMemcpyAsync(b->dsteps.p, h, sizeof(StepDev) * n, hipMemcpyHostToDevice, b->stream));
+    HIPTRY(hipEventRecord(b->slot_ev[s], b->stream));
+    b->slot_used[s] = true;
+    b->slot = (s + 1) % kStepSlots;
+    HIPTRY(hipGraphLaunch(b->gexec, b->stream));
+    for (fitsne_ctx *c : b->ctx) after_graph_steps(c);
     return FITSNE_OK;
 }
 

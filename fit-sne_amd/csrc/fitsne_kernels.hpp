@@ -367,25 +367,24 @@ __device__ __forceinline__ void attr_quad(const u32x4 &kq, const f32x4 &vq4, con
 // In 1D a workgroup holds 80 KB of LDS, so two share a CU if each wave keeps to 64 VGPRs (8 waves per
 // SIMD); without the request the compiler used 70 and the blocks ran in two rounds.  In 2D the LDS
 // (160 KB) allows one workgroup per CU, 4 waves per SIMD.
+// LDS of one k_attr_win workgroup.
 template <int D>
-__global__ __launch_bounds__(kAttThreads) __attribute__((amdgpu_waves_per_eu(D == 1 ? 8 : 4))) void k_attr_win(const int *__restrict__ start,
-                                                          const int *__restrict__ pure,
-                                                          const uint32_t *__restrict__ key,
-                                                          const float *__restrict__ wval,
-                                                          const float *__restrict__ bmax,
-                                                          const float *__restrict__ Y, int N, int R, int NW,
-                                                          float inv_df, int df_one, float mdf,
-                                                          const StepDev *__restrict__ sd, float exag,
-                                                          const DevGeo *__restrict__ dg, float *__restrict__ pos) {
+struct AttrLds {
+    float yw[kWin * D];                   // the staged window's Y
+    float yr[kAttRows * D];               // the row block's Y
+    unsigned long long acc[kAttRows * D];  // the row block's fixed-point sums
+};
+
+// Row block b of the windowed attractive term (the body of k_attr_win, and of its batched form).
+template <int D>
+__device__ __forceinline__ void attr_win_block(const int *__restrict__ start, const int *__restrict__ pure,
+                                               const uint32_t *__restrict__ key, const float *__restrict__ wval,
+                                               const float *__restrict__ bmax, const float *__restrict__ Y, int N,
+                                               int R, int NW, float inv_df, int df_one, float mdf, float scale,
+                                               float *__restrict__ pos, int b, AttrLds<D> &L) {
     constexpr int WL = kWin * D / kAttThreads;  // window floats per thread
-    if (!dg->live) return;
-    const float scale = exag >= 0.f ? exag : sd->exag;  // exag < 0: the step record's (captured steps)
-    __shared__ float yw[kWin * D];
-    __shared__ float yr[kAttRows * D];
-    __shared__ unsigned long long acc[kAttRows * D];
-    const int nblk = (N + R - 1) / R;
-    const int b = xcd_row(blockIdx.x, nblk);
-    if (b >= nblk) return;
+    float *yw = L.yw, *yr = L.yr;
+    unsigned long long *acc = L.acc;
     const int t = threadIdx.x;
     const int r0 = b * R, nr = min(R, N - r0);
     for (int q = t; q < nr * D; q += kAttThreads) {
@@ -505,6 +504,25 @@ __global__ __launch_bounds__(kAttThreads) __attribute__((amdgpu_waves_per_eu(D =
         pos[(long)r0 * D + q] = (float)((double)(long long)acc[q] * invS) * scale;
 }
 
+template <int D>
+__global__ __launch_bounds__(kAttThreads) __attribute__((amdgpu_waves_per_eu(D == 1 ? 8 : 4))) void k_attr_win(const int *__restrict__ start,
+                                                          const int *__restrict__ pure,
+                                                          const uint32_t *__restrict__ key,
+                                                          const float *__restrict__ wval,
+                                                          const float *__restrict__ bmax,
+                                                          const float *__restrict__ Y, int N, int R, int NW,
+                                                          float inv_df, int df_one, float mdf,
+                                                          const StepDev *__restrict__ sd, float exag,
+                                                          const DevGeo *__restrict__ dg, float *__restrict__ pos) {
+    if (!dg->live) return;
+    const float scale = exag >= 0.f ? exag : sd->exag;  // exag < 0: the step record's (captured steps)
+    __shared__ AttrLds<D> L;
+    const int nblk = (N + R - 1) / R;
+    const int b = xcd_row(blockIdx.x, nblk);
+    if (b >= nblk) return;
+    attr_win_block<D>(start, pure, key, wval, bmax, Y, N, R, NW, inv_df, df_one, mdf, scale, pos, b, L);
+}
+
 // ---------------------------------------------------------------- min / max
 // Over the flat sequence Y[0..n): max of all; min of all but index 0 (2D) or of all (1D); Y[0].
 // The 2D reference rule (`else if`, tsne.cpp:1045-1048) equals this whenever Y[0] >= min(rest);
@@ -621,15 +639,18 @@ __global__ __launch_bounds__(1024) void k_minmax_seq(const TV *__restrict__ Y, c
 // The geometry of a captured step, from the min / max partials the previous step (or the upload of Y)
 // left: max and min(rest) from the partials; when Y[first] is the strict minimum the sequential
 // 2D rule is replayed (tsne.cpp:1045-1048); then the box count, geometry and plan of make_geo /
-// make_plan, the refusal of grids beyond n1d_max / mp_max (live = 0, err = 1), and whether the
-// kernel spectra and twiddles of the previous step still apply.  One block of 1024 threads.
+// make_plan, the refusal of grids beyond n1d_max / mp_max (live = 0, err = 1) or of a step whose
+// spread has no usable fixed-point scale (err = 2, spread_fix_scale), and whether the kernel spectra
+// and twiddles of the previous step still apply.  One block of 1024 threads.
+// The body of k_geometry (and of its batched form); one workgroup of 1024.  When tw is given, the same
+// workgroup then writes the twiddle table of the step's FFT length if it changed (k_twiddles_dev's values).
 template <typename TV>
-__global__ __launch_bounds__(1024) void k_geometry(const double *__restrict__ part, int nblk, const TV *__restrict__ Y,
-                                                   long first, const int *__restrict__ pos, long n, int D, int p,
-                                                   double ipi, int mni, int n1d_max, int mp_max,
-                                                   DevGeo *__restrict__ dg) {
+__device__ void geometry_block(const double *__restrict__ part, int nblk, const TV *__restrict__ Y, long first,
+                               const int *__restrict__ pos, long n, int D, int p, double ipi, int mni, int n1d_max,
+                               int mp_max, DevGeo *__restrict__ dg, cx<float> *__restrict__ tw) {
     __shared__ double shx[1024 / kWave], shn[1024 / kWave];
     __shared__ double res[2];
+    __shared__ int twn;
     double mx = -INFINITY, mn = INFINITY;
     for (int b = threadIdx.x; b < nblk; b += 1024) {
         mx = fmax(mx, part[2 * b]);
@@ -669,6 +690,16 @@ __global__ __launch_bounds__(1024) void k_geometry(const double *__restrict__ pa
         Geo g{};
         if (nb >= 1) g = make_geo(D, p, mn, mx, nb);
         o.live = fin && nb >= 1 && g.n1d <= n1d_max && g.Mp <= mp_max;
+        if (o.live) {  // the spread's fixed-point scale, with the first coordinate's gap below the grid
+            const double y0 = (double)Y[first];
+            g.fix = spread_fix_scale(g, (double)(n / D), D == 2 && y0 < mn ? (mn - y0) / g.bw : 0.0);
+            if (g.fix == 0) {
+                o.live = 0;
+                o.err = 2;
+            }
+        } else {
+            o.err = 1;
+        }
         if (o.live) {
             o.g = g;
             o.pl = make_plan(g.Mp);
@@ -678,11 +709,27 @@ __global__ __launch_bounds__(1024) void k_geometry(const double *__restrict__ pa
             o.kc_mx = mx;
             o.tw_stale = o.tw_n != g.Mp;
             o.tw_n = g.Mp;
-        } else {
-            o.err = 1;
         }
         *dg = o;
+        twn = o.live && o.tw_stale ? g.Mp : 0;
     }
+    if (tw) {
+        __syncthreads();
+        const int tn = twn;
+        for (int e = threadIdx.x; e < tn; e += blockDim.x) {
+            double sn, cs;
+            sincospi(-2.0 * (double)e / (double)tn, &sn, &cs);
+            tw[e] = cx<float>{(float)cs, (float)sn};
+        }
+    }
+}
+
+template <typename TV>
+__global__ __launch_bounds__(1024) void k_geometry(const double *__restrict__ part, int nblk, const TV *__restrict__ Y,
+                                                   long first, const int *__restrict__ pos, long n, int D, int p,
+                                                   double ipi, int mni, int n1d_max, int mp_max,
+                                                   DevGeo *__restrict__ dg) {
+    geometry_block<TV>(part, nblk, Y, first, pos, n, D, p, ipi, mni, n1d_max, mp_max, dg, nullptr);
 }
 
 // Twiddle table of the step's FFT length, when it changed.
@@ -764,7 +811,7 @@ __global__ __launch_bounds__(1024) void k_scan_counts(int *__restrict__ count, i
 // ordered points keep this to a few dozen boxes).  Each lane finds its point's box and computes the node-relative charges
 //   A = L,  c = L |e|^2,  b = L e    (e = y - node, in box widths)
 // for the p^D nodes of its box.  Runs of consecutive lanes in one box are summed in FP32 by a segmented
-// scan in the VALU (DPP; a fixed order over fixed points), each run's sums are scaled by 2^38, rounded
+// scan in the VALU (DPP; a fixed order over fixed points), each run's sums are scaled by g.fix (2^38 unless the step's bound needs less), rounded
 // to int64 and added into the workgroup's LDS table slot of its box (open addressing, LDS atomics), and
 // the table goes to the grid with one 64-bit global atomic per (box, node, term).  A run whose box finds
 // no free slot adds to the grid directly.  The sums across runs are exact integers, so the grid, and so
@@ -772,7 +819,6 @@ __global__ __launch_bounds__(1024) void k_scan_counts(int *__restrict__ count, i
 // whatever `pts`: the waves are the same aligned groups of 64 points for every choice.
 // Grid layout: int64 [u][v][4] (A, c, b_x, b_y), u the x node (nbodyfft.cpp:137-138); the row FFT
 // reads it, converts it and clears it for the next iteration.
-constexpr double kFixScale = 274877906944.0;  // 2^38
 constexpr int kSpreadThreads = 256;
 constexpr int kSpreadPts = 1024;  // default points per workgroup (the autotuner may choose another)
 
@@ -806,23 +852,33 @@ constexpr int spread_slots() {
 // under 40 KB, four workgroups share a CU (three at the 150 registers the compiler chose on its own:
 // measured 0.355 against 0.362 ms per step at N = 1M, results unchanged).  For p = 4, 5 the limit would
 // spill most of the kernel's values, so it is not requested.
+// LDS table of one k_spread_f32 workgroup.
 template <int P, int D>
-__global__ __launch_bounds__(kSpreadThreads) __attribute__((amdgpu_waves_per_eu(D == 2 && P <= 3 ? 4 : 1))) void k_spread_f32(const float *__restrict__ Y, int N, int pts,
-                                                               const DevGeo *__restrict__ dg,
-                                                               unsigned long long *__restrict__ grid) {
-    constexpr int NV = (D == 2 ? P * P : P);
-    constexpr int NQ = (D == 2 ? 4 : 3);
-    constexpr int NS = spread_slots<P, D>();
-    __shared__ int tkey[NS];
-    __shared__ unsigned long long tval[NS][NV * NQ];
-    if (!dg->live) return;
+struct SpreadLds {
+    static constexpr int NV = (D == 2 ? P * P : P);
+    static constexpr int NQ = (D == 2 ? 4 : 3);
+    static constexpr int NS = spread_slots<P, D>();
+    int tkey[NS];
+    unsigned long long tval[NS][NV * NQ];
+};
+
+// Workgroup `bid` of the spread (points [bid pts, (bid + 1) pts)); the body of k_spread_f32 and of its
+// batched form.
+template <int P, int D>
+__device__ __forceinline__ void spread_block(const float *__restrict__ Y, int N, int pts, const DevGeo *__restrict__ dg,
+                                             unsigned long long *__restrict__ grid, int bid, SpreadLds<P, D> &L) {
+    constexpr int NV = SpreadLds<P, D>::NV;
+    constexpr int NQ = SpreadLds<P, D>::NQ;
+    constexpr int NS = SpreadLds<P, D>::NS;
+    int *tkey = L.tkey;
+    auto &tval = L.tval;
     const Geo &g = dg->g;
     const int t = threadIdx.x, lane = t % kWave;
     for (int o = t; o < NS; o += kSpreadThreads) tkey[o] = -1;
     for (int o = t; o < NS * NV * NQ; o += kSpreadThreads) (&tval[0][0])[o] = 0ull;
     __syncthreads();
     for (int chunk = 0; chunk < pts / kSpreadThreads; chunk++) {
-        const long i = (long)blockIdx.x * pts + (long)chunk * kSpreadThreads + t;
+        const long i = (long)bid * pts + (long)chunk * kSpreadThreads + t;
         const bool ok = i < N;
         int b = -1 - lane;  // lanes past N: distinct negative keys, never a real run
         float val[NV][NQ];
@@ -893,7 +949,7 @@ __global__ __launch_bounds__(kSpreadThreads) __attribute__((amdgpu_waves_per_eu(
         seg_scan_step<0x118, 0xf, NV, NQ>(val, run);  // row_shr:8
         seg_scan_step<0x142, 0xa, NV, NQ>(val, run);  // row_bcast:15 into rows 1 and 3
         seg_scan_step<0x143, 0xc, NV, NQ>(val, run);  // row_bcast:31 into rows 2 and 3
-        // each run's last lane adds its totals, as 2^38 fixed point, into its box's table slot (open
+        // each run's last lane adds its totals, as g.fix fixed point, into its box's table slot (open
         // addressing), or into the grid when the table has no free slot
         if (tail && b >= 0) {
             int slot = -1;
@@ -911,7 +967,7 @@ __global__ __launch_bounds__(kSpreadThreads) __attribute__((amdgpu_waves_per_eu(
             for (int v = 0; v < NV; v++)
 #pragma unroll
                 for (int q = 0; q < NQ; q++) {
-                    const long long sum = fix_rn_any((double)val[v][q] * kFixScale);
+                    const long long sum = fix_rn_any((double)val[v][q] * g.fix);
                     if (sum == 0) continue;
                     if (slot >= 0) {
                         atomicAdd(&tval[slot][v * NQ + q], (unsigned long long)sum);
@@ -938,13 +994,22 @@ __global__ __launch_bounds__(kSpreadThreads) __attribute__((amdgpu_waves_per_eu(
     }
 }
 
+template <int P, int D>
+__global__ __launch_bounds__(kSpreadThreads) __attribute__((amdgpu_waves_per_eu(D == 2 && P <= 3 ? 4 : 1))) void k_spread_f32(const float *__restrict__ Y, int N, int pts,
+                                                               const DevGeo *__restrict__ dg,
+                                                               unsigned long long *__restrict__ grid) {
+    __shared__ SpreadLds<P, D> L;
+    if (!dg->live) return;
+    spread_block<P, D>(Y, N, pts, dg, grid, blockIdx.x, L);
+}
+
 // int64 grid -> float4 (A, c, b_x, b_y) and clear, for the 1D path (the 2D row FFT does this itself).
 __global__ __launch_bounds__(kThreads) void k_grid_to_f32(const DevGeo *__restrict__ dg,
                                                           unsigned long long *__restrict__ grid,
                                                           float4 *__restrict__ out) {
     if (!dg->live) return;
     const Geo &g = dg->g;
-    const double inv = 1.0 / kFixScale;
+    const double inv = 1.0 / g.fix;
     for (long n = (long)blockIdx.x * kThreads + threadIdx.x; n < g.n1d; n += (long)gridDim.x * kThreads) {
         long long s[4];
 #pragma unroll
@@ -967,6 +1032,27 @@ __global__ __launch_bounds__(kThreads) void k_grid_to_f32(const DevGeo *__restri
 // K_1(a, |b|) over b in (-n1d, n1d) wrapped into Mp, packed as one complex line; its DFT is
 // T_q + i T_1 (both real).  Trow[k][a] = (T_q(a, k), T_1(a, k)), k in [0, H).  Runs only when the
 // step marks the spectra stale.
+template <typename T, int NTHR, int MAXTOT, bool PAD>
+__device__ __forceinline__ void kern_rows_line(const DevGeo *__restrict__ dg, cx<T> *buf, const cx<T> *twl,
+                                               int kind_q, int kind_1, double df, cx<T> *__restrict__ Trow, int a) {
+    const Geo &g = dg->g;
+    const int Mp = g.Mp, n1d = g.n1d;
+    const double du = a * g.h;
+    for (int b = threadIdx.x; b < Mp; b += NTHR) buf[lp<PAD>(b)] = cx<T>{(T)0, (T)0};
+    __syncthreads();
+    for (int b = threadIdx.x; b < n1d; b += NTHR) {
+        const double dv = b * g.h;
+        const double r2 = du * du + dv * dv;
+        cx<T> val{kernel_value<T>(kind_q, r2, df), kernel_value<T>(kind_1, r2, df)};
+        buf[lp<PAD>(b)] = val;
+        if (b > 0) buf[lp<PAD>(Mp - b)] = val;
+    }
+    __syncthreads();
+    fft_lines<T, NTHR, MAXTOT, PAD>(buf, dg->pl, 1, Mp, twl, false);
+    for (int k = threadIdx.x; k < g.H; k += NTHR) Trow[(long)k * n1d + a] = buf[lp<PAD>(k)];
+    __syncthreads();
+}
+
 template <typename T, int NTHR, int MAXTOT, bool PAD = false>
 __global__ __launch_bounds__(NTHR) void k_kern_rows(const DevGeo *__restrict__ dg, const cx<T> *__restrict__ tw,
                                                     int kind_q, int kind_1, double df, cx<T> *__restrict__ Trow) {
@@ -980,20 +1066,7 @@ __global__ __launch_bounds__(NTHR) void k_kern_rows(const DevGeo *__restrict__ d
     for (int slot = blockIdx.x; slot < npad; slot += gridDim.x) {
         const int a = xcd_slot_row(slot, n1d);
         if (a >= n1d) continue;
-        const double du = a * g.h;
-        for (int b = threadIdx.x; b < Mp; b += NTHR) buf[lp<PAD>(b)] = cx<T>{(T)0, (T)0};
-        __syncthreads();
-        for (int b = threadIdx.x; b < n1d; b += NTHR) {
-            const double dv = b * g.h;
-            const double r2 = du * du + dv * dv;
-            cx<T> val{kernel_value<T>(kind_q, r2, df), kernel_value<T>(kind_1, r2, df)};
-            buf[lp<PAD>(b)] = val;
-            if (b > 0) buf[lp<PAD>(Mp - b)] = val;
-        }
-        __syncthreads();
-        fft_lines<T, NTHR, MAXTOT, PAD>(buf, dg->pl, 1, Mp, twl, false);
-        for (int k = threadIdx.x; k < g.H; k += NTHR) Trow[(long)k * n1d + a] = buf[lp<PAD>(k)];
-        __syncthreads();
+        kern_rows_line<T, NTHR, MAXTOT, PAD>(dg, buf, twl, kind_q, kind_1, df, Trow, a);
     }
 }
 
@@ -1024,6 +1097,37 @@ __device__ __forceinline__ void lds_fill(V *buf, int total, F &&f) {
 // kernel K_q(a,b) a h is -i S(m,k) and that of K_q(a,b) b h is -i S(k,m) (K_q symmetric in a, b).
 // Stored for m, k in [0, H):  KQ[k][m] = C_q(m,k), K1[k][m] = C_1(m,k), KS[k][m] = S(m,k),
 // KT[k][m] = S(k,m) (written by column m's workgroup).
+template <typename T, int NTHR, int MAXTOT, bool PAD>
+__device__ __forceinline__ void kern_cols_line(const DevGeo *__restrict__ dg, cx<T> *buf, const cx<T> *twl,
+                                               const cx<T> *__restrict__ Trow, int need_s, T *__restrict__ KQ,
+                                               T *__restrict__ K1, T *__restrict__ KS, T *__restrict__ KT, int k) {
+    const Geo &g = dg->g;
+    const int Mp = g.Mp, n1d = g.n1d, H = g.H;
+    lds_fill<NTHR, PAD>(buf, Mp, [&](int a) {
+        int as;
+        if (a < n1d) as = a;
+        else if (a > Mp - n1d) as = a - Mp;
+        else return cx<T>{(T)0, (T)0};
+        const cx<T> t = Trow[(long)k * n1d + (as < 0 ? -as : as)];
+        const T w = need_s ? (T)1 + (T)(as * g.h) : (T)1;
+        return cx<T>{t.x * w, t.y};
+    });
+    __syncthreads();
+    fft_lines<T, NTHR, MAXTOT, PAD>(buf, dg->pl, 1, Mp, twl, false);
+    for (int m = threadIdx.x; m < H; m += NTHR) {
+        const cx<T> z = buf[lp<PAD>(m)];
+        const T im_neg = buf[lp<PAD>(m == 0 ? 0 : Mp - m)].y;
+        KQ[(long)k * H + m] = z.x;
+        K1[(long)k * H + m] = (z.y + im_neg) * (T)0.5;
+        if (need_s) {
+            const T sv = (im_neg - z.y) * (T)0.5;
+            KS[(long)k * H + m] = sv;
+            KT[(long)m * H + k] = sv;
+        }
+    }
+    __syncthreads();
+}
+
 template <typename T, int NTHR, int MAXTOT, bool PAD = false>
 __global__ __launch_bounds__(NTHR) void k_kern_cols(const DevGeo *__restrict__ dg, const cx<T> *__restrict__ tw,
                                                     const cx<T> *__restrict__ Trow, int need_s,
@@ -1033,35 +1137,13 @@ __global__ __launch_bounds__(NTHR) void k_kern_cols(const DevGeo *__restrict__ d
     const Geo &g = dg->g;
     extern __shared__ unsigned char smem_raw[];
     cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
-    const int Mp = g.Mp, n1d = g.n1d, H = g.H;
+    const int Mp = g.Mp, H = g.H;
     const cx<T> *twl = stage_twiddles(buf + lspan<PAD>(Mp), tw, Mp);
     const int npad = 8 * ((H + 7) / 8);
     for (int slot = blockIdx.x; slot < npad; slot += gridDim.x) {
         const int k = xcd_slot_row(slot, H);
         if (k >= H) continue;
-        lds_fill<NTHR, PAD>(buf, Mp, [&](int a) {
-            int as;
-            if (a < n1d) as = a;
-            else if (a > Mp - n1d) as = a - Mp;
-            else return cx<T>{(T)0, (T)0};
-            const cx<T> t = Trow[(long)k * n1d + (as < 0 ? -as : as)];
-            const T w = need_s ? (T)1 + (T)(as * g.h) : (T)1;
-            return cx<T>{t.x * w, t.y};
-        });
-        __syncthreads();
-        fft_lines<T, NTHR, MAXTOT, PAD>(buf, dg->pl, 1, Mp, twl, false);
-        for (int m = threadIdx.x; m < H; m += NTHR) {
-            const cx<T> z = buf[lp<PAD>(m)];
-            const T im_neg = buf[lp<PAD>(m == 0 ? 0 : Mp - m)].y;
-            KQ[(long)k * H + m] = z.x;
-            K1[(long)k * H + m] = (z.y + im_neg) * (T)0.5;
-            if (need_s) {
-                const T sv = (im_neg - z.y) * (T)0.5;
-                KS[(long)k * H + m] = sv;
-                KT[(long)m * H + k] = sv;
-            }
-        }
-        __syncthreads();
+        kern_cols_line<T, NTHR, MAXTOT, PAD>(dg, buf, twl, Trow, need_s, KQ, K1, KS, KT, k);
     }
 }
 
@@ -1070,6 +1152,47 @@ __global__ __launch_bounds__(NTHR) void k_kern_cols(const DevGeo *__restrict__ d
 // spectra unpacked from the packed line: A = (Z(k) + conj Z(-k)) / 2, B = -i (Z(k) - conj Z(-k)) / 2).
 // With fixgrid != null the input is the exact int64 grid of k_spread_f32 ([u][v][4] = A, c, b_x, b_y in
 // 2^-38 units, offsets in box widths); it is converted and cleared here (each row read once).
+template <typename T, int NTHR, int MAXTOT, bool PAD>
+__device__ __forceinline__ void rows_fwd_line(const DevGeo *__restrict__ dg, cx<T> *buf, const cx<T> *twl,
+                                              const T *__restrict__ grid, int sin, int nterms,
+                                              cx<T> *__restrict__ spec, unsigned long long *__restrict__ fixgrid,
+                                              int u) {
+    const Geo &g = dg->g;
+    const int Mp = g.Mp, n1d = g.n1d, H = g.H;
+    const int lines = (nterms + 1) / 2;
+    const double inv = 1.0 / g.fix;
+    const double unit[4] = {inv, inv * g.bw0 * g.bw0, inv * g.bw0, inv * g.bw0};
+    const long plane = (long)H * n1d;
+    lds_fill<NTHR, PAD>(buf, lines * Mp, [&](int q) {
+        const int l = q / Mp, v = q - l * Mp;
+        cx<T> z{(T)0, (T)0};
+        if (v < n1d) {
+            if (fixgrid) {
+                unsigned long long *src = fixgrid + ((long)u * n1d + v) * 4 + 2 * l;
+                z.x = (T)((double)(long long)src[0] * unit[2 * l]);
+                z.y = (T)((double)(long long)src[1] * unit[2 * l + 1]);
+                src[0] = 0ull;
+                src[1] = 0ull;
+            } else {
+                const T *src = grid + ((long)u * n1d + v) * sin + 2 * l;
+                z.x = src[0];
+                if (2 * l + 1 < nterms) z.y = src[1];
+            }
+        }
+        return z;
+    });
+    __syncthreads();
+    fft_lines<T, NTHR, MAXTOT, PAD>(buf, dg->pl, lines, Mp, twl, false);
+    for (int q = threadIdx.x; q < lines * H; q += NTHR) {
+        const int l = q / H, k = q - l * H;
+        const cx<T> Z = buf[lp<PAD>(l * Mp + k)];
+        const cx<T> Zc = conj(buf[lp<PAD>(l * Mp + (k == 0 ? 0 : Mp - k))]);
+        spec[(2 * l) * plane + (long)k * n1d + u] = (T)0.5 * (Z + Zc);
+        if (2 * l + 1 < nterms) spec[(2 * l + 1) * plane + (long)k * n1d + u] = (T)0.5 * mul_negi(Z - Zc);
+    }
+    __syncthreads();
+}
+
 template <typename T, int NTHR, int MAXTOT, bool PAD = false>
 __global__ __launch_bounds__(NTHR) void k_rows_fwd(const DevGeo *__restrict__ dg, const cx<T> *__restrict__ tw,
                                                    const T *__restrict__ grid, int sin, int nterms,
@@ -1079,44 +1202,14 @@ __global__ __launch_bounds__(NTHR) void k_rows_fwd(const DevGeo *__restrict__ dg
     const Geo &g = dg->g;
     extern __shared__ unsigned char smem_raw[];
     cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
-    const int Mp = g.Mp, n1d = g.n1d, H = g.H;
+    const int Mp = g.Mp, n1d = g.n1d;
     const int lines = (nterms + 1) / 2;
     const cx<T> *twl = stage_twiddles(buf + lspan<PAD>(lines * Mp), tw, Mp);
-    const double inv = 1.0 / kFixScale;
-    const double unit[4] = {inv, inv * g.bw0 * g.bw0, inv * g.bw0, inv * g.bw0};
-    const long plane = (long)H * n1d;
     const int npad = 8 * ((n1d + 7) / 8);
     for (int slot = blockIdx.x; slot < npad; slot += gridDim.x) {
         const int u = xcd_slot_row(slot, n1d);
         if (u >= n1d) continue;
-        lds_fill<NTHR, PAD>(buf, lines * Mp, [&](int q) {
-            const int l = q / Mp, v = q - l * Mp;
-            cx<T> z{(T)0, (T)0};
-            if (v < n1d) {
-                if (fixgrid) {
-                    unsigned long long *src = fixgrid + ((long)u * n1d + v) * 4 + 2 * l;
-                    z.x = (T)((double)(long long)src[0] * unit[2 * l]);
-                    z.y = (T)((double)(long long)src[1] * unit[2 * l + 1]);
-                    src[0] = 0ull;
-                    src[1] = 0ull;
-                } else {
-                    const T *src = grid + ((long)u * n1d + v) * sin + 2 * l;
-                    z.x = src[0];
-                    if (2 * l + 1 < nterms) z.y = src[1];
-                }
-            }
-            return z;
-        });
-        __syncthreads();
-        fft_lines<T, NTHR, MAXTOT, PAD>(buf, dg->pl, lines, Mp, twl, false);
-        for (int q = threadIdx.x; q < lines * H; q += NTHR) {
-            const int l = q / H, k = q - l * H;
-            const cx<T> Z = buf[lp<PAD>(l * Mp + k)];
-            const cx<T> Zc = conj(buf[lp<PAD>(l * Mp + (k == 0 ? 0 : Mp - k))]);
-            spec[(2 * l) * plane + (long)k * n1d + u] = (T)0.5 * (Z + Zc);
-            if (2 * l + 1 < nterms) spec[(2 * l + 1) * plane + (long)k * n1d + u] = (T)0.5 * mul_negi(Z - Zc);
-        }
-        __syncthreads();
+        rows_fwd_line<T, NTHR, MAXTOT, PAD>(dg, buf, twl, grid, sin, nterms, spec, fixgrid, u);
     }
 }
 
@@ -1125,6 +1218,61 @@ __global__ __launch_bounds__(NTHR) void k_rows_fwd(const DevGeo *__restrict__ dg
 //   V1 = K_q A,  G_x = K_x A - K_q b_x,  G_y = K_y A - K_q b_y    (inverse-transformed over u)
 // and the Parseval sums of Z (k weighted 1 at k = 0, Mp/2 and 2 elsewhere):
 //   z0 = sum K_1 |A|^2, z1 = sum Re(conj(c) K_q A), z2 = sum Re(conj(b).K_D A), z3 = sum K_q |b|^2.
+template <int NTHR, int MAXTOT>
+__device__ __forceinline__ void cols_f32_line(const DevGeo *__restrict__ dg, cx<float> *buf, const cx<float> *twl,
+                                              const cx<float> *__restrict__ spec, const float *__restrict__ KQ,
+                                              const float *__restrict__ K1, const float *__restrict__ KS,
+                                              const float *__restrict__ KT, cx<float> *__restrict__ ospec,
+                                              double *__restrict__ zpart, double *shz, int k) {
+    const Geo &g = dg->g;
+    const int Mp = g.Mp, n1d = g.n1d, H = g.H;
+    const long plane = (long)H * n1d;
+    lds_fill<NTHR, true>(buf, 4 * Mp, [&](int q) {
+        const int t = q / Mp, u = q - t * Mp;
+        return u < n1d ? spec[t * plane + (long)k * n1d + u] : cx<float>{0.f, 0.f};
+    });
+    __syncthreads();
+    fft_lines<float, NTHR, MAXTOT, true>(buf, dg->pl, 4, Mp, twl, false);
+    const double wk = (k == 0 || 2 * k == Mp) ? 1.0 : 2.0;
+    double z0 = 0, z1 = 0, z2 = 0, z3 = 0;
+    for (int m = threadIdx.x; m < Mp; m += NTHR) {
+        const int mb = m <= Mp / 2 ? m : Mp - m;
+        const float sg = m <= Mp / 2 ? 1.f : -1.f;
+        const float cq = KQ[(long)k * H + mb], c1 = K1[(long)k * H + mb];
+        const float sx = sg * KS[(long)k * H + mb], sy = KT[(long)k * H + mb];
+        const cx<float> A = buf[lp<true>(m)], C = buf[lp<true>(Mp + m)], BX = buf[lp<true>(2 * Mp + m)],
+                        BY = buf[lp<true>(3 * Mp + m)];
+        const cx<float> KxA = mul_negi(sx * A), KyA = mul_negi(sy * A);
+        buf[lp<true>(m)] = cq * A;
+        buf[lp<true>(Mp + m)] = KxA - cq * BX;
+        buf[lp<true>(2 * Mp + m)] = KyA - cq * BY;
+        const double a2 = (double)A.x * A.x + (double)A.y * A.y;
+        z0 += (double)c1 * a2;
+        z1 += (double)cq * ((double)C.x * A.x + (double)C.y * A.y);
+        // Re(conj(B) (-i s) A) = s Im(conj(B) A)
+        z2 += (double)sx * ((double)BX.x * A.y - (double)BX.y * A.x) +
+              (double)sy * ((double)BY.x * A.y - (double)BY.y * A.x);
+        z3 += (double)cq * ((double)BX.x * BX.x + (double)BX.y * BX.y + (double)BY.x * BY.x + (double)BY.y * BY.y);
+    }
+    __syncthreads();
+    fft_lines<float, NTHR, MAXTOT, true>(buf, dg->pl, 3, Mp, twl, true);
+    for (int q = threadIdx.x; q < 3 * n1d; q += NTHR) {
+        const int t = q / n1d, u = q - t * n1d;
+        ospec[t * plane + (long)k * n1d + u] = buf[lp<true>(t * Mp + u)];
+    }
+    z0 = block_sum(z0 * wk, shz);
+    z1 = block_sum(z1 * wk, shz);
+    z2 = block_sum(z2 * wk, shz);
+    z3 = block_sum(z3 * wk, shz);
+    if (threadIdx.x == 0) {
+        zpart[k * 4 + 0] = z0;
+        zpart[k * 4 + 1] = z1;
+        zpart[k * 4 + 2] = z2;
+        zpart[k * 4 + 3] = z3;
+    }
+    __syncthreads();
+}
+
 template <int NTHR, int MAXTOT>
 __global__ __launch_bounds__(NTHR) void k_cols_f32(const DevGeo *__restrict__ dg, const cx<float> *__restrict__ tw,
                                                    const cx<float> *__restrict__ spec, const float *__restrict__ KQ,
@@ -1136,57 +1284,13 @@ __global__ __launch_bounds__(NTHR) void k_cols_f32(const DevGeo *__restrict__ dg
     extern __shared__ unsigned char smem_raw[];
     cx<float> *buf = reinterpret_cast<cx<float> *>(smem_raw);
     __shared__ double shz[NTHR / kWave];
-    const int Mp = g.Mp, n1d = g.n1d, H = g.H;
-    const long plane = (long)H * n1d;
+    const int Mp = g.Mp, H = g.H;
     const cx<float> *twl = stage_twiddles(buf + lspan<true>(4 * Mp), tw, Mp);
     const int npad = 8 * ((H + 7) / 8);
     for (int slot = blockIdx.x; slot < npad; slot += gridDim.x) {
         const int k = xcd_slot_row(slot, H);
         if (k >= H) continue;
-        lds_fill<NTHR, true>(buf, 4 * Mp, [&](int q) {
-            const int t = q / Mp, u = q - t * Mp;
-            return u < n1d ? spec[t * plane + (long)k * n1d + u] : cx<float>{0.f, 0.f};
-        });
-        __syncthreads();
-        fft_lines<float, NTHR, MAXTOT, true>(buf, dg->pl, 4, Mp, twl, false);
-        const double wk = (k == 0 || 2 * k == Mp) ? 1.0 : 2.0;
-        double z0 = 0, z1 = 0, z2 = 0, z3 = 0;
-        for (int m = threadIdx.x; m < Mp; m += NTHR) {
-            const int mb = m <= Mp / 2 ? m : Mp - m;
-            const float sg = m <= Mp / 2 ? 1.f : -1.f;
-            const float cq = KQ[(long)k * H + mb], c1 = K1[(long)k * H + mb];
-            const float sx = sg * KS[(long)k * H + mb], sy = KT[(long)k * H + mb];
-            const cx<float> A = buf[lp<true>(m)], C = buf[lp<true>(Mp + m)], BX = buf[lp<true>(2 * Mp + m)],
-                            BY = buf[lp<true>(3 * Mp + m)];
-            const cx<float> KxA = mul_negi(sx * A), KyA = mul_negi(sy * A);
-            buf[lp<true>(m)] = cq * A;
-            buf[lp<true>(Mp + m)] = KxA - cq * BX;
-            buf[lp<true>(2 * Mp + m)] = KyA - cq * BY;
-            const double a2 = (double)A.x * A.x + (double)A.y * A.y;
-            z0 += (double)c1 * a2;
-            z1 += (double)cq * ((double)C.x * A.x + (double)C.y * A.y);
-            // Re(conj(B) (-i s) A) = s Im(conj(B) A)
-            z2 += (double)sx * ((double)BX.x * A.y - (double)BX.y * A.x) +
-                  (double)sy * ((double)BY.x * A.y - (double)BY.y * A.x);
-            z3 += (double)cq * ((double)BX.x * BX.x + (double)BX.y * BX.y + (double)BY.x * BY.x + (double)BY.y * BY.y);
-        }
-        __syncthreads();
-        fft_lines<float, NTHR, MAXTOT, true>(buf, dg->pl, 3, Mp, twl, true);
-        for (int q = threadIdx.x; q < 3 * n1d; q += NTHR) {
-            const int t = q / n1d, u = q - t * n1d;
-            ospec[t * plane + (long)k * n1d + u] = buf[lp<true>(t * Mp + u)];
-        }
-        z0 = block_sum(z0 * wk, shz);
-        z1 = block_sum(z1 * wk, shz);
-        z2 = block_sum(z2 * wk, shz);
-        z3 = block_sum(z3 * wk, shz);
-        if (threadIdx.x == 0) {
-            zpart[k * 4 + 0] = z0;
-            zpart[k * 4 + 1] = z1;
-            zpart[k * 4 + 2] = z2;
-            zpart[k * 4 + 3] = z3;
-        }
-        __syncthreads();
+        cols_f32_line<NTHR, MAXTOT>(dg, buf, twl, spec, KQ, K1, KS, KT, ospec, zpart, shz, k);
     }
 }
 
@@ -1235,9 +1339,46 @@ __global__ __launch_bounds__(NTHR) void k_cols_plain(const DevGeo *__restrict__ 
 // Row FFT, inverse.  ospec[t][k][u], k in [0, H): half spectra over v of real rows.  Terms 2l, 2l+1 are
 // rebuilt into one Hermitian-extended complex line (self-conjugate bins taken real), inverse-
 // transformed, scaled by 1 / Mp^2, and written to out[u][v][sout + ...].
-template <typename T, int NTHR, int MAXTOT, bool PAD = false>
+template <typename T, int NTHR, int MAXTOT, bool PAD>
+__device__ __forceinline__ void rows_inv_line(const DevGeo *__restrict__ dg, cx<T> *buf, const cx<T> *twl,
+                                              const cx<T> *__restrict__ ospec, int nterms, T *__restrict__ out,
+                                              int sout, int u) {
+    const Geo &g = dg->g;
+    const int Mp = g.Mp, n1d = g.n1d, H = g.H;
+    const T scale = (T)(1.0 / ((double)Mp * Mp));
+    const long plane = (long)H * n1d;
+    const int lines = (nterms + 1) / 2;
+    lds_fill<NTHR, PAD>(buf, lines * Mp, [&](int q) {
+        const int l = q / Mp, k = q - l * Mp;
+        const bool lo = k < H;
+        const int kk = lo ? k : Mp - k;
+        cx<T> f1 = ospec[(2 * l) * plane + (long)kk * n1d + u];
+        cx<T> f2 = 2 * l + 1 < nterms ? ospec[(2 * l + 1) * plane + (long)kk * n1d + u] : cx<T>{(T)0, (T)0};
+        if (kk == 0 || 2 * kk == Mp) {
+            f1.y = (T)0;
+            f2.y = (T)0;
+        }
+        if (!lo) {
+            f1 = conj(f1);
+            f2 = conj(f2);
+        }
+        return f1 + mul_i(f2);
+    });
+    __syncthreads();
+    fft_lines<T, NTHR, MAXTOT, PAD>(buf, dg->pl, lines, Mp, twl, true);
+    for (int q = threadIdx.x; q < lines * n1d; q += NTHR) {
+        const int l = q / n1d, v = q - l * n1d;
+        const cx<T> z = buf[lp<PAD>(l * Mp + v)];
+        T *dst = out + ((long)u * n1d + v) * sout + 2 * l;
+        dst[0] = z.x * scale;
+        if (2 * l + 1 < nterms) dst[1] = z.y * scale;
+    }
+    __syncthreads();
+}
+
 // zpart != null: the first workgroup also computes the FP32 form's final Z (k_zfinal's reduction and
 // order; NTHR == kThreads) before its lines, one launch fewer on the step's critical path.
+template <typename T, int NTHR, int MAXTOT, bool PAD = false>
 __global__ __launch_bounds__(NTHR) void k_rows_inv(const DevGeo *__restrict__ dg, const cx<T> *__restrict__ tw,
                                                    const cx<T> *__restrict__ ospec, int nterms,
                                                    T *__restrict__ out, int sout, const double *__restrict__ zpart,
@@ -1250,41 +1391,14 @@ __global__ __launch_bounds__(NTHR) void k_rows_inv(const DevGeo *__restrict__ dg
     const Geo &g = dg->g;
     extern __shared__ unsigned char smem_raw[];
     cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
-    const int Mp = g.Mp, n1d = g.n1d, H = g.H;
-    const T scale = (T)(1.0 / ((double)Mp * Mp));
-    const long plane = (long)H * n1d;
+    const int Mp = g.Mp, n1d = g.n1d;
     const int lines = (nterms + 1) / 2;
     const cx<T> *twl = stage_twiddles(buf + lspan<PAD>(lines * Mp), tw, Mp);
     const int npad = 8 * ((n1d + 7) / 8);
     for (int slot = blockIdx.x; slot < npad; slot += gridDim.x) {
         const int u = xcd_slot_row(slot, n1d);
         if (u >= n1d) continue;
-        lds_fill<NTHR, PAD>(buf, lines * Mp, [&](int q) {
-            const int l = q / Mp, k = q - l * Mp;
-            const bool lo = k < H;
-            const int kk = lo ? k : Mp - k;
-            cx<T> f1 = ospec[(2 * l) * plane + (long)kk * n1d + u];
-            cx<T> f2 = 2 * l + 1 < nterms ? ospec[(2 * l + 1) * plane + (long)kk * n1d + u] : cx<T>{(T)0, (T)0};
-            if (kk == 0 || 2 * kk == Mp) {
-                f1.y = (T)0;
-                f2.y = (T)0;
-            }
-            if (!lo) {
-                f1 = conj(f1);
-                f2 = conj(f2);
-            }
-            return f1 + mul_i(f2);
-        });
-        __syncthreads();
-        fft_lines<T, NTHR, MAXTOT, PAD>(buf, dg->pl, lines, Mp, twl, true);
-        for (int q = threadIdx.x; q < lines * n1d; q += NTHR) {
-            const int l = q / n1d, v = q - l * n1d;
-            const cx<T> z = buf[lp<PAD>(l * Mp + v)];
-            T *dst = out + ((long)u * n1d + v) * sout + 2 * l;
-            dst[0] = z.x * scale;
-            if (2 * l + 1 < nterms) dst[1] = z.y * scale;
-        }
-        __syncthreads();
+        rows_inv_line<T, NTHR, MAXTOT, PAD>(dg, buf, twl, ospec, nterms, out, sout, u);
     }
 }
 
@@ -1485,19 +1599,17 @@ __global__ void k_zfinal(const DevGeo *__restrict__ dg, const double *__restrict
 }
 
 // ---------------------------------------------------------------- interpolation + update, FP32 form
+// Points [bid 256, (bid + 1) 256) of the interpolation and update; the body of k_interp_f32 and of its
+// batched form.  sh: kThreads / kWave doubles of LDS.
 template <int P, int D>
-__global__ __launch_bounds__(kThreads) void k_interp_f32(float *__restrict__ Y, const float *__restrict__ pos,
-                                                         const float4 *__restrict__ gout,
-                                                         const DevGeo *__restrict__ dg,
-                                                         const double *__restrict__ zbuf,
-                                                         const StepDev *__restrict__ sd,
-                                                         float *__restrict__ dC, float *__restrict__ uY,
-                                                         float *__restrict__ gains, double *__restrict__ part,
-                                                         int N) {
-    if (!dg->live) return;
+__device__ __forceinline__ void interp_block(float *__restrict__ Y, const float *__restrict__ pos,
+                                             const float4 *__restrict__ gout, const DevGeo *__restrict__ dg,
+                                             const double *__restrict__ zbuf, const StepDev *__restrict__ sd,
+                                             float *__restrict__ dC, float *__restrict__ uY, float *__restrict__ gains,
+                                             double *__restrict__ part, int N, int bid, double *sh) {
     const Geo &g = dg->g;
     const StepDev up = *sd;  // mode 0: write dC only; 1: optimiser update of Y, uY, gains
-    const int i = blockIdx.x * kThreads + threadIdx.x;
+    const int i = bid * kThreads + threadIdx.x;
     double ysum[D];
 #pragma unroll
     for (int d = 0; d < D; d++) ysum[d] = 0.0;
@@ -1613,20 +1725,31 @@ __global__ __launch_bounds__(kThreads) void k_interp_f32(float *__restrict__ Y, 
         }
     }
     if (up.mode == 1) {
-        __shared__ double sh[kThreads / kWave];
 #pragma unroll
         for (int d = 0; d < D; d++) {
             const double s = block_sum(ysum[d], sh);
-            if (threadIdx.x == 0) part[blockIdx.x * D + d] = s;
+            if (threadIdx.x == 0) part[bid * D + d] = s;
         }
     }
 }
 
-// Mean of the updated Y from block partials (fixed order), FP64.
-__global__ void k_mean_final(const DevGeo *__restrict__ dg, const double *__restrict__ part, int nblk, int D,
-                             double N, double *__restrict__ mean) {
+template <int P, int D>
+__global__ __launch_bounds__(kThreads) void k_interp_f32(float *__restrict__ Y, const float *__restrict__ pos,
+                                                         const float4 *__restrict__ gout,
+                                                         const DevGeo *__restrict__ dg,
+                                                         const double *__restrict__ zbuf,
+                                                         const StepDev *__restrict__ sd,
+                                                         float *__restrict__ dC, float *__restrict__ uY,
+                                                         float *__restrict__ gains, double *__restrict__ part,
+                                                         int N) {
     if (!dg->live) return;
     __shared__ double sh[kThreads / kWave];
+    interp_block<P, D>(Y, pos, gout, dg, zbuf, sd, dC, uY, gains, part, N, blockIdx.x, sh);
+}
+
+// Mean of the updated Y from block partials (fixed order), FP64.  One workgroup of kThreads.
+__device__ __forceinline__ void mean_block(const double *__restrict__ part, int nblk, int D, double N,
+                                           double *__restrict__ mean, double *sh) {
     const int T = blockDim.x;
     // Each thread adds part[t], part[t + T], ... of a coordinate in that order.  The loads are issued
     // eight at a time; those past the end read as +0.0, which leaves the sum unchanged (it starts at
@@ -1666,16 +1789,23 @@ __global__ void k_mean_final(const DevGeo *__restrict__ dg, const double *__rest
     }
 }
 
+__global__ void k_mean_final(const DevGeo *__restrict__ dg, const double *__restrict__ part, int nblk, int D,
+                             double N, double *__restrict__ mean) {
+    if (!dg->live) return;
+    __shared__ double sh[kThreads / kWave];
+    mean_block(part, nblk, D, N, mean, sh);
+}
+
 // zeroMean (tsne.cpp:1851-1876): Y -= mean, with the min/max partials of the result for the next
 // geometry (same layout as k_minmax_partial).
+// Workgroup bid of nblk: the body of k_center and of its batched form.
 template <typename TV>
-__global__ __launch_bounds__(kThreads) void k_center(const DevGeo *__restrict__ dg, TV *__restrict__ Y, long n,
-                                                     int D, long skip, const double *__restrict__ mean,
-                                                     double *__restrict__ part) {
-    if (!dg->live) return;
+__device__ __forceinline__ void center_block(TV *__restrict__ Y, long n, int D, long skip,
+                                             const double *__restrict__ mean, double *__restrict__ part, int bid,
+                                             int nblk) {
     double mx = -INFINITY, mn = INFINITY;
     const double m0 = mean[0], m1 = D == 2 ? mean[1] : 0.0;
-    for (long e = (long)blockIdx.x * kThreads + threadIdx.x; e < n; e += (long)gridDim.x * kThreads) {
+    for (long e = (long)bid * kThreads + threadIdx.x; e < n; e += (long)nblk * kThreads) {
         const double m = D > 2 ? mean[e % D] : (D == 2 && (e & 1)) ? m1 : m0;
         const TV v = (TV)((double)Y[e] - m);
         Y[e] = v;
@@ -1696,9 +1826,17 @@ __global__ __launch_bounds__(kThreads) void k_center(const DevGeo *__restrict__ 
             mx = fmax(mx, sh[0][i]);
             mn = fmin(mn, sh[1][i]);
         }
-        part[blockIdx.x * 2 + 0] = mx;
-        part[blockIdx.x * 2 + 1] = mn;
+        part[bid * 2 + 0] = mx;
+        part[bid * 2 + 1] = mn;
     }
+}
+
+template <typename TV>
+__global__ __launch_bounds__(kThreads) void k_center(const DevGeo *__restrict__ dg, TV *__restrict__ Y, long n,
+                                                     int D, long skip, const double *__restrict__ mean,
+                                                     double *__restrict__ part) {
+    if (!dg->live) return;
+    center_block<TV>(Y, n, D, skip, mean, part, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------- spatial reordering (FP32 form)

@@ -76,12 +76,13 @@ def c2_workload(N=1_000_000, D=50, k=50, per_point=45, seed=20261204):
     return (row_P, col_P, val_P), Y, labels
 
 
-def c3_workload(N=2_000_000, k=100, per_point=30, seed=20261205):
+def c3_workload(N=2_000_000, k=100, per_point=75, seed=20261205):
     """C3's gradient inputs: N points in 1D from k clusters with Dirichlet(0.5) weights, centres uniform
     in [-280, 280], sd 5 (span about [-300, 300]); P built like c2_workload's from per_point random
     same-cluster neighbours with weights on the cluster-local 1D gaps (int64 row offsets, int32
-    columns), about 2 * per_point non-zeros per row.  The kNN step of the full configuration (D = 100,
-    K = 150) is not part of the gradient path."""
+    columns), about 2 * per_point non-zeros per row: 75 neighbours give the ~150 per row that K = 150
+    (perplexity 50) leaves after symmetrisation.  The kNN step of the full configuration (D = 100) is
+    not part of the gradient path."""
     rng = np.random.default_rng(seed)
     t0 = time.time()
     weights = rng.dirichlet(np.full(k, 0.5))

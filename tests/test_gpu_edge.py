@@ -15,7 +15,7 @@ import pytest
 
 import fitsne_gpu as fg
 import orc
-from datagen import blobs
+from datagen import blobs, random_csr
 
 pytestmark = pytest.mark.gpu
 
@@ -222,3 +222,34 @@ def test_invalid_input_is_an_error_and_the_context_stays_usable(prec):
     Yin = Y if prec == fg.FP64_PLAIN else Y.astype(np.float32).astype(np.float64)
     want, _ = orc.gradient(P, Yin)
     assert rel(got, want) <= (1e-9 if prec == fg.FP64_PLAIN else 1e-4)
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_first_point_far_below_the_grid_is_refused_by_the_fp32_form(graph):
+    """The 2D min rule leaves xs[0] out of the grid when it is the strict minimum; its extrapolated spread
+    weights grow like its distance (in box widths) to the power 2(p - 1) + 2.  Far enough out they pass
+    the FP32 form's int64 fixed-point range: the step is refused with an error and Y is left unchanged,
+    instead of wrapping the integer grid.  (The FP64 plain path has no such limit; that far out its FFT
+    rounding swamps every other point, so no tolerance applies to it there.)"""
+    say(f"[gpu] far outlier below the 2D grid, captured={graph}")
+    N = 3000
+    P = random_csr(N, 8, seed=21)
+    Y = blobs(N, 2, 5, seed=22, span=40.0)
+    Y[0, 0] = Y.min() - 2.0e6  # ~2.5e6 box widths below the grid
+    ctx = fg.Context(no_dims=2, precision=fg.FP32)
+    ctx.set_P(*P)
+    ctx.set_Y(Y)
+    if graph:
+        ctx.set_graph(True)
+    before = ctx.get_Y()
+    with pytest.raises(fg.FitsneError) as e:
+        ctx.step(0.5, 200.0, 5.0)
+        ctx.sync()
+    assert e.value.code == fg.E_UNSUPPORTED
+    assert np.array_equal(ctx.get_Y(), before)
+    # the context stays usable once the outlier is gone
+    Y[0, 0] = Y[1, 0]
+    ctx.set_Y(Y)
+    ctx.step(0.5, 200.0, 5.0)
+    ctx.sync()
+    ctx.close()

@@ -85,3 +85,69 @@ def test_grid_beyond_the_limit_is_refused_on_the_device():
     host = _ctx(P, Y, 2)
     with pytest.raises(fg.FitsneError):
         host.step(0.5, 200.0, 5.0)
+
+
+def test_host_gradient_after_captured_steps_recomputes_kernel_spectra():
+    """Host-planned and captured steps share the kernel-spectrum buffers: a host gradient at Y0, captured
+    steps at other geometries, then Y0 again must give the gradient of a fresh context (the host step's
+    record of which spectra it holds does not survive a captured step, a reset Y or a new capture)."""
+    P, Y0, _ = synth.small_mixture_workload(N=6000, D=2, span=40.0, seed=8)
+    _note("host gradient, captured steps, set_Y(Y0), host gradient")
+    c = _ctx(P, Y0, 2)
+    g0, z0 = c.gradient()
+    c.set_graph(True)
+    for k in range(4):
+        c.step(0.5, 500.0, 5.0)  # large steps: the grid bounds move
+    c.sync()
+    assert c.last_geometry()["min"] != fg_min(Y0)
+    c.set_graph(False)
+    c.set_Y(Y0)
+    g1, z1 = c.gradient()
+    fresh = _ctx(P, Y0, 2)
+    g2, z2 = fresh.gradient()
+    assert np.array_equal(g1, g2) and z1 == z2
+    assert np.array_equal(g0, g2) and z0 == z2
+
+
+def fg_min(Y):
+    return float(Y.min())
+
+
+def test_fused_batch_follows_member_changes_and_mixed_batches_fall_back():
+    """A member whose Y is reset between batched steps (new layout: the fused batch is rebuilt) still
+    matches its single context bitwise, and so does a host-planned gradient taken from a member between
+    batched steps.  A batch mixing 1D and 2D problems runs each member's own captured step instead."""
+    sizes = [3000, 9000, 5000]
+    data = [synth.small_mixture_workload(N=n, D=2, span=30.0, seed=30 + i) for i, n in enumerate(sizes)]
+    singles = [_ctx(P, Y, 2) for (P, Y, _) in data]
+    members = [_ctx(P, Y, 2) for (P, Y, _) in data]
+    batch = fg.Batch(members)
+    prm = [(0.5, 200.0, 5.0)] * len(sizes)
+    for k in range(3):
+        batch.step(prm)
+        for c in singles:
+            c.step(0.5, 200.0, 5.0)
+    g_m, z_m = members[1].gradient()
+    g_s, z_s = singles[1].gradient()
+    assert np.array_equal(g_m, g_s) and z_m == z_s
+    Ynew = data[2][1] * 1.5
+    members[2].set_Y(Ynew)
+    singles[2].set_Y(Ynew)
+    for k in range(3):
+        batch.step(prm)
+        for c in singles:
+            c.step(0.5, 200.0, 5.0)
+    for s, m in zip(singles, members):
+        assert np.array_equal(s.get_Y(), m.get_Y())
+    batch.close()
+    P1, Y1, _ = synth.small_mixture_workload(N=4000, D=1, span=100.0, seed=40)
+    mixed = [_ctx(data[0][0], data[0][1], 2), _ctx(P1, Y1, 1, df=0.5)]
+    alone = [_ctx(data[0][0], data[0][1], 2), _ctx(P1, Y1, 1, df=0.5)]
+    mb = fg.Batch(mixed)
+    for k in range(4):
+        mb.step(prm[:2])
+        for c in alone:
+            c.step(0.5, 200.0, 5.0)
+    for s, m in zip(alone, mixed):
+        assert np.array_equal(s.get_Y(), m.get_Y())
+    mb.close()

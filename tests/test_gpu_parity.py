@@ -8,6 +8,7 @@ import pytest
 
 import fitsne_gpu as fg
 import orc
+from accuracy import gradient_errors
 from datagen import blobs, random_csr
 
 pytestmark = pytest.mark.gpu
@@ -80,9 +81,15 @@ def test_fp32_gradient_matches_oracle(D, df):
     got, sq = ctx.gradient()
     ctx.close()
     r = rel(got, want)
-    say(f"    rel={r:.3e} Zrel={abs(sq - zq) / zq:.3e}")
+    e = gradient_errors(got, want)
+    say(f"    rel={r:.3e} Zrel={abs(sq - zq) / zq:.3e} per point: max {e['max_rel']:.3e} p50 {e['p50']:.3e} "
+        f"p99 {e['p99']:.3e}")
     assert abs(sq - zq) / abs(zq) <= 1e-5
     assert r <= 1e-4
+    # per point: the largest error against the largest gradient, and the 99th percentile of each point's
+    # error against its own gradient (plus 1e-3 of the rms gradient)
+    assert e["max_rel"] <= 1e-4
+    assert e["p99"] <= 1e-3
 
 
 def test_n_body_fft_2d_mirror_matches_oracle():
