@@ -2595,6 +2595,86 @@ extern "C" int fitsne_evaluate_error_fft(const uint32_t *row_P, const uint32_t *
     return fitsne_kl(c, C);
 }
 
+// ---------------------------------------------------------------- KL without allocation
+// evaluateErrorFft for callers that cannot allocate device memory: the CSR, Y, Z and the block partials
+// live in one file-scope device array (`scratch`, kNoAllocScratch bytes per device, held until the process
+// ends).  Host code never passes the array's name to a kernel or a copy: it takes the device address from
+// hipGetSymbolAddress once per device and uses that.  Calls are serialised by a process-wide lock; two
+// processes have separate arrays.  Nothing in the array needs to start at zero: every word a call reads was
+// written earlier in that call.
+constexpr size_t kNoAllocScratch = 32ull << 20;
+constexpr int kNoAllocMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratch];
+static std::mutex g_noalloc_mu;
+static void *g_noalloc_ptr[kNoAllocMaxDevices];
+
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+extern "C" int fitsne_evaluate_error_fft_noalloc(const uint32_t *row_P, const uint32_t *col_P, const double *val_P,
+                                                 const double *Y, int N, int D, double sum_Q, double df, double *C) {
+    if (!row_P || !col_P || !val_P || !Y || !C || N <= 0 || (D != 1 && D != 2))
+        return fail(FITSNE_E_ARG, "invalid evaluate_error_fft arguments");
+    const long long nnz = (long long)row_P[N];
+    if (row_P[0] != 0) return fail(FITSNE_E_ARG, "row_P must start at 0");
+    for (int i = 0; i < N; i++)
+        if (row_P[i + 1] < row_P[i]) return fail(FITSNE_E_ARG, "row_P must be non-decreasing");
+    for (long long k = 0; k < nnz; k++)
+        if (col_P[k] >= (uint32_t)N) return fail(FITSNE_E_ARG, "col_P entry out of range");
+    for (size_t e = 0; e < (size_t)N * D; e++)
+        if (!std::isfinite(Y[e])) return fail(FITSNE_E_ARG, "Y contains non-finite values");
+    // the same launch as the context path (32 lanes per row), so the partials and their sum are the same
+    const long threads = (long)N * 32;
+    const int nblk = blocks_for(threads);
+    const size_t o_row = 0, o_col = o_row + align256(sizeof(uint32_t) * ((size_t)N + 1));
+    const size_t o_val = o_col + align256(sizeof(uint32_t) * (size_t)std::max<long long>(nnz, 1));
+    const size_t o_y = o_val + align256(sizeof(double) * (size_t)std::max<long long>(nnz, 1));
+    const size_t o_z = o_y + align256(sizeof(double) * (size_t)N * D);
+    const size_t o_part = o_z + align256(sizeof(double) * 2);
+    const size_t o_out = o_part + align256(sizeof(double) * (size_t)nblk);
+    const size_t need = o_out + align256(sizeof(double) * 2);
+    if (need > kNoAllocScratch)
+        return fail(FITSNE_E_ARG, "evaluate_error_fft_noalloc: N = " + std::to_string(N) + ", nnz = " +
+                                      std::to_string(nnz) + " need " + std::to_string(need) +
+                                      " bytes of device scratch; this entry point holds " +
+                                      std::to_string(kNoAllocScratch) + " (use fitsne_evaluate_error_fft)");
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    int dev = 0;
+    HIPTRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNoAllocMaxDevices) return fail(FITSNE_E_UNSUPPORTED, "device index beyond this build's table");
+    if (!g_noalloc_ptr[dev]) {
+        void *ptr = nullptr;
+        HIPTRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) return fail(FITSNE_E_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+        g_noalloc_ptr[dev] = ptr;
+    }
+    unsigned char *base = static_cast<unsigned char *>(g_noalloc_ptr[dev]);
+    auto *rp = reinterpret_cast<uint32_t *>(base + o_row);
+    auto *cp = reinterpret_cast<uint32_t *>(base + o_col);
+    auto *vp = reinterpret_cast<double *>(base + o_val);
+    auto *yp = reinterpret_cast<double *>(base + o_y);
+    auto *zp = reinterpret_cast<double *>(base + o_z);
+    auto *part = reinterpret_cast<double *>(base + o_part);
+    auto *out = reinterpret_cast<double *>(base + o_out);
+    const double z[2] = {sum_Q, 1.0 / sum_Q};
+    HIPTRY(hipMemcpy(rp, row_P, sizeof(uint32_t) * ((size_t)N + 1), hipMemcpyHostToDevice));
+    if (nnz) {
+        HIPTRY(hipMemcpy(cp, col_P, sizeof(uint32_t) * (size_t)nnz, hipMemcpyHostToDevice));
+        HIPTRY(hipMemcpy(vp, val_P, sizeof(double) * (size_t)nnz, hipMemcpyHostToDevice));
+    }
+    HIPTRY(hipMemcpy(yp, Y, sizeof(double) * (size_t)N * D, hipMemcpyHostToDevice));
+    HIPTRY(hipMemcpy(zp, z, sizeof(z), hipMemcpyHostToDevice));
+    if (D == 2)
+        k_kl<double, uint32_t, uint32_t, 2, 32><<<dim3(nblk), dim3(kThreads), 0, 0>>>(rp, cp, vp, 1.0, yp, N, df, zp, part);
+    else
+        k_kl<double, uint32_t, uint32_t, 1, 32><<<dim3(nblk), dim3(kThreads), 0, 0>>>(rp, cp, vp, 1.0, yp, N, df, zp, part);
+    k_sum_final<<<dim3(1), dim3(kThreads), 0, 0>>>(part, nblk, 0.0, out);
+    HIPTRY(hipGetLastError());
+    double res[2];
+    HIPTRY(hipMemcpy(res, out, sizeof(res), hipMemcpyDeviceToHost));
+    *C = res[0];
+    return FITSNE_OK;
+}
+
 extern "C" int fitsne_distances2similarities_batch(const double *D, double *P, int n_points, int K,
                                                    double perplexity, double sigma, int ifSquared,
                                                    const double *beta0, double *beta, int *iters) {

@@ -47,6 +47,7 @@ SIGNATURES = {
     "fitsne_n_body_fft_1d": (_ci, [_ci, _ci, _dp, _dp, _ci, _ci, _dp, _dp, _dp, _dp, _dp, _dp]),
     "fitsne_interpolate": (_ci, [_ci, _ci, _dp, _dp, _dp]),
     "fitsne_evaluate_error_fft": (_ci, [_u32, _u32, _dp, _dp, _ci, _ci, _cd, _cd, C.POINTER(_cd)]),
+    "fitsne_evaluate_error_fft_noalloc": (_ci, [_u32, _u32, _dp, _dp, _ci, _ci, _cd, _cd, C.POINTER(_cd)]),
     "fitsne_distances2similarities_batch": (_ci, [_dp, _dp, _ci, _ci, _cd, _cd, _ci, _vp, _dp, _vp]),
     "fitsne_knn_exact": (_ci, [_dp, _ci, _ci, _ci, _i32, _dp]),
     "fitsne_symmetrize": (_ci, [_u32, _u32, _dp, _ci, _u32, _u32, _dp, C.POINTER(C.c_longlong)]),
@@ -198,6 +199,17 @@ def evaluate_error_fft(P, Y, sum_Q, df=1.0):
     D = 1 if Y.ndim == 1 else Y.shape[1]
     out = _cd()
     check(lib().fitsne_evaluate_error_fft(row, col, val, Y.reshape(-1), N, D, sum_Q, df, C.byref(out)))
+    return out.value
+
+
+def evaluate_error_fft_noalloc(P, Y, sum_Q, df=1.0):
+    """evaluate_error_fft through the entry point that allocates no device memory (fixed 32 MiB scratch)."""
+    row, col, val = P
+    Y = np.ascontiguousarray(Y, dtype=np.float64)
+    N = Y.shape[0]
+    D = 1 if Y.ndim == 1 else Y.shape[1]
+    out = _cd()
+    check(lib().fitsne_evaluate_error_fft_noalloc(row, col, val, Y.reshape(-1), N, D, sum_Q, df, C.byref(out)))
     return out.value
 
 

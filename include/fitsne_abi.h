@@ -109,6 +109,15 @@ int fitsne_interpolate(int n_interpolation_points, int N, const double *y_in_box
 int fitsne_evaluate_error_fft(const uint32_t *row_P, const uint32_t *col_P, const double *val_P,
                               const double *Y, int N, int D, double sum_Q, double df, double *C);
 
+/* No counterpart in the reference: fitsne_evaluate_error_fft for callers that cannot allocate device
+ * memory (code under graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, result and errors, bitwise the same C.  It allocates nothing: the staged CSR, Y and the
+ * partial sums live in a fixed 32 MiB device array per GPU, held until the process ends; sizes whose
+ * staging needs more (about 4 (N+1) + 12 nnz + 8 N D + N bytes) return FITSNE_E_ARG.  Runs on the
+ * calling thread's current device; calls within one process are serialised. */
+int fitsne_evaluate_error_fft_noalloc(const uint32_t *row_P, const uint32_t *col_P, const double *val_P,
+                                      const double *Y, int N, int D, double sum_Q, double df, double *C);
+
 /* Replaces TSNE::distances2similarities (tsne.cpp:1394-1469; tsne.h:105) for a batch of points:
  * D[n_points*K] distances (row i = point i), P[n_points*K] row-normalised affinities, beta[n_points].
  * beta0 (nullable) gives per-point starting betas for the bisection (the reference uses 1.0);
