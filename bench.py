@@ -79,6 +79,38 @@ def measured_traffic(kernel):
     return kib * 1024.0
 
 
+def per_kernel_roofline(alone, in_step, N, nnz, geo, D=2):
+    """HBM roofline of each kernel of the FP32 2D step: algorithmic bytes per launch (each array the kernel
+    must read or write, once) over its average duration when the step's kernels run one after another
+    (`alone`), with its duration inside the overlapped step beside it.  FFT passes move little HBM data
+    (their lines live in LDS); their fraction shows how far they are from being bandwidth bound."""
+    n1d, H = geo["n1d"], geo["fft_len"] // 2 + 1
+    nodes = n1d ** D
+    pts = N * D * 4
+    model = {
+        "attractive": 8 * (N + 1) + nnz * 8 + 2 * pts,
+        "spread_f32": pts + nodes * 4 * 8,
+        "rows_fwd": 2 * nodes * 32 + 4 * H * n1d * 8,
+        "cols_f32": 4 * H * n1d * 8 + 4 * H * H * 4 + 3 * H * n1d * 8,
+        "rows_inv": 3 * H * n1d * 8 + nodes * 16,
+        "interp_f32": 7 * pts + nodes * 16,
+        "kern_rows": H * n1d * 8,
+        "kern_cols": H * n1d * 8 + 4 * H * H * 4,
+    }
+    out = []
+    for name, nbytes in model.items():
+        cnt, tot = alone.get(name, (0, 0.0))
+        if cnt == 0:
+            continue
+        ms = tot / cnt
+        c2, t2 = in_step.get(name, (0, 0.0))
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        out.append({"kernel": name, "bound": "hbm", "algorithmic_bytes": int(nbytes), "avg_ms_alone": round(ms, 4),
+                    "achieved": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
+                    "avg_ms_in_step": round(t2 / c2, 4) if c2 else None})
+    return out
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -347,6 +379,69 @@ def bench_c3(args, fg, synth):
     print(json.dumps(out), flush=True)
 
 
+def bench_trajectory(args, fg, synth):
+    """Step time along a real optimisation: C2's affinities from the PCA start of its D = 50 data (sd 1e-4,
+    as fast_tsne.py), early exaggeration 12 to iteration 250, momentum 0.5 then 0.8, 1000 iterations, with
+    the points left in the order of their first Y, and again with a spatial re-sort (fitsne_reorder, then
+    the autotuner) at the exaggeration switch and every 250 iterations after it.  Reports ms per step
+    over 20-step windows starting at iterations 0, 250, 500 and 980, the whole run's time (re-sorts
+    included) and the final KL."""
+    N = args.n
+    log(f"trajectory: generating C2 with its PCA start, N={N}")
+    P, _, _, Y0 = synth.c2_workload(N, pca=True)
+    lr = max(200.0, N / 12.0)
+    windows = [0, 250, 500, 980]
+    out_runs = {}
+    for mode in ("first_order", "resorted"):
+        c = fg.Context(no_dims=2, precision=fg.FP32)
+        c.set_P(*P)
+        c.set_exaggeration(12.0)
+        c.set_Y(Y0)
+        c.autotune()
+        c.N = N
+        win_ms = {}
+        resort_s = 0.0
+        c.sync()
+        t_run = time.perf_counter()
+        t_win = None
+        for it in range(1000):
+            if it in windows:
+                c.sync()
+                t_win = time.perf_counter()
+            c.step(0.5 if it < 250 else 0.8, lr, 5.0)
+            if t_win is not None and it + 1 - max(w for w in windows if w <= it) == 20:
+                c.sync()
+                win_ms[str(max(w for w in windows if w <= it))] = 1e3 * (time.perf_counter() - t_win) / 20
+                t_win = None
+            if it == 249:
+                c.set_exaggeration(1.0)
+            if mode == "resorted" and it % 250 == 249:
+                c.sync()
+                t0 = time.perf_counter()
+                c.reorder()
+                c.autotune()
+                c.sync()
+                resort_s += time.perf_counter() - t0
+        c.sync()
+        total = time.perf_counter() - t_run
+        kl = c.kl()
+        geo = c.last_geometry()
+        c.close()
+        out_runs[mode] = {"ms_per_step_at": win_ms, "run_s": total, "resort_s": resort_s, "kl": kl,
+                          "n_boxes_end": geo["n_boxes"]}
+        log(f"trajectory {mode}: {win_ms}, run {total:.2f} s, KL {kl:.4f}")
+    out = {
+        "metric": "FP32 gradient step time along a 1000-iteration optimisation at N=1M 2D (C2 affinities, PCA start)",
+        "value": 1000.0 / out_runs["resorted"]["run_s"], "unit": "iters/s (whole run, re-sorts included)",
+        "n_gpus": 1, "steps": 1000, "warmup": 0, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (seeded Gaussian mixture, generated in place)",
+        "config": {"workload": f"C2 affinities, N={N}, PCA start sd 1e-4, early exaggeration 12 to 250, "
+                               f"lr {lr:.1f}"},
+        "runs": out_runs,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def bench_bh(args, fg, synth):
     """Barnes-Hut (nbody_algo = 1, theta = 0.5) at N = 100,000 in 2D: each step builds the reference's
     SPTree on the host in insertion order, then walks it and sums the edge forces on the GPU (bitwise the
@@ -402,7 +497,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-c4", action="store_true", help="leave the batched C4 figure out of the default line")
     ap.add_argument("--profile-steps", type=int, default=5)
-    ap.add_argument("--config", default="c2", choices=["c2", "c3", "c4", "exact", "bh"],
+    ap.add_argument("--config", default="c2", choices=["c2", "c3", "c4", "exact", "bh", "trajectory"],
                     help="c2 (default): the headline step at N=1M 2D; c3: 1D df=0.5 at N=2M; c4: 64 batched "
                          "embeddings; exact: theta = 0 at N=46000 2D; bh: Barnes-Hut at N=100000 2D")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -425,6 +520,8 @@ def main():
         return bench_exact(args, fg)
     if args.config == "bh":
         return bench_bh(args, fg, synth)
+    if args.config == "trajectory":
+        return bench_trajectory(args, fg, synth)
 
     if args.copy_calibration:
         import torch
@@ -475,6 +572,10 @@ def main():
     log(f"FP32: {ms:.3f} ms/step, {value:.1f} iters/s")
     stats = kernel_profile(c32, max(args.profile_steps, 1), prm)
     geo_end = c32.last_geometry()
+    # the same kernels one after another on one stream: each kernel's time alone
+    c32.set_overlap(False)
+    stats_alone = kernel_profile(c32, max(args.profile_steps, 1), prm)
+    geo_alone = c32.last_geometry()
     c32.close()
 
     cnt, tot = stats.get("attractive", (0, 0.0))
@@ -482,6 +583,7 @@ def main():
     att_bytes = 8 * (N + 1) + nnz * (4 + 4) + 2 * N * 2 * 4
     achieved = att_bytes / (att_ms * 1e-3) / 1e9 if att_ms > 0 else 0.0
     kernels_ms = {k: round(v[1] / max(v[0], 1), 4) for k, v in sorted(stats.items(), key=lambda kv: -kv[1][1])}
+    per_kernel = per_kernel_roofline(stats_alone, stats, N, nnz, geo_alone)
 
     acc = {}
     if g64 is not None:
@@ -529,7 +631,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": ATTR_KERNEL, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(ATTR_KERNEL),
-                     "algorithmic_bytes": att_bytes, "avg_ms": att_ms},
+                     "algorithmic_bytes": att_bytes, "avg_ms": att_ms, "per_kernel": per_kernel},
         "cpu_baseline": cpu,
         "fp64_plain": res64,
         "speedup_vs_fp64": (res64["ms_per_step"] / ms) if res64 else None,

@@ -18,6 +18,7 @@
 #include "fitsne_sym.hpp"
 #include "fitsne_bh.hpp"
 #include "fitsne_knn.hpp"
+#include "fitsne_knn_mfma.hpp"
 #include "fitsne_mlp.hpp"
 
 using namespace fitsne;
@@ -109,6 +110,7 @@ struct fitsne_ctx {
     // second stream of the host-planned FP32 step: the attractive term runs on it beside the grid stages
     hipStream_t astream = nullptr;
     int spread_pts = kSpreadPts;  // points per spread workgroup (fitsne_autotune); results do not depend on it
+    bool overlap = true;          // host-planned FP32 step: attractive term on astream beside the grid stages
     hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     hipEvent_t geo_ev = nullptr, kern_ev = nullptr;  // kernel spectra on astream: after the geometry, before cols
     bool join_pending = false;
@@ -568,7 +570,7 @@ static int launch_f32_stages(fitsne_ctx *c, const cx<float> *tw, const StageDims
                sizeof(cx<float>) * (lspan<true>((int)Mp) + Mp), dg, tw, c->Trow.as<const cx<float>>(), 1, c->KQ.as<float>(),
                c->K1.as<float>(), c->KS.as<float>(), c->KT.as<float>());
     };
-    const bool kside = D == 2 && kern && side;
+    const bool kside = D == 2 && kern && side && c->overlap;
     if (kside) {
         HIPTRY(hipEventRecord(c->geo_ev, c->stream));
         HIPTRY(hipStreamWaitEvent(c->astream, c->geo_ev, 0));
@@ -678,18 +680,20 @@ static int pipeline_f32(fitsne_ctx *c, int mode, const fitsne_step_params *sp) {
     // The second stream is made on first use: contexts that only run captured or batched steps never
     // create it, so their streams alone share the device's hardware queues (a second stream per context
     // measured a third fewer batched iterations).
-    if (!c->astream) HIPTRY(hipStreamCreateWithFlags(&c->astream, hipStreamNonBlocking));
-    HIPTRY(hipEventRecord(c->fork_ev, c->stream));
-    HIPTRY(hipStreamWaitEvent(c->astream, c->fork_ev, 0));
-    {
+    if (c->overlap) {
+        if (!c->astream) HIPTRY(hipStreamCreateWithFlags(&c->astream, hipStreamNonBlocking));
+        HIPTRY(hipEventRecord(c->fork_ev, c->stream));
+        HIPTRY(hipStreamWaitEvent(c->astream, c->fork_ev, 0));
         hipStream_t main = c->stream;
         c->stream = c->astream;
         const int rc = launch_attractive<float, D>(c, (float)c->exag, c->pos.as<float>(), live);
         c->stream = main;
         TRY(rc);
+        HIPTRY(hipEventRecord(c->join_ev, c->astream));
+        c->join_pending = true;
+    } else {
+        TRY((launch_attractive<float, D>(c, (float)c->exag, c->pos.as<float>(), live)));
     }
-    HIPTRY(hipEventRecord(c->join_ev, c->astream));
-    c->join_pending = true;
     // the step record (read by the interpolation; the attractive kernel took the exaggeration as an
     // argument) is written beside the attractive kernel
     TRY(put_step(c, mode, sp));
@@ -1761,6 +1765,14 @@ extern "C" int fitsne_set_spread_points(fitsne_ctx *c, int points) {
     return FITSNE_OK;
 }
 
+extern "C" int fitsne_set_overlap(fitsne_ctx *c, int enable) {
+    if (!c) return fail(FITSNE_E_ARG, "null context");
+    HIPTRY(hipSetDevice(c->device));
+    TRY(sync_ctx(c));
+    c->overlap = enable != 0;
+    return FITSNE_OK;
+}
+
 extern "C" int fitsne_set_graph(fitsne_ctx *c, int enable) {
     if (!c) return fail(FITSNE_E_ARG, "null context");
     if (enable && c->precision != FITSNE_FP32) return fail(FITSNE_E_UNSUPPORTED, "captured steps are for the FP32 form");
@@ -2806,11 +2818,110 @@ extern "C" int fitsne_knn_exact(const double *X, int N, int D, int K, int32_t *i
     while (B < 2 * K || B < K + kWave) B *= 2;
     const size_t lds = (size_t)kKnnQ * B * (sizeof(double) + sizeof(int));
     launch(c, "knn64", k_knn_f64, dim3((N + kKnnQ - 1) / kKnnQ), dim3(kWave), lds, c->knnX.as<const double>(),
-           c->knnXT.as<const double>(), N, D, K, B, c->knnI.as<int>(), c->knnD.as<double>());
+           c->knnXT.as<const double>(), N, D, K, B, c->knnI.as<int>(), c->knnD.as<double>(), (const int *)nullptr, N);
     HIPTRY(hipGetLastError());
     HIPTRY(hipMemcpyAsync(idx, c->knnI.p, sizeof(int) * (size_t)N * K, hipMemcpyDeviceToHost, c->stream));
     HIPTRY(hipMemcpyAsync(dist, c->knnD.p, sizeof(double) * (size_t)N * K, hipMemcpyDeviceToHost, c->stream));
     return sync_ctx(c);
+}
+
+// ---------------------------------------------------------------- kNN for large N (FP32 MFMA candidates)
+
+// Device buffers of one call, released on every return.
+struct KnnmBufs {
+    DBuf X, XT, XT4, nrm, nrm64, XS4, nrs, tau, cand, cnt, idx, dist, redo, nredo, nmax;
+    ~KnnmBufs() {
+        DBuf *all[] = {&X, &XT, &XT4, &nrm, &nrm64, &XS4, &nrs, &tau, &cand, &cnt, &idx, &dist, &redo, &nredo, &nmax};
+        for (DBuf *b : all) release(*b);
+    }
+};
+
+template <int NCH>
+static void knnm_passes(hipStream_t st, KnnmBufs &m, int N, int NS) {
+    const int nblk = (N + kKmWaves * kKmQ - 1) / (kKmWaves * kKmQ);
+    k_knnm_tau<NCH><<<dim3(nblk), dim3(kKmWaves * kWave), 0, st>>>(m.XT4.as<const float>(), N, m.nrm.as<const float>(),
+                                                                   m.XS4.as<const float>(), NS,
+                                                                   m.nrs.as<const float>(), m.tau.as<float>());
+    k_knnm_collect<NCH><<<dim3(nblk), dim3(kKmWaves * kWave), 0, st>>>(m.XT4.as<const float>(), N,
+                                                                       m.nrm.as<const float>(),
+                                                                       m.tau.as<const float>(), m.cand.as<int>(),
+                                                                       m.cnt.as<int>());
+}
+
+extern "C" int fitsne_knn_mfma(const double *X, int N, int D, int K, int32_t *idx, double *dist,
+                               long long *recomputed) {
+    if (!X || !idx || !dist || N < 2 || D < 1 || K < 1) return fail(FITSNE_E_ARG, "invalid kNN arguments");
+    if (K > N - 1) return fail(FITSNE_E_ARG, "K must be at most N - 1");
+    if (K > 1000) return fail(FITSNE_E_UNSUPPORTED, "K above 1000 is not supported");
+    if (D > 128) return fail(FITSNE_E_UNSUPPORTED, "the MFMA kNN takes D <= 128 (use fitsne_knn_exact)");
+    if (K > kKmCap / 2) return fail(FITSNE_E_UNSUPPORTED, "the MFMA kNN takes K <= 256 (use fitsne_knn_exact)");
+    const size_t nd = (size_t)N * D;
+    for (size_t i = 0; i < nd; i++)
+        if (!std::isfinite(X[i])) return fail(FITSNE_E_ARG, "X contains non-finite values");
+    const int need = (D + 3) / 4;
+    const int nch = need <= 8 ? 8 : need <= 13 ? 13 : need <= 16 ? 16 : need <= 25 ? 25 : 32;
+    const int NS = (N + kKmStride - 1) / kKmStride;
+    std::lock_guard<std::mutex> lk(g_mirror_mu);
+    fitsne_ctx *c;
+    TRY(mirror_ctx(&c, 1, 3, 1.0, 1, 1.0));
+    HIPTRY(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    KnnmBufs m;
+    TRY(ensure(m.X, sizeof(double) * nd));
+    TRY(ensure(m.XT4, sizeof(float4) * (size_t)nch * N));
+    TRY(ensure(m.nrm, sizeof(float) * N));
+    TRY(ensure(m.nrm64, sizeof(double) * N));
+    TRY(ensure(m.XS4, sizeof(float4) * (size_t)nch * NS));
+    TRY(ensure(m.nrs, sizeof(float) * NS));
+    TRY(ensure(m.tau, sizeof(float) * N));
+    TRY(ensure(m.cand, sizeof(int) * (size_t)N * kKmCap));
+    TRY(ensure(m.cnt, sizeof(int) * N));
+    TRY(ensure(m.idx, sizeof(int) * (size_t)N * K));
+    TRY(ensure(m.dist, sizeof(double) * (size_t)N * K));
+    TRY(ensure(m.redo, sizeof(int) * N));
+    TRY(ensure(m.nredo, sizeof(int)));
+    TRY(ensure(m.nmax, sizeof(double)));
+    HIPTRY(hipMemcpy(m.X.p, X, sizeof(double) * nd, hipMemcpyHostToDevice));
+    k_fill<int><<<dim3(1), dim3(kWave), 0, st>>>(m.nredo.as<int>(), 1, 0);
+    k_knnm_prep<<<dim3((N + 255) / 256), dim3(256), 0, st>>>(m.X.as<const double>(), N, D, nch, m.XT4.as<float4>(),
+                                                             m.nrm.as<float>(), m.nrm64.as<double>(), NS,
+                                                             m.XS4.as<float4>(), m.nrs.as<float>());
+    k_knnm_max<<<dim3(1), dim3(1024), 0, st>>>(m.nrm64.as<const double>(), (long)N, m.nmax.as<double>());
+    switch (nch) {
+    case 8: knnm_passes<8>(st, m, N, NS); break;
+    case 13: knnm_passes<13>(st, m, N, NS); break;
+    case 16: knnm_passes<16>(st, m, N, NS); break;
+    case 25: knnm_passes<25>(st, m, N, NS); break;
+    default: knnm_passes<32>(st, m, N, NS); break;
+    }
+    const double epsu = 2.0 * (4.0 * nch + 8.0) * std::ldexp(1.0, -24);
+    k_knnm_exact<<<dim3(N), dim3(kWave), 0, st>>>(m.X.as<const double>(), N, D, K, m.cand.as<const int>(),
+                                                  m.cnt.as<const int>(), m.tau.as<const float>(),
+                                                  m.nrm64.as<const double>(), m.nmax.as<const double>(), epsu,
+                                                  m.idx.as<int>(), m.dist.as<double>(), m.redo.as<int>(),
+                                                  m.nredo.as<int>());
+    HIPTRY(hipGetLastError());
+    int nredo = 0;
+    HIPTRY(hipMemcpyAsync(&nredo, m.nredo.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPTRY(hipStreamSynchronize(st));
+    if (nredo < 0 || nredo > N) return fail(FITSNE_E_HIP, "inconsistent recompute count");
+    if (nredo > 0) {  // the queries the candidate lists could not settle: the FP64 kernel over all points
+        TRY(ensure(m.XT, sizeof(double) * nd));
+        k_transpose_f64<<<dim3((N + 31) / 32, (D + 31) / 32), dim3(256), 0, st>>>(m.X.as<const double>(), (long)N, D,
+                                                                                m.XT.as<double>());
+        int B = 128;
+        while (B < 2 * K || B < K + kWave) B *= 2;
+        const size_t lds = (size_t)kKnnQ * B * (sizeof(double) + sizeof(int));
+        k_knn_f64<<<dim3((nredo + kKnnQ - 1) / kKnnQ), dim3(kWave), lds, st>>>(
+            m.X.as<const double>(), m.XT.as<const double>(), N, D, K, B, m.idx.as<int>(), m.dist.as<double>(),
+            m.redo.as<const int>(), nredo);
+        HIPTRY(hipGetLastError());
+    }
+    HIPTRY(hipMemcpyAsync(idx, m.idx.p, sizeof(int) * (size_t)N * K, hipMemcpyDeviceToHost, st));
+    HIPTRY(hipMemcpyAsync(dist, m.dist.p, sizeof(double) * (size_t)N * K, hipMemcpyDeviceToHost, st));
+    HIPTRY(hipStreamSynchronize(st));
+    if (recomputed) *recomputed = nredo;
+    return FITSNE_OK;
 }
 
 // ============================================================================ learned β warm start

@@ -75,16 +75,23 @@ __device__ void knn_compact(double *key, int *idx, int cnt, int B) {
     knn_sort(key, idx, B);
 }
 
-// One workgroup = one wavefront = kKnnQ queries.  Dynamic LDS: kKnnQ * B * (8 + 4) bytes, where
-// B = power of two >= max(2K, K + 64, 128) so that a round of 64 appends always fits after a cut.
+// One workgroup = one wavefront = kKnnQ queries: points blockIdx.x * kKnnQ + j, or, with a query list,
+// qlist[blockIdx.x * kKnnQ + j] for list positions below nq.  Dynamic LDS: kKnnQ * B * (8 + 4) bytes,
+// where B = power of two >= max(2K, K + 64, 128) so that a round of 64 appends always fits after a cut.
 __global__ __launch_bounds__(kWave) void k_knn_f64(const double *__restrict__ X, const double *__restrict__ XT,
                                                    int N, int D, int K, int B, int *__restrict__ out_idx,
-                                                   double *__restrict__ out_dist) {
+                                                   double *__restrict__ out_dist, const int *__restrict__ qlist,
+                                                   int nq) {
     extern __shared__ double knn_smem[];
     double *bk = knn_smem;
     int *bi = reinterpret_cast<int *>(knn_smem + (long)kKnnQ * B);
     const int lane = threadIdx.x;
-    const int q0 = blockIdx.x * kKnnQ;
+    int qid[kKnnQ];  // the block's queries (-1: none)
+#pragma unroll
+    for (int j = 0; j < kKnnQ; j++) {
+        const int s = blockIdx.x * kKnnQ + j;
+        qid[j] = qlist ? (s < nq ? qlist[s] : -1) : (s < N ? s : -1);
+    }
     int cnt[kKnnQ];
     double thr[kKnnQ];
 #pragma unroll
@@ -104,7 +111,7 @@ __global__ __launch_bounds__(kWave) void k_knn_f64(const double *__restrict__ X,
                 const double xc = XT[(long)d * N + c];
 #pragma unroll
                 for (int j = 0; j < kKnnQ; j++) {
-                    const int qi = min(q0 + j, N - 1);
+                    const int qi = qid[j] >= 0 ? qid[j] : 0;
                     const double diff = X[(long)qi * D + d] - xc;
                     acc[j] += diff * diff;
                 }
@@ -112,8 +119,8 @@ __global__ __launch_bounds__(kWave) void k_knn_f64(const double *__restrict__ X,
         }
 #pragma unroll
         for (int j = 0; j < kKnnQ; j++) {
-            const int qi = q0 + j;
-            const bool take = c < N && qi < N && c != qi && acc[j] < thr[j];
+            const int qi = qid[j];
+            const bool take = c < N && qi >= 0 && c != qi && acc[j] < thr[j];
             const unsigned long long m = __ballot(take);
             if (m) {
                 const int pos = cnt[j] + __popcll(m & ((1ull << lane) - 1ull));
@@ -135,9 +142,9 @@ __global__ __launch_bounds__(kWave) void k_knn_f64(const double *__restrict__ X,
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kKnnQ; j++) {
-        const int qi = q0 + j;
+        const int qi = qid[j];
         knn_compact(bk + j * B, bi + j * B, cnt[j], B);
-        if (qi < N)
+        if (qi >= 0)
             for (int m = lane; m < K; m += kWave) {
                 out_idx[(long)qi * K + m] = bi[j * B + m];
                 out_dist[(long)qi * K + m] = __dsqrt_rn(bk[j * B + m]);  // correctly rounded, as the host's sqrt

@@ -50,8 +50,10 @@ SIGNATURES = {
     "fitsne_evaluate_error_fft_noalloc": (_ci, [_u32, _u32, _dp, _dp, _ci, _ci, _cd, _cd, C.POINTER(_cd)]),
     "fitsne_distances2similarities_batch": (_ci, [_dp, _dp, _ci, _ci, _cd, _cd, _ci, _vp, _dp, _vp]),
     "fitsne_knn_exact": (_ci, [_dp, _ci, _ci, _ci, _i32, _dp]),
+    "fitsne_knn_mfma": (_ci, [_dp, _ci, _ci, _ci, _i32, _dp, C.POINTER(C.c_longlong)]),
     "fitsne_symmetrize": (_ci, [_u32, _u32, _dp, _ci, _u32, _u32, _dp, C.POINTER(C.c_longlong)]),
     "fitsne_set_graph": (_ci, [_vp, _ci]),
+    "fitsne_set_overlap": (_ci, [_vp, _ci]),
     "fitsne_autotune": (_ci, [_vp, C.POINTER(_ci)]),
     "fitsne_set_spread_points": (_ci, [_vp, _ci]),
     "fitsne_batch_create": (_ci, [C.POINTER(_vp), C.POINTER(_vp), _ci]),
@@ -233,6 +235,18 @@ def knn_exact(X, K):
     dist = np.empty((N, K))
     check(lib().fitsne_knn_exact(X.reshape(-1), N, D, K, idx.reshape(-1), dist.reshape(-1)))
     return idx, dist
+
+
+def knn_mfma(X, K):
+    """The same neighbours as knn_exact for large N (FP32 MFMA candidates, FP64 re-rank): (idx, dist,
+    number of queries recomputed by the FP64 kernel)."""
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    N, D = X.shape
+    idx = np.empty((N, K), dtype=np.int32)
+    dist = np.empty((N, K))
+    redo = C.c_longlong(0)
+    check(lib().fitsne_knn_mfma(X.reshape(-1), N, D, K, idx.reshape(-1), dist.reshape(-1), C.byref(redo)))
+    return idx, dist, redo.value
 
 
 def symmetrize(row, col, val):
@@ -417,6 +431,10 @@ class Context:
     def set_graph(self, enable):
         """Replay steps as one captured graph with device-planned geometry (FP32 form)."""
         check(lib().fitsne_set_graph(self.h, int(enable)))
+
+    def set_overlap(self, enable):
+        """Attractive term beside the grid stages on a second stream (default) or all kernels in order."""
+        check(lib().fitsne_set_overlap(self.h, int(enable)))
 
     def autotune(self):
         """Time the spread's points per workgroup on the current Y and keep the fastest; returns it."""

@@ -57,7 +57,19 @@ def cluster_affinities(X, labels, per_point, rng, chunk=1 << 20):
     return rows, cols, w.reshape(-1)
 
 
-def c2_workload(N=1_000_000, D=50, k=50, per_point=45, seed=20261204):
+def pca_start(X, dims=2):
+    """fast_tsne.py's PCA initialisation: the first `dims` principal components, scaled so that the first
+    has standard deviation 1e-4 (the eigenvectors of the covariance, FP64)."""
+    Xc = X.astype(np.float64)
+    Xc -= Xc.mean(axis=0)
+    w, v = np.linalg.eigh(Xc.T @ Xc)
+    Y = Xc @ v[:, ::-1][:, :dims]
+    Y /= np.std(Y[:, 0])
+    return Y * 0.0001
+
+
+def c2_workload(N=1_000_000, D=50, k=50, per_point=45, seed=20261204, pca=False):
+    """pca=True: also return the PCA start of the D = 50 data (a fourth value)."""
     rng = np.random.default_rng(seed)
     t0 = time.time()
     weights = rng.dirichlet(np.ones(k))
@@ -66,6 +78,7 @@ def c2_workload(N=1_000_000, D=50, k=50, per_point=45, seed=20261204):
     X = centres[labels] + rng.normal(0.0, 1.0, size=(N, D)).astype(np.float32)
     _log(f"C2 data {N}x{D} in {time.time() - t0:.1f}s")
     rows, cols, vals = cluster_affinities(X, labels, per_point, rng)
+    Y0 = pca_start(X) if pca else None
     del X
     _log(f"C2 kNN-style affinities ({rows.size} directed edges) in {time.time() - t0:.1f}s")
     row_P, col_P, val_P = symmetrize_normalize(N, rows, cols, vals)
@@ -73,6 +86,8 @@ def c2_workload(N=1_000_000, D=50, k=50, per_point=45, seed=20261204):
     c2 = rng.uniform(-70.0, 70.0, size=(k, 2))
     Y = c2[labels] + rng.normal(0.0, 3.0, size=(N, 2))
     Y -= Y.mean(axis=0)
+    if pca:
+        return (row_P, col_P, val_P), Y, labels, Y0
     return (row_P, col_P, val_P), Y, labels
 
 

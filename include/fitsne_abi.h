@@ -135,6 +135,14 @@ int fitsne_distances2similarities_batch(const double *D, double *P, int n_points
  * K <= min(N - 1, 1000); X must be finite. */
 int fitsne_knn_exact(const double *X, int N, int D, int K, int32_t *idx, double *dist);
 
+/* The same neighbours as fitsne_knn_exact, bit for bit, for large N: candidates from FP32 MFMA distances
+ * (a sampled threshold, then every point below it), their FP64 distances in the reference's order, and
+ * the FP64 kernel for any query whose candidates cannot be shown to contain its K nearest (their count,
+ * or the K-th distance against the threshold minus the FP32 error bound).  D <= 128, K <= 256;
+ * recomputed (nullable) receives the number of such queries.  Same role as fitsne_knn_exact: the exact
+ * kNN of TSNE::run (tsne.cpp:1643-1726, vptree.h). */
+int fitsne_knn_mfma(const double *X, int N, int D, int K, int32_t *idx, double *dist, long long *recomputed);
+
 /* symmetrizeMatrix (tsne.cpp:1730-1828) on the GPU: the same output rows, entry order and values
  * ((P[n][c] + P[c][n]) / 2, or P[n][c] / 2 for an entry without partner).  Input: a CSR with N rows,
  * row_P[N] < 2^30 entries, no repeated (row, col) entry.  sym_col_P / sym_val_P must hold 2 * row_P[N]
@@ -253,6 +261,11 @@ int fitsne_step(fitsne_ctx *ctx, const fitsne_step_params *prm, double *sum_Q);
  * fitsne_get_Y, fitsne_kl, fitsne_gradient, fitsne_last_geometry, or fitsne_step with sum_Q) return
  * FITSNE_E_UNSUPPORTED.  Re-captured after set_P / set_Y / reorder. */
 int fitsne_set_graph(fitsne_ctx *ctx, int enable);
+/* Host-planned FP32 steps run the attractive term on a second stream beside the grid stages (enable =
+ * 1, the default); 0 runs every kernel of the step on the context's stream, one after another (so that
+ * per-kernel timings are those of each kernel alone).  Results are the same either way.  No counterpart
+ * in the reference (a tuning knob of this build). */
+int fitsne_set_overlap(fitsne_ctx *ctx, int enable);
 
 /* Autotuner (no reference counterpart: the reference's spread is a serial loop).  Times the FP32
  * spread's points per workgroup (256, 512, 1024, 2048) on the context's current Y and geometry and
