@@ -442,6 +442,117 @@ def bench_trajectory(args, fg, synth):
     print(json.dumps(out), flush=True)
 
 
+def normalize_input(X):
+    """TSNE::run's input normalisation (tsne.cpp:153-161): zero mean, then divide by max |x|."""
+    X = X - X.mean(axis=0)
+    X /= np.abs(X).max()
+    return X
+
+
+def bench_c5(args, fg, synth):
+    """C5, the learned warm start at scale: 2M (distance, beta) training pairs from two separately seeded
+    1M-point mixtures (exact kNN on MFMA, K = 90, the reference's FP64 bisection at perplexity 30 on the
+    kNN distances as its list overload passes them, i.e. taken as squared: tsne.cpp:1479), the
+    BF16 MFMA network trained on them, then on a 4M-point, D = 100, 200-cluster mixture: exact kNN, the
+    bisection cold and warm-started (iterations, |H - log perplexity|), the symmetric P, and the full 2D
+    FP32 embedding (1000 iterations, early exaggeration 12 to 250, PCA start)."""
+    N = args.n if args.n != 1_000_000 else 4_000_000
+    K, perp = 90, 30.0
+    logp = np.log(perp)
+
+    def entropy_err(P):
+        with np.errstate(divide="ignore", invalid="ignore"):
+            H = -np.sum(np.where(P > 0, P * np.log(P), 0.0), axis=1)
+        return float(np.abs(H - logp).max())
+
+    dists, betas, t_knn_train = [], [], 0.0
+    for seed in (101, 102):
+        X, _ = synth.c5_mixture(N=N // 4, D=100, k=200, seed=seed)
+        X = normalize_input(X)
+        t0 = time.perf_counter()
+        _, d, redo = fg.knn_mfma(X, K)
+        t_knn_train += time.perf_counter() - t0
+        _, b, _ = fg.distances2similarities(d, perp, squared=True)
+        dists.append(d)
+        betas.append(b)
+        log(f"C5 training mixture seed {seed}: {X.shape[0]} points, kNN recomputed {redo}")
+        del X
+    dtr, btr = np.concatenate(dists), np.concatenate(betas)
+    del dists, betas
+    model = fg.BetaModel(K=K, seed=11)
+    t0 = time.perf_counter()
+    loss = model.fit(dtr, btr, epochs=4, batch=1024, lr=2e-3)
+    t_fit = time.perf_counter() - t0
+    log(f"C5 network trained on {dtr.shape[0]} pairs in {t_fit:.1f} s, loss {loss:.4f}")
+    del dtr, btr
+    log(f"C5 main mixture: N={N}, D=100, 200 clusters")
+    X, _ = synth.c5_mixture(N=N, D=100, k=200, seed=100)
+    X = normalize_input(X)
+    Y0 = synth.pca_start(X)
+    t0 = time.perf_counter()
+    idx, dist, redo = fg.knn_mfma(X, K)
+    t_knn = time.perf_counter() - t0
+    log(f"C5 kNN in {t_knn:.1f} s, {redo} queries recomputed by the FP64 kernel")
+    del X
+    t0 = time.perf_counter()
+    P_cold, b_cold, it_cold = fg.distances2similarities(dist, perp, squared=True)
+    t_cold = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    beta0 = model.predict(dist)
+    t_pred = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    P_warm, b_warm, it_warm = fg.distances2similarities(dist, perp, squared=True, beta0=beta0)
+    t_warm = time.perf_counter() - t0
+    model.close()
+    e_cold, e_warm = entropy_err(P_cold), entropy_err(P_warm)
+    log(f"C5 bisection iterations cold {it_cold.mean():.3f} warm {it_warm.mean():.3f}; |H - log perp| "
+        f"cold {e_cold:.2e} warm {e_warm:.2e}")
+    # symmetric, normalised P from the warm-started rows (tsne.cpp:1730-1828, :321-324)
+    row = (np.arange(N + 1, dtype=np.int64) * K).astype(np.uint32)
+    sr, sc, sv = fg.symmetrize(row, idx.reshape(-1).astype(np.uint32), P_warm.reshape(-1))
+    sv /= sv.sum()
+    del P_cold, P_warm, idx, dist
+    nnz = int(sr[-1])
+    log(f"C5 symmetric P: {nnz} entries ({nnz / N:.1f} per row)")
+    c = fg.Context(no_dims=2, precision=fg.FP32)
+    c.set_P(sr.astype(np.int64), sc.astype(np.int32), sv)
+    c.set_exaggeration(12.0)
+    c.set_Y(Y0)
+    c.autotune()
+    c.N = N
+    lr = max(200.0, N / 12.0)
+    c.sync()
+    t0 = time.perf_counter()
+    for it in range(1000):
+        c.step(0.5 if it < 250 else 0.8, lr, 5.0)
+        if it == 249:
+            c.set_exaggeration(1.0)
+    c.sync()
+    t_emb = time.perf_counter() - t0
+    kl = c.kl()
+    geo = c.last_geometry()
+    c.close()
+    log(f"C5 embedding: 1000 iterations in {t_emb:.1f} s, KL {kl:.4f}, {geo['n_boxes']} boxes per dimension")
+    out = {
+        "metric": "C5 learned warm start: bisection iterations saved at N=4M, D=100, perplexity 30, K=90",
+        "value": float(1.0 - it_warm.mean() / it_cold.mean()), "unit": "fraction of bisection iterations saved",
+        "n_gpus": 1, "steps": 1000, "warmup": 0, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16 network (fp32 accumulate), fp64 bisection, fp32 embedding",
+        "data": "synthetic (seeded Gaussian mixtures, generated in place)",
+        "config": {"workload": f"C5: N={N}, D=100, 200 clusters with sd log-uniform in [0.3, 3], perplexity 30, "
+                               f"K=90; network trained on {N // 2} pairs from two separately seeded mixtures"},
+        "bisection": {"iters_cold_mean": float(it_cold.mean()), "iters_warm_mean": float(it_warm.mean()),
+                      "max_abs_H_minus_log_perp_cold": e_cold, "max_abs_H_minus_log_perp_warm": e_warm,
+                      "max_rel_beta_diff": float(np.max(np.abs(b_warm / b_cold - 1.0))),
+                      "s_cold": t_cold, "s_warm": t_warm, "s_predict": t_pred},
+        "knn": {"s": t_knn, "recomputed_by_fp64": int(redo), "s_training_sets": t_knn_train},
+        "network": {"train_pairs": N // 2, "fit_s": t_fit, "final_loss": loss},
+        "embedding": {"iterations": 1000, "s": t_emb, "ms_per_step": t_emb, "kl": kl, "nnz": nnz,
+                      "n_boxes_end": geo["n_boxes"]},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def bench_bh(args, fg, synth):
     """Barnes-Hut (nbody_algo = 1, theta = 0.5) at N = 100,000 in 2D: each step builds the reference's
     SPTree on the host in insertion order, then walks it and sums the edge forces on the GPU (bitwise the
@@ -497,7 +608,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-c4", action="store_true", help="leave the batched C4 figure out of the default line")
     ap.add_argument("--profile-steps", type=int, default=5)
-    ap.add_argument("--config", default="c2", choices=["c2", "c3", "c4", "exact", "bh", "trajectory"],
+    ap.add_argument("--config", default="c2", choices=["c2", "c3", "c4", "c5", "exact", "bh", "trajectory"],
                     help="c2 (default): the headline step at N=1M 2D; c3: 1D df=0.5 at N=2M; c4: 64 batched "
                          "embeddings; exact: theta = 0 at N=46000 2D; bh: Barnes-Hut at N=100000 2D")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -522,6 +633,8 @@ def main():
         return bench_bh(args, fg, synth)
     if args.config == "trajectory":
         return bench_trajectory(args, fg, synth)
+    if args.config == "c5":
+        return bench_c5(args, fg, synth)
 
     if args.copy_calibration:
         import torch

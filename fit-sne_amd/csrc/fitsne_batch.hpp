@@ -35,12 +35,16 @@ struct BatchProb {
     double *zpart, *zbuf, *part, *mean, *mmpart;
 };
 
-// FFT passes fed from per-stage work counters.
+// FFT passes fed from per-stage work counters.  Each pass runs as two launches: one sized (LDS, and so
+// occupancy) for FFT lengths up to kBatchSmallMp, which most problems of a batch have, and one for the
+// rest, up to the build's limit; a problem's lines go to the class of its FFT length.
 enum { kBsKernRows = 0, kBsKernCols, kBsRowsFwd, kBsCols, kBsRowsInv, kBsStages };
+constexpr int kBsClasses = 2;
+constexpr int kBatchSmallMp = 1024;
 
 struct BatchSched {
-    int ctr[kBsStages];    // next item of each pass (cleared by k_batch_sched every step)
-    int total[kBsStages];  // items of each pass this step
+    int ctr[kBsStages * kBsClasses];    // next item of each pass and class (cleared by kb_sched every step)
+    int total[kBsStages * kBsClasses];  // items of each pass and class this step
 };
 
 // The problem of workgroup / item x: the largest p with pref[p] <= x (pref non-decreasing, x < pref[n]).
@@ -56,58 +60,77 @@ __device__ __forceinline__ int batch_find(const int *__restrict__ pref, int npro
 
 // Geometry (and twiddles when the FFT length changed) of every problem: workgroup p of 1024 threads.
 __global__ __launch_bounds__(1024) void kb_geometry(const BatchProb *__restrict__ bp, int D, int p, double ipi, int mni,
-                                                    int n1d_max, int mp_max) {
+                                                    int n1d_max, int mp_max, double lb) {
     const BatchProb &b = bp[blockIdx.x];
-    geometry_block<float>(b.mmpart, b.cblk, b.Y, b.first, b.inv, (long)b.N * D, D, p, ipi, mni, n1d_max, mp_max,
+    geometry_block<float>(b.mmpart, b.cblk, b.Y, b.first, b.inv, (long)b.N * D, D, p, ipi, mni, n1d_max, mp_max, lb,
                           b.dg, b.tw);
 }
 
-// Line counts of the FFT passes per problem and their exclusive prefixes (pref[s * (nprob + 1) + p]),
-// the totals, and cleared counters.  One workgroup of 1024 threads; problems in chunks of 1024.
+// Line counts of the FFT passes per problem and class, their exclusive prefixes
+// (pref[s * (nprob + 1) + p], s = pass + kBsStages * class), the totals, and cleared counters.  One
+// workgroup of 1024 threads; problems in chunks of 1024; scans by wavefront shuffles, then over the
+// wavefront totals.
 __global__ __launch_bounds__(1024) void kb_sched(const BatchProb *__restrict__ bp, int nprob,
                                                  BatchSched *__restrict__ sch, int *__restrict__ pref) {
-    __shared__ int scan[kBsStages][1024];
-    __shared__ int carry[kBsStages];
-    const int t = threadIdx.x;
-    if (t < kBsStages) carry[t] = 0;
+    constexpr int NS = kBsStages * kBsClasses;
+    constexpr int NW = 1024 / kWave;
+    __shared__ int wt[NS][NW];
+    __shared__ int carry[NS];
+    const int t = threadIdx.x, lane = t % kWave, w = t / kWave;
+    if (t < NS) carry[t] = 0;
     for (int p0 = 0; p0 < nprob; p0 += 1024) {
         const int p = p0 + t;
-        int cnt[kBsStages] = {0, 0, 0, 0, 0};
+        int cnt[NS];
+#pragma unroll
+        for (int s = 0; s < NS; s++) cnt[s] = 0;
         if (p < nprob) {
             const DevGeo *dg = bp[p].dg;
             if (dg->live) {
                 const int n1d = dg->g.n1d, H = dg->g.H;
+                const int o = dg->g.Mp <= kBatchSmallMp ? 0 : kBsStages;
                 if (dg->kspec_stale) {
-                    cnt[kBsKernRows] = n1d;
-                    cnt[kBsKernCols] = H;
+                    cnt[o + kBsKernRows] = n1d;
+                    cnt[o + kBsKernCols] = H;
                 }
-                cnt[kBsRowsFwd] = n1d;
-                cnt[kBsCols] = H;
-                cnt[kBsRowsInv] = n1d;
+                cnt[o + kBsRowsFwd] = n1d;
+                cnt[o + kBsCols] = H;
+                cnt[o + kBsRowsInv] = n1d;
             }
         }
-        __syncthreads();  // the previous chunk's carry has been read
+        int inc[NS];
 #pragma unroll
-        for (int s = 0; s < kBsStages; s++) scan[s][t] = cnt[s];
-        __syncthreads();
-        for (int o = 1; o < 1024; o <<= 1) {  // inclusive Hillis-Steele scan
-            int v[kBsStages];
+        for (int s = 0; s < NS; s++) {
+            int v = cnt[s];
 #pragma unroll
-            for (int s = 0; s < kBsStages; s++) v[s] = t >= o ? scan[s][t - o] : 0;
-            __syncthreads();
-#pragma unroll
-            for (int s = 0; s < kBsStages; s++) scan[s][t] += v[s];
-            __syncthreads();
+            for (int o = 1; o < kWave; o <<= 1) {
+                const int y = __shfl_up(v, o, kWave);
+                if (lane >= o) v += y;
+            }
+            inc[s] = v;
         }
+        __syncthreads();  // the previous chunk's wavefront totals and carry have been read
+        if (lane == kWave - 1) {
+#pragma unroll
+            for (int s = 0; s < NS; s++) wt[s][w] = inc[s];
+        }
+        __syncthreads();
         if (p < nprob) {
 #pragma unroll
-            for (int s = 0; s < kBsStages; s++) pref[s * (nprob + 1) + p] = carry[s] + scan[s][t] - cnt[s];
+            for (int s = 0; s < NS; s++) {
+                int before = carry[s];
+                for (int u = 0; u < w; u++) before += wt[s][u];
+                pref[s * (nprob + 1) + p] = before + inc[s] - cnt[s];
+            }
         }
         __syncthreads();
-        if (t < kBsStages) carry[t] += scan[t][1023];
+        if (t < NS) {
+            int tot = 0;
+            for (int u = 0; u < NW; u++) tot += wt[t][u];
+            carry[t] += tot;
+        }
     }
     __syncthreads();
-    if (t < kBsStages) {
+    if (t < NS) {
         pref[t * (nprob + 1) + nprob] = carry[t];
         sch->total[t] = carry[t];
         sch->ctr[t] = 0;
@@ -135,14 +158,14 @@ __device__ __forceinline__ void batch_items(BatchSched *__restrict__ sch, const 
     }
 }
 
-template <int NTHR, int MAXTOT>
+template <int NTHR, int MAXTOT, int CLS>
 __global__ __launch_bounds__(NTHR) void kb_kern_rows(const BatchProb *__restrict__ bp, int nprob,
                                                      BatchSched *__restrict__ sch, const int *__restrict__ pref,
                                                      int kind_q, int kind_1, double df) {
     extern __shared__ unsigned char smem_raw[];
     cx<float> *buf = reinterpret_cast<cx<float> *>(smem_raw);
     const cx<float> *twl = nullptr;
-    batch_items<kBsKernRows>(sch, pref, nprob, [&](int p, int a, bool fresh) {
+    batch_items<kBsKernRows + kBsStages * CLS>(sch, pref, nprob, [&](int p, int a, bool fresh) {
         const BatchProb &b = bp[p];
         const int Mp = b.dg->g.Mp;
         if (fresh) twl = stage_twiddles(buf + lspan<true>(Mp), b.tw, Mp);
@@ -150,13 +173,13 @@ __global__ __launch_bounds__(NTHR) void kb_kern_rows(const BatchProb *__restrict
     });
 }
 
-template <int NTHR, int MAXTOT>
+template <int NTHR, int MAXTOT, int CLS>
 __global__ __launch_bounds__(NTHR) void kb_kern_cols(const BatchProb *__restrict__ bp, int nprob,
                                                      BatchSched *__restrict__ sch, const int *__restrict__ pref) {
     extern __shared__ unsigned char smem_raw[];
     cx<float> *buf = reinterpret_cast<cx<float> *>(smem_raw);
     const cx<float> *twl = nullptr;
-    batch_items<kBsKernCols>(sch, pref, nprob, [&](int p, int k, bool fresh) {
+    batch_items<kBsKernCols + kBsStages * CLS>(sch, pref, nprob, [&](int p, int k, bool fresh) {
         const BatchProb &b = bp[p];
         const int Mp = b.dg->g.Mp;
         if (fresh) twl = stage_twiddles(buf + lspan<true>(Mp), b.tw, Mp);
@@ -164,13 +187,13 @@ __global__ __launch_bounds__(NTHR) void kb_kern_cols(const BatchProb *__restrict
     });
 }
 
-template <int NTHR, int MAXTOT>
+template <int NTHR, int MAXTOT, int CLS>
 __global__ __launch_bounds__(NTHR) void kb_rows_fwd(const BatchProb *__restrict__ bp, int nprob,
                                                     BatchSched *__restrict__ sch, const int *__restrict__ pref) {
     extern __shared__ unsigned char smem_raw[];
     cx<float> *buf = reinterpret_cast<cx<float> *>(smem_raw);
     const cx<float> *twl = nullptr;
-    batch_items<kBsRowsFwd>(sch, pref, nprob, [&](int p, int u, bool fresh) {
+    batch_items<kBsRowsFwd + kBsStages * CLS>(sch, pref, nprob, [&](int p, int u, bool fresh) {
         const BatchProb &b = bp[p];
         const int Mp = b.dg->g.Mp;
         if (fresh) twl = stage_twiddles(buf + lspan<true>(2 * Mp), b.tw, Mp);
@@ -178,14 +201,14 @@ __global__ __launch_bounds__(NTHR) void kb_rows_fwd(const BatchProb *__restrict_
     });
 }
 
-template <int NTHR, int MAXTOT>
+template <int NTHR, int MAXTOT, int CLS>
 __global__ __launch_bounds__(NTHR) void kb_cols(const BatchProb *__restrict__ bp, int nprob,
                                                 BatchSched *__restrict__ sch, const int *__restrict__ pref) {
     extern __shared__ unsigned char smem_raw[];
     cx<float> *buf = reinterpret_cast<cx<float> *>(smem_raw);
     __shared__ double shz[NTHR / kWave];
     const cx<float> *twl = nullptr;
-    batch_items<kBsCols>(sch, pref, nprob, [&](int p, int k, bool fresh) {
+    batch_items<kBsCols + kBsStages * CLS>(sch, pref, nprob, [&](int p, int k, bool fresh) {
         const BatchProb &b = bp[p];
         const int Mp = b.dg->g.Mp;
         if (fresh) twl = stage_twiddles(buf + lspan<true>(4 * Mp), b.tw, Mp);
@@ -193,13 +216,13 @@ __global__ __launch_bounds__(NTHR) void kb_cols(const BatchProb *__restrict__ bp
     });
 }
 
-template <int NTHR, int MAXTOT>
+template <int NTHR, int MAXTOT, int CLS>
 __global__ __launch_bounds__(NTHR) void kb_rows_inv(const BatchProb *__restrict__ bp, int nprob,
                                                     BatchSched *__restrict__ sch, const int *__restrict__ pref) {
     extern __shared__ unsigned char smem_raw[];
     cx<float> *buf = reinterpret_cast<cx<float> *>(smem_raw);
     const cx<float> *twl = nullptr;
-    batch_items<kBsRowsInv>(sch, pref, nprob, [&](int p, int u, bool fresh) {
+    batch_items<kBsRowsInv + kBsStages * CLS>(sch, pref, nprob, [&](int p, int u, bool fresh) {
         const BatchProb &b = bp[p];
         const int Mp = b.dg->g.Mp;
         if (fresh) twl = stage_twiddles(buf + lspan<true>(2 * Mp), b.tw, Mp);

@@ -175,18 +175,28 @@ __host__ __device__ inline double lagrange_absmax(const Geo &g, double t) {
     return m;
 }
 
+// Largest in-box Lagrange weight, max over j and t in [0, 1] of |L_j(t)| (sampled at 257 points, with a
+// 10% margin that covers the variation of these low-degree polynomials between samples).  It depends
+// only on p: computed once per context on the host and passed to the device geometry.
+inline double lagrange_inbox_bound(int p) {
+    Geo g{};
+    g.p = p;
+    fill_spacings(g, p);
+    double lb = 0;
+    for (int s = 0; s <= 256; s++) lb = fmax(lb, lagrange_absmax(g, s / 256.0));
+    return 1.1 * lb;
+}
+
 // Fixed-point scale of the FP32 spread's int64 grid: each run of points' sums is rounded to an integer
 // at this scale and the integers add exactly, so no node's total may reach 2^62.  A bound on that total:
-// every point of the step in one node at the largest in-box Lagrange weight (sampled on [0, 1] at 257
-// points, with a 10% margin that covers the variation of these low-degree polynomials between samples),
+// every point of the step in one node at the largest in-box Lagrange weight lb (lagrange_inbox_bound),
 // times D for the |e|^2 moment, plus the one point the 2D min rule can leave outside the grid (the first
 // x coordinate, `gap` box widths below box 0, with its extrapolated weights).  2^38 while that bound
 // stays below 2^24; above, the largest power of two that keeps the total under 2^62.  0 (refuse the
 // step) when that is below 2^20: the outlier's weights would swamp every other point's resolution.
-__host__ __device__ inline double spread_fix_scale(const Geo &g, double n_points, double gap) {
-    double lb = 0;
-    for (int s = 0; s <= 256; s++) lb = fmax(lb, lagrange_absmax(g, s / 256.0));
-    lb *= 1.1;
+// Evaluated without contraction, so that host and device agree bit for bit.
+__host__ __device__ inline double spread_fix_scale(const Geo &g, double n_points, double gap, double lb) {
+#pragma clang fp contract(off)
     double bound = n_points * (g.D == 2 ? lb * lb * 2 : lb);
     if (g.D == 2 && gap > 0) bound += 1.1 * lagrange_absmax(g, -gap) * lb * ((1 + gap) * (1 + gap) + 1);
     if (!(bound < 16777216.0)) {          // 2^24

@@ -111,6 +111,7 @@ struct fitsne_ctx {
     hipStream_t astream = nullptr;
     int spread_pts = kSpreadPts;  // points per spread workgroup (fitsne_autotune); results do not depend on it
     bool overlap = true;          // host-planned FP32 step: attractive term on astream beside the grid stages
+    int att_rows = 0;             // rows per attractive workgroup (0: about N / 256); results do not depend on it
     hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     hipEvent_t geo_ev = nullptr, kern_ev = nullptr;  // kernel spectra on astream: after the geometry, before cols
     bool join_pending = false;
@@ -352,7 +353,7 @@ static int geometry(fitsne_ctx *c, double mn, double mx, Geo *g) {
     if (f32) {
         // the first x coordinate lies below the grid when the 2D min rule skipped it (extrapolated)
         const double gap = c->D == 2 && c->mm_first < mn ? (mn - c->mm_first) / g->bw : 0.0;
-        g->fix = spread_fix_scale(*g, (double)c->N, gap);
+        g->fix = spread_fix_scale(*g, (double)c->N, gap, lagrange_inbox_bound(c->p));
         if (g->fix == 0)
             return fail(FITSNE_E_UNSUPPORTED,
                         "the first point lies " + std::to_string(gap) + " box widths below the grid (the 2D min "
@@ -461,7 +462,8 @@ static auto pick_interp_f64(int p) -> decltype(&k_interp_f64<3, D>) {
 static int build_windows(fitsne_ctx *c) {
     const int N = c->N;
     c->win_ok = false;
-    const int R = std::min(kAttRows, 64 * (int)((((long)N + 255) / 256 + 63) / 64));
+    const int R = c->att_rows > 0 ? std::min(kAttRows, c->att_rows)
+                                  : std::min(kAttRows, 64 * (int)((((long)N + 255) / 256 + 63) / 64));
     const int NB = (N + R - 1) / R, NW = (N + kWin - 1) / kWin;
     const long nbw = (long)NB * NW;
     const long long cap = c->nnz + 3ll * nbw;  // every bucket padded to a multiple of 4 entries
@@ -469,13 +471,14 @@ static int build_windows(fitsne_ctx *c) {
     TRY(ensure(c->wcount, sizeof(int) * nbw));
     TRY(ensure(c->wpure, sizeof(int) * nbw));
     TRY(ensure(c->wstart, sizeof(int) * (nbw + 1)));
-    TRY(ensure(c->wbmax, sizeof(float) * NB));
+    TRY(ensure(c->wbmax, sizeof(float) * (NB + 1)));
     TRY(ensure(c->wkey, sizeof(uint32_t) * (size_t)cap));
     TRY(ensure(c->wval, sizeof(float) * (size_t)cap));
     const size_t lds = sizeof(int) * 2 * (size_t)NW;
     launch(c, "win_count", k_win_count, dim3(NB), dim3(kAttThreads), lds, c->rowP.as<const int64_t>(),
            c->colP.as<const uint32_t>(), c->val.as<const float>(), N, R, NW, c->wcount.as<int>(), c->wpure.as<int>(),
            c->wbmax.as<float>());
+    launch(c, "bmax_all", k_bmax_all, dim3(1), dim3(1024), 0, c->wbmax.as<float>(), NB);
     launch(c, "scan_counts", k_scan_counts, dim3(1), dim3(1024), 0, c->wcount.as<int>(), c->wstart.as<int>(), (int)nbw);
     launch(c, "win_fill", k_win_fill, dim3(NB), dim3(kAttThreads), lds, c->rowP.as<const int64_t>(),
            c->colP.as<const uint32_t>(), c->val.as<const float>(), N, R, NW, c->wstart.as<const int>(),
@@ -739,7 +742,7 @@ static int launch_step_dev(fitsne_ctx *c) {
     const int cblk = std::min(blocks_for((long)c->N * D), 1024);
     launch(c, "geometry", k_geometry<float>, dim3(1), dim3(1024), 0, c->mmpart.as<const double>(), cblk,
            c->Y.as<const float>(), first_coord(c), c->permuted ? c->inv_d.as<const int>() : (const int *)nullptr,
-           (long)c->N * D, D, c->p, c->ipi, c->mni, n1d_max, mp_max, c->dgeo.as<DevGeo>());
+           (long)c->N * D, D, c->p, c->ipi, c->mni, n1d_max, mp_max, lagrange_inbox_bound(c->p), c->dgeo.as<DevGeo>());
     launch(c, "twiddles", k_twiddles_dev<float>, dim3(8), dim3(256), 0, c->dgeo.as<const DevGeo>(),
            c->dtw.as<cx<float>>());
     TRY((launch_attractive<float, D>(c, 1.f, c->pos.as<float>(), c->dgeo.as<const DevGeo>(), true)));
@@ -806,6 +809,59 @@ static int tune_spread(fitsne_ctx *c) {
         drop_graph(c);
     }
     return FITSNE_OK;
+}
+
+// Rows per attractive workgroup, chosen by timing the windowed kernel alone (best of three) for each
+// candidate: about N / 256 (enough workgroups for the chip at any N), 1024, 2048 and 4096 rows.  The
+// windowed copy of P is rebuilt for each candidate and for the choice.  The forces do not depend on it
+// (one fixed-point scale for all rows), only the time: small row blocks re-stage the column windows more
+// often, large ones leave CUs idle at small N.
+template <int D>
+static int tune_attractive(fitsne_ctx *c) {
+    if (c->nnz <= 0) return FITSNE_OK;
+    const int cand[] = {0, 1024, 2048, 4096};
+    const DevGeo *live;
+    TRY(live_record(c, &live));
+    TRY(ensure(c->pos, sizeof(float) * (size_t)c->N * D));
+    TRY(ensure(c->dstep, sizeof(StepDev)));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPTRY(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return fail(FITSNE_E_HIP, "event creation failed");
+    }
+    float best_ms = INFINITY;
+    int best = c->att_rows;
+    int rc = FITSNE_OK;
+    for (int r : cand) {
+        c->att_rows = r;
+        c->win_ok = false;
+        rc = build_windows(c);
+        if (rc != FITSNE_OK || !c->win_ok) break;
+        float t = INFINITY;
+        for (int rep = 0; rep < 3 && rc == FITSNE_OK; rep++) {
+            if (hipEventRecord(e0, c->stream) != hipSuccess) rc = fail(FITSNE_E_HIP, "event record failed");
+            if (rc == FITSNE_OK) rc = launch_attractive<float, D>(c, (float)c->exag, c->pos.as<float>(), live);
+            if (rc == FITSNE_OK && hipEventRecord(e1, c->stream) != hipSuccess) rc = fail(FITSNE_E_HIP, "event record failed");
+            float ms = INFINITY;
+            if (rc == FITSNE_OK && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                rc = fail(FITSNE_E_HIP, "event timing failed");
+            t = std::min(t, ms);
+        }
+        if (rc != FITSNE_OK) break;
+        if (t < best_ms) {
+            best_ms = t;
+            best = r;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->att_rows = best;
+    c->win_ok = false;
+    drop_graph(c);
+    TRY(rc);
+    TRY(build_windows(c));
+    return sync_ctx(c);
 }
 
 // ============================================================================ FP64 plain pipeline
@@ -1571,8 +1627,10 @@ extern "C" int fitsne_set_reorder(fitsne_ctx *c, int enable) {
     return FITSNE_OK;
 }
 
-// threshold below which set_Y keeps the caller's order (everything fits in cache anyway)
-constexpr int kReorderMinN = 65536;
+// threshold below which set_Y keeps the caller's order.  Small embeddings are sorted too: the spread's
+// runs of same-box points (and so its atomics) depend on the order whatever the size (measured: 64
+// unsorted problems of 2,000 to 65,000 points made the batched spread five times slower per point).
+constexpr int kReorderMinN = 2048;
 
 extern "C" int fitsne_set_Y(fitsne_ctx *c, const double *Y) {
     if (!c || !Y) return fail(FITSNE_E_ARG, "null argument");
@@ -1748,7 +1806,29 @@ extern "C" int fitsne_autotune(fitsne_ctx *c, int *spread_points) {
     TRY(check_graph(c));
     TRY(sync_ctx(c));
     TRY(c->D == 2 ? tune_spread<2>(c) : tune_spread<1>(c));
+    TRY(c->D == 2 ? tune_attractive<2>(c) : tune_attractive<1>(c));
     if (spread_points) *spread_points = c->spread_pts;
+    return FITSNE_OK;
+}
+
+extern "C" int fitsne_autotune_stats(fitsne_ctx *c, int *spread_points, int *attractive_rows) {
+    if (!c) return fail(FITSNE_E_ARG, "null argument");
+    if (spread_points) *spread_points = c->spread_pts;
+    if (attractive_rows) *attractive_rows = c->win_ok ? c->win_R : c->att_rows;
+    return FITSNE_OK;
+}
+
+extern "C" int fitsne_set_attractive_rows(fitsne_ctx *c, int rows) {
+    if (!c) return fail(FITSNE_E_ARG, "null argument");
+    if (rows != 0 && (rows < kWave || rows > kAttRows || rows % kWave != 0))
+        return fail(FITSNE_E_ARG, "attractive rows per workgroup must be 0 (automatic) or a multiple of 64 in [64, 4096]");
+    HIPTRY(hipSetDevice(c->device));
+    TRY(sync_ctx(c));
+    if (rows != c->att_rows) {
+        c->att_rows = rows;
+        c->win_ok = false;
+        drop_graph(c);
+    }
     return FITSNE_OK;
 }
 
@@ -1829,7 +1909,7 @@ struct fitsne_batch {
     int slot = 0;
     DBuf dprob, dsteps, dsched, dpref, dspre;
     int tot_attr = 0, tot_spread = 0, tot_interp = 0, tot_center = 0;
-    int g_kern = 0, g_rows = 0, g_cols = 0;
+    int g_kern[kBsClasses] = {}, g_rows[kBsClasses] = {}, g_cols[kBsClasses] = {};
     std::vector<long> gens;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -1866,32 +1946,78 @@ static int persistent_blocks(K kernel, int threads, size_t lds, int cus, int *ou
     return FITSNE_OK;
 }
 
+// LDS of the batched FFT passes for FFT lengths up to Mp.
+struct BatchLds {
+    size_t kern, rows, cols;
+};
+static BatchLds batch_lds(int Mp) {
+    return {sizeof(cx<float>) * (lspan<true>(Mp) + (size_t)Mp), sizeof(cx<float>) * (lspan<true>(2 * Mp) + (size_t)Mp),
+            sizeof(cx<float>) * (lspan<true>(4 * Mp) + (size_t)Mp)};
+}
+constexpr int kBatchClassMp[kBsClasses] = {kBatchSmallMp, kCapMp2};
+
+// The five FFT passes of one length class (CLS), each one persistent launch.
+template <int CLS, int MP>
+static void batch_fft_class(fitsne_batch *b, hipStream_t st, const BatchProb *bp, BatchSched *sch, int *pref,
+                            int pass, int kind_q, int kind_1) {
+    const int n = (int)b->ctx.size();
+    const BatchLds L = batch_lds(MP);
+    switch (pass) {
+    case kBsKernRows:
+        kb_kern_rows<kKernThreads, MP, CLS><<<dim3(b->g_kern[CLS]), dim3(kKernThreads), L.kern, st>>>(bp, n, sch, pref,
+                                                                                                 kind_q, kind_1, b->df);
+        break;
+    case kBsKernCols:
+        kb_kern_cols<kKernThreads, MP, CLS><<<dim3(b->g_kern[CLS]), dim3(kKernThreads), L.kern, st>>>(bp, n, sch, pref);
+        break;
+    case kBsRowsFwd:
+        kb_rows_fwd<kRowThreads, 2 * MP, CLS><<<dim3(b->g_rows[CLS]), dim3(kRowThreads), L.rows, st>>>(bp, n, sch, pref);
+        break;
+    case kBsCols:
+        kb_cols<kColThreads, 4 * MP, CLS><<<dim3(b->g_cols[CLS]), dim3(kColThreads), L.cols, st>>>(bp, n, sch, pref);
+        break;
+    default:
+        kb_rows_inv<kRowThreads, 2 * MP, CLS><<<dim3(b->g_rows[CLS]), dim3(kRowThreads), L.rows, st>>>(bp, n, sch, pref);
+        break;
+    }
+}
+
+template <int CLS, int MP>
+static int batch_class_blocks(fitsne_batch *b, int cus) {
+    const BatchLds L = batch_lds(MP);
+    TRY(persistent_blocks(kb_kern_rows<kKernThreads, MP, CLS>, kKernThreads, L.kern, cus, &b->g_kern[CLS]));
+    TRY(persistent_blocks(kb_rows_fwd<kRowThreads, 2 * MP, CLS>, kRowThreads, L.rows, cus, &b->g_rows[CLS]));
+    TRY(persistent_blocks(kb_cols<kColThreads, 4 * MP, CLS>, kColThreads, L.cols, cus, &b->g_cols[CLS]));
+    return FITSNE_OK;
+}
+
 template <int P>
 static int batch_capture_p(fitsne_batch *b, const BatchProb *bp, BatchSched *sch, int *pref, const int *spre) {
     const int n = (int)b->ctx.size();
     const bool df1 = b->df == 1.0;
     const int kind_q = df1 ? FITSNE_KERNEL_SQUARED_CAUCHY : FITSNE_KERNEL_SQUARED_GENERAL;
     const int kind_1 = FITSNE_KERNEL_GENERAL;
-    const size_t Mp = (size_t)kCapMp2;
-    const size_t lds_k = sizeof(cx<float>) * (lspan<true>((int)Mp) + Mp);
-    const size_t lds_r = sizeof(cx<float>) * (lspan<true>(2 * (int)Mp) + Mp);
-    const size_t lds_c = sizeof(cx<float>) * (lspan<true>(4 * (int)Mp) + Mp);
     const int *apre = spre, *sppre = spre + (n + 1), *ipre = spre + 2 * (n + 1), *cpre = spre + 3 * (n + 1);
     hipStream_t st = b->stream;
-    kb_geometry<<<dim3(n), dim3(1024), 0, st>>>(bp, 2, P, b->ipi, b->mni, kCapN1d2, kCapMp2);
+    kb_geometry<<<dim3(n), dim3(1024), 0, st>>>(bp, 2, P, b->ipi, b->mni, kCapN1d2, kCapMp2, lagrange_inbox_bound(P));
+    // the schedule first: a workgroup launched behind the attractive term waits for one of its
+    // workgroups (160 KB of LDS each) to leave a CU
+    kb_sched<<<dim3(1), dim3(1024), 0, st>>>(bp, n, sch, pref);
     HIPTRY(hipEventRecord(b->fork_ev, st));
     HIPTRY(hipStreamWaitEvent(b->astream, b->fork_ev, 0));
     kb_attr<<<dim3(b->tot_attr), dim3(kAttThreads), 0, b->astream>>>(bp, n, apre, (float)(1.0 / b->df), (int)df1,
                                                                       (float)(0.5 * std::sqrt(b->df)));
     HIPTRY(hipEventRecord(b->join_ev, b->astream));
-    kb_sched<<<dim3(1), dim3(1024), 0, st>>>(bp, n, sch, pref);
-    kb_kern_rows<kKernThreads, kKernTot><<<dim3(b->g_kern), dim3(kKernThreads), lds_k, st>>>(bp, n, sch, pref, kind_q,
-                                                                                              kind_1, b->df);
-    kb_kern_cols<kKernThreads, kKernTot><<<dim3(b->g_kern), dim3(kKernThreads), lds_k, st>>>(bp, n, sch, pref);
+    auto passes = [&](int pass) {
+        batch_fft_class<0, kBatchSmallMp>(b, st, bp, sch, pref, pass, kind_q, kind_1);
+        batch_fft_class<1, kCapMp2>(b, st, bp, sch, pref, pass, kind_q, kind_1);
+    };
+    passes(kBsKernRows);
+    passes(kBsKernCols);
     kb_spread<P><<<dim3(b->tot_spread), dim3(kSpreadThreads), 0, st>>>(bp, n, sppre);
-    kb_rows_fwd<kRowThreads, kRowTot><<<dim3(b->g_rows), dim3(kRowThreads), lds_r, st>>>(bp, n, sch, pref);
-    kb_cols<kColThreads, kColTot32><<<dim3(b->g_cols), dim3(kColThreads), lds_c, st>>>(bp, n, sch, pref);
-    kb_rows_inv<kRowThreads, kRowTot><<<dim3(b->g_rows), dim3(kRowThreads), lds_r, st>>>(bp, n, sch, pref);
+    passes(kBsRowsFwd);
+    passes(kBsCols);
+    passes(kBsRowsInv);
     kb_zfinal<<<dim3(n), dim3(kThreads), 0, st>>>(bp, (int)df1);
     HIPTRY(hipStreamWaitEvent(st, b->join_ev, 0));
     kb_interp<P><<<dim3(b->tot_interp), dim3(kThreads), 0, st>>>(bp, n, ipre);
@@ -1909,6 +2035,13 @@ static int batch_prepare(fitsne_batch *b) {
     HIPTRY(hipStreamSynchronize(b->stream));
     for (fitsne_ctx *c : b->ctx) {
         if (c->N <= 0 || !c->Y.p) return fail(FITSNE_E_ARG, "P and Y must be set before a batched step");
+        // the largest row blocks: the batch fills the chip with problems, so each workgroup should
+        // re-stage the column windows as rarely as possible (the forces do not depend on it)
+        if (c->att_rows != kAttRows) {
+            c->att_rows = kAttRows;
+            c->win_ok = false;
+            drop_graph(c);
+        }
         TRY(prepare_captured(c));
     }
     TRY(ensure(b->dsteps, sizeof(StepDev) * n));
@@ -1969,19 +2102,14 @@ static int batch_prepare(fitsne_batch *b) {
     TRY(ensure(b->dprob, sizeof(BatchProb) * n));
     TRY(ensure(b->dspre, sizeof(int) * spre.size()));
     TRY(ensure(b->dsched, sizeof(BatchSched)));
-    TRY(ensure(b->dpref, sizeof(int) * kBsStages * (size_t)(n + 1)));
+    TRY(ensure(b->dpref, sizeof(int) * kBsStages * kBsClasses * (size_t)(n + 1)));
     HIPTRY(hipMemcpyAsync(b->dprob.p, bp.data(), sizeof(BatchProb) * n, hipMemcpyHostToDevice, b->stream));
     HIPTRY(hipMemcpyAsync(b->dspre.p, spre.data(), sizeof(int) * spre.size(), hipMemcpyHostToDevice, b->stream));
     HIPTRY(hipStreamSynchronize(b->stream));  // the host vectors go out of scope
     int cus = 0;
     HIPTRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device));
-    const size_t Mp = (size_t)kCapMp2;
-    TRY(persistent_blocks(kb_kern_rows<kKernThreads, kKernTot>, kKernThreads,
-                          sizeof(cx<float>) * (lspan<true>((int)Mp) + Mp), cus, &b->g_kern));
-    TRY(persistent_blocks(kb_rows_fwd<kRowThreads, kRowTot>, kRowThreads,
-                          sizeof(cx<float>) * (lspan<true>(2 * (int)Mp) + Mp), cus, &b->g_rows));
-    TRY(persistent_blocks(kb_cols<kColThreads, kColTot32>, kColThreads,
-                          sizeof(cx<float>) * (lspan<true>(4 * (int)Mp) + Mp), cus, &b->g_cols));
+    TRY((batch_class_blocks<0, kBatchSmallMp>(b, cus)));
+    TRY((batch_class_blocks<1, kCapMp2>(b, cus)));
     const BatchProb *dbp = b->dprob.as<const BatchProb>();
     auto *sch = b->dsched.as<BatchSched>();
     int *pref = b->dpref.as<int>();
@@ -2047,8 +2175,16 @@ extern "C" int fitsne_batch_create(fitsne_batch **out, fitsne_ctx **ctxs, int n)
         for (int j = 0; j < i; j++)
             if (ctxs[j] == ctxs[i]) return fail(FITSNE_E_ARG, "a context appears twice in the batch");
     }
-    // every problem gets its own spread workgroup size (results do not depend on it)
-    for (int i = 0; i < n; i++) TRY(fitsne_autotune(ctxs[i], nullptr));
+    // every problem gets its own spread workgroup size (results do not depend on it); the attractive row
+    // blocks of a fused batch are set below
+    for (int i = 0; i < n; i++) {
+        fitsne_ctx *c = ctxs[i];
+        if (c->N <= 0 || !c->Y.p) return fail(FITSNE_E_ARG, "set P and Y before batching");
+        HIPTRY(hipSetDevice(c->device));
+        TRY(check_graph(c));
+        TRY(sync_ctx(c));
+        TRY(c->D == 2 ? tune_spread<2>(c) : tune_spread<1>(c));
+    }
     auto *b = new fitsne_batch();
     b->ctx.assign(ctxs, ctxs + n);
     const fitsne_ctx *c0 = ctxs[0];

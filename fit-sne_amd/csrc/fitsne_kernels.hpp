@@ -148,14 +148,15 @@ __global__ __launch_bounds__(kThreads) void k_attractive(const TI *__restrict__ 
 // LDS, and walks its non-empty buckets: window w of Y sits in LDS (the next one is loaded into registers
 // meanwhile), every lane takes 4 consecutive entries per 16-byte load, gathers both points from LDS and
 // adds
-//   round(p_ij q_ij (y_i - y_j) S_b)
+//   round(p_ij q_ij (y_i - y_j) S)
 // to the row's accumulator (LDS 64-bit atomics).  A pure quad reads its row once and issues one atomic
 // per coordinate; a mixed quad merges its consecutive entries of one row and issues one per run.  Most
 // entries are in pure quads, so the per-run atomics of a wave (issued whenever any lane changes row)
 // are confined to the short mixed sections.
-// S_b = 2^(50 - e), 2^e > max over the block's rows of sum_j p_ij * sqrt(df)/2, bounds every partial
-// sum by 2^50 since |q d| <= sqrt(df)/2: the integer sums are exact, so the forces do not depend on the
-// order of the entries and are bitwise repeatable.  Same formula as k_attractive (tsne.cpp:1121-1137).
+// S = 2^(50 - e), 2^e > max over all rows of sum_j p_ij * sqrt(df)/2 (bmax[NB], the maximum of the row
+// blocks' maxima), bounds every partial sum by 2^50 since |q d| <= sqrt(df)/2: the integer sums are
+// exact, so the forces depend neither on the order of the entries nor on the row-block size R, and are
+// bitwise repeatable.  Same formula as k_attractive (tsne.cpp:1121-1137).
 constexpr int kWin = 8192;
 constexpr int kAttRows = 4096;
 constexpr int kAttThreads = 1024;
@@ -238,6 +239,18 @@ __global__ __launch_bounds__(kAttThreads) void k_win_count(const int64_t *__rest
         pure[(long)b * NW + w] = hist[w];
     }
     if (t == 0) bmax[b] = __uint_as_float(smax);
+}
+
+// bmax[nb] = max of bmax[0 .. nb) (one workgroup of 1024; non-negative floats order as their bits).
+__global__ __launch_bounds__(1024) void k_bmax_all(float *__restrict__ bmax, int nb) {
+    __shared__ unsigned int m;
+    if (threadIdx.x == 0) m = 0u;
+    __syncthreads();
+    unsigned int v = 0u;
+    for (int b = threadIdx.x; b < nb; b += 1024) v = max(v, __float_as_uint(bmax[b]));
+    atomicMax(&m, v);
+    __syncthreads();
+    if (threadIdx.x == 0) bmax[nb] = __uint_as_float(m);
 }
 
 // Scatter the entries into their buckets (start[] from the exclusive scan of the counts) and write the
@@ -391,7 +404,7 @@ __device__ __forceinline__ void attr_win_block(const int *__restrict__ start, co
         yr[q] = Y[(long)r0 * D + q];
         acc[q] = 0ull;
     }
-    const double bound = (double)bmax[b] * mdf;
+    const double bound = (double)bmax[(N + R - 1) / R] * mdf;  // the maximum over all row blocks
     int e = 0;
     if (bound > 0.0) (void)frexp(bound, &e);  // bound < 2^e
     const double S = ldexp(1.0, 50 - e), invS = ldexp(1.0, e - 50);
@@ -647,7 +660,7 @@ __global__ __launch_bounds__(1024) void k_minmax_seq(const TV *__restrict__ Y, c
 template <typename TV>
 __device__ void geometry_block(const double *__restrict__ part, int nblk, const TV *__restrict__ Y, long first,
                                const int *__restrict__ pos, long n, int D, int p, double ipi, int mni, int n1d_max,
-                               int mp_max, DevGeo *__restrict__ dg, cx<float> *__restrict__ tw) {
+                               int mp_max, double lb, DevGeo *__restrict__ dg, cx<float> *__restrict__ tw) {
     __shared__ double shx[1024 / kWave], shn[1024 / kWave];
     __shared__ double res[2];
     __shared__ int twn;
@@ -692,7 +705,7 @@ __device__ void geometry_block(const double *__restrict__ part, int nblk, const 
         o.live = fin && nb >= 1 && g.n1d <= n1d_max && g.Mp <= mp_max;
         if (o.live) {  // the spread's fixed-point scale, with the first coordinate's gap below the grid
             const double y0 = (double)Y[first];
-            g.fix = spread_fix_scale(g, (double)(n / D), D == 2 && y0 < mn ? (mn - y0) / g.bw : 0.0);
+            g.fix = spread_fix_scale(g, (double)(n / D), D == 2 && y0 < mn ? (mn - y0) / g.bw : 0.0, lb);
             if (g.fix == 0) {
                 o.live = 0;
                 o.err = 2;
@@ -727,9 +740,9 @@ __device__ void geometry_block(const double *__restrict__ part, int nblk, const 
 template <typename TV>
 __global__ __launch_bounds__(1024) void k_geometry(const double *__restrict__ part, int nblk, const TV *__restrict__ Y,
                                                    long first, const int *__restrict__ pos, long n, int D, int p,
-                                                   double ipi, int mni, int n1d_max, int mp_max,
+                                                   double ipi, int mni, int n1d_max, int mp_max, double lb,
                                                    DevGeo *__restrict__ dg) {
-    geometry_block<TV>(part, nblk, Y, first, pos, n, D, p, ipi, mni, n1d_max, mp_max, dg, nullptr);
+    geometry_block<TV>(part, nblk, Y, first, pos, n, D, p, ipi, mni, n1d_max, mp_max, lb, dg, nullptr);
 }
 
 // Twiddle table of the step's FFT length, when it changed.

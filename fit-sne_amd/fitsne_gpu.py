@@ -55,6 +55,8 @@ SIGNATURES = {
     "fitsne_set_graph": (_ci, [_vp, _ci]),
     "fitsne_set_overlap": (_ci, [_vp, _ci]),
     "fitsne_autotune": (_ci, [_vp, C.POINTER(_ci)]),
+    "fitsne_autotune_stats": (_ci, [_vp, C.POINTER(_ci), C.POINTER(_ci)]),
+    "fitsne_set_attractive_rows": (_ci, [_vp, _ci]),
     "fitsne_set_spread_points": (_ci, [_vp, _ci]),
     "fitsne_batch_create": (_ci, [C.POINTER(_vp), C.POINTER(_vp), _ci]),
     "fitsne_batch_step": (_ci, [_vp, C.POINTER(StepParams)]),
@@ -441,6 +443,16 @@ class Context:
         out = C.c_int(0)
         check(lib().fitsne_autotune(self.h, C.byref(out)))
         return out.value
+
+    def tuned(self):
+        """(points per spread workgroup, rows per attractive workgroup) now in use."""
+        a, b = C.c_int(0), C.c_int(0)
+        check(lib().fitsne_autotune_stats(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def set_attractive_rows(self, rows):
+        """Rows per attractive workgroup (0: automatic); results do not depend on it."""
+        check(lib().fitsne_set_attractive_rows(self.h, int(rows)))
 
     def set_spread_points(self, points):
         """Points per spread workgroup (a multiple of 256 in [256, 8192]); results do not depend on it."""

@@ -1,4 +1,5 @@
-"""The autotuner's knob, the spread's points per workgroup, leaves every result bitwise unchanged.
+"""The autotuner's knobs, the spread's points per workgroup and the attractive term's rows per workgroup,
+leave every result bitwise unchanged.
 
 The spread's waves are the same aligned groups of 64 points for any workgroup size, and everything
 after the in-wave sums is exact integer arithmetic, so the FP32 gradient, Z and a run of optimiser
@@ -51,6 +52,30 @@ def test_spread_workgroup_size_does_not_change_results(D, N):
     assert chosen in (256, 512, 1024, 2048)
     np.testing.assert_array_equal(g, g0)
     np.testing.assert_array_equal(y, y0)
+
+
+@pytest.mark.parametrize("D,N", [(2, 30_000), (1, 20_000)])
+def test_attractive_row_block_does_not_change_results(D, N):
+    """The attractive term's rows per workgroup change only which workgroup sums which rows: with one
+    fixed-point scale for all rows the forces, and so every step, are bitwise the same."""
+    Y = blobs(N, D, 6, seed=41 + D, span=40.0)
+    P = random_csr(N, 15, seed=6)
+    out = []
+    for rows in (0, 64, 1024, 4096):
+        c = fg.Context(no_dims=D, precision=fg.FP32)
+        c.set_P(*P)
+        c.set_Y(Y)
+        c.set_attractive_rows(rows)
+        g, z = c.gradient()
+        for _ in range(3):
+            c.step(momentum=0.5, learning_rate=200.0)
+        out.append((g, z, c.get_Y(), c.tuned()[1]))
+        c.close()
+    for g, z, y, r in out[1:]:
+        np.testing.assert_array_equal(g, out[0][0])
+        assert z == out[0][1]
+        np.testing.assert_array_equal(y, out[0][2])
+    assert [o[3] for o in out[1:]] == [64, 1024, 4096]
 
 
 def test_spread_points_are_validated():

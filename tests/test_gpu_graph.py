@@ -20,9 +20,10 @@ def _note(msg):
     print(f"[graph test] {msg}", flush=True)
 
 
-def _ctx(P, Y, D, df=1.0, graph=False):
+def _ctx(P, Y, D, df=1.0, graph=False, reorder=True):
     c = fg.Context(no_dims=D, precision=fg.FP32, df=df)
     c.set_P(*P)
+    c.set_reorder(reorder)
     c.set_Y(Y)
     c.N = Y.shape[0]
     if graph:
@@ -34,12 +35,13 @@ def _schedule(k):
     return 0.5 if k < 6 else 0.8
 
 
-@pytest.mark.parametrize("N,D,df", [(5000, 2, 1.0), (70000, 2, 1.0), (20000, 1, 0.5)])
-def test_captured_steps_bitwise_equal_host_planned(N, D, df):
+@pytest.mark.parametrize("N,D,df,reorder", [(5000, 2, 1.0, False), (5000, 2, 1.0, True), (70000, 2, 1.0, True),
+                                             (20000, 1, 0.5, True), (20000, 1, 0.5, False)])
+def test_captured_steps_bitwise_equal_host_planned(N, D, df, reorder):
     P, Y, _ = synth.small_mixture_workload(N=N, D=D, span=40.0 if D == 2 else 200.0, seed=3)
-    _note(f"N={N} D={D} df={df}")
-    a = _ctx(P, Y, D, df)
-    b = _ctx(P, Y, D, df, graph=True)
+    _note(f"N={N} D={D} df={df} reorder={reorder}")
+    a = _ctx(P, Y, D, df, reorder=reorder)
+    b = _ctx(P, Y, D, df, graph=True, reorder=reorder)
     lr = max(200.0, N / 12.0)
     for k in range(12):
         za = a.step(_schedule(k), lr, 5.0, want_sumq=(k % 5 == 4))
