@@ -276,7 +276,7 @@ __global__ __launch_bounds__(kThreads) void kb_mean(const BatchProb *__restrict_
     const BatchProb &b = bp[blockIdx.x];
     if (!b.dg->live) return;
     __shared__ double sh[kThreads / kWave];
-    mean_block(b.part, (b.N + kThreads - 1) / kThreads, 2, (double)b.N, b.mean, sh);
+    mean_block(b.part, (b.N + kInterpPts - 1) / kInterpPts, 2, (double)b.N, b.mean, sh);
 }
 
 __global__ __launch_bounds__(kThreads) void kb_center(const BatchProb *__restrict__ bp, int nprob,

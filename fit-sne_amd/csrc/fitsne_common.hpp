@@ -55,6 +55,7 @@ struct Geo {
     int D;
     int p;
     double fix;      // fixed-point scale of the FP32 spread's integer grid (spread_fix_scale)
+    int fix_fast;    // every run sum of 64 points stays below 2^50 at that scale (spread_fix_fast)
 };
 
 // Stage radices of one FFT length (Stockham).
@@ -209,6 +210,16 @@ __host__ __device__ inline double spread_fix_scale(const Geo &g, double n_points
     return 274877906944.0;  // 2^38
 }
 
+// Whether the spread may round its run sums with the 2^51-range rounding alone: a wavefront's run holds at
+// most 64 points, each with every value below lb^D D (in the box) or the extrapolated first point's bound,
+// so the run sums stay below 2^50 at scale fix unless that outlier is far out.
+__host__ __device__ inline int spread_fix_fast(const Geo &g, double gap, double lb) {
+#pragma clang fp contract(off)
+    double per = g.D == 2 ? lb * lb * 2 : lb;
+    if (g.D == 2 && gap > 0) per = fmax(per, 1.1 * lagrange_absmax(g, -gap) * lb * ((1 + gap) * (1 + gap) + 1));
+    return 64.0 * per * g.fix < 1125899906842624.0;  // 2^50
+}
+
 // Workgroup-to-row map over npad = 8 ceil(n / 8) slots: consecutive workgroups go to different XCDs
 // (round-robin dispatch), so each XCD takes a contiguous range of rows and its L2 merges their
 // lines.  A bijection of [0, npad); slots mapping to rows >= n are skipped by the caller.
@@ -254,6 +265,24 @@ __device__ __forceinline__ double block_sum(double v, double *sh) {
     if (threadIdx.x == 0)
         for (int i = 0; i < (int)(blockDim.x / kWave); i++) r += sh[i];
     return r;
+}
+
+// Block-wide max and min (results valid in thread 0); sh holds 2 blockDim.x / kWave doubles.
+__device__ __forceinline__ void block_maxmin(double &mx, double &mn, double *sh) {
+    mx = wave_max(mx);
+    mn = wave_min(mn);
+    const int nw = blockDim.x / kWave, w = threadIdx.x / kWave, l = threadIdx.x % kWave;
+    __syncthreads();
+    if (l == 0) {
+        sh[w] = mx;
+        sh[nw + w] = mn;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int i = 1; i < nw; i++) {
+            mx = fmax(mx, sh[i]);
+            mn = fmin(mn, sh[nw + i]);
+        }
 }
 
 // Lagrange basis on p equispaced in-box nodes (the reference's interpolate(), nbodyfft.cpp:322-333):

@@ -132,6 +132,7 @@ struct fitsne_ctx {
     int win_R = 0, win_NW = 0;
     // grids, spectra
     DBuf gin, gout, spec, ospec, Trow, KQ, K1, KS, KT, zpart, zbuf, part, mean, mmpart, mm, fixgrid;
+    DBuf mmu;      // per block and coordinate (max, min) of the updated Y, from interp_f32 for mean_mm
     // device records of the step (geometry and plan; optimiser parameters) and the captured step
     DBuf dgeo, dstep, dtw;
     DBuf dlive;                  // a geometry record with live = 1, for launches before the step's geometry
@@ -163,7 +164,8 @@ struct fitsne_ctx {
     // twiddles per FFT length
     std::map<int, DBuf> tw32, tw64;
     // min/max handoff from the last step
-    double *h_mm = nullptr;  // pinned [4]
+    double *h_mm = nullptr;      // pinned [4]
+    double *h_mm_dev = nullptr;  // h_mm as the device addresses it
     hipEvent_t mm_event = nullptr;
     bool mm_pending = false, mm_valid = false;
     double mm_min = 0, mm_max = 0;
@@ -353,7 +355,9 @@ static int geometry(fitsne_ctx *c, double mn, double mx, Geo *g) {
     if (f32) {
         // the first x coordinate lies below the grid when the 2D min rule skipped it (extrapolated)
         const double gap = c->D == 2 && c->mm_first < mn ? (mn - c->mm_first) / g->bw : 0.0;
-        g->fix = spread_fix_scale(*g, (double)c->N, gap, lagrange_inbox_bound(c->p));
+        const double lb = lagrange_inbox_bound(c->p);
+        g->fix = spread_fix_scale(*g, (double)c->N, gap, lb);
+        g->fix_fast = spread_fix_fast(*g, gap, lb);
         if (g->fix == 0)
             return fail(FITSNE_E_UNSUPPORTED,
                         "the first point lies " + std::to_string(gap) + " box widths below the grid (the 2D min "
@@ -592,7 +596,12 @@ static int launch_f32_stages(fitsne_ctx *c, const cx<float> *tw, const StageDims
         launch(c, "rows_fwd", k_rows_fwd<float, kRowThreads, kRowTot, true>, dim3(sd.gr_rows), dim3(kRowThreads),
                sizeof(cx<float>) * (lspan<true>(2 * (int)Mp) + Mp), dg, tw, (const float *)nullptr, 4, 4, c->spec.as<cx<float>>(),
                c->fixgrid.as<unsigned long long>());
-        if (kside) HIPTRY(hipStreamWaitEvent(c->stream, c->kern_ev, 0));
+        if (kside) {
+            // kern_ev follows the attractive kernel on the side stream, so this one wait also joins it:
+            // each cross-stream wait costs about 10 us of queue latency, even on a finished event
+            HIPTRY(hipStreamWaitEvent(c->stream, c->kern_ev, 0));
+            c->join_pending = false;
+        }
         launch(c, "cols_f32", k_cols_f32<kColThreads, kColTot32>, dim3(sd.gr_cols), dim3(kColThreads),
                sizeof(cx<float>) * (lspan<true>(4 * (int)Mp) + Mp), dg, tw, c->spec.as<const cx<float>>(), c->KQ.as<const float>(),
                c->K1.as<const float>(), c->KS.as<const float>(), c->KT.as<const float>(), c->ospec.as<cx<float>>(),
@@ -610,22 +619,34 @@ static int launch_f32_stages(fitsne_ctx *c, const cx<float> *tw, const StageDims
         launch(c, "zfinal", k_zfinal, dim3(1), dim3(kThreads), 0, dg, c->zpart.as<const double>(), (int)df1, (double)N,
                c->zbuf.as<double>());
     }
-    const int nblk = blocks_for(N);
+    const int nblk = (N + kInterpPts - 1) / kInterpPts;
     if (c->join_pending) {
         HIPTRY(hipStreamWaitEvent(c->stream, c->join_ev, 0));
         c->join_pending = false;
     }
+    // host-planned updates: the interpolation also leaves the per-coordinate max / min of the new Y, and
+    // mean_mm turns them into those of the centred Y in pinned memory, so that the host plans the next
+    // geometry while the centring runs
+    const bool mm_host = side && mode == 1;
     launch(c, "interp_f32", pick_interp_f32<D>(p), dim3(nblk), dim3(kThreads), 0, c->Y.as<float>(),
            c->pos.as<const float>(), c->gout.as<const float4>(), dg, c->zbuf.as<const double>(),
            c->dstep.as<const StepDev>(), c->dC.as<float>(), c->uY.as<float>(), c->gains.as<float>(),
-           c->part.as<double>(), N);
+           c->part.as<double>(), N, mm_host ? c->mmu.as<double>() : (double *)nullptr, skip_coord(c));
     if (mode == 1) {
         const long n = (long)N * D;
         const int cblk = std::min(blocks_for(n), 1024);
-        launch(c, "mean_final", k_mean_final, dim3(1), dim3(kThreads), 0, dg, c->part.as<const double>(), nblk, D,
-               (double)N, c->mean.as<double>());
-        launch(c, "center", k_center<float>, dim3(cblk), dim3(kThreads), 0, dg, c->Y.as<float>(), n, D, skip_coord(c),
-               c->mean.as<const double>(), c->mmpart.as<double>());
+        if (mm_host) {
+            launch(c, "mean_mm", k_mean_mm<D>, dim3(1), dim3(kThreads), 0, dg, c->part.as<const double>(),
+                   c->mmu.as<const double>(), nblk, (double)N, c->Y.as<const float>(), first_coord(c),
+                   c->mean.as<double>(), c->h_mm_dev);
+            HIPTRY(hipEventRecord(c->mm_event, c->stream));
+        } else {
+            launch(c, "mean_final", k_mean_final, dim3(1), dim3(kThreads), 0, dg, c->part.as<const double>(), nblk,
+                   D, (double)N, c->mean.as<double>());
+        }
+        // the min / max partials of the centred Y, for a device-planned step's geometry
+        launch(c, "center", k_center<float>, dim3(cblk), dim3(kThreads), 0, dg, c->Y.as<float>(), n, D,
+               skip_coord(c), c->mean.as<const double>(), c->mmpart.as<double>());
     }
     return FITSNE_OK;
 }
@@ -643,6 +664,7 @@ static int ensure_f32_buffers(fitsne_ctx *c, int n1d, int Mp) {
     TRY(ensure(c->mmpart, sizeof(double) * 2 * std::min(blocks_for((long)N * D), 1024)));
     TRY(ensure(c->mm, sizeof(double) * 4));
     TRY(ensure(c->zbuf, sizeof(double) * 4));
+    TRY(ensure(c->mmu, sizeof(double) * 2 * D * (size_t)blocks_for(N)));
     TRY(ensure(c->gin, sizeof(float4) * nodes));
     TRY(ensure(c->gout, sizeof(float4) * nodes));
     if (c->fixgrid.cap < sizeof(unsigned long long) * 4 * (size_t)nodes) {
@@ -717,12 +739,7 @@ static int pipeline_f32(fitsne_ctx *c, int mode, const fitsne_step_params *sp) {
     }
     const StageDims sd{g.Mp, 8 * ((g.n1d + 7) / 8), 8 * ((g.H + 7) / 8), blocks_for(g.n1d)};
     TRY(launch_f32_stages<D>(c, tw, sd, kern, mode, true));
-    if (mode == 1) {
-        const int cblk = std::min(blocks_for((long)c->N * D), 1024);
-        launch(c, "minmax_final", k_minmax_final<float>, dim3(1), dim3(kWave), 0, c->mmpart.as<const double>(), cblk,
-               c->Y.as<const float>(), first_coord(c), c->mm.as<double>());
-        HIPTRY(hipMemcpyAsync(c->h_mm, c->mm.p, sizeof(double) * 3, hipMemcpyDeviceToHost, c->stream));
-        HIPTRY(hipEventRecord(c->mm_event, c->stream));
+    if (mode == 1) {  // k_mean_mm writes (max, min, Y[first]) of the centred Y to h_mm before mm_event
         c->mm_pending = true;
         c->mm_valid = false;
         c->mm_parts_valid = true;
@@ -1290,6 +1307,7 @@ extern "C" int fitsne_ctx_create(fitsne_ctx **out, int device, int precision, in
         hipEventCreateWithFlags(&c->geo_ev, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->kern_ev, hipEventDisableTiming) != hipSuccess ||
         hipHostMalloc((void **)&c->h_mm, sizeof(double) * 4, hipHostMallocDefault) != hipSuccess ||
+        hipHostGetDevicePointer((void **)&c->h_mm_dev, c->h_mm, 0) != hipSuccess ||
         hipEventCreateWithFlags(&c->mm_event, hipEventDisableTiming) != hipSuccess) {
         delete c;
         return fail(FITSNE_E_HIP, "stream/event/pinned allocation failed");
@@ -1309,7 +1327,7 @@ extern "C" int fitsne_ctx_destroy(fitsne_ctx *c) {
     DBuf *all[] = {&c->rowP, &c->colP, &c->val, &c->Y, &c->uY, &c->gains, &c->pos, &c->dC, &c->wcount, &c->wstart, &c->wpure,
                    &c->wkey, &c->wval, &c->wbmax, &c->knnX, &c->knnXT, &c->knnI, &c->knnD, &c->dgeo,
                    &c->dstep, &c->dtw, &c->dlive, &c->klpart, &c->gin, &c->gout, &c->spec, &c->ospec, &c->Trow, &c->KQ,
-                   &c->K1, &c->KS, &c->KT, &c->zpart, &c->zbuf, &c->part, &c->mean, &c->mmpart, &c->mm, &c->xs,
+                   &c->K1, &c->KS, &c->KT, &c->zpart, &c->zbuf, &c->part, &c->mean, &c->mmpart, &c->mm, &c->mmu, &c->xs,
                    &c->ys, &c->box, &c->wx, &c->wy, &c->ch, &c->w, &c->V, &c->potA, &c->potB,
                    &c->inv_d, &c->colP2, &c->val2, &c->tmp, &c->fixgrid, &c->att, &c->rep, &c->zrow,
                    &c->sy_row, &c->sy_col, &c->sy_val, &c->sy_erow, &c->sy_keys, &c->sy_keys2, &c->sy_vals,
@@ -1954,8 +1972,6 @@ static BatchLds batch_lds(int Mp) {
     return {sizeof(cx<float>) * (lspan<true>(Mp) + (size_t)Mp), sizeof(cx<float>) * (lspan<true>(2 * Mp) + (size_t)Mp),
             sizeof(cx<float>) * (lspan<true>(4 * Mp) + (size_t)Mp)};
 }
-constexpr int kBatchClassMp[kBsClasses] = {kBatchSmallMp, kCapMp2};
-
 // The five FFT passes of one length class (CLS), each one persistent launch.
 template <int CLS, int MP>
 static void batch_fft_class(fitsne_batch *b, hipStream_t st, const BatchProb *bp, BatchSched *sch, int *pref,
@@ -2086,7 +2102,7 @@ static int batch_prepare(fitsne_batch *b) {
         q.mean = c->mean.as<double>();
         q.mmpart = c->mmpart.as<double>();
         const long a = (c->N + c->win_R - 1) / c->win_R, sp = (c->N + c->spread_pts - 1) / c->spread_pts;
-        const long ip = blocks_for(c->N), ce = q.cblk;
+        const long ip = (c->N + kInterpPts - 1) / kInterpPts, ce = q.cblk;
         if ((long)apre[i] + a > INT32_MAX / 2 || (long)sppre[i] + sp > INT32_MAX / 2 ||
             (long)ipre[i] + ip > INT32_MAX / 2)
             return fail(FITSNE_E_UNSUPPORTED, "batch too large for one launch per stage");

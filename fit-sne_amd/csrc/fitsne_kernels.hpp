@@ -483,7 +483,27 @@ __device__ __forceinline__ void attr_win_block(const int *__restrict__ start, co
             pre[u] = q < lim ? Y[base + q] : 0.f;
         }
     };
-    if (w < NW) fetch(w);
+    // Each thread takes quads qq and qq + kAttThreads of a round.  In 2D it loads the next round's pair
+    // (in the same bucket, or the first pair of the next staged bucket) before it computes, so that two
+    // pairs per thread are in flight across the compute and across the barriers of a window change; in
+    // 1D, held to 64 registers for eight waves per SIMD, the waves alone keep the loads in flight.
+    constexpr bool kAhead = D == 2;
+    u32x4 ka = {0u, 0u, 0u, 0u}, kb = {0u, 0u, 0u, 0u};
+    f32x4 va = {0.f, 0.f, 0.f, 0.f}, vb = {0.f, 0.f, 0.f, 0.f};
+    auto load_pair = [&](int qq, int q1) {
+        if (qq < q1) {
+            ka = __builtin_nontemporal_load(&K4[qq]);
+            va = __builtin_nontemporal_load(&V4[qq]);
+        }
+        if (qq + kAttThreads < q1) {
+            kb = __builtin_nontemporal_load(&K4[qq + kAttThreads]);
+            vb = __builtin_nontemporal_load(&V4[qq + kAttThreads]);
+        }
+    };
+    if (w < NW) {
+        fetch(w);
+        if (kAhead) load_pair((st[w] >> 2) + t, st[w + 1] >> 2);
+    }
     while (w < NW) {
         const int q0 = st[w] >> 2, qm = (st[w] + pu[w]) >> 2, q1 = st[w + 1] >> 2;
         int wn = w + 1;
@@ -493,21 +513,22 @@ __device__ __forceinline__ void attr_win_block(const int *__restrict__ start, co
         for (int u = 0; u < WL; u++) yw[t + u * kAttThreads] = pre[u];
         __syncthreads();
         if (wn < NW) fetch(wn);  // consumed after this bucket
-        for (int qq = q0 + t; qq < q1; qq += 2 * kAttThreads) {
-            const u32x4 ka = __builtin_nontemporal_load(&K4[qq]);
-            const f32x4 va = __builtin_nontemporal_load(&V4[qq]);
-            const bool two = qq + kAttThreads < q1;
-            u32x4 kb = {0u, 0u, 0u, 0u};
-            f32x4 vb = {0.f, 0.f, 0.f, 0.f};
-            if (two) {
-                kb = __builtin_nontemporal_load(&K4[qq + kAttThreads]);
-                vb = __builtin_nontemporal_load(&V4[qq + kAttThreads]);
+        int qq = q0 + t;
+        if (kAhead && qq >= q1 && wn < NW) load_pair((st[wn] >> 2) + t, st[wn + 1] >> 2);  // no quad here
+        for (; qq < q1; qq += 2 * kAttThreads) {
+            if (!kAhead) load_pair(qq, q1);
+            const u32x4 ca = ka, cb = kb;
+            const f32x4 cva = va, cvb = vb;
+            const int qn = qq + 2 * kAttThreads;
+            if (kAhead) {
+                if (qn < q1) load_pair(qn, q1);
+                else if (wn < NW) load_pair((st[wn] >> 2) + t, st[wn + 1] >> 2);
             }
-            if (qq < qm) attr_quad_pure<D>(ka, va, yr, yw, acc, inv_df, df_one, S);
-            else attr_quad<D>(ka, va, yr, yw, acc, inv_df, df_one, S);
-            if (two) {
-                if (qq + kAttThreads < qm) attr_quad_pure<D>(kb, vb, yr, yw, acc, inv_df, df_one, S);
-                else attr_quad<D>(kb, vb, yr, yw, acc, inv_df, df_one, S);
+            if (qq < qm) attr_quad_pure<D>(ca, cva, yr, yw, acc, inv_df, df_one, S);
+            else attr_quad<D>(ca, cva, yr, yw, acc, inv_df, df_one, S);
+            if (qq + kAttThreads < q1) {
+                if (qq + kAttThreads < qm) attr_quad_pure<D>(cb, cvb, yr, yw, acc, inv_df, df_one, S);
+                else attr_quad<D>(cb, cvb, yr, yw, acc, inv_df, df_one, S);
             }
         }
         w = wn;
@@ -705,7 +726,9 @@ __device__ void geometry_block(const double *__restrict__ part, int nblk, const 
         o.live = fin && nb >= 1 && g.n1d <= n1d_max && g.Mp <= mp_max;
         if (o.live) {  // the spread's fixed-point scale, with the first coordinate's gap below the grid
             const double y0 = (double)Y[first];
-            g.fix = spread_fix_scale(g, (double)(n / D), D == 2 && y0 < mn ? (mn - y0) / g.bw : 0.0, lb);
+            const double gap = D == 2 && y0 < mn ? (mn - y0) / g.bw : 0.0;
+            g.fix = spread_fix_scale(g, (double)(n / D), gap, lb);
+            g.fix_fast = spread_fix_fast(g, gap, lb);
             if (g.fix == 0) {
                 o.live = 0;
                 o.err = 2;
@@ -842,14 +865,18 @@ template <int CTRL, int RMASK, int NV, int NQ>
 __device__ __forceinline__ void seg_scan_step(float (&val)[NV][NQ], int run) {
     const int src_run = __builtin_amdgcn_update_dpp(0, run, CTRL, RMASK, 0xf, false);
     const bool take = src_run == run;
+    float y[NV][NQ];
 #pragma unroll
     for (int v = 0; v < NV; v++)
 #pragma unroll
-        for (int q = 0; q < NQ; q++) {
-            const float y = __int_as_float(
-                __builtin_amdgcn_update_dpp(0, __float_as_int(val[v][q]), CTRL, RMASK, 0xf, false));
-            val[v][q] += take ? y : 0.f;
-        }
+        for (int q = 0; q < NQ; q++)
+            y[v][q] = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(val[v][q]), CTRL, RMASK, 0xf, false));
+    if (take) {  // one exec mask for all the adds of the step
+#pragma unroll
+        for (int v = 0; v < NV; v++)
+#pragma unroll
+            for (int q = 0; q < NQ; q++) val[v][q] += y[v][q];
+    }
 }
 
 // Table slots of the spread: the table (sums and keys) stays within 40 KB, so that four workgroups fit
@@ -963,9 +990,10 @@ __device__ __forceinline__ void spread_block(const float *__restrict__ Y, int N,
         seg_scan_step<0x142, 0xa, NV, NQ>(val, run);  // row_bcast:15 into rows 1 and 3
         seg_scan_step<0x143, 0xc, NV, NQ>(val, run);  // row_bcast:31 into rows 2 and 3
         // each run's last lane adds its totals, as g.fix fixed point, into its box's table slot (open
-        // addressing), or into the grid when the table has no free slot
-        if (tail && b >= 0) {
-            int slot = -1;
+        // addressing), or into the grid when the table has no free slot (rare: a separate pass)
+        const bool emit = tail && b >= 0;
+        int slot = -1;
+        if (emit) {
             int h = b % NS;
             for (int probe = 0; probe < NS; probe++) {
                 const int k = atomicCAS(&tkey[h], -1, b);
@@ -975,21 +1003,32 @@ __device__ __forceinline__ void spread_block(const float *__restrict__ Y, int N,
                 }
                 h = h + 1 == NS ? 0 : h + 1;
             }
-            const int bx = D == 2 ? b % g.nb : b, by = D == 2 ? b / g.nb : 0;
+        }
+        if (emit && slot >= 0) {
+            if (g.fix_fast) {  // uniform: every run sum within the 2^51 range of fix_rn
 #pragma unroll
-            for (int v = 0; v < NV; v++)
+                for (int v = 0; v < NV; v++)
 #pragma unroll
-                for (int q = 0; q < NQ; q++) {
-                    const long long sum = fix_rn_any((double)val[v][q] * g.fix);
-                    if (sum == 0) continue;
-                    if (slot >= 0) {
-                        atomicAdd(&tval[slot][v * NQ + q], (unsigned long long)sum);
-                    } else {
+                    for (int q = 0; q < NQ; q++)
+                        atomicAdd(&tval[slot][v * NQ + q], (unsigned long long)fix_rn((double)val[v][q] * g.fix));
+            } else {
+#pragma unroll
+                for (int v = 0; v < NV; v++)
+#pragma unroll
+                    for (int q = 0; q < NQ; q++)
+                        atomicAdd(&tval[slot][v * NQ + q], (unsigned long long)fix_rn_any((double)val[v][q] * g.fix));
+            }
+        }
+        if (__ballot(emit && slot < 0)) {
+            if (emit && slot < 0) {
+                const int bx = D == 2 ? b % g.nb : b, by = D == 2 ? b / g.nb : 0;
+                for (int v = 0; v < NV; v++)
+                    for (int q = 0; q < NQ; q++) {
                         const long node =
                             D == 2 ? (long)(bx * P + v / P) * g.n1d + by * P + v % P : (long)bx * P + v;
-                        atomicAdd(&grid[node * 4 + q], (unsigned long long)sum);
+                        atomicAdd(&grid[node * 4 + q], (unsigned long long)fix_rn_any((double)val[v][q] * g.fix));
                     }
-                }
+            }
         }
     }
     __syncthreads();
@@ -1612,91 +1651,108 @@ __global__ void k_zfinal(const DevGeo *__restrict__ dg, const double *__restrict
 }
 
 // ---------------------------------------------------------------- interpolation + update, FP32 form
-// Points [bid 256, (bid + 1) 256) of the interpolation and update; the body of k_interp_f32 and of its
-// batched form.  sh: kThreads / kWave doubles of LDS.
+// Each thread takes kInterpPpt points, kThreads apart: points [bid kInterpPts, (bid + 1) kInterpPts) per
+// workgroup.  All per-point loads of a thread are issued before any of them is used, then all grid
+// reads, so that one wave keeps both points' memory traffic in flight.
+constexpr int kInterpPpt = 2;
+constexpr int kInterpPts = kInterpPpt * kThreads;
+
+// Lagrange weights of one coordinate and the offsets from the nodes (in grid units times bw0).
+template <int P, int D>
+__device__ __forceinline__ int interp_axis(float yv, const Geo &g, float (&L)[P], float (&e)[P]) {
+    double t;
+    const int b = box_coord<D>((double)yv, g, t);
+    float dt[P];
+#pragma unroll
+    for (int a = 0; a < P; a++) dt[a] = (float)(t - g.sp[a]);
+#pragma unroll
+    for (int a = 0; a < P; a++) {
+        float v = 1.f;
+#pragma unroll
+        for (int c = 0; c < P; c++)
+            if (c != a) v *= dt[c];
+        L[a] = v * g.rdenf[a];
+        e[a] = dt[a] * (float)g.bw0;
+    }
+    return b;
+}
+
+// Workgroup bid of the interpolation and update; the body of k_interp_f32 and of its batched form.
+// sh: kThreads / kWave doubles of LDS (3 D kThreads / kWave with mmu).
 template <int P, int D>
 __device__ __forceinline__ void interp_block(float *__restrict__ Y, const float *__restrict__ pos,
                                              const float4 *__restrict__ gout, const DevGeo *__restrict__ dg,
                                              const double *__restrict__ zbuf, const StepDev *__restrict__ sd,
                                              float *__restrict__ dC, float *__restrict__ uY, float *__restrict__ gains,
-                                             double *__restrict__ part, int N, int bid, double *sh) {
+                                             double *__restrict__ part, int N, int bid, double *sh,
+                                             double *__restrict__ mmu = nullptr, long skip = -1) {
+    constexpr int U = kInterpPpt;
+    int idx[U];
+    float y[U][D], pi[U][D], u[U][D], gn[U][D];
+#pragma unroll
+    for (int k = 0; k < U; k++) {
+        idx[k] = bid * kInterpPts + k * kThreads + threadIdx.x;
+        if (idx[k] < N) {  // the momentum state is there in every mode; it is used only by updates
+            load_pt<float, D>(Y, idx[k], y[k]);
+            // read once per step: streamed past the caches
+#pragma unroll
+            for (int d = 0; d < D; d++) {
+                pi[k][d] = __builtin_nontemporal_load(&pos[(long)idx[k] * D + d]);
+                u[k][d] = __builtin_nontemporal_load(&uY[(long)idx[k] * D + d]);
+                gn[k][d] = __builtin_nontemporal_load(&gains[(long)idx[k] * D + d]);
+            }
+        }
+    }
     const Geo &g = dg->g;
     const StepDev up = *sd;  // mode 0: write dC only; 1: optimiser update of Y, uY, gains
-    const int i = bid * kThreads + threadIdx.x;
-    double ysum[D];
+    const float zinv = (float)zbuf[1];
+    float Lx[U][P], ex[U][P], Ly[U][P], ey[U][P];
+    float4 G[U][D == 2 ? P * P : P];
 #pragma unroll
-    for (int d = 0; d < D; d++) ysum[d] = 0.0;
-    if (i < N) {
-        // every per-point load first, so that they are in flight together
-        const bool mom = up.mode != 0 && !up.plain_descent;
-        float y[D], pi[D], u[D], gn[D];
-        load_pt<float, D>(Y, i, y);
+    for (int k = 0; k < U; k++) {
+        if (idx[k] < N) {
+            const int bx = interp_axis<P, D>(y[k][0], g, Lx[k], ex[k]);
+            if constexpr (D == 2) {
+                const int by = interp_axis<P, D>(y[k][1], g, Ly[k], ey[k]);
 #pragma unroll
-        for (int d = 0; d < D; d++) {
-            pi[d] = pos[(long)i * D + d];
-            u[d] = mom ? uY[(long)i * D + d] : 0.f;
-            gn[d] = mom ? gains[(long)i * D + d] : 0.f;
-        }
-        const float zinv = (float)zbuf[1];
-        float Lx[P], ex[P], Ly[P], ey[P];
-        int bx, by = 0;
-        {
-            double t;
-            bx = box_coord<D>((double)y[0], g, t);
-            float dt[P];
+                for (int a = 0; a < P; a++) {
+                    const float4 *rowp = gout + (long)(bx * P + a) * g.n1d + by * P;
 #pragma unroll
-            for (int a = 0; a < P; a++) dt[a] = (float)(t - g.sp[a]);
+                    for (int c = 0; c < P; c++) G[k][a * P + c] = rowp[c];
+                }
+            } else {
 #pragma unroll
-            for (int a = 0; a < P; a++) {
-                float v = 1.f;
-#pragma unroll
-                for (int c = 0; c < P; c++)
-                    if (c != a) v *= dt[c];
-                Lx[a] = v * g.rdenf[a];
-                ex[a] = dt[a] * (float)g.bw0;
+                for (int a = 0; a < P; a++) G[k][a] = gout[(long)bx * P + a];
             }
         }
-        if constexpr (D == 2) {
-            double t;
-            by = box_coord<D>((double)y[1], g, t);
-            float dt[P];
+    }
+    double ysum[U][D];
 #pragma unroll
-            for (int a = 0; a < P; a++) dt[a] = (float)(t - g.sp[a]);
+    for (int k = 0; k < U; k++) {
+        const int i = idx[k];
 #pragma unroll
-            for (int a = 0; a < P; a++) {
-                float v = 1.f;
-#pragma unroll
-                for (int c = 0; c < P; c++)
-                    if (c != a) v *= dt[c];
-                Ly[a] = v * g.rdenf[a];
-                ey[a] = dt[a] * (float)g.bw0;
-            }
-        }
+        for (int d = 0; d < D; d++) ysum[k][d] = 0.0;
+        if (i >= N) continue;
         float neg[D];
 #pragma unroll
         for (int d = 0; d < D; d++) neg[d] = 0.f;
         if constexpr (D == 2) {
 #pragma unroll
-            for (int a = 0; a < P; a++) {
-                const float4 *rowp = gout + (long)(bx * P + a) * g.n1d + by * P;
+            for (int a = 0; a < P; a++)
 #pragma unroll
                 for (int c = 0; c < P; c++) {
-                    const float4 G = rowp[c];
-                    const float w = Lx[a] * Ly[c];
-                    neg[0] += w * (ex[a] * G.x + G.y);
-                    neg[1] += w * (ey[c] * G.x + G.z);
+                    const float4 Gv = G[k][a * P + c];
+                    const float w = Lx[k][a] * Ly[k][c];
+                    neg[0] += w * (ex[k][a] * Gv.x + Gv.y);
+                    neg[1] += w * (ey[k][c] * Gv.x + Gv.z);
                 }
-            }
         } else {
 #pragma unroll
-            for (int a = 0; a < P; a++) {
-                const float4 G = gout[(long)bx * P + a];
-                neg[0] += Lx[a] * (ex[a] * G.x + G.y);
-            }
+            for (int a = 0; a < P; a++) neg[0] += Lx[k][a] * (ex[k][a] * G[k][a].x + G[k][a].y);
         }
         float dy[D];
 #pragma unroll
-        for (int d = 0; d < D; d++) dy[d] = pi[d] - neg[d] * zinv;
+        for (int d = 0; d < D; d++) dy[d] = pi[k][d] - neg[d] * zinv;
         if (up.mode == 0) {
 #pragma unroll
             for (int d = 0; d < D; d++) dC[(long)i * D + d] = dy[d];
@@ -1704,44 +1760,87 @@ __device__ __forceinline__ void interp_block(float *__restrict__ Y, const float 
             float yn[D];
             if (up.plain_descent) {
 #pragma unroll
-                for (int d = 0; d < D; d++) yn[d] = y[d] - dy[d];
+                for (int d = 0; d < D; d++) yn[d] = y[k][d] - dy[d];
             } else {
                 float nrm = 0.f;
 #pragma unroll
                 for (int d = 0; d < D; d++) {
-                    const float sd = dy[d] == 0.f ? 0.f : (dy[d] < 0.f ? -1.f : 1.f);
-                    const float su = u[d] == 0.f ? 0.f : (u[d] < 0.f ? -1.f : 1.f);
-                    gn[d] = sd != su ? gn[d] + 0.2f : gn[d] * 0.8f;
-                    if (gn[d] < 0.01f) gn[d] = 0.01f;
-                    u[d] = up.momentum * u[d] - up.lr * gn[d] * dy[d];
-                    nrm += u[d] * u[d];
+                    const float sdv = dy[d] == 0.f ? 0.f : (dy[d] < 0.f ? -1.f : 1.f);
+                    const float su = u[k][d] == 0.f ? 0.f : (u[k][d] < 0.f ? -1.f : 1.f);
+                    gn[k][d] = sdv != su ? gn[k][d] + 0.2f : gn[k][d] * 0.8f;
+                    if (gn[k][d] < 0.01f) gn[k][d] = 0.01f;
+                    u[k][d] = up.momentum * u[k][d] - up.lr * gn[k][d] * dy[d];
+                    nrm += u[k][d] * u[k][d];
                 }
                 if (up.max_step_norm > 0.f) {
                     nrm = sqrtf(nrm);
                     if (nrm > up.max_step_norm) {
 #pragma unroll
-                        for (int d = 0; d < D; d++) u[d] *= up.max_step_norm / nrm;
+                        for (int d = 0; d < D; d++) u[k][d] *= up.max_step_norm / nrm;
                     }
                 }
 #pragma unroll
                 for (int d = 0; d < D; d++) {
-                    uY[(long)i * D + d] = u[d];
-                    gains[(long)i * D + d] = gn[d];
-                    yn[d] = y[d] + u[d];
+                    __builtin_nontemporal_store(u[k][d], &uY[(long)i * D + d]);
+                    __builtin_nontemporal_store(gn[k][d], &gains[(long)i * D + d]);
+                    yn[d] = y[k][d] + u[k][d];
                 }
             }
 #pragma unroll
             for (int d = 0; d < D; d++) {
                 Y[(long)i * D + d] = yn[d];
-                ysum[d] = (double)yn[d];
+                ysum[k][d] = (double)yn[d];
             }
         }
     }
-    if (up.mode == 1) {
+    if (up.mode == 1 && !mmu) {
 #pragma unroll
         for (int d = 0; d < D; d++) {
-            const double s = block_sum(ysum[d], sh);
+            double t = ysum[0][d];
+#pragma unroll
+            for (int k = 1; k < U; k++) t += ysum[k][d];
+            const double s = block_sum(t, sh);
             if (threadIdx.x == 0) part[bid * D + d] = s;
+        }
+    } else if (up.mode == 1) {
+        // the same sums (block_sum's order), and with them in one pass the per coordinate max and min of the
+        // updated Y (before its mean is taken off), the min without the flat element `skip`: k_mean_mm
+        // turns them into those of the centred Y.  sh holds 3 D kThreads / kWave doubles.
+        constexpr int NW = kThreads / kWave;
+        double v[3 * D];
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            double t = ysum[0][d], mx = -INFINITY, mn = INFINITY;
+#pragma unroll
+            for (int k = 1; k < U; k++) t += ysum[k][d];
+#pragma unroll
+            for (int k = 0; k < U; k++) {
+                if (idx[k] < N) mx = fmax(mx, ysum[k][d]);
+                if (idx[k] < N && (long)idx[k] * D + d != skip) mn = fmin(mn, ysum[k][d]);
+            }
+            v[d] = wave_sum(t);
+            v[D + 2 * d] = wave_max(mx);
+            v[D + 2 * d + 1] = wave_min(mn);
+        }
+        const int w = threadIdx.x / kWave;
+        __syncthreads();
+        if (threadIdx.x % kWave == 0)
+#pragma unroll
+            for (int k = 0; k < 3 * D; k++) sh[k * NW + w] = v[k];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int d = 0; d < D; d++) {
+                double r = 0.0, mx = sh[(D + 2 * d) * NW], mn = sh[(D + 2 * d + 1) * NW];
+                for (int k = 0; k < NW; k++) r += sh[d * NW + k];
+                for (int k = 1; k < NW; k++) {
+                    mx = fmax(mx, sh[(D + 2 * d) * NW + k]);
+                    mn = fmin(mn, sh[(D + 2 * d + 1) * NW + k]);
+                }
+                part[bid * D + d] = r;
+                mmu[(bid * D + d) * 2 + 0] = mx;
+                mmu[(bid * D + d) * 2 + 1] = mn;
+            }
         }
     }
 }
@@ -1754,10 +1853,10 @@ __global__ __launch_bounds__(kThreads) void k_interp_f32(float *__restrict__ Y, 
                                                          const StepDev *__restrict__ sd,
                                                          float *__restrict__ dC, float *__restrict__ uY,
                                                          float *__restrict__ gains, double *__restrict__ part,
-                                                         int N) {
+                                                         int N, double *__restrict__ mmu, long skip) {
     if (!dg->live) return;
-    __shared__ double sh[kThreads / kWave];
-    interp_block<P, D>(Y, pos, gout, dg, zbuf, sd, dC, uY, gains, part, N, blockIdx.x, sh);
+    __shared__ double sh[3 * D * kThreads / kWave];
+    interp_block<P, D>(Y, pos, gout, dg, zbuf, sd, dC, uY, gains, part, N, blockIdx.x, sh, mmu, skip);
 }
 
 // Mean of the updated Y from block partials (fixed order), FP64.  One workgroup of kThreads.
@@ -1807,6 +1906,50 @@ __global__ void k_mean_final(const DevGeo *__restrict__ dg, const double *__rest
     if (!dg->live) return;
     __shared__ double sh[kThreads / kWave];
     mean_block(part, nblk, D, N, mean, sh);
+}
+
+// k_mean_final, and the min / max that zeroMean will leave, for the host's next geometry: out = (max, min,
+// Y[first]) of the centred Y, written to pinned host memory.  Centring maps each coordinate d through
+// f_d(y) = (float)((double)y - mean_d), which never decreases, so the max and min of the centred values
+// are f_d of the per-coordinate max and min that k_interp_f32 left in mmu (nblk blocks, D coordinates).
+template <int D>
+__global__ __launch_bounds__(kThreads) void k_mean_mm(const DevGeo *__restrict__ dg, const double *__restrict__ part,
+                                                      const double *__restrict__ mmu, int nblk, double N,
+                                                      const float *__restrict__ Y, long first,
+                                                      double *__restrict__ mean, double *__restrict__ out) {
+    if (!dg->live) return;
+    __shared__ double sh[kThreads / kWave];
+    mean_block(part, nblk, D, N, mean, sh);  // thread 0 wrote mean[0, D)
+    __shared__ double shm[2 * kThreads / kWave];
+    double gmx = -INFINITY, gmn = INFINITY;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        double mx = -INFINITY, mn = INFINITY;
+        const double2 *m2 = reinterpret_cast<const double2 *>(mmu) + d;
+        for (int b = threadIdx.x; b < nblk; b += 4 * kThreads) {  // loads issued four at a time
+            double2 u[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                u[k] = b + k * kThreads < nblk ? m2[(long)(b + k * kThreads) * D] : make_double2(-INFINITY, INFINITY);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                mx = fmax(mx, u[k].x);
+                mn = fmin(mn, u[k].y);
+            }
+        }
+        block_maxmin(mx, mn, shm);
+        if (threadIdx.x == 0) {
+            const double m = mean[d];
+            gmx = fmax(gmx, (double)(float)(mx - m));
+            gmn = fmin(gmn, (double)(float)(mn - m));
+        }
+    }
+    if (threadIdx.x == 0) {
+        out[0] = gmx;
+        out[1] = gmn;
+        out[2] = (double)(float)((double)Y[first] - mean[0]);
+        __threadfence_system();
+    }
 }
 
 // zeroMean (tsne.cpp:1851-1876): Y -= mean, with the min/max partials of the result for the next
