@@ -37,10 +37,13 @@ struct BatchProb {
 
 // FFT passes fed from per-stage work counters.  Each pass runs as two launches: one sized (LDS, and so
 // occupancy) for FFT lengths up to kBatchSmallMp, which most problems of a batch have, and one for the
-// rest, up to the build's limit; a problem's lines go to the class of its FFT length.
+// rest, up to the build's limit; a problem's lines go to the class of its FFT length.  (A third class
+// for lengths up to 512, with half the threads and LDS, was no faster: per complex element the passes
+// run at the single-context passes' rate, and the extra launches add their latency.)
 enum { kBsKernRows = 0, kBsKernCols, kBsRowsFwd, kBsCols, kBsRowsInv, kBsStages };
 constexpr int kBsClasses = 2;
 constexpr int kBatchSmallMp = 1024;
+__host__ __device__ constexpr int batch_class(int Mp) { return Mp <= kBatchSmallMp ? 0 : 1; }
 
 struct BatchSched {
     int ctr[kBsStages * kBsClasses];    // next item of each pass and class (cleared by kb_sched every step)
@@ -87,7 +90,7 @@ __global__ __launch_bounds__(1024) void kb_sched(const BatchProb *__restrict__ b
             const DevGeo *dg = bp[p].dg;
             if (dg->live) {
                 const int n1d = dg->g.n1d, H = dg->g.H;
-                const int o = dg->g.Mp <= kBatchSmallMp ? 0 : kBsStages;
+                const int o = kBsStages * batch_class(dg->g.Mp);
                 if (dg->kspec_stale) {
                     cnt[o + kBsKernRows] = n1d;
                     cnt[o + kBsKernCols] = H;
@@ -137,24 +140,37 @@ __global__ __launch_bounds__(1024) void kb_sched(const BatchProb *__restrict__ b
     }
 }
 
+// Problems whose line prefixes a persistent pass keeps in LDS (larger batches search global memory).
+constexpr int kBatchPfLds = 512;
+
 // Workgroup loop of a persistent FFT pass: takes items of stage S until they run out and calls
 // line(problem, line index, problem changed).  Every call of `line` ends with a barrier, so the item
-// slot and the staged twiddles are free again when the next item is taken.
+// slot and the staged twiddles are free again when the next item is taken.  The next item is claimed
+// before the current one is worked, so the counter's round trip overlaps the line; the problem of an
+// item is found in an LDS copy of the stage's prefixes.
 template <int S, typename F>
 __device__ __forceinline__ void batch_items(BatchSched *__restrict__ sch, const int *__restrict__ pref, int nprob,
                                             F &&line) {
     __shared__ int s_item;
+    __shared__ int s_pf[kBatchPfLds];
     const int *pf = pref + S * (nprob + 1);
+    const bool pf_lds = nprob + 1 <= kBatchPfLds;
+    if (pf_lds)
+        for (int q = threadIdx.x; q <= nprob; q += blockDim.x) s_pf[q] = pf[q];
     const int total = sch->total[S];
+    if (threadIdx.x == 0) s_item = atomicAdd(&sch->ctr[S], 1);
+    __syncthreads();
+    int it = s_item;
     int cur = -1;
-    for (;;) {
-        if (threadIdx.x == 0) s_item = atomicAdd(&sch->ctr[S], 1);
-        __syncthreads();
-        const int it = s_item;
-        if (it >= total) break;
-        const int p = batch_find(pf, nprob, it);
-        line(p, it - pf[p], p != cur);
+    while (it < total) {
+        int next = 0;
+        if (threadIdx.x == 0) next = atomicAdd(&sch->ctr[S], 1);
+        const int p = pf_lds ? batch_find(s_pf, nprob, it) : batch_find(pf, nprob, it);
+        line(p, it - (pf_lds ? s_pf[p] : pf[p]), p != cur);
         cur = p;
+        if (threadIdx.x == 0) s_item = next;  // every thread read s_item before the line's barriers
+        __syncthreads();
+        it = s_item;
     }
 }
 
@@ -206,7 +222,7 @@ __global__ __launch_bounds__(NTHR) void kb_cols(const BatchProb *__restrict__ bp
                                                 BatchSched *__restrict__ sch, const int *__restrict__ pref) {
     extern __shared__ unsigned char smem_raw[];
     cx<float> *buf = reinterpret_cast<cx<float> *>(smem_raw);
-    __shared__ double shz[NTHR / kWave];
+    __shared__ double shz[kColZLanes / kWave];
     const cx<float> *twl = nullptr;
     batch_items<kBsCols + kBsStages * CLS>(sch, pref, nprob, [&](int p, int k, bool fresh) {
         const BatchProb &b = bp[p];
@@ -246,7 +262,8 @@ __global__ __launch_bounds__(kAttThreads) __attribute__((amdgpu_waves_per_eu(4))
     const int p = batch_find(pref, nprob, blockIdx.x);
     const BatchProb &b = bp[p];
     if (!b.dg->live) return;
-    attr_win_block<2>(b.wstart, b.wpure, b.wkey, b.wval, b.wbmax, b.Y, b.N, b.win_R, b.win_NW, inv_df, df_one, mdf,
+    attr_win_block<2>(b.wstart, b.wpure, b.wkey, b.wval, b.wbmax, b.Y, b.N, b.win_R, b.win_NW, (const int *)nullptr,
+                      (b.N + b.win_R - 1) / b.win_R, inv_df, df_one, mdf,
                       b.sd->exag, b.pos, blockIdx.x - pref[p], L);
 }
 

@@ -127,9 +127,10 @@ struct fitsne_ctx {
     // exact kNN (affinity build)
     DBuf knnX, knnXT, knnI, knnD;
     // windowed copy of P for the FP32 attractive term (k_attr_win), built on first use
-    DBuf wcount, wstart, wpure, wkey, wval, wbmax;
+    DBuf wcount, wstart, wpure, wkey, wval, wbmax, wrstart;
     bool win_ok = false;
-    int win_R = 0, win_NW = 0;
+    int win_R = 0, win_NB = 0, win_NW = 0;
+    bool win_bal = false;  // row blocks from balance_blocks (starts in wrstart)
     // grids, spectra
     DBuf gin, gout, spec, ospec, Trow, KQ, K1, KS, KT, zpart, zbuf, part, mean, mmpart, mm, fixgrid;
     DBuf mmu;      // per block and coordinate (max, min) of the updated Y, from interp_f32 for mean_mm
@@ -461,17 +462,127 @@ static auto pick_interp_f64(int p) -> decltype(&k_interp_f64<3, D>) {
     }
 }
 
+// Row blocks of about equal cost for the windowed attractive kernel (att_rows == kAttBalanced).  One
+// workgroup per CU holds a block, all start together, and the kernel lasts as long as its slowest block,
+// so the blocks are cut by a cost model instead of a row count: per block, a time per quad of entries,
+// per staged window (two barriers and a drain of the quad pipeline) and per small bucket (read from
+// global memory), fitted to per-workgroup timestamps at N = 1M (0.00133, 1.5 and 0.4 µs; the staged
+// windows explain most of the spread of block times).  The entries of every chunk of kAttChunk rows
+// per window come from k_win_count at that block size; a greedy pass over the chunks closes a block
+// when adding the next chunk would pass the target cost or 4096 rows, and the smallest target that
+// needs no more blocks than the CUs (times the rounds N needs) is found by bisection.  Any partition
+// gives the same forces bit for bit (one fixed-point scale for the problem).
+constexpr int kAttBalanced = -1;
+constexpr int kAttChunk = 256;
+
+static int balance_blocks(fitsne_ctx *c, int NW, std::vector<int> &rs) {
+    const int N = c->N;
+    const int nch = (N + kAttChunk - 1) / kAttChunk;
+    DBuf cnt, pur, bm;
+    TRY(ensure(cnt, sizeof(int) * (size_t)nch * NW));
+    TRY(ensure(pur, sizeof(int) * (size_t)nch * NW));
+    TRY(ensure(bm, sizeof(float) * (size_t)nch));
+    launch(c, "win_count", k_win_count, dim3(nch), dim3(kAttThreads), sizeof(int) * 2 * (size_t)NW,
+           c->rowP.as<const int64_t>(), c->colP.as<const uint32_t>(), c->val.as<const float>(), N, kAttChunk, NW,
+           (const int *)nullptr, cnt.as<int>(), pur.as<int>(), bm.as<float>());
+    std::vector<int> h((size_t)nch * NW);
+    HIPTRY(hipMemcpyAsync(h.data(), cnt.p, sizeof(int) * h.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPTRY(hipStreamSynchronize(c->stream));
+    release(cnt);
+    release(pur);
+    release(bm);
+    // sparse chunks: (window, entries) pairs
+    std::vector<int> off(nch + 1, 0), win, ent;
+    for (int k = 0; k < nch; k++) {
+        for (int w = 0; w < NW; w++) {
+            const int e = h[(size_t)k * NW + w];
+            if (e > 0) {
+                win.push_back(w);
+                ent.push_back(e);
+            }
+        }
+        off[k + 1] = (int)win.size();
+    }
+    constexpr double kQuad = 0.00133, kStaged = 1.5, kSmall = 0.4;
+    std::vector<int> acc(NW, 0);
+    std::vector<int> touched;
+    auto status = [](int e) { const int q = (e + 3) / 4; return q == 0 ? 0 : q < kSmallQuads ? 1 : 2; };
+    // greedy partition for target T: block starts into out (if given), returns the number of blocks
+    auto greedy = [&](double T, std::vector<int> *out) {
+        int nb = 0, rows = 0;
+        double cost = 0.0;
+        touched.clear();
+        if (out) out->assign(1, 0);
+        for (int k = 0; k < nch; k++) {
+            const int kr = std::min(kAttChunk, N - k * kAttChunk);
+            double d = 0.0;
+            for (int i = off[k]; i < off[k + 1]; i++) {
+                const int s0 = status(acc[win[i]]), s1 = status(acc[win[i]] + ent[i]);
+                d += kQuad * ent[i] / 4.0 + (s1 != s0 ? (s1 == 2 ? kStaged : 0.0) - (s0 == 2 ? kStaged : 0.0) +
+                                                            (s1 == 1 ? kSmall : 0.0) - (s0 == 1 ? kSmall : 0.0)
+                                                      : 0.0);
+            }
+            if (rows > 0 && (rows + kr > kAttRows || cost + d > T)) {  // close the block before chunk k
+                for (int w : touched) acc[w] = 0;
+                touched.clear();
+                nb++;
+                if (out) out->push_back(k * kAttChunk);
+                rows = 0;
+                cost = 0.0;
+                d = 0.0;
+                for (int i = off[k]; i < off[k + 1]; i++)
+                    d += kQuad * ent[i] / 4.0 + (status(ent[i]) == 2 ? kStaged : status(ent[i]) == 1 ? kSmall : 0.0);
+            }
+            for (int i = off[k]; i < off[k + 1]; i++) {
+                if (acc[win[i]] == 0) touched.push_back(win[i]);
+                acc[win[i]] += ent[i];
+            }
+            rows += kr;
+            cost += d;
+        }
+        for (int w : touched) acc[w] = 0;
+        touched.clear();
+        if (out) out->push_back(N);
+        return nb + 1;
+    };
+    int cus = 0;
+    HIPTRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+    const long rounds = std::max(1l, ((long)N + (long)cus * kAttRows - 1) / ((long)cus * kAttRows));
+    const int want = (int)std::min<long>((long)cus * rounds, (N + kAttChunk - 1) / kAttChunk);
+    double lo = 0.0, hi = 1.0;
+    while (greedy(hi, nullptr) > want && hi < 1e12) hi *= 2.0;
+    for (int it = 0; it < 40; it++) {
+        const double mid = 0.5 * (lo + hi);
+        if (greedy(mid, nullptr) <= want) hi = mid;
+        else lo = mid;
+    }
+    greedy(hi, &rs);
+    return FITSNE_OK;
+}
+
 // Bucketed copy of the CSR for k_attr_win: counts per (row block, window), scan, scatter.  Used when
 // the entry offsets fit an int and the window cursors fit in LDS; otherwise the CSR kernel runs.
 static int build_windows(fitsne_ctx *c) {
     const int N = c->N;
     c->win_ok = false;
-    const int R = c->att_rows > 0 ? std::min(kAttRows, c->att_rows)
-                                  : std::min(kAttRows, 64 * (int)((((long)N + 255) / 256 + 63) / 64));
-    const int NB = (N + R - 1) / R, NW = (N + kWin - 1) / kWin;
+    const bool bal = c->att_rows == kAttBalanced;
+    const int NW = (N + kWin - 1) / kWin;
+    if (c->nnz <= 0 || NW > 8192) return FITSNE_OK;
+    std::vector<int> rs;
+    if (bal) TRY(balance_blocks(c, NW, rs));
+    const int R = bal ? kAttRows
+                      : c->att_rows > 0 ? std::min(kAttRows, c->att_rows)
+                                        : std::min(kAttRows, 64 * (int)((((long)N + 255) / 256 + 63) / 64));
+    const int NB = bal ? (int)rs.size() - 1 : (N + R - 1) / R;
     const long nbw = (long)NB * NW;
     const long long cap = c->nnz + 3ll * nbw;  // every bucket padded to a multiple of 4 entries
-    if (c->nnz <= 0 || NW > 8192 || nbw >= (1l << 28) || cap >= (1ll << 31) - 1) return FITSNE_OK;
+    if (nbw >= (1l << 28) || cap >= (1ll << 31) - 1) return FITSNE_OK;
+    if (bal) {
+        TRY(ensure(c->wrstart, sizeof(int) * rs.size()));
+        HIPTRY(hipMemcpyAsync(c->wrstart.p, rs.data(), sizeof(int) * rs.size(), hipMemcpyHostToDevice, c->stream));
+        HIPTRY(hipStreamSynchronize(c->stream));  // rs goes out of scope
+    }
+    const int *rstart = bal ? c->wrstart.as<const int>() : nullptr;
     TRY(ensure(c->wcount, sizeof(int) * nbw));
     TRY(ensure(c->wpure, sizeof(int) * nbw));
     TRY(ensure(c->wstart, sizeof(int) * (nbw + 1)));
@@ -480,16 +591,18 @@ static int build_windows(fitsne_ctx *c) {
     TRY(ensure(c->wval, sizeof(float) * (size_t)cap));
     const size_t lds = sizeof(int) * 2 * (size_t)NW;
     launch(c, "win_count", k_win_count, dim3(NB), dim3(kAttThreads), lds, c->rowP.as<const int64_t>(),
-           c->colP.as<const uint32_t>(), c->val.as<const float>(), N, R, NW, c->wcount.as<int>(), c->wpure.as<int>(),
-           c->wbmax.as<float>());
+           c->colP.as<const uint32_t>(), c->val.as<const float>(), N, R, NW, rstart, c->wcount.as<int>(),
+           c->wpure.as<int>(), c->wbmax.as<float>());
     launch(c, "bmax_all", k_bmax_all, dim3(1), dim3(1024), 0, c->wbmax.as<float>(), NB);
     launch(c, "scan_counts", k_scan_counts, dim3(1), dim3(1024), 0, c->wcount.as<int>(), c->wstart.as<int>(), (int)nbw);
     launch(c, "win_fill", k_win_fill, dim3(NB), dim3(kAttThreads), lds, c->rowP.as<const int64_t>(),
-           c->colP.as<const uint32_t>(), c->val.as<const float>(), N, R, NW, c->wstart.as<const int>(),
+           c->colP.as<const uint32_t>(), c->val.as<const float>(), N, R, NW, rstart, c->wstart.as<const int>(),
            c->wpure.as<const int>(), c->wkey.as<uint32_t>(), c->wval.as<float>());
     HIPTRY(hipGetLastError());
     c->win_R = R;
+    c->win_NB = NB;
     c->win_NW = NW;
+    c->win_bal = bal;
     c->win_ok = true;
     return FITSNE_OK;
 }
@@ -515,11 +628,11 @@ static int launch_attractive(fitsne_ctx *c, TV scale, TV *pos, const DevGeo *dg 
     if constexpr (std::is_same<TV, float>::value) {
         if (!c->win_ok) TRY(build_windows(c));
         if (c->win_ok) {
-            const int NB = (c->N + c->win_R - 1) / c->win_R;
+            const int NB = c->win_NB;
             launch(c, "attractive", k_attr_win<D>, dim3(8 * ((NB + 7) / 8)), dim3(kAttThreads), 0,
                    c->wstart.as<const int>(), c->wpure.as<const int>(), c->wkey.as<const uint32_t>(),
-                   c->wval.as<const float>(),
-                   c->wbmax.as<const float>(), c->Y.as<const float>(), c->N, c->win_R, c->win_NW, inv_df, df1,
+                   c->wval.as<const float>(), c->wbmax.as<const float>(), c->Y.as<const float>(), c->N, c->win_R,
+                   c->win_NW, c->win_bal ? c->wrstart.as<const int>() : (const int *)nullptr, NB, inv_df, df1,
                    (float)(0.5 * std::sqrt(c->df)), c->dstep.as<const StepDev>(), captured ? -1.f : (float)scale,
                    dg, pos);
             return FITSNE_OK;
@@ -836,7 +949,7 @@ static int tune_spread(fitsne_ctx *c) {
 template <int D>
 static int tune_attractive(fitsne_ctx *c) {
     if (c->nnz <= 0) return FITSNE_OK;
-    const int cand[] = {0, 1024, 2048, 4096};
+    const int cand[] = {0, 1024, 2048, 4096, kAttBalanced};
     const DevGeo *live;
     TRY(live_record(c, &live));
     TRY(ensure(c->pos, sizeof(float) * (size_t)c->N * D));
@@ -1325,7 +1438,7 @@ extern "C" int fitsne_ctx_destroy(fitsne_ctx *c) {
     if (c->astream) (void)hipStreamSynchronize(c->astream);
     drop_graph(c);
     DBuf *all[] = {&c->rowP, &c->colP, &c->val, &c->Y, &c->uY, &c->gains, &c->pos, &c->dC, &c->wcount, &c->wstart, &c->wpure,
-                   &c->wkey, &c->wval, &c->wbmax, &c->knnX, &c->knnXT, &c->knnI, &c->knnD, &c->dgeo,
+                   &c->wkey, &c->wval, &c->wbmax, &c->wrstart, &c->knnX, &c->knnXT, &c->knnI, &c->knnD, &c->dgeo,
                    &c->dstep, &c->dtw, &c->dlive, &c->klpart, &c->gin, &c->gout, &c->spec, &c->ospec, &c->Trow, &c->KQ,
                    &c->K1, &c->KS, &c->KT, &c->zpart, &c->zbuf, &c->part, &c->mean, &c->mmpart, &c->mm, &c->mmu, &c->xs,
                    &c->ys, &c->box, &c->wx, &c->wy, &c->ch, &c->w, &c->V, &c->potA, &c->potB,
@@ -1832,14 +1945,15 @@ extern "C" int fitsne_autotune(fitsne_ctx *c, int *spread_points) {
 extern "C" int fitsne_autotune_stats(fitsne_ctx *c, int *spread_points, int *attractive_rows) {
     if (!c) return fail(FITSNE_E_ARG, "null argument");
     if (spread_points) *spread_points = c->spread_pts;
-    if (attractive_rows) *attractive_rows = c->win_ok ? c->win_R : c->att_rows;
+    if (attractive_rows) *attractive_rows = c->win_ok ? (c->win_bal ? kAttBalanced : c->win_R) : c->att_rows;
     return FITSNE_OK;
 }
 
 extern "C" int fitsne_set_attractive_rows(fitsne_ctx *c, int rows) {
     if (!c) return fail(FITSNE_E_ARG, "null argument");
-    if (rows != 0 && (rows < kWave || rows > kAttRows || rows % kWave != 0))
-        return fail(FITSNE_E_ARG, "attractive rows per workgroup must be 0 (automatic) or a multiple of 64 in [64, 4096]");
+    if (rows != 0 && rows != kAttBalanced && (rows < kWave || rows > kAttRows || rows % kWave != 0))
+        return fail(FITSNE_E_ARG, "attractive rows per workgroup must be 0 (automatic), -1 (balanced blocks) or a "
+                                  "multiple of 64 in [64, 4096]");
     HIPTRY(hipSetDevice(c->device));
     TRY(sync_ctx(c));
     if (rows != c->att_rows) {
@@ -1972,38 +2086,48 @@ static BatchLds batch_lds(int Mp) {
     return {sizeof(cx<float>) * (lspan<true>(Mp) + (size_t)Mp), sizeof(cx<float>) * (lspan<true>(2 * Mp) + (size_t)Mp),
             sizeof(cx<float>) * (lspan<true>(4 * Mp) + (size_t)Mp)};
 }
+// Length class CLS of the batched FFT passes: its largest FFT length and the workgroup size of each pass.
+template <int CLS>
+struct BatchClass {
+    static constexpr int MP = CLS == 0 ? kBatchSmallMp : kCapMp2;
+    static constexpr int KT = kKernThreads, RT = kRowThreads, CT = kColThreads;
+};
+static_assert(kColThreads == kColZLanes, "the column pass's Z lanes are the single-context workgroup");
+
 // The five FFT passes of one length class (CLS), each one persistent launch.
-template <int CLS, int MP>
+template <int CLS>
 static void batch_fft_class(fitsne_batch *b, hipStream_t st, const BatchProb *bp, BatchSched *sch, int *pref,
                             int pass, int kind_q, int kind_1) {
+    using C = BatchClass<CLS>;
     const int n = (int)b->ctx.size();
-    const BatchLds L = batch_lds(MP);
+    const BatchLds L = batch_lds(C::MP);
     switch (pass) {
     case kBsKernRows:
-        kb_kern_rows<kKernThreads, MP, CLS><<<dim3(b->g_kern[CLS]), dim3(kKernThreads), L.kern, st>>>(bp, n, sch, pref,
-                                                                                                 kind_q, kind_1, b->df);
+        kb_kern_rows<C::KT, C::MP, CLS><<<dim3(b->g_kern[CLS]), dim3(C::KT), L.kern, st>>>(bp, n, sch, pref, kind_q,
+                                                                                          kind_1, b->df);
         break;
     case kBsKernCols:
-        kb_kern_cols<kKernThreads, MP, CLS><<<dim3(b->g_kern[CLS]), dim3(kKernThreads), L.kern, st>>>(bp, n, sch, pref);
+        kb_kern_cols<C::KT, C::MP, CLS><<<dim3(b->g_kern[CLS]), dim3(C::KT), L.kern, st>>>(bp, n, sch, pref);
         break;
     case kBsRowsFwd:
-        kb_rows_fwd<kRowThreads, 2 * MP, CLS><<<dim3(b->g_rows[CLS]), dim3(kRowThreads), L.rows, st>>>(bp, n, sch, pref);
+        kb_rows_fwd<C::RT, 2 * C::MP, CLS><<<dim3(b->g_rows[CLS]), dim3(C::RT), L.rows, st>>>(bp, n, sch, pref);
         break;
     case kBsCols:
-        kb_cols<kColThreads, 4 * MP, CLS><<<dim3(b->g_cols[CLS]), dim3(kColThreads), L.cols, st>>>(bp, n, sch, pref);
+        kb_cols<C::CT, 4 * C::MP, CLS><<<dim3(b->g_cols[CLS]), dim3(C::CT), L.cols, st>>>(bp, n, sch, pref);
         break;
     default:
-        kb_rows_inv<kRowThreads, 2 * MP, CLS><<<dim3(b->g_rows[CLS]), dim3(kRowThreads), L.rows, st>>>(bp, n, sch, pref);
+        kb_rows_inv<C::RT, 2 * C::MP, CLS><<<dim3(b->g_rows[CLS]), dim3(C::RT), L.rows, st>>>(bp, n, sch, pref);
         break;
     }
 }
 
-template <int CLS, int MP>
+template <int CLS>
 static int batch_class_blocks(fitsne_batch *b, int cus) {
-    const BatchLds L = batch_lds(MP);
-    TRY(persistent_blocks(kb_kern_rows<kKernThreads, MP, CLS>, kKernThreads, L.kern, cus, &b->g_kern[CLS]));
-    TRY(persistent_blocks(kb_rows_fwd<kRowThreads, 2 * MP, CLS>, kRowThreads, L.rows, cus, &b->g_rows[CLS]));
-    TRY(persistent_blocks(kb_cols<kColThreads, 4 * MP, CLS>, kColThreads, L.cols, cus, &b->g_cols[CLS]));
+    using C = BatchClass<CLS>;
+    const BatchLds L = batch_lds(C::MP);
+    TRY(persistent_blocks(kb_kern_rows<C::KT, C::MP, CLS>, C::KT, L.kern, cus, &b->g_kern[CLS]));
+    TRY(persistent_blocks(kb_rows_fwd<C::RT, 2 * C::MP, CLS>, C::RT, L.rows, cus, &b->g_rows[CLS]));
+    TRY(persistent_blocks(kb_cols<C::CT, 4 * C::MP, CLS>, C::CT, L.cols, cus, &b->g_cols[CLS]));
     return FITSNE_OK;
 }
 
@@ -2025,8 +2149,8 @@ static int batch_capture_p(fitsne_batch *b, const BatchProb *bp, BatchSched *sch
                                                                       (float)(0.5 * std::sqrt(b->df)));
     HIPTRY(hipEventRecord(b->join_ev, b->astream));
     auto passes = [&](int pass) {
-        batch_fft_class<0, kBatchSmallMp>(b, st, bp, sch, pref, pass, kind_q, kind_1);
-        batch_fft_class<1, kCapMp2>(b, st, bp, sch, pref, pass, kind_q, kind_1);
+        batch_fft_class<0>(b, st, bp, sch, pref, pass, kind_q, kind_1);
+        batch_fft_class<1>(b, st, bp, sch, pref, pass, kind_q, kind_1);
     };
     passes(kBsKernRows);
     passes(kBsKernCols);
@@ -2124,8 +2248,8 @@ static int batch_prepare(fitsne_batch *b) {
     HIPTRY(hipStreamSynchronize(b->stream));  // the host vectors go out of scope
     int cus = 0;
     HIPTRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device));
-    TRY((batch_class_blocks<0, kBatchSmallMp>(b, cus)));
-    TRY((batch_class_blocks<1, kCapMp2>(b, cus)));
+    TRY(batch_class_blocks<0>(b, cus));
+    TRY(batch_class_blocks<1>(b, cus));
     const BatchProb *dbp = b->dprob.as<const BatchProb>();
     auto *sch = b->dsched.as<BatchSched>();
     int *pref = b->dpref.as<int>();

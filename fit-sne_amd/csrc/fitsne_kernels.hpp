@@ -205,12 +205,14 @@ __device__ __forceinline__ void window_groups(bool valid, int w, F &&f) {
 __global__ __launch_bounds__(kAttThreads) void k_win_count(const int64_t *__restrict__ rowP,
                                                            const uint32_t *__restrict__ colP,
                                                            const float *__restrict__ val, int N, int R, int NW,
+                                                           const int *__restrict__ rstart,
                                                            int *__restrict__ count, int *__restrict__ pure,
                                                            float *__restrict__ bmax) {
     extern __shared__ int hist[];  // [0, NW): pure entries, [NW, 2 NW): mixed entries
     __shared__ unsigned int smax;
     const int b = blockIdx.x, t = threadIdx.x, lane = t % kWave, wv = t / kWave;
-    const int r0 = b * R, r1 = min(N, r0 + R);
+    // row block b: [b R, (b + 1) R), or [rstart[b], rstart[b + 1]) for a balanced partition
+    const int r0 = rstart ? rstart[b] : b * R, r1 = rstart ? rstart[b + 1] : min(N, r0 + R);
     for (int w = t; w < 2 * NW; w += kAttThreads) hist[w] = 0;
     if (t == 0) smax = 0u;
     __syncthreads();
@@ -260,12 +262,13 @@ __global__ __launch_bounds__(1024) void k_bmax_all(float *__restrict__ bmax, int
 __global__ __launch_bounds__(kAttThreads) void k_win_fill(const int64_t *__restrict__ rowP,
                                                           const uint32_t *__restrict__ colP,
                                                           const float *__restrict__ val, int N, int R, int NW,
+                                                          const int *__restrict__ rstart,
                                                           const int *__restrict__ start,
                                                           const int *__restrict__ pure, uint32_t *__restrict__ key,
                                                           float *__restrict__ wval) {
     extern __shared__ int cur[];  // [0, NW): pure cursors, [NW, 2 NW): mixed cursors
     const int b = blockIdx.x, t = threadIdx.x, lane = t % kWave, wv = t / kWave;
-    const int r0 = b * R, r1 = min(N, r0 + R);
+    const int r0 = rstart ? rstart[b] : b * R, r1 = rstart ? rstart[b + 1] : min(N, r0 + R);
     for (int w = t; w < 2 * NW; w += kAttThreads) cur[w] = 0;
     __syncthreads();
     const int *st = start + (long)b * NW;
@@ -393,18 +396,19 @@ template <int D>
 __device__ __forceinline__ void attr_win_block(const int *__restrict__ start, const int *__restrict__ pure,
                                                const uint32_t *__restrict__ key, const float *__restrict__ wval,
                                                const float *__restrict__ bmax, const float *__restrict__ Y, int N,
-                                               int R, int NW, float inv_df, int df_one, float mdf, float scale,
-                                               float *__restrict__ pos, int b, AttrLds<D> &L) {
+                                               int R, int NW, const int *__restrict__ rstart, int NB, float inv_df,
+                                               int df_one, float mdf, float scale, float *__restrict__ pos, int b,
+                                               AttrLds<D> &L) {
     constexpr int WL = kWin * D / kAttThreads;  // window floats per thread
     float *yw = L.yw, *yr = L.yr;
     unsigned long long *acc = L.acc;
     const int t = threadIdx.x;
-    const int r0 = b * R, nr = min(R, N - r0);
+    const int r0 = rstart ? rstart[b] : b * R, nr = rstart ? rstart[b + 1] - r0 : min(R, N - r0);
     for (int q = t; q < nr * D; q += kAttThreads) {
         yr[q] = Y[(long)r0 * D + q];
         acc[q] = 0ull;
     }
-    const double bound = (double)bmax[(N + R - 1) / R] * mdf;  // the maximum over all row blocks
+    const double bound = (double)bmax[NB] * mdf;  // the maximum over all NB row blocks
     int e = 0;
     if (bound > 0.0) (void)frexp(bound, &e);  // bound < 2^e
     const double S = ldexp(1.0, 50 - e), invS = ldexp(1.0, e - 50);
@@ -545,16 +549,17 @@ __global__ __launch_bounds__(kAttThreads) __attribute__((amdgpu_waves_per_eu(D =
                                                           const float *__restrict__ wval,
                                                           const float *__restrict__ bmax,
                                                           const float *__restrict__ Y, int N, int R, int NW,
+                                                          const int *__restrict__ rstart, int nblk,
                                                           float inv_df, int df_one, float mdf,
                                                           const StepDev *__restrict__ sd, float exag,
                                                           const DevGeo *__restrict__ dg, float *__restrict__ pos) {
     if (!dg->live) return;
     const float scale = exag >= 0.f ? exag : sd->exag;  // exag < 0: the step record's (captured steps)
     __shared__ AttrLds<D> L;
-    const int nblk = (N + R - 1) / R;
     const int b = xcd_row(blockIdx.x, nblk);
     if (b >= nblk) return;
-    attr_win_block<D>(start, pure, key, wval, bmax, Y, N, R, NW, inv_df, df_one, mdf, scale, pos, b, L);
+    attr_win_block<D>(start, pure, key, wval, bmax, Y, N, R, NW, rstart, nblk, inv_df, df_one, mdf, scale, pos, b,
+                      L);
 }
 
 // ---------------------------------------------------------------- min / max
@@ -1270,6 +1275,30 @@ __global__ __launch_bounds__(NTHR) void k_rows_fwd(const DevGeo *__restrict__ dg
 //   V1 = K_q A,  G_x = K_x A - K_q b_x,  G_y = K_y A - K_q b_y    (inverse-transformed over u)
 // and the Parseval sums of Z (k weighted 1 at k = 0, Mp/2 and 2 elsewhere):
 //   z0 = sum K_1 |A|^2, z1 = sum Re(conj(c) K_q A), z2 = sum Re(conj(b).K_D A), z3 = sum K_q |b|^2.
+// Lanes of the column pass's Z sums (the single-context pass's workgroup size).
+constexpr int kColZLanes = 512;
+
+// Sum over NTHR * VPT virtual lanes (lane t + j NTHR holds v[j] of thread t), in the order block_sum
+// takes with that many threads: each wavefront of lanes by shuffles, then the wavefront totals in lane
+// order from 0.0 (result valid in thread 0).  sh: VPT NTHR / kWave doubles.
+template <int NTHR, int VPT>
+__device__ __forceinline__ double lanes_sum(const double (&v)[VPT], double wk, double *sh) {
+    constexpr int NW = NTHR / kWave;
+    const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
+    double t[VPT];
+#pragma unroll
+    for (int j = 0; j < VPT; j++) t[j] = wave_sum(v[j] * wk);
+    __syncthreads();
+    if (l == 0)
+#pragma unroll
+        for (int j = 0; j < VPT; j++) sh[j * NW + w] = t[j];
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < VPT * NW; i++) r += sh[i];
+    return r;
+}
+
 template <int NTHR, int MAXTOT>
 __device__ __forceinline__ void cols_f32_line(const DevGeo *__restrict__ dg, cx<float> *buf, const cx<float> *twl,
                                               const cx<float> *__restrict__ spec, const float *__restrict__ KQ,
@@ -1286,8 +1315,16 @@ __device__ __forceinline__ void cols_f32_line(const DevGeo *__restrict__ dg, cx<
     __syncthreads();
     fft_lines<float, NTHR, MAXTOT, true>(buf, dg->pl, 4, Mp, twl, false);
     const double wk = (k == 0 || 2 * k == Mp) ? 1.0 : 2.0;
-    double z0 = 0, z1 = 0, z2 = 0, z3 = 0;
-    for (int m = threadIdx.x; m < Mp; m += NTHR) {
+    // Z partial sums in the order of kColZLanes lanes whatever the workgroup size: thread t stands for
+    // lanes t + j NTHR, each summing m = lane, lane + kColZLanes, ...
+    constexpr int VPT = kColZLanes / NTHR;
+    static_assert(VPT * NTHR == kColZLanes && NTHR % kWave == 0, "column pass: threads must divide the Z lanes");
+    double z0[VPT], z1[VPT], z2[VPT], z3[VPT];
+#pragma unroll
+    for (int j = 0; j < VPT; j++) z0[j] = z1[j] = z2[j] = z3[j] = 0.0;
+#pragma unroll
+    for (int j = 0; j < VPT; j++)
+    for (int m = threadIdx.x + j * NTHR; m < Mp; m += kColZLanes) {
         const int mb = m <= Mp / 2 ? m : Mp - m;
         const float sg = m <= Mp / 2 ? 1.f : -1.f;
         const float cq = KQ[(long)k * H + mb], c1 = K1[(long)k * H + mb];
@@ -1299,12 +1336,12 @@ __device__ __forceinline__ void cols_f32_line(const DevGeo *__restrict__ dg, cx<
         buf[lp<true>(Mp + m)] = KxA - cq * BX;
         buf[lp<true>(2 * Mp + m)] = KyA - cq * BY;
         const double a2 = (double)A.x * A.x + (double)A.y * A.y;
-        z0 += (double)c1 * a2;
-        z1 += (double)cq * ((double)C.x * A.x + (double)C.y * A.y);
+        z0[j] += (double)c1 * a2;
+        z1[j] += (double)cq * ((double)C.x * A.x + (double)C.y * A.y);
         // Re(conj(B) (-i s) A) = s Im(conj(B) A)
-        z2 += (double)sx * ((double)BX.x * A.y - (double)BX.y * A.x) +
-              (double)sy * ((double)BY.x * A.y - (double)BY.y * A.x);
-        z3 += (double)cq * ((double)BX.x * BX.x + (double)BX.y * BX.y + (double)BY.x * BY.x + (double)BY.y * BY.y);
+        z2[j] += (double)sx * ((double)BX.x * A.y - (double)BX.y * A.x) +
+                 (double)sy * ((double)BY.x * A.y - (double)BY.y * A.x);
+        z3[j] += (double)cq * ((double)BX.x * BX.x + (double)BX.y * BX.y + (double)BY.x * BY.x + (double)BY.y * BY.y);
     }
     __syncthreads();
     fft_lines<float, NTHR, MAXTOT, true>(buf, dg->pl, 3, Mp, twl, true);
@@ -1312,15 +1349,15 @@ __device__ __forceinline__ void cols_f32_line(const DevGeo *__restrict__ dg, cx<
         const int t = q / n1d, u = q - t * n1d;
         ospec[t * plane + (long)k * n1d + u] = buf[lp<true>(t * Mp + u)];
     }
-    z0 = block_sum(z0 * wk, shz);
-    z1 = block_sum(z1 * wk, shz);
-    z2 = block_sum(z2 * wk, shz);
-    z3 = block_sum(z3 * wk, shz);
+    const double s0 = lanes_sum<NTHR, VPT>(z0, wk, shz);
+    const double s1 = lanes_sum<NTHR, VPT>(z1, wk, shz);
+    const double s2 = lanes_sum<NTHR, VPT>(z2, wk, shz);
+    const double s3 = lanes_sum<NTHR, VPT>(z3, wk, shz);
     if (threadIdx.x == 0) {
-        zpart[k * 4 + 0] = z0;
-        zpart[k * 4 + 1] = z1;
-        zpart[k * 4 + 2] = z2;
-        zpart[k * 4 + 3] = z3;
+        zpart[k * 4 + 0] = s0;
+        zpart[k * 4 + 1] = s1;
+        zpart[k * 4 + 2] = s2;
+        zpart[k * 4 + 3] = s3;
     }
     __syncthreads();
 }
@@ -1335,7 +1372,7 @@ __global__ __launch_bounds__(NTHR) void k_cols_f32(const DevGeo *__restrict__ dg
     const Geo &g = dg->g;
     extern __shared__ unsigned char smem_raw[];
     cx<float> *buf = reinterpret_cast<cx<float> *>(smem_raw);
-    __shared__ double shz[NTHR / kWave];
+    __shared__ double shz[kColZLanes / kWave];
     const int Mp = g.Mp, H = g.H;
     const cx<float> *twl = stage_twiddles(buf + lspan<true>(4 * Mp), tw, Mp);
     const int npad = 8 * ((H + 7) / 8);

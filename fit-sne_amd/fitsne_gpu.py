@@ -451,7 +451,7 @@ class Context:
         return a.value, b.value
 
     def set_attractive_rows(self, rows):
-        """Rows per attractive workgroup (0: automatic); results do not depend on it."""
+        """Rows per attractive workgroup (0: automatic, -1: balanced blocks); results do not depend on it."""
         check(lib().fitsne_set_attractive_rows(self.h, int(rows)))
 
     def set_spread_points(self, points):

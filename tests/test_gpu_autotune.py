@@ -56,12 +56,13 @@ def test_spread_workgroup_size_does_not_change_results(D, N):
 
 @pytest.mark.parametrize("D,N", [(2, 30_000), (1, 20_000)])
 def test_attractive_row_block_does_not_change_results(D, N):
-    """The attractive term's rows per workgroup change only which workgroup sums which rows: with one
-    fixed-point scale for all rows the forces, and so every step, are bitwise the same."""
+    """The attractive term's rows per workgroup, fixed or balanced by the cost model (-1), change only
+    which workgroup sums which rows: with one fixed-point scale for all rows the forces, and so every
+    step, are bitwise the same."""
     Y = blobs(N, D, 6, seed=41 + D, span=40.0)
     P = random_csr(N, 15, seed=6)
     out = []
-    for rows in (0, 64, 1024, 4096):
+    for rows in (0, 64, 1024, 4096, -1):
         c = fg.Context(no_dims=D, precision=fg.FP32)
         c.set_P(*P)
         c.set_Y(Y)
@@ -75,7 +76,7 @@ def test_attractive_row_block_does_not_change_results(D, N):
         np.testing.assert_array_equal(g, out[0][0])
         assert z == out[0][1]
         np.testing.assert_array_equal(y, out[0][2])
-    assert [o[3] for o in out[1:]] == [64, 1024, 4096]
+    assert [o[3] for o in out[1:]] == [64, 1024, 4096, -1]
 
 
 def test_spread_points_are_validated():
@@ -86,3 +87,24 @@ def test_spread_points_are_validated():
     with pytest.raises(fg.FitsneError):
         c.autotune()  # no P and Y yet
     c.close()
+
+
+def test_balanced_row_blocks_at_scale():
+    """Blocks cut by the cost model (-1) at a size where they differ from the fixed ones (up to 4096 rows
+    each, one per CU): the gradient and Z are bitwise those of the automatic row blocks."""
+    N = 400_000
+    Y = blobs(N, 2, 8, seed=51, span=60.0)
+    P = random_csr(N, 20, seed=7)
+    print(f"[autotune test] balanced blocks N={N}", flush=True)
+    res = []
+    for rows in (0, -1):
+        c = fg.Context(no_dims=2, precision=fg.FP32)
+        c.set_P(*P)
+        c.set_Y(Y)
+        c.set_attractive_rows(rows)
+        g, z = c.gradient()
+        res.append((g, z, c.tuned()[1]))
+        c.close()
+    np.testing.assert_array_equal(res[1][0], res[0][0])
+    assert res[1][1] == res[0][1]
+    assert res[1][2] == -1
