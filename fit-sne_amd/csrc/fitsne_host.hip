@@ -886,9 +886,10 @@ static int launch_step_dev(fitsne_ctx *c) {
 // ============================================================================ autotuner
 // Points per spread workgroup, chosen by timing: each candidate runs the spread on the context's current
 // Y and geometry on its own (best of three launches), and the fastest is kept.  The grid, and so every
-// result, is the same for every candidate (the spread's waves are the same aligned groups of 64 points);
-// only the time differs: small problems want more, smaller workgroups, large ones fewer table flushes.
-constexpr int kSpreadPtsCand[] = {256, 512, 1024, 2048};
+// result, is the same for every candidate (the spread's waves are the same aligned groups of 128
+// points); only the time differs: small problems want more, smaller workgroups, large ones fewer table
+// flushes.
+constexpr int kSpreadPtsCand[] = {512, 1024, 2048, 4096};
 
 template <int D>
 static int tune_spread(fitsne_ctx *c) {
@@ -1966,8 +1967,8 @@ extern "C" int fitsne_set_attractive_rows(fitsne_ctx *c, int rows) {
 
 extern "C" int fitsne_set_spread_points(fitsne_ctx *c, int points) {
     if (!c) return fail(FITSNE_E_ARG, "null argument");
-    if (points < kSpreadThreads || points > 8192 || points % kSpreadThreads != 0)
-        return fail(FITSNE_E_ARG, "spread points per workgroup must be a multiple of 256 in [256, 8192]");
+    if (points < 2 * kSpreadThreads || points > 8192 || points % (2 * kSpreadThreads) != 0)
+        return fail(FITSNE_E_ARG, "spread points per workgroup must be a multiple of 512 in [512, 8192]");
     HIPTRY(hipSetDevice(c->device));
     TRY(sync_ctx(c));
     if (points != c->spread_pts) {

@@ -848,16 +848,18 @@ __global__ __launch_bounds__(1024) void k_scan_counts(int *__restrict__ count, i
 }
 
 // ---------------------------------------------------------------- spread, FP32 form
-// A workgroup takes `pts` consecutive points (a multiple of kSpreadThreads; memory order; spatially
-// ordered points keep this to a few dozen boxes).  Each lane finds its point's box and computes the node-relative charges
+// A workgroup takes `pts` consecutive points (a multiple of 2 kSpreadThreads; memory order; spatially
+// ordered points keep this to a few dozen boxes).  Each lane takes two consecutive points, finds their
+// boxes and computes the node-relative charges
 //   A = L,  c = L |e|^2,  b = L e    (e = y - node, in box widths)
-// for the p^D nodes of its box.  Runs of consecutive lanes in one box are summed in FP32 by a segmented
-// scan in the VALU (DPP; a fixed order over fixed points), each run's sums are scaled by g.fix (2^38 unless the step's bound needs less), rounded
+// for the p^D nodes of each point's box.  Runs of consecutive points in one box are summed in FP32 (a
+// lane's pair, then a segmented scan over the lanes in the VALU: DPP, a fixed order over fixed points),
+// each run's sums are scaled by g.fix (2^38 unless the step's bound needs less), rounded
 // to int64 and added into the workgroup's LDS table slot of its box (open addressing, LDS atomics), and
 // the table goes to the grid with one 64-bit global atomic per (box, node, term).  A run whose box finds
 // no free slot adds to the grid directly.  The sums across runs are exact integers, so the grid, and so
 // the whole gradient, is bitwise repeatable whatever the order in which workgroups and waves run, and
-// whatever `pts`: the waves are the same aligned groups of 64 points for every choice.
+// whatever `pts`: the waves are the same aligned groups of 128 points for every choice.
 // Grid layout: int64 [u][v][4] (A, c, b_x, b_y), u the x node (nbodyfft.cpp:137-138); the row FFT
 // reads it, converts it and clears it for the next iteration.
 constexpr int kSpreadThreads = 256;
@@ -907,8 +909,64 @@ struct SpreadLds {
     unsigned long long tval[NS][NV * NQ];
 };
 
+// Charges of one point at its box: the NV x NQ values of its Lagrange weights (2D: A, c, b_x, b_y per
+// node; 1D: A, c, b per node), and its box number.
+template <int P, int D>
+__device__ __forceinline__ int point_charges(const float (&y)[D], const Geo &g, float (&val)[D == 2 ? P * P : P][D == 2 ? 4 : 3]) {
+    double tx;
+    int box = box_coord<D>((double)y[0], g, tx);
+    float Lx[P], dx[P];
+#pragma unroll
+    for (int a = 0; a < P; a++) dx[a] = (float)(tx - g.sp[a]);
+#pragma unroll
+    for (int a = 0; a < P; a++) {
+        float v = 1.f;
+#pragma unroll
+        for (int c = 0; c < P; c++)
+            if (c != a) v *= dx[c];
+        Lx[a] = v * g.rdenf[a];
+    }
+    if constexpr (D == 2) {
+        double ty;
+        box += box_coord<D>((double)y[1], g, ty) * g.nb;
+        float Ly[P], dy[P];
+#pragma unroll
+        for (int a = 0; a < P; a++) dy[a] = (float)(ty - g.sp[a]);
+#pragma unroll
+        for (int a = 0; a < P; a++) {
+            float v = 1.f;
+#pragma unroll
+            for (int c = 0; c < P; c++)
+                if (c != a) v *= dy[c];
+            Ly[a] = v * g.rdenf[a];
+        }
+#pragma unroll
+        for (int a = 0; a < P; a++)
+#pragma unroll
+            for (int c = 0; c < P; c++) {
+                const float w = Lx[a] * Ly[c];
+                val[a * P + c][0] = w;
+                val[a * P + c][1] = w * (dx[a] * dx[a] + dy[c] * dy[c]);
+                val[a * P + c][2] = w * dx[a];
+                val[a * P + c][3] = w * dy[c];
+            }
+    } else {
+#pragma unroll
+        for (int a = 0; a < P; a++) {
+            val[a][0] = Lx[a];
+            val[a][1] = Lx[a] * dx[a] * dx[a];
+            val[a][2] = Lx[a] * dx[a];
+        }
+    }
+    return box;
+}
+
 // Workgroup `bid` of the spread (points [bid pts, (bid + 1) pts)); the body of k_spread_f32 and of its
-// batched form.
+// batched form.  Each lane takes two consecutive points, a and b, so a wavefront covers 128 points and
+// its scan and flush serve twice as many points.  A lane's running sum is that of b's box: a joins it
+// when both share a box; otherwise a joins the left neighbour's sum when it continues that lane's b
+// box, and is flushed on its own in the rare remaining case (a box holding a alone between two others).
+// Runs of lanes with the same b box are summed by the segmented scan, and each run's last lane flushes.
 template <int P, int D>
 __device__ __forceinline__ void spread_block(const float *__restrict__ Y, int N, int pts, const DevGeo *__restrict__ dg,
                                              unsigned long long *__restrict__ grid, int bid, SpreadLds<P, D> &L) {
@@ -922,94 +980,22 @@ __device__ __forceinline__ void spread_block(const float *__restrict__ Y, int N,
     for (int o = t; o < NS; o += kSpreadThreads) tkey[o] = -1;
     for (int o = t; o < NS * NV * NQ; o += kSpreadThreads) (&tval[0][0])[o] = 0ull;
     __syncthreads();
-    for (int chunk = 0; chunk < pts / kSpreadThreads; chunk++) {
-        const long i = (long)bid * pts + (long)chunk * kSpreadThreads + t;
-        const bool ok = i < N;
-        int b = -1 - lane;  // lanes past N: distinct negative keys, never a real run
-        float val[NV][NQ];
-#pragma unroll
-        for (int v = 0; v < NV; v++)
-#pragma unroll
-            for (int q = 0; q < NQ; q++) val[v][q] = 0.f;
-        if (ok) {
-            float y[D];
-            load_pt<float, D>(Y, i, y);
-            double tx;
-            b = box_coord<D>((double)y[0], g, tx);
-            float Lx[P], dx[P];
-#pragma unroll
-            for (int a = 0; a < P; a++) dx[a] = (float)(tx - g.sp[a]);
-#pragma unroll
-            for (int a = 0; a < P; a++) {
-                float v = 1.f;
-#pragma unroll
-                for (int c = 0; c < P; c++)
-                    if (c != a) v *= dx[c];
-                Lx[a] = v * g.rdenf[a];
-            }
-            if constexpr (D == 2) {
-                double ty;
-                b += box_coord<D>((double)y[1], g, ty) * g.nb;
-                float Ly[P], dy[P];
-#pragma unroll
-                for (int a = 0; a < P; a++) dy[a] = (float)(ty - g.sp[a]);
-#pragma unroll
-                for (int a = 0; a < P; a++) {
-                    float v = 1.f;
-#pragma unroll
-                    for (int c = 0; c < P; c++)
-                        if (c != a) v *= dy[c];
-                    Ly[a] = v * g.rdenf[a];
-                }
-#pragma unroll
-                for (int a = 0; a < P; a++)
-#pragma unroll
-                    for (int c = 0; c < P; c++) {
-                        const float w = Lx[a] * Ly[c];
-                        val[a * P + c][0] = w;
-                        val[a * P + c][1] = w * (dx[a] * dx[a] + dy[c] * dy[c]);
-                        val[a * P + c][2] = w * dx[a];
-                        val[a * P + c][3] = w * dy[c];
-                    }
-            } else {
-#pragma unroll
-                for (int a = 0; a < P; a++) {
-                    val[a][0] = Lx[a];
-                    val[a][1] = Lx[a] * dx[a] * dx[a];
-                    val[a][2] = Lx[a] * dx[a];
-                }
-            }
-        }
-        // runs of equal boxes among consecutive lanes: a segmented inclusive scan over the wave (DPP row
-        // shifts and row broadcasts, in the VALU) leaves each run's sum in its last lane.  The wave's
-        // points and their order are fixed, so the float sums are bitwise repeatable.
-        const int prev = __shfl_up(b, 1, kWave);
-        const bool head = lane == 0 || b != prev;
-        const unsigned long long heads = __ballot(head);
-        const int run = __popcll(heads & ((2ull << lane) - 1ull));  // 1-based run of this lane
-        const bool tail = lane == kWave - 1 || ((heads >> (lane + 1)) & 1ull);
-        seg_scan_step<0x111, 0xf, NV, NQ>(val, run);  // row_shr:1
-        seg_scan_step<0x112, 0xf, NV, NQ>(val, run);  // row_shr:2
-        seg_scan_step<0x114, 0xf, NV, NQ>(val, run);  // row_shr:4
-        seg_scan_step<0x118, 0xf, NV, NQ>(val, run);  // row_shr:8
-        seg_scan_step<0x142, 0xa, NV, NQ>(val, run);  // row_bcast:15 into rows 1 and 3
-        seg_scan_step<0x143, 0xc, NV, NQ>(val, run);  // row_bcast:31 into rows 2 and 3
-        // each run's last lane adds its totals, as g.fix fixed point, into its box's table slot (open
-        // addressing), or into the grid when the table has no free slot (rare: a separate pass)
-        const bool emit = tail && b >= 0;
+    // flush `val` of the lanes with `on` into box `box`: its LDS table slot (open addressing), or the
+    // grid when the table has no free slot (rare: a separate pass)
+    auto flush = [&](bool on, int box, const float (&val)[NV][NQ]) {
         int slot = -1;
-        if (emit) {
-            int h = b % NS;
+        if (on) {
+            int h = box % NS;
             for (int probe = 0; probe < NS; probe++) {
-                const int k = atomicCAS(&tkey[h], -1, b);
-                if (k == -1 || k == b) {
+                const int k = atomicCAS(&tkey[h], -1, box);
+                if (k == -1 || k == box) {
                     slot = h;
                     break;
                 }
                 h = h + 1 == NS ? 0 : h + 1;
             }
         }
-        if (emit && slot >= 0) {
+        if (on && slot >= 0) {
             if (g.fix_fast) {  // uniform: every run sum within the 2^51 range of fix_rn
 #pragma unroll
                 for (int v = 0; v < NV; v++)
@@ -1024,9 +1010,9 @@ __device__ __forceinline__ void spread_block(const float *__restrict__ Y, int N,
                         atomicAdd(&tval[slot][v * NQ + q], (unsigned long long)fix_rn_any((double)val[v][q] * g.fix));
             }
         }
-        if (__ballot(emit && slot < 0)) {
-            if (emit && slot < 0) {
-                const int bx = D == 2 ? b % g.nb : b, by = D == 2 ? b / g.nb : 0;
+        if (__ballot(on && slot < 0)) {
+            if (on && slot < 0) {
+                const int bx = D == 2 ? box % g.nb : box, by = D == 2 ? box / g.nb : 0;
                 for (int v = 0; v < NV; v++)
                     for (int q = 0; q < NQ; q++) {
                         const long node =
@@ -1035,6 +1021,77 @@ __device__ __forceinline__ void spread_block(const float *__restrict__ Y, int N,
                     }
             }
         }
+    };
+    for (int chunk = 0; chunk < pts / (2 * kSpreadThreads); chunk++) {
+        const long i = (long)bid * pts + (long)chunk * 2 * kSpreadThreads + 2 * (long)t;  // point a; b = i + 1
+        float ya[D], yb[D];
+        const bool oka = i < N, okb = i + 1 < N;
+        if (okb) {
+            if constexpr (D == 2) {
+                const float4 v = reinterpret_cast<const float4 *>(Y)[i >> 1];
+                ya[0] = v.x;
+                ya[1] = v.y;
+                yb[0] = v.z;
+                yb[1] = v.w;
+            } else {
+                const float2 v = reinterpret_cast<const float2 *>(Y)[i >> 1];
+                ya[0] = v.x;
+                yb[0] = v.y;
+            }
+        } else if (oka) {
+            load_pt<float, D>(Y, i, ya);
+        }
+        float val[NV][NQ];
+#pragma unroll
+        for (int v = 0; v < NV; v++)
+#pragma unroll
+            for (int q = 0; q < NQ; q++) val[v][q] = 0.f;
+        // boxes (lanes past N: distinct negative keys, never a real box or run)
+        int ka = -1 - 2 * lane, kb = -2 - 2 * lane;
+        if (oka) ka = point_charges<P, D>(ya, g, val);
+        if (okb) {
+            double tb;
+            kb = box_coord<D>((double)yb[0], g, tb);
+            if constexpr (D == 2) kb += box_coord<D>((double)yb[1], g, tb) * g.nb;
+        }
+        const int kb_prev = __shfl_up(kb, 1, kWave), ka_next = __shfl_down(ka, 1, kWave),
+                  kb_next = __shfl_down(kb, 1, kWave);
+        const bool pure = ka == kb;                                          // a is in b's box
+        const bool give = !pure && lane > 0 && ka == kb_prev;                // a continues the left lane's box
+        const bool take = lane < kWave - 1 && ka_next == kb && ka_next != kb_next;  // the right lane's a is ours
+        if (__ballot(oka && !pure && !give)) flush(oka && !pure && !give, ka, val);
+        // a's charges: kept (pure), moved one lane to the left (give / take), or already flushed
+#pragma unroll
+        for (int v = 0; v < NV; v++)
+#pragma unroll
+            for (int q = 0; q < NQ; q++) {
+                const float right = __int_as_float(
+                    __builtin_amdgcn_update_dpp(0, __float_as_int(val[v][q]), 0x130, 0xf, 0xf, false));  // wave_shl:1
+                const float own = pure ? val[v][q] : 0.f;
+                val[v][q] = take ? own + right : own;
+            }
+        if (okb) {
+            float vb[NV][NQ];
+            (void)point_charges<P, D>(yb, g, vb);
+#pragma unroll
+            for (int v = 0; v < NV; v++)
+#pragma unroll
+                for (int q = 0; q < NQ; q++) val[v][q] += vb[v][q];
+        }
+        // runs of equal b boxes among consecutive lanes: a segmented inclusive scan over the wave (DPP row
+        // shifts and row broadcasts, in the VALU) leaves each run's sum in its last lane.  The wave's
+        // points and their order are fixed, so the float sums are bitwise repeatable.
+        const bool head = lane == 0 || kb != kb_prev;
+        const unsigned long long heads = __ballot(head);
+        const int run = __popcll(heads & ((2ull << lane) - 1ull));  // 1-based run of this lane
+        seg_scan_step<0x111, 0xf, NV, NQ>(val, run);  // row_shr:1
+        seg_scan_step<0x112, 0xf, NV, NQ>(val, run);  // row_shr:2
+        seg_scan_step<0x114, 0xf, NV, NQ>(val, run);  // row_shr:4
+        seg_scan_step<0x118, 0xf, NV, NQ>(val, run);  // row_shr:8
+        seg_scan_step<0x142, 0xa, NV, NQ>(val, run);  // row_bcast:15 into rows 1 and 3
+        seg_scan_step<0x143, 0xc, NV, NQ>(val, run);  // row_bcast:31 into rows 2 and 3
+        const bool tail = lane == kWave - 1 || kb != kb_next;
+        flush(tail && kb >= 0, kb, val);
     }
     __syncthreads();
     // table -> grid

@@ -455,7 +455,7 @@ class Context:
         check(lib().fitsne_set_attractive_rows(self.h, int(rows)))
 
     def set_spread_points(self, points):
-        """Points per spread workgroup (a multiple of 256 in [256, 8192]); results do not depend on it."""
+        """Points per spread workgroup (a multiple of 512 in [512, 8192]); results do not depend on it."""
         check(lib().fitsne_set_spread_points(self.h, int(points)))
 
     def reorder(self):

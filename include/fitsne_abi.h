@@ -268,7 +268,7 @@ int fitsne_set_graph(fitsne_ctx *ctx, int enable);
 int fitsne_set_overlap(fitsne_ctx *ctx, int enable);
 
 /* Autotuner (no reference counterpart: the reference's spread is a serial loop).  Times, on the
- * context's current Y and geometry, the FP32 spread's points per workgroup (256, 512, 1024, 2048) and
+ * context's current Y and geometry, the FP32 spread's points per workgroup (512, 1024, 2048, 4096) and
  * the attractive term's row blocks (about N / 256, 1024, 2048 or 4096 rows, or blocks of up to 4096 rows
  * balanced by a cost model, one per CU; the windowed copy of P is rebuilt for each), and keeps the
  * fastest of each; every result is bitwise the same for any choice.
@@ -282,7 +282,7 @@ int fitsne_autotune_stats(fitsne_ctx *ctx, int *spread_points, int *attractive_r
  * rows cut so that their modelled times match, one per CU; else a multiple of 64 up to 4096).  The forces
  * are bitwise the same for every value. */
 int fitsne_set_attractive_rows(fitsne_ctx *ctx, int rows);
-/* Set the spread's points per workgroup directly (a multiple of 256 in [256, 8192]; default 1024). */
+/* Set the spread's points per workgroup directly (a multiple of 512 in [512, 8192]; default 1024). */
 int fitsne_set_spread_points(fitsne_ctx *ctx, int points);
 
 /* Batched embeddings (no reference counterpart): one optimisation step of every context of the batch

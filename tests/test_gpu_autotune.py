@@ -1,9 +1,9 @@
 """The autotuner's knobs, the spread's points per workgroup and the attractive term's rows per workgroup,
 leave every result bitwise unchanged.
 
-The spread's waves are the same aligned groups of 64 points for any workgroup size, and everything
+The spread's waves are the same aligned groups of 128 points for any workgroup size, and everything
 after the in-wave sums is exact integer arithmetic, so the FP32 gradient, Z and a run of optimiser
-steps (host-planned and captured) must be identical for 256, 512, 1024 and 2048 points per
+steps (host-planned and captured) must be identical for 512, 1024, 2048 and 4096 points per
 workgroup.  The autotuner must return one of those and leave the results as they were.
 """
 import numpy as np
@@ -40,16 +40,16 @@ def test_spread_workgroup_size_does_not_change_results(D, N):
     P = random_csr(N, 15, seed=5)
     print(f"[autotune test] D={D} N={N}", flush=True)
     _, g0, z0, y0 = _run(P, Y, 1024, D)
-    for pts in (256, 512, 2048):
+    for pts in (512, 2048, 4096):
         _, g, z, y = _run(P, Y, pts, D)
         np.testing.assert_array_equal(g, g0)
         assert z == z0
         np.testing.assert_array_equal(y, y0)
-    _, g, z, y = _run(P, Y, 256, D, graph=True)
+    _, g, z, y = _run(P, Y, 512, D, graph=True)
     _, gr, zr, yr = _run(P, Y, 1024, D, graph=True)
     np.testing.assert_array_equal(y, yr)
     chosen, g, z, y = _run(P, Y, None, D)
-    assert chosen in (256, 512, 1024, 2048)
+    assert chosen in (512, 1024, 2048, 4096)
     np.testing.assert_array_equal(g, g0)
     np.testing.assert_array_equal(y, y0)
 
@@ -81,7 +81,7 @@ def test_attractive_row_block_does_not_change_results(D, N):
 
 def test_spread_points_are_validated():
     c = fg.Context(no_dims=2, precision=fg.FP32)
-    for bad in (0, 100, 300, 16384):
+    for bad in (0, 100, 256, 300, 768, 16384):
         with pytest.raises(fg.FitsneError):
             c.set_spread_points(bad)
     with pytest.raises(fg.FitsneError):
