@@ -44,6 +44,10 @@ enum { kBsKernRows = 0, kBsKernCols, kBsRowsFwd, kBsCols, kBsRowsInv, kBsStages 
 constexpr int kBsClasses = 2;
 constexpr int kBatchSmallMp = 1024;
 __host__ __device__ constexpr int batch_class(int Mp) { return Mp <= kBatchSmallMp ? 0 : 1; }
+// Lines (rows or columns) per work item: as many as the class's buffers hold at FFT length Mp, so that a
+// workgroup's threads and barriers serve several short lines at once and the transposed reads and writes
+// of the row passes touch runs of consecutive rows.
+__host__ __device__ constexpr int batch_item_lines(int Mp) { return Mp <= kBatchSmallMp ? kBatchSmallMp / Mp : 1; }
 
 struct BatchSched {
     int ctr[kBsStages * kBsClasses];    // next item of each pass and class (cleared by kb_sched every step)
@@ -89,7 +93,8 @@ __global__ __launch_bounds__(1024) void kb_sched(const BatchProb *__restrict__ b
         if (p < nprob) {
             const DevGeo *dg = bp[p].dg;
             if (dg->live) {
-                const int n1d = dg->g.n1d, H = dg->g.H;
+                const int nl = batch_item_lines(dg->g.Mp);
+                const int n1d = (dg->g.n1d + nl - 1) / nl, H = (dg->g.H + nl - 1) / nl;  // items
                 const int o = kBsStages * batch_class(dg->g.Mp);
                 if (dg->kspec_stale) {
                     cnt[o + kBsKernRows] = n1d;
@@ -144,7 +149,8 @@ __global__ __launch_bounds__(1024) void kb_sched(const BatchProb *__restrict__ b
 constexpr int kBatchPfLds = 512;
 
 // Workgroup loop of a persistent FFT pass: takes items of stage S until they run out and calls
-// line(problem, line index, problem changed).  Every call of `line` ends with a barrier, so the item
+// line(problem, item index, problem changed).  An item is batch_item_lines consecutive lines; the lines
+// fill the buffer from its start and the twiddles sit after the class's largest buffer (MAXTOT).  Every call of `line` ends with a barrier, so the item
 // slot and the staged twiddles are free again when the next item is taken.  The next item is claimed
 // before the current one is worked, so the counter's round trip overlaps the line; the problem of an
 // item is found in an LDS copy of the stage's prefixes.
@@ -184,8 +190,10 @@ __global__ __launch_bounds__(NTHR) void kb_kern_rows(const BatchProb *__restrict
     batch_items<kBsKernRows + kBsStages * CLS>(sch, pref, nprob, [&](int p, int a, bool fresh) {
         const BatchProb &b = bp[p];
         const int Mp = b.dg->g.Mp;
-        if (fresh) twl = stage_twiddles(buf + lspan<true>(Mp), b.tw, Mp);
-        kern_rows_line<float, NTHR, MAXTOT, true>(b.dg, buf, twl, kind_q, kind_1, df, b.Trow, a);
+        if (fresh) twl = stage_twiddles(buf + lspan<true>(MAXTOT), b.tw, Mp);
+        const int nl = batch_item_lines(Mp), a0 = a * nl;
+        kern_rows_line<float, NTHR, MAXTOT, true>(b.dg, buf, twl, kind_q, kind_1, df, b.Trow, a0,
+                                                  min(nl, b.dg->g.n1d - a0));
     });
 }
 
@@ -198,8 +206,10 @@ __global__ __launch_bounds__(NTHR) void kb_kern_cols(const BatchProb *__restrict
     batch_items<kBsKernCols + kBsStages * CLS>(sch, pref, nprob, [&](int p, int k, bool fresh) {
         const BatchProb &b = bp[p];
         const int Mp = b.dg->g.Mp;
-        if (fresh) twl = stage_twiddles(buf + lspan<true>(Mp), b.tw, Mp);
-        kern_cols_line<float, NTHR, MAXTOT, true>(b.dg, buf, twl, b.Trow, 1, b.KQ, b.K1, b.KS, b.KT, k);
+        if (fresh) twl = stage_twiddles(buf + lspan<true>(MAXTOT), b.tw, Mp);
+        const int nl = batch_item_lines(Mp), k0 = k * nl;
+        kern_cols_line<float, NTHR, MAXTOT, true>(b.dg, buf, twl, b.Trow, 1, b.KQ, b.K1, b.KS, b.KT, k0,
+                                                  min(nl, b.dg->g.H - k0));
     });
 }
 
@@ -212,8 +222,10 @@ __global__ __launch_bounds__(NTHR) void kb_rows_fwd(const BatchProb *__restrict_
     batch_items<kBsRowsFwd + kBsStages * CLS>(sch, pref, nprob, [&](int p, int u, bool fresh) {
         const BatchProb &b = bp[p];
         const int Mp = b.dg->g.Mp;
-        if (fresh) twl = stage_twiddles(buf + lspan<true>(2 * Mp), b.tw, Mp);
-        rows_fwd_line<float, NTHR, MAXTOT, true>(b.dg, buf, twl, (const float *)nullptr, 4, 4, b.spec, b.fixgrid, u);
+        if (fresh) twl = stage_twiddles(buf + lspan<true>(MAXTOT), b.tw, Mp);
+        const int nl = batch_item_lines(Mp), u0 = u * nl;
+        rows_fwd_line<float, NTHR, MAXTOT, true>(b.dg, buf, twl, (const float *)nullptr, 4, 4, b.spec, b.fixgrid, u0,
+                                                 min(nl, b.dg->g.n1d - u0));
     });
 }
 
@@ -222,13 +234,15 @@ __global__ __launch_bounds__(NTHR) void kb_cols(const BatchProb *__restrict__ bp
                                                 BatchSched *__restrict__ sch, const int *__restrict__ pref) {
     extern __shared__ unsigned char smem_raw[];
     cx<float> *buf = reinterpret_cast<cx<float> *>(smem_raw);
-    __shared__ double shz[kColZLanes / kWave];
+    __shared__ double shz[4 * kColZLanes / kWave];
     const cx<float> *twl = nullptr;
     batch_items<kBsCols + kBsStages * CLS>(sch, pref, nprob, [&](int p, int k, bool fresh) {
         const BatchProb &b = bp[p];
         const int Mp = b.dg->g.Mp;
-        if (fresh) twl = stage_twiddles(buf + lspan<true>(4 * Mp), b.tw, Mp);
-        cols_f32_line<NTHR, MAXTOT>(b.dg, buf, twl, b.spec, b.KQ, b.K1, b.KS, b.KT, b.ospec, b.zpart, shz, k);
+        if (fresh) twl = stage_twiddles(buf + lspan<true>(MAXTOT), b.tw, Mp);
+        const int nl = batch_item_lines(Mp), k0 = k * nl;
+        cols_f32_line<NTHR, MAXTOT>(b.dg, buf, twl, b.spec, b.KQ, b.K1, b.KS, b.KT, b.ospec, b.zpart, shz, k0,
+                                    min(nl, b.dg->g.H - k0));
     });
 }
 
@@ -241,8 +255,10 @@ __global__ __launch_bounds__(NTHR) void kb_rows_inv(const BatchProb *__restrict_
     batch_items<kBsRowsInv + kBsStages * CLS>(sch, pref, nprob, [&](int p, int u, bool fresh) {
         const BatchProb &b = bp[p];
         const int Mp = b.dg->g.Mp;
-        if (fresh) twl = stage_twiddles(buf + lspan<true>(2 * Mp), b.tw, Mp);
-        rows_inv_line<float, NTHR, MAXTOT, true>(b.dg, buf, twl, b.ospec, 3, reinterpret_cast<float *>(b.gout), 4, u);
+        if (fresh) twl = stage_twiddles(buf + lspan<true>(MAXTOT), b.tw, Mp);
+        const int nl = batch_item_lines(Mp), u0 = u * nl;
+        rows_inv_line<float, NTHR, MAXTOT, true>(b.dg, buf, twl, b.ospec, 3, reinterpret_cast<float *>(b.gout), 4, u0,
+                                                 min(nl, b.dg->g.n1d - u0));
     });
 }
 

@@ -1148,22 +1148,27 @@ __global__ __launch_bounds__(kThreads) void k_grid_to_f32(const DevGeo *__restri
 // step marks the spectra stale.
 template <typename T, int NTHR, int MAXTOT, bool PAD>
 __device__ __forceinline__ void kern_rows_line(const DevGeo *__restrict__ dg, cx<T> *buf, const cx<T> *twl,
-                                               int kind_q, int kind_1, double df, cx<T> *__restrict__ Trow, int a) {
+                                               int kind_q, int kind_1, double df, cx<T> *__restrict__ Trow, int a,
+                                               int nl = 1) {
+    // rows a .. a + nl - 1 (nl Mp <= MAXTOT), line r of the buffer for row a + r
     const Geo &g = dg->g;
-    const int Mp = g.Mp, n1d = g.n1d;
-    const double du = a * g.h;
-    for (int b = threadIdx.x; b < Mp; b += NTHR) buf[lp<PAD>(b)] = cx<T>{(T)0, (T)0};
+    const int Mp = g.Mp, n1d = g.n1d, H = g.H;
+    for (int b = threadIdx.x; b < nl * Mp; b += NTHR) buf[lp<PAD>(b)] = cx<T>{(T)0, (T)0};
     __syncthreads();
-    for (int b = threadIdx.x; b < n1d; b += NTHR) {
-        const double dv = b * g.h;
+    for (int q = threadIdx.x; q < nl * n1d; q += NTHR) {
+        const int r = q / n1d, b = q - r * n1d;
+        const double du = (a + r) * g.h, dv = b * g.h;
         const double r2 = du * du + dv * dv;
         cx<T> val{kernel_value<T>(kind_q, r2, df), kernel_value<T>(kind_1, r2, df)};
-        buf[lp<PAD>(b)] = val;
-        if (b > 0) buf[lp<PAD>(Mp - b)] = val;
+        buf[lp<PAD>(r * Mp + b)] = val;
+        if (b > 0) buf[lp<PAD>(r * Mp + Mp - b)] = val;
     }
     __syncthreads();
-    fft_lines<T, NTHR, MAXTOT, PAD>(buf, dg->pl, 1, Mp, twl, false);
-    for (int k = threadIdx.x; k < g.H; k += NTHR) Trow[(long)k * n1d + a] = buf[lp<PAD>(k)];
+    fft_lines<T, NTHR, MAXTOT, PAD>(buf, dg->pl, nl, Mp, twl, false);
+    for (int q = threadIdx.x; q < nl * H; q += NTHR) {  // the rows of one frequency are contiguous in Trow
+        const int k = q / nl, r = q - k * nl;
+        Trow[(long)k * n1d + a + r] = buf[lp<PAD>(r * Mp + k)];
+    }
     __syncthreads();
 }
 
@@ -1204,6 +1209,26 @@ __device__ __forceinline__ void lds_fill(V *buf, int total, F &&f) {
     }
 }
 
+// lds_fill with element q stored at slot(q) instead of q (a permutation of [0, total)), so that the loads
+// of consecutive threads can follow the source's order rather than the buffer's.
+template <int NTHR, bool PAD, typename V, typename S, typename F>
+__device__ __forceinline__ void lds_fill_slot(V *buf, int total, S &&slot, F &&f) {
+    constexpr int B = 8;
+    for (int q0 = threadIdx.x; q0 < total; q0 += B * NTHR) {
+        V v[B];
+#pragma unroll
+        for (int s = 0; s < B; s++) {
+            const int q = q0 + s * NTHR;
+            if (q < total) v[s] = f(q);
+        }
+#pragma unroll
+        for (int s = 0; s < B; s++) {
+            const int q = q0 + s * NTHR;
+            if (q < total) buf[lp<PAD>(slot(q))] = v[s];
+        }
+    }
+}
+
 // Kernel spectra, column pass: for column frequency k, the line z(a) = T_q(|a|,k)(1 + a h) + i T_1(|a|,k)
 // (a signed).  Its DFT is C_q(m,k) + i (C_1(m,k) - S(m,k)) with C even and S odd in m, so
 //   C_q = Re, C_1 = (Im(m) + Im(-m)) / 2, S = (Im(-m) - Im(m)) / 2,
@@ -1214,29 +1239,33 @@ __device__ __forceinline__ void lds_fill(V *buf, int total, F &&f) {
 template <typename T, int NTHR, int MAXTOT, bool PAD>
 __device__ __forceinline__ void kern_cols_line(const DevGeo *__restrict__ dg, cx<T> *buf, const cx<T> *twl,
                                                const cx<T> *__restrict__ Trow, int need_s, T *__restrict__ KQ,
-                                               T *__restrict__ K1, T *__restrict__ KS, T *__restrict__ KT, int k) {
+                                               T *__restrict__ K1, T *__restrict__ KS, T *__restrict__ KT, int k,
+                                               int nc = 1) {
+    // columns k .. k + nc - 1 (nc Mp <= MAXTOT), line c of the buffer for column k + c
     const Geo &g = dg->g;
     const int Mp = g.Mp, n1d = g.n1d, H = g.H;
-    lds_fill<NTHR, PAD>(buf, Mp, [&](int a) {
+    lds_fill<NTHR, PAD>(buf, nc * Mp, [&](int q) {
+        const int c = q / Mp, a = q - c * Mp;
         int as;
         if (a < n1d) as = a;
         else if (a > Mp - n1d) as = a - Mp;
         else return cx<T>{(T)0, (T)0};
-        const cx<T> t = Trow[(long)k * n1d + (as < 0 ? -as : as)];
+        const cx<T> t = Trow[(long)(k + c) * n1d + (as < 0 ? -as : as)];
         const T w = need_s ? (T)1 + (T)(as * g.h) : (T)1;
         return cx<T>{t.x * w, t.y};
     });
     __syncthreads();
-    fft_lines<T, NTHR, MAXTOT, PAD>(buf, dg->pl, 1, Mp, twl, false);
-    for (int m = threadIdx.x; m < H; m += NTHR) {
-        const cx<T> z = buf[lp<PAD>(m)];
-        const T im_neg = buf[lp<PAD>(m == 0 ? 0 : Mp - m)].y;
-        KQ[(long)k * H + m] = z.x;
-        K1[(long)k * H + m] = (z.y + im_neg) * (T)0.5;
+    fft_lines<T, NTHR, MAXTOT, PAD>(buf, dg->pl, nc, Mp, twl, false);
+    for (int q = threadIdx.x; q < nc * H; q += NTHR) {
+        const int c = q / H, m = q - c * H, kc = k + c;
+        const cx<T> z = buf[lp<PAD>(c * Mp + m)];
+        const T im_neg = buf[lp<PAD>(c * Mp + (m == 0 ? 0 : Mp - m))].y;
+        KQ[(long)kc * H + m] = z.x;
+        K1[(long)kc * H + m] = (z.y + im_neg) * (T)0.5;
         if (need_s) {
             const T sv = (im_neg - z.y) * (T)0.5;
-            KS[(long)k * H + m] = sv;
-            KT[(long)m * H + k] = sv;
+            KS[(long)kc * H + m] = sv;
+            KT[(long)m * H + kc] = sv;
         }
     }
     __syncthreads();
@@ -1270,15 +1299,17 @@ template <typename T, int NTHR, int MAXTOT, bool PAD>
 __device__ __forceinline__ void rows_fwd_line(const DevGeo *__restrict__ dg, cx<T> *buf, const cx<T> *twl,
                                               const T *__restrict__ grid, int sin, int nterms,
                                               cx<T> *__restrict__ spec, unsigned long long *__restrict__ fixgrid,
-                                              int u) {
+                                              int u0, int nl = 1) {
+    // rows u0 .. u0 + nl - 1 (nl lines Mp <= MAXTOT); buffer line r lines + l for row u0 + r
     const Geo &g = dg->g;
     const int Mp = g.Mp, n1d = g.n1d, H = g.H;
     const int lines = (nterms + 1) / 2;
     const double inv = 1.0 / g.fix;
     const double unit[4] = {inv, inv * g.bw0 * g.bw0, inv * g.bw0, inv * g.bw0};
     const long plane = (long)H * n1d;
-    lds_fill<NTHR, PAD>(buf, lines * Mp, [&](int q) {
-        const int l = q / Mp, v = q - l * Mp;
+    lds_fill<NTHR, PAD>(buf, nl * lines * Mp, [&](int q) {
+        const int ln = q / Mp, v = q - ln * Mp;
+        const int r = ln / lines, l = ln - r * lines, u = u0 + r;
         cx<T> z{(T)0, (T)0};
         if (v < n1d) {
             if (fixgrid) {
@@ -1296,11 +1327,12 @@ __device__ __forceinline__ void rows_fwd_line(const DevGeo *__restrict__ dg, cx<
         return z;
     });
     __syncthreads();
-    fft_lines<T, NTHR, MAXTOT, PAD>(buf, dg->pl, lines, Mp, twl, false);
-    for (int q = threadIdx.x; q < lines * H; q += NTHR) {
-        const int l = q / H, k = q - l * H;
-        const cx<T> Z = buf[lp<PAD>(l * Mp + k)];
-        const cx<T> Zc = conj(buf[lp<PAD>(l * Mp + (k == 0 ? 0 : Mp - k))]);
+    fft_lines<T, NTHR, MAXTOT, PAD>(buf, dg->pl, nl * lines, Mp, twl, false);
+    for (int q = threadIdx.x; q < nl * lines * H; q += NTHR) {  // rows fastest: contiguous in spec
+        const int lk = q / nl, r = q - lk * nl;
+        const int l = lk / H, k = lk - l * H, b = (r * lines + l) * Mp, u = u0 + r;
+        const cx<T> Z = buf[lp<PAD>(b + k)];
+        const cx<T> Zc = conj(buf[lp<PAD>(b + (k == 0 ? 0 : Mp - k))]);
         spec[(2 * l) * plane + (long)k * n1d + u] = (T)0.5 * (Z + Zc);
         if (2 * l + 1 < nterms) spec[(2 * l + 1) * plane + (long)k * n1d + u] = (T)0.5 * mul_negi(Z - Zc);
     }
@@ -1335,86 +1367,97 @@ __global__ __launch_bounds__(NTHR) void k_rows_fwd(const DevGeo *__restrict__ dg
 // Lanes of the column pass's Z sums (the single-context pass's workgroup size).
 constexpr int kColZLanes = 512;
 
-// Sum over NTHR * VPT virtual lanes (lane t + j NTHR holds v[j] of thread t), in the order block_sum
-// takes with that many threads: each wavefront of lanes by shuffles, then the wavefront totals in lane
-// order from 0.0 (result valid in thread 0).  sh: VPT NTHR / kWave doubles.
-template <int NTHR, int VPT>
-__device__ __forceinline__ double lanes_sum(const double (&v)[VPT], double wk, double *sh) {
+// Sums over NTHR * VPT virtual lanes (lane t + j NTHR holds v[i][j] of thread t) of NV values at once, each
+// in the order block_sum takes with that many threads: each wavefront of lanes by shuffles, then the
+// wavefront totals in lane order from 0.0 (results valid in thread 0).  sh: NV VPT NTHR / kWave doubles.
+template <int NTHR, int VPT, int NV>
+__device__ __forceinline__ void lanes_sum(const double (&v)[NV][VPT], double wk, double *sh, double (&out)[NV]) {
     constexpr int NW = NTHR / kWave;
     const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
-    double t[VPT];
+    double t[NV][VPT];
 #pragma unroll
-    for (int j = 0; j < VPT; j++) t[j] = wave_sum(v[j] * wk);
+    for (int i = 0; i < NV; i++)
+#pragma unroll
+        for (int j = 0; j < VPT; j++) t[i][j] = wave_sum(v[i][j] * wk);
     __syncthreads();
     if (l == 0)
 #pragma unroll
-        for (int j = 0; j < VPT; j++) sh[j * NW + w] = t[j];
+        for (int i = 0; i < NV; i++)
+#pragma unroll
+            for (int j = 0; j < VPT; j++) sh[(i * VPT + j) * NW + w] = t[i][j];
     __syncthreads();
-    double r = 0.0;
-    if (threadIdx.x == 0)
-        for (int i = 0; i < VPT * NW; i++) r += sh[i];
-    return r;
+#pragma unroll
+    for (int i = 0; i < NV; i++) {
+        double r = 0.0;
+        if (threadIdx.x == 0)
+            for (int q = 0; q < VPT * NW; q++) r += sh[i * VPT * NW + q];
+        out[i] = r;
+    }
 }
 
+// Column frequencies k .. k + nc - 1 (4 nc Mp <= MAXTOT) of the column pass: 4 lines per column forward
+// (A, c, b_x, b_y), the kernel spectra products into 3 lines per column (written over the inputs, column
+// after column, so that the products of column c sit in lines 3c .. 3c + 2), Parseval's Z terms, and the
+// 3 nc lines inverse.
 template <int NTHR, int MAXTOT>
 __device__ __forceinline__ void cols_f32_line(const DevGeo *__restrict__ dg, cx<float> *buf, const cx<float> *twl,
                                               const cx<float> *__restrict__ spec, const float *__restrict__ KQ,
                                               const float *__restrict__ K1, const float *__restrict__ KS,
                                               const float *__restrict__ KT, cx<float> *__restrict__ ospec,
-                                              double *__restrict__ zpart, double *shz, int k) {
+                                              double *__restrict__ zpart, double *shz, int k0, int nc = 1) {
     const Geo &g = dg->g;
     const int Mp = g.Mp, n1d = g.n1d, H = g.H;
     const long plane = (long)H * n1d;
-    lds_fill<NTHR, true>(buf, 4 * Mp, [&](int q) {
-        const int t = q / Mp, u = q - t * Mp;
-        return u < n1d ? spec[t * plane + (long)k * n1d + u] : cx<float>{0.f, 0.f};
+    lds_fill<NTHR, true>(buf, 4 * nc * Mp, [&](int q) {
+        const int ln = q / Mp, u = q - ln * Mp, c = ln >> 2, t = ln & 3;
+        return u < n1d ? spec[t * plane + (long)(k0 + c) * n1d + u] : cx<float>{0.f, 0.f};
     });
     __syncthreads();
-    fft_lines<float, NTHR, MAXTOT, true>(buf, dg->pl, 4, Mp, twl, false);
-    const double wk = (k == 0 || 2 * k == Mp) ? 1.0 : 2.0;
+    fft_lines<float, NTHR, MAXTOT, true>(buf, dg->pl, 4 * nc, Mp, twl, false);
     // Z partial sums in the order of kColZLanes lanes whatever the workgroup size: thread t stands for
     // lanes t + j NTHR, each summing m = lane, lane + kColZLanes, ...
     constexpr int VPT = kColZLanes / NTHR;
     static_assert(VPT * NTHR == kColZLanes && NTHR % kWave == 0, "column pass: threads must divide the Z lanes");
-    double z0[VPT], z1[VPT], z2[VPT], z3[VPT];
+    for (int c = 0; c < nc; c++) {
+        const int k = k0 + c;
+        const double wk = (k == 0 || 2 * k == Mp) ? 1.0 : 2.0;
+        const int in = 4 * c * Mp, outl = 3 * c * Mp;
+        double z[4][VPT];
 #pragma unroll
-    for (int j = 0; j < VPT; j++) z0[j] = z1[j] = z2[j] = z3[j] = 0.0;
+        for (int j = 0; j < VPT; j++) z[0][j] = z[1][j] = z[2][j] = z[3][j] = 0.0;
 #pragma unroll
-    for (int j = 0; j < VPT; j++)
-    for (int m = threadIdx.x + j * NTHR; m < Mp; m += kColZLanes) {
-        const int mb = m <= Mp / 2 ? m : Mp - m;
-        const float sg = m <= Mp / 2 ? 1.f : -1.f;
-        const float cq = KQ[(long)k * H + mb], c1 = K1[(long)k * H + mb];
-        const float sx = sg * KS[(long)k * H + mb], sy = KT[(long)k * H + mb];
-        const cx<float> A = buf[lp<true>(m)], C = buf[lp<true>(Mp + m)], BX = buf[lp<true>(2 * Mp + m)],
-                        BY = buf[lp<true>(3 * Mp + m)];
-        const cx<float> KxA = mul_negi(sx * A), KyA = mul_negi(sy * A);
-        buf[lp<true>(m)] = cq * A;
-        buf[lp<true>(Mp + m)] = KxA - cq * BX;
-        buf[lp<true>(2 * Mp + m)] = KyA - cq * BY;
-        const double a2 = (double)A.x * A.x + (double)A.y * A.y;
-        z0[j] += (double)c1 * a2;
-        z1[j] += (double)cq * ((double)C.x * A.x + (double)C.y * A.y);
-        // Re(conj(B) (-i s) A) = s Im(conj(B) A)
-        z2[j] += (double)sx * ((double)BX.x * A.y - (double)BX.y * A.x) +
-                 (double)sy * ((double)BY.x * A.y - (double)BY.y * A.x);
-        z3[j] += (double)cq * ((double)BX.x * BX.x + (double)BX.y * BX.y + (double)BY.x * BY.x + (double)BY.y * BY.y);
+        for (int j = 0; j < VPT; j++)
+        for (int m = threadIdx.x + j * NTHR; m < Mp; m += kColZLanes) {
+            const int mb = m <= Mp / 2 ? m : Mp - m;
+            const float sg = m <= Mp / 2 ? 1.f : -1.f;
+            const float cq = KQ[(long)k * H + mb], c1 = K1[(long)k * H + mb];
+            const float sx = sg * KS[(long)k * H + mb], sy = KT[(long)k * H + mb];
+            const cx<float> A = buf[lp<true>(in + m)], C = buf[lp<true>(in + Mp + m)],
+                            BX = buf[lp<true>(in + 2 * Mp + m)], BY = buf[lp<true>(in + 3 * Mp + m)];
+            const cx<float> KxA = mul_negi(sx * A), KyA = mul_negi(sy * A);
+            buf[lp<true>(outl + m)] = cq * A;
+            buf[lp<true>(outl + Mp + m)] = KxA - cq * BX;
+            buf[lp<true>(outl + 2 * Mp + m)] = KyA - cq * BY;
+            const double a2 = (double)A.x * A.x + (double)A.y * A.y;
+            z[0][j] += (double)c1 * a2;
+            z[1][j] += (double)cq * ((double)C.x * A.x + (double)C.y * A.y);
+            // Re(conj(B) (-i s) A) = s Im(conj(B) A)
+            z[2][j] += (double)sx * ((double)BX.x * A.y - (double)BX.y * A.x) +
+                       (double)sy * ((double)BY.x * A.y - (double)BY.y * A.x);
+            z[3][j] += (double)cq * ((double)BX.x * BX.x + (double)BX.y * BX.y + (double)BY.x * BY.x +
+                                     (double)BY.y * BY.y);
+        }
+        double zs[4];
+        lanes_sum<NTHR, VPT, 4>(z, wk, shz, zs);  // its barriers also order this column's writes before the next
+        if (threadIdx.x == 0)
+#pragma unroll
+            for (int i = 0; i < 4; i++) zpart[k * 4 + i] = zs[i];
     }
     __syncthreads();
-    fft_lines<float, NTHR, MAXTOT, true>(buf, dg->pl, 3, Mp, twl, true);
-    for (int q = threadIdx.x; q < 3 * n1d; q += NTHR) {
-        const int t = q / n1d, u = q - t * n1d;
-        ospec[t * plane + (long)k * n1d + u] = buf[lp<true>(t * Mp + u)];
-    }
-    const double s0 = lanes_sum<NTHR, VPT>(z0, wk, shz);
-    const double s1 = lanes_sum<NTHR, VPT>(z1, wk, shz);
-    const double s2 = lanes_sum<NTHR, VPT>(z2, wk, shz);
-    const double s3 = lanes_sum<NTHR, VPT>(z3, wk, shz);
-    if (threadIdx.x == 0) {
-        zpart[k * 4 + 0] = s0;
-        zpart[k * 4 + 1] = s1;
-        zpart[k * 4 + 2] = s2;
-        zpart[k * 4 + 3] = s3;
+    fft_lines<float, NTHR, MAXTOT, true>(buf, dg->pl, 3 * nc, Mp, twl, true);
+    for (int q = threadIdx.x; q < 3 * nc * n1d; q += NTHR) {
+        const int ln = q / n1d, u = q - ln * n1d, c = ln / 3, t = ln - 3 * c;
+        ospec[t * plane + (long)(k0 + c) * n1d + u] = buf[lp<true>(ln * Mp + u)];
     }
     __syncthreads();
 }
@@ -1429,7 +1472,7 @@ __global__ __launch_bounds__(NTHR) void k_cols_f32(const DevGeo *__restrict__ dg
     const Geo &g = dg->g;
     extern __shared__ unsigned char smem_raw[];
     cx<float> *buf = reinterpret_cast<cx<float> *>(smem_raw);
-    __shared__ double shz[kColZLanes / kWave];
+    __shared__ double shz[4 * kColZLanes / kWave];
     const int Mp = g.Mp, H = g.H;
     const cx<float> *twl = stage_twiddles(buf + lspan<true>(4 * Mp), tw, Mp);
     const int npad = 8 * ((H + 7) / 8);
@@ -1488,14 +1531,20 @@ __global__ __launch_bounds__(NTHR) void k_cols_plain(const DevGeo *__restrict__ 
 template <typename T, int NTHR, int MAXTOT, bool PAD>
 __device__ __forceinline__ void rows_inv_line(const DevGeo *__restrict__ dg, cx<T> *buf, const cx<T> *twl,
                                               const cx<T> *__restrict__ ospec, int nterms, T *__restrict__ out,
-                                              int sout, int u) {
+                                              int sout, int u0, int nl = 1) {
+    // rows u0 .. u0 + nl - 1 (nl lines Mp <= MAXTOT); buffer line r lines + l for row u0 + r.  Element q
+    // of the fill takes rows fastest, so that the reads of consecutive threads are contiguous in ospec.
     const Geo &g = dg->g;
     const int Mp = g.Mp, n1d = g.n1d, H = g.H;
     const T scale = (T)(1.0 / ((double)Mp * Mp));
     const long plane = (long)H * n1d;
     const int lines = (nterms + 1) / 2;
-    lds_fill<NTHR, PAD>(buf, lines * Mp, [&](int q) {
-        const int l = q / Mp, k = q - l * Mp;
+    auto slot = [&](int q) {
+        const int lk = q / nl, r = q - lk * nl, l = lk / Mp, k = lk - l * Mp;
+        return (r * lines + l) * Mp + k;
+    };
+    lds_fill_slot<NTHR, PAD>(buf, nl * lines * Mp, slot, [&](int q) {
+        const int lk = q / nl, r = q - lk * nl, l = lk / Mp, k = lk - l * Mp, u = u0 + r;
         const bool lo = k < H;
         const int kk = lo ? k : Mp - k;
         cx<T> f1 = ospec[(2 * l) * plane + (long)kk * n1d + u];
@@ -1511,11 +1560,11 @@ __device__ __forceinline__ void rows_inv_line(const DevGeo *__restrict__ dg, cx<
         return f1 + mul_i(f2);
     });
     __syncthreads();
-    fft_lines<T, NTHR, MAXTOT, PAD>(buf, dg->pl, lines, Mp, twl, true);
-    for (int q = threadIdx.x; q < lines * n1d; q += NTHR) {
-        const int l = q / n1d, v = q - l * n1d;
-        const cx<T> z = buf[lp<PAD>(l * Mp + v)];
-        T *dst = out + ((long)u * n1d + v) * sout + 2 * l;
+    fft_lines<T, NTHR, MAXTOT, PAD>(buf, dg->pl, nl * lines, Mp, twl, true);
+    for (int q = threadIdx.x; q < nl * lines * n1d; q += NTHR) {
+        const int ln = q / n1d, v = q - ln * n1d, r = ln / lines, l = ln - r * lines;
+        const cx<T> z = buf[lp<PAD>(ln * Mp + v)];
+        T *dst = out + ((long)(u0 + r) * n1d + v) * sout + 2 * l;
         dst[0] = z.x * scale;
         if (2 * l + 1 < nterms) dst[1] = z.y * scale;
     }
