@@ -47,7 +47,14 @@ __host__ __device__ constexpr int batch_class(int Mp) { return Mp <= kBatchSmall
 // Lines (rows or columns) per work item: as many as the class's buffers hold at FFT length Mp, so that a
 // workgroup's threads and barriers serve several short lines at once and the transposed reads and writes
 // of the row passes touch runs of consecutive rows.
-__host__ __device__ constexpr int batch_item_lines(int Mp) { return Mp <= kBatchSmallMp ? kBatchSmallMp / Mp : 1; }
+// Short-line class buffers: lines of up to kBatchItemMp elements in all per item for the kernel-spectrum
+// and row passes, kBatchSmallMp for the column pass (whose four lines per column already fill it; at
+// kBatchItemMp its 82 KB of LDS left one workgroup per CU and the batched step took 1.39 against 1.32 ms).
+constexpr int kBatchItemMp = 2048;
+__host__ __device__ constexpr int batch_item_cap(int S) { return S == kBsCols ? kBatchSmallMp : kBatchItemMp; }
+__host__ __device__ constexpr int batch_item_lines(int S, int Mp) {
+    return Mp <= kBatchSmallMp ? batch_item_cap(S) / Mp : 1;
+}
 
 struct BatchSched {
     int ctr[kBsStages * kBsClasses];    // next item of each pass and class (cleared by kb_sched every step)
@@ -93,16 +100,16 @@ __global__ __launch_bounds__(1024) void kb_sched(const BatchProb *__restrict__ b
         if (p < nprob) {
             const DevGeo *dg = bp[p].dg;
             if (dg->live) {
-                const int nl = batch_item_lines(dg->g.Mp);
-                const int n1d = (dg->g.n1d + nl - 1) / nl, H = (dg->g.H + nl - 1) / nl;  // items
-                const int o = kBsStages * batch_class(dg->g.Mp);
+                const int Mp = dg->g.Mp, n1d = dg->g.n1d, H = dg->g.H;
+                auto items = [&](int S, int lines) { const int nl = batch_item_lines(S, Mp); return (lines + nl - 1) / nl; };
+                const int o = kBsStages * batch_class(Mp);
                 if (dg->kspec_stale) {
-                    cnt[o + kBsKernRows] = n1d;
-                    cnt[o + kBsKernCols] = H;
+                    cnt[o + kBsKernRows] = items(kBsKernRows, n1d);
+                    cnt[o + kBsKernCols] = items(kBsKernCols, H);
                 }
-                cnt[o + kBsRowsFwd] = n1d;
-                cnt[o + kBsCols] = H;
-                cnt[o + kBsRowsInv] = n1d;
+                cnt[o + kBsRowsFwd] = items(kBsRowsFwd, n1d);
+                cnt[o + kBsCols] = items(kBsCols, H);
+                cnt[o + kBsRowsInv] = items(kBsRowsInv, n1d);
             }
         }
         int inc[NS];
@@ -191,7 +198,7 @@ __global__ __launch_bounds__(NTHR) void kb_kern_rows(const BatchProb *__restrict
         const BatchProb &b = bp[p];
         const int Mp = b.dg->g.Mp;
         if (fresh) twl = stage_twiddles(buf + lspan<true>(MAXTOT), b.tw, Mp);
-        const int nl = batch_item_lines(Mp), a0 = a * nl;
+        const int nl = batch_item_lines(kBsKernRows, Mp), a0 = a * nl;
         kern_rows_line<float, NTHR, MAXTOT, true>(b.dg, buf, twl, kind_q, kind_1, df, b.Trow, a0,
                                                   min(nl, b.dg->g.n1d - a0));
     });
@@ -207,7 +214,7 @@ __global__ __launch_bounds__(NTHR) void kb_kern_cols(const BatchProb *__restrict
         const BatchProb &b = bp[p];
         const int Mp = b.dg->g.Mp;
         if (fresh) twl = stage_twiddles(buf + lspan<true>(MAXTOT), b.tw, Mp);
-        const int nl = batch_item_lines(Mp), k0 = k * nl;
+        const int nl = batch_item_lines(kBsKernCols, Mp), k0 = k * nl;
         kern_cols_line<float, NTHR, MAXTOT, true>(b.dg, buf, twl, b.Trow, 1, b.KQ, b.K1, b.KS, b.KT, k0,
                                                   min(nl, b.dg->g.H - k0));
     });
@@ -223,7 +230,7 @@ __global__ __launch_bounds__(NTHR) void kb_rows_fwd(const BatchProb *__restrict_
         const BatchProb &b = bp[p];
         const int Mp = b.dg->g.Mp;
         if (fresh) twl = stage_twiddles(buf + lspan<true>(MAXTOT), b.tw, Mp);
-        const int nl = batch_item_lines(Mp), u0 = u * nl;
+        const int nl = batch_item_lines(kBsRowsFwd, Mp), u0 = u * nl;
         rows_fwd_line<float, NTHR, MAXTOT, true>(b.dg, buf, twl, (const float *)nullptr, 4, 4, b.spec, b.fixgrid, u0,
                                                  min(nl, b.dg->g.n1d - u0));
     });
@@ -240,7 +247,7 @@ __global__ __launch_bounds__(NTHR) void kb_cols(const BatchProb *__restrict__ bp
         const BatchProb &b = bp[p];
         const int Mp = b.dg->g.Mp;
         if (fresh) twl = stage_twiddles(buf + lspan<true>(MAXTOT), b.tw, Mp);
-        const int nl = batch_item_lines(Mp), k0 = k * nl;
+        const int nl = batch_item_lines(kBsCols, Mp), k0 = k * nl;
         cols_f32_line<NTHR, MAXTOT>(b.dg, buf, twl, b.spec, b.KQ, b.K1, b.KS, b.KT, b.ospec, b.zpart, shz, k0,
                                     min(nl, b.dg->g.H - k0));
     });
@@ -256,7 +263,7 @@ __global__ __launch_bounds__(NTHR) void kb_rows_inv(const BatchProb *__restrict_
         const BatchProb &b = bp[p];
         const int Mp = b.dg->g.Mp;
         if (fresh) twl = stage_twiddles(buf + lspan<true>(MAXTOT), b.tw, Mp);
-        const int nl = batch_item_lines(Mp), u0 = u * nl;
+        const int nl = batch_item_lines(kBsRowsInv, Mp), u0 = u * nl;
         rows_inv_line<float, NTHR, MAXTOT, true>(b.dg, buf, twl, b.ospec, 3, reinterpret_cast<float *>(b.gout), 4, u0,
                                                  min(nl, b.dg->g.n1d - u0));
     });

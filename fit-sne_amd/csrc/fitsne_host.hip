@@ -2087,10 +2087,12 @@ static BatchLds batch_lds(int Mp) {
     return {sizeof(cx<float>) * (lspan<true>(Mp) + (size_t)Mp), sizeof(cx<float>) * (lspan<true>(2 * Mp) + (size_t)Mp),
             sizeof(cx<float>) * (lspan<true>(4 * Mp) + (size_t)Mp)};
 }
-// Length class CLS of the batched FFT passes: its largest FFT length and the workgroup size of each pass.
+// Length class CLS of the batched FFT passes: the buffer of each pass in FFT-length units (the short
+// lines fill it with several lines per item) and the workgroup size of each pass.
 template <int CLS>
 struct BatchClass {
-    static constexpr int MP = CLS == 0 ? kBatchSmallMp : kCapMp2;
+    static constexpr int MP = CLS == 0 ? kBatchItemMp : kCapMp2;       // kernel-spectrum and row passes
+    static constexpr int MPC = CLS == 0 ? kBatchSmallMp : kCapMp2;     // column pass
     static constexpr int KT = kKernThreads, RT = kRowThreads, CT = kColThreads;
 };
 static_assert(kColThreads == kColZLanes, "the column pass's Z lanes are the single-context workgroup");
@@ -2114,7 +2116,8 @@ static void batch_fft_class(fitsne_batch *b, hipStream_t st, const BatchProb *bp
         kb_rows_fwd<C::RT, 2 * C::MP, CLS><<<dim3(b->g_rows[CLS]), dim3(C::RT), L.rows, st>>>(bp, n, sch, pref);
         break;
     case kBsCols:
-        kb_cols<C::CT, 4 * C::MP, CLS><<<dim3(b->g_cols[CLS]), dim3(C::CT), L.cols, st>>>(bp, n, sch, pref);
+        kb_cols<C::CT, 4 * C::MPC, CLS><<<dim3(b->g_cols[CLS]), dim3(C::CT), batch_lds(C::MPC).cols, st>>>(bp, n, sch,
+                                                                                                      pref);
         break;
     default:
         kb_rows_inv<C::RT, 2 * C::MP, CLS><<<dim3(b->g_rows[CLS]), dim3(C::RT), L.rows, st>>>(bp, n, sch, pref);
@@ -2128,7 +2131,7 @@ static int batch_class_blocks(fitsne_batch *b, int cus) {
     const BatchLds L = batch_lds(C::MP);
     TRY(persistent_blocks(kb_kern_rows<C::KT, C::MP, CLS>, C::KT, L.kern, cus, &b->g_kern[CLS]));
     TRY(persistent_blocks(kb_rows_fwd<C::RT, 2 * C::MP, CLS>, C::RT, L.rows, cus, &b->g_rows[CLS]));
-    TRY(persistent_blocks(kb_cols<C::CT, 4 * C::MP, CLS>, C::CT, L.cols, cus, &b->g_cols[CLS]));
+    TRY(persistent_blocks(kb_cols<C::CT, 4 * C::MPC, CLS>, C::CT, batch_lds(C::MPC).cols, cus, &b->g_cols[CLS]));
     return FITSNE_OK;
 }
 
