@@ -114,6 +114,8 @@ struct fitsne_ctx {
     int att_rows = 0;             // rows per attractive workgroup (0: about N / 256); results do not depend on it
     hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     hipEvent_t geo_ev = nullptr, kern_ev = nullptr;  // kernel spectra on astream: after the geometry, before cols
+    hipEvent_t att_ev = nullptr;  // 1D: on astream just before the attractive kernel; the spread waits for it
+    bool att_go = false;
     bool join_pending = false;
 
     int N = 0;
@@ -144,6 +146,11 @@ struct fitsne_ctx {
     hipGraphExec_t gexec = nullptr;
     bool use_graph = false;      // fitsne_step replays a captured step (FP32 form)
     bool mm_parts_valid = false; // mmpart holds the min / max partials of the resident Y
+    // FP32 host-planned steps leave zeroMean pending: Y in memory lacks the last mean (in `mean`), and
+    // the next step's readers (attractive, spread, interpolation, min rule) apply it on load; anything
+    // else that touches Y applies it first (materialize)
+    bool ctr_lazy = false;
+    bool tw_fresh = false;  // a twiddle table was written on the stream since the last step's stages
     bool graph_ran = false;      // a captured step ran since the last error check
     // plain path extras
     DBuf xs, ys, box, wx, wy, ch, w, V, potA, potB;
@@ -259,6 +266,7 @@ static int twiddles(fitsne_ctx *c, int n, const cx<T> **out) {
         DBuf b;
         TRY(ensure(b, sizeof(cx<T>) * (size_t)n));
         launch(c, "twiddles", k_twiddles<T>, dim3((n + 255) / 256), dim3(256), 0, b.as<cx<T>>(), n);
+        c->tw_fresh = true;  // written on c->stream just now: a reader on another stream must wait
         it = m.emplace(n, b).first;
     }
     *out = it->second.template as<cx<T>>();
@@ -310,7 +318,8 @@ template <typename TV>
 static int resolve_quirk(fitsne_ctx *c, double y0, double *mn, double *mx) {
     if (c->D != 2 || !(y0 < *mn)) return FITSNE_OK;
     launch(c, "minmax_seq", k_minmax_seq<TV>, dim3(1), dim3(1024), 0, c->Y.as<const TV>(),
-           c->permuted ? c->inv_d.as<const int>() : (const int *)nullptr, (long)c->N * 2, c->mm.as<double>());
+           c->permuted ? c->inv_d.as<const int>() : (const int *)nullptr, (long)c->N * 2, c->mm.as<double>(),
+           c->ctr_lazy ? c->mean.as<const double>() : (const double *)nullptr);
     HIPTRY(hipMemcpyAsync(c->h_mm, c->mm.p, sizeof(double) * 2, hipMemcpyDeviceToHost, c->stream));
     TRY(sync_ctx(c));
     *mx = c->h_mm[0];
@@ -386,8 +395,7 @@ static int geometry(fitsne_ctx *c, double mn, double mx, Geo *g) {
 // Host-planned geometry of this step into the device record the kernels read (live; kernel spectra
 // stale when they must be recomputed).  The cache fields of the captured path are reset, so a later
 // captured step recomputes its spectra and twiddles.
-static int put_geo(fitsne_ctx *c, const Geo &g, bool kspec_stale) {
-    TRY(ensure(c->dgeo, sizeof(DevGeo)));
+static DevGeo host_record(const fitsne_ctx *c, const Geo &g, bool kspec_stale) {
     DevGeo v{};
     v.g = g;
     v.pl = make_plan(g.Mp);
@@ -395,7 +403,29 @@ static int put_geo(fitsne_ctx *c, const Geo &g, bool kspec_stale) {
     v.kspec_stale = kspec_stale ? 1 : 0;
     v.mn = c->last_min;
     v.mx = c->last_max;
-    launch(c, "set_geo", k_set_geo, dim3(1), dim3(kWave), 0, v, c->dgeo.as<DevGeo>());
+    return v;
+}
+
+static int put_geo(fitsne_ctx *c, const Geo &g, bool kspec_stale) {
+    TRY(ensure(c->dgeo, sizeof(DevGeo)));
+    launch(c, "set_geo", k_set_geo, dim3(1), dim3(kWave), 0, host_record(c, g, kspec_stale), c->dgeo.as<DevGeo>());
+    return FITSNE_OK;
+}
+
+// Apply a pending zeroMean to Y in memory (k_center, with the min / max partials a captured step's
+// geometry starts from).  Every entry point that reads or replaces Y other than an FP32 update step
+// calls this first; the step's readers of Y on the second stream are done by then (the first stream
+// joined them before its column pass).
+static int materialize(fitsne_ctx *c) {
+    if (!c->ctr_lazy) return FITSNE_OK;
+    HIPTRY(hipSetDevice(c->device));
+    const long n = (long)c->N * c->D;
+    const int cblk = std::min(blocks_for(n), 1024);
+    TRY(ensure(c->mmpart, sizeof(double) * 2 * cblk));
+    launch(c, "center", k_center<float>, dim3(cblk), dim3(kThreads), 0, c->dgeo.as<const DevGeo>(), c->Y.as<float>(),
+           n, c->D, skip_coord(c), c->mean.as<const double>(), c->mmpart.as<double>());
+    c->ctr_lazy = false;
+    c->mm_parts_valid = true;
     return FITSNE_OK;
 }
 
@@ -423,6 +453,15 @@ static auto pick_spread_f32(int p) -> decltype(&k_spread_f32<3, D>) {
     case 3: return k_spread_f32<3, D>;
     case 4: return k_spread_f32<4, D>;
     default: return k_spread_f32<5, D>;
+    }
+}
+template <int D>
+static auto pick_spread_f32_v(int p) -> decltype(&k_spread_f32_v<3, D>) {
+    switch (p) {
+    case 2: return k_spread_f32_v<2, D>;
+    case 3: return k_spread_f32_v<3, D>;
+    case 4: return k_spread_f32_v<4, D>;
+    default: return k_spread_f32_v<5, D>;
     }
 }
 template <int D>
@@ -621,12 +660,15 @@ static int live_record(fitsne_ctx *c, const DevGeo **out) {
 
 // FP32: the windowed kernel reads the P multiplier from the step record (scale is for the CSR kernel)
 // and skips the step when the geometry record dg says so.
+// ctr: a pending centring of Y (windowed kernel only).
 template <typename TV, int D>
-static int launch_attractive(fitsne_ctx *c, TV scale, TV *pos, const DevGeo *dg = nullptr, bool captured = false) {
+static int launch_attractive(fitsne_ctx *c, TV scale, TV *pos, const DevGeo *dg = nullptr, bool captured = false,
+                             const double *ctr = nullptr) {
     const TV inv_df = (TV)(1.0 / c->df);
     const int df1 = c->df == 1.0;
     if constexpr (std::is_same<TV, float>::value) {
         if (!c->win_ok) TRY(build_windows(c));
+        if (ctr && !c->win_ok) return fail(FITSNE_E_UNSUPPORTED, "pending centring without the windowed layout");
         if (c->win_ok) {
             const int NB = c->win_NB;
             launch(c, "attractive", k_attr_win<D>, dim3(8 * ((NB + 7) / 8)), dim3(kAttThreads), 0,
@@ -634,7 +676,7 @@ static int launch_attractive(fitsne_ctx *c, TV scale, TV *pos, const DevGeo *dg 
                    c->wval.as<const float>(), c->wbmax.as<const float>(), c->Y.as<const float>(), c->N, c->win_R,
                    c->win_NW, c->win_bal ? c->wrstart.as<const int>() : (const int *)nullptr, NB, inv_df, df1,
                    (float)(0.5 * std::sqrt(c->df)), c->dstep.as<const StepDev>(), captured ? -1.f : (float)scale,
-                   dg, pos);
+                   dg, pos, ctr);
             return FITSNE_OK;
         }
     }
@@ -674,35 +716,64 @@ struct StageDims {
 // second stream (behind the attractive kernel there), after the geometry record, so that they run
 // beside the spread and the forward row pass; the column pass waits for them.  (Measured at N = 1M:
 // this beats computing them on the main stream before or after the spread by ~0.015 ms per step.)
+// ctr: the centring still pending on Y (host-planned steps), applied by the spread and the interpolation.
+// hv: the host-planned geometry record; the spread and the kernel spectra take it by value and the
+// spread stores it for the other passes, so no launch or cross-stream wait stands before them (without
+// hv the record is already in device memory: captured steps, or set by put_geo).
 template <int D>
 static int launch_f32_stages(fitsne_ctx *c, const cx<float> *tw, const StageDims &sd, bool kern, int mode,
-                             bool side) {
+                             bool side, const double *ctr = nullptr, const DevGeo *hv = nullptr) {
     const int N = c->N, p = c->p;
     const bool df1 = c->df == 1.0;
+    if (hv) TRY(ensure(c->dgeo, sizeof(DevGeo)));  // the spread stores the record there (a fresh context's first step)
     const DevGeo *dg = c->dgeo.as<const DevGeo>();
+    if (!dg) return fail(FITSNE_E_ARG, "no geometry record");
     const int kind_q = df1 ? FITSNE_KERNEL_SQUARED_CAUCHY : FITSNE_KERNEL_SQUARED_GENERAL;
     const int kind_1 = FITSNE_KERNEL_GENERAL;
     auto kernel_spectra = [&]() {
         const size_t Mp = (size_t)sd.Mp;
-        launch(c, "kern_rows", k_kern_rows<float, kKernThreads, kKernTot, true>, dim3(sd.gr_rows), dim3(kKernThreads),
-               sizeof(cx<float>) * (lspan<true>((int)Mp) + Mp), dg, tw, kind_q, kind_1, c->df, c->Trow.as<cx<float>>());
-        launch(c, "kern_cols", k_kern_cols<float, kKernThreads, kKernTot, true>, dim3(sd.gr_cols), dim3(kKernThreads),
-               sizeof(cx<float>) * (lspan<true>((int)Mp) + Mp), dg, tw, c->Trow.as<const cx<float>>(), 1, c->KQ.as<float>(),
-               c->K1.as<float>(), c->KS.as<float>(), c->KT.as<float>());
+        const size_t lds = sizeof(cx<float>) * (lspan<true>((int)Mp) + Mp);
+        if (hv) {
+            launch(c, "kern_rows", k_kern_rows_v<float, kKernThreads, kKernTot, true>, dim3(sd.gr_rows),
+                   dim3(kKernThreads), lds, *hv, tw, kind_q, kind_1, c->df, c->Trow.as<cx<float>>());
+            launch(c, "kern_cols", k_kern_cols_v<float, kKernThreads, kKernTot, true>, dim3(sd.gr_cols),
+                   dim3(kKernThreads), lds, *hv, tw, c->Trow.as<const cx<float>>(), 1, c->KQ.as<float>(),
+                   c->K1.as<float>(), c->KS.as<float>(), c->KT.as<float>());
+        } else {
+            launch(c, "kern_rows", k_kern_rows<float, kKernThreads, kKernTot, true>, dim3(sd.gr_rows),
+                   dim3(kKernThreads), lds, dg, tw, kind_q, kind_1, c->df, c->Trow.as<cx<float>>());
+            launch(c, "kern_cols", k_kern_cols<float, kKernThreads, kKernTot, true>, dim3(sd.gr_cols),
+                   dim3(kKernThreads), lds, dg, tw, c->Trow.as<const cx<float>>(), 1, c->KQ.as<float>(),
+                   c->K1.as<float>(), c->KS.as<float>(), c->KT.as<float>());
+        }
     };
     const bool kside = D == 2 && kern && side && c->overlap;
     if (kside) {
-        HIPTRY(hipEventRecord(c->geo_ev, c->stream));
-        HIPTRY(hipStreamWaitEvent(c->astream, c->geo_ev, 0));
+        // the spectra read the record (by value with hv) and the twiddles: wait for the stream only when
+        // it wrote either in this step
+        if (!hv || c->tw_fresh) {
+            HIPTRY(hipEventRecord(c->geo_ev, c->stream));
+            HIPTRY(hipStreamWaitEvent(c->astream, c->geo_ev, 0));
+        }
         hipStream_t main = c->stream;
         c->stream = c->astream;
         kernel_spectra();
         c->stream = main;
         HIPTRY(hipEventRecord(c->kern_ev, c->astream));
     }
+    c->tw_fresh = false;
+    if (c->att_go && side) {
+        HIPTRY(hipStreamWaitEvent(c->stream, c->att_ev, 0));
+        c->att_go = false;
+    }
     const int pts = c->spread_pts;
-    launch(c, "spread_f32", pick_spread_f32<D>(p), dim3((N + pts - 1) / pts), dim3(kSpreadThreads), 0,
-           c->Y.as<const float>(), N, pts, dg, c->fixgrid.as<unsigned long long>());
+    if (hv) {
+        launch(c, "spread_f32", pick_spread_f32_v<D>(p), dim3((N + pts - 1) / pts), dim3(kSpreadThreads), 0,
+               c->Y.as<const float>(), N, pts, *hv, c->dgeo.as<DevGeo>(), c->fixgrid.as<unsigned long long>(), ctr);
+    } else {
+        launch(c, "spread_f32", pick_spread_f32<D>(p), dim3((N + pts - 1) / pts), dim3(kSpreadThreads), 0,
+               c->Y.as<const float>(), N, pts, dg, c->fixgrid.as<unsigned long long>(), ctr);
+    }
     if constexpr (D == 2) {
         const size_t Mp = (size_t)sd.Mp;
         if (kern && !kside) kernel_spectra();
@@ -744,7 +815,7 @@ static int launch_f32_stages(fitsne_ctx *c, const cx<float> *tw, const StageDims
     launch(c, "interp_f32", pick_interp_f32<D>(p), dim3(nblk), dim3(kThreads), 0, c->Y.as<float>(),
            c->pos.as<const float>(), c->gout.as<const float4>(), dg, c->zbuf.as<const double>(),
            c->dstep.as<const StepDev>(), c->dC.as<float>(), c->uY.as<float>(), c->gains.as<float>(),
-           c->part.as<double>(), N, mm_host ? c->mmu.as<double>() : (double *)nullptr, skip_coord(c));
+           c->part.as<double>(), N, mm_host ? c->mmu.as<double>() : (double *)nullptr, skip_coord(c), ctr);
     if (mode == 1) {
         const long n = (long)N * D;
         const int cblk = std::min(blocks_for(n), 1024);
@@ -757,9 +828,15 @@ static int launch_f32_stages(fitsne_ctx *c, const cx<float> *tw, const StageDims
             launch(c, "mean_final", k_mean_final, dim3(1), dim3(kThreads), 0, dg, c->part.as<const double>(), nblk,
                    D, (double)N, c->mean.as<double>());
         }
-        // the min / max partials of the centred Y, for a device-planned step's geometry
-        launch(c, "center", k_center<float>, dim3(cblk), dim3(kThreads), 0, dg, c->Y.as<float>(), n, D,
-               skip_coord(c), c->mean.as<const double>(), c->mmpart.as<double>());
+        if (mm_host && c->win_ok) {
+            // zeroMean left pending: the next step's readers apply the mean now in `mean` on load
+            c->ctr_lazy = true;
+        } else {
+            // the min / max partials of the centred Y, for a device-planned step's geometry
+            launch(c, "center", k_center<float>, dim3(cblk), dim3(kThreads), 0, dg, c->Y.as<float>(), n, D,
+                   skip_coord(c), c->mean.as<const double>(), c->mmpart.as<double>());
+            c->ctr_lazy = false;
+        }
     }
     return FITSNE_OK;
 }
@@ -811,6 +888,11 @@ template <int D>
 static int pipeline_f32(fitsne_ctx *c, int mode, const fitsne_step_params *sp) {
     TRY(ensure(c->pos, sizeof(float) * (size_t)c->N * D));
     TRY(ensure(c->dstep, sizeof(StepDev)));
+    // a pending centring is carried only from one update step to the next (whose min / max the
+    // previous one left); otherwise it is applied here, before anything reads Y
+    if (c->ctr_lazy && (mode != 1 || !c->mm_pending)) TRY(materialize(c));
+    const double *ctr = c->ctr_lazy ? c->mean.as<const double>() : nullptr;
+    c->att_go = false;
     const DevGeo *live;
     TRY(live_record(c, &live));
     // the attractive term on the second stream (after everything queued so far), beside the grid
@@ -822,15 +904,24 @@ static int pipeline_f32(fitsne_ctx *c, int mode, const fitsne_step_params *sp) {
         if (!c->astream) HIPTRY(hipStreamCreateWithFlags(&c->astream, hipStreamNonBlocking));
         HIPTRY(hipEventRecord(c->fork_ev, c->stream));
         HIPTRY(hipStreamWaitEvent(c->astream, c->fork_ev, 0));
+        if (D == 1) {
+            // In 1D the host, once it has the previous step's min / max, can queue the spread as soon as
+            // the fork releases the attractive kernel; when the spread's many small workgroups reach the
+            // CUs first, the attractive kernel's (two per CU, 80 KB of LDS each) wait behind them
+            // (0.59 against 0.51 ms per step at N = 2M, from run to run).  The spread waits for this
+            // event, recorded just before the attractive kernel, so that kernel always starts first.
+            HIPTRY(hipEventRecord(c->att_ev, c->astream));
+            c->att_go = true;
+        }
         hipStream_t main = c->stream;
         c->stream = c->astream;
-        const int rc = launch_attractive<float, D>(c, (float)c->exag, c->pos.as<float>(), live);
+        const int rc = launch_attractive<float, D>(c, (float)c->exag, c->pos.as<float>(), live, false, ctr);
         c->stream = main;
         TRY(rc);
         HIPTRY(hipEventRecord(c->join_ev, c->astream));
         c->join_pending = true;
     } else {
-        TRY((launch_attractive<float, D>(c, (float)c->exag, c->pos.as<float>(), live)));
+        TRY((launch_attractive<float, D>(c, (float)c->exag, c->pos.as<float>(), live, false, ctr)));
     }
     // the step record (read by the interpolation; the attractive kernel took the exaggeration as an
     // argument) is written beside the attractive kernel
@@ -843,7 +934,7 @@ static int pipeline_f32(fitsne_ctx *c, int mode, const fitsne_step_params *sp) {
     TRY(twiddles<float>(c, g.Mp, &tw));
     TRY(ensure_f32_buffers<D>(c, g.n1d, g.Mp));
     const bool kern = D == 2 && !(c->kcache && c->kc_mn == mn && c->kc_mx == mx && c->kc_nb == g.nb);
-    TRY(put_geo(c, g, kern));
+    const DevGeo hv = host_record(c, g, kern);  // stored by the spread (k_set_geo's role)
     if (kern) {
         c->kcache = true;
         c->kc_mn = mn;
@@ -851,11 +942,11 @@ static int pipeline_f32(fitsne_ctx *c, int mode, const fitsne_step_params *sp) {
         c->kc_nb = g.nb;
     }
     const StageDims sd{g.Mp, 8 * ((g.n1d + 7) / 8), 8 * ((g.H + 7) / 8), blocks_for(g.n1d)};
-    TRY(launch_f32_stages<D>(c, tw, sd, kern, mode, true));
+    TRY(launch_f32_stages<D>(c, tw, sd, kern, mode, true, ctr, &hv));
     if (mode == 1) {  // k_mean_mm writes (max, min, Y[first]) of the centred Y to h_mm before mm_event
         c->mm_pending = true;
         c->mm_valid = false;
-        c->mm_parts_valid = true;
+        c->mm_parts_valid = !c->ctr_lazy;  // k_center's partials
     }
     c->have_grad = true;
     HIPTRY(hipGetLastError());
@@ -888,8 +979,11 @@ static int launch_step_dev(fitsne_ctx *c) {
 // Y and geometry on its own (best of three launches), and the fastest is kept.  The grid, and so every
 // result, is the same for every candidate (the spread's waves are the same aligned groups of 128
 // points); only the time differs: small problems want more, smaller workgroups, large ones fewer table
-// flushes.
+// flushes.  A larger candidate is kept only when it is more than 3% faster alone: inside the step the
+// spread fills the CUs the attractive kernel frees, which smaller workgroups do better (at N = 2M in 1D,
+// 1024 points measured 4% slower per step than 512, though alone the two are within noise).
 constexpr int kSpreadPtsCand[] = {512, 1024, 2048, 4096};
+constexpr float kSpreadGain = 0.97f;
 
 template <int D>
 static int tune_spread(fitsne_ctx *c) {
@@ -915,7 +1009,8 @@ static int tune_spread(fitsne_ctx *c) {
         for (int rep = 0; rep < 3 && rc == FITSNE_OK; rep++) {
             if (hipEventRecord(e0, c->stream) != hipSuccess) rc = fail(FITSNE_E_HIP, "event record failed");
             launch(c, "spread_f32", pick_spread_f32<D>(c->p), dim3((N + pts - 1) / pts), dim3(kSpreadThreads), 0,
-                   c->Y.as<const float>(), N, pts, c->dgeo.as<const DevGeo>(), c->fixgrid.as<unsigned long long>());
+                   c->Y.as<const float>(), N, pts, c->dgeo.as<const DevGeo>(), c->fixgrid.as<unsigned long long>(),
+                   (const double *)nullptr);
             if (hipEventRecord(e1, c->stream) != hipSuccess) rc = fail(FITSNE_E_HIP, "event record failed");
             // the grid is zero between iterations: clear what the trial added
             launch(c, "fill", k_fill<unsigned long long>, dim3(std::min(blocks_for(nw), 2048)), dim3(kThreads), 0,
@@ -926,7 +1021,7 @@ static int tune_spread(fitsne_ctx *c) {
             t = std::min(t, ms);
         }
         if (rc != FITSNE_OK) break;
-        if (t < best_ms) {
+        if (t < kSpreadGain * best_ms) {
             best_ms = t;
             best = pts;
         }
@@ -1353,11 +1448,12 @@ static int run_pipeline(fitsne_ctx *c, int mode, const fitsne_step_params *sp) {
     return c->D == 2 ? pipeline_f64<2>(c, mode, sp) : pipeline_f64<1>(c, mode, sp);
 }
 
+// Z through pinned slot 3: slots 0-2 may hold the (max, min, Y[first]) k_mean_mm left for the next step.
 static int read_sumq(fitsne_ctx *c, double *sum_Q) {
     double z[2];
-    HIPTRY(hipMemcpyAsync(c->h_mm + 2, c->zbuf.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPTRY(hipMemcpyAsync(c->h_mm + 3, c->zbuf.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     TRY(sync_ctx(c));
-    z[0] = c->h_mm[2];
+    z[0] = c->h_mm[3];
     c->last_sumQ = z[0];
     if (sum_Q) *sum_Q = z[0];
     return FITSNE_OK;
@@ -1420,6 +1516,7 @@ extern "C" int fitsne_ctx_create(fitsne_ctx **out, int device, int precision, in
         hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->geo_ev, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->kern_ev, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->att_ev, hipEventDisableTiming) != hipSuccess ||
         hipHostMalloc((void **)&c->h_mm, sizeof(double) * 4, hipHostMallocDefault) != hipSuccess ||
         hipHostGetDevicePointer((void **)&c->h_mm_dev, c->h_mm, 0) != hipSuccess ||
         hipEventCreateWithFlags(&c->mm_event, hipEventDisableTiming) != hipSuccess) {
@@ -1460,6 +1557,7 @@ extern "C" int fitsne_ctx_destroy(fitsne_ctx *c) {
     if (c->join_ev) (void)hipEventDestroy(c->join_ev);
     if (c->geo_ev) (void)hipEventDestroy(c->geo_ev);
     if (c->kern_ev) (void)hipEventDestroy(c->kern_ev);
+    if (c->att_ev) (void)hipEventDestroy(c->att_ev);
     if (c->astream) (void)hipStreamDestroy(c->astream);
     if (c->h_mm) (void)hipHostFree(c->h_mm);
     if (c->bh_pin) (void)hipHostFree(c->bh_pin);
@@ -1474,6 +1572,8 @@ static int set_P_impl(fitsne_ctx *c, int N, const TI *row_P, const TC *col_P, co
     if (c->precision == FITSNE_FP64_EXACT)
         return fail(FITSNE_E_ARG, "an exact context takes a dense P (fitsne_set_P_dense / fitsne_exact_affinities)");
     HIPTRY(hipSetDevice(c->device));
+    TRY(materialize(c));
+    TRY(sync_ctx(c));
     const long long nnz = (long long)row_P[N];
     if (nnz < 0 || (long long)row_P[0] != 0) return fail(FITSNE_E_ARG, "row_P must start at 0");
     for (int i = 0; i < N; i++)
@@ -1742,6 +1842,8 @@ extern "C" int fitsne_reorder(fitsne_ctx *c) {
     if (!c) return fail(FITSNE_E_ARG, "null context");
     if (c->N <= 0 || !c->Y.p) return fail(FITSNE_E_ARG, "set P and Y before reordering");
     HIPTRY(hipSetDevice(c->device));
+    TRY(materialize(c));
+    TRY(sync_ctx(c));
     return reorder_points(c);
 }
 
@@ -1772,6 +1874,7 @@ extern "C" int fitsne_set_Y(fitsne_ctx *c, const double *Y) {
         if (!std::isfinite(Y[e])) return fail(FITSNE_E_ARG, "Y contains non-finite values");
     HIPTRY(hipSetDevice(c->device));
     TRY(sync_ctx(c));
+    c->ctr_lazy = false;  // Y is replaced: a pending centring is void
     const int D = c->D;
     if (c->precision == FITSNE_FP32) {
         std::vector<float> y(n), z(n, 0.f), one(n, 1.f);
@@ -1805,6 +1908,7 @@ extern "C" int fitsne_get_Y(fitsne_ctx *c, double *Y) {
     if (!c || !Y) return fail(FITSNE_E_ARG, "null argument");
     HIPTRY(hipSetDevice(c->device));
     TRY(check_graph(c));
+    TRY(materialize(c));
     TRY(sync_ctx(c));
     const size_t n = (size_t)c->N * c->D;
     if (c->precision == FITSNE_FP32) {
@@ -1936,6 +2040,7 @@ extern "C" int fitsne_autotune(fitsne_ctx *c, int *spread_points) {
     if (c->N <= 0 || !c->Y.p) return fail(FITSNE_E_ARG, "set P and Y before tuning");
     HIPTRY(hipSetDevice(c->device));
     TRY(check_graph(c));
+    TRY(materialize(c));
     TRY(sync_ctx(c));
     TRY(c->D == 2 ? tune_spread<2>(c) : tune_spread<1>(c));
     TRY(c->D == 2 ? tune_attractive<2>(c) : tune_attractive<1>(c));
@@ -1956,6 +2061,7 @@ extern "C" int fitsne_set_attractive_rows(fitsne_ctx *c, int rows) {
         return fail(FITSNE_E_ARG, "attractive rows per workgroup must be 0 (automatic), -1 (balanced blocks) or a "
                                   "multiple of 64 in [64, 4096]");
     HIPTRY(hipSetDevice(c->device));
+    TRY(materialize(c));  // the next step may lay P out without the windows the pending centring needs
     TRY(sync_ctx(c));
     if (rows != c->att_rows) {
         c->att_rows = rows;
@@ -1991,6 +2097,7 @@ extern "C" int fitsne_set_graph(fitsne_ctx *c, int enable) {
     if (enable && c->precision != FITSNE_FP32) return fail(FITSNE_E_UNSUPPORTED, "captured steps are for the FP32 form");
     // a captured step may still run: wait for it (and report a refused one) before its graph goes
     HIPTRY(hipSetDevice(c->device));
+    TRY(materialize(c));
     TRY(sync_ctx(c));
     TRY(check_graph(c));
     drop_graph(c);
@@ -2003,6 +2110,7 @@ extern "C" int fitsne_step(fitsne_ctx *c, const fitsne_step_params *prm, double 
     if (c->use_graph) {
         if (c->N <= 0) return fail(FITSNE_E_ARG, "P and Y must be set before a step");
         HIPTRY(hipSetDevice(c->device));
+        TRY(materialize(c));
         if (!c->gexec) TRY(capture_step(c));
         TRY(ensure_partials(c));
         TRY(put_step(c, 1, prm));
@@ -2326,6 +2434,7 @@ extern "C" int fitsne_batch_create(fitsne_batch **out, fitsne_ctx **ctxs, int n)
         if (c->N <= 0 || !c->Y.p) return fail(FITSNE_E_ARG, "set P and Y before batching");
         HIPTRY(hipSetDevice(c->device));
         TRY(check_graph(c));
+        TRY(materialize(c));
         TRY(sync_ctx(c));
         TRY(c->D == 2 ? tune_spread<2>(c) : tune_spread<1>(c));
     }
@@ -2396,6 +2505,7 @@ extern "C" int fitsne_batch_step(fitsne_batch *b, const fitsne_step_params *prm)
             fitsne_ctx *c = b->ctx[i];
             HIPTRY(hipSetDevice(c->device));
             c->use_graph = true;
+            TRY(materialize(c));
             if (!c->gexec) TRY(capture_step(c));
             TRY(ensure_partials(c));
             TRY(put_step(c, 1, &prm[i]));
@@ -2408,7 +2518,10 @@ extern "C" int fitsne_batch_step(fitsne_batch *b, const fitsne_step_params *prm)
     bool stale = !b->gexec;
     for (int i = 0; i < n && !stale; i++) stale = b->ctx[i]->layout_gen != b->gens[i];
     if (stale) TRY(batch_prepare(b));
-    for (fitsne_ctx *c : b->ctx) TRY(ensure_partials(c));
+    for (fitsne_ctx *c : b->ctx) {
+        TRY(materialize(c));
+        TRY(ensure_partials(c));
+    }
     const int s = b->slot;
     if (b->slot_used[s]) HIPTRY(hipEventSynchronize(b->slot_ev[s]));
     StepDev *h = b->h_steps + (size_t)s * n;
@@ -2434,6 +2547,7 @@ extern "C" int fitsne_batch_step(fitsne_batch *b, const fitsne_step_params *prm)
 extern "C" int fitsne_kl(fitsne_ctx *c, double *C) {
     if (!c || !C) return fail(FITSNE_E_ARG, "null argument");
     TRY(check_graph(c));
+    TRY(materialize(c));  // before the KL's sum overwrites the mean
     if (c->precision == FITSNE_FP64_EXACT || c->precision == FITSNE_FP64_BH) {
         if (c->N <= 0) return fail(FITSNE_E_ARG, "P and Y must be set before the KL divergence");
         HIPTRY(hipSetDevice(c->device));
@@ -2530,6 +2644,7 @@ extern "C" int fitsne_last_geometry(fitsne_ctx *c, double *out) {
 extern "C" long long fitsne_debug_read(fitsne_ctx *c, const char *name, void *dst, long long bytes) {
     if (!c || !name || !dst || bytes < 0) return fail(FITSNE_E_ARG, "null argument");
     HIPTRY(hipSetDevice(c->device));
+    TRY(materialize(c));
     TRY(sync_ctx(c));
     const std::string n(name);
     DBuf *b = n == "grid_in" ? &c->gin : n == "grid_out" ? &c->gout : n == "kq" ? &c->KQ : n == "k1" ? &c->K1

@@ -54,6 +54,11 @@ __device__ __forceinline__ void load_pt(const TV *__restrict__ Y, long i, TV (&y
     }
 }
 
+// zeroMean applied on load: f_d(y) = (float)((double)y - mean_d), exactly what k_center stores, so a
+// reader of a Y whose centring is still pending (the mean in ctr) sees the same floats; with mean 0 it is
+// the identity.
+__device__ __forceinline__ float centred(float y, double m) { return (float)((double)y - m); }
+
 // Copy the twiddle table into LDS behind the workgroup's lines (the FFT stages read it once per
 // butterfly input); visible after the caller's next __syncthreads.
 template <typename T>
@@ -311,9 +316,11 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // Four entries of one row (a quad of the pure section).
-template <int D>
+// SH: yw is the uncentred Y in global memory, centred on load by (m0, m1).
+template <int D, bool SH = false>
 __device__ __forceinline__ void attr_quad_pure(const u32x4 &kq, const f32x4 &vq4, const float *yr, const float *yw,
-                                               unsigned long long *acc, float inv_df, int df_one, double S) {
+                                               unsigned long long *acc, float inv_df, int df_one, double S,
+                                               double m0 = 0.0, double m1 = 0.0) {
     const uint32_t kk[4] = {kq.x, kq.y, kq.z, kq.w};
     const float v[4] = {vq4.x, vq4.y, vq4.z, vq4.w};
     const int il = (int)(kk[0] >> 16);
@@ -330,7 +337,8 @@ __device__ __forceinline__ void attr_quad_pure(const u32x4 &kq, const f32x4 &vq4
         float diff[D], d2 = 0.f;
 #pragma unroll
         for (int d = 0; d < D; d++) {
-            diff[d] = yi[d] - yw[jl * D + d];
+            const float yj = SH ? centred(yw[jl * D + d], d ? m1 : m0) : yw[jl * D + d];
+            diff[d] = yi[d] - yj;
             d2 += diff[d] * diff[d];
         }
         const float q = __builtin_amdgcn_rcpf(df_one ? 1.f + d2 : 1.f + d2 * inv_df);
@@ -343,9 +351,10 @@ __device__ __forceinline__ void attr_quad_pure(const u32x4 &kq, const f32x4 &vq4
 }
 
 // Four consecutive entries of the mixed section (any rows).
-template <int D>
+template <int D, bool SH = false>
 __device__ __forceinline__ void attr_quad(const u32x4 &kq, const f32x4 &vq4, const float *yr, const float *yw,
-                                          unsigned long long *acc, float inv_df, int df_one, double S) {
+                                          unsigned long long *acc, float inv_df, int df_one, double S,
+                                          double m0 = 0.0, double m1 = 0.0) {
     const uint32_t kk[4] = {kq.x, kq.y, kq.z, kq.w};
     const float v[4] = {vq4.x, vq4.y, vq4.z, vq4.w};
     int cur = -1;
@@ -358,7 +367,8 @@ __device__ __forceinline__ void attr_quad(const u32x4 &kq, const f32x4 &vq4, con
         float diff[D], d2 = 0.f;
 #pragma unroll
         for (int d = 0; d < D; d++) {
-            diff[d] = yr[il * D + d] - yw[jl * D + d];
+            const float yj = SH ? centred(yw[jl * D + d], d ? m1 : m0) : yw[jl * D + d];
+            diff[d] = yr[il * D + d] - yj;
             d2 += diff[d] * diff[d];
         }
         // hardware reciprocal (1 ulp): well inside the FP32 form's error, a tenth of a division's cost
@@ -398,15 +408,22 @@ __device__ __forceinline__ void attr_win_block(const int *__restrict__ start, co
                                                const float *__restrict__ bmax, const float *__restrict__ Y, int N,
                                                int R, int NW, const int *__restrict__ rstart, int NB, float inv_df,
                                                int df_one, float mdf, float scale, float *__restrict__ pos, int b,
-                                               AttrLds<D> &L) {
+                                               AttrLds<D> &L, const double *__restrict__ ctr = nullptr) {
     constexpr int WL = kWin * D / kAttThreads;  // window floats per thread
     float *yw = L.yw, *yr = L.yr;
     unsigned long long *acc = L.acc;
     const int t = threadIdx.x;
+    // a pending centring (ctr) is applied as Y is staged; a thread's flat indices q = t + u kAttThreads
+    // all hold coordinate t mod D
+    const bool sh = ctr != nullptr;
+    const double m0 = sh ? ctr[0] : 0.0, m1 = D == 2 && sh ? ctr[1] : 0.0;
     const int r0 = rstart ? rstart[b] : b * R, nr = rstart ? rstart[b + 1] - r0 : min(R, N - r0);
-    for (int q = t; q < nr * D; q += kAttThreads) {
-        yr[q] = Y[(long)r0 * D + q];
-        acc[q] = 0ull;
+    {
+        const double mt = D == 2 && (t & 1) ? m1 : m0;
+        for (int q = t; q < nr * D; q += kAttThreads) {
+            yr[q] = sh ? centred(Y[(long)r0 * D + q], mt) : Y[(long)r0 * D + q];
+            acc[q] = 0ull;
+        }
     }
     const double bound = (double)bmax[NB] * mdf;  // the maximum over all NB row blocks
     int e = 0;
@@ -466,8 +483,13 @@ __device__ __forceinline__ void attr_win_block(const int *__restrict__ start, co
             const float *ywg = Y + (long)ww * kWin * D;  // the window's Y in global memory
             const u32x4 ka = __builtin_nontemporal_load(&K4[qq]);
             const f32x4 va = __builtin_nontemporal_load(&V4[qq]);
-            if (qq < qm) attr_quad_pure<D>(ka, va, yr, ywg, acc, inv_df, df_one, S);
-            else attr_quad<D>(ka, va, yr, ywg, acc, inv_df, df_one, S);
+            if (sh) {  // uniform: the centring's FP64 work only when one is pending
+                if (qq < qm) attr_quad_pure<D, true>(ka, va, yr, ywg, acc, inv_df, df_one, S, m0, m1);
+                else attr_quad<D, true>(ka, va, yr, ywg, acc, inv_df, df_one, S, m0, m1);
+            } else {
+                if (qq < qm) attr_quad_pure<D>(ka, va, yr, ywg, acc, inv_df, df_one, S);
+                else attr_quad<D>(ka, va, yr, ywg, acc, inv_df, df_one, S);
+            }
         }
     }
     __syncthreads();  // the list lives in the window buffer
@@ -477,6 +499,8 @@ __device__ __forceinline__ void attr_win_block(const int *__restrict__ start, co
     };
     int w = 0;
     while (w < NW && !staged(w)) w++;
+    // the window's loads stay in flight across the current bucket; a pending centring is applied as
+    // they are stored to LDS
     float pre[WL];
     auto fetch = [&](int ww) {
         const long base = (long)ww * kWin * D;
@@ -513,6 +537,13 @@ __device__ __forceinline__ void attr_win_block(const int *__restrict__ start, co
         int wn = w + 1;
         while (wn < NW && !staged(wn)) wn++;
         __syncthreads();  // the previous window's readers are done
+        if (sh) {
+            // the lane's mean, read here (after the barrier) so that it is not held across the loop;
+            // the padding past N is centred too and never read
+            const double mt = ctr[D == 2 ? (t & 1) : 0];
+#pragma unroll
+            for (int u = 0; u < WL; u++) pre[u] = centred(pre[u], mt);
+        }
 #pragma unroll
         for (int u = 0; u < WL; u++) yw[t + u * kAttThreads] = pre[u];
         __syncthreads();
@@ -552,14 +583,15 @@ __global__ __launch_bounds__(kAttThreads) __attribute__((amdgpu_waves_per_eu(D =
                                                           const int *__restrict__ rstart, int nblk,
                                                           float inv_df, int df_one, float mdf,
                                                           const StepDev *__restrict__ sd, float exag,
-                                                          const DevGeo *__restrict__ dg, float *__restrict__ pos) {
+                                                          const DevGeo *__restrict__ dg, float *__restrict__ pos,
+                                                          const double *__restrict__ ctr) {
     if (!dg->live) return;
     const float scale = exag >= 0.f ? exag : sd->exag;  // exag < 0: the step record's (captured steps)
     __shared__ AttrLds<D> L;
     const int b = xcd_row(blockIdx.x, nblk);
     if (b >= nblk) return;
     attr_win_block<D>(start, pure, key, wval, bmax, Y, N, R, NW, rstart, nblk, inv_df, df_one, mdf, scale, pos, b,
-                      L);
+                      L, ctr);
 }
 
 // ---------------------------------------------------------------- min / max
@@ -625,21 +657,28 @@ __global__ void k_minmax_final(const double *__restrict__ part, int nblk, const 
 
 // Exact sequential `if (v > max) max = v; else if (v < min) min = v;` over Y[0..n) in one block:
 // chunk maxima, their exclusive prefix maximum, then each chunk replays the rule from its prefix.
+// ctr: a pending centring of the FP32 Y (2D), applied on load
 template <typename TV>
-__device__ __forceinline__ double seq_at(const TV *__restrict__ Y, const int *__restrict__ pos, long e) {
-    return pos ? (double)Y[2L * pos[e >> 1] + (e & 1)] : (double)Y[e];
+__device__ __forceinline__ double seq_at(const TV *__restrict__ Y, const int *__restrict__ pos, long e,
+                                         const double *__restrict__ ctr) {
+    const TV v = pos ? Y[2L * pos[e >> 1] + (e & 1)] : Y[e];
+    if constexpr (std::is_same<TV, float>::value) {
+        if (ctr) return (double)centred(v, ctr[e & 1]);
+    }
+    return (double)v;
 }
 
 // One block of 1024 threads; out[0] = max, out[1] = min.
 template <typename TV>
-__device__ void minmax_seq_block(const TV *__restrict__ Y, const int *__restrict__ pos, long n, double *out) {
+__device__ void minmax_seq_block(const TV *__restrict__ Y, const int *__restrict__ pos, long n, double *out,
+                                 const double *__restrict__ ctr = nullptr) {
     __shared__ double cmax[1024];
     __shared__ double cmin[1024];
     const int t = threadIdx.x;
     const long chunk = (n + 1023) / 1024;
     const long b = t * chunk, e = b + chunk < n ? b + chunk : n;
     double m = -INFINITY;
-    for (long i = b; i < e; i++) m = fmax(m, seq_at(Y, pos, i));
+    for (long i = b; i < e; i++) m = fmax(m, seq_at(Y, pos, i, ctr));
     cmax[t] = m;
     __syncthreads();
     if (t == 0) {
@@ -654,7 +693,7 @@ __device__ void minmax_seq_block(const TV *__restrict__ Y, const int *__restrict
     __syncthreads();
     double mx = cmax[t], mn = INFINITY;
     for (long i = b; i < e; i++) {
-        const double v = seq_at(Y, pos, i);
+        const double v = seq_at(Y, pos, i, ctr);
         if (v > mx) mx = v;
         else if (v < mn) mn = v;
     }
@@ -670,8 +709,8 @@ __device__ void minmax_seq_block(const TV *__restrict__ Y, const int *__restrict
 
 template <typename TV>
 __global__ __launch_bounds__(1024) void k_minmax_seq(const TV *__restrict__ Y, const int *__restrict__ pos, long n,
-                                                     double *__restrict__ out) {
-    minmax_seq_block(Y, pos, n, out);
+                                                     double *__restrict__ out, const double *__restrict__ ctr) {
+    minmax_seq_block(Y, pos, n, out, ctr);
 }
 
 // ---------------------------------------------------------------- step setup on the device
@@ -969,7 +1008,8 @@ __device__ __forceinline__ int point_charges(const float (&y)[D], const Geo &g, 
 // Runs of lanes with the same b box are summed by the segmented scan, and each run's last lane flushes.
 template <int P, int D>
 __device__ __forceinline__ void spread_block(const float *__restrict__ Y, int N, int pts, const DevGeo *__restrict__ dg,
-                                             unsigned long long *__restrict__ grid, int bid, SpreadLds<P, D> &L) {
+                                             unsigned long long *__restrict__ grid, int bid, SpreadLds<P, D> &L,
+                                             const double *__restrict__ ctr = nullptr) {
     constexpr int NV = SpreadLds<P, D>::NV;
     constexpr int NQ = SpreadLds<P, D>::NQ;
     constexpr int NS = SpreadLds<P, D>::NS;
@@ -1041,6 +1081,13 @@ __device__ __forceinline__ void spread_block(const float *__restrict__ Y, int N,
         } else if (oka) {
             load_pt<float, D>(Y, i, ya);
         }
+        if (ctr) {  // pending centring (uniform branch)
+#pragma unroll
+            for (int d = 0; d < D; d++) {
+                ya[d] = oka ? centred(ya[d], ctr[d]) : 0.f;
+                yb[d] = okb ? centred(yb[d], ctr[d]) : 0.f;
+            }
+        }
         float val[NV][NQ];
 #pragma unroll
         for (int v = 0; v < NV; v++)
@@ -1111,10 +1158,25 @@ __device__ __forceinline__ void spread_block(const float *__restrict__ Y, int N,
 template <int P, int D>
 __global__ __launch_bounds__(kSpreadThreads) __attribute__((amdgpu_waves_per_eu(D == 2 && P <= 3 ? 4 : 1))) void k_spread_f32(const float *__restrict__ Y, int N, int pts,
                                                                const DevGeo *__restrict__ dg,
-                                                               unsigned long long *__restrict__ grid) {
+                                                               unsigned long long *__restrict__ grid,
+                                                               const double *__restrict__ ctr) {
     __shared__ SpreadLds<P, D> L;
     if (!dg->live) return;
-    spread_block<P, D>(Y, N, pts, dg, grid, blockIdx.x, L);
+    spread_block<P, D>(Y, N, pts, dg, grid, blockIdx.x, L, ctr);
+}
+
+// The same with a host-planned geometry record passed by value; the first workgroup also stores it to
+// `out` for the passes that follow on the stream (this replaces k_set_geo, which, queued behind the
+// attractive kernel's workgroups, would hold the spread back).
+template <int P, int D>
+__global__ __launch_bounds__(kSpreadThreads) __attribute__((amdgpu_waves_per_eu(D == 2 && P <= 3 ? 4 : 1))) void k_spread_f32_v(const float *__restrict__ Y, int N, int pts,
+                                                               const DevGeo v, DevGeo *__restrict__ out,
+                                                               unsigned long long *__restrict__ grid,
+                                                               const double *__restrict__ ctr) {
+    __shared__ SpreadLds<P, D> L;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *out = v;
+    if (!v.live) return;
+    spread_block<P, D>(Y, N, pts, &v, grid, blockIdx.x, L, ctr);
 }
 
 // int64 grid -> float4 (A, c, b_x, b_y) and clear, for the 1D path (the 2D row FFT does this itself).
@@ -1172,9 +1234,9 @@ __device__ __forceinline__ void kern_rows_line(const DevGeo *__restrict__ dg, cx
     __syncthreads();
 }
 
-template <typename T, int NTHR, int MAXTOT, bool PAD = false>
-__global__ __launch_bounds__(NTHR) void k_kern_rows(const DevGeo *__restrict__ dg, const cx<T> *__restrict__ tw,
-                                                    int kind_q, int kind_1, double df, cx<T> *__restrict__ Trow) {
+template <typename T, int NTHR, int MAXTOT, bool PAD>
+__device__ __forceinline__ void kern_rows_body(const DevGeo *__restrict__ dg, const cx<T> *__restrict__ tw,
+                                               int kind_q, int kind_1, double df, cx<T> *__restrict__ Trow) {
     if (!dg->live || !dg->kspec_stale) return;
     const Geo &g = dg->g;
     extern __shared__ unsigned char smem_raw[];
@@ -1187,6 +1249,20 @@ __global__ __launch_bounds__(NTHR) void k_kern_rows(const DevGeo *__restrict__ d
         if (a >= n1d) continue;
         kern_rows_line<T, NTHR, MAXTOT, PAD>(dg, buf, twl, kind_q, kind_1, df, Trow, a);
     }
+}
+
+template <typename T, int NTHR, int MAXTOT, bool PAD = false>
+__global__ __launch_bounds__(NTHR) void k_kern_rows(const DevGeo *__restrict__ dg, const cx<T> *__restrict__ tw,
+                                                    int kind_q, int kind_1, double df, cx<T> *__restrict__ Trow) {
+    kern_rows_body<T, NTHR, MAXTOT, PAD>(dg, tw, kind_q, kind_1, df, Trow);
+}
+
+// The same with a host-planned geometry record passed by value, so that the pass need not wait for the
+// record to reach device memory (it runs on another queue than the spread that stores it).
+template <typename T, int NTHR, int MAXTOT, bool PAD = false>
+__global__ __launch_bounds__(NTHR) void k_kern_rows_v(const DevGeo v, const cx<T> *__restrict__ tw, int kind_q,
+                                                      int kind_1, double df, cx<T> *__restrict__ Trow) {
+    kern_rows_body<T, NTHR, MAXTOT, PAD>(&v, tw, kind_q, kind_1, df, Trow);
 }
 
 // buf[q] = f(q) for q < total, the loads of a thread issued eight at a time before their LDS writes, so
@@ -1271,11 +1347,10 @@ __device__ __forceinline__ void kern_cols_line(const DevGeo *__restrict__ dg, cx
     __syncthreads();
 }
 
-template <typename T, int NTHR, int MAXTOT, bool PAD = false>
-__global__ __launch_bounds__(NTHR) void k_kern_cols(const DevGeo *__restrict__ dg, const cx<T> *__restrict__ tw,
-                                                    const cx<T> *__restrict__ Trow, int need_s,
-                                                    T *__restrict__ KQ, T *__restrict__ K1, T *__restrict__ KS,
-                                                    T *__restrict__ KT) {
+template <typename T, int NTHR, int MAXTOT, bool PAD>
+__device__ __forceinline__ void kern_cols_body(const DevGeo *__restrict__ dg, const cx<T> *__restrict__ tw,
+                                               const cx<T> *__restrict__ Trow, int need_s, T *__restrict__ KQ,
+                                               T *__restrict__ K1, T *__restrict__ KS, T *__restrict__ KT) {
     if (!dg->live || !dg->kspec_stale) return;
     const Geo &g = dg->g;
     extern __shared__ unsigned char smem_raw[];
@@ -1288,6 +1363,22 @@ __global__ __launch_bounds__(NTHR) void k_kern_cols(const DevGeo *__restrict__ d
         if (k >= H) continue;
         kern_cols_line<T, NTHR, MAXTOT, PAD>(dg, buf, twl, Trow, need_s, KQ, K1, KS, KT, k);
     }
+}
+
+template <typename T, int NTHR, int MAXTOT, bool PAD = false>
+__global__ __launch_bounds__(NTHR) void k_kern_cols(const DevGeo *__restrict__ dg, const cx<T> *__restrict__ tw,
+                                                    const cx<T> *__restrict__ Trow, int need_s,
+                                                    T *__restrict__ KQ, T *__restrict__ K1, T *__restrict__ KS,
+                                                    T *__restrict__ KT) {
+    kern_cols_body<T, NTHR, MAXTOT, PAD>(dg, tw, Trow, need_s, KQ, K1, KS, KT);
+}
+
+template <typename T, int NTHR, int MAXTOT, bool PAD = false>
+__global__ __launch_bounds__(NTHR) void k_kern_cols_v(const DevGeo v, const cx<T> *__restrict__ tw,
+                                                      const cx<T> *__restrict__ Trow, int need_s,
+                                                      T *__restrict__ KQ, T *__restrict__ K1, T *__restrict__ KS,
+                                                      T *__restrict__ KT) {
+    kern_cols_body<T, NTHR, MAXTOT, PAD>(&v, tw, Trow, need_s, KQ, K1, KS, KT);
 }
 
 // Row FFT, forward.  grid: [n1d][n1d][sin] real (u = x node, v = y node); terms 2l, 2l+1 packed as
@@ -1828,7 +1919,8 @@ __device__ __forceinline__ void interp_block(float *__restrict__ Y, const float 
                                              const double *__restrict__ zbuf, const StepDev *__restrict__ sd,
                                              float *__restrict__ dC, float *__restrict__ uY, float *__restrict__ gains,
                                              double *__restrict__ part, int N, int bid, double *sh,
-                                             double *__restrict__ mmu = nullptr, long skip = -1) {
+                                             double *__restrict__ mmu = nullptr, long skip = -1,
+                                             const double *__restrict__ ctr = nullptr) {
     constexpr int U = kInterpPpt;
     int idx[U];
     float y[U][D], pi[U][D], u[U][D], gn[U][D];
@@ -1837,6 +1929,10 @@ __device__ __forceinline__ void interp_block(float *__restrict__ Y, const float 
         idx[k] = bid * kInterpPts + k * kThreads + threadIdx.x;
         if (idx[k] < N) {  // the momentum state is there in every mode; it is used only by updates
             load_pt<float, D>(Y, idx[k], y[k]);
+            if (ctr) {  // pending centring (uniform branch)
+#pragma unroll
+                for (int d = 0; d < D; d++) y[k][d] = centred(y[k][d], ctr[d]);
+            }
             // read once per step: streamed past the caches
 #pragma unroll
             for (int d = 0; d < D; d++) {
@@ -1996,10 +2092,11 @@ __global__ __launch_bounds__(kThreads) void k_interp_f32(float *__restrict__ Y, 
                                                          const StepDev *__restrict__ sd,
                                                          float *__restrict__ dC, float *__restrict__ uY,
                                                          float *__restrict__ gains, double *__restrict__ part,
-                                                         int N, double *__restrict__ mmu, long skip) {
+                                                         int N, double *__restrict__ mmu, long skip,
+                                                         const double *__restrict__ ctr) {
     if (!dg->live) return;
     __shared__ double sh[3 * D * kThreads / kWave];
-    interp_block<P, D>(Y, pos, gout, dg, zbuf, sd, dC, uY, gains, part, N, blockIdx.x, sh, mmu, skip);
+    interp_block<P, D>(Y, pos, gout, dg, zbuf, sd, dC, uY, gains, part, N, blockIdx.x, sh, mmu, skip, ctr);
 }
 
 // Mean of the updated Y from block partials (fixed order), FP64.  One workgroup of kThreads.

@@ -156,6 +156,44 @@ def test_2d_min_rule_leaves_the_first_point_outside_the_grid():
     assert rel(got32, orc.gradient(P, Y32)[0]) <= 1e-4
 
 
+def test_update_steps_carry_the_min_rule_and_the_centring_between_steps():
+    """FP32 update steps leave zeroMean pending (the next step's readers of Y apply the mean on load)
+    and pass (max, min, Y[first]) of the centred Y to the next step's geometry through pinned memory.
+    Steps that ask for Z, with Y read back after each (which applies the centring in memory), must give
+    the same embedding bit for bit as uninterrupted steps.  The first point is kept the strict minimum
+    with its second coordinate below every other point's, so every step replays the sequential 2D min
+    rule (tsne.cpp:1045-1048), whose result then differs from the min of the rest."""
+    N = 4000
+    row, col, val = ragged(N, 8, empty_frac=0.0, max_len=12)
+    val = val.copy()
+    val[(col == 0)] = 0.0  # the first point feels no attraction: repulsion keeps it off the rest
+    val[row[0]:row[1]] = 0.0
+    P = (row, col, val / val.sum())
+    Y = blobs(N, 2, 6, seed=9, span=40.0)
+    low = Y[1:].min()
+    Y[0] = (low - 0.3, low - 0.2)  # under half a box below the rest
+    prm = dict(momentum=0.5, learning_rate=100.0)
+    a = fg.Context(no_dims=2, precision=fg.FP32)
+    b = fg.Context(no_dims=2, precision=fg.FP32)
+    for c in (a, b):
+        c.set_P(*P)
+        c.set_Y(Y)
+    say(f"[gpu] pending centring and min rule across steps, N={N}")
+    for it in range(6):
+        z = a.step(**prm, want_sumq=True)
+        ya = a.get_Y()
+        assert np.isfinite(z) and z > 0
+        rest = np.delete(ya.ravel(), [0, 1])
+        assert ya[0, 0] < ya[0, 1] < rest.min(), f"step {it}: the first point left the strict minimum"
+        mn, _ = orc.minmax_quirk(ya)
+        assert mn > ya[0, 1]  # the sequential rule's minimum is not the min of the rest
+        b.step(**prm)
+    assert np.array_equal(a.get_Y(), b.get_Y())
+    assert abs(a.kl() - b.kl()) == 0.0
+    a.close()
+    b.close()
+
+
 @pytest.mark.parametrize("prec", [fg.FP64_PLAIN, fg.FP32])
 def test_strictly_increasing_2d_coordinates_are_refused(prec):
     """x0 < y0 < x1 < y1 < ...: the reference's `else if` rule never assigns the minimum (each value is
