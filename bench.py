@@ -672,7 +672,7 @@ def main():
         log("FP64 plain context")
         c64 = fg.Context(no_dims=2, precision=fg.FP64_PLAIN)
         c64.set_P(*P)
-        c64.set_Y(Y)
+        c64.set_Y(Y.astype(np.float32).astype(np.float64))  # the Y the FP32 context holds
         g64, z64 = c64.gradient()
         dt64 = timed_steps(c64, args.steps, args.warmup, prm)
         res64 = {"ms_per_step": 1e3 * dt64 / args.steps, "iters_per_s": args.steps / dt64}
