@@ -115,6 +115,49 @@ def fg_min(Y):
     return float(Y.min())
 
 
+@pytest.mark.parametrize("D", [2, 1])
+def test_update_steps_interleaved_with_other_calls_bitwise_equal(D):
+    """Host-planned update steps leave zeroMean pending for the next step's readers of Y; every other
+    entry point applies it first.  Steps interleaved with those calls (read Y, KL, gradient, captured
+    steps, the autotuner, other workgroup sizes, Z requests) must give the uninterrupted steps' embedding
+    bit for bit: none of the calls changes a result, only when the centring reaches memory."""
+    N = 30000 if D == 2 else 40000
+    P, Y, _ = synth.small_mixture_workload(N=N, D=D, span=40.0 if D == 2 else 200.0, seed=21)
+    _note(f"interleaved calls, N={N} D={D}")
+    a = _ctx(P, Y, D)
+    b = _ctx(P, Y, D)
+    lr = max(200.0, N / 12.0)
+    for k in range(14):
+        a.step(_schedule(k), lr, 5.0)
+        if k == 3:
+            b.set_graph(True)
+        if k == 6:
+            b.set_graph(False)
+        zb = b.step(_schedule(k), lr, 5.0, want_sumq=(k % 4 == 1))
+        if zb is not None:
+            assert np.isfinite(zb) and zb > 0
+        if k == 1:
+            b.get_Y()
+        elif k == 2:
+            b.kl()
+        elif k == 7:
+            b.gradient()
+        elif k == 8:
+            b.autotune()
+        elif k == 9:
+            b.set_attractive_rows(1024)
+        elif k == 10:
+            b.set_spread_points(1024)
+        elif k == 11:
+            b.set_attractive_rows(-1)
+    ya, yb = a.get_Y(), b.get_Y()
+    _note(f"max |dY| = {np.abs(ya - yb).max():.3e}")
+    assert np.array_equal(ya, yb)
+    assert a.kl() == b.kl()
+    a.close()
+    b.close()
+
+
 def test_fused_batch_follows_member_changes_and_mixed_batches_fall_back():
     """A member whose Y is reset between batched steps (new layout: the fused batch is rebuilt) still
     matches its single context bitwise, and so does a host-planned gradient taken from a member between
