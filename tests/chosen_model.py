@@ -20,19 +20,40 @@ import numpy as np
 import orc
 
 
+def _stages(m):
+    """Stages of the product's FFT plan for m = 2^a 3^b (radices 16, 8, 4, 2 then 9, 3)."""
+    s = 0
+    while m % 16 == 0:
+        s, m = s + 1, m // 16
+    for r in (8, 4, 2):
+        if m % r == 0:
+            s, m = s + 1, m // r
+    while m % 9 == 0:
+        s, m = s + 1, m // 9
+    while m % 3 == 0:
+        s, m = s + 1, m // 3
+    return s
+
+
 def fft_size(n):
-    """Smallest 2^a 3^b >= n: the product's FFT length for a linear convolution of n1d nodes
-    (n = 2 n1d - 1).  Any length >= 2 n1d - 1 gives the same convolution as the reference's 2 n1d."""
+    """The product's FFT length for a linear convolution of n1d nodes (n = 2 n1d - 1): the smallest
+    2^a 3^b >= n, or for lines up to 512 the 2^a 3^b in [n, next power of two] with the least
+    length x stages.  Any length >= 2 n1d - 1 gives the same convolution as the reference's 2 n1d."""
+    pw = 1
+    while pw < n:
+        pw *= 2
     best = None
     p3 = 1
-    while p3 < 2 * n:
+    while p3 <= pw:
         m = p3
         while m < n:
             m *= 2
-        if best is None or m < best:
-            best = m
+        if m <= pw:
+            cost = (m * _stages(m), m) if pw <= 512 else (m, m)
+            if best is None or cost < best[0]:
+                best = (cost, m)
         p3 *= 3
-    return best
+    return best[1]
 
 
 def _lagrange(t, sp):

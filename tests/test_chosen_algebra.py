@@ -13,6 +13,14 @@ def test_fft_size():
     assert fft_size(2 * 525 - 1) == 1152
     assert fft_size(2 * 1800 - 1) == 3888
     assert fft_size(2 * 150 - 1) == 324
+    assert fft_size(2 * 240 - 1) == 512  # short lines: 512 = 16 x 16 x 2 (3 stages) over 486 (4)
+    assert fft_size(2 * 105 - 1) == 256  # 16 x 16 over 216 = 8 x 27 (3 stages)
+    assert fft_size(2 * 450 - 1) == 972  # longer lines: the smallest
+    for n in range(1, 5000, 3):
+        pw = 1
+        while pw < n:
+            pw *= 2
+        assert n <= fft_size(n) <= pw
     for n in range(1, 3000, 7):
         m = fft_size(n)
         assert m >= n
