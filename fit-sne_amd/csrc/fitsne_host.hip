@@ -512,10 +512,12 @@ static auto pick_interp_f64(int p) -> decltype(&k_interp_f64<3, D>) {
 // needs no more blocks than the CUs (times the rounds N needs) is found by bisection.  Any partition
 // gives the same forces bit for bit (one fixed-point scale for the problem).
 constexpr int kAttBalanced = -1;
+// Longest row block of the windowed attractive kernel: its LDS holds the block's Y and int64 sums.
+static int att_rows_max(const fitsne_ctx *c) { return c->D == 1 ? kAttRows1 : kAttRows; }
 constexpr int kAttChunk = 256;
 
 static int balance_blocks(fitsne_ctx *c, int NW, std::vector<int> &rs) {
-    const int N = c->N;
+    const int N = c->N, amax = att_rows_max(c);
     const int nch = (N + kAttChunk - 1) / kAttChunk;
     DBuf cnt, pur, bm;
     TRY(ensure(cnt, sizeof(int) * (size_t)nch * NW));
@@ -561,7 +563,7 @@ static int balance_blocks(fitsne_ctx *c, int NW, std::vector<int> &rs) {
                                                             (s1 == 1 ? kSmall : 0.0) - (s0 == 1 ? kSmall : 0.0)
                                                       : 0.0);
             }
-            if (rows > 0 && (rows + kr > kAttRows || cost + d > T)) {  // close the block before chunk k
+            if (rows > 0 && (rows + kr > amax || cost + d > T)) {  // close the block before chunk k
                 for (int w : touched) acc[w] = 0;
                 touched.clear();
                 nb++;
@@ -586,7 +588,7 @@ static int balance_blocks(fitsne_ctx *c, int NW, std::vector<int> &rs) {
     };
     int cus = 0;
     HIPTRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
-    const long rounds = std::max(1l, ((long)N + (long)cus * kAttRows - 1) / ((long)cus * kAttRows));
+    const long rounds = std::max(1l, ((long)N + (long)cus * amax - 1) / ((long)cus * amax));
     const int want = (int)std::min<long>((long)cus * rounds, (N + kAttChunk - 1) / kAttChunk);
     double lo = 0.0, hi = 1.0;
     while (greedy(hi, nullptr) > want && hi < 1e12) hi *= 2.0;
@@ -609,9 +611,10 @@ static int build_windows(fitsne_ctx *c) {
     if (c->nnz <= 0 || NW > 8192) return FITSNE_OK;
     std::vector<int> rs;
     if (bal) TRY(balance_blocks(c, NW, rs));
-    const int R = bal ? kAttRows
-                      : c->att_rows > 0 ? std::min(kAttRows, c->att_rows)
-                                        : std::min(kAttRows, 64 * (int)((((long)N + 255) / 256 + 63) / 64));
+    const int amax = att_rows_max(c);
+    const int R = bal ? amax
+                      : c->att_rows > 0 ? std::min(amax, c->att_rows)
+                                        : std::min(amax, 64 * (int)((((long)N + 255) / 256 + 63) / 64));
     const int NB = bal ? (int)rs.size() - 1 : (N + R - 1) / R;
     const long nbw = (long)NB * NW;
     const long long cap = c->nnz + 3ll * nbw;  // every bucket padded to a multiple of 4 entries
@@ -2062,9 +2065,9 @@ extern "C" int fitsne_autotune_stats(fitsne_ctx *c, int *spread_points, int *att
 
 extern "C" int fitsne_set_attractive_rows(fitsne_ctx *c, int rows) {
     if (!c) return fail(FITSNE_E_ARG, "null argument");
-    if (rows != 0 && rows != kAttBalanced && (rows < kWave || rows > kAttRows || rows % kWave != 0))
+    if (rows != 0 && rows != kAttBalanced && (rows < kWave || rows > att_rows_max(c) || rows % kWave != 0))
         return fail(FITSNE_E_ARG, "attractive rows per workgroup must be 0 (automatic), -1 (balanced blocks) or a "
-                                  "multiple of 64 in [64, 4096]");
+                                  "multiple of 64 in [64, 4096] (8192 in 1D)");
     HIPTRY(hipSetDevice(c->device));
     TRY(materialize(c));  // the next step may lay P out without the windows the pending centring needs
     TRY(sync_ctx(c));

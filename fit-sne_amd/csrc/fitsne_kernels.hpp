@@ -141,7 +141,7 @@ __global__ __launch_bounds__(kThreads) void k_attractive(const TI *__restrict__ 
 // ---------------------------------------------------------------- attractive forces, windowed (FP32 form)
 // For points in spatial order the neighbours of a row lie in a few ranges of columns, but they are
 // scattered inside them, so the CSR kernel's gathers are one L2 request per entry.  This layout works
-// from LDS instead.  Rows are cut into blocks of R (<= kAttRows) consecutive points and columns into
+// from LDS instead.  Rows are cut into blocks of R (<= kAttRowsD<D>) consecutive points and columns into
 // windows of kWin points.  The entries of row block b whose column falls in window w form bucket (b, w);
 // buckets are stored contiguously, b-major then w, each entry as
 //   key = (row - b R) << 16 | (col - w kWin)   (u32)   and   p_ij (f32).
@@ -164,6 +164,9 @@ __global__ __launch_bounds__(kThreads) void k_attractive(const TI *__restrict__ 
 // bitwise repeatable.  Same formula as k_attractive (tsne.cpp:1121-1137).
 constexpr int kWin = 8192;
 constexpr int kAttRows = 4096;
+constexpr int kAttRows1 = 8192;  // 1D: its smaller LDS record per row allows blocks twice as long
+template <int D>
+constexpr int kAttRowsD = D == 1 ? kAttRows1 : kAttRows;
 constexpr int kAttThreads = 1024;
 // A bucket of fewer quads is processed from a flat list before the windows, with its columns' Y read
 // from global memory: staging a whole window (64 KB and two barriers) for a few entries cost more.
@@ -390,15 +393,13 @@ __device__ __forceinline__ void attr_quad(const u32x4 &kq, const f32x4 &vq4, con
     for (int d = 0; d < D; d++) atomicAdd(&acc[cur * D + d], (unsigned long long)run[d]);
 }
 
-// In 1D a workgroup holds 80 KB of LDS, so two share a CU if each wave keeps to 64 VGPRs (8 waves per
-// SIMD); without the request the compiler used 70 and the blocks ran in two rounds.  In 2D the LDS
-// (160 KB) allows one workgroup per CU, 4 waves per SIMD.
-// LDS of one k_attr_win workgroup.
+// LDS of one k_attr_win workgroup: one workgroup per CU, 4 waves per SIMD (160 KB in 2D; 128 KB in 1D,
+// whose blocks are twice as long so that N = 2M still needs a single round of workgroups).
 template <int D>
 struct AttrLds {
     float yw[kWin * D];                   // the staged window's Y
-    float yr[kAttRows * D];               // the row block's Y
-    unsigned long long acc[kAttRows * D];  // the row block's fixed-point sums
+    float yr[kAttRowsD<D> * D];               // the row block's Y
+    unsigned long long acc[kAttRowsD<D> * D];  // the row block's fixed-point sums
 };
 
 // Row block b of the windowed attractive term (the body of k_attr_win, and of its batched form).
@@ -511,11 +512,9 @@ __device__ __forceinline__ void attr_win_block(const int *__restrict__ start, co
             pre[u] = q < lim ? Y[base + q] : 0.f;
         }
     };
-    // Each thread takes quads qq and qq + kAttThreads of a round.  In 2D it loads the next round's pair
-    // (in the same bucket, or the first pair of the next staged bucket) before it computes, so that two
-    // pairs per thread are in flight across the compute and across the barriers of a window change; in
-    // 1D, held to 64 registers for eight waves per SIMD, the waves alone keep the loads in flight.
-    constexpr bool kAhead = D == 2;
+    // Each thread takes quads qq and qq + kAttThreads of a round.  It loads the next round's pair (in
+    // the same bucket, or the first pair of the next staged bucket) before it computes, so that two
+    // pairs per thread are in flight across the compute and across the barriers of a window change.
     u32x4 ka = {0u, 0u, 0u, 0u}, kb = {0u, 0u, 0u, 0u};
     f32x4 va = {0.f, 0.f, 0.f, 0.f}, vb = {0.f, 0.f, 0.f, 0.f};
     auto load_pair = [&](int qq, int q1) {
@@ -530,7 +529,7 @@ __device__ __forceinline__ void attr_win_block(const int *__restrict__ start, co
     };
     if (w < NW) {
         fetch(w);
-        if (kAhead) load_pair((st[w] >> 2) + t, st[w + 1] >> 2);
+        load_pair((st[w] >> 2) + t, st[w + 1] >> 2);
     }
     while (w < NW) {
         const int q0 = st[w] >> 2, qm = (st[w] + pu[w]) >> 2, q1 = st[w + 1] >> 2;
@@ -549,16 +548,13 @@ __device__ __forceinline__ void attr_win_block(const int *__restrict__ start, co
         __syncthreads();
         if (wn < NW) fetch(wn);  // consumed after this bucket
         int qq = q0 + t;
-        if (kAhead && qq >= q1 && wn < NW) load_pair((st[wn] >> 2) + t, st[wn + 1] >> 2);  // no quad here
+        if (qq >= q1 && wn < NW) load_pair((st[wn] >> 2) + t, st[wn + 1] >> 2);  // no quad here
         for (; qq < q1; qq += 2 * kAttThreads) {
-            if (!kAhead) load_pair(qq, q1);
             const u32x4 ca = ka, cb = kb;
             const f32x4 cva = va, cvb = vb;
             const int qn = qq + 2 * kAttThreads;
-            if (kAhead) {
-                if (qn < q1) load_pair(qn, q1);
-                else if (wn < NW) load_pair((st[wn] >> 2) + t, st[wn + 1] >> 2);
-            }
+            if (qn < q1) load_pair(qn, q1);
+            else if (wn < NW) load_pair((st[wn] >> 2) + t, st[wn + 1] >> 2);
             if (qq < qm) attr_quad_pure<D>(ca, cva, yr, yw, acc, inv_df, df_one, S);
             else attr_quad<D>(ca, cva, yr, yw, acc, inv_df, df_one, S);
             if (qq + kAttThreads < q1) {
@@ -574,7 +570,7 @@ __device__ __forceinline__ void attr_win_block(const int *__restrict__ start, co
 }
 
 template <int D>
-__global__ __launch_bounds__(kAttThreads) __attribute__((amdgpu_waves_per_eu(D == 1 ? 8 : 4))) void k_attr_win(const int *__restrict__ start,
+__global__ __launch_bounds__(kAttThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_attr_win(const int *__restrict__ start,
                                                           const int *__restrict__ pure,
                                                           const uint32_t *__restrict__ key,
                                                           const float *__restrict__ wval,

@@ -270,8 +270,9 @@ int fitsne_set_overlap(fitsne_ctx *ctx, int enable);
 /* Autotuner (no reference counterpart: the reference's spread is a serial loop).  Times, on the
  * context's current Y and geometry, the FP32 spread's points per workgroup (512, 1024, 2048, 4096) and
  * the attractive term's row blocks (about N / 256, 1024, 2048 or 4096 rows, or blocks of up to 4096 rows
- * balanced by a cost model, one per CU; the windowed copy of P is rebuilt for each), and keeps the
- * fastest of each; every result is bitwise the same for any choice.
+ * in 2D, 8192 in 1D, balanced by a cost model, one per CU; the windowed copy of P is rebuilt for each),
+ * and keeps the fastest of each (a larger spread size only when more than 3% faster); every result is
+ * bitwise the same for any choice.
  * spread_points (may be null) receives the spread's choice.  FP32 contexts with P and Y set.
  * fitsne_batch_create tunes each context's spread; a fused batch uses 4096-row attractive blocks. */
 int fitsne_autotune(fitsne_ctx *ctx, int *spread_points);
@@ -279,8 +280,8 @@ int fitsne_autotune(fitsne_ctx *ctx, int *spread_points);
  * (nullable outputs). */
 int fitsne_autotune_stats(fitsne_ctx *ctx, int *spread_points, int *attractive_rows);
 /* Rows per attractive workgroup of an FP32 context (0: automatic, about N / 256; -1: blocks of up to 4096
- * rows cut so that their modelled times match, one per CU; else a multiple of 64 up to 4096).  The forces
- * are bitwise the same for every value. */
+ * rows (8192 in 1D) cut so that their modelled times match, one per CU; else a multiple of 64 up to 4096
+ * (8192 in 1D)).  The forces are bitwise the same for every value. */
 int fitsne_set_attractive_rows(fitsne_ctx *ctx, int rows);
 /* Set the spread's points per workgroup directly (a multiple of 512 in [512, 8192]; default 1024). */
 int fitsne_set_spread_points(fitsne_ctx *ctx, int points);

@@ -62,7 +62,8 @@ def test_attractive_row_block_does_not_change_results(D, N):
     Y = blobs(N, D, 6, seed=41 + D, span=40.0)
     P = random_csr(N, 15, seed=6)
     out = []
-    for rows in (0, 64, 1024, 4096, -1):
+    cand = (0, 64, 1024, 4096, -1) + ((8192,) if D == 1 else ())  # 1D blocks reach 8192 rows
+    for rows in cand:
         c = fg.Context(no_dims=D, precision=fg.FP32)
         c.set_P(*P)
         c.set_Y(Y)
@@ -76,7 +77,18 @@ def test_attractive_row_block_does_not_change_results(D, N):
         np.testing.assert_array_equal(g, out[0][0])
         assert z == out[0][1]
         np.testing.assert_array_equal(y, out[0][2])
-    assert [o[3] for o in out[1:]] == [64, 1024, 4096, -1]
+    assert [o[3] for o in out[1:]] == list(cand[1:])
+
+
+def test_attractive_rows_are_validated():
+    """Row blocks are limited by the kernel's LDS: 4096 rows in 2D, 8192 in 1D, multiples of 64."""
+    for D, top in ((2, 4096), (1, 8192)):
+        c = fg.Context(no_dims=D, precision=fg.FP32)
+        c.set_attractive_rows(top)
+        for bad in (-2, 32, 100, top + 64):
+            with pytest.raises(fg.FitsneError):
+                c.set_attractive_rows(bad)
+        c.close()
 
 
 def test_spread_points_are_validated():
