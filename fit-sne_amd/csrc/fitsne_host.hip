@@ -113,7 +113,9 @@ struct fitsne_ctx {
     bool overlap = true;          // host-planned FP32 step: attractive term on astream beside the grid stages
     int att_rows = 0;             // rows per attractive workgroup (0: about N / 256); results do not depend on it
     hipEvent_t fork_ev = nullptr, join_ev = nullptr;
-    hipEvent_t geo_ev = nullptr, kern_ev = nullptr;  // kernel spectra on astream: after the geometry, before cols
+    // kernel spectra on astream: after the geometry record or a new twiddle table (when the main stream
+    // wrote either in this step), before the column pass
+    hipEvent_t geo_ev = nullptr, kern_ev = nullptr;
     hipEvent_t att_ev = nullptr;  // 1D: on astream just before the attractive kernel; the spread waits for it
     bool att_go = false;
     bool join_pending = false;
@@ -716,8 +718,8 @@ struct StageDims {
 
 // Spread, FFT convolution, Z, interpolation (+ update, zeroMean) of the FP32 form, reading geometry,
 // plan and step parameters from the context's device records.  side: the kernel spectra go on the
-// second stream (behind the attractive kernel there), after the geometry record, so that they run
-// beside the spread and the forward row pass; the column pass waits for them.  (Measured at N = 1M:
+// second stream (behind the attractive kernel there), so that they run beside the spread and the
+// forward row pass; the column pass waits for them.  (Measured at N = 1M:
 // this beats computing them on the main stream before or after the spread by ~0.015 ms per step.)
 // ctr: the centring still pending on Y (host-planned steps), applied by the spread and the interpolation.
 // hv: the host-planned geometry record; the spread and the kernel spectra take it by value and the
