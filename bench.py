@@ -233,7 +233,7 @@ def c4_measure(args, fg, synth, one_by_one=True):
 
 
 def bench_c4(args, fg, synth):
-    r = c4_measure(args, fg, synth)
+    r = c4_measure(args, fg, synth, one_by_one=not args.no_captured)
     out = {
         "metric": "aggregate gradient iterations/s over 64 batched 2D embeddings (C4)",
         "value": r["value"], "unit": "problem-iters/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -608,6 +608,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-c4", action="store_true", help="leave the batched C4 figure out of the default line")
+    ap.add_argument("--no-captured", action="store_true",
+                    help="--config c4: leave out the run of each problem's captured step one after another")
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--config", default="c2", choices=["c2", "c3", "c4", "c5", "exact", "bh", "trajectory"],
                     help="c2 (default): the headline step at N=1M 2D; c3: 1D df=0.5 at N=2M; c4: 64 batched "
