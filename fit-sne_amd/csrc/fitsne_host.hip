@@ -277,6 +277,12 @@ static int twiddles(fitsne_ctx *c, int n, const cx<T> **out) {
 
 static int blocks_for(long n, int per = kThreads) { return (int)((n + per - 1) / per); }
 
+// Workgroups of the passes over Y that leave per-block min / max partials (centring, min / max, and
+// the captured and batched geometry that reduces them): up to 2048 coordinates each, at most 1024.
+// Every writer and reader of a context's partials uses this count.  (With one coordinate per thread,
+// the 64-problem batch's centring ran ~25,000 small workgroups.)
+static int part_blocks(long n) { return std::min(blocks_for(n, 8 * kThreads), 1024); }
+
 // Number of bits that hold every value in [0, v].
 static int bits_for(unsigned long long v) {
     int b = 1;
@@ -300,7 +306,7 @@ static long skip_coord(const fitsne_ctx *c) { return c->D == 2 ? first_coord(c) 
 template <typename TV>
 static int minmax_now(fitsne_ctx *c, double *mn, double *mx) {
     const long n = (long)c->N * c->D;
-    const int nblk = std::min(blocks_for(n), 1024);
+    const int nblk = part_blocks(n);
     TRY(ensure(c->mmpart, sizeof(double) * 2 * nblk));
     TRY(ensure(c->mm, sizeof(double) * 4));
     launch(c, "minmax_partial", k_minmax_partial<TV>, dim3(nblk), dim3(kThreads), 0, c->Y.as<const TV>(), n,
@@ -422,7 +428,7 @@ static int materialize(fitsne_ctx *c) {
     if (!c->ctr_lazy) return FITSNE_OK;
     HIPTRY(hipSetDevice(c->device));
     const long n = (long)c->N * c->D;
-    const int cblk = std::min(blocks_for(n), 1024);
+    const int cblk = part_blocks(n);
     TRY(ensure(c->mmpart, sizeof(double) * 2 * cblk));
     launch(c, "center", k_center<float>, dim3(cblk), dim3(kThreads), 0, c->dgeo.as<const DevGeo>(), c->Y.as<float>(),
            n, c->D, skip_coord(c), c->mean.as<const double>(), c->mmpart.as<double>());
@@ -823,7 +829,7 @@ static int launch_f32_stages(fitsne_ctx *c, const cx<float> *tw, const StageDims
            c->part.as<double>(), N, mm_host ? c->mmu.as<double>() : (double *)nullptr, skip_coord(c), ctr);
     if (mode == 1) {
         const long n = (long)N * D;
-        const int cblk = std::min(blocks_for(n), 1024);
+        const int cblk = part_blocks(n);
         if (mm_host) {
             launch(c, "mean_mm", k_mean_mm<D>, dim3(1), dim3(kThreads), 0, dg, c->part.as<const double>(),
                    c->mmu.as<const double>(), nblk, (double)N, c->Y.as<const float>(), first_coord(c),
@@ -856,7 +862,7 @@ static int ensure_f32_buffers(fitsne_ctx *c, int n1d, int Mp) {
     TRY(ensure(c->dC, sizeof(float) * (size_t)N * D));
     TRY(ensure(c->part, sizeof(double) * (size_t)blocks_for(N) * D));
     TRY(ensure(c->mean, sizeof(double) * 2));
-    TRY(ensure(c->mmpart, sizeof(double) * 2 * std::min(blocks_for((long)N * D), 1024)));
+    TRY(ensure(c->mmpart, sizeof(double) * 2 * part_blocks((long)N * D)));
     TRY(ensure(c->mm, sizeof(double) * 4));
     TRY(ensure(c->zbuf, sizeof(double) * 4));
     TRY(ensure(c->mmu, sizeof(double) * 2 * D * (size_t)blocks_for(N)));
@@ -965,7 +971,7 @@ static int pipeline_f32(fitsne_ctx *c, int mode, const fitsne_step_params *sp) {
 template <int D>
 static int launch_step_dev(fitsne_ctx *c) {
     const int n1d_max = D == 2 ? kCapN1d2 : kCapN1d1, mp_max = D == 2 ? kCapMp2 : kCapMp1;
-    const int cblk = std::min(blocks_for((long)c->N * D), 1024);
+    const int cblk = part_blocks((long)c->N * D);
     launch(c, "geometry", k_geometry<float>, dim3(1), dim3(1024), 0, c->mmpart.as<const double>(), cblk,
            c->Y.as<const float>(), first_coord(c), c->permuted ? c->inv_d.as<const int>() : (const int *)nullptr,
            (long)c->N * D, D, c->p, c->ipi, c->mni, n1d_max, mp_max, lagrange_inbox_bound(c->p), c->dgeo.as<DevGeo>());
@@ -1219,7 +1225,7 @@ static int pipeline_f64(fitsne_ctx *c, int mode, const fitsne_step_params *sp) {
                sp->max_step_norm, sp->plain_descent, c->part.as<double>());
         TRY(ensure(c->mean, sizeof(double) * 2));
         const long n = (long)N * D;
-        const int cblk = std::min(blocks_for(n), 1024);
+        const int cblk = part_blocks(n);
         TRY(ensure(c->mmpart, sizeof(double) * 2 * cblk));
         TRY(ensure(c->mm, sizeof(double) * 4));
         launch(c, "mean_final", k_mean_final, dim3(1), dim3(kThreads), 0, c->dgeo.as<const DevGeo>(), c->part.as<const double>(), nblk, D,
@@ -1269,7 +1275,7 @@ static int pipeline_exact(fitsne_ctx *c, int mode, const fitsne_step_params *sp)
         const int nblk = blocks_for(N);
         TRY(ensure(c->part, sizeof(double) * (size_t)nblk * D));
         TRY(ensure(c->mean, sizeof(double) * kExMaxDims));
-        const int cblk = std::min(blocks_for(nd), 1024);
+        const int cblk = part_blocks(nd);
         TRY(ensure(c->mmpart, sizeof(double) * 2 * cblk));
         launch(c, "update64", k_update_f64, dim3(nblk), dim3(kThreads), 0, c->Y.as<double>(), c->uY.as<double>(),
                c->gains.as<double>(), c->dC.as<const double>(), N, D, sp->momentum, sp->learning_rate,
@@ -1394,7 +1400,7 @@ static int pipeline_bh(fitsne_ctx *c, int mode, const fitsne_step_params *sp) {
         const int nblk = blocks_for(N);
         TRY(ensure(c->part, sizeof(double) * (size_t)nblk * D));
         TRY(ensure(c->mean, sizeof(double) * kExMaxDims));
-        const int cblk = std::min(blocks_for(nd), 1024);
+        const int cblk = part_blocks(nd);
         TRY(ensure(c->mmpart, sizeof(double) * 2 * cblk));
         launch(c, "update64", k_update_f64, dim3(nblk), dim3(kThreads), 0, c->Y.as<double>(), c->uY.as<double>(),
                c->gains.as<double>(), c->dC.as<const double>(), N, D, sp->momentum, sp->learning_rate,
@@ -2002,7 +2008,7 @@ static int capture_step(fitsne_ctx *c) {
 static int ensure_partials(fitsne_ctx *c) {
     if (c->mm_parts_valid) return FITSNE_OK;
     const long n = (long)c->N * c->D;
-    const int nblk = std::min(blocks_for(n), 1024);
+    const int nblk = part_blocks(n);
     TRY(ensure(c->mmpart, sizeof(double) * 2 * nblk));
     launch(c, "minmax_partial", k_minmax_partial<float>, dim3(nblk), dim3(kThreads), 0, c->Y.as<const float>(), n,
            skip_coord(c), c->mmpart.as<double>());
@@ -2322,7 +2328,7 @@ static int batch_prepare(fitsne_batch *b) {
         q.gains = c->gains.as<float>();
         q.N = c->N;
         q.spread_pts = c->spread_pts;
-        q.cblk = std::min(blocks_for((long)c->N * 2), 1024);
+        q.cblk = part_blocks((long)c->N * 2);
         q.first = first_coord(c);
         q.inv = c->permuted ? c->inv_d.as<const int>() : nullptr;
         q.wstart = c->wstart.as<const int>();
