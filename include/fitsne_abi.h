@@ -250,7 +250,9 @@ int fitsne_get_Y(fitsne_ctx *ctx, double *Y);
 int fitsne_gradient(fitsne_ctx *ctx, double *dC, double *sum_Q);
 
 /* One iteration of TSNE::run's hot loop (tsne.cpp:440-531): gradient, gains/momentum/clip update,
- * Y += uY and zeroMean(Y).  Asynchronous on the context's stream; sum_Q (nullable) forces a sync. */
+ * Y += uY and zeroMean(Y).  Asynchronous on the context's stream; sum_Q (nullable) forces a sync.
+ * The FP32 form's host-planned steps apply zeroMean as the next step reads Y, or before any other call
+ * touches it (fitsne_get_Y, fitsne_kl, ...): every result is bitwise what an immediate zeroMean gives. */
 int fitsne_step(fitsne_ctx *ctx, const fitsne_step_params *prm, double *sum_Q);
 
 /* Captured steps (FP32 form; no reference counterpart).  When enabled, fitsne_step replays one hipGraph
