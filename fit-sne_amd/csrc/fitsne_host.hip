@@ -118,6 +118,11 @@ struct fitsne_ctx {
     hipEvent_t geo_ev = nullptr, kern_ev = nullptr;
     hipEvent_t att_ev = nullptr;  // 1D: on astream just before the attractive kernel; the spread waits for it
     bool att_go = false;
+    // 1D host-planned FP32 step: the kernel line's spectra on a third stream, beside the attractive kernel
+    // and the spread, into kline[3][Mp]; kl_go orders them after the main stream's work so far, kl_ev
+    // orders the convolution after them
+    hipStream_t kstream = nullptr;
+    hipEvent_t kl_go = nullptr, kl_ev = nullptr;
     bool join_pending = false;
 
     int N = 0;
@@ -127,6 +132,7 @@ struct fitsne_ctx {
     DBuf rowP, colP, val;  // rowP int64[N+1], colP uint32[nnz], val T[nnz]
     // embedding state (element type = float for FP32, double for FP64 plain)
     DBuf Y, uY, gains, pos, dC;
+    DBuf kline;
     // sort (FP32)
     // exact kNN (affinity build)
     DBuf knnX, knnXT, knnI, knnD;
@@ -253,6 +259,7 @@ static int sync_ctx(fitsne_ctx *c) {
     HIPTRY(hipGetLastError());
     HIPTRY(hipStreamSynchronize(c->stream));
     if (c->astream) HIPTRY(hipStreamSynchronize(c->astream));
+    if (c->kstream) HIPTRY(hipStreamSynchronize(c->kstream));
     return flush_timing(c);
 }
 
@@ -713,6 +720,10 @@ static int launch_attractive(fitsne_ctx *c, TV scale, TV *pos, const DevGeo *dg 
 // Largest grids of the captured path (the build's limits): 2D FFT length 2048 (n1d <= 1024), 1D 4096.
 constexpr int kCapN1d2 = kMaxLen2d / 2, kCapMp2 = kMaxLen2d;
 constexpr int kCapMp1 = kMaxTot32 / 2, kCapN1d1 = kCapMp1 / 2;
+// k_kline1d's workgroup: one wave per SIMD, so that it fits in the registers the 1D attractive kernel's
+// workgroup leaves on its CU (four waves per SIMD at 88 VGPRs each, of 512).  With two waves per SIMD
+// (80 VGPRs each) it did not fit, and waited for that kernel's end.
+constexpr int kKline1dThreads = 256;
 
 // Launch sizes of the grid stages: exact for a host-planned step, the build's limits for a captured one.
 struct StageDims {
@@ -772,6 +783,21 @@ static int launch_f32_stages(fitsne_ctx *c, const cx<float> *tw, const StageDims
         c->stream = main;
         HIPTRY(hipEventRecord(c->kern_ev, c->astream));
     }
+    // 1D: the kernel line (its spectra depend only on the geometry record and the twiddles) on the third
+    // stream, after the main stream's work so far (the twiddle table, the last convolution's read of kline)
+    const bool kl_side = D == 1 && hv && side && c->overlap;
+    if (kl_side) {
+        TRY(ensure(c->kline, sizeof(float) * 3 * (size_t)sd.Mp));
+        if (!c->kstream) HIPTRY(hipStreamCreateWithFlags(&c->kstream, hipStreamNonBlocking));
+        HIPTRY(hipEventRecord(c->kl_go, c->stream));
+        HIPTRY(hipStreamWaitEvent(c->kstream, c->kl_go, 0));
+        hipStream_t main = c->stream;
+        c->stream = c->kstream;
+        launch(c, "kline1d", k_kline1d<float, kKline1dThreads, kCapMp1>, dim3(1), dim3(kKline1dThreads),
+               sizeof(cx<float>) * (size_t)sd.Mp, *hv, tw, kind_q, kind_1, c->df, c->kline.as<float>());
+        c->stream = main;
+        HIPTRY(hipEventRecord(c->kl_ev, c->kstream));
+    }
     c->tw_fresh = false;
     if (c->att_go && side) {
         HIPTRY(hipStreamWaitEvent(c->stream, c->att_ev, 0));
@@ -808,9 +834,11 @@ static int launch_f32_stages(fitsne_ctx *c, const cx<float> *tw, const StageDims
     } else {
         launch(c, "grid_to_f32", k_grid_to_f32, dim3(sd.gr_nodes), dim3(kThreads), 0, dg,
                c->fixgrid.as<unsigned long long>(), c->gin.as<float4>());
+        if (kl_side) HIPTRY(hipStreamWaitEvent(c->stream, c->kl_ev, 0));
         launch(c, "conv1d_f32", k_conv1d<float, kConv1dThreads, kMaxTot32>, dim3(1), dim3(kConv1dThreads),
                sizeof(cx<float>) * 2 * (size_t)sd.Mp, dg, tw, kind_q, kind_1, c->df, 1, c->gin.as<const float>(), 4, 3,
-               0, c->gout.as<float>(), 4, c->zpart.as<double>(), (const float *)nullptr);
+               0, c->gout.as<float>(), 4, c->zpart.as<double>(), (const float *)nullptr,
+               kl_side ? c->kline.as<const float>() : (const float *)nullptr);
         launch(c, "zfinal", k_zfinal, dim3(1), dim3(kThreads), 0, dg, c->zpart.as<const double>(), (int)df1, (double)N,
                c->zbuf.as<double>());
     }
@@ -1139,7 +1167,7 @@ static int plain_pass(fitsne_ctx *c, const Geo &g, const cx<double> *tw, int set
     } else {
         launch(c, "conv1d_f64", k_conv1d<double, kConv1dThreads, kMaxTot64>, dim3(1), dim3(kConv1dThreads),
                sizeof(cx<double>) * g.Mp, c->dgeo.as<const DevGeo>(), tw, kind1d, kind1d, c->df, 0, c->w.as<const double>(), nt, nt, use_k1,
-               c->V.as<double>(), nt, (double *)nullptr, (const double *)nullptr);
+               c->V.as<double>(), nt, (double *)nullptr, (const double *)nullptr, (const double *)nullptr);
     }
     launch(c, "fill", k_fill<double>, dim3(std::min(blocks_for((long)N * nt), 2048)), dim3(kThreads), 0,
            pot.as<double>(), (long)N * nt, 0.0);
@@ -1533,6 +1561,8 @@ extern "C" int fitsne_ctx_create(fitsne_ctx **out, int device, int precision, in
         hipEventCreateWithFlags(&c->geo_ev, kEvFlags) != hipSuccess ||
         hipEventCreateWithFlags(&c->kern_ev, kEvFlags) != hipSuccess ||
         hipEventCreateWithFlags(&c->att_ev, kEvFlags) != hipSuccess ||
+        hipEventCreateWithFlags(&c->kl_go, kEvFlags) != hipSuccess ||
+        hipEventCreateWithFlags(&c->kl_ev, kEvFlags) != hipSuccess ||
         hipHostMalloc((void **)&c->h_mm, sizeof(double) * 4, hipHostMallocDefault) != hipSuccess ||
         hipHostGetDevicePointer((void **)&c->h_mm_dev, c->h_mm, 0) != hipSuccess ||
         hipEventCreateWithFlags(&c->mm_event, kEvFlags) != hipSuccess) {
@@ -1550,6 +1580,7 @@ extern "C" int fitsne_ctx_destroy(fitsne_ctx *c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->astream) (void)hipStreamSynchronize(c->astream);
+    if (c->kstream) (void)hipStreamSynchronize(c->kstream);
     drop_graph(c);
     DBuf *all[] = {&c->rowP, &c->colP, &c->val, &c->Y, &c->uY, &c->gains, &c->pos, &c->dC, &c->wcount, &c->wstart, &c->wpure,
                    &c->wkey, &c->wval, &c->wbmax, &c->wrstart, &c->knnX, &c->knnXT, &c->knnI, &c->knnD, &c->dgeo,
@@ -1559,7 +1590,7 @@ extern "C" int fitsne_ctx_destroy(fitsne_ctx *c) {
                    &c->inv_d, &c->colP2, &c->val2, &c->tmp, &c->fixgrid, &c->att, &c->rep, &c->zrow,
                    &c->sy_row, &c->sy_col, &c->sy_val, &c->sy_erow, &c->sy_keys, &c->sy_keys2, &c->sy_vals,
                    &c->sy_vals2, &c->sy_partner, &c->sy_cnt, &c->sy_start, &c->sy_okeys, &c->sy_okeys2, &c->sy_tmp,
-                   &c->sy_scol, &c->sy_sval, &c->bh_tree_d, &c->bh_neg, &c->bh_Q};
+                   &c->sy_scol, &c->sy_sval, &c->bh_tree_d, &c->bh_neg, &c->bh_Q, &c->kline};
     for (DBuf *b : all) release(*b);
     for (auto &kv : c->tw32) release(kv.second);
     for (auto &kv : c->tw64) release(kv.second);
@@ -1574,7 +1605,10 @@ extern "C" int fitsne_ctx_destroy(fitsne_ctx *c) {
     if (c->geo_ev) (void)hipEventDestroy(c->geo_ev);
     if (c->kern_ev) (void)hipEventDestroy(c->kern_ev);
     if (c->att_ev) (void)hipEventDestroy(c->att_ev);
+    if (c->kl_go) (void)hipEventDestroy(c->kl_go);
+    if (c->kl_ev) (void)hipEventDestroy(c->kl_ev);
     if (c->astream) (void)hipStreamDestroy(c->astream);
+    if (c->kstream) (void)hipStreamDestroy(c->kstream);
     if (c->h_mm) (void)hipHostFree(c->h_mm);
     if (c->bh_pin) (void)hipHostFree(c->bh_pin);
     if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
@@ -2968,7 +3002,7 @@ extern "C" int fitsne_n_body_fft_1d(int N, int n_terms, const double *Y, const d
            c->wx.as<const double>(), c->wy.as<const double>(), c->ch.as<const double>(), nt, N, g, c->w.as<double>());
     launch(c, "conv1d_f64", k_conv1d<double, kConv1dThreads, kMaxTot64>, dim3(1), dim3(kConv1dThreads),
            sizeof(cx<double>) * g.Mp, c->dgeo.as<const DevGeo>(), tw, 0, 0, 1.0, 0, c->w.as<const double>(), nt, nt, 0, c->V.as<double>(),
-           nt, (double *)nullptr, c->KQ.as<const double>());
+           nt, (double *)nullptr, c->KQ.as<const double>(), (const double *)nullptr);
     launch(c, "interp_f64", pick_interp_f64<1>(p), dim3(blocks_for(N)), dim3(kThreads), 0, c->box.as<const int>(),
            c->wx.as<const double>(), c->wy.as<const double>(), c->V.as<const double>(), nt, N, g, c->potA.as<double>());
     HIPTRY(hipMemcpyAsync(potentialsQij, c->potA.p, sizeof(double) * (size_t)N * nt, hipMemcpyDeviceToHost,

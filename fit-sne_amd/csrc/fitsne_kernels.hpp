@@ -1692,11 +1692,71 @@ __global__ __launch_bounds__(NTHR) void k_rows_inv(const DevGeo *__restrict__ dg
 // k_kern_cols; data lines z = A + i b and c; V1 = C_q A, G = -i S A - C_q b, one inverse of V1 + i G;
 // Z partial sums z0..z3 as in k_cols_f32 (full spectrum, no half-plane weights).
 // Plain (f32form = 0): real even spectrum of K_q (or K_1 when use_k1) times each packed pair of terms.
+// The kernel line of k_conv1d: the kernel values at the grid offsets, circularly embedded, and their
+// transform, left in buf[0, Mp).  The arithmetic of every element is the same for any NTHR and MAXM.
+template <typename T, int NTHR, int MAXM>
+__device__ __forceinline__ void conv1d_kernel_fft(const DevGeo *__restrict__ dg, const cx<T> *__restrict__ tw,
+                                                  int kind_q, int kind_1, double df, int f32form, int use_k1,
+                                                  cx<T> *buf) {
+    const Geo &g = dg->g;
+    const int Mp = g.Mp, n1d = g.n1d;
+    for (int q = threadIdx.x; q < Mp; q += NTHR) buf[q] = cx<T>{(T)0, (T)0};
+    __syncthreads();
+    for (int a = threadIdx.x; a < n1d; a += NTHR) {
+        const double d = a * g.h;
+        const T kq = kernel_value<T>(kind_q, d * d, df), k1 = kernel_value<T>(kind_1, d * d, df);
+        if (f32form) {
+            buf[a] = cx<T>{kq * ((T)1 + (T)d), k1};
+            if (a > 0) buf[Mp - a] = cx<T>{kq * ((T)1 - (T)d), k1};
+        } else {
+            buf[a] = cx<T>{use_k1 ? k1 : kq, (T)0};
+            if (a > 0) buf[Mp - a] = buf[a];
+        }
+    }
+    __syncthreads();
+    fft_lines<T, NTHR, MAXM>(buf, dg->pl, 1, Mp, tw, false);
+}
+
+// Bin m of the transformed kernel line: C_q (the squared kernel's, real), and C_1, S, the even and odd
+// parts of the imaginary channel (the FP32 form carries the kernel and its d-weighted copy in one line).
+template <typename T>
+__device__ __forceinline__ void kernel_line_bin(const cx<T> *buf, int m, int Mp, T &cq, T &c1, T &ss) {
+    const cx<T> kz = buf[m];
+    const T imn = buf[m == 0 ? 0 : Mp - m].y;
+    cq = kz.x;
+    c1 = (kz.y + imn) * (T)0.5;
+    ss = (imn - kz.y) * (T)0.5;
+}
+
+// The FP32 form's kernel line alone, into kline[3][Mp] = (C_q, C_1, S), with the step's geometry record
+// by value: one workgroup on a stream of its own from the step's start, so that k_conv1d is left with the
+// data lines.  Sized to share a CU with the 1D attractive kernel, so that it does not wait for that
+// kernel's CUs: one line of LDS (32 KB at Mp = 4096, the 1D limit, beside its 128 KB of the CU's 160)
+// and one wave per SIMD (NTHR = 256) at 80 VGPRs beside its four at 88 (of 512).
+template <typename T, int NTHR, int MAXM>
+__global__ __launch_bounds__(NTHR) void k_kline1d(const DevGeo v, const cx<T> *__restrict__ tw, int kind_q, int kind_1,
+                                                  double df, T *__restrict__ kline) {
+    if (!v.live) return;
+    extern __shared__ unsigned char smem_raw[];
+    cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
+    conv1d_kernel_fft<T, NTHR, MAXM>(&v, tw, kind_q, kind_1, df, 1, 0, buf);
+    const int Mp = v.g.Mp;
+    for (int m = threadIdx.x; m < Mp; m += NTHR) {
+        T cq, c1, ss;
+        kernel_line_bin(buf, m, Mp, cq, c1, ss);
+        kline[m] = cq;
+        kline[Mp + m] = c1;
+        kline[2 * Mp + m] = ss;
+    }
+}
+
+// kline (FP32 form, nullable): the kernel line's spectra from k_kline1d instead of computing them here.
 template <typename T, int NTHR, int MAXM>
 __global__ __launch_bounds__(NTHR) void k_conv1d(const DevGeo *__restrict__ dg, const cx<T> *__restrict__ tw, int kind_q,
                                                  int kind_1, double df, int f32form, const T *__restrict__ grid,
                                                  int sin, int nterms, int use_k1, T *__restrict__ out, int sout,
-                                                 double *__restrict__ zsum, const T *__restrict__ kspec) {
+                                                 double *__restrict__ zsum, const T *__restrict__ kspec,
+                                                 const T *__restrict__ kline) {
     constexpr int NC = (MAXM + NTHR - 1) / NTHR;
     if (!dg->live) return;
     const Geo &g = dg->g;
@@ -1712,34 +1772,21 @@ __global__ __launch_bounds__(NTHR) void k_conv1d(const DevGeo *__restrict__ dg, 
             cq[c] = m < Mp ? kspec[m] : (T)0;
             c1[c] = ss[c] = (T)0;
         }
-    } else {
-        // kernel line
-        for (int q = threadIdx.x; q < Mp; q += NTHR) buf[q] = cx<T>{(T)0, (T)0};
-        __syncthreads();
-        for (int a = threadIdx.x; a < n1d; a += NTHR) {
-            const double d = a * g.h;
-            const T kq = kernel_value<T>(kind_q, d * d, df), k1 = kernel_value<T>(kind_1, d * d, df);
-            if (f32form) {
-                buf[a] = cx<T>{kq * ((T)1 + (T)d), k1};
-                if (a > 0) buf[Mp - a] = cx<T>{kq * ((T)1 - (T)d), k1};
-            } else {
-                buf[a] = cx<T>{use_k1 ? k1 : kq, (T)0};
-                if (a > 0) buf[Mp - a] = buf[a];
-            }
+    } else if (kline) {
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            const int m = threadIdx.x + c * NTHR;
+            cq[c] = m < Mp ? kline[m] : (T)0;
+            c1[c] = m < Mp ? kline[Mp + m] : (T)0;
+            ss[c] = m < Mp ? kline[2 * Mp + m] : (T)0;
         }
-        __syncthreads();
-        fft_lines<T, NTHR, MAXM>(buf, pl, 1, Mp, tw, false);
+    } else {
+        conv1d_kernel_fft<T, NTHR, MAXM>(dg, tw, kind_q, kind_1, df, f32form, use_k1, buf);
 #pragma unroll
         for (int c = 0; c < NC; c++) {
             const int m = threadIdx.x + c * NTHR;
             cq[c] = c1[c] = ss[c] = (T)0;
-            if (m < Mp) {
-                const cx<T> kz = buf[m];
-                const T imn = buf[m == 0 ? 0 : Mp - m].y;
-                cq[c] = kz.x;
-                c1[c] = (kz.y + imn) * (T)0.5;
-                ss[c] = (imn - kz.y) * (T)0.5;
-            }
+            if (m < Mp) kernel_line_bin(buf, m, Mp, cq[c], c1[c], ss[c]);
         }
         __syncthreads();
     }

@@ -2,7 +2,9 @@
 
 The captured step runs the same kernels on the same data as the host-planned step; only the launch
 shapes (sized for the build's largest grid) and the place where the geometry is computed differ, and
-the geometry rules are the same functions on host and device.  So after any number of steps the
+the geometry rules are the same functions on host and device (in 1D the host-planned step transforms the
+kernel line in a workgroup of its own, the captured step inside the convolution, with the same
+per-element arithmetic).  So after any number of steps the
 embedding must be bitwise the host-planned one, in 2D and 1D, with and without the spatial reorder,
 and each embedding of a batch bitwise its own single-context run.  A grid beyond the build's limits
 is refused on the device and leaves the embedding unchanged.
