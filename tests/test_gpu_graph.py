@@ -160,6 +160,39 @@ def test_update_steps_interleaved_with_other_calls_bitwise_equal(D):
     b.close()
 
 
+@pytest.mark.parametrize("D,df", [(2, 1.0), (1, 1.0), (1, 0.5)])
+def test_side_streams_leave_results_bitwise_unchanged(D, df):
+    """A host-planned step runs the attractive term on a second stream and, in 1D, the convolution's
+    kernel line on a third; with overlap off every kernel runs in turn on the context's stream.  Both
+    must give the same embedding, Z and gradient bit for bit, also across a reset to a three times wider
+    Y (a longer FFT line: the kernel-line and spectrum buffers grow) and back."""
+    N = 20000
+    P, Y, _ = synth.small_mixture_workload(N=N, D=D, span=40.0 if D == 2 else 200.0, seed=33)
+    _note(f"overlap on / off, N={N} D={D} df={df}")
+    a = _ctx(P, Y, D, df)
+    b = _ctx(P, Y, D, df)
+    b.set_overlap(False)
+    lr = max(200.0, N / 12.0)
+    lens = []
+    for phase, Yp in enumerate([Y, 3.0 * Y, Y]):
+        if phase:
+            a.set_Y(Yp)
+            b.set_Y(Yp)
+        for k in range(5):
+            za = a.step(_schedule(k), lr, 5.0, want_sumq=(k == 4))
+            zb = b.step(_schedule(k), lr, 5.0, want_sumq=(k == 4))
+            assert za == zb
+        lens.append(a.last_geometry()["fft_len"])
+        assert np.array_equal(a.get_Y(), b.get_Y()), phase
+    _note(f"FFT lengths per phase: {lens}")
+    assert lens[1] > lens[0]
+    ga, za = a.gradient()
+    gb, zb = b.gradient()
+    assert np.array_equal(ga, gb) and za == zb
+    a.close()
+    b.close()
+
+
 def test_fused_batch_follows_member_changes_and_mixed_batches_fall_back():
     """A member whose Y is reset between batched steps (new layout: the fused batch is rebuilt) still
     matches its single context bitwise, and so does a host-planned gradient taken from a member between
