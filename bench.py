@@ -57,13 +57,13 @@ def kernel_profile(ctx, steps, prm):
     return stats
 
 
-def measured_traffic(kernel):
-    """HBM bytes per dispatch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary,
-    each counter scaled by what it read for a plain 64 MiB device copy in the same runs; None when the
-    summary is for another kernel."""
+def measured_traffic(kernel, summary="attractive_c2_n1m_2d_hbm_traffic.md"):
+    """HBM bytes per dispatch of `kernel` from a committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+    under profiles/, each counter scaled by what it read for a plain 64 MiB device copy in the same
+    runs; None when the summary is missing or for another kernel."""
     import re
 
-    path = os.path.join(ROOT, "profiles", "attractive_c2_n1m_2d_hbm_traffic.md")
+    path = os.path.join(ROOT, "profiles", summary)
     try:
         text = open(path).read()
     except OSError:
@@ -368,7 +368,8 @@ def bench_c3(args, fg, synth):
                    "n_boxes": geo["n_boxes"], "n1d": geo["n1d"], "fft_len": geo["fft_len"],
                    "spread_points_per_workgroup": spread_pts},
         "roofline": {"bound": "hbm", "kernel": "k_attr_win<1>", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic("k_attr_win<1>", "attractive_c3_n2m_1d_hbm_traffic.md"),
                      "algorithmic_bytes": att_bytes, "avg_ms": att_ms},
         "cpu_baseline": cpu,
         "fp64_plain": res64,
@@ -626,6 +627,18 @@ def main():
     import fitsne_gpu as fg
     import synth
 
+    if args.copy_calibration:
+        import torch
+
+        # source written first and then pushed out of L2 and the MALL by 448 MiB of later writes
+        src = torch.ones(512 * 2 ** 20 // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty(64 * 2 ** 20 // 4, dtype=torch.float32, device="cuda")
+        torch.cuda.synchronize()
+        dst.copy_(src[: dst.numel()])
+        torch.cuda.synchronize()
+        del src, dst
+        log("copy calibration: one copy kernel reading 64 MiB (cold) and writing 64 MiB")
+
     if args.config == "c4":
         return bench_c4(args, fg, synth)
     if args.config == "c3":
@@ -638,18 +651,6 @@ def main():
         return bench_trajectory(args, fg, synth)
     if args.config == "c5":
         return bench_c5(args, fg, synth)
-
-    if args.copy_calibration:
-        import torch
-
-        # source written first and then pushed out of L2 and the MALL by 448 MiB of later writes
-        src = torch.ones(512 * 2 ** 20 // 4, dtype=torch.float32, device="cuda")
-        dst = torch.empty(64 * 2 ** 20 // 4, dtype=torch.float32, device="cuda")
-        torch.cuda.synchronize()
-        dst.copy_(src[: dst.numel()])
-        torch.cuda.synchronize()
-        del src, dst
-        log("copy calibration: one copy kernel reading 64 MiB (cold) and writing 64 MiB")
 
     N = args.n
     log(f"generating the C2 workload N={N}")
