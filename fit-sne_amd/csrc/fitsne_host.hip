@@ -116,13 +116,14 @@ struct fitsne_ctx {
     // kernel spectra on astream: after the geometry record or a new twiddle table (when the main stream
     // wrote either in this step), before the column pass
     hipEvent_t geo_ev = nullptr, kern_ev = nullptr;
-    hipEvent_t att_ev = nullptr;  // 1D: on astream just before the attractive kernel; the spread waits for it
-    bool att_go = false;
     // 1D host-planned FP32 step: the kernel line's spectra on a third stream, beside the attractive kernel
     // and the spread, into kline[3][Mp]; kl_go orders them after the main stream's work so far, kl_ev
     // orders the convolution after them
     hipStream_t kstream = nullptr;
     hipEvent_t kl_go = nullptr, kl_ev = nullptr;
+    // 1D host-planned FP32 step: the attractive kernel stays on the main stream and the rest of the step
+    // is queued on astream until the interpolation, which returns to main (swap_main: main meanwhile)
+    hipStream_t swap_main = nullptr;
     bool join_pending = false;
 
     int N = 0;
@@ -738,6 +739,8 @@ struct StageDims {
 // second stream (behind the attractive kernel there), so that they run beside the spread and the
 // forward row pass; the column pass waits for them.  (Measured at N = 1M:
 // this beats computing them on the main stream before or after the spread by ~0.015 ms per step.)
+// In 1D host-planned steps the stages up to Z run on the second stream (c->swap_main holds the main
+// stream, where the attractive kernel runs), and the interpolation returns to the main stream.
 // ctr: the centring still pending on Y (host-planned steps), applied by the spread and the interpolation.
 // hv: the host-planned geometry record; the spread and the kernel spectra take it by value and the
 // spread stores it for the other passes, so no launch or cross-stream wait stands before them (without
@@ -799,10 +802,6 @@ static int launch_f32_stages(fitsne_ctx *c, const cx<float> *tw, const StageDims
         HIPTRY(hipEventRecord(c->kl_ev, c->kstream));
     }
     c->tw_fresh = false;
-    if (c->att_go && side) {
-        HIPTRY(hipStreamWaitEvent(c->stream, c->att_ev, 0));
-        c->att_go = false;
-    }
     const int pts = c->spread_pts;
     if (hv) {
         launch(c, "spread_f32", pick_spread_f32_v<D>(p), dim3((N + pts - 1) / pts), dim3(kSpreadThreads), 0,
@@ -843,6 +842,13 @@ static int launch_f32_stages(fitsne_ctx *c, const cx<float> *tw, const StageDims
                c->zbuf.as<double>());
     }
     const int nblk = (N + kInterpPts - 1) / kInterpPts;
+    if (c->swap_main) {
+        // back to the main stream, behind the attractive kernel, once the grid stages are done
+        HIPTRY(hipEventRecord(c->join_ev, c->stream));
+        c->stream = c->swap_main;
+        c->swap_main = nullptr;
+        HIPTRY(hipStreamWaitEvent(c->stream, c->join_ev, 0));
+    }
     if (c->join_pending) {
         HIPTRY(hipStreamWaitEvent(c->stream, c->join_ev, 0));
         c->join_pending = false;
@@ -931,27 +937,39 @@ static int pipeline_f32(fitsne_ctx *c, int mode, const fitsne_step_params *sp) {
     // previous one left); otherwise it is applied here, before anything reads Y
     if (c->ctr_lazy && (mode != 1 || !c->mm_pending)) TRY(materialize(c));
     const double *ctr = c->ctr_lazy ? c->mean.as<const double>() : nullptr;
-    c->att_go = false;
     const DevGeo *live;
     TRY(live_record(c, &live));
-    // the attractive term on the second stream (after everything queued so far), beside the grid
-    // stages; the interpolation waits for it
+    // 2D: the attractive term on the second stream (after everything queued so far), beside the grid
+    // stages; the interpolation waits for it.
     // The second stream is made on first use: contexts that only run captured or batched steps never
     // create it, so their streams alone share the device's hardware queues (a second stream per context
     // measured a third fewer batched iterations).
-    if (c->overlap) {
+    // In 1D the attractive kernel sets the step's length, so it stays on the main stream, where it starts
+    // as soon as the previous step's last kernel ends (a wait across streams costs ~10 us even on a
+    // finished event); the grid stages go to the second stream, which waits for the fork, and the
+    // interpolation returns to the main stream behind the attractive kernel with one wait for them.
+    // The fork's latency also lets the attractive kernel take its CUs before the spread's many small
+    // workgroups arrive (when the spread came first, C3 steps took 0.59 against 0.51 ms).
+    struct SwapGuard {
+        fitsne_ctx *c;
+        ~SwapGuard() {
+            if (c->swap_main) {
+                c->stream = c->swap_main;
+                c->swap_main = nullptr;
+            }
+        }
+    } swap_guard{c};
+    if (c->overlap && D == 1) {
         if (!c->astream) HIPTRY(hipStreamCreateWithFlags(&c->astream, hipStreamNonBlocking));
         HIPTRY(hipEventRecord(c->fork_ev, c->stream));
         HIPTRY(hipStreamWaitEvent(c->astream, c->fork_ev, 0));
-        if (D == 1) {
-            // In 1D the host, once it has the previous step's min / max, can queue the spread as soon as
-            // the fork releases the attractive kernel; when the spread's many small workgroups reach the
-            // CUs first, the attractive kernel's (two per CU, 80 KB of LDS each) wait behind them
-            // (0.59 against 0.51 ms per step at N = 2M, from run to run).  The spread waits for this
-            // event, recorded just before the attractive kernel, so that kernel always starts first.
-            HIPTRY(hipEventRecord(c->att_ev, c->astream));
-            c->att_go = true;
-        }
+        TRY((launch_attractive<float, D>(c, (float)c->exag, c->pos.as<float>(), live, false, ctr)));
+        c->swap_main = c->stream;
+        c->stream = c->astream;
+    } else if (c->overlap) {
+        if (!c->astream) HIPTRY(hipStreamCreateWithFlags(&c->astream, hipStreamNonBlocking));
+        HIPTRY(hipEventRecord(c->fork_ev, c->stream));
+        HIPTRY(hipStreamWaitEvent(c->astream, c->fork_ev, 0));
         hipStream_t main = c->stream;
         c->stream = c->astream;
         const int rc = launch_attractive<float, D>(c, (float)c->exag, c->pos.as<float>(), live, false, ctr);
@@ -1560,7 +1578,6 @@ extern "C" int fitsne_ctx_create(fitsne_ctx **out, int device, int precision, in
         hipEventCreateWithFlags(&c->join_ev, kEvFlags) != hipSuccess ||
         hipEventCreateWithFlags(&c->geo_ev, kEvFlags) != hipSuccess ||
         hipEventCreateWithFlags(&c->kern_ev, kEvFlags) != hipSuccess ||
-        hipEventCreateWithFlags(&c->att_ev, kEvFlags) != hipSuccess ||
         hipEventCreateWithFlags(&c->kl_go, kEvFlags) != hipSuccess ||
         hipEventCreateWithFlags(&c->kl_ev, kEvFlags) != hipSuccess ||
         hipHostMalloc((void **)&c->h_mm, sizeof(double) * 4, hipHostMallocDefault) != hipSuccess ||
@@ -1604,7 +1621,6 @@ extern "C" int fitsne_ctx_destroy(fitsne_ctx *c) {
     if (c->join_ev) (void)hipEventDestroy(c->join_ev);
     if (c->geo_ev) (void)hipEventDestroy(c->geo_ev);
     if (c->kern_ev) (void)hipEventDestroy(c->kern_ev);
-    if (c->att_ev) (void)hipEventDestroy(c->att_ev);
     if (c->kl_go) (void)hipEventDestroy(c->kl_go);
     if (c->kl_ev) (void)hipEventDestroy(c->kl_ev);
     if (c->astream) (void)hipStreamDestroy(c->astream);

@@ -435,7 +435,7 @@ class Context:
         check(lib().fitsne_set_graph(self.h, int(enable)))
 
     def set_overlap(self, enable):
-        """Attractive term beside the grid stages on a second stream, and in 1D the convolution's kernel
+        """Attractive term and grid stages side by side on two streams, and in 1D the convolution's kernel
         line on a third (default), or all kernels in order on one stream.  Results are the same."""
         check(lib().fitsne_set_overlap(self.h, int(enable)))
 

@@ -263,9 +263,9 @@ int fitsne_step(fitsne_ctx *ctx, const fitsne_step_params *prm, double *sum_Q);
  * fitsne_get_Y, fitsne_kl, fitsne_gradient, fitsne_last_geometry, or fitsne_step with sum_Q) return
  * FITSNE_E_UNSUPPORTED.  Re-captured after set_P / set_Y / reorder. */
 int fitsne_set_graph(fitsne_ctx *ctx, int enable);
-/* Host-planned FP32 steps run the attractive term on a second stream beside the grid stages, and in 1D
- * the convolution's kernel line on a third (enable = 1, the default); 0 runs every kernel of the step on
- * the context's stream, one after another (so that per-kernel timings are those of each kernel alone).  Results are the same either way.  No counterpart
+/* Host-planned FP32 steps run the attractive term and the grid stages side by side on two streams, and in
+ * 1D the convolution's kernel line on a third (enable = 1, the default); 0 runs every kernel of the step
+ * on the context's stream, one after another (so that per-kernel timings are those of each kernel alone).  Results are the same either way.  No counterpart
  * in the reference (a tuning knob of this build). */
 int fitsne_set_overlap(fitsne_ctx *ctx, int enable);
 

@@ -162,8 +162,10 @@ def test_update_steps_interleaved_with_other_calls_bitwise_equal(D):
 
 @pytest.mark.parametrize("D,df", [(2, 1.0), (1, 1.0), (1, 0.5)])
 def test_side_streams_leave_results_bitwise_unchanged(D, df):
-    """A host-planned step runs the attractive term on a second stream and, in 1D, the convolution's
-    kernel line on a third; with overlap off every kernel runs in turn on the context's stream.  Both
+    """A host-planned step runs the attractive term and the grid stages side by side on two streams (in
+    1D the attractive term stays on the context's stream and the grid stages move) and, in 1D, the
+    convolution's kernel line on a third; with overlap off every kernel runs in turn on the context's
+    stream.  Both
     must give the same embedding, Z and gradient bit for bit, also across a reset to a three times wider
     Y (a longer FFT line: the kernel-line and spectrum buffers grow) and back."""
     N = 20000
