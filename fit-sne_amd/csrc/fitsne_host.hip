@@ -787,7 +787,8 @@ static int launch_f32_stages(fitsne_ctx *c, const cx<float> *tw, const StageDims
         HIPTRY(hipEventRecord(c->kern_ev, c->astream));
     }
     // 1D: the kernel line (its spectra depend only on the geometry record and the twiddles) on the third
-    // stream, after the main stream's work so far (the twiddle table, the last convolution's read of kline)
+    // stream, after the work queued so far on the grid stages' stream (the twiddle table, the last
+    // convolution's read of kline)
     const bool kl_side = D == 1 && hv && side && c->overlap;
     if (kl_side) {
         TRY(ensure(c->kline, sizeof(float) * 3 * (size_t)sd.Mp));
